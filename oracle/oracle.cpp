@@ -1,0 +1,818 @@
+// oracle.cpp -- CPU FP64 restatement of rheoTool's log-conformation stress step.
+//
+// TEST INFRASTRUCTURE ONLY: the checker for the HIP path, never part of it.
+// Build with -O2 -ffp-contract=off so every expression rounds exactly as
+// written (no fused multiply-add), matching the reference's plain C++ loops.
+//
+// Reference files (paths relative to of90/src/libs/):
+//   constitutiveEquations/constitutiveEqs/utils/boilerLog.H
+//   constitutiveEquations/constitutiveEqs/constitutiveEq/constitutiveEq.C
+//   constitutiveEquations/constitutiveEqs/{Oldroyd-B,Giesekus,FENE-P,PTT}/*Log/*.C
+//   gaussDefCmpwConvectionScheme/gaussDefCmpwConvectionScheme.C
+//   sparseMatrixSolvers/segregated/{sparseSolver.C,eigenSolver/eigenSolver.C}
+//   constitutiveEquations/constitutiveEqs/utils/jacobi.H
+// OpenFOAM-9 / Eigen 3.2.9 behaviour (third party, absent from the reference
+// tree) is restated from their published algorithms and marked [OF9]/[Eigen].
+#include "oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+namespace {
+
+typedef double T9[9];
+
+// OpenFOAM tensor inner product t1 & t2 (row-major), sum order k = 0,1,2.
+inline void dotTT(const double* a, const double* b, double* c) {
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            c[3 * i + j] = a[3 * i] * b[j] + a[3 * i + 1] * b[3 + j] + a[3 * i + 2] * b[6 + j];
+}
+inline void transp(const double* a, double* t) {
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) t[3 * i + j] = a[3 * j + i];
+}
+inline void expandSymm(const double* s, double* t) {
+    t[0] = s[0]; t[1] = s[1]; t[2] = s[2];
+    t[3] = s[1]; t[4] = s[3]; t[5] = s[4];
+    t[6] = s[2]; t[7] = s[4]; t[8] = s[5];
+}
+// [OF9] symm(tensor)
+inline void symmT(const double* t, double* s) {
+    s[0] = t[0];
+    s[1] = 0.5 * (t[1] + t[3]);
+    s[2] = 0.5 * (t[2] + t[6]);
+    s[3] = t[4];
+    s[4] = 0.5 * (t[5] + t[7]);
+    s[5] = t[8];
+}
+// innerP (constitutiveEq.C:471-518): isFirstT -> t1^T & t2 & t1, else t1 & t2 & t1^T
+inline void innerP(const double* t1, const double* t2, bool isFirstT, double* r) {
+    T9 tt, tmp;
+    transp(t1, tt);
+    if (isFirstT) {
+        dotTT(tt, t2, tmp);
+        dotTT(tmp, t1, r);
+    } else {
+        dotTT(t1, t2, tmp);
+        dotTT(tmp, tt, r);
+    }
+}
+// [OF9] det(tensor) and inv(tensor) = cofactor / det
+inline double det9(const double* t) {
+    return t[0] * t[4] * t[8] + t[1] * t[5] * t[6] + t[2] * t[3] * t[7] - t[0] * t[5] * t[7] -
+           t[1] * t[3] * t[8] - t[2] * t[4] * t[6];
+}
+inline void inv9(const double* t, double* r) {
+    const double d = det9(t);
+    r[0] = (t[4] * t[8] - t[7] * t[5]) / d;
+    r[1] = (t[2] * t[7] - t[1] * t[8]) / d;
+    r[2] = (t[1] * t[5] - t[2] * t[4]) / d;
+    r[3] = (t[6] * t[5] - t[3] * t[8]) / d;
+    r[4] = (t[0] * t[8] - t[2] * t[6]) / d;
+    r[5] = (t[3] * t[2] - t[0] * t[5]) / d;
+    r[6] = (t[3] * t[7] - t[6] * t[4]) / d;
+    r[7] = (t[6] * t[1] - t[0] * t[7]) / d;
+    r[8] = (t[0] * t[4] - t[1] * t[3]) / d;
+}
+const double I9[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+
+// [OF9] inv(Field<tensor>) first-cell shift: if the first cell has a diagonal
+// component with |T_ii|^2/|T|^2 < small(1e-15), 1 is added to that component
+// of every cell before inverting and subtracted after.
+void invShiftMask(const double* first, bool mask[3]) {
+    double s = 0;
+    for (int q = 0; q < 9; ++q) s += first[q] * first[q];
+    mask[0] = first[0] * first[0] / s < 1e-15;
+    mask[1] = first[4] * first[4] / s < 1e-15;
+    mask[2] = first[8] * first[8] / s < 1e-15;
+}
+inline void invField(const double* t, const bool mask[3], double* r) {
+    if (!(mask[0] || mask[1] || mask[2])) {
+        inv9(t, r);
+        return;
+    }
+    T9 tp;
+    for (int q = 0; q < 9; ++q) tp[q] = t[q];
+    if (mask[0]) tp[0] += 1.0;
+    if (mask[1]) tp[4] += 1.0;
+    if (mask[2]) tp[8] += 1.0;
+    inv9(tp, r);
+    if (mask[0]) r[0] -= 1.0;
+    if (mask[1]) r[4] -= 1.0;
+    if (mask[2]) r[8] -= 1.0;
+}
+
+// ---------------- eigen-decomposition restatements ----------------------
+
+// [Eigen 3.2.9] SelfAdjointEigenSolver<Matrix3d>::compute restated from the
+// algorithm: scale by max|a_ij|, Householder reduction to tridiagonal form,
+// implicit symmetric QR steps with Wilkinson shift (Golub & Van Loan 8.3.2),
+// deflation when |e_i| <= 2 eps (|d_i| + |d_i+1|), ascending sort.
+void eigQR(const double* th, double ev[3], double Q[9]) {
+    double a[3][3] = {{th[0], th[1], th[2]}, {th[1], th[3], th[4]}, {th[2], th[4], th[5]}};
+    double scale = 0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) scale = std::max(scale, std::fabs(a[i][j]));
+    if (scale == 0) scale = 1;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) a[i][j] /= scale;
+    double d[3], e[2];
+    double Z[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    d[0] = a[0][0];
+    if (a[2][0] == 0.0) {
+        d[1] = a[1][1];
+        d[2] = a[2][2];
+        e[0] = a[1][0];
+        e[1] = a[2][1];
+    } else {
+        // reflector H = [[c, s],[s,-c]] on rows/cols 1,2 maps (a10, a20) to (b, 0)
+        const double b = std::sqrt(a[1][0] * a[1][0] + a[2][0] * a[2][0]);
+        const double c = a[1][0] / b, s = a[2][0] / b;
+        const double p = a[1][1], q = a[2][2], r = a[2][1];
+        d[1] = c * c * p + 2.0 * c * s * r + s * s * q;
+        d[2] = s * s * p - 2.0 * c * s * r + c * c * q;
+        e[0] = b;
+        e[1] = c * s * (p - q) + (s * s - c * c) * r;
+        Z[1][1] = c; Z[1][2] = s;
+        Z[2][1] = s; Z[2][2] = -c;
+    }
+    const double eps = std::numeric_limits<double>::epsilon();
+    const double tiny = std::numeric_limits<double>::min();
+    // dense symmetric tridiagonal T = Z^T A Z; rotations applied as T <- G^T T G
+    double T[3][3] = {{d[0], e[0], 0.0}, {e[0], d[1], e[1]}, {0.0, e[1], d[2]}};
+    for (int iter = 0; iter < 90; ++iter) {
+        for (int i = 0; i < 2; ++i)
+            if (std::fabs(T[i + 1][i]) <= 2.0 * eps * (std::fabs(T[i][i]) + std::fabs(T[i + 1][i + 1])) ||
+                std::fabs(T[i + 1][i]) <= tiny)
+                T[i + 1][i] = T[i][i + 1] = 0.0;
+        int end = 2;
+        while (end > 0 && T[end][end - 1] == 0.0) --end;
+        if (end == 0) break;
+        int start = end - 1;
+        while (start > 0 && T[start][start - 1] != 0.0) --start;
+        // Wilkinson shift from the trailing 2x2 of the active block
+        const double ta = T[end - 1][end - 1], tc = T[end][end], tb = T[end][end - 1];
+        const double td = (ta - tc) * 0.5;
+        double mu;
+        if (td == 0.0) {
+            mu = tc - std::fabs(tb);
+        } else {
+            const double h = std::hypot(td, tb);
+            mu = tc - tb * tb / (td + (td > 0 ? h : -h));
+        }
+        double x = T[start][start] - mu, z = T[start + 1][start];
+        for (int k = start; k < end; ++k) {
+            // G = [[c, s], [-s, c]] on (k, k+1) with G^T [x; z] = [r; 0]
+            double c = 1.0, s = 0.0;
+            if (z != 0.0) {
+                const double r = std::hypot(x, z);
+                c = x / r;
+                s = -z / r;
+            }
+            for (int j = 0; j < 3; ++j) {
+                const double tk = T[k][j], tk1 = T[k + 1][j];
+                T[k][j] = c * tk - s * tk1;
+                T[k + 1][j] = s * tk + c * tk1;
+            }
+            for (int i = 0; i < 3; ++i) {
+                const double tk = T[i][k], tk1 = T[i][k + 1];
+                T[i][k] = c * tk - s * tk1;
+                T[i][k + 1] = s * tk + c * tk1;
+            }
+            for (int r = 0; r < 3; ++r) {
+                const double zk = Z[r][k], zk1 = Z[r][k + 1];
+                Z[r][k] = c * zk - s * zk1;
+                Z[r][k + 1] = s * zk + c * zk1;
+            }
+            if (k > start) T[k + 1][k - 1] = T[k - 1][k + 1] = 0.0;  // bulge annihilated
+            if (k + 1 < end) {
+                x = T[k + 1][k];
+                z = T[k + 2][k];
+            }
+        }
+        T[2][0] = T[0][2] = 0.0;
+    }
+    for (int i = 0; i < 3; ++i) d[i] = T[i][i];
+    int idx[3] = {0, 1, 2};
+    // selection sort, ascending (stable for ties)
+    for (int i = 0; i < 2; ++i) {
+        int k = i;
+        for (int j = i + 1; j < 3; ++j)
+            if (d[idx[j]] < d[idx[k]]) k = j;
+        std::swap(idx[i], idx[k]);
+    }
+    for (int c = 0; c < 3; ++c) {
+        ev[c] = d[idx[c]] * scale;
+        for (int r = 0; r < 3; ++r) Q[3 * r + c] = Z[r][idx[c]];
+    }
+}
+
+// utils/jacobi.H:7-158 restated: cyclic Jacobi with the threshold strategy
+// (tresh = 0.2 sm^2 for the first three sweeps, skip of negligible
+// off-diagonals after four), at most 50 sweeps.  Returns the eigenvalues of
+// theta (not their exponential) in Jacobi order.
+void eigJacobiH(const double* th, double D[3], double Vout[9]) {
+    double A[3][3] = {{th[0], th[1], th[2]}, {th[1], th[3], th[4]}, {th[2], th[4], th[5]}};
+    double V[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    double B[3], Zs[3];
+    for (int p = 0; p < 3; ++p) {
+        B[p] = D[p] = A[p][p];
+        Zs[p] = 0;
+    }
+    for (int sweep = 1; sweep <= 50; ++sweep) {
+        double sm = 0;
+        for (int p = 0; p < 2; ++p)
+            for (int q = p + 1; q < 3; ++q) sm += std::fabs(A[p][q]);
+        const double tresh = sweep < 4 ? 0.2 * sm * sm : 0.0;
+        for (int p = 0; p < 2; ++p)
+            for (int q = p + 1; q < 3; ++q) {
+                const double g = 100.0 * std::fabs(A[p][q]);
+                if (sweep > 4 && std::fabs(D[p]) + g == std::fabs(D[p]) &&
+                    std::fabs(D[q]) + g == std::fabs(D[q])) {
+                    A[p][q] = 0;
+                } else if (std::fabs(A[p][q]) > tresh) {
+                    double h = D[q] - D[p], t;
+                    if (std::fabs(h) + g == std::fabs(h)) {
+                        t = A[p][q] / h;
+                    } else {
+                        const double thet = 0.5 * h / A[p][q];
+                        t = 1.0 / (std::fabs(thet) + std::sqrt(1.0 + thet * thet));
+                        if (thet < 0) t = -t;
+                    }
+                    const double c = 1.0 / std::sqrt(1 + t * t), s = t * c, tau = s / (1.0 + c);
+                    h = t * A[p][q];
+                    Zs[p] -= h; Zs[q] += h;
+                    D[p] -= h; D[q] += h;
+                    A[p][q] = 0;
+                    auto rot = [&](double& x, double& y) {
+                        const double gg = x, hh = y;
+                        x = gg - s * (hh + gg * tau);
+                        y = hh + s * (gg - hh * tau);
+                    };
+                    for (int j = 0; j < p; ++j) rot(A[j][p], A[j][q]);
+                    for (int j = p + 1; j < q; ++j) rot(A[p][j], A[j][q]);
+                    for (int j = q + 1; j < 3; ++j) rot(A[p][j], A[q][j]);
+                    for (int j = 0; j < 3; ++j) rot(V[j][p], V[j][q]);
+                }
+            }
+        for (int p = 0; p < 3; ++p) {
+            B[p] += Zs[p];
+            D[p] = B[p];
+            Zs[p] = 0;
+        }
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) Vout[3 * i + j] = V[i][j];
+}
+
+// ---------------- [OF9] lduMatrix primitives -----------------------------
+struct Ldu {
+    int N, F;
+    const int32_t *l, *u;
+    const double *diag, *lower, *upper;
+};
+
+void Amul(const Ldu& A, const double* x, double* y) {
+    for (int c = 0; c < A.N; ++c) y[c] = A.diag[c] * x[c];
+    for (int f = 0; f < A.F; ++f) {
+        y[A.u[f]] += A.lower[f] * x[A.l[f]];
+        y[A.l[f]] += A.upper[f] * x[A.u[f]];
+    }
+}
+void Tmul(const Ldu& A, const double* x, double* y) {
+    for (int c = 0; c < A.N; ++c) y[c] = A.diag[c] * x[c];
+    for (int f = 0; f < A.F; ++f) {
+        y[A.u[f]] += A.upper[f] * x[A.l[f]];
+        y[A.l[f]] += A.lower[f] * x[A.u[f]];
+    }
+}
+void sumA(const Ldu& A, double* s) {
+    for (int c = 0; c < A.N; ++c) s[c] = A.diag[c];
+    for (int f = 0; f < A.F; ++f) {
+        s[A.u[f]] += A.lower[f];
+        s[A.l[f]] += A.upper[f];
+    }
+}
+double sumMag(const std::vector<double>& v) {
+    double s = 0;
+    for (double x : v) s += std::fabs(x);
+    return s;
+}
+double sumProd(const double* a, const double* b, int n) {
+    double s = 0;
+    for (int i = 0; i < n; ++i) s += a[i] * b[i];
+    return s;
+}
+// [OF9] lduMatrix::solver::normFactor
+double normFactor(const Ldu& A, const double* psi, const double* b, const double* Apsi) {
+    std::vector<double> t(A.N);
+    sumA(A, t.data());
+    double avg = 0;
+    for (int c = 0; c < A.N; ++c) avg += psi[c];
+    avg /= A.N;
+    for (int c = 0; c < A.N; ++c) t[c] *= avg;
+    double s = 0;
+    for (int c = 0; c < A.N; ++c) s += std::fabs(Apsi[c] - t[c]) + std::fabs(b[c] - t[c]);
+    return s + 1e-20;
+}
+bool converged(double finalRes, double initRes, const orc_controls* c) {
+    return finalRes < c->tolerance || (c->relTol > 0 && finalRes < c->relTol * initRes);
+}
+
+// [OF9] preconditioners: none, diagonal, DILU (with losort forward sweep)
+struct Precond {
+    const Ldu& A;
+    int kind;
+    std::vector<double> rD;
+    std::vector<int32_t> losort;
+    Precond(const Ldu& a, int k) : A(a), kind(k) {
+        if (kind == ORC_PC_DIAGONAL) {
+            rD.resize(A.N);
+            for (int c = 0; c < A.N; ++c) rD[c] = 1.0 / A.diag[c];
+        } else if (kind == ORC_PC_DILU) {
+            rD.assign(A.diag, A.diag + A.N);
+            for (int f = 0; f < A.F; ++f) rD[A.u[f]] -= A.upper[f] * A.lower[f] / rD[A.l[f]];
+            for (int c = 0; c < A.N; ++c) rD[c] = 1.0 / rD[c];
+            // faces sorted by upper address, stable in face index
+            std::vector<int32_t> cnt(A.N + 1, 0);
+            for (int f = 0; f < A.F; ++f) cnt[A.u[f] + 1]++;
+            for (int c = 0; c < A.N; ++c) cnt[c + 1] += cnt[c];
+            losort.resize(A.F);
+            for (int f = 0; f < A.F; ++f) losort[cnt[A.u[f]]++] = f;
+        }
+    }
+    void apply(double* w, const double* r) const {
+        if (kind == ORC_PC_NONE) {
+            std::memcpy(w, r, sizeof(double) * A.N);
+        } else if (kind == ORC_PC_DIAGONAL) {
+            for (int c = 0; c < A.N; ++c) w[c] = rD[c] * r[c];
+        } else {
+            for (int c = 0; c < A.N; ++c) w[c] = rD[c] * r[c];
+            for (int k = 0; k < A.F; ++k) {
+                const int f = losort[k];
+                w[A.u[f]] -= rD[A.u[f]] * A.lower[f] * w[A.l[f]];
+            }
+            for (int f = A.F - 1; f >= 0; --f) w[A.l[f]] -= rD[A.l[f]] * A.upper[f] * w[A.u[f]];
+        }
+    }
+    void applyT(double* w, const double* r) const {
+        if (kind != ORC_PC_DILU) {
+            apply(w, r);
+            return;
+        }
+        for (int c = 0; c < A.N; ++c) w[c] = rD[c] * r[c];
+        for (int k = 0; k < A.F; ++k) {
+            const int f = losort[k];
+            w[A.u[f]] -= rD[A.u[f]] * A.upper[f] * w[A.l[f]];
+        }
+        for (int f = A.F - 1; f >= 0; --f) w[A.l[f]] -= rD[A.l[f]] * A.lower[f] * w[A.u[f]];
+    }
+};
+
+// [OF9] PBiCGStab::solve (van der Vorst BiCGStab, OpenFOAM residual control)
+orc_perf pbicgstab(const Ldu& A, const double* b, const orc_controls* ctl, double* psi) {
+    const int N = A.N;
+    orc_perf perf{0, 0, 0, 0, 1};
+    std::vector<double> yA(N), pA(N), rA(N);
+    Amul(A, psi, yA.data());
+    for (int c = 0; c < N; ++c) rA[c] = b[c] - yA[c];
+    const double nf = normFactor(A, psi, b, yA.data());
+    perf.initialResidual = sumMag(rA) / nf;
+    perf.finalResidual = perf.initialResidual;
+    if (ctl->minIter > 0 || !converged(perf.finalResidual, perf.initialResidual, ctl)) {
+        std::vector<double> AyA(N), sA(N), zA(N), tA(N), rA0(rA);
+        double rA0rA = 0, alpha = 0, omega = 0;
+        Precond P(A, ctl->precond);
+        do {
+            const double rA0rAold = rA0rA;
+            rA0rA = sumProd(rA0.data(), rA.data(), N);
+            if (std::fabs(rA0rA) < 1e-300) break;
+            if (perf.nIterations == 0) {
+                pA = rA;
+            } else {
+                if (std::fabs(omega) < 1e-300) break;
+                const double beta = (rA0rA / rA0rAold) * (alpha / omega);
+                for (int c = 0; c < N; ++c) pA[c] = rA[c] + beta * (pA[c] - omega * AyA[c]);
+            }
+            P.apply(yA.data(), pA.data());
+            Amul(A, yA.data(), AyA.data());
+            const double rA0AyA = sumProd(rA0.data(), AyA.data(), N);
+            alpha = rA0rA / rA0AyA;
+            for (int c = 0; c < N; ++c) sA[c] = rA[c] - alpha * AyA[c];
+            perf.finalResidual = sumMag(sA) / nf;
+            if (converged(perf.finalResidual, perf.initialResidual, ctl)) {
+                for (int c = 0; c < N; ++c) psi[c] += alpha * yA[c];
+                perf.nIterations++;
+                perf.converged = 1;
+                return perf;
+            }
+            P.apply(zA.data(), sA.data());
+            Amul(A, zA.data(), tA.data());
+            const double tAtA = sumProd(tA.data(), tA.data(), N);
+            omega = sumProd(tA.data(), sA.data(), N) / tAtA;
+            for (int c = 0; c < N; ++c) {
+                psi[c] += alpha * yA[c] + omega * zA[c];
+                rA[c] = sA[c] - omega * tA[c];
+            }
+            perf.finalResidual = sumMag(rA) / nf;
+        } while ((++perf.nIterations < ctl->maxIter &&
+                  !converged(perf.finalResidual, perf.initialResidual, ctl)) ||
+                 perf.nIterations < ctl->minIter);
+    }
+    perf.converged = converged(perf.finalResidual, perf.initialResidual, ctl) ? 1 : 0;
+    return perf;
+}
+
+// [OF9] PBiCG::solve (bi-conjugate gradients, the tutorials' solver)
+orc_perf pbicg(const Ldu& A, const double* b, const orc_controls* ctl, double* psi) {
+    const int N = A.N;
+    orc_perf perf{0, 0, 0, 0, 1};
+    std::vector<double> pA(N), wA(N), qA(N), rA(N), rT(N);
+    Amul(A, psi, wA.data());
+    Tmul(A, psi, qA.data());
+    for (int c = 0; c < N; ++c) {
+        rA[c] = b[c] - wA[c];
+        rT[c] = b[c] - qA[c];
+    }
+    const double nf = normFactor(A, psi, b, wA.data());
+    perf.initialResidual = sumMag(rA) / nf;
+    perf.finalResidual = perf.initialResidual;
+    if (ctl->minIter > 0 || !converged(perf.finalResidual, perf.initialResidual, ctl)) {
+        std::vector<double> pT(N, 0.0), wT(N);
+        double wArT = 1e20, wArTold;
+        Precond P(A, ctl->precond);
+        do {
+            wArTold = wArT;
+            P.apply(wA.data(), rA.data());
+            P.applyT(wT.data(), rT.data());
+            wArT = sumProd(wA.data(), rT.data(), N);
+            if (perf.nIterations == 0) {
+                pA = wA;
+                pT = wT;
+            } else {
+                const double beta = wArT / wArTold;
+                for (int c = 0; c < N; ++c) {
+                    pA[c] = wA[c] + beta * pA[c];
+                    pT[c] = wT[c] + beta * pT[c];
+                }
+            }
+            Amul(A, pA.data(), wA.data());
+            Tmul(A, pT.data(), wT.data());
+            const double wApT = sumProd(wA.data(), pT.data(), N);
+            if (std::fabs(wApT) / nf < 1e-300) break;
+            const double alpha = wArT / wApT;
+            for (int c = 0; c < N; ++c) {
+                psi[c] += alpha * pA[c];
+                rA[c] -= alpha * wA[c];
+                rT[c] -= alpha * wT[c];
+            }
+            perf.finalResidual = sumMag(rA) / nf;
+        } while ((++perf.nIterations < ctl->maxIter &&
+                  !converged(perf.finalResidual, perf.initialResidual, ctl)) ||
+                 perf.nIterations < ctl->minIter);
+    }
+    perf.converged = converged(perf.finalResidual, perf.initialResidual, ctl) ? 1 : 0;
+    return perf;
+}
+
+}  // namespace
+
+extern "C" {
+
+void orc_grad_gauss(const orc_mesh* m, const double* U, const double* Ub, double* g) {
+    const int N = m->nCells, F = m->nInternal;
+    std::fill(g, g + 9 * (size_t)N, 0.0);
+    for (int f = 0; f < F; ++f) {
+        const int o = m->owner[f], n = m->neighbour[f];
+        double uf[3];
+        for (int d = 0; d < 3; ++d) uf[d] = m->w[f] * (U[3 * o + d] - U[3 * n + d]) + U[3 * n + d];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                const double v = m->Sf[3 * f + i] * uf[j];
+                g[9 * (size_t)o + 3 * i + j] += v;
+                g[9 * (size_t)n + 3 * i + j] -= v;
+            }
+    }
+    for (int p = 0; p < m->nPatches; ++p) {
+        if (m->patchEmpty[p]) continue;
+        for (int k = 0; k < m->patchSize[p]; ++k) {
+            const int b = m->patchStart[p] + k;
+            const int64_t f = (int64_t)F + b;
+            const int c = m->owner[f];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) g[9 * (size_t)c + 3 * i + j] += m->Sf[3 * f + i] * Ub[3 * b + j];
+        }
+    }
+    for (int c = 0; c < N; ++c)
+        for (int q = 0; q < 9; ++q) g[9 * (size_t)c + q] /= m->V[c];
+}
+
+void orc_rhs(const orc_params* p, int32_t n, const double* gradU, const double* theta,
+             const double* eigVals, const double* eigVecs, double* S, double* fOut) {
+    bool mask[3] = {false, false, false};
+    if (p->ofInvQuirk && n > 0) invShiftMask(eigVals, mask);
+    const bool ptt = p->model == ORC_PTT_LINEAR || p->model == ORC_PTT_EXP;
+    for (int c = 0; c < n; ++c) {
+        const double* L = gradU + 9 * (size_t)c;
+        const double* V = eigVecs + 9 * (size_t)c;
+        const double* Lam = eigVals + 9 * (size_t)c;
+        // boilerLog.H:25-31: M = V^T (L^T [- zeta symm(L)]) V
+        T9 LT, M;
+        transp(L, LT);
+        if (ptt) {
+            double sL[6];
+            symmT(L, sL);
+            T9 zs;
+            expandSymm(sL, zs);
+            for (int q = 0; q < 9; ++q) LT[q] = LT[q] - p->zeta * zs[q];
+        }
+        innerP(V, LT, true, M);
+        // decomposeGradU (constitutiveEq.C:323-358)
+        T9 Bt = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Ot = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        Bt[0] = M[0];
+        Bt[4] = M[4];
+        Bt[8] = M[8];
+        const double lx = Lam[0], ly = Lam[4], lz = Lam[8];
+        Ot[1] = (ly * M[1] + lx * M[3]) / (ly - lx + 1e-16);
+        Ot[2] = (lz * M[2] + lx * M[6]) / (lz - lx + 1e-16);
+        Ot[5] = (lz * M[5] + ly * M[7]) / (lz - ly + 1e-16);
+        Ot[3] = -Ot[1];
+        Ot[6] = -Ot[2];
+        Ot[7] = -Ot[5];
+        T9 Om, B;
+        innerP(V, Ot, false, Om);
+        innerP(V, Bt, false, B);
+        // (omega & theta) - (theta & omega) + 2 B
+        T9 th, Ot1, Ot2, R;
+        expandSymm(theta + 6 * (size_t)c, th);
+        dotTT(Om, th, Ot1);
+        dotTT(th, Om, Ot2);
+        for (int q = 0; q < 9; ++q) R[q] = Ot1[q] - Ot2[q] + 2.0 * B[q];
+        // model term
+        T9 invL, X;
+        invField(Lam, mask, invL);
+        const double rl = 1.0 / p->lambda;
+        if (p->model == ORC_OLDROYDB) {
+            T9 D, Y;
+            for (int q = 0; q < 9; ++q) D[q] = invL[q] - I9[q];
+            innerP(V, D, false, Y);
+            for (int q = 0; q < 9; ++q) X[q] = rl * Y[q];
+        } else if (p->model == ORC_GIESEKUS) {
+            T9 D, trhs, t1, C, tt, CT;
+            for (int q = 0; q < 9; ++q) D[q] = invL[q] - I9[q];
+            T9 VT;
+            transp(V, VT);
+            dotTT(V, D, t1);
+            dotTT(t1, VT, trhs);
+            dotTT(V, Lam, t1);
+            dotTT(t1, VT, C);
+            dotTT(trhs, trhs, tt);
+            dotTT(C, tt, CT);
+            for (int q = 0; q < 9; ++q) X[q] = rl * (trhs[q] - p->alpha * CT[q]);
+        } else if (p->model == ORC_FENEP) {
+            T9 VT, t1, C, D, Y;
+            transp(V, VT);
+            dotTT(V, Lam, t1);
+            dotTT(t1, VT, C);
+            const double f = p->L2 / (p->L2 - (C[0] + C[4] + C[8]));
+            const double a = p->L2 / (p->L2 - 3.);
+            if (fOut) fOut[c] = f;
+            for (int q = 0; q < 9; ++q) D[q] = a * invL[q] - f * I9[q];
+            dotTT(V, D, t1);
+            dotTT(t1, VT, Y);
+            for (int q = 0; q < 9; ++q) X[q] = rl * Y[q];
+        } else {  // PTT linear / exponential
+            T9 VT, t1, C, D, Y;
+            transp(V, VT);
+            for (int q = 0; q < 9; ++q) D[q] = invL[q] - I9[q];
+            dotTT(V, D, t1);
+            dotTT(t1, VT, Y);
+            dotTT(V, Lam, t1);
+            dotTT(t1, VT, C);
+            const double k = p->epsilon / (1 - p->zeta);
+            const double z = k * (C[0] + C[4] + C[8] - 3.);
+            const double fac = p->model == ORC_PTT_LINEAR ? 1. + z : std::exp(z);
+            for (int q = 0; q < 9; ++q) X[q] = (rl * Y[q]) * fac;
+        }
+        for (int q = 0; q < 9; ++q) R[q] = R[q] + X[q];
+        symmT(R, S + 6 * (size_t)c);
+    }
+}
+
+void orc_assemble(const orc_mesh* m, const double* phi, const double* theta, const double* S,
+                  double deltaT, const int32_t* bcType, const double* bcValue, double* diag,
+                  double* lower, double* upper, double* source, double* iC, double* bC) {
+    const int N = m->nCells, F = m->nInternal;
+    const double rDeltaT = 1.0 / deltaT;
+    // fvm::ddt Euler
+    std::vector<double> ddtDiag(N), divDiag(N, 0.0);
+    for (int c = 0; c < N; ++c) ddtDiag[c] = rDeltaT * m->V[c];
+    // fvm::div upwind (gaussDefCmpwConvectionScheme.C:93-100); pos(0) = 1
+    for (int f = 0; f < F; ++f) {
+        const double upw = phi[f] >= 0 ? 1.0 : 0.0;
+        lower[f] = -upw * phi[f];
+        upper[f] = (1.0 - upw) * phi[f];
+    }
+    for (int f = 0; f < F; ++f) {  // negSumDiag
+        divDiag[m->owner[f]] -= lower[f];
+        divDiag[m->neighbour[f]] -= upper[f];
+    }
+    for (int c = 0; c < N; ++c) diag[c] = ddtDiag[c] + divDiag[c];
+    // patch coefficients with plim = 1 (:102-116)
+    for (int p = 0; p < m->nPatches; ++p)
+        for (int k = 0; k < m->patchSize[p]; ++k) {
+            const int b = m->patchStart[p] + k;
+            const double pphi = phi[(int64_t)F + b];
+            for (int q = 0; q < 6; ++q) {
+                double ic = 0, bc = 0;
+                if (bcType[p] == ORC_BC_ZEROGRADIENT) {
+                    ic = pphi * 1.0;
+                    bc = -pphi * 0.0;
+                } else if (bcType[p] == ORC_BC_FIXEDVALUE) {
+                    ic = pphi * 0.0;
+                    bc = -pphi * bcValue[6 * (size_t)b + q];
+                }
+                iC[6 * (size_t)b + q] = ic;
+                bC[6 * (size_t)b + q] = bc;
+            }
+        }
+    // sources: ddt (rDeltaT*theta_old*V) + (== S): V*S
+    for (int c = 0; c < N; ++c)
+        for (int q = 0; q < 6; ++q) {
+            const double sd = rDeltaT * theta[6 * (size_t)c + q] * m->V[c];
+            source[6 * (size_t)c + q] = sd + m->V[c] * S[6 * (size_t)c + q];
+        }
+}
+
+void orc_relax(const orc_mesh* m, const int32_t* bcType, double alpha, const double* psi,
+               double* D, const double* lower, const double* upper, double* S,
+               const double* iC) {
+    if (alpha <= 0) return;
+    const int N = m->nCells, F = m->nInternal;
+    std::vector<double> D0(D, D + N), sumOff(N, 0.0);
+    for (int f = 0; f < F; ++f) {
+        sumOff[m->neighbour[f]] += std::fabs(lower[f]);
+        sumOff[m->owner[f]] += std::fabs(upper[f]);
+    }
+    for (int p = 0; p < m->nPatches; ++p) {
+        if (m->patchEmpty[p] || bcType[p] == ORC_BC_EMPTY) continue;
+        for (int k = 0; k < m->patchSize[p]; ++k) {
+            const int b = m->patchStart[p] + k;
+            double mx = 0;
+            for (int q = 0; q < 6; ++q) mx = std::max(mx, std::fabs(iC[6 * (size_t)b + q]));
+            D[m->owner[(int64_t)F + b]] += mx;
+        }
+    }
+    for (int c = 0; c < N; ++c) D[c] = std::max(std::fabs(D[c]), sumOff[c]);
+    for (int c = 0; c < N; ++c) D[c] /= alpha;
+    for (int p = 0; p < m->nPatches; ++p) {
+        if (m->patchEmpty[p] || bcType[p] == ORC_BC_EMPTY) continue;
+        for (int k = 0; k < m->patchSize[p]; ++k) {
+            const int b = m->patchStart[p] + k;
+            double mn = iC[6 * (size_t)b];
+            for (int q = 1; q < 6; ++q) mn = std::min(mn, iC[6 * (size_t)b + q]);
+            D[m->owner[(int64_t)F + b]] -= mn;
+        }
+    }
+    for (int c = 0; c < N; ++c)
+        for (int q = 0; q < 6; ++q) S[6 * (size_t)c + q] += (D[c] - D0[c]) * psi[6 * (size_t)c + q];
+}
+
+orc_perf orc_ldu_solve(int32_t nCells, int32_t nFaces, const int32_t* l, const int32_t* u,
+                       const double* diag, const double* lower, const double* upper,
+                       const double* b, const orc_controls* c, double* x) {
+    Ldu A{nCells, nFaces, l, u, diag, lower, upper};
+    return c->solver == ORC_PBICG ? pbicg(A, b, c, x) : pbicgstab(A, b, c, x);
+}
+
+double orc_foam_residual(int32_t nCells, int32_t nFaces, const int32_t* l, const int32_t* u,
+                         const double* diag, const double* lower, const double* upper,
+                         const double* b, const double* x) {
+    Ldu A{nCells, nFaces, l, u, diag, lower, upper};
+    std::vector<double> Ax(nCells), r(nCells);
+    Amul(A, x, Ax.data());
+    for (int c = 0; c < nCells; ++c) r[c] = b[c] - Ax[c];
+    return sumMag(r) / normFactor(A, x, b, Ax.data());
+}
+
+void orc_solve_segregated(const orc_mesh* m, const double* diag, const double* lower,
+                          const double* upper, const double* source, const double* iC,
+                          const double* bC, const int32_t* bcType, const int32_t valid[6],
+                          const orc_controls* c, double* psi, orc_perf perf[6]) {
+    const int N = m->nCells, F = m->nInternal;
+    std::vector<double> src(source, source + 6 * (size_t)N);
+    // addBoundarySource (non-coupled patches; empty patches have no faces)
+    for (int p = 0; p < m->nPatches; ++p) {
+        if (m->patchEmpty[p] || bcType[p] == ORC_BC_EMPTY) continue;
+        for (int k = 0; k < m->patchSize[p]; ++k) {
+            const int b = m->patchStart[p] + k;
+            const int cell = m->owner[(int64_t)F + b];
+            for (int q = 0; q < 6; ++q) src[6 * (size_t)cell + q] += bC[6 * (size_t)b + q];
+        }
+    }
+    std::vector<double> d(N), x(N), bb(N);
+    for (int q = 0; q < 6; ++q) {
+        perf[q] = orc_perf{0, 0, 0, 0, 0};
+        if (valid[q] == -1) continue;
+        std::copy(diag, diag + N, d.begin());
+        // addBoundaryDiag
+        for (int p = 0; p < m->nPatches; ++p) {
+            if (m->patchEmpty[p] || bcType[p] == ORC_BC_EMPTY) continue;
+            for (int k = 0; k < m->patchSize[p]; ++k) {
+                const int b = m->patchStart[p] + k;
+                d[m->owner[(int64_t)F + b]] += iC[6 * (size_t)b + q];
+            }
+        }
+        for (int cl = 0; cl < N; ++cl) {
+            x[cl] = psi[6 * (size_t)cl + q];
+            bb[cl] = src[6 * (size_t)cl + q];
+        }
+        perf[q] = orc_ldu_solve(N, F, m->owner, m->neighbour, d.data(), lower, upper, bb.data(), c,
+                                x.data());
+        for (int cl = 0; cl < N; ++cl) psi[6 * (size_t)cl + q] = x[cl];
+    }
+}
+
+void orc_calc_eig(int32_t n, const double* theta, double* eigVals, double* eigVecs, int32_t method) {
+    for (int c = 0; c < n; ++c) {
+        double ev[3], Q[9];
+        if (method == 1) {
+            double D[3], Vv[9];
+            eigJacobiH(theta + 6 * (size_t)c, D, Vv);
+            int idx[3] = {0, 1, 2};
+            std::sort(idx, idx + 3, [&](int a, int b) { return D[a] < D[b]; });
+            for (int k = 0; k < 3; ++k) {
+                ev[k] = D[idx[k]];
+                for (int r = 0; r < 3; ++r) Q[3 * r + k] = Vv[3 * r + idx[k]];
+            }
+        } else {
+            eigQR(theta + 6 * (size_t)c, ev, Q);
+        }
+        double* e = eigVals + 9 * (size_t)c;
+        for (int q = 0; q < 9; ++q) e[q] *= 0.;
+        e[0] = std::exp(ev[0]);
+        e[4] = std::exp(ev[1]);
+        e[8] = std::exp(ev[2]);
+        for (int q = 0; q < 9; ++q) eigVecs[9 * (size_t)c + q] = Q[q];
+    }
+}
+
+void orc_tau(const orc_params* p, int32_t n, const double* eigVals, const double* eigVecs,
+             const double* fOld, double* tau) {
+    for (int c = 0; c < n; ++c) {
+        const double* V = eigVecs + 9 * (size_t)c;
+        const double* Lam = eigVals + 9 * (size_t)c;
+        T9 VT, t1, C, D;
+        transp(V, VT);
+        dotTT(V, Lam, t1);
+        dotTT(t1, VT, C);
+        double coef;
+        if (p->model == ORC_FENEP) {
+            const double a = p->L2 / (p->L2 - 3.);
+            const double f = fOld[c];
+            for (int q = 0; q < 9; ++q) D[q] = f * C[q] - a * I9[q];
+            coef = p->etaP / p->lambda;
+        } else {
+            for (int q = 0; q < 9; ++q) D[q] = C[q] - I9[q];
+            coef = (p->model == ORC_PTT_LINEAR || p->model == ORC_PTT_EXP)
+                       ? p->etaP / (p->lambda * (1 - p->zeta))
+                       : p->etaP / p->lambda;
+        }
+        double s[6];
+        symmT(D, s);
+        for (int q = 0; q < 6; ++q) tau[6 * (size_t)c + q] = coef * s[q];
+    }
+}
+
+void orc_correct(const orc_mesh* m, const orc_params* p, const double* U, const double* Ub,
+                 const double* phi, const double* gradU, double deltaT, const int32_t* bcType,
+                 const double* bcValue, const int32_t valid[6], const orc_controls* c,
+                 double relaxFactor, double* theta, double* eigVals, double* eigVecs, double* tau,
+                 orc_perf perf[6]) {
+    const int N = m->nCells, F = m->nInternal, NB = m->nBoundary;
+    std::vector<double> L;
+    if (!gradU) {
+        L.resize(9 * (size_t)N);
+        orc_grad_gauss(m, U, Ub, L.data());
+        gradU = L.data();
+    }
+    std::vector<double> S(6 * (size_t)N), f(N, 0.0);
+    orc_rhs(p, N, gradU, theta, eigVals, eigVecs, S.data(), f.data());
+    std::vector<double> diag(N), lower(F), upper(F), source(6 * (size_t)N), iC(6 * (size_t)NB),
+        bC(6 * (size_t)NB);
+    orc_assemble(m, phi, theta, S.data(), deltaT, bcType, bcValue, diag.data(), lower.data(),
+                 upper.data(), source.data(), iC.data(), bC.data());
+    if (relaxFactor > 0)
+        orc_relax(m, bcType, relaxFactor, theta, diag.data(), lower.data(), upper.data(),
+                  source.data(), iC.data());
+    orc_solve_segregated(m, diag.data(), lower.data(), upper.data(), source.data(), iC.data(),
+                         bC.data(), bcType, valid, c, theta, perf);
+    orc_calc_eig(N, theta, eigVals, eigVecs, 0);
+    orc_tau(p, N, eigVals, eigVecs, f.data(), tau);
+}
+
+}  // extern "C"

@@ -1,0 +1,136 @@
+/*
+ * oracle.h -- CPU FP64 restatement of rheoTool's log-conformation stress step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the
+ * checker.  The product path (librheo_gpu.so) never calls into it.
+ *
+ * Each function follows the reference text cited beside it (paths relative to
+ * the rheoTool tree, of90/src/libs/...).  Where the reference calls
+ * OpenFOAM-9 or Eigen (third-party, not in the reference tree) the behaviour
+ * restated is marked [OF9] / [Eigen 3.2.9].
+ *
+ * Layouts (host, AoS, as OpenFOAM stores them):
+ *   symmTensor: 6 doubles xx xy xz yy yz zz
+ *   tensor:     9 doubles row-major xx xy xz yx yy yz zx zy zz
+ *   vector:     3 doubles
+ */
+#ifndef RHEO_ORACLE_H
+#define RHEO_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* LDU-addressed mesh (primitiveMesh conventions, see rtcase.h) */
+typedef struct {
+    int32_t nCells, nInternal, nBoundary, nPatches;
+    const int32_t* owner;      /* [nInternal + nBoundary] */
+    const int32_t* neighbour;  /* [nInternal] */
+    const double* Sf;          /* [3*(nInternal + nBoundary)] */
+    const double* w;           /* [nInternal] */
+    const double* V;           /* [nCells] */
+    const int32_t* patchStart; /* [nPatches], boundary-face index */
+    const int32_t* patchSize;
+    const int32_t* patchEmpty; /* 1: empty patch (no faces in the fvPatch sense) */
+} orc_mesh;
+
+enum { ORC_OLDROYDB = 0, ORC_GIESEKUS = 1, ORC_FENEP = 2, ORC_PTT_LINEAR = 3, ORC_PTT_EXP = 4 };
+
+typedef struct {
+    int32_t model;
+    double rho, etaS, etaP, lambda;
+    double alpha;   /* Giesekus mobility */
+    double L2;      /* FENE-P extensibility */
+    double epsilon; /* PTT */
+    double zeta;    /* PTT */
+    int32_t ofInvQuirk; /* 1: apply [OF9] inv(tensorField) first-cell component shift */
+} orc_params;
+
+enum { ORC_BC_ZEROGRADIENT = 0, ORC_BC_FIXEDVALUE = 1, ORC_BC_EMPTY = 2 };
+
+enum { ORC_PBICGSTAB = 0, ORC_PBICG = 1 };
+enum { ORC_PC_NONE = 0, ORC_PC_DIAGONAL = 1, ORC_PC_DILU = 2 };
+
+typedef struct {
+    int32_t solver, precond;
+    double tolerance, relTol;
+    int32_t maxIter, minIter;
+} orc_controls;
+
+typedef struct {
+    double initialResidual, finalResidual;
+    int32_t nIterations, converged, solved;
+} orc_perf;
+
+/* [OF9] fvc::grad(U), Gauss linear (boilerLog.H:1).  Ub per boundary face. */
+void orc_grad_gauss(const orc_mesh* m, const double* U, const double* Ub, double* gradU);
+
+/* Per-cell right-hand side S = symm(Omega.theta - theta.Omega + 2B + X)
+ * of the model, from the old eigen state (boilerLog.H, constitutiveEq.C:323-358,
+ * Oldroyd_BLog.C:141-164, GiesekusLog.C:142-160, FENE_PLog.C:142-166,
+ * PTTLog.C:184-196,241-253).  fOut (may be NULL) receives FENE-P's f. */
+void orc_rhs(const orc_params* p, int32_t n, const double* gradU, const double* theta,
+             const double* eigVals, const double* eigVecs, double* S, double* fOut);
+
+/* Euler ddt + upwind GaussDefCmpw div + (== S) assembly
+ * (GDC/gaussDefCmpwConvectionScheme.C:70-118, [OF9] EulerDdtScheme, fvMatrix
+ * operator==).  bcType/bcValue per patch/boundary face for theta.
+ * Outputs: diag[N], lower/upper[nInternal], source[6N],
+ * internalCoeffs/boundaryCoeffs[6*nBoundary]. */
+void orc_assemble(const orc_mesh* m, const double* phi, const double* theta, const double* S,
+                  double deltaT, const int32_t* bcType, const double* bcValue, double* diag,
+                  double* lower, double* upper, double* source, double* internalCoeffs,
+                  double* boundaryCoeffs);
+
+/* [OF9] fvMatrix::relax(1) on the assembled system (Oldroyd_BLog.C:166). */
+void orc_relax(const orc_mesh* m, const int32_t* bcType, double alpha, const double* psi,
+               double* diag, const double* lower, const double* upper, double* source,
+               const double* internalCoeffs);
+
+/* [OF9] fvMatrix<symmTensor>::solveSegregated (restated in
+ * SMS/segregated/eigenSolver/eigenSolver.C:409-563): per valid component,
+ * add boundary diag/source and run the selected lduMatrix solver.
+ * validComponents[6] is +1 / -1.  psi[6N] updated in place. */
+void orc_solve_segregated(const orc_mesh* m, const double* diag, const double* lower,
+                          const double* upper, const double* source,
+                          const double* internalCoeffs, const double* boundaryCoeffs,
+                          const int32_t* bcType, const int32_t validComponents[6],
+                          const orc_controls* c, double* psi, orc_perf perf[6]);
+
+/* One scalar lduMatrix solve (diag already includes boundary coefficients). */
+orc_perf orc_ldu_solve(int32_t nCells, int32_t nFaces, const int32_t* l, const int32_t* u,
+                       const double* diag, const double* lower, const double* upper,
+                       const double* b, const orc_controls* c, double* x);
+
+/* [SMS/segregated/sparseSolver.C:129-180] normalised residual of one component */
+double orc_foam_residual(int32_t nCells, int32_t nFaces, const int32_t* l, const int32_t* u,
+                         const double* diag, const double* lower, const double* upper,
+                         const double* b, const double* x);
+
+/* calcEig (constitutiveEq.C:360-428).  method 0: tridiagonal + implicit QR
+ * with Eigen's scaling/deflation/ascending sort [Eigen 3.2.9
+ * SelfAdjointEigenSolver]; method 1: the cyclic Jacobi of utils/jacobi.H
+ * (result re-sorted ascending). */
+void orc_calc_eig(int32_t n, const double* theta, double* eigVals, double* eigVecs, int32_t method);
+
+/* tau from the new eigen state (Oldroyd_BLog.C:175, GiesekusLog.C:172,
+ * FENE_PLog.C:178 with f from the old state, PTTLog.C:264). */
+void orc_tau(const orc_params* p, int32_t n, const double* eigVals, const double* eigVecs,
+             const double* fOld, double* tau);
+
+/* One full constitutiveEq::correct() of a *Log model (Oldroyd_BLog.C:127-179).
+ * gradU may be NULL (then Gauss linear from U/Ub).  theta/eigVals/eigVecs
+ * are updated in place, tau written.  relaxFactor <= 0 disables relax(). */
+void orc_correct(const orc_mesh* m, const orc_params* p, const double* U, const double* Ub,
+                 const double* phi, const double* gradU, double deltaT, const int32_t* bcType,
+                 const double* bcValue, const int32_t validComponents[6],
+                 const orc_controls* c, double relaxFactor, double* theta, double* eigVals,
+                 double* eigVecs, double* tau, orc_perf perf[6]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

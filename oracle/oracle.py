@@ -1,0 +1,209 @@
+"""ctypes wrapper of the CPU FP64 oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker.  The product path never
+imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+OLDROYDB, GIESEKUS, FENEP, PTT_LINEAR, PTT_EXP = 0, 1, 2, 3, 4
+BC_ZEROGRADIENT, BC_FIXEDVALUE, BC_EMPTY = 0, 1, 2
+PBICGSTAB, PBICG = 0, 1
+PC_NONE, PC_DIAGONAL, PC_DILU = 0, 1, 2
+
+MODEL_IDS = {"Oldroyd-BLog": OLDROYDB, "GiesekusLog": GIESEKUS, "FENE-PLog": FENEP,
+             "PTTLog": PTT_LINEAR}
+
+
+class Mesh(C.Structure):
+    _fields_ = [("nCells", C.c_int32), ("nInternal", C.c_int32), ("nBoundary", C.c_int32),
+                ("nPatches", C.c_int32), ("owner", C.c_void_p), ("neighbour", C.c_void_p),
+                ("Sf", C.c_void_p), ("w", C.c_void_p), ("V", C.c_void_p),
+                ("patchStart", C.c_void_p), ("patchSize", C.c_void_p),
+                ("patchEmpty", C.c_void_p)]
+
+
+class Params(C.Structure):
+    _fields_ = [("model", C.c_int32), ("rho", C.c_double), ("etaS", C.c_double),
+                ("etaP", C.c_double), ("lambda_", C.c_double), ("alpha", C.c_double),
+                ("L2", C.c_double), ("epsilon", C.c_double), ("zeta", C.c_double),
+                ("ofInvQuirk", C.c_int32)]
+
+
+class Controls(C.Structure):
+    _fields_ = [("solver", C.c_int32), ("precond", C.c_int32), ("tolerance", C.c_double),
+                ("relTol", C.c_double), ("maxIter", C.c_int32), ("minIter", C.c_int32)]
+
+
+class Perf(C.Structure):
+    _fields_ = [("initialResidual", C.c_double), ("finalResidual", C.c_double),
+                ("nIterations", C.c_int32), ("converged", C.c_int32), ("solved", C.c_int32)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        vp = C.c_void_p
+        L.orc_grad_gauss.argtypes = [vp] * 4
+        L.orc_rhs.argtypes = [vp, C.c_int32] + [vp] * 6
+        L.orc_assemble.argtypes = [vp, vp, vp, vp, C.c_double] + [vp] * 8
+        L.orc_relax.argtypes = [vp, vp, C.c_double] + [vp] * 6
+        L.orc_solve_segregated.argtypes = [vp] * 12
+        L.orc_ldu_solve.restype = Perf
+        L.orc_ldu_solve.argtypes = [C.c_int32, C.c_int32] + [vp] * 8
+        L.orc_foam_residual.restype = C.c_double
+        L.orc_foam_residual.argtypes = [C.c_int32, C.c_int32] + [vp] * 7
+        L.orc_calc_eig.argtypes = [C.c_int32, vp, vp, vp, C.c_int32]
+        L.orc_tau.argtypes = [vp, C.c_int32, vp, vp, vp, vp]
+        L.orc_correct.argtypes = [vp, vp, vp, vp, vp, vp, C.c_double, vp, vp, vp, vp,
+                                  C.c_double, vp, vp, vp, vp, vp]
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class OracleMesh:
+    """Keeps the numpy arrays alive behind an orc_mesh struct."""
+
+    def __init__(self, m):
+        self.m = m
+        self.empty = np.ascontiguousarray((m.patchKind == 2).astype(np.int32))
+        self.s = Mesh(m.nCells, m.nInternal, m.nBoundary, len(m.patchStart), _p(m.owner),
+                      _p(m.neighbour), _p(m.Sf), _p(m.w), _p(m.V), _p(m.patchStart),
+                      _p(m.patchSize), _p(self.empty))
+
+    @property
+    def ref(self):
+        return C.byref(self.s)
+
+
+def params(model="Oldroyd-BLog", etaS=0.59, etaP=0.41, lam=1.0, alpha=0.0, L2=100.0,
+           epsilon=0.0, zeta=0.0, rho=1.0, ptt_exp=False, of_inv_quirk=False):
+    mid = MODEL_IDS[model]
+    if model == "PTTLog" and ptt_exp:
+        mid = PTT_EXP
+    return Params(mid, rho, etaS, etaP, lam, alpha, L2, epsilon, zeta, int(of_inv_quirk))
+
+
+def controls(solver=PBICGSTAB, precond=PC_DILU, tol=1e-10, relTol=0.0, maxIter=1000, minIter=0):
+    return Controls(solver, precond, tol, relTol, maxIter, minIter)
+
+
+def default_bc(m):
+    return np.where(m.patchKind == 2, BC_EMPTY, BC_ZEROGRADIENT).astype(np.int32)
+
+
+def grad(m, U, Ub):
+    om = OracleMesh(m)
+    g = np.empty((m.nCells, 3, 3))
+    Ub = np.ascontiguousarray(Ub) if m.nBoundary else np.zeros((1, 3))
+    lib().orc_grad_gauss(om.ref, _p(np.ascontiguousarray(U)), _p(Ub), _p(g))
+    return g
+
+
+def rhs(p, gradU, theta, eigVals, eigVecs):
+    n = theta.shape[0]
+    S = np.empty((n, 6))
+    f = np.zeros(n)
+    lib().orc_rhs(C.byref(p), n, _p(np.ascontiguousarray(gradU)), _p(np.ascontiguousarray(theta)),
+                  _p(np.ascontiguousarray(eigVals)), _p(np.ascontiguousarray(eigVecs)), _p(S),
+                  _p(f))
+    return S, f
+
+
+def assemble(m, phi, theta, S, dt, bc=None, bcval=None):
+    om = OracleMesh(m)
+    N, F, NB = m.nCells, m.nInternal, m.nBoundary
+    bc = default_bc(m) if bc is None else bc
+    bcval = np.zeros((max(NB, 1), 6)) if bcval is None else bcval
+    out = dict(diag=np.empty(N), lower=np.empty(F), upper=np.empty(F), source=np.empty((N, 6)),
+               iC=np.empty((max(NB, 1), 6)), bC=np.empty((max(NB, 1), 6)))
+    lib().orc_assemble(om.ref, _p(phi), _p(np.ascontiguousarray(theta)), _p(S), dt, _p(bc),
+                       _p(bcval), _p(out["diag"]), _p(out["lower"]), _p(out["upper"]),
+                       _p(out["source"]), _p(out["iC"]), _p(out["bC"]))
+    return out
+
+
+def solve_segregated(m, sys, psi, ctl=None, bc=None, valid=None):
+    om = OracleMesh(m)
+    bc = default_bc(m) if bc is None else bc
+    valid = m.validComponents if valid is None else np.asarray(valid, np.int32)
+    ctl = controls() if ctl is None else ctl
+    perf = (Perf * 6)()
+    psi = np.ascontiguousarray(psi, dtype=np.float64).copy()
+    lib().orc_solve_segregated(om.ref, _p(sys["diag"]), _p(sys["lower"]), _p(sys["upper"]),
+                               _p(sys["source"]), _p(sys["iC"]), _p(sys["bC"]), _p(bc),
+                               _p(valid), C.byref(ctl), _p(psi), perf)
+    return psi, [perf[i] for i in range(6)]
+
+
+def ldu_solve(n, l, u, diag, lower, upper, b, x0, ctl=None):
+    ctl = controls() if ctl is None else ctl
+    x = np.ascontiguousarray(x0, dtype=np.float64).copy()
+    perf = lib().orc_ldu_solve(n, len(l), _p(l), _p(u), _p(diag), _p(lower), _p(upper), _p(b),
+                               C.byref(ctl), _p(x))
+    return x, perf
+
+
+def foam_residual(n, l, u, diag, lower, upper, b, x):
+    return lib().orc_foam_residual(n, len(l), _p(l), _p(u), _p(diag), _p(lower), _p(upper), _p(b),
+                                   _p(np.ascontiguousarray(x)))
+
+
+def calc_eig(theta, method=0):
+    n = theta.shape[0]
+    ev = np.zeros((n, 3, 3))
+    vec = np.empty((n, 3, 3))
+    lib().orc_calc_eig(n, _p(np.ascontiguousarray(theta)), _p(ev), _p(vec), method)
+    return ev, vec
+
+
+def tau(p, eigVals, eigVecs, fOld=None):
+    n = eigVals.shape[0]
+    t = np.empty((n, 6))
+    fOld = np.zeros(n) if fOld is None else fOld
+    lib().orc_tau(C.byref(p), n, _p(np.ascontiguousarray(eigVals)),
+                  _p(np.ascontiguousarray(eigVecs)), _p(fOld), _p(t))
+    return t
+
+
+def correct(m, p, fields, dt, ctl=None, bc=None, bcval=None, gradU=None, relax=0.0,
+            use_grad=False):
+    """One constitutiveEq::correct(); returns (theta, eigVals, eigVecs, tau, perf)."""
+    om = OracleMesh(m)
+    bc = default_bc(m) if bc is None else bc
+    NB = m.nBoundary
+    bcval = np.zeros((max(NB, 1), 6)) if bcval is None else bcval
+    ctl = controls() if ctl is None else ctl
+    th = fields.theta.copy()
+    ev = fields.eigVals.copy()
+    vec = fields.eigVecs.copy()
+    t = np.empty_like(th)
+    perf = (Perf * 6)()
+    g = gradU if gradU is not None else (fields.gradU if use_grad else None)
+    Ub = fields.Ub if NB else np.zeros((1, 3))
+    lib().orc_correct(om.ref, C.byref(p), _p(fields.U), _p(Ub), _p(fields.phi),
+                      _p(None if g is None else np.ascontiguousarray(g)), dt, _p(bc), _p(bcval),
+                      _p(m.validComponents), C.byref(ctl), relax, _p(th), _p(ev), _p(vec), _p(t),
+                      perf)
+    return th, ev, vec, t, [perf[i] for i in range(6)]

@@ -1,0 +1,273 @@
+"""Writes the committed golden fixtures for the oracle and the HIP path.
+
+Every expected value here comes from numpy / scipy (LAPACK eigh, dense linear
+algebra, scipy.sparse spsolve, scipy.optimize.fsolve) applied to formulations
+that are independent of the oracle's code:
+
+* eig.npz       -- seeded symmetric 3x3 theta (generic, degenerate,
+                   near-degenerate, diagonal, 1e-12..1e6 spreads) with
+                   numpy.linalg.eigh eigenvalues and reconstructed
+                   C = V exp(L) V^T.
+* rhs.npz       -- log-conformation right-hand sides from the Daleckii-Krein
+                   derivative of the matrix logarithm applied to the
+                   conformation-tensor equation (an identity of the
+                   log-conformation method for distinct eigenvalues), per model.
+* steady.npz    -- steady homogeneous-flow conformation tensors from fsolve on
+                   the conformation-tensor ODE (shear and uniaxial extension),
+                   at which the log-conformation right-hand side must vanish.
+* ldu_<m>.npz   -- tiny meshes: dense upwind+Euler matrix and right-hand side
+                   assembled cell-by-cell from the face list, and the spsolve
+                   solution of every component.
+
+Run from the repo root: python tests/golden/make_golden.py
+Meshes and fields come from rheotool_amd.cases (the synthetic case
+generator), which is input preparation shared by every path.
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+import scipy.optimize
+import scipy.sparse
+import scipy.sparse.linalg
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+from rheotool_amd import cases  # noqa: E402
+
+IJ = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+
+
+def sym6(M):
+    return np.array([M[..., i, j] for i, j in IJ]).T if M.ndim == 3 else np.array(
+        [M[i, j] for i, j in IJ])
+
+
+def full(t6):
+    t6 = np.asarray(t6)
+    M = np.empty(t6.shape[:-1] + (3, 3))
+    for q, (i, j) in enumerate(IJ):
+        M[..., i, j] = t6[..., q]
+        M[..., j, i] = t6[..., q]
+    return M
+
+
+def rand_rot(rng):
+    q = rng.standard_normal(4)
+    q /= np.linalg.norm(q)
+    a, b, c, d = q
+    return np.array([[1 - 2 * (c * c + d * d), 2 * (b * c - a * d), 2 * (b * d + a * c)],
+                     [2 * (b * c + a * d), 1 - 2 * (b * b + d * d), 2 * (c * d - a * b)],
+                     [2 * (b * d - a * c), 2 * (c * d + a * b), 1 - 2 * (b * b + c * c)]])
+
+
+# ----------------------------------------------------------------- eigen
+def make_eig(rng):
+    th = []
+    for _ in range(200):  # generic
+        th.append(sym6(full(rng.standard_normal(6))))
+    for _ in range(100):  # config-2 style: ln l in [-20, 5]
+        R = rand_rot(rng)
+        th.append(sym6(R @ np.diag(rng.uniform(-20, 5, 3)) @ R.T))
+    for _ in range(100):  # 1e-12 .. 1e6 conformation eigenvalues
+        R = rand_rot(rng)
+        th.append(sym6(R @ np.diag(rng.uniform(np.log(1e-12), np.log(1e6), 3)) @ R.T))
+    for _ in range(50):  # exactly degenerate pair
+        R = rand_rot(rng)
+        a, b = rng.uniform(-5, 5, 2)
+        th.append(sym6(R @ np.diag([a, a, b]) @ R.T))
+    for _ in range(50):  # near-degenerate pair (gap 1e-9 .. 1e-5)
+        R = rand_rot(rng)
+        a = rng.uniform(-5, 5)
+        th.append(sym6(R @ np.diag([a, a + 10 ** rng.uniform(-9, -5), a + 1]) @ R.T))
+    for _ in range(30):  # diagonal
+        th.append(np.array([rng.uniform(-3, 3), 0, 0, rng.uniform(-3, 3), 0, rng.uniform(-3, 3)]))
+    th.append(np.zeros(6))            # isotropic, theta = 0
+    th.append(np.array([2.0, 0, 0, 2.0, 0, 2.0]))  # triple degenerate
+    th = np.array(th)
+    w, v = np.linalg.eigh(full(th))
+    C = np.einsum("nik,nk,njk->nij", v, np.exp(w), v)
+    np.savez_compressed(os.path.join(HERE, "eig.npz"), theta=th, evals=w, C=C)
+
+
+# ------------------------------------------------- log-conformation RHS
+def model_P(model, C, prm):
+    """Conformation-tensor relaxation term: dC/dt = kC + Ck^T + P(C)/lambda."""
+    I = np.eye(3)
+    if model == "Oldroyd-BLog":
+        return -(C - I)
+    if model == "GiesekusLog":
+        return -((C - I) + prm["alpha"] * (C - I) @ (C - I))
+    if model == "FENE-PLog":
+        L2 = prm["L2"]
+        f = L2 / (L2 - np.trace(C))
+        a = L2 / (L2 - 3.0)
+        return -(f * C - a * I)
+    if model == "PTTLog":
+        Y = 1.0 + prm["epsilon"] / (1 - prm["zeta"]) * (np.trace(C) - 3.0)
+        return -Y * (C - I)
+    raise KeyError(model)
+
+
+def dlog(C, E):
+    """Daleckii-Krein: directional derivative of logm at SPD C along E."""
+    lam, V = np.linalg.eigh(C)
+    Et = V.T @ E @ V
+    G = np.empty((3, 3))
+    for i in range(3):
+        for j in range(3):
+            if i == j:
+                G[i, j] = 1.0 / lam[i]
+            else:
+                G[i, j] = (np.log(lam[i]) - np.log(lam[j])) / (lam[i] - lam[j])
+    return V @ (Et * G) @ V.T
+
+
+def log_rhs(model, C, gradU, prm):
+    """d(log C)/dt for a homogeneous flow, conformation form -> log form."""
+    kap = gradU.T  # kappa_ij = d u_i / d x_j ; gradU_ij = d u_j / d x_i
+    if model == "PTTLog":
+        D = 0.5 * (kap + kap.T)
+        kap = kap - prm["zeta"] * D  # Gordon-Schowalter derivative
+    E = kap @ C + C @ kap.T + model_P(model, C, prm) / prm["lambda"]
+    return dlog(C, E)
+
+
+MODELS = {
+    "Oldroyd-BLog": dict(lambda_=1.7, alpha=0.0, L2=100.0, epsilon=0.0, zeta=0.0),
+    "GiesekusLog": dict(lambda_=0.8, alpha=0.2, L2=100.0, epsilon=0.0, zeta=0.0),
+    "FENE-PLog": dict(lambda_=2.3, alpha=0.0, L2=100.0, epsilon=0.0, zeta=0.0),
+    "PTTLog": dict(lambda_=1.1, alpha=0.0, L2=100.0, epsilon=0.25, zeta=0.1),
+}
+
+
+def make_rhs(rng):
+    out = {}
+    for model, prm in MODELS.items():
+        p = dict(prm)
+        p["lambda"] = p.pop("lambda_")
+        ths, evs, vecs, Ls, S = [], [], [], [], []
+        for _ in range(60):
+            R = rand_rot(rng)
+            hi = np.log(25.0) if model == "FENE-PLog" else 3.0
+            ln = np.sort(rng.uniform(-3, hi, 3))
+            if np.min(np.diff(ln)) < 0.05:
+                continue
+            C = R @ np.diag(np.exp(ln)) @ R.T
+            th = R @ np.diag(ln) @ R.T
+            L = rng.standard_normal((3, 3))
+            w, V = np.linalg.eigh(th)  # eigen state as calcEig stores it
+            ths.append(sym6(th))
+            evs.append(np.diag(np.exp(w)))
+            vecs.append(V)
+            Ls.append(L)
+            S.append(sym6(log_rhs(model, C, L, p)))
+        key = model.replace("-", "_")
+        out[key + "_theta"] = np.array(ths)
+        out[key + "_eigVals"] = np.array(evs)
+        out[key + "_eigVecs"] = np.array(vecs)
+        out[key + "_gradU"] = np.array(Ls)
+        out[key + "_S"] = np.array(S)
+        out[key + "_params"] = np.array([p["lambda"], p["alpha"], p["L2"], p["epsilon"], p["zeta"]])
+    np.savez_compressed(os.path.join(HERE, "rhs.npz"), **out)
+
+
+def make_steady():
+    """Steady homogeneous flows: solve kC + Ck^T + P(C)/lambda = 0 for SPD C."""
+    out = {}
+    flows = {
+        "shear": lambda r: np.array([[0, 0, 0], [r, 0, 0], [0, 0, 0.0]]),  # u_x = r y
+        "extension": lambda r: np.diag([r, -r / 2, -r / 2]),                 # uniaxial
+    }
+    rates = {"shear": [0.3, 1.0, 3.0], "extension": [0.1, 0.2]}
+    for model, prm in MODELS.items():
+        p = dict(prm)
+        p["lambda"] = p.pop("lambda_")
+        for fname, mk in flows.items():
+            for r in rates[fname]:
+                if fname == "extension" and r * p["lambda"] >= 0.5 and model == "Oldroyd-BLog":
+                    continue
+                gU = mk(r)
+                kap = gU.T
+                if model == "PTTLog":
+                    D = 0.5 * (kap + kap.T)
+                    kap = kap - p["zeta"] * D
+
+                def resid(x):
+                    C = full(x)
+                    return sym6(kap @ C + C @ kap.T + model_P(model, C, p) / p["lambda"])
+
+                sol = scipy.optimize.fsolve(resid, sym6(np.eye(3)), xtol=1e-14, full_output=True)
+                x = sol[0]
+                C = full(x)
+                if np.max(np.abs(resid(x))) > 1e-11 or np.min(np.linalg.eigvalsh(C)) <= 0:
+                    continue
+                key = f"{model.replace('-', '_')}__{fname}__{r}"
+                out[key + "__C"] = C
+                out[key + "__gradU"] = gU
+    np.savez_compressed(os.path.join(HERE, "steady.npz"), **out)
+
+
+# ------------------------------------------------------- tiny LDU systems
+def dense_system(m, phi, theta, S, dt):
+    """Dense upwind + Euler matrix, built per cell from its faces.
+
+    Row P: V/dt + sum over faces of max(F_out, 0) on the diagonal and
+    min(F_out, 0) towards the other cell, F_out the flux leaving P;
+    zeroGradient boundary faces add F_out to the diagonal."""
+    N, F = m.nCells, m.nInternal
+    A = np.zeros((N, N))
+    b = np.zeros((N, 6))
+    for c in range(N):
+        A[c, c] += m.V[c] / dt
+        b[c] += m.V[c] * (theta[c] / dt + S[c])
+    for f in range(F):
+        P, Q = m.owner[f], m.neighbour[f]
+        for row, other, fout in ((P, Q, phi[f]), (Q, P, -phi[f])):
+            if fout > 0 or (fout == 0 and row == P):
+                A[row, row] += fout
+            else:
+                A[row, other] += fout
+    for p in range(m.nPatches):
+        if m.patchKind[p] == cases.EMPTY:
+            continue
+        for k in range(m.patchSize[p]):
+            fb = F + m.patchStart[p] + k
+            A[m.owner[fb], m.owner[fb]] += phi[fb]
+    return A, b
+
+
+def make_ldu(rng):
+    specs = {
+        "2x2x1": dict(n=(2, 2, 1), twoD=True, channel=True, jitter=0.0, perm=0, kind=0),
+        "3x3x1": dict(n=(3, 3, 1), twoD=True, channel=True, jitter=0.0, perm=0, kind=0),
+        "3x3x3": dict(n=(3, 3, 3), twoD=False, channel=False, jitter=0.0, perm=0, kind=1),
+        "4x4x4j": dict(n=(4, 4, 4), twoD=False, channel=False, jitter=0.1, perm=3, kind=1),
+        "5x4x1p": dict(n=(5, 4, 1), twoD=True, channel=True, jitter=0.1, perm=5, kind=0),
+    }
+    for name, s in specs.items():
+        m = cases.cartesian(*s["n"], twoD=s["twoD"], channel=s["channel"], jitter=s["jitter"],
+                            perm_seed=s["perm"])
+        f = cases.make_fields(m, s["kind"], seed=7, lnLo=-3, lnHi=2)
+        S = rng.standard_normal((m.nCells, 6))
+        dt = 0.05
+        A, b = dense_system(m, f.phi, f.theta, S, dt)
+        x = np.empty_like(b)
+        As = scipy.sparse.csc_matrix(A)
+        for q in range(6):
+            x[:, q] = scipy.sparse.linalg.spsolve(As, b[:, q])
+        np.savez_compressed(os.path.join(HERE, f"ldu_{name}.npz"), A=A, b=b, x=x, S=S, dt=dt,
+                            theta=f.theta, phi=f.phi, spec=np.array(
+                                [*s["n"], int(s["twoD"]), int(s["channel"]), s["jitter"],
+                                 s["perm"], s["kind"]], dtype=np.float64))
+
+
+if __name__ == "__main__":
+    rng = np.random.default_rng(20261202)
+    make_eig(rng)
+    make_rhs(rng)
+    make_steady()
+    make_ldu(rng)
+    print("golden fixtures written to", HERE)
