@@ -1,0 +1,224 @@
+#!/usr/bin/env python3
+"""Stress-step throughput of the log-conformation step on one MI355X.
+
+Default workload (the headline): the 16.8M-cell precision form -- a 256^3
+box with cells permuted by seed and nodes jittered +-10%, FENE-P-Log
+(L2=100, Wi=5, beta=0.59, dt=5e-4), FP32 Krylov + FP64 refinement to the
+reference's 1e-10 normalised residual, FP64 eigen/log/exp.  One step = one
+constitutiveEq::correct() (gradient, decomposition, assembly, six-component
+solve, eigen-decomposition, tau) with every input resident in HBM.
+
+Also reported on the same inputs: the plain FP64 drop-in (time and per-field
+error of the mixed result against it), the dominant kernel's HBM roofline
+fraction from HIP events, per-step p50/p99 latency, and the CPU oracle
+(reference algorithm, PBiCGStab + DILU, one thread) timed on a bounded
+sample of the same workload.
+
+python bench.py [--steps K] [--warmup W] [--workload precision|dropin|channel]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "stress-step cell-updates/s at 1e-9 rel err vs FP64; HBM roofline %; p99 ms"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # name: (mesh kwargs, model, model kwargs, lam, dt, ln range, precision)
+    "precision": dict(n=256, perm_seed=3, jitter=0.1, model="FENE-PLog", kw=dict(L2=100.0),
+                      lam=5.0, dt=5e-4, ln=(-8.0, float(np.log(30.0))), precision="mixed",
+                      desc="256^3 box, cells permuted (seed 3), nodes jittered +-10%, FENE-P-Log "
+                           "L2=100 Wi=5 beta=0.59 dt=5e-4; FP32 Krylov + FP64 refinement to "
+                           "1e-10; FP64 eigen/log/exp"),
+    "dropin": dict(n=128, perm_seed=0, jitter=0.0, model="GiesekusLog", kw=dict(alpha=0.1),
+                   lam=10.0, dt=5e-4, ln=(-9.0, 5.0), precision="double",
+                   desc="128^3 box, Taylor-Green + shear, Giesekus-Log alpha=0.1 Wi=10 dt=5e-4; "
+                        "plain FP64 PBiCGStab to 1e-10"),
+    "channel": dict(n=128, model="Oldroyd-BLog", kw={}, lam=1.0, dt=1e-3, precision="double",
+                    desc="128x128x1 channel, Oldroyd-B-Log Wi=1 beta=0.59 dt=1e-3; FP64"),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def make_case(w, n=None):
+    from rheotool_amd import cases
+    n = n or w["n"]
+    if w["model"] == "Oldroyd-BLog" and "perm_seed" not in w:
+        return cases.channel2d(n)
+    return cases.box3d(n, lnLo=w["ln"][0], lnHi=w["ln"][1], perm_seed=w.get("perm_seed", 0),
+                       jitter=w.get("jitter", 0.0))
+
+
+def new_model(ctx, dm, w, f, precision):
+    from rheotool_amd import abi
+    eq = ctx.constitutiveEq("", dm, abi.model_dict(w["model"], etaS=0.59, etaP=0.41, lam=w["lam"],
+                                                   **w["kw"]))
+    eq.set_solver(abi.solver_dict(1e-10, precision=precision))
+    eq.set_state(f.theta, f.eigVals, f.eigVecs)
+    eq.set_flow(f.U, f.Ub, f.phi)
+    return eq
+
+
+def cpu_baseline(w, n_sample):
+    """The oracle (reference algorithm: PBiCGStab + DILU, FP64, one thread) on a
+    bounded sample of the same workload."""
+    from oracle import oracle as O
+    m, f = make_case(w, n_sample)
+    p = O.params(w["model"], etaS=0.59, etaP=0.41, lam=w["lam"], alpha=w["kw"].get("alpha", 0.0),
+                 L2=w["kw"].get("L2", 100.0))
+    t0 = time.perf_counter()
+    th, ev, vec, tau, perf = O.correct(m, p, f, w["dt"])
+    dt = time.perf_counter() - t0
+    return m, f, (th, tau), dict(value=m.nCells / dt, unit="cell-updates/s", cores=1, kind="port",
+                                 sample=f"one full stress step (oracle, PBiCGStab+DILU, FP64) on the "
+                                        f"{n_sample}^3 version of the same workload "
+                                        f"({m.nCells} cells), {dt:.2f} s",
+                                 iterations=[int(q.nIterations) for q in perf])
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="precision", choices=sorted(WORKLOADS))
+    ap.add_argument("--n", type=int, default=0, help="override the mesh size")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64-compare", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=64)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported by this build "
+              "(the stress step shards by cell partition; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from rheotool_amd import abi
+    w = WORKLOADS[args.workload]
+    n = args.n or w["n"]
+    t0 = time.perf_counter()
+    m, f = make_case(w, n)
+    log(f"[bench] case {args.workload}: {m.nCells} cells, {m.nInternal} faces, "
+        f"generated in {time.perf_counter() - t0:.1f} s")
+    ctx = abi.Context(0)
+    t0 = time.perf_counter()
+    dm = ctx.mesh(m)
+    log(f"[bench] mesh uploaded in {time.perf_counter() - t0:.1f} s")
+    eq = new_model(ctx, dm, w, f, w["precision"])
+    log(f"[bench] device bytes in use {ctx.bytes_in_use / 1e9:.2f} GB")
+
+    # one step from the initial state, also through the FP64 path: error table
+    errors = None
+    fp64_ms = None
+    eq.correct(w["dt"])
+    first = dict(theta=eq.theta(), tau=eq.tau())
+    log(f"[bench] first step perf {eq.solver_perf()}")
+    if w["precision"] == "mixed" and not args.no_fp64_compare:
+        ref = new_model(ctx, dm, w, f, "double")
+        ref.correct(w["dt"])
+        errors = dict(theta=rel_l2(first["theta"], ref.theta()), tau=rel_l2(first["tau"], ref.tau()))
+        log(f"[bench] mixed vs FP64 drop-in after one step: {errors}")
+        for _ in range(max(0, args.warmup - 1)):
+            ref.correct(w["dt"])
+        ts = []
+        for _ in range(min(args.steps, 5)):
+            t1 = time.perf_counter()
+            ref.correct(w["dt"])
+            ts.append(time.perf_counter() - t1)
+        fp64_ms = 1e3 * float(np.median(ts))
+        log(f"[bench] FP64 drop-in median step {fp64_ms:.2f} ms, perf {ref.solver_perf()}")
+        ref.close()
+    for _ in range(max(0, args.warmup - 1)):
+        eq.correct(w["dt"])
+
+    # timed region: K steps, inputs resident in HBM, kernel events on
+    ctx.reset_stats()
+    ctx.set_profiling(True)
+    step_ms = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t1 = time.perf_counter()
+        eq.correct(w["dt"])
+        step_ms.append(1e3 * (time.perf_counter() - t1))
+    elapsed = time.perf_counter() - t_start
+    stats = ctx.kernel_stats()
+    ctx.set_profiling(False)
+    iters = [p.nIterations for p in eq.solver_perf()]
+    log(f"[bench] last step perf {eq.solver_perf()}")
+    value = m.nCells * args.steps / elapsed
+    dom = max(stats.items(), key=lambda kv: kv[1]["ms"])
+    name, st = dom
+    avg_ms = st["ms"] / st["launches"]
+    achieved = st["bytes"] / st["launches"] / (avg_ms * 1e-3) / 1e9
+    step_gpu_ms = sum(s["ms"] for s in stats.values()) / args.steps
+    step_bytes = sum(s["bytes"] for s in stats.values()) / args.steps
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+    if os.path.exists(tpath):
+        try:
+            traffic = json.load(open(tpath)).get(name)
+        except Exception:
+            traffic = None
+    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, kernel=name,
+                    avg_launch_ms=round(avg_ms, 4), launches_per_step=st["launches"] / args.steps)
+    kernels = {k: dict(ms_per_step=round(v["ms"] / args.steps, 3),
+                       GBps=round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None)
+               for k, v in sorted(stats.items(), key=lambda kv: -kv[1]["ms"])}
+
+    cpu = None
+    err_sample = None
+    if not args.no_cpu_baseline and w["model"] != "Oldroyd-BLog" or (
+            not args.no_cpu_baseline and args.workload == "channel"):
+        ns = args.cpu_sample if args.workload != "channel" else n
+        ms, fs, (th_o, tau_o), cpu = cpu_baseline(w, ns)
+        dms = ctx.mesh(ms)
+        es = new_model(ctx, dms, w, fs, w["precision"])
+        es.correct(w["dt"])
+        err_sample = dict(theta=rel_l2(es.theta(), th_o), tau=rel_l2(es.tau(), tau_o))
+        es.close()
+        dms.close()
+        log(f"[bench] cpu baseline {cpu['value']:.3e} cell-updates/s; GPU vs oracle on the sample "
+            f"{err_sample}")
+
+    line = dict(
+        metric=METRIC, value=round(value, 1), unit="cell-updates/s", n_gpus=1, steps=args.steps,
+        warmup=args.warmup, ms_per_step=round(1e3 * elapsed / args.steps, 3),
+        higher_is_better=True, scaling="weak", vs_baseline=None,
+        dtype="f32+f64" if w["precision"] == "mixed" else "f64", data="synthetic",
+        config=dict(workload=w["desc"], cells=m.nCells, internal_faces=m.nInternal,
+                    model=w["model"], precision=w["precision"]),
+        roofline=roofline, cpu_baseline=cpu,
+        p50_ms=round(float(np.percentile(step_ms, 50)), 3),
+        p99_ms=round(float(np.percentile(step_ms, 99)), 3),
+        gpu_ms_per_step=round(step_gpu_ms, 3),
+        step_algorithmic_GB=round(step_bytes / 1e9, 3),
+        step_roofline_frac=round(step_bytes / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
+        iterations=iters,
+        fp64_dropin_ms_per_step=None if fp64_ms is None else round(fp64_ms, 3),
+        speedup_vs_fp64_dropin=None if fp64_ms is None else round(fp64_ms / (1e3 * elapsed / args.steps), 3),
+        err_vs_fp64=errors, err_vs_oracle_sample=err_sample, kernels=kernels,
+    )
+    print(json.dumps(line), flush=True)
+    eq.close()
+    dm.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

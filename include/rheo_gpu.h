@@ -1,0 +1,270 @@
+/*
+ * rheo_gpu.h -- C-ABI of the MI355X (gfx950) log-conformation stress step.
+ *
+ * This is the drop-in boundary for rheoTool's viscoelastic stress step:
+ * constitutiveEq::correct() of the *Log models and the segregated lduMatrix
+ * solve of the six theta components behind fvMatrix::solve().  Plain C types
+ * only: pointers, sizes, status codes.  Host arrays passed in are borrowed for
+ * the duration of the call; the library never frees caller memory.
+ *
+ * Layouts of host arrays are OpenFOAM's (AoS per cell/face):
+ *   vector      3 doubles  x y z
+ *   symmTensor  6 doubles  xx xy xz yy yz zz
+ *   tensor      9 doubles  xx xy xz yx yy yz zx zy zz (row-major)
+ * Reference paths are relative to rheoTool's of90/src/libs/.
+ *
+ * Threading: a context binds one device and one HIP stream; calls on one
+ * context are not thread-safe, separate contexts are independent (as the
+ * reference's single-threaded-per-rank execution).
+ */
+#ifndef RHEO_GPU_H
+#define RHEO_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status codes
+ * Reference: FatalError aborts the process (unknown model type
+ * constitutiveEquations/constitutiveEqs/constitutiveEq/newConstitutiveEq.C:47-58,
+ * unknown solver sparseMatrixSolvers/segregated/sparseSolver.C:48-56, missing
+ * dictionary keys via dict.lookup).  Here they become status codes and the
+ * same message text is kept for rt_last_error().  Non-convergence is not an
+ * error in the reference (maxIter just stops); it is reported as the warning
+ * bit RT_W_NOT_CONVERGED on an otherwise successful call. */
+enum rt_status {
+    RT_OK = 0,
+    RT_E_UNKNOWN_TYPE = 1, /* unknown constitutiveEq / solver / preconditioner keyword */
+    RT_E_BAD_ARG = 2,      /* NULL or inconsistent argument, missing dictionary key   */
+    RT_E_OOM = 3,          /* device allocation failed or HBM budget exceeded          */
+    RT_E_HIP = 4,          /* HIP runtime error                                        */
+    RT_E_UNSUPPORTED = 5,  /* valid in the reference but outside this path (e.g.
+                              transform patches, coupled interfaces)                    */
+    RT_E_NO_DEVICE = 6,    /* no gfx950 device visible                                 */
+    RT_W_NOT_CONVERGED = 0x100
+};
+
+typedef struct rt_ctx rt_ctx;
+typedef struct rt_mesh rt_mesh;
+typedef struct rt_model rt_model;
+
+/* ---------------------------------------------------------------- context */
+
+/* Number of visible HIP devices (0 when none). */
+int rt_device_count(void);
+
+/* Creates a context on `device` with one HIP stream.  hbm_budget bytes caps
+ * the library's own device allocations (0 = no cap).  Replaces nothing in the
+ * reference (CPU-only); it is the device/stream owner every call needs. */
+int rt_ctx_create(int device, size_t hbm_budget, rt_ctx** out);
+void rt_ctx_destroy(rt_ctx* ctx);
+
+/* Last error message of the context (empty string when none). */
+const char* rt_last_error(const rt_ctx* ctx);
+
+/* Device bytes currently allocated by the context. */
+size_t rt_ctx_bytes_in_use(const rt_ctx* ctx);
+
+/* Kernel timing: when enabled, every launch is bracketed by HIP events on the
+ * context's stream.  rt_ctx_kernel_stats fills up to `cap` records (name,
+ * launches, total ms) and returns the number of distinct kernels. */
+int rt_ctx_set_profiling(rt_ctx* ctx, int enable);
+typedef struct {
+    char name[48];
+    int64_t launches;
+    double total_ms;
+    double bytes; /* algorithmic bytes summed over the launches */
+} rt_kernel_stat;
+int rt_ctx_kernel_stats(rt_ctx* ctx, rt_kernel_stat* out, int cap);
+void rt_ctx_reset_stats(rt_ctx* ctx);
+
+/* ------------------------------------------------------------------- mesh
+ * The primitiveMesh/lduAddressing the reference's path runs on:
+ * lowerAddr(=owner)/upperAddr(=neighbour) per internal face with
+ * owner < neighbour in upper-triangular order, boundary faces grouped by
+ * patch after the internal ones, patchAddr = owner of each boundary face
+ * (sparseMatrixSolvers/segregated/eigenSolver/eigenSolver.C:600-641).
+ * Geometry: face area vectors Sf (out of the owner), linear weights w,
+ * cell volumes V -- the inputs of fvc::grad(U) Gauss linear
+ * (constitutiveEqs/utils/boilerLog.H:1) and of fvm::ddt. */
+enum rt_patch_kind { RT_PATCH_PATCH = 0, RT_PATCH_WALL = 1, RT_PATCH_EMPTY = 2 };
+
+typedef struct {
+    int32_t start; /* first face of the patch, counted from the first boundary face */
+    int32_t size;
+    int32_t kind;  /* rt_patch_kind; empty patches contribute nothing (2D cases) */
+} rt_patch;
+
+typedef struct {
+    int32_t nCells;
+    int32_t nInternalFaces;
+    int32_t nBoundaryFaces;
+    int32_t nPatches;
+    const int32_t* owner;     /* [nInternalFaces + nBoundaryFaces] */
+    const int32_t* neighbour; /* [nInternalFaces] */
+    const double* Sf;         /* [3*(nInternalFaces + nBoundaryFaces)] */
+    const double* weights;    /* [nInternalFaces] */
+    const double* V;          /* [nCells] */
+    const rt_patch* patches;  /* [nPatches] */
+} rt_mesh_desc;
+
+enum rt_renumber { RT_RENUMBER_AUTO = 0, RT_RENUMBER_NONE = 1, RT_RENUMBER_RCM = 2 };
+
+/* Uploads a mesh.  The device copy is renumbered for locality (reverse
+ * Cuthill-McKee on the face graph) when `renumber` asks for it or, with
+ * RT_RENUMBER_AUTO, when the input ordering has poor locality.  Every field
+ * crossing the C-ABI stays in the caller's numbering. */
+int rt_mesh_create(rt_ctx* ctx, const rt_mesh_desc* desc, int renumber, rt_mesh** out);
+void rt_mesh_destroy(rt_mesh* mesh);
+
+/* validComponents of a symmTensor on this mesh ([OF9]
+ * fvMesh::validComponents<symmTensor>(), restated in
+ * sparseMatrixSolvers/segregated/eigenSolver/eigenSolver.C:69-78): +1 solved,
+ * -1 frozen (xz, yz of a 2D case). */
+int rt_mesh_valid_components(const rt_mesh* mesh, int32_t out[6]);
+
+/* ------------------------------------------------------- solver controls
+ * The fvSolution `solvers { theta { ... } }` entry
+ * (e.g. tutorials/rheoFoam/Channel/Oldroyd-BLog/system/fvSolution:29-42),
+ * as OpenFOAM dictionary text:
+ *   solver PBiCGStab;  (PBiCG accepted: solved with PBiCGStab)
+ *   preconditioner DILU;  (DILU -> multicolour ILU0; DIC/diagonal -> Jacobi;
+ *                          none; or the explicit names multicolourILU0, Jacobi)
+ *   tolerance 1e-10; relTol 0; maxIter 1000; minIter 0;
+ *   precision double;  (or "mixed": FP32 Krylov + FP64 iterative refinement)
+ * Unknown solver / preconditioner keywords -> RT_E_UNKNOWN_TYPE listing the
+ * valid ones (sparseSolver.C:48-56 style). */
+typedef struct {
+    double initialResidual;
+    double finalResidual;
+    int32_t nIterations;  /* Krylov iterations (inner iterations for "mixed") */
+    int32_t nRefinements; /* FP64 refinement sweeps ("mixed"), 0 otherwise */
+    int32_t converged;
+    int32_t solved;       /* 0 for a frozen (invalid) component */
+} rt_solver_perf;
+
+/* ----------------------------------------------------------------- model
+ * constitutiveEq run-time selection:
+ *   constitutiveEq::New(name, U, phi, dict)   newConstitutiveEq.C:32-61
+ * keyed by dict.lookup("type"): "Oldroyd-BLog" (Oldroyd_BLog.H:132),
+ * "GiesekusLog" (GiesekusLog.H:133), "FENE-PLog" (FENE_PLog.H:133),
+ * "PTTLog" (PTTLog.H:167, destructionFunctionType linear|exponential).
+ * `dict` is the text of the `parameters { ... }` sub-dictionary of
+ * constant/constitutiveProperties in OpenFOAM syntax, dimension sets
+ * allowed (e.g. "type Oldroyd-BLog; rho rho [1 -3 0 0 0 0 0] 1; etaS ...;
+ * etaP ...; lambda ...; stabilization coupling;").  Keys read as the
+ * reference constructors do (Oldroyd_BLog.C:114-121 etc.); a missing key is
+ * RT_E_BAD_ARG with the key named.  The model owns theta, tau, eigVals and
+ * eigVecs on the device (Oldroyd_BLog.C:52-113); U and phi are referenced. */
+int rt_model_create(rt_ctx* ctx, const char* name, const rt_mesh* mesh, const char* dict,
+                    rt_model** out);
+void rt_model_destroy(rt_model* model);
+
+/* Type name of the selected model (TypeName of the reference class). */
+const char* rt_model_type(const rt_model* model);
+
+/* State: theta [6N] (MUST_READ); eigVals/eigVecs [9N] each (READ_IF_PRESENT:
+ * NULL leaves them at the identity tensor, as the reference's default);
+ * tau [6N] may be NULL. */
+int rt_model_set_state(rt_model* model, const double* theta, const double* eigVals,
+                       const double* eigVecs, const double* tau);
+
+/* Boundary condition of theta on one patch: 0 zeroGradient, 1 fixedValue
+ * (values [6*patchSize]), 2 empty.  Default: zeroGradient (empty on empty
+ * patches). */
+enum rt_bc { RT_BC_ZEROGRADIENT = 0, RT_BC_FIXEDVALUE = 1, RT_BC_EMPTY = 2 };
+int rt_model_set_bc(rt_model* model, int patch, int bcType, const double* values);
+
+/* The referenced flow fields (constitutiveEq.H:72-75 hold U and phi by
+ * reference): U [3N], Ub [3*nBoundaryFaces] (boundary values of U, used by
+ * the Gauss gradient), phi [nInternalFaces + nBoundaryFaces].  Copied to the
+ * device; call again whenever the caller's U/phi change. */
+int rt_model_set_flow(rt_model* model, const double* U, const double* Ub, const double* phi);
+
+/* Solver controls for the theta equation (text as above). */
+int rt_model_set_solver(rt_model* model, const char* controls);
+
+/* Relaxation factor of the theta equation ([OF9] fvMatrix::relax(),
+ * Oldroyd_BLog.C:166): <= 0 (default) means no relaxationFactors entry. */
+int rt_model_set_relax(rt_model* model, double factor);
+
+/* constitutiveEq::correct(alpha, gradU)  (constitutiveEq.H:346-350,
+ * Oldroyd_BLog.C:127-179): decompose grad(U)^T, assemble theta's
+ * ddt + upwind div == source, solve the valid components, eigen-decompose
+ * theta, rebuild tau.  gradU [9N] may be NULL (Gauss linear from U); alpha
+ * is ignored, as by every single-phase *Log model.  deltaT is
+ * runTime.deltaT(). */
+int rt_model_correct(rt_model* model, double deltaT, const double* gradU);
+
+enum rt_field { RT_THETA = 0, RT_TAU = 1, RT_EIGVALS = 2, RT_EIGVECS = 3 };
+/* Copies a model field to the host: theta/tau [6N], eigVals/eigVecs [9N]. */
+int rt_model_get(const rt_model* model, int field, double* out);
+
+/* Solver performance of the last correct(), one record per symmTensor
+ * component (fvMatrix::solveSegregated's SolverPerformance vector). */
+int rt_model_solver_perf(const rt_model* model, rt_solver_perf perf[6]);
+
+/* ------------------------------------------------ segregated solve surface
+ * sparseSolver<symmTensor>::solve(fvMatrix<symmTensor>&)
+ * (sparseMatrixSolvers/segregated/sparseSolver.H:190-193) and
+ * fvMatrix<symmTensor>::solve() behind it: one call solves every valid
+ * component of A psi = source with A's LDU coefficients shared by the
+ * components.  Boundary coefficients as the fvMatrix holds them:
+ * internalCoeffs are added to the diagonal and boundaryCoeffs to the source
+ * (sparseSolver.C:72-127) on non-empty patches.  psi [6N] is updated in
+ * place. */
+typedef struct {
+    const double* diag;           /* [nCells] */
+    const double* lower;          /* [nInternalFaces] */
+    const double* upper;          /* [nInternalFaces] */
+    const double* source;         /* [6*nCells] */
+    const double* internalCoeffs; /* [6*nBoundaryFaces], may be NULL */
+    const double* boundaryCoeffs; /* [6*nBoundaryFaces], may be NULL */
+} rt_ldu_symm;
+
+int rt_symm_solve(rt_ctx* ctx, const rt_mesh* mesh, const rt_ldu_symm* sys, double* psi,
+                  const char* controls, rt_solver_perf perf[6]);
+
+/* ------------------------------------------------------------------ serving
+ * A device-resident request loop: independent stress steps (different
+ * meshes, models, Weissenberg numbers) are admitted against an HBM budget
+ * and advanced together.  No reference counterpart (the reference runs one
+ * case per process); it serves callers that batch many small cases. */
+typedef struct rt_server rt_server;
+
+typedef struct {
+    const rt_mesh_desc* mesh;
+    const char* dict;     /* constitutiveProperties parameters text */
+    const char* controls; /* fvSolution text, NULL for the server default */
+    const double* theta;  /* [6N] */
+    const double* eigVals;
+    const double* eigVecs;
+    const double* U;      /* [3N] */
+    const double* Ub;     /* [3*nBoundaryFaces] */
+    const double* phi;    /* [nFaces] */
+    double deltaT;
+} rt_request;
+
+int rt_server_create(rt_ctx* ctx, size_t hbm_budget, int max_inflight, rt_server** out);
+void rt_server_destroy(rt_server* server);
+/* Admits the request when the HBM budget allows, else queues it.  All host
+ * arrays are copied before return.  *ticket identifies the request. */
+int rt_server_submit(rt_server* server, const rt_request* req, int64_t* ticket);
+/* Advances every in-flight request by one batched round of device work and
+ * admits queued ones as memory frees up.  Returns the number of requests
+ * still in flight or queued. */
+int rt_server_progress(rt_server* server);
+/* 1 when the request is finished (then theta/tau [6N] are written, either
+ * may be NULL, and *latency_ms is submit-to-finish); 0 when still pending;
+ * -(status code) when the request failed (message in rt_last_error of the
+ * server's context). */
+int rt_server_poll(rt_server* server, int64_t ticket, double* theta, double* tau,
+                   rt_solver_perf perf[6], double* latency_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RHEO_GPU_H */

@@ -1,0 +1,417 @@
+"""ctypes binding of librheo_gpu.so (the C-ABI of include/rheo_gpu.h).
+
+Mirrors the reference's call surface for the stress step:
+
+* :class:`Context`                 -- device, stream, HBM budget, kernel timing
+* :class:`Mesh`                    -- lduAddressing + geometry (primitiveMesh)
+* :meth:`Context.constitutiveEq`   -- constitutiveEq::New(name, U, phi, dict)
+  returning a :class:`ConstitutiveEq` with ``correct()``, ``tau()``,
+  ``theta()``, ``eigVals()``, ``eigVecs()``
+* :meth:`Context.symm_solve`       -- sparseSolver<symmTensor>::solve(fvMatrix&)
+
+There is no fallback: if the HIP library is missing or no gfx950 device is
+visible, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_LIB = None
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "librheo_gpu.so")
+
+RT_OK, RT_E_UNKNOWN_TYPE, RT_E_BAD_ARG, RT_E_OOM, RT_E_HIP, RT_E_UNSUPPORTED, RT_E_NO_DEVICE = range(7)
+RT_W_NOT_CONVERGED = 0x100
+RT_THETA, RT_TAU, RT_EIGVALS, RT_EIGVECS = range(4)
+RT_RENUMBER_AUTO, RT_RENUMBER_NONE, RT_RENUMBER_RCM = range(3)
+RT_BC_ZEROGRADIENT, RT_BC_FIXEDVALUE, RT_BC_EMPTY = range(3)
+
+
+class RheoError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[rt status {code}] {msg}")
+        self.code = code
+        self.msg = msg
+
+
+class Patch(C.Structure):
+    _fields_ = [("start", C.c_int32), ("size", C.c_int32), ("kind", C.c_int32)]
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("nCells", C.c_int32), ("nInternalFaces", C.c_int32),
+                ("nBoundaryFaces", C.c_int32), ("nPatches", C.c_int32),
+                ("owner", C.c_void_p), ("neighbour", C.c_void_p), ("Sf", C.c_void_p),
+                ("weights", C.c_void_p), ("V", C.c_void_p), ("patches", C.c_void_p)]
+
+
+class SolverPerf(C.Structure):
+    _fields_ = [("initialResidual", C.c_double), ("finalResidual", C.c_double),
+                ("nIterations", C.c_int32), ("nRefinements", C.c_int32),
+                ("converged", C.c_int32), ("solved", C.c_int32)]
+
+    def __repr__(self):
+        return (f"SolverPerf(init={self.initialResidual:.3e}, final={self.finalResidual:.3e}, "
+                f"it={self.nIterations}, ref={self.nRefinements}, conv={self.converged}, "
+                f"solved={self.solved})")
+
+
+class Request(C.Structure):
+    _fields_ = [("mesh", C.c_void_p), ("dict", C.c_char_p), ("controls", C.c_char_p),
+                ("theta", C.c_void_p), ("eigVals", C.c_void_p), ("eigVecs", C.c_void_p),
+                ("U", C.c_void_p), ("Ub", C.c_void_p), ("phi", C.c_void_p),
+                ("deltaT", C.c_double)]
+
+
+class KernelStat(C.Structure):
+    _fields_ = [("name", C.c_char * 48), ("launches", C.c_int64), ("total_ms", C.c_double),
+                ("bytes", C.c_double)]
+
+
+class LduSymm(C.Structure):
+    _fields_ = [("diag", C.c_void_p), ("lower", C.c_void_p), ("upper", C.c_void_p),
+                ("source", C.c_void_p), ("internalCoeffs", C.c_void_p),
+                ("boundaryCoeffs", C.c_void_p)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with rheotool_amd.build.build_gpu()")
+        L = C.CDLL(LIB_PATH)
+        vp, i32 = C.c_void_p, C.c_int
+        sig = {
+            "rt_device_count": (i32, []),
+            "rt_ctx_create": (i32, [i32, C.c_size_t, vp]),
+            "rt_ctx_destroy": (None, [vp]),
+            "rt_last_error": (C.c_char_p, [vp]),
+            "rt_ctx_bytes_in_use": (C.c_size_t, [vp]),
+            "rt_ctx_set_profiling": (i32, [vp, i32]),
+            "rt_ctx_kernel_stats": (i32, [vp, vp, i32]),
+            "rt_ctx_reset_stats": (None, [vp]),
+            "rt_mesh_create": (i32, [vp, vp, i32, vp]),
+            "rt_mesh_destroy": (None, [vp]),
+            "rt_mesh_valid_components": (i32, [vp, vp]),
+            "rt_model_create": (i32, [vp, C.c_char_p, vp, C.c_char_p, vp]),
+            "rt_model_destroy": (None, [vp]),
+            "rt_model_type": (C.c_char_p, [vp]),
+            "rt_model_set_state": (i32, [vp, vp, vp, vp, vp]),
+            "rt_model_set_bc": (i32, [vp, i32, i32, vp]),
+            "rt_model_set_flow": (i32, [vp, vp, vp, vp]),
+            "rt_model_set_solver": (i32, [vp, C.c_char_p]),
+            "rt_model_set_relax": (i32, [vp, C.c_double]),
+            "rt_model_correct": (i32, [vp, C.c_double, vp]),
+            "rt_model_get": (i32, [vp, i32, vp]),
+            "rt_model_solver_perf": (i32, [vp, vp]),
+            "rt_symm_solve": (i32, [vp, vp, vp, vp, C.c_char_p, vp]),
+            "rt_server_create": (i32, [vp, C.c_size_t, i32, vp]),
+            "rt_server_destroy": (None, [vp]),
+            "rt_server_submit": (i32, [vp, vp, vp]),
+            "rt_server_progress": (i32, [vp]),
+            "rt_server_poll": (i32, [vp, C.c_int64, vp, vp, vp, vp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def device_count() -> int:
+    return lib().rt_device_count()
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+class Context:
+    def __init__(self, device=0, hbm_budget=0):
+        h = C.c_void_p()
+        rc = lib().rt_ctx_create(device, hbm_budget, C.byref(h))
+        if rc != RT_OK:
+            raise RheoError(rc, "rt_ctx_create failed (no gfx950 device?)")
+        self.h = h
+
+    def check(self, rc, what=""):
+        if rc != RT_OK and rc != RT_W_NOT_CONVERGED:
+            raise RheoError(rc, f"{what}: {lib().rt_last_error(self.h).decode()}")
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rt_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def bytes_in_use(self):
+        return lib().rt_ctx_bytes_in_use(self.h)
+
+    def set_profiling(self, on=True):
+        self.check(lib().rt_ctx_set_profiling(self.h, int(on)), "set_profiling")
+
+    def kernel_stats(self):
+        buf = (KernelStat * 64)()
+        n = lib().rt_ctx_kernel_stats(self.h, buf, 64)
+        if n < 0:
+            raise RheoError(-n, lib().rt_last_error(self.h).decode())
+        return {buf[i].name.decode(): dict(launches=buf[i].launches, ms=buf[i].total_ms,
+                                           bytes=buf[i].bytes) for i in range(n)}
+
+    def reset_stats(self):
+        lib().rt_ctx_reset_stats(self.h)
+
+    def mesh(self, m, renumber=RT_RENUMBER_AUTO) -> "Mesh":
+        return Mesh(self, m, renumber)
+
+    def constitutiveEq(self, name, mesh, dict_text) -> "ConstitutiveEq":
+        return ConstitutiveEq(self, name, mesh, dict_text)
+
+    def symm_solve(self, mesh, diag, lower, upper, source, psi, controls, internalCoeffs=None,
+                   boundaryCoeffs=None):
+        """sparseSolver<symmTensor>::solve: returns (psi, perf[6])."""
+        keep = [_f64(diag), _f64(lower), _f64(upper), _f64(source),
+                None if internalCoeffs is None else _f64(internalCoeffs),
+                None if boundaryCoeffs is None else _f64(boundaryCoeffs)]
+        s = LduSymm(*[_p(k) for k in keep])
+        x = _f64(psi).copy()
+        perf = (SolverPerf * 6)()
+        rc = lib().rt_symm_solve(self.h, mesh.h, C.byref(s), _p(x), controls.encode(), perf)
+        self.check(rc, "rt_symm_solve")
+        return x, [perf[i] for i in range(6)]
+
+
+class Mesh:
+    """Device mesh built from a rheotool_amd.cases.Mesh (or any object with the
+    same arrays)."""
+
+    def __init__(self, ctx: Context, m, renumber=RT_RENUMBER_AUTO):
+        self.ctx = ctx
+        self.src = m
+        self._keep = dict(owner=np.ascontiguousarray(m.owner, np.int32),
+                          nbr=np.ascontiguousarray(m.neighbour, np.int32),
+                          Sf=_f64(m.Sf), w=_f64(m.w), V=_f64(m.V))
+        pt = (Patch * max(1, len(m.patchStart)))()
+        for i in range(len(m.patchStart)):
+            pt[i] = Patch(int(m.patchStart[i]), int(m.patchSize[i]), int(m.patchKind[i]))
+        self._pt = pt
+        k = self._keep
+        d = MeshDesc(m.nCells, m.nInternal, m.nBoundary, len(m.patchStart), _p(k["owner"]),
+                     _p(k["nbr"]), _p(k["Sf"]), _p(k["w"]), _p(k["V"]),
+                     C.cast(pt, C.c_void_p))
+        h = C.c_void_p()
+        ctx.check(lib().rt_mesh_create(ctx.h, C.byref(d), renumber, C.byref(h)), "rt_mesh_create")
+        self.h = h
+        self.nCells = m.nCells
+
+    @property
+    def validComponents(self):
+        v = np.zeros(6, np.int32)
+        lib().rt_mesh_valid_components(self.h, _p(v))
+        return v
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rt_mesh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ConstitutiveEq:
+    """constitutiveEq selected at run time from a constitutiveProperties dictionary."""
+
+    def __init__(self, ctx: Context, name, mesh: Mesh, dict_text):
+        self.ctx, self.mesh = ctx, mesh
+        h = C.c_void_p()
+        ctx.check(lib().rt_model_create(ctx.h, name.encode(), mesh.h, dict_text.encode(),
+                                        C.byref(h)), "constitutiveEq::New")
+        self.h = h
+        self.N = mesh.nCells
+
+    @property
+    def type(self):
+        return lib().rt_model_type(self.h).decode()
+
+    def set_state(self, theta, eigVals=None, eigVecs=None, tau=None):
+        N = self.N
+        keep = [_f64(theta, (N, 6)), None if eigVals is None else _f64(eigVals, (N, 3, 3)),
+                None if eigVecs is None else _f64(eigVecs, (N, 3, 3)),
+                None if tau is None else _f64(tau, (N, 6))]
+        self.ctx.check(lib().rt_model_set_state(self.h, *[_p(k) for k in keep]), "set_state")
+
+    def set_flow(self, U, Ub, phi):
+        keep = [_f64(U), _f64(Ub) if Ub is not None and np.size(Ub) else None, _f64(phi)]
+        self.ctx.check(lib().rt_model_set_flow(self.h, *[_p(k) for k in keep]), "set_flow")
+
+    def set_bc(self, patch, bc_type, values=None):
+        v = None if values is None else _f64(values)
+        self.ctx.check(lib().rt_model_set_bc(self.h, patch, bc_type, _p(v)), "set_bc")
+
+    def set_solver(self, controls: str):
+        self.ctx.check(lib().rt_model_set_solver(self.h, controls.encode()), "set_solver")
+
+    def set_relax(self, factor: float):
+        self.ctx.check(lib().rt_model_set_relax(self.h, factor), "set_relax")
+
+    def correct(self, deltaT, gradU=None):
+        g = None if gradU is None else _f64(gradU, (self.N, 3, 3))
+        return self.ctx.check(lib().rt_model_correct(self.h, deltaT, _p(g)), "correct")
+
+    def _get(self, which, shape):
+        out = np.empty(shape)
+        self.ctx.check(lib().rt_model_get(self.h, which, _p(out)), "get")
+        return out
+
+    def theta(self):
+        return self._get(RT_THETA, (self.N, 6))
+
+    def tau(self):
+        return self._get(RT_TAU, (self.N, 6))
+
+    def eigVals(self):
+        return self._get(RT_EIGVALS, (self.N, 3, 3))
+
+    def eigVecs(self):
+        return self._get(RT_EIGVECS, (self.N, 3, 3))
+
+    def solver_perf(self):
+        perf = (SolverPerf * 6)()
+        lib().rt_model_solver_perf(self.h, perf)
+        return [perf[i] for i in range(6)]
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rt_model_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def model_dict(model="Oldroyd-BLog", etaS=0.59, etaP=0.41, lam=1.0, alpha=None, L2=None,
+               epsilon=None, zeta=None, destruction="linear", rho=1.0):
+    """constitutiveProperties `parameters` text in OpenFOAM syntax."""
+    lines = [f"type {model};", f"rho rho [1 -3 0 0 0 0 0] {rho!r};",
+             f"etaS etaS [1 -1 -1 0 0 0 0] {etaS!r};", f"etaP etaP [1 -1 -1 0 0 0 0] {etaP!r};",
+             f"lambda lambda [0 0 1 0 0 0 0] {lam!r};"]
+    if alpha is not None:
+        lines.append(f"alpha alpha [0 0 0 0 0 0 0] {alpha!r};")
+    if L2 is not None:
+        lines.append(f"L2 L2 [0 0 0 0 0 0 0] {L2!r};")
+    if epsilon is not None:
+        lines.append(f"epsilon epsilon [0 0 0 0 0 0 0] {epsilon!r};")
+    if zeta is not None:
+        lines.append(f"zeta zeta [0 0 0 0 0 0 0] {zeta!r};")
+    if model == "PTTLog":
+        lines.append(f"destructionFunctionType {destruction};")
+    lines.append("stabilization coupling;")
+    return "parameters\n{\n    " + "\n    ".join(lines) + "\n}\n"
+
+
+def solver_dict(tol=1e-10, precond="DILU", precision="double", maxIter=1000, relTol=0.0,
+                innerRelTol=None):
+    s = (f"solver PBiCGStab; preconditioner {precond}; tolerance {tol!r}; relTol {relTol!r}; "
+         f"maxIter {maxIter}; minIter 0; precision {precision};")
+    if innerRelTol is not None:
+        s += f" innerRelTol {innerRelTol!r};"
+    return s
+
+
+def mesh_desc(m):
+    """rt_mesh_desc for a cases.Mesh; returns (desc, keepalive)."""
+    keep = dict(owner=np.ascontiguousarray(m.owner, np.int32),
+                nbr=np.ascontiguousarray(m.neighbour, np.int32), Sf=_f64(m.Sf), w=_f64(m.w),
+                V=_f64(m.V))
+    pt = (Patch * max(1, len(m.patchStart)))()
+    for i in range(len(m.patchStart)):
+        pt[i] = Patch(int(m.patchStart[i]), int(m.patchSize[i]), int(m.patchKind[i]))
+    keep["pt"] = pt
+    d = MeshDesc(m.nCells, m.nInternal, m.nBoundary, len(m.patchStart), _p(keep["owner"]),
+                 _p(keep["nbr"]), _p(keep["Sf"]), _p(keep["w"]), _p(keep["V"]),
+                 C.cast(pt, C.c_void_p))
+    return d, keep
+
+
+class Server:
+    """Continuous-batching request loop (rt_server_*)."""
+
+    def __init__(self, ctx: Context, hbm_budget=0, max_inflight=8):
+        h = C.c_void_p()
+        ctx.check(lib().rt_server_create(ctx.h, hbm_budget, max_inflight, C.byref(h)),
+                  "rt_server_create")
+        self.ctx, self.h = ctx, h
+        self.sizes = {}
+
+    def submit(self, m, fields, dict_text, deltaT, controls=None) -> int:
+        d, keep = mesh_desc(m)
+        arrs = [_f64(fields.theta), _f64(fields.eigVals), _f64(fields.eigVecs), _f64(fields.U),
+                _f64(fields.Ub) if m.nBoundary else None, _f64(fields.phi)]
+        q = Request(C.cast(C.pointer(d), C.c_void_p), dict_text.encode(),
+                    None if controls is None else controls.encode(), *[_p(a) for a in arrs],
+                    deltaT)
+        t = C.c_int64()
+        self.ctx.check(lib().rt_server_submit(self.h, C.byref(q), C.byref(t)), "rt_server_submit")
+        self.sizes[t.value] = m.nCells
+        return t.value
+
+    def progress(self) -> int:
+        n = lib().rt_server_progress(self.h)
+        if n < 0:
+            raise RheoError(-n, lib().rt_last_error(self.ctx.h).decode())
+        return n
+
+    def poll(self, ticket):
+        """None while pending, else dict(theta, tau, perf, latency_ms)."""
+        N = self.sizes[ticket]
+        th, tau = np.empty((N, 6)), np.empty((N, 6))
+        perf = (SolverPerf * 6)()
+        lat = C.c_double()
+        rc = lib().rt_server_poll(self.h, ticket, _p(th), _p(tau), perf, C.byref(lat))
+        if rc == 0:
+            return None
+        if rc < 0:
+            raise RheoError(-rc, lib().rt_last_error(self.ctx.h).decode())
+        return dict(theta=th, tau=tau, perf=[perf[i] for i in range(6)], latency_ms=lat.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rt_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,726 @@
+// kernels_krylov.hip -- lockstep multi-component PBiCGStab for the theta solve (gfx950).
+//
+// The reference solves each valid symmTensor component separately with
+// OpenFOAM's PBiCGStab (the segregated loop restated in
+// sparseMatrixSolvers/segregated/eigenSolver/eigenSolver.C:422-562, residual
+// normalisation sparseSolver.C:129-180).  The components share the matrix, so
+// here one pass over the matrix serves every component still iterating:
+// each kernel walks the sliced-ELL rows once and updates all active
+// components, with per-component convergence masks.  Every scalar of the
+// recurrence (rho, alpha, omega, beta, residuals, flags) stays on the device:
+// the last workgroup of each reduction finalises it, so no host round trip is
+// needed inside an iteration.
+//
+// Preconditioning is right-Jacobi with the column-scaled matrix
+// Ahat = A D^-1 stored explicitly: A M^-1 p = p + offdiag(Ahat) p.
+//
+// Precision: the vector type T is double (FP64 drop-in) or float (inner
+// solve of the FP64-refined "mixed" solve).  x (theta), b, the matrix used
+// for the true residual and all dot products are FP64.
+#include "rt_internal.h"
+
+namespace rt {
+
+constexpr int KBLK = 256;
+
+__device__ __forceinline__ int kxcd_block(int G) {
+    const int b = blockIdx.x;
+    if (G % 8 != 0) return b;
+    return (b % 8) * (G / 8) + b / 8;
+}
+__device__ __forceinline__ void krows(int32_t N, int32_t& r0, int32_t& r1) {
+    const int G = gridDim.x;
+    const int b = kxcd_block(G);
+    int64_t per = ((int64_t)N + G - 1) / G;
+    per = (per + KBLK - 1) / KBLK * KBLK;
+    int64_t a = (int64_t)b * per;
+    r0 = (int32_t)(a < N ? a : N);
+    r1 = (int32_t)(a + per < N ? a + per : N);
+}
+__device__ __forceinline__ double kwave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <int NQ>
+__device__ bool kreduce_last(double (&v)[NQ], double* partials, int32_t* counter,
+                             double (&sums)[NQ]) {
+    __shared__ double red[NQ][KBLK / 64];
+    __shared__ int amLast;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        double s = kwave_sum(v[q]);
+        if (lane == 0) red[q][wid] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < NQ) {
+        double s = 0;
+        for (int w = 0; w < KBLK / 64; ++w) s += red[threadIdx.x][w];
+        partials[(int64_t)blockIdx.x * NQ + threadIdx.x] = s;
+    }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) amLast = atomicAdd(counter, 1) == (int)gridDim.x - 1;
+    __syncthreads();
+    if (!amLast) return false;
+    __threadfence();
+    const volatile double* vp = partials;
+    double acc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) acc[q] = 0;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += KBLK)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) acc[q] += vp[(int64_t)i * NQ + q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        double s = kwave_sum(acc[q]);
+        if (lane == 0) red[q][wid] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            double s = 0;
+            for (int w = 0; w < KBLK / 64; ++w) s += red[q][w];
+            sums[q] = s;
+        }
+        *counter = 0;
+    }
+    return true;
+}
+
+struct KryCtl {
+    double tol, relTol, innerRelTol;
+    int32_t maxIter, minIter, maxInner, maxRefine;
+    int32_t mixed;
+};
+
+template <class T>
+struct KryArgs {
+    int32_t N;
+    const int64_t* ellOff;
+    const int32_t* ellWidth;
+    const int32_t* ellCol;
+    const double* aval;  // FP64 matrix (true residual)
+    const double* diag;
+    const T* ahat;       // A D^-1 off-diagonal values
+    const double* rD;    // 1 / diag
+    const double* b;     // [6][N]
+    double* x;           // [6][N]
+    T *r, *r0, *p, *v, *t;  // [6][N]
+    double* partials;
+    int32_t* counters;   // one per kernel kind
+    CompState* cs;
+    int32_t valid[6];
+    KryCtl ctl;
+};
+
+__device__ __forceinline__ bool conv_outer(const CompState& c, const KryCtl& k, double res) {
+    return res < k.tol || (k.relTol > 0 && res < k.relTol * c.initRes);
+}
+__device__ __forceinline__ bool conv_inner(const CompState& c, const KryCtl& k, double res) {
+    return k.mixed ? res < c.innerTarget : conv_outer(c, k, res);
+}
+
+// begin an inner Krylov solve on a component whose r, r0 were just set
+__device__ void start_inner(CompState& c, const KryCtl& k, double rr) {
+    c.innerIter = 0;
+    c.rho = rr;
+    c.exitAtS = 0;
+    if (k.mixed) c.innerTarget = fmax(k.innerRelTol * c.finalRes, 0.3 * k.tol);
+    if (fabs(rr) < 1e-300) {  // singular: nothing to iterate on
+        c.active = 0;
+        c.converged = conv_outer(c, k, c.finalRes);
+        return;
+    }
+    c.active = 1;
+}
+
+// ---- true residual r = b - A x (FP64), init or refinement -----------------
+// mode 0: initial (sets normFactor, initRes); mode 1: refinement sweep.
+template <class T>
+__global__ void __launch_bounds__(KBLK) k_residual(KryArgs<T> a, int mode) {
+    bool on[6];
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        on[c] = a.valid[c] > 0 && (mode == 0 || a.cs[c].needRefine);
+        any |= on[c];
+    }
+    if (!any) return;
+    int32_t r0, r1;
+    krows(a.N, r0, r1);
+    const int32_t N = a.N;
+    double q[18];
+#pragma unroll
+    for (int i = 0; i < 18; ++i) q[i] = 0;
+    double xbar[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) xbar[c] = a.cs[c].xbar;
+    for (int32_t r = r0 + threadIdx.x; r < r1; r += KBLK) {
+        const double d = a.diag[r];
+        double y[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) y[c] = on[c] ? d * a.x[(int64_t)c * N + r] : 0.0;
+        double sumA = d;
+        const int s = r >> 6, lane = r & 63;
+        const int64_t base = a.ellOff[s] + lane;
+        const int width = a.ellWidth[s];
+        for (int k = 0; k < width; ++k) {
+            const int64_t e = base + (int64_t)SLICE * k;
+            const int32_t col = a.ellCol[e];
+            const double av = a.aval[e];
+            sumA += av;
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+                if (on[c]) y[c] += av * a.x[(int64_t)c * N + col];
+        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            if (!on[c]) continue;
+            const double bb = a.b[(int64_t)c * N + r];
+            const double res = bb - y[c];
+            const T rt = (T)res;
+            a.r[(int64_t)c * N + r] = rt;
+            a.r0[(int64_t)c * N + r] = rt;
+            q[c] += fabs(res);
+            q[6 + c] += (double)rt * (double)rt;
+            if (mode == 0) {
+                const double pA = sumA * xbar[c];
+                q[12 + c] += fabs(y[c] - pA) + fabs(bb - pA);
+            }
+        }
+    }
+    double sums[18];
+    if (kreduce_last<18>(q, a.partials, a.counters + 0, sums) && threadIdx.x == 0) {
+        for (int c = 0; c < 6; ++c) {
+            if (!on[c]) continue;
+            CompState& cs = a.cs[c];
+            if (mode == 0) {
+                cs.normFactor = sums[12 + c] + 1e-20;
+                cs.initRes = sums[c] / cs.normFactor;
+                cs.finalRes = cs.initRes;
+                cs.iter = 0;
+                cs.nRefine = 0;
+                cs.solved = 1;
+                cs.converged = 0;
+                cs.needRefine = 0;
+                if (a.ctl.minIter > 0 || !conv_outer(cs, a.ctl, cs.initRes)) {
+                    start_inner(cs, a.ctl, sums[6 + c]);
+                } else {
+                    cs.active = 0;
+                    cs.converged = 1;
+                }
+            } else {
+                cs.needRefine = 0;
+                cs.nRefine += 1;
+                cs.finalRes = sums[c] / cs.normFactor;
+                if (conv_outer(cs, a.ctl, cs.finalRes) && cs.iter >= a.ctl.minIter) {
+                    cs.converged = 1;
+                    cs.active = 0;
+                } else if (cs.nRefine >= a.ctl.maxRefine || cs.iter >= a.ctl.maxIter) {
+                    cs.active = 0;
+                } else {
+                    start_inner(cs, a.ctl, sums[6 + c]);
+                }
+            }
+        }
+    }
+}
+
+// ---- p = r  |  p = r + beta (p - omega v) ---------------------------------
+template <class T>
+__global__ void __launch_bounds__(KBLK) k_kry_p(KryArgs<T> a) {
+    bool on[6];
+    bool first[6];
+    T beta[6], omega[6];
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        on[c] = a.cs[c].active;
+        first[c] = a.cs[c].innerIter == 0;
+        beta[c] = (T)a.cs[c].beta;
+        omega[c] = (T)a.cs[c].omega;
+        any |= on[c];
+    }
+    if (!any) return;
+    const int64_t N = a.N;
+    for (int64_t i = (int64_t)blockIdx.x * KBLK + threadIdx.x; i < N; i += (int64_t)gridDim.x * KBLK) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            if (!on[c]) continue;
+            const int64_t j = c * N + i;
+            const T rr = a.r[j];
+            a.p[j] = first[c] ? rr : rr + beta[c] * (a.p[j] - omega[c] * a.v[j]);
+        }
+    }
+}
+
+// ---- v = A M^-1 p = p + Ahat p ;  sigma = (r0, v) -------------------------
+template <class T>
+__global__ void __launch_bounds__(KBLK) k_kry_v(KryArgs<T> a) {
+    bool on[6];
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        on[c] = a.cs[c].active;
+        any |= on[c];
+    }
+    if (!any) return;
+    int32_t r0, r1;
+    krows(a.N, r0, r1);
+    const int64_t N = a.N;
+    double q[6] = {0, 0, 0, 0, 0, 0};
+    for (int32_t r = r0 + threadIdx.x; r < r1; r += KBLK) {
+        T acc[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) acc[c] = on[c] ? a.p[c * N + r] : (T)0;
+        const int s = r >> 6, lane = r & 63;
+        const int64_t base = a.ellOff[s] + lane;
+        const int width = a.ellWidth[s];
+        for (int k = 0; k < width; ++k) {
+            const int64_t e = base + (int64_t)SLICE * k;
+            const int32_t col = a.ellCol[e];
+            const T av = a.ahat[e];
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+                if (on[c]) acc[c] += av * a.p[c * N + col];
+        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            if (!on[c]) continue;
+            a.v[c * N + r] = acc[c];
+            q[c] += (double)a.r0[c * N + r] * (double)acc[c];
+        }
+    }
+    double sums[6];
+    if (kreduce_last<6>(q, a.partials, a.counters + 1, sums) && threadIdx.x == 0) {
+        for (int c = 0; c < 6; ++c)
+            if (on[c]) a.cs[c].alpha = a.cs[c].rho / sums[c];
+    }
+}
+
+// ---- s = r - alpha v ; t = A M^-1 s ; |s|, (t,s), (t,t) --------------------
+template <class T>
+__global__ void __launch_bounds__(KBLK) k_kry_t(KryArgs<T> a) {
+    bool on[6];
+    T alpha[6];
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        on[c] = a.cs[c].active;
+        alpha[c] = (T)a.cs[c].alpha;
+        any |= on[c];
+    }
+    if (!any) return;
+    int32_t r0, r1;
+    krows(a.N, r0, r1);
+    const int64_t N = a.N;
+    double q[18];
+#pragma unroll
+    for (int i = 0; i < 18; ++i) q[i] = 0;
+    for (int32_t r = r0 + threadIdx.x; r < r1; r += KBLK) {
+        T sv[6], acc[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            sv[c] = on[c] ? a.r[c * N + r] - alpha[c] * a.v[c * N + r] : (T)0;
+            acc[c] = sv[c];
+        }
+        const int s = r >> 6, lane = r & 63;
+        const int64_t base = a.ellOff[s] + lane;
+        const int width = a.ellWidth[s];
+        for (int k = 0; k < width; ++k) {
+            const int64_t e = base + (int64_t)SLICE * k;
+            const int32_t col = a.ellCol[e];
+            const T av = a.ahat[e];
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+                if (on[c]) acc[c] += av * (a.r[c * N + col] - alpha[c] * a.v[c * N + col]);
+        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            if (!on[c]) continue;
+            a.t[c * N + r] = acc[c];
+            q[c] += fabs((double)sv[c]);
+            q[6 + c] += (double)acc[c] * (double)sv[c];
+            q[12 + c] += (double)acc[c] * (double)acc[c];
+        }
+    }
+    double sums[18];
+    if (kreduce_last<18>(q, a.partials, a.counters + 2, sums) && threadIdx.x == 0) {
+        for (int c = 0; c < 6; ++c) {
+            if (!on[c]) continue;
+            CompState& cs = a.cs[c];
+            const double res = sums[c] / cs.normFactor;
+            if (!a.ctl.mixed) cs.finalRes = res;
+            if (conv_inner(cs, a.ctl, res)) {
+                cs.exitAtS = 1;
+            } else {
+                cs.omega = sums[6 + c] / sums[12 + c];
+            }
+        }
+    }
+}
+
+// ---- x += alpha M^-1 p + omega M^-1 s ; r = s - omega t ; |r|, (r0, r) ------
+template <class T>
+__global__ void __launch_bounds__(KBLK) k_kry_xr(KryArgs<T> a) {
+    bool on[6], ex[6];
+    double alpha[6], omega[6];
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        on[c] = a.cs[c].active;
+        ex[c] = a.cs[c].exitAtS;
+        alpha[c] = a.cs[c].alpha;
+        omega[c] = a.cs[c].omega;
+        any |= on[c];
+    }
+    if (!any) return;
+    int32_t r0, r1;
+    krows(a.N, r0, r1);
+    const int64_t N = a.N;
+    double q[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) q[i] = 0;
+    for (int32_t r = r0 + threadIdx.x; r < r1; r += KBLK) {
+        const double rd = a.rD[r];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            if (!on[c]) continue;
+            const int64_t j = c * N + r;
+            const double y = rd * (double)a.p[j];
+            if (ex[c]) {
+                a.x[j] += alpha[c] * y;
+                continue;
+            }
+            const T sv = a.r[j] - (T)alpha[c] * a.v[j];
+            const double z = rd * (double)sv;
+            a.x[j] += alpha[c] * y + omega[c] * z;
+            const T rn = sv - (T)omega[c] * a.t[j];
+            a.r[j] = rn;
+            q[c] += fabs((double)rn);
+            q[6 + c] += (double)a.r0[j] * (double)rn;
+        }
+    }
+    double sums[12];
+    if (kreduce_last<12>(q, a.partials, a.counters + 3, sums) && threadIdx.x == 0) {
+        for (int c = 0; c < 6; ++c) {
+            if (!on[c]) continue;
+            CompState& cs = a.cs[c];
+            cs.iter += 1;
+            cs.innerIter += 1;
+            bool stop;
+            if (cs.exitAtS) {
+                cs.exitAtS = 0;
+                stop = true;
+            } else {
+                const double res = sums[c] / cs.normFactor;
+                if (!a.ctl.mixed) cs.finalRes = res;
+                const bool conv = conv_inner(cs, a.ctl, res);
+                bool cont = (!conv && cs.iter < a.ctl.maxIter) ||
+                            (!a.ctl.mixed && cs.iter < a.ctl.minIter);
+                if (a.ctl.mixed && cs.innerIter >= a.ctl.maxInner) cont = false;
+                stop = !cont;
+                if (cont) {
+                    cs.rhoOld = cs.rho;
+                    cs.rho = sums[6 + c];
+                    if (fabs(cs.rho) < 1e-300 || fabs(cs.omega) < 1e-300) {
+                        stop = true;
+                    } else {
+                        cs.beta = (cs.rho / cs.rhoOld) * (cs.alpha / cs.omega);
+                    }
+                }
+            }
+            if (stop) {
+                cs.active = 0;
+                if (a.ctl.mixed) {
+                    cs.needRefine = 1;
+                } else {
+                    cs.converged = conv_outer(cs, a.ctl, cs.finalRes);
+                }
+            }
+        }
+    }
+}
+
+// ---- Ahat = offdiag(A) D^-1 (column scaled), rD = 1/diag ---------------------
+__global__ void k_rdiag(int32_t N, const double* __restrict__ diag, double* __restrict__ rD) {
+    const int64_t r = (int64_t)blockIdx.x * KBLK + threadIdx.x;
+    if (r < N) rD[r] = 1.0 / diag[r];
+}
+template <class T>
+__global__ void k_scale(int64_t nEll, const int32_t* __restrict__ col, const double* __restrict__ aval,
+                        const double* __restrict__ rD, T* __restrict__ ahat) {
+    const int64_t e = (int64_t)blockIdx.x * KBLK + threadIdx.x;
+    if (e < nEll) ahat[e] = (T)(aval[e] * rD[col[e]]);
+}
+
+// ----------------------------------------------------------------- driver
+static inline unsigned kblk(int64_t n) { return (unsigned)((n + KBLK - 1) / KBLK); }
+
+void Krylov::ensure(Mesh& m, bool mixed) {
+    if (!ctx) ctx = m.ctx;
+    const size_t vec = (size_t)6 * m.N;
+    if (N != m.N) {
+        N = m.N;
+        grid = grid_for(ctx, N);
+        r = DBuf(); r0 = DBuf(); p = DBuf(); v = DBuf(); t = DBuf();
+        rs = DBuf(); r0s = DBuf(); ps = DBuf(); vs = DBuf(); ts = DBuf();
+        ahat32 = DBuf(); ahat64 = DBuf();
+        partials = ctx->alloc((size_t)grid * 24 * sizeof(double));
+        state = ctx->alloc(sizeof(SolveState));
+        RT_HIP(hipMemsetAsync(state.p, 0, sizeof(SolveState), ctx->stream));
+        rD = ctx->alloc(std::max<size_t>(1, (size_t)m.N) * sizeof(double));
+    }
+    if (!mixed && !r.p) {
+        r = ctx->alloc(vec * 8); r0 = ctx->alloc(vec * 8); p = ctx->alloc(vec * 8);
+        v = ctx->alloc(vec * 8); t = ctx->alloc(vec * 8);
+        ahat64 = ctx->alloc(std::max<int64_t>(1, m.nEll) * 8);
+    }
+    if (mixed && !rs.p) {
+        rs = ctx->alloc(vec * 4); r0s = ctx->alloc(vec * 4); ps = ctx->alloc(vec * 4);
+        vs = ctx->alloc(vec * 4); ts = ctx->alloc(vec * 4);
+        ahat32 = ctx->alloc(std::max<int64_t>(1, m.nEll) * 4);
+    }
+}
+
+// Resumable solve: begin() launches the initial residual, every advance()
+// reads the device state of the previous round (non-blocking when asked) and
+// launches the next round, so several solves can progress on their own
+// streams at once (the serving loop) or one can run to completion
+// (rt_model_correct / rt_symm_solve).
+struct SolveRun {
+    bool mixed = false;
+    KryArgs<float> af{};
+    KryArgs<double> ad{};
+    int G = 1, pgrid = 1;
+    double N = 0, ne = 0, nValid = 0;
+    int32_t valid[6];
+    CompState* host = nullptr;  // pinned
+    hipEvent_t ev = nullptr;
+    bool pending = false, done = false;
+    int rounds = 0;
+    ~SolveRun() {
+        if (host) (void)hipHostFree(host);
+        if (ev) (void)hipEventDestroy(ev);
+    }
+};
+
+void SolveRunDeleter::operator()(SolveRun* r) const { delete r; }
+
+namespace {
+
+template <class T>
+KryArgs<T>& argsOf(SolveRun& R);
+template <>
+KryArgs<float>& argsOf<float>(SolveRun& R) { return R.af; }
+template <>
+KryArgs<double>& argsOf<double>(SolveRun& R) { return R.ad; }
+
+template <class T>
+void fill_args(Ctx* ctx, const SolveArgs& sa, SolveRun& R) {
+    Mesh& m = *sa.mesh;
+    Krylov& K = *sa.kry;
+    const bool mixed = std::is_same<T, float>::value;
+    const int32_t N = m.N;
+    SolveState* st = K.state.as<SolveState>();
+    double* rD = K.rD.as<double>();
+    {
+        ProfScope ps(ctx, "k_rdiag", (double)N * 16);
+        hipLaunchKernelGGL(k_rdiag, dim3(kblk(N)), dim3(KBLK), 0, ctx->stream, N, sa.diag, rD);
+    }
+    T* ahat = mixed ? (T*)K.ahat32.p : (T*)K.ahat64.p;
+    if (m.nEll > 0) {
+        ProfScope ps(ctx, "k_scale", (double)m.nEll * (4 + 8 + sizeof(T)) + (double)N * 8);
+        hipLaunchKernelGGL(k_scale<T>, dim3(kblk(m.nEll)), dim3(KBLK), 0, ctx->stream, m.nEll,
+                           m.ellCol.as<int32_t>(), sa.aval, rD, ahat);
+    }
+    RT_HIP(hipGetLastError());
+    KryArgs<T>& a = argsOf<T>(R);
+    a = KryArgs<T>{};
+    a.N = N;
+    a.ellOff = m.ellOff.as<int64_t>();
+    a.ellWidth = m.ellWidth.as<int32_t>();
+    a.ellCol = m.ellCol.as<int32_t>();
+    a.aval = sa.aval;
+    a.diag = sa.diag;
+    a.ahat = ahat;
+    a.rD = rD;
+    a.b = sa.b;
+    a.x = sa.x;
+    if (mixed) {
+        a.r = (T*)K.rs.p; a.r0 = (T*)K.r0s.p; a.p = (T*)K.ps.p; a.v = (T*)K.vs.p; a.t = (T*)K.ts.p;
+    } else {
+        a.r = (T*)K.r.p; a.r0 = (T*)K.r0.p; a.p = (T*)K.p.p; a.v = (T*)K.v.p; a.t = (T*)K.t.p;
+    }
+    a.partials = K.partials.as<double>();
+    a.counters = st->counter;
+    a.cs = st->c;
+    for (int c = 0; c < 6; ++c) a.valid[c] = sa.valid[c];
+    const Controls& C = *sa.ctl;
+    a.ctl.tol = C.tolerance;
+    a.ctl.relTol = C.relTol;
+    a.ctl.innerRelTol = C.innerRelTol;
+    a.ctl.maxIter = C.maxIter;
+    a.ctl.minIter = C.minIter;
+    a.ctl.maxInner = 200;
+    a.ctl.maxRefine = C.maxRefine;
+    a.ctl.mixed = mixed ? 1 : 0;
+}
+
+// algorithmic bytes per launch for nc participating components: ELL entries
+// (col + value) once per SpMV, each vector once per pass
+struct Bytes {
+    double N, ne, w;
+    double res(double nc) const { return N * 8.0 + ne * 12 + nc * N * (8 + 8 + 2 * w); }
+    double p(double nc) const { return nc * N * 4 * w; }
+    double v(double nc) const { return ne * (4 + w) + nc * N * 3 * w; }
+    double t(double nc) const { return ne * (4 + w) + nc * N * 3 * w; }
+    double xr(double nc) const { return N * 8.0 + nc * N * (16 + 6 * w); }
+};
+
+template <class T>
+void launch_init(Ctx* ctx, SolveRun& R) {
+    const bool mixed = std::is_same<T, float>::value;
+    Bytes B{R.N, R.ne, (double)sizeof(T)};
+    ProfScope ps(ctx, mixed ? "k_residual<f32>" : "k_residual<f64>", B.res(R.nValid));
+    hipLaunchKernelGGL(k_residual<T>, dim3(R.G), dim3(KBLK), 0, ctx->stream, argsOf<T>(R), 0);
+}
+
+template <class T>
+void launch_round(Ctx* ctx, SolveRun& R, double nAct, double nRef) {
+    const bool mixed = std::is_same<T, float>::value;
+    Bytes B{R.N, R.ne, (double)sizeof(T)};
+    KryArgs<T>& a = argsOf<T>(R);
+    if (nRef > 0) {
+        ProfScope ps(ctx, mixed ? "k_residual<f32>" : "k_residual<f64>", B.res(nRef));
+        hipLaunchKernelGGL(k_residual<T>, dim3(R.G), dim3(KBLK), 0, ctx->stream, a, 1);
+        nAct += nRef;  // components restarted by the refinement sweep join this round
+    }
+    {
+        ProfScope ps(ctx, mixed ? "k_kry_p<f32>" : "k_kry_p<f64>", B.p(nAct));
+        hipLaunchKernelGGL(k_kry_p<T>, dim3(R.pgrid), dim3(KBLK), 0, ctx->stream, a);
+    }
+    {
+        ProfScope ps(ctx, mixed ? "k_kry_v<f32>" : "k_kry_v<f64>", B.v(nAct));
+        hipLaunchKernelGGL(k_kry_v<T>, dim3(R.G), dim3(KBLK), 0, ctx->stream, a);
+    }
+    {
+        ProfScope ps(ctx, mixed ? "k_kry_t<f32>" : "k_kry_t<f64>", B.t(nAct));
+        hipLaunchKernelGGL(k_kry_t<T>, dim3(R.G), dim3(KBLK), 0, ctx->stream, a);
+    }
+    {
+        ProfScope ps(ctx, mixed ? "k_kry_xr<f32>" : "k_kry_xr<f64>", B.xr(nAct));
+        hipLaunchKernelGGL(k_kry_xr<T>, dim3(R.G), dim3(KBLK), 0, ctx->stream, a);
+    }
+}
+
+void queue_pull(Ctx* ctx, Krylov& K, SolveRun& R) {
+    SolveState* st = K.state.as<SolveState>();
+    RT_HIP(hipMemcpyAsync(R.host, st->c, sizeof(CompState) * 6, hipMemcpyDeviceToHost, ctx->stream));
+    RT_HIP(hipEventRecord(R.ev, ctx->stream));
+    R.pending = true;
+}
+
+}  // namespace
+
+void solve_begin(Ctx* ctx, const SolveArgs& sa) {
+    Mesh& m = *sa.mesh;
+    Krylov& K = *sa.kry;
+    const bool mixed = sa.ctl->mixed;
+    K.ensure(m, mixed);
+    if (!K.run) {
+        K.run.reset(new SolveRun);
+        RT_HIP(hipHostMalloc((void**)&K.run->host, sizeof(CompState) * 6, hipHostMallocDefault));
+        RT_HIP(hipEventCreateWithFlags(&K.run->ev, hipEventDisableTiming));
+    }
+    SolveRun& R = *K.run;
+    R.mixed = mixed;
+    R.G = K.grid;
+    R.pgrid = std::min<int>(K.grid, (int)kblk(m.N));
+    R.N = m.N;
+    R.ne = (double)m.nEll;
+    R.nValid = 0;
+    for (int c = 0; c < 6; ++c) {
+        R.valid[c] = sa.valid[c];
+        R.nValid += sa.valid[c] > 0;
+    }
+    R.done = false;
+    R.rounds = 0;
+    if (mixed) {
+        fill_args<float>(ctx, sa, R);
+        launch_init<float>(ctx, R);
+    } else {
+        fill_args<double>(ctx, sa, R);
+        launch_init<double>(ctx, R);
+    }
+    RT_HIP(hipGetLastError());
+    queue_pull(ctx, K, R);
+}
+
+bool solve_advance(Ctx* ctx, Krylov& K, bool block) {
+    SolveRun& R = *K.run;
+    if (R.done) return true;
+    if (R.pending) {
+        if (block) {
+            RT_HIP(hipEventSynchronize(R.ev));
+        } else {
+            hipError_t e = hipEventQuery(R.ev);
+            if (e == hipErrorNotReady) return false;
+            RT_HIP(e);
+        }
+        R.pending = false;
+    }
+    double nAct = 0, nRef = 0;
+    for (int c = 0; c < 6; ++c) {
+        if (R.valid[c] <= 0) continue;
+        nAct += R.host[c].active != 0;
+        nRef += R.host[c].needRefine != 0;
+    }
+    if ((nAct == 0 && nRef == 0) || R.rounds >= 100000) {
+        R.done = true;
+        return true;
+    }
+    if (R.mixed) launch_round<float>(ctx, R, nAct, nRef);
+    else launch_round<double>(ctx, R, nAct, nRef);
+    RT_HIP(hipGetLastError());
+    ++R.rounds;
+    queue_pull(ctx, K, R);
+    return false;
+}
+
+void solve_finish(Krylov& K, rt_solver_perf perf[6]) {
+    SolveRun& R = *K.run;
+    for (int c = 0; c < 6; ++c) {
+        rt_solver_perf& pf = perf[c];
+        pf = rt_solver_perf{0, 0, 0, 0, 0, 0};
+        if (R.valid[c] <= 0) continue;
+        pf.initialResidual = R.host[c].initRes;
+        pf.finalResidual = R.host[c].finalRes;
+        pf.nIterations = R.host[c].iter;
+        pf.nRefinements = R.mixed ? R.host[c].nRefine : 0;
+        pf.converged = R.host[c].converged;
+        pf.solved = 1;
+    }
+}
+
+void solve_lockstep(Ctx* ctx, const SolveArgs& a, rt_solver_perf perf[6]) {
+    solve_begin(ctx, a);
+    while (!solve_advance(ctx, *a.kry, true)) {
+    }
+    solve_finish(*a.kry, perf);
+}
+
+int grid_for(const Ctx* ctx, int32_t N) {
+    const int64_t need = ((int64_t)N + KBLK - 1) / KBLK;
+    const int64_t cap = (int64_t)ctx->numCUs * 8;
+    int64_t g = need < cap ? need : cap;
+    if (g >= 64) g = g / 8 * 8;
+    return (int)std::max<int64_t>(1, g);
+}
+
+}  // namespace rt

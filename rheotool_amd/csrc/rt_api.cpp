@@ -1,0 +1,777 @@
+// rt_api.cpp -- host runtime and C-ABI entry points (include/rheo_gpu.h).
+//
+// Context (device, stream, HBM budget, kernel timing), mesh and model
+// handles, the constitutiveEq run-time selection from dictionary text, the
+// correct() sequence and the sparseSolver<symmTensor>::solve surface.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <set>
+
+#include "rt_dict.h"
+#include "rt_internal.h"
+
+namespace rt {
+
+void buildMesh(Ctx* ctx, const rt_mesh_desc* d, int renumber, Mesh& m);
+
+// ------------------------------------------------------------------ memory
+DBuf& DBuf::operator=(DBuf&& o) noexcept {
+    if (this != &o) {
+        release();
+        ctx = o.ctx;
+        p = o.p;
+        bytes = o.bytes;
+        o.ctx = nullptr;
+        o.p = nullptr;
+        o.bytes = 0;
+    }
+    return *this;
+}
+void DBuf::release() {
+    if (p) {
+        (void)hipFree(p);
+        if (ctx) ctx->inUse -= bytes;
+    }
+    p = nullptr;
+    bytes = 0;
+    ctx = nullptr;
+}
+
+DBuf Ctx::alloc(size_t bytes) {
+    if (bytes == 0) bytes = 8;
+    bytes = (bytes + 255) / 256 * 256;
+    if (budget && inUse + bytes > budget)
+        throw Error{RT_E_OOM, "HBM budget exceeded: " + std::to_string(inUse + bytes) + " > " +
+                                  std::to_string(budget) + " bytes"};
+    DBuf b;
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error{RT_E_OOM, "hipMalloc(" + std::to_string(bytes) + ") failed: " + hipGetErrorString(e)};
+    }
+    b.ctx = this;
+    b.bytes = bytes;
+    inUse += bytes;
+    return b;
+}
+
+void* Ctx::pinnedHost(size_t bytes) {
+    if (pinnedBytes < bytes) {
+        if (pinned) (void)hipHostFree(pinned);
+        pinned = nullptr;
+        RT_HIP(hipHostMalloc(&pinned, bytes, hipHostMallocDefault));
+        pinnedBytes = bytes;
+    }
+    return pinned;
+}
+
+hipEvent_t Ctx::getEvent() {
+    if (eventPool.empty()) {
+        hipEvent_t e;
+        RT_HIP(hipEventCreate(&e));
+        return e;
+    }
+    hipEvent_t e = eventPool.back();
+    eventPool.pop_back();
+    return e;
+}
+
+void Ctx::flushProfile() {
+    if (pending.empty()) return;
+    RT_HIP(hipStreamSynchronize(stream));
+    for (auto& p : pending) {
+        float ms = 0;
+        RT_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+        auto& s = stats[p.name];
+        s.launches += 1;
+        s.ms += ms;
+        s.bytes += p.bytes;
+        eventPool.push_back(p.a);
+        eventPool.push_back(p.b);
+    }
+    pending.clear();
+}
+
+// -------------------------------------------------------------- controls
+Controls parseControls(const char* text) {
+    Controls c;
+    c.tolerance = 1e-6;
+    if (!text) return c;
+    Dict d = parseDict(text);
+    // accept either the bare entry or `theta { ... }` / `solvers { theta {...} }`
+    const Dict* e = &d;
+    if (!d.has("solver")) {
+        if (d.subDicts.count("solvers")) e = &d.subDicts.at("solvers");
+        for (auto& kv : e->subDicts)
+            if (kv.second.has("solver")) {
+                e = &kv.second;
+                break;
+            }
+    }
+    std::string s;
+    if (e->word("solver", s)) {
+        if (s != "PBiCGStab" && s != "PBiCG")
+            throw Error{RT_E_UNKNOWN_TYPE, "Unknown asymmetric matrix solver " + s +
+                                               "\n\nValid asymmetric matrix solvers are :\n2\n(\nPBiCG\nPBiCGStab\n)"};
+        c.solverName = s;
+    }
+    std::string pc = "none";
+    if (e->subDicts.count("preconditioner")) {
+        e->subDicts.at("preconditioner").word("preconditioner", pc);
+    } else {
+        e->word("preconditioner", pc);
+    }
+    c.precondName = pc;
+    if (pc == "DILU" || pc == "multicolourILU0" || pc == "diagonal" || pc == "Jacobi" ||
+        pc == "DIC") {
+        c.precond = PC_JACOBI;
+    } else if (pc == "none") {
+        c.precond = PC_NONE;
+    } else {
+        throw Error{RT_E_UNKNOWN_TYPE, "Unknown asymmetric matrix preconditioner " + pc +
+                                           "\n\nValid asymmetric matrix preconditioners are :\n5\n(\nDILU\nJacobi\ndiagonal\nmulticolourILU0\nnone\n)"};
+    }
+    double v;
+    if (e->scalar("tolerance", v)) c.tolerance = v;
+    if (e->scalar("relTol", v)) c.relTol = v;
+    if (e->scalar("maxIter", v)) c.maxIter = (int)v;
+    if (e->scalar("minIter", v)) c.minIter = (int)v;
+    if (e->scalar("innerRelTol", v)) c.innerRelTol = v;
+    if (e->scalar("maxRefine", v)) c.maxRefine = (int)v;
+    if (e->word("precision", s)) {
+        if (s == "mixed") c.mixed = true;
+        else if (s == "double") c.mixed = false;
+        else throw Error{RT_E_UNKNOWN_TYPE, "Unknown precision " + s + " (valid: double, mixed)"};
+    }
+    return c;
+}
+
+// ------------------------------------------------------------ model setup
+namespace {
+
+const char* kTypes[] = {"FENE-PLog", "GiesekusLog", "Oldroyd-BLog", "PTTLog"};
+
+double need(const Dict& d, const char* key, const std::string& dictName) {
+    double v;
+    if (!d.scalar(key, v))
+        throw Error{RT_E_BAD_ARG, std::string("keyword ") + key + " is undefined in dictionary " + dictName};
+    return v;
+}
+
+void parseModel(Model& M, const char* text) {
+    if (!text) throw Error{RT_E_BAD_ARG, "rt_model_create: NULL dictionary"};
+    Dict full = parseDict(text);
+    const Dict& d = parametersOf(full);
+    const std::string dn = "constitutiveProperties.parameters";
+    std::string type;
+    if (!d.word("type", type)) throw Error{RT_E_BAD_ARG, "keyword type is undefined in dictionary " + dn};
+    ModelParams& p = M.prm;
+    if (type == "Oldroyd-BLog") p.kind = M_OLDROYDB;
+    else if (type == "GiesekusLog") p.kind = M_GIESEKUS;
+    else if (type == "FENE-PLog") p.kind = M_FENEP;
+    else if (type == "PTTLog") p.kind = M_PTT_LINEAR;
+    else {
+        std::string msg = "Unknown constitutiveEq type " + type + "\n\nValid constitutiveEq types are :\n" +
+                          std::to_string(sizeof(kTypes) / sizeof(kTypes[0])) + "\n(\n";
+        for (const char* t : kTypes) msg += std::string(t) + "\n";
+        throw Error{RT_E_UNKNOWN_TYPE, msg + ")"};
+    }
+    M.typeName = type;
+    p.rho = need(d, "rho", dn);
+    p.etaS = need(d, "etaS", dn);
+    p.etaP = need(d, "etaP", dn);
+    if (p.kind == M_PTT_LINEAR) {
+        p.epsilon = need(d, "epsilon", dn);
+        p.zeta = need(d, "zeta", dn);
+    }
+    p.lambda = need(d, "lambda", dn);
+    if (p.kind == M_GIESEKUS) p.alpha = need(d, "alpha", dn);
+    if (p.kind == M_FENEP) p.L2 = need(d, "L2", dn);
+    if (p.kind == M_PTT_LINEAR) {
+        std::string f;
+        if (!d.word("destructionFunctionType", f))
+            throw Error{RT_E_BAD_ARG, "keyword destructionFunctionType is undefined in dictionary " + dn};
+        if (f == "linear") p.kind = M_PTT_LINEAR;
+        else if (f == "exponential") p.kind = M_PTT_EXP;
+        else if (f == "generalized")
+            throw Error{RT_E_UNSUPPORTED, "PTTLog destructionFunctionType generalized is not supported on the device path"};
+        else
+            throw Error{RT_E_BAD_ARG, f + " is not in enumeration: 3(linear exponential generalized)"};
+    }
+    // checkForStab (constitutiveEq.C:430-439): the key is mandatory
+    std::string stab;
+    if (!d.word("stabilization", stab))
+        throw Error{RT_E_BAD_ARG, "keyword stabilization is undefined in dictionary " + dn};
+    if (stab != "none" && stab != "BSD" && stab != "coupling")
+        throw Error{RT_E_BAD_ARG, stab + " is not in enumeration: 3(none BSD coupling)"};
+    // thermo functions: only the isothermal default (Constant) is on this path
+    for (const char* tf : {"thermoLambda", "thermoEta"}) {
+        auto it = d.subDicts.find(tf);
+        if (it != d.subDicts.end()) {
+            std::string t = "Constant";
+            it->second.word("type", t);
+            if (t != "Constant")
+                throw Error{RT_E_UNSUPPORTED, std::string(tf) + " type " + t +
+                                                  " (non-isothermal) is not supported on the device path"};
+        }
+    }
+    double q;
+    if (d.scalar("ofInvQuirk", q)) p.ofInvQuirk = q != 0;
+}
+
+unsigned nb(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+// staging upload of a host AoS array, then transpose/permute on the device
+static void uploadAos(Ctx* ctx, DBuf& stage, const double* host, int64_t count) {
+    const size_t bytes = (size_t)count * sizeof(double);
+    if (stage.bytes < bytes) stage = ctx->alloc(bytes);
+    RT_HIP(hipMemcpyAsync(stage.p, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+}
+
+struct StagingRef {
+    DBuf& buf;
+};
+static StagingRef staging(Ctx* c) { return StagingRef{c->stage}; }
+
+int modelInvMask(Model& M) {
+    if (!M.prm.ofInvQuirk) return 0;
+    const Mesh& m = *M.mesh;
+    const int32_t r = m.perm[0];
+    double l[3];
+    for (int q = 0; q < 3; ++q)
+        RT_HIP(hipMemcpyAsync(&l[q], M.eigVal.as<double>() + (int64_t)q * m.N + r, sizeof(double),
+                              hipMemcpyDeviceToHost, M.ctx->stream));
+    RT_HIP(hipStreamSynchronize(M.ctx->stream));
+    const double s = l[0] * l[0] + l[1] * l[1] + l[2] * l[2];
+    int mask = 0;
+    for (int q = 0; q < 3; ++q)
+        if (l[q] * l[q] / s < 1e-15) mask |= 1 << q;
+    return mask;
+}
+
+void modelCorrectBegin(Model& M, double deltaT, const double* gradU) {
+    if (!M.hasState) throw Error{RT_E_BAD_ARG, "correct(): theta was never set (rt_model_set_state)"};
+    if (!M.hasFlow) throw Error{RT_E_BAD_ARG, "correct(): U/phi were never set (rt_model_set_flow)"};
+    if (!(deltaT > 0)) throw Error{RT_E_BAD_ARG, "correct(): deltaT must be positive"};
+    Ctx* ctx = M.ctx;
+    Mesh& m = *M.mesh;
+    const int32_t N = m.N;
+    M.kry.ensure(m, M.ctl.mixed);
+    if (!M.diag.p) {
+        M.diag = ctx->alloc((size_t)N * 8);
+        M.aval = ctx->alloc((size_t)std::max<int64_t>(1, m.nEll) * 8);
+        M.b = ctx->alloc((size_t)6 * N * 8);
+        RT_HIP(hipMemsetAsync(M.b.p, 0, (size_t)6 * N * 8, ctx->stream));
+    }
+    if (M.prm.kind == M_FENEP && !M.fOld.p) M.fOld = ctx->alloc((size_t)N * 8);
+    if (gradU) {
+        if (!M.gradU.p) M.gradU = ctx->alloc((size_t)9 * N * 8);
+        DBuf& st = staging(ctx).buf;
+        uploadAos(ctx, st, gradU, (int64_t)9 * N);
+        launch_aos_to_soa(ctx, N, 9, st.as<double>(), m.permD.as<int32_t>(), M.gradU.as<double>());
+    }
+    SolveState* st = M.kry.state.as<SolveState>();
+    AssembleArgs a{};
+    a.N = N;
+    a.ellOff = m.ellOff.as<int64_t>();
+    a.ellWidth = m.ellWidth.as<int32_t>();
+    a.ellCol = m.ellCol.as<int32_t>();
+    a.ellFid = m.ellFid.as<int32_t>();
+    a.bPtr = m.bPtr.as<int32_t>();
+    a.bFace = m.bFace.as<int32_t>();
+    a.bPatch = m.bPatch.as<int32_t>();
+    a.bcType = M.bcType.as<int32_t>();
+    a.bcVal = M.bcVal.as<double>();
+    a.NB = m.NB;
+    a.nFaces = (int64_t)m.F + m.NB;
+    a.Sf = m.Sf.as<double>();
+    a.w = m.w.as<double>();
+    a.V = m.V.as<double>();
+    a.U = M.U.as<double>();
+    a.Ub = M.Ub.as<double>();
+    a.phi = M.phi.as<double>();
+    a.gradU = gradU ? M.gradU.as<double>() : nullptr;
+    a.theta = M.theta.as<double>();
+    a.eigVal = M.eigVal.as<double>();
+    a.eigVec = M.eigVec.as<double>();
+    a.rDeltaT = 1.0 / deltaT;
+    a.relax = M.relax;
+    a.prm = M.prm;
+    a.invMask = modelInvMask(M);
+    a.aval = M.aval.as<double>();
+    a.diag = M.diag.as<double>();
+    a.b = M.b.as<double>();
+    a.fOld = M.prm.kind == M_FENEP ? M.fOld.as<double>() : nullptr;
+    a.partials = M.kry.partials.as<double>();
+    a.counter = st->counter + 4;
+    a.cs = st->c;
+    for (int q = 0; q < 6; ++q) a.valid[q] = m.valid[q];
+    launch_assemble(ctx, a, M.kry.grid);
+    if (!M.pendingSolve) M.pendingSolve = new SolveArgs{};
+    SolveArgs& sa = *M.pendingSolve;
+    sa = SolveArgs{};
+    sa.mesh = &m;
+    sa.kry = &M.kry;
+    sa.diag = M.diag.as<double>();
+    sa.aval = M.aval.as<double>();
+    sa.b = M.b.as<double>();
+    sa.x = M.theta.as<double>();
+    sa.valid = m.valid;
+    sa.ctl = &M.ctl;
+    sa.xbarKnown = true;
+    solve_begin(ctx, sa);
+}
+
+bool modelCorrectAdvance(Model& M, bool block) {
+    Ctx* ctx = M.ctx;
+    if (!solve_advance(ctx, M.kry, block)) return false;
+    solve_finish(M.kry, M.perf);
+    launch_eig_tau(ctx, M.mesh->N, M.prm, M.theta.as<double>(), M.eigVal.as<double>(),
+                   M.eigVec.as<double>(), M.tau.as<double>(),
+                   M.prm.kind == M_FENEP ? M.fOld.as<double>() : nullptr);
+    return true;
+}
+
+void modelCorrect(Model& M, double deltaT, const double* gradU) {
+    modelCorrectBegin(M, deltaT, gradU);
+    while (!modelCorrectAdvance(M, true)) {
+    }
+    RT_HIP(hipStreamSynchronize(M.ctx->stream));
+}
+
+size_t modelBytesEstimate(int32_t N, int32_t F, int32_t NB, int64_t nEll, bool mixed) {
+    const double n = N, f = F, nb = NB, ne = (double)nEll;
+    double by = 0;
+    // mesh: ELL col+fid, Sf, w, V, perm, boundary lists
+    by += ne * 8 + 3 * 8 * (f + nb) + 8 * f + 8 * n + 4 * n + 4 * (n + 1) + 12 * nb;
+    // model state + flow + matrix: theta tau eigVal eigVec U b diag fOld, Ub, phi, bc, aval
+    by += 8 * n * (6 + 6 + 3 + 9 + 3 + 6 + 1 + 1) + 8 * (3 * nb + f + nb) + 6 * 8 * nb + 8 * ne;
+    // Krylov: 5 vectors x 6 components, rD, ahat, partials, state
+    by += 6 * n * 5 * (mixed ? 4 : 8) + 8 * n + ne * (mixed ? 4 : 8) + 2048 * 24 * 8 + 4096;
+    // staging for AoS transfers (9 comps)
+    by += 9 * 8 * n;
+    return (size_t)(by * 1.1) + (1u << 20);
+}
+
+}  // namespace rt
+
+using namespace rt;
+
+// ======================================================================= C-ABI
+namespace {
+
+template <class F>
+int guarded(rt::Ctx* ctx, F&& f) {
+    try {
+        f();
+        if (ctx) ctx->lastError.clear();
+        return RT_OK;
+    } catch (const rt::Error& e) {
+        if (ctx) ctx->lastError = e.msg;
+        return e.code;
+    } catch (const std::exception& e) {
+        if (ctx) ctx->lastError = e.what();
+        return RT_E_BAD_ARG;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int rt_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+int rt_ctx_create(int device, size_t hbm_budget, rt_ctx** out) {
+    if (!out) return RT_E_BAD_ARG;
+    *out = nullptr;
+    int n = rt_device_count();
+    if (n <= 0 || device < 0 || device >= n) return RT_E_NO_DEVICE;
+    auto* c = new rt_ctx;
+    int rc = guarded(&c->impl, [&] {
+        Ctx& x = c->impl;
+        x.device = device;
+        RT_HIP(hipSetDevice(device));
+        hipDeviceProp_t prop;
+        RT_HIP(hipGetDeviceProperties(&prop, device));
+        if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+            throw Error{RT_E_NO_DEVICE, std::string("device ") + std::to_string(device) + " is " +
+                                            prop.gcnArchName + "; this library is built for gfx950"};
+        x.numCUs = prop.multiProcessorCount;
+        x.budget = hbm_budget;
+        RT_HIP(hipStreamCreateWithFlags(&x.stream, hipStreamNonBlocking));
+    });
+    if (rc != RT_OK) {
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return RT_OK;
+}
+
+void rt_ctx_destroy(rt_ctx* ctx) {
+    if (!ctx) return;
+    Ctx& x = ctx->impl;
+    (void)hipSetDevice(x.device);
+    (void)hipStreamSynchronize(x.stream);
+    for (auto& p : x.pending) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    for (auto e : x.eventPool) (void)hipEventDestroy(e);
+    if (x.pinned) (void)hipHostFree(x.pinned);
+    x.stage.release();
+    if (x.stream) (void)hipStreamDestroy(x.stream);
+    delete ctx;
+}
+
+const char* rt_last_error(const rt_ctx* ctx) { return ctx ? ctx->impl.lastError.c_str() : "NULL context"; }
+
+size_t rt_ctx_bytes_in_use(const rt_ctx* ctx) { return ctx ? ctx->impl.inUse : 0; }
+
+int rt_ctx_set_profiling(rt_ctx* ctx, int enable) {
+    if (!ctx) return RT_E_BAD_ARG;
+    return guarded(&ctx->impl, [&] {
+        ctx->impl.flushProfile();
+        ctx->impl.profiling = enable != 0;
+    });
+}
+
+int rt_ctx_kernel_stats(rt_ctx* ctx, rt_kernel_stat* out, int cap) {
+    if (!ctx) return -RT_E_BAD_ARG;
+    int n = 0;
+    int rc = guarded(&ctx->impl, [&] {
+        ctx->impl.flushProfile();
+        for (auto& kv : ctx->impl.stats) {
+            if (out && n < cap) {
+                std::memset(out[n].name, 0, sizeof(out[n].name));
+                std::strncpy(out[n].name, kv.first.c_str(), sizeof(out[n].name) - 1);
+                out[n].launches = kv.second.launches;
+                out[n].total_ms = kv.second.ms;
+                out[n].bytes = kv.second.bytes;
+            }
+            ++n;
+        }
+    });
+    return rc == RT_OK ? n : -rc;
+}
+
+void rt_ctx_reset_stats(rt_ctx* ctx) {
+    if (!ctx) return;
+    try {
+        ctx->impl.flushProfile();
+    } catch (...) {
+    }
+    ctx->impl.stats.clear();
+}
+
+int rt_mesh_create(rt_ctx* ctx, const rt_mesh_desc* desc, int renumber, rt_mesh** out) {
+    if (!ctx || !out) return RT_E_BAD_ARG;
+    *out = nullptr;
+    auto* m = new rt_mesh;
+    int rc = guarded(&ctx->impl, [&] {
+        RT_HIP(hipSetDevice(ctx->impl.device));
+        buildMesh(&ctx->impl, desc, renumber, m->impl);
+    });
+    if (rc != RT_OK) {
+        delete m;
+        return rc;
+    }
+    *out = m;
+    return RT_OK;
+}
+
+void rt_mesh_destroy(rt_mesh* mesh) { delete mesh; }
+
+int rt_mesh_valid_components(const rt_mesh* mesh, int32_t out[6]) {
+    if (!mesh || !out) return RT_E_BAD_ARG;
+    for (int q = 0; q < 6; ++q) out[q] = mesh->impl.valid[q];
+    return RT_OK;
+}
+
+int rt_model_create(rt_ctx* ctx, const char* name, const rt_mesh* mesh, const char* dict,
+                    rt_model** out) {
+    if (!ctx || !out) return RT_E_BAD_ARG;
+    *out = nullptr;
+    if (!mesh) {
+        ctx->impl.lastError = "rt_model_create: NULL mesh";
+        return RT_E_BAD_ARG;
+    }
+    auto* mo = new rt_model;
+    int rc = guarded(&ctx->impl, [&] {
+        Model& M = mo->impl;
+        Ctx* c = &ctx->impl;
+        RT_HIP(hipSetDevice(c->device));
+        M.ctx = c;
+        M.mesh = const_cast<Mesh*>(&mesh->impl);
+        M.name = name ? name : "";
+        parseModel(M, dict);
+        const int32_t N = M.mesh->N, NB = M.mesh->NB;
+        M.theta = c->alloc((size_t)6 * N * 8);
+        M.tau = c->alloc((size_t)6 * N * 8);
+        M.eigVal = c->alloc((size_t)3 * N * 8);
+        M.eigVec = c->alloc((size_t)9 * N * 8);
+        M.U = c->alloc((size_t)3 * N * 8);
+        M.Ub = c->alloc((size_t)3 * std::max(1, NB) * 8);
+        M.phi = c->alloc(((size_t)M.mesh->F + NB) * 8);
+        M.bcType = c->alloc((size_t)std::max(1, M.mesh->nPatches) * 4);
+        M.bcVal = c->alloc((size_t)6 * std::max(1, NB) * 8);
+        RT_HIP(hipMemsetAsync(M.bcVal.p, 0, M.bcVal.bytes, c->stream));
+        RT_HIP(hipMemsetAsync(M.tau.p, 0, M.tau.bytes, c->stream));
+        M.hBcType.resize(M.mesh->nPatches);
+        for (int p = 0; p < M.mesh->nPatches; ++p)
+            M.hBcType[p] = M.mesh->patches[p].kind == RT_PATCH_EMPTY ? RT_BC_EMPTY : RT_BC_ZEROGRADIENT;
+        if (!M.hBcType.empty())
+            RT_HIP(hipMemcpyAsync(M.bcType.p, M.hBcType.data(), M.hBcType.size() * 4,
+                                  hipMemcpyHostToDevice, c->stream));
+        // READ_IF_PRESENT default of eigVals/eigVecs: identity tensor
+        launch_fill(c, M.eigVal.as<double>(), (int64_t)3 * N, 1.0);
+        launch_fill(c, M.eigVec.as<double>(), (int64_t)9 * N, 0.0);
+        for (int q : {0, 4, 8}) launch_fill(c, M.eigVec.as<double>() + (int64_t)q * N, N, 1.0);
+        M.ctl = parseControls(nullptr);
+        RT_HIP(hipStreamSynchronize(c->stream));
+    });
+    if (rc != RT_OK) {
+        delete mo;
+        return rc;
+    }
+    *out = mo;
+    return RT_OK;
+}
+
+void rt_model_destroy(rt_model* model) {
+    if (!model) return;
+    delete model->impl.pendingSolve;
+    delete model;
+}
+
+const char* rt_model_type(const rt_model* model) { return model ? model->impl.typeName.c_str() : ""; }
+
+int rt_model_set_state(rt_model* model, const double* theta, const double* eigVals,
+                       const double* eigVecs, const double* tau) {
+    if (!model) return RT_E_BAD_ARG;
+    Model& M = model->impl;
+    return guarded(M.ctx, [&] {
+        if (!theta) throw Error{RT_E_BAD_ARG, "rt_model_set_state: theta is required (MUST_READ)"};
+        Ctx* c = M.ctx;
+        Mesh& m = *M.mesh;
+        const int32_t N = m.N;
+        RT_HIP(hipSetDevice(c->device));
+        DBuf& st = staging(c).buf;
+        const int32_t* perm = m.permD.as<int32_t>();
+        uploadAos(c, st, theta, (int64_t)6 * N);
+        launch_aos_to_soa(c, N, 6, st.as<double>(), perm, M.theta.as<double>());
+        if (eigVals) {
+            RT_HIP(hipStreamSynchronize(c->stream));
+            uploadAos(c, st, eigVals, (int64_t)9 * N);
+            launch_eigvals_from_tensor(c, N, st.as<double>(), perm, M.eigVal.as<double>());
+        }
+        if (eigVecs) {
+            RT_HIP(hipStreamSynchronize(c->stream));
+            uploadAos(c, st, eigVecs, (int64_t)9 * N);
+            launch_aos_to_soa(c, N, 9, st.as<double>(), perm, M.eigVec.as<double>());
+        }
+        if (tau) {
+            RT_HIP(hipStreamSynchronize(c->stream));
+            uploadAos(c, st, tau, (int64_t)6 * N);
+            launch_aos_to_soa(c, N, 6, st.as<double>(), perm, M.tau.as<double>());
+        }
+        RT_HIP(hipStreamSynchronize(c->stream));
+        M.hasState = true;
+    });
+}
+
+int rt_model_set_bc(rt_model* model, int patch, int bcType, const double* values) {
+    if (!model) return RT_E_BAD_ARG;
+    Model& M = model->impl;
+    return guarded(M.ctx, [&] {
+        Mesh& m = *M.mesh;
+        if (patch < 0 || patch >= m.nPatches) throw Error{RT_E_BAD_ARG, "rt_model_set_bc: no such patch"};
+        if (bcType < RT_BC_ZEROGRADIENT || bcType > RT_BC_EMPTY)
+            throw Error{RT_E_UNKNOWN_TYPE, "rt_model_set_bc: unknown patchField type"};
+        const bool emptyPatch = m.patches[patch].kind == RT_PATCH_EMPTY;
+        if (emptyPatch != (bcType == RT_BC_EMPTY))
+            throw Error{RT_E_BAD_ARG, "rt_model_set_bc: empty patches take exactly the empty condition"};
+        M.hBcType[patch] = bcType;
+        RT_HIP(hipMemcpy(M.bcType.as<int32_t>() + patch, &bcType, 4, hipMemcpyHostToDevice));
+        if (bcType == RT_BC_FIXEDVALUE) {
+            if (!values) throw Error{RT_E_BAD_ARG, "rt_model_set_bc: fixedValue needs values"};
+            const rt_patch& p = m.patches[patch];
+            std::vector<double> col(p.size);
+            for (int q = 0; q < 6; ++q) {
+                for (int k = 0; k < p.size; ++k) col[k] = values[6 * (size_t)k + q];
+                RT_HIP(hipMemcpy(M.bcVal.as<double>() + (int64_t)q * m.NB + p.start, col.data(),
+                                 col.size() * 8, hipMemcpyHostToDevice));
+            }
+        }
+    });
+}
+
+int rt_model_set_flow(rt_model* model, const double* U, const double* Ub, const double* phi) {
+    if (!model) return RT_E_BAD_ARG;
+    Model& M = model->impl;
+    return guarded(M.ctx, [&] {
+        Mesh& m = *M.mesh;
+        if (!U || !phi || (m.NB > 0 && !Ub))
+            throw Error{RT_E_BAD_ARG, "rt_model_set_flow: U, Ub and phi are required"};
+        Ctx* c = M.ctx;
+        RT_HIP(hipSetDevice(c->device));
+        DBuf& st = staging(c).buf;
+        uploadAos(c, st, U, (int64_t)3 * m.N);
+        launch_aos_to_soa(c, m.N, 3, st.as<double>(), m.permD.as<int32_t>(), M.U.as<double>());
+        if (m.NB > 0) {
+            std::vector<double> ub((size_t)3 * m.NB);
+            for (int32_t b = 0; b < m.NB; ++b)
+                for (int q = 0; q < 3; ++q) ub[(size_t)q * m.NB + b] = Ub[3 * (size_t)b + q];
+            RT_HIP(hipMemcpyAsync(M.Ub.p, ub.data(), ub.size() * 8, hipMemcpyHostToDevice, c->stream));
+            RT_HIP(hipStreamSynchronize(c->stream));
+        }
+        RT_HIP(hipMemcpyAsync(M.phi.p, phi, ((size_t)m.F + m.NB) * 8, hipMemcpyHostToDevice, c->stream));
+        RT_HIP(hipStreamSynchronize(c->stream));
+        M.hasFlow = true;
+    });
+}
+
+int rt_model_set_solver(rt_model* model, const char* controls) {
+    if (!model) return RT_E_BAD_ARG;
+    Model& M = model->impl;
+    return guarded(M.ctx, [&] { M.ctl = parseControls(controls); });
+}
+
+int rt_model_set_relax(rt_model* model, double factor) {
+    if (!model) return RT_E_BAD_ARG;
+    model->impl.relax = factor;
+    return RT_OK;
+}
+
+int rt_model_correct(rt_model* model, double deltaT, const double* gradU) {
+    if (!model) return RT_E_BAD_ARG;
+    Model& M = model->impl;
+    int rc = guarded(M.ctx, [&] {
+        RT_HIP(hipSetDevice(M.ctx->device));
+        modelCorrect(M, deltaT, gradU);
+    });
+    if (rc != RT_OK) return rc;
+    for (int q = 0; q < 6; ++q)
+        if (M.perf[q].solved && !M.perf[q].converged) return RT_W_NOT_CONVERGED;
+    return RT_OK;
+}
+
+int rt_model_get(const rt_model* model, int field, double* out) {
+    if (!model || !out) return RT_E_BAD_ARG;
+    Model& M = const_cast<Model&>(model->impl);
+    return guarded(M.ctx, [&] {
+        Ctx* c = M.ctx;
+        Mesh& m = *M.mesh;
+        const int32_t N = m.N;
+        RT_HIP(hipSetDevice(c->device));
+        DBuf& st = staging(c).buf;
+        int nc = field == RT_THETA || field == RT_TAU ? 6 : 9;
+        if (field < RT_THETA || field > RT_EIGVECS) throw Error{RT_E_BAD_ARG, "rt_model_get: unknown field"};
+        const size_t bytes = (size_t)nc * N * 8;
+        if (st.bytes < bytes) st = c->alloc(bytes);
+        const int32_t* perm = m.permD.as<int32_t>();
+        if (field == RT_THETA) launch_soa_to_aos(c, N, 6, M.theta.as<double>(), perm, st.as<double>());
+        else if (field == RT_TAU) launch_soa_to_aos(c, N, 6, M.tau.as<double>(), perm, st.as<double>());
+        else if (field == RT_EIGVALS) launch_eigvals_to_tensor(c, N, M.eigVal.as<double>(), perm, st.as<double>());
+        else launch_soa_to_aos(c, N, 9, M.eigVec.as<double>(), perm, st.as<double>());
+        RT_HIP(hipMemcpyAsync(out, st.p, bytes, hipMemcpyDeviceToHost, c->stream));
+        RT_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int rt_model_solver_perf(const rt_model* model, rt_solver_perf perf[6]) {
+    if (!model || !perf) return RT_E_BAD_ARG;
+    for (int q = 0; q < 6; ++q) perf[q] = model->impl.perf[q];
+    return RT_OK;
+}
+
+int rt_symm_solve(rt_ctx* ctx, const rt_mesh* mesh, const rt_ldu_symm* sys, double* psi,
+                  const char* controls, rt_solver_perf perf[6]) {
+    if (!ctx) return RT_E_BAD_ARG;
+    return guarded(&ctx->impl, [&] {
+        if (!mesh || !sys || !psi || !perf || !sys->diag || !sys->source ||
+            (mesh->impl.F > 0 && (!sys->lower || !sys->upper)))
+            throw Error{RT_E_BAD_ARG, "rt_symm_solve: incomplete system"};
+        Ctx* c = &ctx->impl;
+        Mesh& m = const_cast<Mesh&>(mesh->impl);
+        RT_HIP(hipSetDevice(c->device));
+        Controls ctl = parseControls(controls);
+        const int32_t N = m.N, F = m.F, NB = m.NB;
+        // boundary coefficients (sparseSolver.C:72-127): iC -> diag, bC -> source
+        std::vector<double> d(sys->diag, sys->diag + N);
+        std::vector<double> src(sys->source, sys->source + (size_t)6 * N);
+        for (int32_t p = 0; p < m.nPatches; ++p) {
+            const rt_patch& P = m.patches[p];
+            if (P.kind == RT_PATCH_EMPTY) continue;
+            for (int32_t k = 0; k < P.size; ++k) {
+                const int32_t b = P.start + k;
+                const int32_t cell = m.hOwner[(size_t)F + b];
+                if (sys->internalCoeffs) {
+                    const double* ic = sys->internalCoeffs + 6 * (size_t)b;
+                    for (int q = 1; q < 6; ++q)
+                        if (ic[q] != ic[0] && m.valid[q] > 0)
+                            throw Error{RT_E_UNSUPPORTED,
+                                        "rt_symm_solve: component-dependent internalCoeffs (transform patches) are not supported"};
+                    d[cell] += ic[0];
+                }
+                if (sys->boundaryCoeffs)
+                    for (int q = 0; q < 6; ++q) src[6 * (size_t)cell + q] += sys->boundaryCoeffs[6 * (size_t)b + q];
+            }
+        }
+        std::unique_ptr<Krylov> Kp(new Krylov);  // workspace of this call
+        Krylov& K = *Kp;
+        K.ensure(m, ctl.mixed);
+        DBuf dD = c->alloc((size_t)N * 8), dA = c->alloc((size_t)std::max<int64_t>(1, m.nEll) * 8),
+             dB = c->alloc((size_t)6 * N * 8), dX = c->alloc((size_t)6 * N * 8),
+             dLo = c->alloc((size_t)std::max(1, F) * 8), dUp = c->alloc((size_t)std::max(1, F) * 8);
+        std::vector<double> dperm(N);
+        for (int32_t r = 0; r < N; ++r) dperm[r] = d[m.iperm[r]];
+        RT_HIP(hipMemcpyAsync(dD.p, dperm.data(), (size_t)N * 8, hipMemcpyHostToDevice, c->stream));
+        if (F > 0) {
+            RT_HIP(hipMemcpyAsync(dLo.p, sys->lower, (size_t)F * 8, hipMemcpyHostToDevice, c->stream));
+            RT_HIP(hipMemcpyAsync(dUp.p, sys->upper, (size_t)F * 8, hipMemcpyHostToDevice, c->stream));
+        }
+        launch_ldu_to_ell(c, m, dLo.as<double>(), dUp.as<double>(), dA.as<double>());
+        DBuf& st = staging(c).buf;
+        uploadAos(c, st, src.data(), (int64_t)6 * N);
+        launch_aos_to_soa(c, N, 6, st.as<double>(), m.permD.as<int32_t>(), dB.as<double>());
+        RT_HIP(hipStreamSynchronize(c->stream));
+        uploadAos(c, st, psi, (int64_t)6 * N);
+        launch_aos_to_soa(c, N, 6, st.as<double>(), m.permD.as<int32_t>(), dX.as<double>());
+        // gAverage(psi) per component for the normalisation factor
+        SolveState* ss = K.state.as<SolveState>();
+        for (int q = 0; q < 6; ++q) {
+            double s = 0;
+            for (int32_t i = 0; i < N; ++i) s += psi[6 * (size_t)i + q];
+            const double xbar = s / N;
+            RT_HIP(hipMemcpyAsync(&ss->c[q].xbar, &xbar, 8, hipMemcpyHostToDevice, c->stream));
+            RT_HIP(hipStreamSynchronize(c->stream));
+        }
+        SolveArgs sa{};
+        sa.mesh = &m;
+        sa.kry = &K;
+        sa.diag = dD.as<double>();
+        sa.aval = dA.as<double>();
+        sa.b = dB.as<double>();
+        sa.x = dX.as<double>();
+        sa.valid = m.valid;
+        sa.ctl = &ctl;
+        sa.xbarKnown = true;
+        solve_lockstep(c, sa, perf);
+        launch_soa_to_aos(c, N, 6, dX.as<double>(), m.permD.as<int32_t>(), st.as<double>());
+        RT_HIP(hipMemcpyAsync(psi, st.p, (size_t)6 * N * 8, hipMemcpyDeviceToHost, c->stream));
+        RT_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,329 @@
+// rt_internal.h -- internal types shared by the host runtime and the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "rheo_gpu.h"
+
+namespace rt {
+
+// ----------------------------------------------------------------- errors
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define RT_HIP(call)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            throw ::rt::Error{RT_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+// ----------------------------------------------------------------- context
+struct Ctx;
+
+// Device buffer with budget accounting; move-only.
+struct DBuf {
+    Ctx* ctx = nullptr;
+    void* p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept { *this = std::move(o); }
+    DBuf& operator=(DBuf&& o) noexcept;
+    ~DBuf() { release(); }
+    void release();
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct KernelStat {
+    int64_t launches = 0;
+    double ms = 0;
+    double bytes = 0;
+};
+
+struct Ctx {
+    int device = 0;
+    int numCUs = 256;
+    hipStream_t stream = nullptr;
+    size_t budget = 0, inUse = 0;
+    std::string lastError;
+    // profiling
+    bool profiling = false;
+    struct Pending {
+        std::string name;
+        hipEvent_t a, b;
+        double bytes;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> eventPool;
+    std::map<std::string, KernelStat> stats;
+    // pinned host scratch for status read-back
+    void* pinned = nullptr;
+    size_t pinnedBytes = 0;
+    // device staging buffer for host AoS uploads/downloads
+    DBuf stage;
+
+    DBuf alloc(size_t bytes);
+    void* pinnedHost(size_t bytes);
+    hipEvent_t getEvent();
+    void flushProfile();  // synchronises and folds pending events into stats
+};
+
+// Launch helper that records HIP events around the launch when profiling.
+struct ProfScope {
+    Ctx* ctx;
+    const char* name;
+    double bytes;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(Ctx* c, const char* n, double by) : ctx(c), name(n), bytes(by) {
+        if (ctx->profiling) {
+            a = ctx->getEvent();
+            b = ctx->getEvent();
+            RT_HIP(hipEventRecord(a, ctx->stream));
+        }
+    }
+    ~ProfScope() noexcept(false) {
+        if (a) {
+            RT_HIP(hipEventRecord(b, ctx->stream));
+            ctx->pending.push_back({name, a, b, bytes});
+        }
+    }
+};
+
+// ------------------------------------------------------------------- mesh
+// Device layout.  Rows are cells in the device ("internal") numbering.
+// Sliced ELL with 64-row slices: entry k of row r = 64*s + lane lives at
+// ellOff[s] + 64*k + lane; within a row entries are in increasing face id,
+// padding (col = row, fid = RT_FID_PAD) at the end.
+// fid >= 0: the row is the owner of face fid; fid < 0: the row is the
+// neighbour of face (-fid-1).
+constexpr int32_t RT_FID_PAD = INT32_MIN;
+constexpr int SLICE = 64;
+
+struct Mesh {
+    Ctx* ctx = nullptr;
+    int32_t N = 0, F = 0, NB = 0, nPatches = 0;
+    int32_t nSlices = 0;
+    int64_t nEll = 0;  // total ELL entries incl. padding
+    int maxWidth = 0;
+    int emptyDir = -1;
+    int32_t valid[6];
+    bool renumbered = false;
+    std::vector<int32_t> perm;   // external cell -> internal row
+    std::vector<int32_t> iperm;  // internal row -> external cell
+    std::vector<rt_patch> patches;
+    // host copies needed by the model / solve surface
+    std::vector<int32_t> hOwner, hNeighbour;  // external numbering
+    // device arrays
+    DBuf ellOff;    // int64 [nSlices+1]
+    DBuf ellWidth;  // int32 [nSlices]
+    DBuf ellCol;    // int32 [nEll]
+    DBuf ellFid;    // int32 [nEll]
+    DBuf bPtr;      // int32 [N+1]   boundary faces of each row (non-empty patches)
+    DBuf bFace;     // int32 [...]  boundary face index b (0..NB-1), ascending
+    DBuf bPatch;    // int32 [NB]    patch of every boundary face
+    DBuf Sf;        // double [3][F+NB] SoA
+    DBuf w;         // double [F]
+    DBuf V;         // double [N] internal order
+    DBuf permD;     // int32 [N] internal row -> external cell
+    double meanAbsBandwidth = 0;
+};
+
+// --------------------------------------------------------- solver controls
+enum Precond { PC_NONE = 0, PC_JACOBI = 1, PC_MCILU0 = 2 };
+struct Controls {
+    std::string solverName = "PBiCGStab";
+    std::string precondName = "DILU";
+    int precond = PC_JACOBI;
+    double tolerance = 1e-6, relTol = 0.0;
+    int maxIter = 1000, minIter = 0;
+    bool mixed = false;
+    double innerRelTol = 1e-5;  // FP32 inner solve reduction per refinement sweep
+    int maxRefine = 12;
+};
+// Parses an fvSolution-style entry; throws Error.
+Controls parseControls(const char* text);
+
+// Per-component solver state living on the device, written by the last
+// workgroup of each reduction kernel and read by the next kernels.
+struct alignas(64) CompState {
+    double rho, rhoOld, alpha, omega, beta;
+    double normFactor, initRes, finalRes, xbar;
+    double tmp0, tmp1, tmp2;
+    int32_t iter, active, converged, solved;
+    int32_t exitAtS, needRefine, nRefine, innerIter;
+    double innerTarget;  // inner (FP32) residual target per sweep
+    double pad[3];
+};
+static_assert(sizeof(CompState) % 8 == 0, "CompState layout");
+
+struct SolveState {
+    CompState c[6];
+    int32_t anyActive;
+    int32_t counter[7];  // ticket counters of the reduction kernels
+    double misc[8];
+};
+
+struct SolveRun;  // host side of a resumable solve (kernels_krylov.hip)
+struct SolveRunDeleter {
+    void operator()(SolveRun* r) const;
+};
+
+// Workspace of the lockstep Krylov solver (all 6 components, SoA).
+struct Krylov {
+    Ctx* ctx = nullptr;
+    int32_t N = 0;
+    int grid = 0;  // workgroups of the row kernels
+    DBuf r, r0, p, v, t;   // FP64 [6][N]
+    DBuf rs, r0s, ps, vs, ts;  // FP32 [6][N] (mixed precision inner solve)
+    DBuf ahat32;           // float [nEll]  column-scaled off-diagonals A D^-1
+    DBuf ahat64;           // double [nEll]
+    DBuf rD;               // double [N]    1/diag
+    DBuf partials;         // double [grid][24]
+    DBuf state;            // SolveState
+    std::unique_ptr<SolveRun, SolveRunDeleter> run;
+    void ensure(Mesh& m, bool mixed);
+};
+
+// ------------------------------------------------------------------- model
+enum ModelKind { M_OLDROYDB = 0, M_GIESEKUS = 1, M_FENEP = 2, M_PTT_LINEAR = 3, M_PTT_EXP = 4 };
+
+struct ModelParams {
+    int kind = M_OLDROYDB;
+    double rho = 1, etaS = 0, etaP = 0, lambda = 1;
+    double alpha = 0, L2 = 0, epsilon = 0, zeta = 0;
+    int ofInvQuirk = 0;
+};
+
+struct Model {
+    Ctx* ctx = nullptr;
+    Mesh* mesh = nullptr;
+    std::string name, typeName;
+    ModelParams prm;
+    Controls ctl;
+    double relax = 0;
+    // device state, SoA in internal numbering
+    DBuf theta, tau;        // [6][N]
+    DBuf eigVal;            // [3][N] conformation eigenvalues (diagonal of eigVals)
+    DBuf eigVec;            // [9][N]
+    DBuf U;                 // [3][N]
+    DBuf Ub;                // [3][NB]
+    DBuf phi;               // [F+NB]
+    DBuf gradU;             // [9][N] when supplied
+    DBuf diag, aval, b, fOld;  // matrix/source of the step
+    DBuf bcType;            // int32 [nPatches]
+    DBuf bcVal;             // double [6][NB]
+    std::vector<int32_t> hBcType;
+    bool hasFlow = false, hasState = false;
+    Krylov kry;
+    rt_solver_perf perf[6];
+    struct SolveArgs* pendingSolve = nullptr;  // set between correct begin/advance
+};
+
+// ---------------------------------------------------------------- kernels
+struct AssembleArgs {
+    int32_t N;
+    const int64_t* ellOff;
+    const int32_t* ellWidth;
+    const int32_t* ellCol;
+    const int32_t* ellFid;
+    const int32_t* bPtr;
+    const int32_t* bFace;
+    const int32_t* bPatch;
+    const int32_t* bcType;
+    const double* bcVal;  // [6][NB]
+    int32_t NB;
+    int64_t nFaces;       // F + NB (SoA stride of Sf)
+    const double* Sf;
+    const double* w;
+    const double* V;
+    const double* U;
+    const double* Ub;
+    const double* phi;
+    const double* gradU;  // nullable
+    const double* theta;
+    const double* eigVal;
+    const double* eigVec;
+    double rDeltaT;
+    double relax;         // <= 0: none
+    ModelParams prm;
+    int32_t invMask;      // [OF9] inv() first-cell shift bits (ofInvQuirk)
+    double* aval;
+    double* diag;
+    double* b;            // [6][N]
+    double* fOld;         // FENE-P f, nullable
+    double* partials;     // [grid][6] sums of theta (mean for normFactor)
+    int32_t* counter;
+    CompState* cs;
+    int32_t valid[6];
+};
+
+void launch_assemble(Ctx* ctx, const AssembleArgs& a, int grid);
+void launch_eig_tau(Ctx* ctx, int32_t N, const ModelParams& prm, const double* theta,
+                    double* eigVal, double* eigVec, double* tau, const double* fOld);
+void launch_aos_to_soa(Ctx* ctx, int32_t N, int ncomp, const double* aos, const int32_t* perm,
+                       double* soa);
+void launch_soa_to_aos(Ctx* ctx, int32_t N, int ncomp, const double* soa, const int32_t* perm,
+                       double* aos);
+void launch_eigvals_from_tensor(Ctx* ctx, int32_t N, const double* aos9, const int32_t* perm,
+                                double* eigVal);
+void launch_eigvals_to_tensor(Ctx* ctx, int32_t N, const double* eigVal, const int32_t* perm,
+                              double* aos9);
+void launch_fill(Ctx* ctx, double* p, int64_t n, double v);
+
+// user-supplied LDU -> ELL values and diagonal (the sparseSolver surface)
+void launch_ldu_to_ell(Ctx* ctx, const Mesh& m, const double* lower, const double* upper,
+                       double* aval);
+
+struct SolveArgs {
+    Mesh* mesh;
+    Krylov* kry;
+    const double* diag;  // [N]
+    const double* aval;  // ELL values
+    const double* b;     // [6][N]
+    double* x;           // [6][N] (theta in place)
+    const int32_t* valid;
+    const Controls* ctl;
+    bool xbarKnown;      // CompState.xbar already set (by the assemble kernel)
+};
+// Lockstep PBiCGStab over the valid components; fills perf.
+void solve_lockstep(Ctx* ctx, const SolveArgs& a, rt_solver_perf perf[6]);
+// The same as a resumable job: begin launches the initial residual;
+// advance launches the next round once the previous round's state is on the
+// host (block = false returns false immediately when it is not yet) and
+// returns true when every component is finished; finish fills perf.
+void solve_begin(Ctx* ctx, const SolveArgs& a);
+bool solve_advance(Ctx* ctx, Krylov& K, bool block);
+void solve_finish(Krylov& K, rt_solver_perf perf[6]);
+
+int grid_for(const Ctx* ctx, int32_t N);
+
+// correct() as a resumable job (the serving loop interleaves several):
+// begin launches the assembly and the initial residual; advance returns true
+// once the solve has finished and the eigen/tau kernel has been launched.
+void modelCorrectBegin(Model& M, double deltaT, const double* gradU);
+bool modelCorrectAdvance(Model& M, bool block);
+void modelCorrect(Model& M, double deltaT, const double* gradU);
+// device bytes a model of this size allocates (for admission control)
+size_t modelBytesEstimate(int32_t N, int32_t F, int32_t NB, int64_t nEll, bool mixed);
+
+}  // namespace rt
+
+struct rt_ctx {
+    rt::Ctx impl;
+};
+struct rt_mesh {
+    rt::Mesh impl;
+};
+struct rt_model {
+    rt::Model impl;
+};

@@ -1,0 +1,296 @@
+// rt_serve.cpp -- continuous-batching request loop for independent stress steps.
+//
+// Requests (a mesh, a model dictionary, the state and the flow of one
+// stress step) are admitted while a slot and the HBM budget allow, then
+// advanced together: every progress() call moves each in-flight request one
+// round further (assembly, Krylov rounds, eigen/tau, download) without
+// blocking on any of them.  Each slot owns a context with its own stream, so
+// the rounds of different requests overlap on the device, and requests that
+// arrive while others run are admitted as soon as a slot and memory free up.
+// A request's kernels depend only on its own data (fixed grids, per-request
+// reductions in a fixed order), so its result does not depend on the mix.
+#include <chrono>
+#include <deque>
+#include <map>
+
+#include "rt_internal.h"
+
+namespace rt {
+
+struct Request {
+    int64_t ticket = 0;
+    // host copies of the submission
+    std::vector<int32_t> owner, neighbour;
+    std::vector<double> Sf, w, V;
+    std::vector<rt_patch> patches;
+    rt_mesh_desc desc{};
+    std::string dict, controls;
+    std::vector<double> theta, eigVals, eigVecs, U, Ub, phi;
+    double deltaT = 0;
+    size_t estBytes = 0;
+    // runtime
+    int slot = -1;
+    rt_mesh* mesh = nullptr;
+    rt_model* model = nullptr;
+    enum { QUEUED, RUNNING, FINISHING, DONE, FAILED } state = QUEUED;
+    hipEvent_t done = nullptr;
+    std::chrono::steady_clock::time_point tSubmit, tDone;
+    std::vector<double> outTheta, outTau;
+    rt_solver_perf perf[6];
+    int rc = RT_OK;
+    std::string err;
+};
+
+}  // namespace rt
+
+using namespace rt;
+
+struct rt_server {
+    rt_ctx* parent = nullptr;
+    size_t budget = 0, inflightBytes = 0;
+    std::vector<rt_ctx*> slots;
+    std::vector<Request*> slotReq;
+    std::deque<Request*> queue;
+    std::map<int64_t, std::unique_ptr<Request>> all;
+    int64_t nextTicket = 1;
+    std::string defaultControls = "solver PBiCGStab; preconditioner DILU; tolerance 1e-10; relTol 0; "
+                                  "maxIter 1000; precision mixed;";
+};
+
+namespace {
+
+void releaseRequest(rt_server* S, Request* r) {
+    if (r->model) rt_model_destroy(r->model);
+    if (r->mesh) rt_mesh_destroy(r->mesh);
+    r->model = nullptr;
+    r->mesh = nullptr;
+    if (r->done) (void)hipEventDestroy(r->done);
+    r->done = nullptr;
+    if (r->slot >= 0) {
+        S->slotReq[r->slot] = nullptr;
+        S->inflightBytes -= r->estBytes;
+        r->slot = -1;
+    }
+    // the submission copies are no longer needed
+    std::vector<int32_t>().swap(r->owner);
+    std::vector<int32_t>().swap(r->neighbour);
+    for (auto* v : {&r->Sf, &r->w, &r->V, &r->theta, &r->eigVals, &r->eigVecs, &r->U, &r->Ub, &r->phi})
+        std::vector<double>().swap(*v);
+}
+
+void fail(rt_server* S, Request* r, int rc, const std::string& msg) {
+    r->rc = rc;
+    r->err = msg;
+    r->state = Request::FAILED;
+    r->tDone = std::chrono::steady_clock::now();
+    releaseRequest(S, r);
+}
+
+// creates the device objects of a request in a free slot and launches its first work
+void admit(rt_server* S, Request* r, int slot) {
+    rt_ctx* c = S->slots[slot];
+    r->slot = slot;
+    S->slotReq[slot] = r;
+    S->inflightBytes += r->estBytes;
+    int rc = rt_mesh_create(c, &r->desc, RT_RENUMBER_AUTO, &r->mesh);
+    if (rc == RT_OK) rc = rt_model_create(c, "", r->mesh, r->dict.c_str(), &r->model);
+    if (rc == RT_OK) rc = rt_model_set_solver(r->model, r->controls.c_str());
+    if (rc == RT_OK)
+        rc = rt_model_set_state(r->model, r->theta.data(), r->eigVals.empty() ? nullptr : r->eigVals.data(),
+                                r->eigVecs.empty() ? nullptr : r->eigVecs.data(), nullptr);
+    if (rc == RT_OK)
+        rc = rt_model_set_flow(r->model, r->U.data(), r->Ub.empty() ? nullptr : r->Ub.data(), r->phi.data());
+    if (rc != RT_OK) {
+        fail(S, r, rc, rt_last_error(c));
+        return;
+    }
+    try {
+        RT_HIP(hipSetDevice(c->impl.device));
+        modelCorrectBegin(r->model->impl, r->deltaT, nullptr);
+        RT_HIP(hipEventCreateWithFlags(&r->done, hipEventDisableTiming));
+        r->state = Request::RUNNING;
+    } catch (const Error& e) {
+        fail(S, r, e.code, e.msg);
+    }
+}
+
+void advance(rt_server* S, Request* r) {
+    rt_ctx* c = S->slots[r->slot];
+    try {
+        RT_HIP(hipSetDevice(c->impl.device));
+        if (r->state == Request::RUNNING) {
+            if (modelCorrectAdvance(r->model->impl, false)) {
+                RT_HIP(hipEventRecord(r->done, c->impl.stream));
+                r->state = Request::FINISHING;
+            }
+        }
+        if (r->state == Request::FINISHING) {
+            hipError_t e = hipEventQuery(r->done);
+            if (e == hipErrorNotReady) return;
+            RT_HIP(e);
+            const size_t N = (size_t)r->desc.nCells;
+            r->outTheta.resize(6 * N);
+            r->outTau.resize(6 * N);
+            int rc = rt_model_get(r->model, RT_THETA, r->outTheta.data());
+            if (rc == RT_OK) rc = rt_model_get(r->model, RT_TAU, r->outTau.data());
+            if (rc != RT_OK) {
+                fail(S, r, rc, rt_last_error(c));
+                return;
+            }
+            rt_model_solver_perf(r->model, r->perf);
+            r->rc = RT_OK;
+            for (int q = 0; q < 6; ++q)
+                if (r->perf[q].solved && !r->perf[q].converged) r->rc = RT_W_NOT_CONVERGED;
+            r->state = Request::DONE;
+            r->tDone = std::chrono::steady_clock::now();
+            releaseRequest(S, r);
+        }
+    } catch (const Error& e) {
+        fail(S, r, e.code, e.msg);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int rt_server_create(rt_ctx* ctx, size_t hbm_budget, int max_inflight, rt_server** out) {
+    if (!ctx || !out) return RT_E_BAD_ARG;
+    *out = nullptr;
+    if (max_inflight <= 0 || max_inflight > 64) {
+        ctx->impl.lastError = "rt_server_create: max_inflight must be in 1..64";
+        return RT_E_BAD_ARG;
+    }
+    auto* S = new rt_server;
+    S->parent = ctx;
+    S->budget = hbm_budget;
+    for (int i = 0; i < max_inflight; ++i) {
+        rt_ctx* c = nullptr;
+        int rc = rt_ctx_create(ctx->impl.device, 0, &c);
+        if (rc != RT_OK) {
+            for (auto* x : S->slots) rt_ctx_destroy(x);
+            delete S;
+            ctx->impl.lastError = "rt_server_create: slot context creation failed";
+            return rc;
+        }
+        S->slots.push_back(c);
+        S->slotReq.push_back(nullptr);
+    }
+    *out = S;
+    return RT_OK;
+}
+
+void rt_server_destroy(rt_server* S) {
+    if (!S) return;
+    for (auto& kv : S->all)
+        if (kv.second->slot >= 0) {
+            (void)hipStreamSynchronize(S->slots[kv.second->slot]->impl.stream);
+            releaseRequest(S, kv.second.get());
+        }
+    for (auto* c : S->slots) rt_ctx_destroy(c);
+    delete S;
+}
+
+int rt_server_submit(rt_server* S, const rt_request* q, int64_t* ticket) {
+    if (!S || !q || !ticket) return RT_E_BAD_ARG;
+    rt_ctx* P = S->parent;
+    const rt_mesh_desc* d = q->mesh;
+    if (!d || !q->dict || !q->theta || !q->U || !q->phi || d->nCells <= 0 || !(q->deltaT > 0) ||
+        (d->nBoundaryFaces > 0 && !q->Ub)) {
+        P->impl.lastError = "rt_server_submit: incomplete request";
+        return RT_E_BAD_ARG;
+    }
+    auto r = std::make_unique<Request>();
+    const size_t N = d->nCells, F = d->nInternalFaces, NB = d->nBoundaryFaces, NF = F + NB;
+    r->owner.assign(d->owner, d->owner + NF);
+    r->neighbour.assign(d->neighbour, d->neighbour + F);
+    r->Sf.assign(d->Sf, d->Sf + 3 * NF);
+    r->w.assign(d->weights, d->weights + F);
+    r->V.assign(d->V, d->V + N);
+    r->patches.assign(d->patches, d->patches + d->nPatches);
+    r->desc = *d;
+    r->desc.owner = r->owner.data();
+    r->desc.neighbour = r->neighbour.data();
+    r->desc.Sf = r->Sf.data();
+    r->desc.weights = r->w.data();
+    r->desc.V = r->V.data();
+    r->desc.patches = r->patches.data();
+    r->dict = q->dict;
+    r->controls = q->controls ? q->controls : S->defaultControls;
+    r->theta.assign(q->theta, q->theta + 6 * N);
+    if (q->eigVals) r->eigVals.assign(q->eigVals, q->eigVals + 9 * N);
+    if (q->eigVecs) r->eigVecs.assign(q->eigVecs, q->eigVecs + 9 * N);
+    r->U.assign(q->U, q->U + 3 * N);
+    if (NB) r->Ub.assign(q->Ub, q->Ub + 3 * NB);
+    r->phi.assign(q->phi, q->phi + NF);
+    r->deltaT = q->deltaT;
+    // nEll upper bound: 2F entries plus padding of one slice width per slice
+    const bool mixed = r->controls.find("mixed") != std::string::npos;
+    r->estBytes = modelBytesEstimate((int32_t)N, (int32_t)F, (int32_t)NB, (int64_t)(2 * F + 16 * N), mixed);
+    if (S->budget && r->estBytes > S->budget) {
+        P->impl.lastError = "rt_server_submit: request needs " + std::to_string(r->estBytes) +
+                            " bytes, more than the whole HBM budget";
+        return RT_E_OOM;
+    }
+    r->ticket = S->nextTicket++;
+    r->tSubmit = std::chrono::steady_clock::now();
+    *ticket = r->ticket;
+    S->queue.push_back(r.get());
+    S->all[r->ticket] = std::move(r);
+    return RT_OK;
+}
+
+int rt_server_progress(rt_server* S) {
+    if (!S) return -RT_E_BAD_ARG;
+    // admission: first fit over the queue while a slot and memory are free
+    for (auto it = S->queue.begin(); it != S->queue.end();) {
+        int slot = -1;
+        for (size_t i = 0; i < S->slots.size(); ++i)
+            if (!S->slotReq[i]) {
+                slot = (int)i;
+                break;
+            }
+        if (slot < 0) break;
+        Request* r = *it;
+        if (S->budget && S->inflightBytes + r->estBytes > S->budget) {
+            ++it;
+            continue;
+        }
+        it = S->queue.erase(it);
+        admit(S, r, slot);
+    }
+    int live = (int)S->queue.size();
+    for (size_t i = 0; i < S->slots.size(); ++i) {
+        Request* r = S->slotReq[i];
+        if (!r) continue;
+        advance(S, r);
+        if (S->slotReq[i]) ++live;
+    }
+    return live;
+}
+
+int rt_server_poll(rt_server* S, int64_t ticket, double* theta, double* tau, rt_solver_perf perf[6],
+                   double* latency_ms) {
+    if (!S) return -RT_E_BAD_ARG;
+    auto it = S->all.find(ticket);
+    if (it == S->all.end()) {
+        S->parent->impl.lastError = "rt_server_poll: unknown ticket";
+        return -RT_E_BAD_ARG;
+    }
+    Request* r = it->second.get();
+    if (r->state == Request::FAILED) {
+        S->parent->impl.lastError = r->err;
+        return -r->rc;
+    }
+    if (r->state != Request::DONE) return 0;
+    const size_t n = r->outTheta.size();
+    if (theta) std::copy(r->outTheta.begin(), r->outTheta.end(), theta);
+    if (tau) std::copy(r->outTau.begin(), r->outTau.begin() + n, tau);
+    if (perf)
+        for (int q = 0; q < 6; ++q) perf[q] = r->perf[q];
+    if (latency_ms)
+        *latency_ms = std::chrono::duration<double, std::milli>(r->tDone - r->tSubmit).count();
+    return 1;
+}
+
+}  // extern "C"

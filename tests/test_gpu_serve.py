@@ -1,0 +1,99 @@
+"""The continuous-batching server: every request's result is the same alone
+and in a mix (bit-exact: a request's kernels depend only on its own data),
+requests arriving while others run are admitted as slots free up, the HBM
+budget gates admission, and results agree with the FP64 oracle."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from rheotool_amd import abi, cases
+
+pytestmark = pytest.mark.gpu
+
+
+def make_request(i):
+    """Mixed sizes, 2D/3D, three models, Wi log-uniform on [0.1, 20]."""
+    rng = np.random.default_rng(100 + i)
+    twoD = bool(rng.integers(2))
+    ncell = int(np.exp(rng.uniform(np.log(1e3), np.log(3e4))))
+    if twoD:
+        n = max(8, int(np.sqrt(ncell)))
+        m = cases.cartesian(n, n, 1, twoD=True, channel=True)
+    else:
+        n = max(6, int(round(ncell ** (1 / 3))))
+        m = cases.cartesian(n, n, n, channel=True)
+    f = cases.make_fields(m, cases.FIELDS_CHANNEL, seed=1000 + i)
+    model = ["Oldroyd-BLog", "GiesekusLog", "PTTLog"][i % 3]
+    lam = float(np.exp(rng.uniform(np.log(0.1), np.log(20.0))))
+    kw = {"GiesekusLog": dict(alpha=0.2), "PTTLog": dict(epsilon=0.25, zeta=0.0)}.get(model, {})
+    dt = 0.1 / n
+    return m, f, model, lam, kw, dt
+
+
+def run_alone(ctx, req, controls):
+    m, f, model, lam, kw, dt = req
+    dm = ctx.mesh(m)
+    eq = ctx.constitutiveEq("", dm, abi.model_dict(model, lam=lam, **kw))
+    eq.set_solver(controls)
+    eq.set_state(f.theta, f.eigVals, f.eigVecs)
+    eq.set_flow(f.U, f.Ub, f.phi)
+    eq.correct(dt)
+    return eq.theta(), eq.tau()
+
+
+@pytest.mark.parametrize("precision", ["mixed", "double"])
+def test_same_result_alone_and_in_mix(precision):
+    controls = abi.solver_dict(1e-10, precision=precision)
+    reqs = [make_request(i) for i in range(10)]
+    with abi.Context(0) as ctx:
+        alone = [run_alone(ctx, r, controls) for r in reqs]
+        srv = abi.Server(ctx, hbm_budget=0, max_inflight=4)
+        tickets = {}
+        # staggered arrivals: submit some, progress, submit more while running
+        for i, r in enumerate(reqs):
+            m, f, model, lam, kw, dt = r
+            tickets[i] = srv.submit(m, f, abi.model_dict(model, lam=lam, **kw), dt, controls)
+            if i % 3 == 2:
+                for _ in range(3):
+                    srv.progress()
+        spins = 0
+        while srv.progress() > 0:
+            spins += 1
+            assert spins < 2_000_000
+        for i, r in enumerate(reqs):
+            out = srv.poll(tickets[i])
+            assert out is not None
+            assert np.array_equal(out["theta"], alone[i][0]), i
+            assert np.array_equal(out["tau"], alone[i][1]), i
+            assert out["latency_ms"] > 0
+        srv.close()
+
+
+def test_server_results_match_oracle():
+    with abi.Context(0) as ctx:
+        srv = abi.Server(ctx, max_inflight=3)
+        reqs = [make_request(i) for i in range(4)]
+        t = [srv.submit(r[0], r[1], abi.model_dict(r[2], lam=r[3], **r[4]), r[5]) for r in reqs]
+        while srv.progress() > 0:
+            pass
+        for k, r in zip(t, reqs):
+            m, f, model, lam, kw, dt = r
+            out = srv.poll(k)
+            p = O.params(model, etaS=0.59, etaP=0.41, lam=lam, alpha=kw.get("alpha", 0.0),
+                         epsilon=kw.get("epsilon", 0.0), zeta=kw.get("zeta", 0.0))
+            th, _, _, tau, _ = O.correct(m, p, f, dt)
+            v = m.validComponents > 0
+            assert np.linalg.norm(out["theta"][:, v] - th[:, v]) / np.linalg.norm(th[:, v]) < 1e-9
+            assert np.linalg.norm(out["tau"] - tau) / np.linalg.norm(tau) < 1e-9
+        srv.close()
+
+
+def test_budget_gates_admission():
+    with abi.Context(0) as ctx:
+        reqs = [make_request(i) for i in range(3)]
+        srv = abi.Server(ctx, hbm_budget=1 << 20, max_inflight=2)  # 1 MiB: nothing fits
+        with pytest.raises(abi.RheoError) as e:
+            srv.submit(reqs[0][0], reqs[0][1], abi.model_dict(reqs[0][2], lam=reqs[0][3], **reqs[0][4]),
+                       reqs[0][5])
+        assert e.value.code == abi.RT_E_OOM
+        srv.close()
