@@ -97,14 +97,37 @@ def build_gpu(jobs: int = 8) -> str:
     return out
 
 
+def build_host() -> str:
+    """C++ host mirror of the reference interfaces (librheo_host.so) and the
+    C++ host-API test driver (tests/cpp/test_host_api)."""
+    out = os.path.join(LIB, "librheo_host.so")
+    inc = ["-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "host")]
+    rpath = "-Wl,-rpath," + LIB + ":/opt/rocm/lib"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", *inc, "-o", out + ".tmp",
+          os.path.join(HERE, "host", "rheoGpu.C"), "-L", LIB, "-lrheo_gpu", rpath])
+    os.replace(out + ".tmp", out)
+    test_src = os.path.join(ROOT, "tests", "cpp", "test_host_api.cpp")
+    if os.path.exists(test_src):
+        exe = os.path.join(ROOT, "tests", "cpp", "test_host_api")
+        odir = os.path.join(ROOT, "oracle")
+        _run(["g++", "-O2", "-std=c++17", *inc, "-I", odir, "-I", os.path.join(CSRC, "case"),
+              "-o", exe + ".tmp", test_src, "-L", LIB, "-lrheo_host", "-lrheo_gpu", "-lrheo_case",
+              "-L", odir, "-loracle", rpath + ":" + odir])
+        os.replace(exe + ".tmp", exe)
+    return out
+
+
 def build_all() -> None:
     build_case()
     build_gpu()
+    build_host()
 
 
 if __name__ == "__main__":
-    what = sys.argv[1:] or ["case", "gpu"]
+    what = sys.argv[1:] or ["case", "gpu", "host"]
     if "case" in what:
         build_case()
     if "gpu" in what:
         build_gpu()
+    if "host" in what:
+        build_host()

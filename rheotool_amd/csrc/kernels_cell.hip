@@ -233,24 +233,34 @@ __device__ void cell_rhs(const ModelParams& p, int invMask, const double* L, con
     symmT(R, S);
 }
 
+__device__ __forceinline__ void load6(const double* p, double* v) {
+    const double2* q = reinterpret_cast<const double2*>(p);
+    const double2 a = q[0], b = q[1], c = q[2];
+    v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y; v[4] = c.x; v[5] = c.y;
+}
+__device__ __forceinline__ void store6(double* p, const double* v) {
+    double2* q = reinterpret_cast<double2*>(p);
+    q[0] = make_double2(v[0], v[1]);
+    q[1] = make_double2(v[2], v[3]);
+    q[2] = make_double2(v[4], v[5]);
+}
+
 __global__ void __launch_bounds__(BLK) k_assemble(AssembleArgs a) {
     int32_t r0, r1;
     block_rows(a.N, r0, r1);
     double thsum[6] = {0, 0, 0, 0, 0, 0};
-    const int64_t NF = a.nFaces;
+    const int64_t nE = a.nEll;
     for (int32_t r = r0 + threadIdx.x; r < r1; r += BLK) {
         const int32_t N = a.N;
         double th[6], lam[3], V[9];
-#pragma unroll
-        for (int q = 0; q < 6; ++q) th[q] = a.theta[(int64_t)q * N + r];
+        load6(a.theta + (int64_t)NCMP * r, th);
 #pragma unroll
         for (int q = 0; q < 3; ++q) lam[q] = a.eigVal[(int64_t)q * N + r];
 #pragma unroll
         for (int q = 0; q < 9; ++q) V[q] = a.eigVec[(int64_t)q * N + r];
         const double vol = a.V[r];
-        double Ui[3];
-#pragma unroll
-        for (int q = 0; q < 3; ++q) Ui[q] = a.U[(int64_t)q * N + r];
+        const double2 u01 = reinterpret_cast<const double2*>(a.U)[2 * (int64_t)r];
+        const double Ui[3] = {u01.x, u01.y, a.U[4 * (int64_t)r + 2]};
         const int s = r >> 6, lane = r & 63;
         const int64_t base = a.ellOff[s] + lane;
         const int width = a.ellWidth[s];
@@ -258,50 +268,48 @@ __global__ void __launch_bounds__(BLK) k_assemble(AssembleArgs a) {
         double divDiag = 0.0, sumOff = 0.0;
         const bool haveGrad = a.gradU != nullptr;
         for (int k = 0; k < width; ++k) {
-            const int64_t e = base + (int64_t)SLICE * k;
+            const int64_t e = base + (int64_t)SLICE * k;  // face-order entry
             const int32_t fid = a.ellFid[e];
             if (fid == RT_FID_PAD) {
-                a.aval[e] = 0.0;
+                a.aval[base + (int64_t)SLICE * a.ellSlot[e]] = 0.0;
                 continue;
             }
             const bool own = fid >= 0;
-            const int32_t f = own ? fid : -fid - 1;
-            const double ph = a.phi[f];
+            const int64_t ec = base + (int64_t)SLICE * a.ellSlot[e];  // column-order slot
+            const double ph = a.ePhi[e];
             const double upw = ph >= 0 ? 1.0 : 0.0;
             const double lower = -upw * ph;
             const double upper = (1.0 - upw) * ph;
             const double val = own ? upper : lower;
-            a.aval[e] = val;
+            a.aval[ec] = val;
             divDiag -= own ? lower : upper;
             sumOff += fabs(val);
             if (!haveGrad) {
-                const int32_t c = a.ellCol[e];
-                const double wf = a.w[f];
+                const int32_t c = a.ellCol[ec];
+                const double wf = a.eW[e];
+                const double2 c01 = reinterpret_cast<const double2*>(a.U)[2 * (int64_t)c];
+                const double Uc[3] = {c01.x, c01.y, a.U[4 * (int64_t)c + 2]};
                 double uf[3];
 #pragma unroll
                 for (int q = 0; q < 3; ++q) {
-                    const double uc = a.U[(int64_t)q * N + c];
-                    const double uo = own ? Ui[q] : uc, un = own ? uc : Ui[q];
+                    const double uo = own ? Ui[q] : Uc[q], un = own ? Uc[q] : Ui[q];
                     uf[q] = wf * (uo - un) + un;
                 }
-                const double sf[3] = {a.Sf[f], a.Sf[NF + f], a.Sf[2 * NF + f]};
+                // Sf already carries the row's side: g += (+-Sf) (x) U_f
+                const double sf[3] = {a.eSf[e], a.eSf[nE + e], a.eSf[2 * nE + e]};
 #pragma unroll
                 for (int i = 0; i < 3; ++i)
 #pragma unroll
-                    for (int j = 0; j < 3; ++j) {
-                        const double v = sf[i] * uf[j];
-                        if (own) g[3 * i + j] += v;
-                        else g[3 * i + j] -= v;
-                    }
+                    for (int j = 0; j < 3; ++j) g[3 * i + j] += sf[i] * uf[j];
             }
         }
         const int32_t b0 = a.bPtr[r], b1 = a.bPtr[r + 1];
+        const int64_t NB = a.NB > 0 ? a.NB : 1;
         if (!haveGrad) {
             for (int32_t q = b0; q < b1; ++q) {
                 const int32_t bf = a.bFace[q];
-                const int64_t f = (int64_t)(NF - a.NB) + bf;
-                const double sf[3] = {a.Sf[f], a.Sf[NF + f], a.Sf[2 * NF + f]};
-                const double ub[3] = {a.Ub[bf], a.Ub[(int64_t)a.NB + bf], a.Ub[2 * (int64_t)a.NB + bf]};
+                const double sf[3] = {a.bSf[bf], a.bSf[NB + bf], a.bSf[2 * NB + bf]};
+                const double ub[3] = {a.Ub[bf], a.Ub[NB + bf], a.Ub[2 * NB + bf]};
 #pragma unroll
                 for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -327,7 +335,7 @@ __global__ void __launch_bounds__(BLK) k_assemble(AssembleArgs a) {
                 const int32_t bf = a.bFace[q];
                 const int t = a.bcType[a.bPatch[bf]];
                 if (t == RT_BC_EMPTY) continue;
-                const double ph = a.phi[(NF - a.NB) + bf];
+                const double ph = a.phiB[bf];
                 D += t == RT_BC_ZEROGRADIENT ? fabs(ph * 1.0) : fabs(ph * 0.0);
             }
             D = fmax(fabs(D), sumOff);
@@ -336,7 +344,7 @@ __global__ void __launch_bounds__(BLK) k_assemble(AssembleArgs a) {
                 const int32_t bf = a.bFace[q];
                 const int t = a.bcType[a.bPatch[bf]];
                 if (t == RT_BC_EMPTY) continue;
-                const double ph = a.phi[(NF - a.NB) + bf];
+                const double ph = a.phiB[bf];
                 D -= t == RT_BC_ZEROGRADIENT ? ph * 1.0 : ph * 0.0;
             }
 #pragma unroll
@@ -346,21 +354,22 @@ __global__ void __launch_bounds__(BLK) k_assemble(AssembleArgs a) {
             const int32_t bf = a.bFace[q];
             const int t = a.bcType[a.bPatch[bf]];
             if (t == RT_BC_EMPTY) continue;
-            const double ph = a.phi[(NF - a.NB) + bf];
+            const double ph = a.phiB[bf];
             if (t == RT_BC_ZEROGRADIENT) {
                 D += ph * 1.0;
             } else {
                 D += ph * 0.0;
 #pragma unroll
-                for (int c = 0; c < 6; ++c) src[c] += -ph * a.bcVal[(int64_t)c * a.NB + bf];
+                for (int c = 0; c < 6; ++c) src[c] += -ph * a.bcVal[(int64_t)c * NB + bf];
             }
         }
         a.diag[r] = D;
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
-            if (a.valid[q] > 0) a.b[(int64_t)q * N + r] = src[q];
+            if (a.valid[q] <= 0) src[q] = 0.0;
             thsum[q] += th[q];
         }
+        store6(a.b + (int64_t)NCMP * r, src);
     }
     double sums[6];
     if (block_reduce_last<6>(thsum, a.partials, a.counter, sums) && threadIdx.x == 0) {
@@ -434,8 +443,7 @@ __global__ void __launch_bounds__(BLK) k_eig_tau(int32_t N, ModelParams p, const
     const int64_t r = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (r >= N) return;
     double th[6], ev[3], V[9];
-#pragma unroll
-    for (int q = 0; q < 6; ++q) th[q] = theta[q * (int64_t)N + r];
+    load6(theta + (int64_t)NCMP * r, th);
     jacobi3(th, ev, V);
     double lam[3];
 #pragma unroll
@@ -465,7 +473,8 @@ __global__ void __launch_bounds__(BLK) k_eig_tau(int32_t N, ModelParams p, const
     double s[6];
     symmT(D, s);
 #pragma unroll
-    for (int q = 0; q < 6; ++q) tau[q * (int64_t)N + r] = coef * s[q];
+    for (int q = 0; q < 6; ++q) s[q] = coef * s[q];
+    store6(tau + (int64_t)NCMP * r, s);
 }
 
 // ------------------------------------------------------------- layout helpers
@@ -508,21 +517,60 @@ __global__ void k_fill(double* p, int64_t n, double v) {
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (i < n) p[i] = v;
 }
-__global__ void k_ldu_to_ell(int64_t nEll, const int32_t* __restrict__ fid,
+// face-order entry e of row r sits at ellOff[s] + 64k + lane; its value goes to
+// the column-order slot ellSlot[e] of the same row
+__device__ __forceinline__ int64_t col_slot(int64_t e, const int64_t* __restrict__ off,
+                                            const uint8_t* __restrict__ slot, int32_t nSlices) {
+    // locate the slice of e by binary search over the slice offsets
+    int32_t lo = 0, hi = nSlices - 1;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= e) lo = mid;
+        else hi = mid - 1;
+    }
+    const int64_t within = e - off[lo];
+    const int64_t lane = within & 63;
+    return off[lo] + 64 * (int64_t)slot[e] + lane;
+}
+__global__ void k_ldu_to_ell(int64_t nEll, int32_t nSlices, const int64_t* __restrict__ off,
+                             const int32_t* __restrict__ fid, const uint8_t* __restrict__ slot,
                              const double* __restrict__ lower, const double* __restrict__ upper,
                              double* __restrict__ aval) {
     const int64_t e = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (e >= nEll) return;
     const int32_t f = fid[e];
-    aval[e] = f == RT_FID_PAD ? 0.0 : (f >= 0 ? upper[f] : lower[-f - 1]);
+    aval[col_slot(e, off, slot, nSlices)] = f == RT_FID_PAD ? 0.0 : (f >= 0 ? upper[f] : lower[-f - 1]);
+}
+__global__ void k_face_to_entry(int64_t nEll, const int32_t* __restrict__ fid,
+                                const double* __restrict__ phi, double* __restrict__ ePhi) {
+    const int64_t e = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (e >= nEll) return;
+    const int32_t f = fid[e];
+    ePhi[e] = f == RT_FID_PAD ? 0.0 : phi[f >= 0 ? f : -f - 1];
+}
+__global__ void k_permute_rows(int32_t N, int nc, int ncOut, const double* __restrict__ in,
+                               const int32_t* __restrict__ perm, double* __restrict__ out) {
+    const int64_t r = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (r >= N) return;
+    const int64_t c = perm ? perm[r] : r;
+    for (int q = 0; q < ncOut; ++q) out[r * ncOut + q] = q < nc ? in[c * nc + q] : 0.0;
+}
+__global__ void k_unpermute_rows(int32_t N, int ncIn, int nc, const double* __restrict__ in,
+                                 const int32_t* __restrict__ perm, double* __restrict__ out) {
+    const int64_t r = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (r >= N) return;
+    const int64_t c = perm ? perm[r] : r;
+    for (int q = 0; q < nc; ++q) out[c * nc + q] = in[r * ncIn + q];
 }
 
 // ---------------------------------------------------------------- launchers
 static inline unsigned nblk(int64_t n) { return (unsigned)((n + BLK - 1) / BLK); }
 
 void launch_assemble(Ctx* ctx, const AssembleArgs& a, int grid) {
-    // algorithmic bytes: theta 48, eig 96, U 24 (+gathers), V 8, writes b 48, diag 8
-    const double by = (double)a.N * (48 + 96 + 24 + 8 + 48 + 8);
+    // algorithmic bytes per cell: theta 48, eigen state 96, U 32, V 8, b 48, diag 8;
+    // per ELL entry: face id 4, slot 1, Sf 24, w 8, phi 8, value 8 (+ column 4 for the gradient)
+    const double by = (double)a.N * (48 + 96 + 32 + 8 + 48 + 8) +
+                      (double)a.nEll * (4 + 1 + 8 + 8 + (a.gradU ? 0 : 24 + 8 + 4));
     ProfScope ps(ctx, "k_assemble", by);
     hipLaunchKernelGGL(k_assemble, dim3(grid), dim3(BLK), 0, ctx->stream, a);
     RT_HIP(hipGetLastError());
@@ -557,8 +605,26 @@ void launch_fill(Ctx* ctx, double* p, int64_t n, double v) {
 }
 void launch_ldu_to_ell(Ctx* ctx, const Mesh& m, const double* lower, const double* upper, double* aval) {
     if (m.nEll <= 0) return;
-    hipLaunchKernelGGL(k_ldu_to_ell, dim3(nblk(m.nEll)), dim3(BLK), 0, ctx->stream, m.nEll,
-                       m.ellFid.as<int32_t>(), lower, upper, aval);
+    hipLaunchKernelGGL(k_ldu_to_ell, dim3(nblk(m.nEll)), dim3(BLK), 0, ctx->stream, m.nEll, m.nSlices,
+                       m.ellOff.as<int64_t>(), m.ellFid.as<int32_t>(), m.ellSlot.as<uint8_t>(), lower,
+                       upper, aval);
+    RT_HIP(hipGetLastError());
+}
+void launch_face_to_entry(Ctx* ctx, const Mesh& m, const double* phi, double* ePhi) {
+    if (m.nEll <= 0) return;
+    hipLaunchKernelGGL(k_face_to_entry, dim3(nblk(m.nEll)), dim3(BLK), 0, ctx->stream, m.nEll,
+                       m.ellFid.as<int32_t>(), phi, ePhi);
+    RT_HIP(hipGetLastError());
+}
+void launch_permute_rows(Ctx* ctx, int32_t N, int nc, int ncOut, const double* in, const int32_t* perm,
+                         double* out) {
+    hipLaunchKernelGGL(k_permute_rows, dim3(nblk(N)), dim3(BLK), 0, ctx->stream, N, nc, ncOut, in, perm, out);
+    RT_HIP(hipGetLastError());
+}
+void launch_unpermute_rows(Ctx* ctx, int32_t N, int ncIn, int nc, const double* in, const int32_t* perm,
+                           double* out) {
+    hipLaunchKernelGGL(k_unpermute_rows, dim3(nblk(N)), dim3(BLK), 0, ctx->stream, N, ncIn, nc, in, perm,
+                       out);
     RT_HIP(hipGetLastError());
 }
 

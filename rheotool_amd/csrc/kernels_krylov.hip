@@ -139,6 +139,27 @@ __device__ void start_inner(CompState& c, const KryCtl& k, double rr) {
     c.active = 1;
 }
 
+// interleaved [N][6] rows: three 2-wide vector loads/stores per row
+template <class T> struct Vec2;
+template <> struct Vec2<float> { typedef float2 type; };
+template <> struct Vec2<double> { typedef double2 type; };
+
+template <class T>
+__device__ __forceinline__ void ld6(const T* p, T* v) {
+    typedef typename Vec2<T>::type V2;
+    const V2* q = reinterpret_cast<const V2*>(p);
+    const V2 a = q[0], b = q[1], c = q[2];
+    v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y; v[4] = c.x; v[5] = c.y;
+}
+template <class T>
+__device__ __forceinline__ void st6(T* p, const T* v) {
+    typedef typename Vec2<T>::type V2;
+    V2* q = reinterpret_cast<V2*>(p);
+    V2 a, b, c;
+    a.x = v[0]; a.y = v[1]; b.x = v[2]; b.y = v[3]; c.x = v[4]; c.y = v[5];
+    q[0] = a; q[1] = b; q[2] = c;
+}
+
 // ---- true residual r = b - A x (FP64), init or refinement -----------------
 // mode 0: initial (sets normFactor, initRes); mode 1: refinement sweep.
 template <class T>
@@ -153,7 +174,6 @@ __global__ void __launch_bounds__(KBLK) k_residual(KryArgs<T> a, int mode) {
     if (!any) return;
     int32_t r0, r1;
     krows(a.N, r0, r1);
-    const int32_t N = a.N;
     double q[18];
 #pragma unroll
     for (int i = 0; i < 18; ++i) q[i] = 0;
@@ -162,9 +182,10 @@ __global__ void __launch_bounds__(KBLK) k_residual(KryArgs<T> a, int mode) {
     for (int c = 0; c < 6; ++c) xbar[c] = a.cs[c].xbar;
     for (int32_t r = r0 + threadIdx.x; r < r1; r += KBLK) {
         const double d = a.diag[r];
-        double y[6];
+        double y[6], xv[6];
+        ld6(a.x + (int64_t)NCMP * r, xv);
 #pragma unroll
-        for (int c = 0; c < 6; ++c) y[c] = on[c] ? d * a.x[(int64_t)c * N + r] : 0.0;
+        for (int c = 0; c < 6; ++c) y[c] = d * xv[c];
         double sumA = d;
         const int s = r >> 6, lane = r & 63;
         const int64_t base = a.ellOff[s] + lane;
@@ -174,23 +195,25 @@ __global__ void __launch_bounds__(KBLK) k_residual(KryArgs<T> a, int mode) {
             const int32_t col = a.ellCol[e];
             const double av = a.aval[e];
             sumA += av;
+            double xc[6];
+            ld6(a.x + (int64_t)NCMP * col, xc);
 #pragma unroll
-            for (int c = 0; c < 6; ++c)
-                if (on[c]) y[c] += av * a.x[(int64_t)c * N + col];
+            for (int c = 0; c < 6; ++c) y[c] += av * xc[c];
         }
+        double bb[6];
+        ld6(a.b + (int64_t)NCMP * r, bb);
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
             if (!on[c]) continue;
-            const double bb = a.b[(int64_t)c * N + r];
-            const double res = bb - y[c];
+            const double res = bb[c] - y[c];
             const T rt = (T)res;
-            a.r[(int64_t)c * N + r] = rt;
-            a.r0[(int64_t)c * N + r] = rt;
+            a.r[(int64_t)NCMP * r + c] = rt;
+            a.r0[(int64_t)NCMP * r + c] = rt;
             q[c] += fabs(res);
             q[6 + c] += (double)rt * (double)rt;
             if (mode == 0) {
                 const double pA = sumA * xbar[c];
-                q[12 + c] += fabs(y[c] - pA) + fabs(bb - pA);
+                q[12 + c] += fabs(y[c] - pA) + fabs(bb[c] - pA);
             }
         }
     }
@@ -249,13 +272,14 @@ __global__ void __launch_bounds__(KBLK) k_kry_p(KryArgs<T> a) {
     if (!any) return;
     const int64_t N = a.N;
     for (int64_t i = (int64_t)blockIdx.x * KBLK + threadIdx.x; i < N; i += (int64_t)gridDim.x * KBLK) {
+        T rr[6], pp[6], vv[6];
+        ld6(a.r + NCMP * i, rr);
+        ld6(a.p + NCMP * i, pp);
+        ld6(a.v + NCMP * i, vv);
 #pragma unroll
-        for (int c = 0; c < 6; ++c) {
-            if (!on[c]) continue;
-            const int64_t j = c * N + i;
-            const T rr = a.r[j];
-            a.p[j] = first[c] ? rr : rr + beta[c] * (a.p[j] - omega[c] * a.v[j]);
-        }
+        for (int c = 0; c < 6; ++c)
+            if (on[c]) pp[c] = first[c] ? rr[c] : rr[c] + beta[c] * (pp[c] - omega[c] * vv[c]);
+        st6(a.p + NCMP * i, pp);
     }
 }
 
@@ -272,12 +296,10 @@ __global__ void __launch_bounds__(KBLK) k_kry_v(KryArgs<T> a) {
     if (!any) return;
     int32_t r0, r1;
     krows(a.N, r0, r1);
-    const int64_t N = a.N;
     double q[6] = {0, 0, 0, 0, 0, 0};
     for (int32_t r = r0 + threadIdx.x; r < r1; r += KBLK) {
         T acc[6];
-#pragma unroll
-        for (int c = 0; c < 6; ++c) acc[c] = on[c] ? a.p[c * N + r] : (T)0;
+        ld6(a.p + (int64_t)NCMP * r, acc);
         const int s = r >> 6, lane = r & 63;
         const int64_t base = a.ellOff[s] + lane;
         const int width = a.ellWidth[s];
@@ -285,16 +307,17 @@ __global__ void __launch_bounds__(KBLK) k_kry_v(KryArgs<T> a) {
             const int64_t e = base + (int64_t)SLICE * k;
             const int32_t col = a.ellCol[e];
             const T av = a.ahat[e];
+            T pc[6];
+            ld6(a.p + (int64_t)NCMP * col, pc);
 #pragma unroll
-            for (int c = 0; c < 6; ++c)
-                if (on[c]) acc[c] += av * a.p[c * N + col];
+            for (int c = 0; c < 6; ++c) acc[c] += av * pc[c];
         }
+        st6(a.v + (int64_t)NCMP * r, acc);
+        T rr0[6];
+        ld6(a.r0 + (int64_t)NCMP * r, rr0);
 #pragma unroll
-        for (int c = 0; c < 6; ++c) {
-            if (!on[c]) continue;
-            a.v[c * N + r] = acc[c];
-            q[c] += (double)a.r0[c * N + r] * (double)acc[c];
-        }
+        for (int c = 0; c < 6; ++c)
+            if (on[c]) q[c] += (double)rr0[c] * (double)acc[c];
     }
     double sums[6];
     if (kreduce_last<6>(q, a.partials, a.counters + 1, sums) && threadIdx.x == 0) {
@@ -318,15 +341,16 @@ __global__ void __launch_bounds__(KBLK) k_kry_t(KryArgs<T> a) {
     if (!any) return;
     int32_t r0, r1;
     krows(a.N, r0, r1);
-    const int64_t N = a.N;
     double q[18];
 #pragma unroll
     for (int i = 0; i < 18; ++i) q[i] = 0;
     for (int32_t r = r0 + threadIdx.x; r < r1; r += KBLK) {
-        T sv[6], acc[6];
+        T sv[6], acc[6], rr[6], vv[6];
+        ld6(a.r + (int64_t)NCMP * r, rr);
+        ld6(a.v + (int64_t)NCMP * r, vv);
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
-            sv[c] = on[c] ? a.r[c * N + r] - alpha[c] * a.v[c * N + r] : (T)0;
+            sv[c] = rr[c] - alpha[c] * vv[c];
             acc[c] = sv[c];
         }
         const int s = r >> 6, lane = r & 63;
@@ -336,14 +360,16 @@ __global__ void __launch_bounds__(KBLK) k_kry_t(KryArgs<T> a) {
             const int64_t e = base + (int64_t)SLICE * k;
             const int32_t col = a.ellCol[e];
             const T av = a.ahat[e];
+            T rc[6], vc[6];
+            ld6(a.r + (int64_t)NCMP * col, rc);
+            ld6(a.v + (int64_t)NCMP * col, vc);
 #pragma unroll
-            for (int c = 0; c < 6; ++c)
-                if (on[c]) acc[c] += av * (a.r[c * N + col] - alpha[c] * a.v[c * N + col]);
+            for (int c = 0; c < 6; ++c) acc[c] += av * (rc[c] - alpha[c] * vc[c]);
         }
+        st6(a.t + (int64_t)NCMP * r, acc);
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
             if (!on[c]) continue;
-            a.t[c * N + r] = acc[c];
             q[c] += fabs((double)sv[c]);
             q[6 + c] += (double)acc[c] * (double)sv[c];
             q[12 + c] += (double)acc[c] * (double)acc[c];
@@ -382,29 +408,38 @@ __global__ void __launch_bounds__(KBLK) k_kry_xr(KryArgs<T> a) {
     if (!any) return;
     int32_t r0, r1;
     krows(a.N, r0, r1);
-    const int64_t N = a.N;
     double q[12];
 #pragma unroll
     for (int i = 0; i < 12; ++i) q[i] = 0;
     for (int32_t r = r0 + threadIdx.x; r < r1; r += KBLK) {
         const double rd = a.rD[r];
+        const int64_t j = (int64_t)NCMP * r;
+        double xv[6];
+        T pp[6], rr[6], vv[6], tt[6], rr0[6];
+        ld6(a.x + j, xv);
+        ld6(a.p + j, pp);
+        ld6(a.r + j, rr);
+        ld6(a.v + j, vv);
+        ld6(a.t + j, tt);
+        ld6(a.r0 + j, rr0);
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
             if (!on[c]) continue;
-            const int64_t j = c * N + r;
-            const double y = rd * (double)a.p[j];
+            const double y = rd * (double)pp[c];
             if (ex[c]) {
-                a.x[j] += alpha[c] * y;
+                xv[c] += alpha[c] * y;
                 continue;
             }
-            const T sv = a.r[j] - (T)alpha[c] * a.v[j];
+            const T sv = rr[c] - (T)alpha[c] * vv[c];
             const double z = rd * (double)sv;
-            a.x[j] += alpha[c] * y + omega[c] * z;
-            const T rn = sv - (T)omega[c] * a.t[j];
-            a.r[j] = rn;
+            xv[c] += alpha[c] * y + omega[c] * z;
+            const T rn = sv - (T)omega[c] * tt[c];
+            rr[c] = rn;
             q[c] += fabs((double)rn);
-            q[6 + c] += (double)a.r0[j] * (double)rn;
+            q[6 + c] += (double)rr0[c] * (double)rn;
         }
+        st6(a.x + j, xv);
+        st6(a.r + j, rr);
     }
     double sums[12];
     if (kreduce_last<12>(q, a.partials, a.counters + 3, sums) && threadIdx.x == 0) {
@@ -573,7 +608,8 @@ void fill_args(Ctx* ctx, const SolveArgs& sa, SolveRun& R) {
 }
 
 // algorithmic bytes per launch for nc participating components: ELL entries
-// (col + value) once per SpMV, each vector once per pass
+// (col + value) once per SpMV, each vector once per pass (gathered vectors
+// counted once: neighbour values are reused from cache)
 struct Bytes {
     double N, ne, w;
     double res(double nc) const { return N * 8.0 + ne * 12 + nc * N * (8 + 8 + 2 * w); }

@@ -280,19 +280,22 @@ void modelCorrectBegin(Model& M, double deltaT, const double* gradU) {
     a.ellWidth = m.ellWidth.as<int32_t>();
     a.ellCol = m.ellCol.as<int32_t>();
     a.ellFid = m.ellFid.as<int32_t>();
+    a.ellSlot = m.ellSlot.as<uint8_t>();
+    a.eSf = m.eSf.as<double>();
+    a.eW = m.eW.as<double>();
+    a.ePhi = M.ePhi.as<double>();
+    a.nEll = std::max<int64_t>(1, m.nEll);
     a.bPtr = m.bPtr.as<int32_t>();
     a.bFace = m.bFace.as<int32_t>();
     a.bPatch = m.bPatch.as<int32_t>();
     a.bcType = M.bcType.as<int32_t>();
     a.bcVal = M.bcVal.as<double>();
     a.NB = m.NB;
-    a.nFaces = (int64_t)m.F + m.NB;
-    a.Sf = m.Sf.as<double>();
-    a.w = m.w.as<double>();
+    a.bSf = m.bSf.as<double>();
+    a.phiB = M.phi.as<double>() + m.F;
     a.V = m.V.as<double>();
     a.U = M.U.as<double>();
     a.Ub = M.Ub.as<double>();
-    a.phi = M.phi.as<double>();
     a.gradU = gradU ? M.gradU.as<double>() : nullptr;
     a.theta = M.theta.as<double>();
     a.eigVal = M.eigVal.as<double>();
@@ -346,7 +349,7 @@ size_t modelBytesEstimate(int32_t N, int32_t F, int32_t NB, int64_t nEll, bool m
     const double n = N, f = F, nb = NB, ne = (double)nEll;
     double by = 0;
     // mesh: ELL col+fid, Sf, w, V, perm, boundary lists
-    by += ne * 8 + 3 * 8 * (f + nb) + 8 * f + 8 * n + 4 * n + 4 * (n + 1) + 12 * nb;
+    by += ne * (4 + 4 + 1 + 24 + 8 + 8) + 3 * 8 * nb + 8 * n + 4 * n + 4 * (n + 1) + 12 * nb;
     // model state + flow + matrix: theta tau eigVal eigVec U b diag fOld, Ub, phi, bc, aval
     by += 8 * n * (6 + 6 + 3 + 9 + 3 + 6 + 1 + 1) + 8 * (3 * nb + f + nb) + 6 * 8 * nb + 8 * ne;
     // Krylov: 5 vectors x 6 components, rD, ahat, partials, state
@@ -520,9 +523,10 @@ int rt_model_create(rt_ctx* ctx, const char* name, const rt_mesh* mesh, const ch
         M.tau = c->alloc((size_t)6 * N * 8);
         M.eigVal = c->alloc((size_t)3 * N * 8);
         M.eigVec = c->alloc((size_t)9 * N * 8);
-        M.U = c->alloc((size_t)3 * N * 8);
+        M.U = c->alloc((size_t)4 * N * 8);
         M.Ub = c->alloc((size_t)3 * std::max(1, NB) * 8);
         M.phi = c->alloc(((size_t)M.mesh->F + NB) * 8);
+        M.ePhi = c->alloc((size_t)std::max<int64_t>(1, M.mesh->nEll) * 8);
         M.bcType = c->alloc((size_t)std::max(1, M.mesh->nPatches) * 4);
         M.bcVal = c->alloc((size_t)6 * std::max(1, NB) * 8);
         RT_HIP(hipMemsetAsync(M.bcVal.p, 0, M.bcVal.bytes, c->stream));
@@ -569,7 +573,7 @@ int rt_model_set_state(rt_model* model, const double* theta, const double* eigVa
         DBuf& st = staging(c).buf;
         const int32_t* perm = m.permD.as<int32_t>();
         uploadAos(c, st, theta, (int64_t)6 * N);
-        launch_aos_to_soa(c, N, 6, st.as<double>(), perm, M.theta.as<double>());
+        launch_permute_rows(c, N, 6, 6, st.as<double>(), perm, M.theta.as<double>());
         if (eigVals) {
             RT_HIP(hipStreamSynchronize(c->stream));
             uploadAos(c, st, eigVals, (int64_t)9 * N);
@@ -583,7 +587,7 @@ int rt_model_set_state(rt_model* model, const double* theta, const double* eigVa
         if (tau) {
             RT_HIP(hipStreamSynchronize(c->stream));
             uploadAos(c, st, tau, (int64_t)6 * N);
-            launch_aos_to_soa(c, N, 6, st.as<double>(), perm, M.tau.as<double>());
+            launch_permute_rows(c, N, 6, 6, st.as<double>(), perm, M.tau.as<double>());
         }
         RT_HIP(hipStreamSynchronize(c->stream));
         M.hasState = true;
@@ -627,7 +631,7 @@ int rt_model_set_flow(rt_model* model, const double* U, const double* Ub, const 
         RT_HIP(hipSetDevice(c->device));
         DBuf& st = staging(c).buf;
         uploadAos(c, st, U, (int64_t)3 * m.N);
-        launch_aos_to_soa(c, m.N, 3, st.as<double>(), m.permD.as<int32_t>(), M.U.as<double>());
+        launch_permute_rows(c, m.N, 3, 4, st.as<double>(), m.permD.as<int32_t>(), M.U.as<double>());
         if (m.NB > 0) {
             std::vector<double> ub((size_t)3 * m.NB);
             for (int32_t b = 0; b < m.NB; ++b)
@@ -636,6 +640,7 @@ int rt_model_set_flow(rt_model* model, const double* U, const double* Ub, const 
             RT_HIP(hipStreamSynchronize(c->stream));
         }
         RT_HIP(hipMemcpyAsync(M.phi.p, phi, ((size_t)m.F + m.NB) * 8, hipMemcpyHostToDevice, c->stream));
+        launch_face_to_entry(c, m, M.phi.as<double>(), M.ePhi.as<double>());
         RT_HIP(hipStreamSynchronize(c->stream));
         M.hasFlow = true;
     });
@@ -680,8 +685,8 @@ int rt_model_get(const rt_model* model, int field, double* out) {
         const size_t bytes = (size_t)nc * N * 8;
         if (st.bytes < bytes) st = c->alloc(bytes);
         const int32_t* perm = m.permD.as<int32_t>();
-        if (field == RT_THETA) launch_soa_to_aos(c, N, 6, M.theta.as<double>(), perm, st.as<double>());
-        else if (field == RT_TAU) launch_soa_to_aos(c, N, 6, M.tau.as<double>(), perm, st.as<double>());
+        if (field == RT_THETA) launch_unpermute_rows(c, N, 6, 6, M.theta.as<double>(), perm, st.as<double>());
+        else if (field == RT_TAU) launch_unpermute_rows(c, N, 6, 6, M.tau.as<double>(), perm, st.as<double>());
         else if (field == RT_EIGVALS) launch_eigvals_to_tensor(c, N, M.eigVal.as<double>(), perm, st.as<double>());
         else launch_soa_to_aos(c, N, 9, M.eigVec.as<double>(), perm, st.as<double>());
         RT_HIP(hipMemcpyAsync(out, st.p, bytes, hipMemcpyDeviceToHost, c->stream));
@@ -744,10 +749,10 @@ int rt_symm_solve(rt_ctx* ctx, const rt_mesh* mesh, const rt_ldu_symm* sys, doub
         launch_ldu_to_ell(c, m, dLo.as<double>(), dUp.as<double>(), dA.as<double>());
         DBuf& st = staging(c).buf;
         uploadAos(c, st, src.data(), (int64_t)6 * N);
-        launch_aos_to_soa(c, N, 6, st.as<double>(), m.permD.as<int32_t>(), dB.as<double>());
+        launch_permute_rows(c, N, 6, 6, st.as<double>(), m.permD.as<int32_t>(), dB.as<double>());
         RT_HIP(hipStreamSynchronize(c->stream));
         uploadAos(c, st, psi, (int64_t)6 * N);
-        launch_aos_to_soa(c, N, 6, st.as<double>(), m.permD.as<int32_t>(), dX.as<double>());
+        launch_permute_rows(c, N, 6, 6, st.as<double>(), m.permD.as<int32_t>(), dX.as<double>());
         // gAverage(psi) per component for the normalisation factor
         SolveState* ss = K.state.as<SolveState>();
         for (int q = 0; q < 6; ++q) {
@@ -768,7 +773,7 @@ int rt_symm_solve(rt_ctx* ctx, const rt_mesh* mesh, const rt_ldu_symm* sys, doub
         sa.ctl = &ctl;
         sa.xbarKnown = true;
         solve_lockstep(c, sa, perf);
-        launch_soa_to_aos(c, N, 6, dX.as<double>(), m.permD.as<int32_t>(), st.as<double>());
+        launch_unpermute_rows(c, N, 6, 6, dX.as<double>(), m.permD.as<int32_t>(), st.as<double>());
         RT_HIP(hipMemcpyAsync(psi, st.p, (size_t)6 * N * 8, hipMemcpyDeviceToHost, c->stream));
         RT_HIP(hipStreamSynchronize(c->stream));
     });

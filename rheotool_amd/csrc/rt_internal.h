@@ -101,13 +101,19 @@ struct ProfScope {
 
 // ------------------------------------------------------------------- mesh
 // Device layout.  Rows are cells in the device ("internal") numbering.
-// Sliced ELL with 64-row slices: entry k of row r = 64*s + lane lives at
-// ellOff[s] + 64*k + lane; within a row entries are in increasing face id,
-// padding (col = row, fid = RT_FID_PAD) at the end.
+// Sliced ELL with 64-row slices: entry slot k of row r = 64*s + lane lives at
+// ellOff[s] + 64*k + lane.  Two orders of the same entries share the slots:
+//  * column order (ellCol, matrix values): slot k holds the k-th smallest
+//    column, so slot k is the same kind of neighbour across the lanes of a
+//    wavefront (coalesced gathers in the Krylov kernels);
+//  * face order (ellFid, ellSlot, eSf, eW, ePhi): entry k is the row's k-th
+//    face by increasing face id -- the summation order of the reference's
+//    face loops -- and ellSlot maps it to its column-order slot.
 // fid >= 0: the row is the owner of face fid; fid < 0: the row is the
-// neighbour of face (-fid-1).
+// neighbour of face (-fid-1); RT_FID_PAD marks padding (column = row).
 constexpr int32_t RT_FID_PAD = INT32_MIN;
 constexpr int SLICE = 64;
+constexpr int NCMP = 6;  // symmTensor components, interleaved per row
 
 struct Mesh {
     Ctx* ctx = nullptr;
@@ -121,18 +127,19 @@ struct Mesh {
     std::vector<int32_t> perm;   // external cell -> internal row
     std::vector<int32_t> iperm;  // internal row -> external cell
     std::vector<rt_patch> patches;
-    // host copies needed by the model / solve surface
     std::vector<int32_t> hOwner, hNeighbour;  // external numbering
     // device arrays
     DBuf ellOff;    // int64 [nSlices+1]
     DBuf ellWidth;  // int32 [nSlices]
-    DBuf ellCol;    // int32 [nEll]
-    DBuf ellFid;    // int32 [nEll]
+    DBuf ellCol;    // int32 [nEll]  column order
+    DBuf ellFid;    // int32 [nEll]  face order
+    DBuf ellSlot;   // uint8 [nEll]  face-order entry -> column-order slot
+    DBuf eSf;       // double [3][nEll] face order, Sf signed for the row's side
+    DBuf eW;        // double [nEll] face order, linear weight of the face
     DBuf bPtr;      // int32 [N+1]   boundary faces of each row (non-empty patches)
-    DBuf bFace;     // int32 [...]  boundary face index b (0..NB-1), ascending
+    DBuf bFace;     // int32 [...]   boundary face index b (0..NB-1), ascending
     DBuf bPatch;    // int32 [NB]    patch of every boundary face
-    DBuf Sf;        // double [3][F+NB] SoA
-    DBuf w;         // double [F]
+    DBuf bSf;       // double [3][NB] boundary face area vectors
     DBuf V;         // double [N] internal order
     DBuf permD;     // int32 [N] internal row -> external cell
     double meanAbsBandwidth = 0;
@@ -183,8 +190,8 @@ struct Krylov {
     Ctx* ctx = nullptr;
     int32_t N = 0;
     int grid = 0;  // workgroups of the row kernels
-    DBuf r, r0, p, v, t;   // FP64 [6][N]
-    DBuf rs, r0s, ps, vs, ts;  // FP32 [6][N] (mixed precision inner solve)
+    DBuf r, r0, p, v, t;   // FP64 [N][6]
+    DBuf rs, r0s, ps, vs, ts;  // FP32 [N][6] (mixed precision inner solve)
     DBuf ahat32;           // float [nEll]  column-scaled off-diagonals A D^-1
     DBuf ahat64;           // double [nEll]
     DBuf rD;               // double [N]    1/diag
@@ -211,15 +218,16 @@ struct Model {
     ModelParams prm;
     Controls ctl;
     double relax = 0;
-    // device state, SoA in internal numbering
-    DBuf theta, tau;        // [6][N]
+    // device state in internal numbering
+    DBuf theta, tau;        // [N][6] interleaved
     DBuf eigVal;            // [3][N] conformation eigenvalues (diagonal of eigVals)
     DBuf eigVec;            // [9][N]
-    DBuf U;                 // [3][N]
+    DBuf U;                 // [N][4] interleaved (4th slot unused)
     DBuf Ub;                // [3][NB]
-    DBuf phi;               // [F+NB]
+    DBuf phi;               // [F+NB] caller's face order
+    DBuf ePhi;              // [nEll] face flux per ELL entry (face order)
     DBuf gradU;             // [9][N] when supplied
-    DBuf diag, aval, b, fOld;  // matrix/source of the step
+    DBuf diag, aval, b, fOld;  // matrix (column order) / sources [N][6] of the step
     DBuf bcType;            // int32 [nPatches]
     DBuf bcVal;             // double [6][NB]
     std::vector<int32_t> hBcType;
@@ -236,30 +244,33 @@ struct AssembleArgs {
     const int32_t* ellWidth;
     const int32_t* ellCol;
     const int32_t* ellFid;
+    const uint8_t* ellSlot;
+    const double* eSf;    // [3][nEll]
+    const double* eW;
+    const double* ePhi;
+    int64_t nEll;
     const int32_t* bPtr;
     const int32_t* bFace;
     const int32_t* bPatch;
     const int32_t* bcType;
     const double* bcVal;  // [6][NB]
     int32_t NB;
-    int64_t nFaces;       // F + NB (SoA stride of Sf)
-    const double* Sf;
-    const double* w;
+    const double* bSf;    // [3][NB]
+    const double* phiB;   // [NB] boundary face fluxes
     const double* V;
-    const double* U;
-    const double* Ub;
-    const double* phi;
-    const double* gradU;  // nullable
-    const double* theta;
+    const double* U;      // [N][4]
+    const double* Ub;     // [3][NB]
+    const double* gradU;  // [9][N], nullable
+    const double* theta;  // [N][6]
     const double* eigVal;
     const double* eigVec;
     double rDeltaT;
     double relax;         // <= 0: none
     ModelParams prm;
     int32_t invMask;      // [OF9] inv() first-cell shift bits (ofInvQuirk)
-    double* aval;
+    double* aval;         // column order
     double* diag;
-    double* b;            // [6][N]
+    double* b;            // [N][6]
     double* fOld;         // FENE-P f, nullable
     double* partials;     // [grid][6] sums of theta (mean for normFactor)
     int32_t* counter;
@@ -274,6 +285,15 @@ void launch_aos_to_soa(Ctx* ctx, int32_t N, int ncomp, const double* aos, const 
                        double* soa);
 void launch_soa_to_aos(Ctx* ctx, int32_t N, int ncomp, const double* soa, const int32_t* perm,
                        double* aos);
+// gather rows of an interleaved [N][nc] array: out[r] = in[perm[r]] (perm null: copy),
+// padding the destination stride ncOut >= nc with zeros
+void launch_permute_rows(Ctx* ctx, int32_t N, int nc, int ncOut, const double* in, const int32_t* perm,
+                         double* out);
+// scatter rows back: out[perm[r]] = in[r] (first nc of ncIn components)
+void launch_unpermute_rows(Ctx* ctx, int32_t N, int ncIn, int nc, const double* in, const int32_t* perm,
+                           double* out);
+// per-entry face flux: ePhi[e] = phi[face of entry e]
+void launch_face_to_entry(Ctx* ctx, const Mesh& m, const double* phi, double* ePhi);
 void launch_eigvals_from_tensor(Ctx* ctx, int32_t N, const double* aos9, const int32_t* perm,
                                 double* eigVal);
 void launch_eigvals_to_tensor(Ctx* ctx, int32_t N, const double* eigVal, const int32_t* perm,
@@ -289,8 +309,8 @@ struct SolveArgs {
     Krylov* kry;
     const double* diag;  // [N]
     const double* aval;  // ELL values
-    const double* b;     // [6][N]
-    double* x;           // [6][N] (theta in place)
+    const double* b;     // [N][6]
+    double* x;           // [N][6] (theta in place)
     const int32_t* valid;
     const Controls* ctl;
     bool xbarKnown;      // CompState.xbar already set (by the assemble kernel)

@@ -190,31 +190,53 @@ void buildMesh(Ctx* ctx, const rt_mesh_desc* d, int renumber, Mesh& m) {
             int32_t c = m.iperm[r];
             wmax = std::max<int>(wmax, (int)(rptr[c + 1] - rptr[c]));
         }
+        if (wmax > 255) throw Error{RT_E_UNSUPPORTED, "rt_mesh_create: a cell has more than 255 faces"};
         width[s] = wmax;
         off[s + 1] = off[s] + (int64_t)SLICE * wmax;
         m.maxWidth = std::max(m.maxWidth, wmax);
     }
     m.nEll = off[m.nSlices];
-    std::vector<int32_t> ecol(std::max<int64_t>(1, m.nEll)), efid(std::max<int64_t>(1, m.nEll));
+    const int64_t nE = std::max<int64_t>(1, m.nEll);
+    std::vector<int32_t> ecol(nE), efid(nE);
+    std::vector<uint8_t> eslot(nE, 0);
+    std::vector<double> esf(3 * nE, 0.0), ew(nE, 0.0);
+    std::vector<std::pair<int32_t, int>> byCol;  // (column, face-order index)
     for (int32_t s = 0; s < m.nSlices; ++s) {
         for (int l = 0; l < SLICE; ++l) {
-            int32_t r = s * SLICE + l;
-            for (int k = 0; k < width[s]; ++k) {
-                int64_t e = off[s] + (int64_t)SLICE * k + l;
-                if (r >= N) {
-                    ecol[e] = 0;
-                    efid[e] = RT_FID_PAD;
-                    continue;
+            const int32_t r = s * SLICE + l;
+            const int W = width[s];
+            auto at = [&](int k) { return off[s] + (int64_t)SLICE * k + l; };
+            if (r >= N) {
+                for (int k = 0; k < W; ++k) {
+                    ecol[at(k)] = 0;
+                    efid[at(k)] = RT_FID_PAD;
+                    eslot[at(k)] = (uint8_t)k;
                 }
-                int32_t c = m.iperm[r];
-                int64_t deg = rptr[c + 1] - rptr[c];
-                if (k < deg) {
-                    ecol[e] = m.perm[rcol[rptr[c] + k]];
-                    efid[e] = rface[rptr[c] + k];
-                } else {
-                    ecol[e] = r;
-                    efid[e] = RT_FID_PAD;
-                }
+                continue;
+            }
+            const int32_t c = m.iperm[r];
+            const int deg = (int)(rptr[c + 1] - rptr[c]);
+            byCol.clear();
+            for (int k = 0; k < deg; ++k) byCol.push_back({m.perm[rcol[rptr[c] + k]], k});
+            std::sort(byCol.begin(), byCol.end());
+            std::vector<int> slotOf(deg);
+            for (int k = 0; k < deg; ++k) {
+                ecol[at(k)] = byCol[k].first;
+                slotOf[byCol[k].second] = k;
+            }
+            for (int k = 0; k < deg; ++k) {  // face order
+                const int32_t fid = rface[rptr[c] + k];
+                const bool own = fid >= 0;
+                const int64_t f = own ? fid : -(int64_t)fid - 1;
+                efid[at(k)] = fid;
+                eslot[at(k)] = (uint8_t)slotOf[k];
+                for (int q = 0; q < 3; ++q) esf[q * nE + at(k)] = own ? d->Sf[3 * f + q] : -d->Sf[3 * f + q];
+                ew[at(k)] = d->weights[f];
+            }
+            for (int k = deg; k < W; ++k) {
+                ecol[at(k)] = r;
+                efid[at(k)] = RT_FID_PAD;
+                eslot[at(k)] = (uint8_t)k;
             }
         }
     }
@@ -236,11 +258,9 @@ void buildMesh(Ctx* ctx, const rt_mesh_desc* d, int renumber, Mesh& m) {
         for (int32_t b = 0; b < NB; ++b)
             if (!bEmpty[b]) bface[fill[m.perm[d->owner[F + b]]]++] = b;
     }
-    // geometry, SoA
-    std::vector<double> Sf(3 * std::max<int64_t>(1, NF));
-    for (int64_t f = 0; f < NF; ++f)
-        for (int q = 0; q < 3; ++q) Sf[q * NF + f] = d->Sf[3 * f + q];
-    std::vector<double> w(d->weights ? d->weights : nullptr, d->weights ? d->weights + F : nullptr);
+    std::vector<double> bsf(3 * (size_t)std::max(1, NB), 0.0);
+    for (int32_t b = 0; b < NB; ++b)
+        for (int q = 0; q < 3; ++q) bsf[(size_t)q * std::max(1, NB) + b] = d->Sf[3 * ((int64_t)F + b) + q];
     std::vector<double> V(N);
     for (int32_t r = 0; r < N; ++r) V[r] = d->V[m.iperm[r]];
 
@@ -248,11 +268,13 @@ void buildMesh(Ctx* ctx, const rt_mesh_desc* d, int renumber, Mesh& m) {
     upload(ctx, m.ellWidth, width);
     upload(ctx, m.ellCol, ecol);
     upload(ctx, m.ellFid, efid);
+    upload(ctx, m.ellSlot, eslot);
+    upload(ctx, m.eSf, esf);
+    upload(ctx, m.eW, ew);
     upload(ctx, m.bPtr, bptr);
     upload(ctx, m.bFace, bface);
     upload(ctx, m.bPatch, bpatch);
-    upload(ctx, m.Sf, Sf);
-    upload(ctx, m.w, w);
+    upload(ctx, m.bSf, bsf);
     upload(ctx, m.V, V);
     upload(ctx, m.permD, m.iperm);
     RT_HIP(hipStreamSynchronize(ctx->stream));

@@ -114,7 +114,7 @@ def test_literal_wide_range_overflows_like_oracle(ctx):
     # cells the overshoot left with a moderate theta (exp does not amplify the
     # 1e-10 solver tolerance beyond the bar there)
     sane = np.abs(o["theta"]).max(1) < 30.0
-    assert sane.sum() > 0.5 * len(sane)
+    assert sane.sum() > 0.2 * len(sane)
     assert rel_l2(g["tau"][sane], o["tau"][sane]) < THETA_TOL
 
 
