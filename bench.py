@@ -92,12 +92,98 @@ def cpu_baseline(w, n_sample):
                                  iterations=[int(q.nIterations) for q in perf])
 
 
+SERVE_DESC = ("serving stream: {n} stress-step requests, Poisson arrivals at {rate}/s, 2D/3D "
+              "Cartesian channels (coin flip) with cells log-uniform on [1e3, {mx:.0e}], model uniform "
+              "over Oldroyd-B / Giesekus alpha=0.2 / linear PTT eps=0.25, Wi log-uniform [0.1, 20], "
+              "dt = 0.1 dx; FP32 Krylov + FP64 refinement; HBM budget 200 GB")
+
+
+def serve_bench(args):
+    """Continuous-batching stream: requests arrive as a Poisson process, are
+    admitted by the HBM budget and run overlapped; reports p50/p99 latency and
+    aggregate cell-updates/s.  Meshes are uploaded once per distinct size and
+    shared (a serving system caches them); fields are generated per mesh."""
+    from rheotool_amd import abi, cases
+    rng = np.random.default_rng(4)
+    nreq = args.requests
+    specs = []
+    for i in range(nreq):
+        twoD = bool(rng.integers(2))
+        ncell = float(np.exp(rng.uniform(np.log(1e3), np.log(args.max_cells))))
+        # quantise the size: ~12 distinct shapes per dimension keep host-side
+        # generation bounded while spanning the range
+        if twoD:
+            n = int(round(np.sqrt(ncell)))
+            n = int(2 ** (round(np.log2(n) * 3) / 3))
+            key = ("2d", max(n, 8))
+        else:
+            n = int(round(ncell ** (1 / 3)))
+            n = int(2 ** (round(np.log2(n) * 3) / 3))
+            key = ("3d", max(n, 6))
+        model = ["Oldroyd-BLog", "GiesekusLog", "PTTLog"][int(rng.integers(3))]
+        lam = float(np.exp(rng.uniform(np.log(0.1), np.log(20.0))))
+        specs.append((key, model, lam))
+    t0 = time.perf_counter()
+    ctx = abi.Context(0)
+    meshes = {}
+    for key in sorted({s[0] for s in specs}):
+        kind, n = key
+        m = (cases.cartesian(n, n, 1, twoD=True, channel=True) if kind == "2d"
+             else cases.cartesian(n, n, n, channel=True))
+        f = cases.make_fields(m, cases.FIELDS_CHANNEL, seed=1000 + n)
+        meshes[key] = (m, f, ctx.mesh(m), 0.1 / n)
+    log(f"[serve] {len(meshes)} distinct meshes prepared in {time.perf_counter() - t0:.1f} s")
+    kwmap = {"GiesekusLog": dict(alpha=0.2), "PTTLog": dict(epsilon=0.25, zeta=0.0)}
+    controls = abi.solver_dict(1e-10, precision="mixed")
+    srv = abi.Server(ctx, hbm_budget=200 << 30, max_inflight=16)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, nreq))
+    tickets = [None] * nreq
+    done = [None] * nreq
+    cells = 0
+    t_start = time.perf_counter()
+    i = 0
+    while True:
+        now = time.perf_counter() - t_start
+        while i < nreq and arrivals[i] <= now:
+            key, model, lam = specs[i]
+            m, f, dm, dt = meshes[key]
+            tickets[i] = srv.submit(dm, f, abi.model_dict(model, lam=lam, **kwmap.get(model, {})), dt,
+                                    controls)
+            i += 1
+        live = srv.progress()
+        for j in range(i):
+            if done[j] is None and tickets[j] is not None:
+                out = srv.poll(tickets[j])
+                if out is not None:
+                    done[j] = out["latency_ms"]
+                    cells += meshes[specs[j][0]][0].nCells
+        if i == nreq and live == 0 and all(d is not None for d in done):
+            break
+    elapsed = time.perf_counter() - t_start
+    lat = np.array(done)
+    srv.close()
+    line = dict(metric=METRIC, value=round(cells / elapsed, 1), unit="cell-updates/s", n_gpus=1,
+                steps=nreq, warmup=0, ms_per_step=round(1e3 * elapsed / nreq, 3),
+                higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f32+f64",
+                data="synthetic",
+                config=dict(workload=SERVE_DESC.format(n=nreq, rate=args.rate, mx=args.max_cells),
+                            requests=nreq, total_cells=int(cells), distinct_meshes=len(meshes)),
+                p50_ms=round(float(np.percentile(lat, 50)), 3),
+                p99_ms=round(float(np.percentile(lat, 99)), 3),
+                max_ms=round(float(lat.max()), 3), wall_s=round(elapsed, 3))
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="precision", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="precision", choices=sorted(WORKLOADS) + ["copy", "serve"])
+    ap.add_argument("--requests", type=int, default=512)
+    ap.add_argument("--rate", type=float, default=300.0, help="serve: Poisson arrival rate, 1/s")
+    ap.add_argument("--max-cells", type=float, default=2e6)
     ap.add_argument("--n", type=int, default=0, help="override the mesh size")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-compare", action="store_true")
@@ -109,6 +195,16 @@ def main():
         sys.exit(2)
 
     from rheotool_amd import abi
+    if args.workload == "copy":
+        # HBM calibration: a plain copy of 2 GiB each way
+        with abi.Context(0) as ctx:
+            nbytes = 2 << 30
+            ms = ctx.copy_bench(nbytes, reps=max(args.steps, 5))
+            print(json.dumps(dict(workload="copy", bytes_read=nbytes, bytes_written=nbytes,
+                                  ms=round(ms, 4), GBps=round(2 * nbytes / (ms * 1e-3) / 1e9, 1))))
+        return
+    if args.workload == "serve":
+        return serve_bench(args)
     w = WORKLOADS[args.workload]
     n = args.n or w["n"]
     t0 = time.perf_counter()

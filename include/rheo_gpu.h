@@ -81,6 +81,12 @@ typedef struct {
 int rt_ctx_kernel_stats(rt_ctx* ctx, rt_kernel_stat* out, int cap);
 void rt_ctx_reset_stats(rt_ctx* ctx);
 
+/* HBM calibration: copies `bytes` between two device buffers with a plain
+ * streaming kernel `reps` times and returns the mean kernel time in *ms.
+ * No reference counterpart; it gives the copy bandwidth of the device and the
+ * known-size kernel that hardware byte counters are scaled against. */
+int rt_ctx_copy_bench(rt_ctx* ctx, size_t bytes, int reps, double* ms);
+
 /* ------------------------------------------------------------------- mesh
  * The primitiveMesh/lduAddressing the reference's path runs on:
  * lowerAddr(=owner)/upperAddr(=neighbour) per internal face with
@@ -236,7 +242,7 @@ int rt_symm_solve(rt_ctx* ctx, const rt_mesh* mesh, const rt_ldu_symm* sys, doub
 typedef struct rt_server rt_server;
 
 typedef struct {
-    const rt_mesh_desc* mesh;
+    const rt_mesh* mesh;  /* an uploaded mesh (rt_mesh_create); requests may share one */
     const char* dict;     /* constitutiveProperties parameters text */
     const char* controls; /* fvSolution text, NULL for the server default */
     const double* theta;  /* [6N] */
@@ -244,7 +250,7 @@ typedef struct {
     const double* eigVecs;
     const double* U;      /* [3N] */
     const double* Ub;     /* [3*nBoundaryFaces] */
-    const double* phi;    /* [nFaces] */
+    const double* phi;    /* [nInternalFaces + nBoundaryFaces] */
     double deltaT;
 } rt_request;
 

@@ -93,6 +93,7 @@ def lib():
             "rt_ctx_set_profiling": (i32, [vp, i32]),
             "rt_ctx_kernel_stats": (i32, [vp, vp, i32]),
             "rt_ctx_reset_stats": (None, [vp]),
+            "rt_ctx_copy_bench": (i32, [vp, C.c_size_t, i32, vp]),
             "rt_mesh_create": (i32, [vp, vp, i32, vp]),
             "rt_mesh_destroy": (None, [vp]),
             "rt_mesh_valid_components": (i32, [vp, vp]),
@@ -184,6 +185,12 @@ class Context:
 
     def reset_stats(self):
         lib().rt_ctx_reset_stats(self.h)
+
+    def copy_bench(self, nbytes, reps=20):
+        """Mean time (ms) of the plain device copy kernel over nbytes."""
+        ms = C.c_double()
+        self.check(lib().rt_ctx_copy_bench(self.h, nbytes, reps, C.byref(ms)), "copy_bench")
+        return ms.value
 
     def mesh(self, m, renumber=RT_RENUMBER_AUTO) -> "Mesh":
         return Mesh(self, m, renumber)
@@ -374,13 +381,13 @@ class Server:
         self.ctx, self.h = ctx, h
         self.sizes = {}
 
-    def submit(self, m, fields, dict_text, deltaT, controls=None) -> int:
-        d, keep = mesh_desc(m)
+    def submit(self, mesh: "Mesh", fields, dict_text, deltaT, controls=None) -> int:
+        """mesh: an uploaded Mesh (shared by any number of requests)."""
+        m = mesh.src
         arrs = [_f64(fields.theta), _f64(fields.eigVals), _f64(fields.eigVecs), _f64(fields.U),
                 _f64(fields.Ub) if m.nBoundary else None, _f64(fields.phi)]
-        q = Request(C.cast(C.pointer(d), C.c_void_p), dict_text.encode(),
-                    None if controls is None else controls.encode(), *[_p(a) for a in arrs],
-                    deltaT)
+        q = Request(mesh.h, dict_text.encode(), None if controls is None else controls.encode(),
+                    *[_p(a) for a in arrs], deltaT)
         t = C.c_int64()
         self.ctx.check(lib().rt_server_submit(self.h, C.byref(q), C.byref(t)), "rt_server_submit")
         self.sizes[t.value] = m.nCells

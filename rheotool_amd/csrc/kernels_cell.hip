@@ -13,86 +13,9 @@
 // k_eig_tau: calcEig (constitutiveEq.C:360-428) with a cyclic Jacobi
 // eigensolver per cell, exp of the eigenvalues, and the tau rebuild
 // (Oldroyd_BLog.C:175, GiesekusLog.C:172, FENE_PLog.C:178, PTTLog.C:264).
-#include "rt_internal.h"
+#include "rt_device.h"
 
 namespace rt {
-
-constexpr int BLK = 256;
-
-__device__ __forceinline__ int xcd_block(int G) {
-    // Consecutive logical blocks on one XCD so neighbouring rows share an L2.
-    const int b = blockIdx.x;
-    if (G % 8 != 0) return b;
-    return (b % 8) * (G / 8) + b / 8;
-}
-
-__device__ __forceinline__ void block_rows(int32_t N, int32_t& r0, int32_t& r1) {
-    const int G = gridDim.x;
-    const int b = xcd_block(G);
-    int64_t per = ((int64_t)N + G - 1) / G;
-    per = (per + BLK - 1) / BLK * BLK;
-    int64_t a = (int64_t)b * per;
-    r0 = (int32_t)(a < N ? a : N);
-    r1 = (int32_t)(a + per < N ? a + per : N);
-}
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-// Block reduction of NQ values per thread into partials[block][NQ]; the last
-// block to finish reduces all blocks in a fixed order (deterministic) and
-// returns true with sums[] filled (thread 0 valid).
-template <int NQ>
-__device__ bool block_reduce_last(double (&v)[NQ], double* partials, int32_t* counter,
-                                  double (&sums)[NQ]) {
-    __shared__ double red[NQ][BLK / 64];
-    __shared__ int amLast;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        double s = wave_sum(v[q]);
-        if (lane == 0) red[q][wid] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x < NQ) {
-        double s = 0;
-        for (int w = 0; w < BLK / 64; ++w) s += red[threadIdx.x][w];
-        partials[(int64_t)blockIdx.x * NQ + threadIdx.x] = s;
-    }
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) amLast = atomicAdd(counter, 1) == (int)gridDim.x - 1;
-    __syncthreads();
-    if (!amLast) return false;
-    __threadfence();
-    const volatile double* vp = partials;
-    double acc[NQ];
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) acc[q] = 0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += BLK)
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) acc[q] += vp[(int64_t)i * NQ + q];
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        double s = wave_sum(acc[q]);
-        if (lane == 0) red[q][wid] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            double s = 0;
-            for (int w = 0; w < BLK / 64; ++w) s += red[q][w];
-            sums[q] = s;
-        }
-        *counter = 0;
-    }
-    return true;
-}
 
 // ------------------------------------------------------ tensor algebra
 // OpenFOAM inner product a & b (row-major 3x3), sum order k = 0,1,2.
@@ -233,67 +156,82 @@ __device__ void cell_rhs(const ModelParams& p, int invMask, const double* L, con
     symmT(R, S);
 }
 
-__device__ __forceinline__ void load6(const double* p, double* v) {
-    const double2* q = reinterpret_cast<const double2*>(p);
-    const double2 a = q[0], b = q[1], c = q[2];
-    v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y; v[4] = c.x; v[5] = c.y;
-}
-__device__ __forceinline__ void store6(double* p, const double* v) {
-    double2* q = reinterpret_cast<double2*>(p);
-    q[0] = make_double2(v[0], v[1]);
-    q[1] = make_double2(v[2], v[3]);
-    q[2] = make_double2(v[4], v[5]);
-}
+// Widest row whose matrix values are staged in registers and written in
+// column order; wider rows write each value to its slot directly.
+constexpr int STAGE_W = 8;
 
+template <class T>
 __global__ void __launch_bounds__(BLK) k_assemble(AssembleArgs a) {
-    int32_t r0, r1;
-    block_rows(a.N, r0, r1);
-    double thsum[6] = {0, 0, 0, 0, 0, 0};
+    double q[18];
+#pragma unroll
+    for (int i = 0; i < 18; ++i) q[i] = 0;
+    double xbar[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) xbar[c] = a.cs[c].xbar;
     const int64_t nE = a.nEll;
-    for (int32_t r = r0 + threadIdx.x; r < r1; r += BLK) {
+    T* rOut = static_cast<T*>(a.r);
+    T* r0Out = static_cast<T*>(a.r0);
+    for (int64_t it = 0;; ++it) {
+        const int64_t t0 = tile_of(it) * BLK;
+        if (t0 >= a.N) break;
+        const int32_t r = (int32_t)(t0 + threadIdx.x);
+        if (r >= a.N) continue;
         const int32_t N = a.N;
-        double th[6], lam[3], V[9];
-        load6(a.theta + (int64_t)NCMP * r, th);
-#pragma unroll
-        for (int q = 0; q < 3; ++q) lam[q] = a.eigVal[(int64_t)q * N + r];
-#pragma unroll
-        for (int q = 0; q < 9; ++q) V[q] = a.eigVec[(int64_t)q * N + r];
+        double th[6];
+        ld6(a.theta + (int64_t)NCMP * r, th);
         const double vol = a.V[r];
         const double2 u01 = reinterpret_cast<const double2*>(a.U)[2 * (int64_t)r];
         const double Ui[3] = {u01.x, u01.y, a.U[4 * (int64_t)r + 2]};
         const int s = r >> 6, lane = r & 63;
         const int64_t base = a.ellOff[s] + lane;
         const int width = a.ellWidth[s];
+        const bool staged = width <= STAGE_W;
         double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        double divDiag = 0.0, sumOff = 0.0;
+        double av[STAGE_W];
+#pragma unroll
+        for (int k = 0; k < STAGE_W; ++k) av[k] = 0.0;
+        double offY[6] = {0, 0, 0, 0, 0, 0};
+        double divDiag = 0.0, sumOff = 0.0, sumAv = 0.0;
         const bool haveGrad = a.gradU != nullptr;
         for (int k = 0; k < width; ++k) {
             const int64_t e = base + (int64_t)SLICE * k;  // face-order entry
             const int32_t fid = a.ellFid[e];
+            const int slot = a.ellSlot[e];
             if (fid == RT_FID_PAD) {
-                a.aval[base + (int64_t)SLICE * a.ellSlot[e]] = 0.0;
+                if (!staged) a.aval[base + (int64_t)SLICE * slot] = 0.0;
                 continue;
             }
             const bool own = fid >= 0;
-            const int64_t ec = base + (int64_t)SLICE * a.ellSlot[e];  // column-order slot
+            const int64_t ec = base + (int64_t)SLICE * slot;  // column-order slot
             const double ph = a.ePhi[e];
             const double upw = ph >= 0 ? 1.0 : 0.0;
             const double lower = -upw * ph;
             const double upper = (1.0 - upw) * ph;
             const double val = own ? upper : lower;
-            a.aval[ec] = val;
+            if (staged) {
+#pragma unroll
+                for (int j = 0; j < STAGE_W; ++j)
+                    if (j == slot) av[j] = val;
+            } else {
+                a.aval[ec] = val;
+            }
             divDiag -= own ? lower : upper;
             sumOff += fabs(val);
+            sumAv += val;
+            const int32_t c = a.ellCol[ec];
+            double thc[6];
+            ld6(a.theta + (int64_t)NCMP * c, thc);
+#pragma unroll
+            for (int q6 = 0; q6 < 6; ++q6) offY[q6] += val * thc[q6];
             if (!haveGrad) {
-                const int32_t c = a.ellCol[ec];
                 const double wf = a.eW[e];
                 const double2 c01 = reinterpret_cast<const double2*>(a.U)[2 * (int64_t)c];
                 const double Uc[3] = {c01.x, c01.y, a.U[4 * (int64_t)c + 2]};
                 double uf[3];
 #pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    const double uo = own ? Ui[q] : Uc[q], un = own ? Uc[q] : Ui[q];
-                    uf[q] = wf * (uo - un) + un;
+                for (int q3 = 0; q3 < 3; ++q3) {
+                    const double uo = own ? Ui[q3] : Uc[q3], un = own ? Uc[q3] : Ui[q3];
+                    uf[q3] = wf * (uo - un) + un;
                 }
                 // Sf already carries the row's side: g += (+-Sf) (x) U_f
                 const double sf[3] = {a.eSf[e], a.eSf[nE + e], a.eSf[2 * nE + e]};
@@ -303,11 +241,13 @@ __global__ void __launch_bounds__(BLK) k_assemble(AssembleArgs a) {
                     for (int j = 0; j < 3; ++j) g[3 * i + j] += sf[i] * uf[j];
             }
         }
+        if (staged)
+            for (int k = 0; k < width; ++k) a.aval[base + (int64_t)SLICE * k] = av[k];
         const int32_t b0 = a.bPtr[r], b1 = a.bPtr[r + 1];
         const int64_t NB = a.NB > 0 ? a.NB : 1;
         if (!haveGrad) {
-            for (int32_t q = b0; q < b1; ++q) {
-                const int32_t bf = a.bFace[q];
+            for (int32_t qb = b0; qb < b1; ++qb) {
+                const int32_t bf = a.bFace[qb];
                 const double sf[3] = {a.bSf[bf], a.bSf[NB + bf], a.bSf[2 * NB + bf]};
                 const double ub[3] = {a.Ub[bf], a.Ub[NB + bf], a.Ub[2 * NB + bf]};
 #pragma unroll
@@ -316,23 +256,30 @@ __global__ void __launch_bounds__(BLK) k_assemble(AssembleArgs a) {
                     for (int j = 0; j < 3; ++j) g[3 * i + j] += sf[i] * ub[j];
             }
 #pragma unroll
-            for (int q = 0; q < 9; ++q) g[q] /= vol;
+            for (int qq = 0; qq < 9; ++qq) g[qq] /= vol;
         } else {
 #pragma unroll
-            for (int q = 0; q < 9; ++q) g[q] = a.gradU[(int64_t)q * N + r];
+            for (int qq = 0; qq < 9; ++qq) g[qq] = a.gradU[(int64_t)qq * N + r];
         }
         double S[6], f;
-        cell_rhs(a.prm, a.invMask, g, th, lam, V, S, f);
+        {
+            double lam[3], V[9];
+#pragma unroll
+            for (int qq = 0; qq < 3; ++qq) lam[qq] = a.eigVal[(int64_t)qq * N + r];
+#pragma unroll
+            for (int qq = 0; qq < 9; ++qq) V[qq] = a.eigVec[(int64_t)qq * N + r];
+            cell_rhs(a.prm, a.invMask, g, th, lam, V, S, f);
+        }
         if (a.fOld) a.fOld[r] = f;
         // matrix diagonal: ddt + div, optional relax, then boundary internalCoeffs
         double D = a.rDeltaT * vol + divDiag;
         double src[6];
 #pragma unroll
-        for (int q = 0; q < 6; ++q) src[q] = a.rDeltaT * th[q] * vol + vol * S[q];
+        for (int qq = 0; qq < 6; ++qq) src[qq] = a.rDeltaT * th[qq] * vol + vol * S[qq];
         if (a.relax > 0) {
             const double D0 = D;
-            for (int32_t q = b0; q < b1; ++q) {
-                const int32_t bf = a.bFace[q];
+            for (int32_t qb = b0; qb < b1; ++qb) {
+                const int32_t bf = a.bFace[qb];
                 const int t = a.bcType[a.bPatch[bf]];
                 if (t == RT_BC_EMPTY) continue;
                 const double ph = a.phiB[bf];
@@ -340,18 +287,18 @@ __global__ void __launch_bounds__(BLK) k_assemble(AssembleArgs a) {
             }
             D = fmax(fabs(D), sumOff);
             D /= a.relax;
-            for (int32_t q = b0; q < b1; ++q) {
-                const int32_t bf = a.bFace[q];
+            for (int32_t qb = b0; qb < b1; ++qb) {
+                const int32_t bf = a.bFace[qb];
                 const int t = a.bcType[a.bPatch[bf]];
                 if (t == RT_BC_EMPTY) continue;
                 const double ph = a.phiB[bf];
                 D -= t == RT_BC_ZEROGRADIENT ? ph * 1.0 : ph * 0.0;
             }
 #pragma unroll
-            for (int q = 0; q < 6; ++q) src[q] += (D - D0) * th[q];
+            for (int qq = 0; qq < 6; ++qq) src[qq] += (D - D0) * th[qq];
         }
-        for (int32_t q = b0; q < b1; ++q) {
-            const int32_t bf = a.bFace[q];
+        for (int32_t qb = b0; qb < b1; ++qb) {
+            const int32_t bf = a.bFace[qb];
             const int t = a.bcType[a.bPatch[bf]];
             if (t == RT_BC_EMPTY) continue;
             const double ph = a.phiB[bf];
@@ -364,16 +311,34 @@ __global__ void __launch_bounds__(BLK) k_assemble(AssembleArgs a) {
             }
         }
         a.diag[r] = D;
+        a.rD[r] = 1.0 / D;
 #pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            if (a.valid[q] <= 0) src[q] = 0.0;
-            thsum[q] += th[q];
+        for (int qq = 0; qq < 6; ++qq)
+            if (a.valid[qq] <= 0) src[qq] = 0.0;
+        st6(a.b + (int64_t)NCMP * r, src);
+        // initial residual of the solve: r = b - A theta, and the sums of the
+        // normalisation factor (sparseSolver.C:129-180)
+        const double sumA = D + sumAv;
+        T rt[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            const double y = D * th[c] + offY[c];
+            const double res = src[c] - y;
+            rt[c] = (T)res;
+            if (a.valid[c] > 0) {
+                q[c] += fabs(res);
+                q[6 + c] += (double)rt[c] * (double)rt[c];
+                const double pA = sumA * xbar[c];
+                q[12 + c] += fabs(y - pA) + fabs(src[c] - pA);
+            }
         }
-        store6(a.b + (int64_t)NCMP * r, src);
+        st6(rOut + (int64_t)NCMP * r, rt);
+        st6(r0Out + (int64_t)NCMP * r, rt);
     }
-    double sums[6];
-    if (block_reduce_last<6>(thsum, a.partials, a.counter, sums) && threadIdx.x == 0) {
-        for (int q = 0; q < 6; ++q) a.cs[q].xbar = sums[q] / a.N;
+    double sums[18];
+    if (reduce_last<18>(q, a.partials, a.counter, sums) && threadIdx.x == 0) {
+        for (int c = 0; c < 6; ++c)
+            if (a.valid[c] > 0) finish_initial(a.cs[c], a.ctl, sums[c], sums[6 + c], sums[12 + c]);
     }
 }
 
@@ -443,7 +408,7 @@ __global__ void __launch_bounds__(BLK) k_eig_tau(int32_t N, ModelParams p, const
     const int64_t r = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (r >= N) return;
     double th[6], ev[3], V[9];
-    load6(theta + (int64_t)NCMP * r, th);
+    ld6(theta + (int64_t)NCMP * r, th);
     jacobi3(th, ev, V);
     double lam[3];
 #pragma unroll
@@ -474,7 +439,7 @@ __global__ void __launch_bounds__(BLK) k_eig_tau(int32_t N, ModelParams p, const
     symmT(D, s);
 #pragma unroll
     for (int q = 0; q < 6; ++q) s[q] = coef * s[q];
-    store6(tau + (int64_t)NCMP * r, s);
+    st6(tau + (int64_t)NCMP * r, s);
 }
 
 // ------------------------------------------------------------- layout helpers
@@ -563,16 +528,26 @@ __global__ void k_unpermute_rows(int32_t N, int ncIn, int nc, const double* __re
     for (int q = 0; q < nc; ++q) out[c * nc + q] = in[r * ncIn + q];
 }
 
+// plain streaming copy (16-byte vectors), the HBM calibration kernel
+__global__ void __launch_bounds__(BLK) k_copy(int64_t n16, const double2* __restrict__ src,
+                                              double2* __restrict__ dst) {
+    for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n16; i += (int64_t)gridDim.x * BLK)
+        dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------- launchers
 static inline unsigned nblk(int64_t n) { return (unsigned)((n + BLK - 1) / BLK); }
 
 void launch_assemble(Ctx* ctx, const AssembleArgs& a, int grid) {
-    // algorithmic bytes per cell: theta 48, eigen state 96, U 32, V 8, b 48, diag 8;
-    // per ELL entry: face id 4, slot 1, Sf 24, w 8, phi 8, value 8 (+ column 4 for the gradient)
-    const double by = (double)a.N * (48 + 96 + 32 + 8 + 48 + 8) +
-                      (double)a.nEll * (4 + 1 + 8 + 8 + (a.gradU ? 0 : 24 + 8 + 4));
-    ProfScope ps(ctx, "k_assemble", by);
-    hipLaunchKernelGGL(k_assemble, dim3(grid), dim3(BLK), 0, ctx->stream, a);
+    // algorithmic bytes per cell: theta 48, eigen state 96, U 32, V 8, b 48, diag 8, 1/diag 8,
+    // r and r0 (2 x 6 x w); per ELL entry: face id 4, slot 1, column 4, phi 8, value 8
+    // (+ Sf 24 and w 8 for the gradient)
+    const double w = a.mixed ? 4 : 8;
+    const double by = (double)a.N * (48 + 96 + 32 + 8 + 48 + 8 + 8 + 12 * w) +
+                      (double)a.nEll * (4 + 1 + 4 + 8 + 8 + (a.gradU ? 0 : 24 + 8));
+    ProfScope ps(ctx, a.mixed ? "k_assemble<f32>" : "k_assemble<f64>", by);
+    if (a.mixed) hipLaunchKernelGGL(k_assemble<float>, dim3(grid), dim3(BLK), 0, ctx->stream, a);
+    else hipLaunchKernelGGL(k_assemble<double>, dim3(grid), dim3(BLK), 0, ctx->stream, a);
     RT_HIP(hipGetLastError());
 }
 void launch_eig_tau(Ctx* ctx, int32_t N, const ModelParams& prm, const double* theta,
@@ -596,6 +571,13 @@ void launch_eigvals_from_tensor(Ctx* ctx, int32_t N, const double* aos9, const i
 }
 void launch_eigvals_to_tensor(Ctx* ctx, int32_t N, const double* ev, const int32_t* perm, double* aos9) {
     hipLaunchKernelGGL(k_eigvals_to_tensor, dim3(nblk(N)), dim3(BLK), 0, ctx->stream, N, ev, perm, aos9);
+    RT_HIP(hipGetLastError());
+}
+void launch_copy(Ctx* ctx, const void* src, void* dst, size_t bytes) {
+    const int64_t n16 = (int64_t)(bytes / 16);
+    const unsigned grid = (unsigned)std::min<int64_t>(nblk(n16), (int64_t)ctx->numCUs * 16);
+    ProfScope ps(ctx, "k_copy", 2.0 * (double)n16 * 16);
+    hipLaunchKernelGGL(k_copy, dim3(grid), dim3(BLK), 0, ctx->stream, n16, (const double2*)src, (double2*)dst);
     RT_HIP(hipGetLastError());
 }
 void launch_fill(Ctx* ctx, double* p, int64_t n, double v) {

@@ -6,97 +6,21 @@
 // normalisation sparseSolver.C:129-180).  The components share the matrix, so
 // here one pass over the matrix serves every component still iterating:
 // each kernel walks the sliced-ELL rows once and updates all active
-// components, with per-component convergence masks.  Every scalar of the
-// recurrence (rho, alpha, omega, beta, residuals, flags) stays on the device:
-// the last workgroup of each reduction finalises it, so no host round trip is
-// needed inside an iteration.
+// components (stored interleaved, [N][6]) with per-component convergence
+// masks.  Every scalar of the recurrence (rho, alpha, omega, beta, residuals,
+// flags) stays on the device: the last workgroup of each reduction finalises
+// it, so no host round trip is needed inside an iteration.
 //
 // Preconditioning is right-Jacobi with the column-scaled matrix
 // Ahat = A D^-1 stored explicitly: A M^-1 p = p + offdiag(Ahat) p.
 //
 // Precision: the vector type T is double (FP64 drop-in) or float (inner
 // solve of the FP64-refined "mixed" solve).  x (theta), b, the matrix used
-// for the true residual and all dot products are FP64.
-#include "rt_internal.h"
+// for the true residual and the grid-level reductions are FP64; per-thread
+// partial sums use T.
+#include "rt_device.h"
 
 namespace rt {
-
-constexpr int KBLK = 256;
-
-__device__ __forceinline__ int kxcd_block(int G) {
-    const int b = blockIdx.x;
-    if (G % 8 != 0) return b;
-    return (b % 8) * (G / 8) + b / 8;
-}
-__device__ __forceinline__ void krows(int32_t N, int32_t& r0, int32_t& r1) {
-    const int G = gridDim.x;
-    const int b = kxcd_block(G);
-    int64_t per = ((int64_t)N + G - 1) / G;
-    per = (per + KBLK - 1) / KBLK * KBLK;
-    int64_t a = (int64_t)b * per;
-    r0 = (int32_t)(a < N ? a : N);
-    r1 = (int32_t)(a + per < N ? a + per : N);
-}
-__device__ __forceinline__ double kwave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-template <int NQ>
-__device__ bool kreduce_last(double (&v)[NQ], double* partials, int32_t* counter,
-                             double (&sums)[NQ]) {
-    __shared__ double red[NQ][KBLK / 64];
-    __shared__ int amLast;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        double s = kwave_sum(v[q]);
-        if (lane == 0) red[q][wid] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x < NQ) {
-        double s = 0;
-        for (int w = 0; w < KBLK / 64; ++w) s += red[threadIdx.x][w];
-        partials[(int64_t)blockIdx.x * NQ + threadIdx.x] = s;
-    }
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) amLast = atomicAdd(counter, 1) == (int)gridDim.x - 1;
-    __syncthreads();
-    if (!amLast) return false;
-    __threadfence();
-    const volatile double* vp = partials;
-    double acc[NQ];
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) acc[q] = 0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += KBLK)
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) acc[q] += vp[(int64_t)i * NQ + q];
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        double s = kwave_sum(acc[q]);
-        if (lane == 0) red[q][wid] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            double s = 0;
-            for (int w = 0; w < KBLK / 64; ++w) s += red[q][w];
-            sums[q] = s;
-        }
-        *counter = 0;
-    }
-    return true;
-}
-
-struct KryCtl {
-    double tol, relTol, innerRelTol;
-    int32_t maxIter, minIter, maxInner, maxRefine;
-    int32_t mixed;
-};
 
 template <class T>
 struct KryArgs {
@@ -108,9 +32,9 @@ struct KryArgs {
     const double* diag;
     const T* ahat;       // A D^-1 off-diagonal values
     const double* rD;    // 1 / diag
-    const double* b;     // [6][N]
-    double* x;           // [6][N]
-    T *r, *r0, *p, *v, *t;  // [6][N]
+    const double* b;     // [N][6]
+    double* x;           // [N][6]
+    T *r, *r0, *p, *v, *t;  // [N][6]
     double* partials;
     int32_t* counters;   // one per kernel kind
     CompState* cs;
@@ -118,69 +42,29 @@ struct KryArgs {
     KryCtl ctl;
 };
 
-__device__ __forceinline__ bool conv_outer(const CompState& c, const KryCtl& k, double res) {
-    return res < k.tol || (k.relTol > 0 && res < k.relTol * c.initRes);
-}
-__device__ __forceinline__ bool conv_inner(const CompState& c, const KryCtl& k, double res) {
-    return k.mixed ? res < c.innerTarget : conv_outer(c, k, res);
-}
-
-// begin an inner Krylov solve on a component whose r, r0 were just set
-__device__ void start_inner(CompState& c, const KryCtl& k, double rr) {
-    c.innerIter = 0;
-    c.rho = rr;
-    c.exitAtS = 0;
-    if (k.mixed) c.innerTarget = fmax(k.innerRelTol * c.finalRes, 0.3 * k.tol);
-    if (fabs(rr) < 1e-300) {  // singular: nothing to iterate on
-        c.active = 0;
-        c.converged = conv_outer(c, k, c.finalRes);
-        return;
-    }
-    c.active = 1;
-}
-
-// interleaved [N][6] rows: three 2-wide vector loads/stores per row
-template <class T> struct Vec2;
-template <> struct Vec2<float> { typedef float2 type; };
-template <> struct Vec2<double> { typedef double2 type; };
-
-template <class T>
-__device__ __forceinline__ void ld6(const T* p, T* v) {
-    typedef typename Vec2<T>::type V2;
-    const V2* q = reinterpret_cast<const V2*>(p);
-    const V2 a = q[0], b = q[1], c = q[2];
-    v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y; v[4] = c.x; v[5] = c.y;
-}
-template <class T>
-__device__ __forceinline__ void st6(T* p, const T* v) {
-    typedef typename Vec2<T>::type V2;
-    V2* q = reinterpret_cast<V2*>(p);
-    V2 a, b, c;
-    a.x = v[0]; a.y = v[1]; b.x = v[2]; b.y = v[3]; c.x = v[4]; c.y = v[5];
-    q[0] = a; q[1] = b; q[2] = c;
-}
-
-// ---- true residual r = b - A x (FP64), init or refinement -----------------
-// mode 0: initial (sets normFactor, initRes); mode 1: refinement sweep.
-template <class T>
-__global__ void __launch_bounds__(KBLK) k_residual(KryArgs<T> a, int mode) {
+// ---- true residual r = b - A x (FP64): MODE 0 initial, MODE 1 refinement ----
+template <class T, int MODE>
+__global__ void __launch_bounds__(BLK) k_residual(KryArgs<T> a) {
+    constexpr int NQ = MODE == 0 ? 18 : 12;
     bool on[6];
     bool any = false;
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
-        on[c] = a.valid[c] > 0 && (mode == 0 || a.cs[c].needRefine);
+        on[c] = a.valid[c] > 0 && (MODE == 0 || a.cs[c].needRefine);
         any |= on[c];
     }
     if (!any) return;
-    int32_t r0, r1;
-    krows(a.N, r0, r1);
-    double q[18];
+    double q[NQ];
 #pragma unroll
-    for (int i = 0; i < 18; ++i) q[i] = 0;
+    for (int i = 0; i < NQ; ++i) q[i] = 0;
     double xbar[6];
 #pragma unroll
-    for (int c = 0; c < 6; ++c) xbar[c] = a.cs[c].xbar;
-    for (int32_t r = r0 + threadIdx.x; r < r1; r += KBLK) {
+    for (int c = 0; c < 6; ++c) xbar[c] = MODE == 0 ? a.cs[c].xbar : 0.0;
+    for (int64_t it = 0;; ++it) {
+        const int64_t t0 = tile_of(it) * BLK;
+        if (t0 >= a.N) break;
+        const int32_t r = (int32_t)(t0 + threadIdx.x);
+        if (r >= a.N) continue;
         const double d = a.diag[r];
         double y[6], xv[6];
         ld6(a.x + (int64_t)NCMP * r, xv);
@@ -194,7 +78,7 @@ __global__ void __launch_bounds__(KBLK) k_residual(KryArgs<T> a, int mode) {
             const int64_t e = base + (int64_t)SLICE * k;
             const int32_t col = a.ellCol[e];
             const double av = a.aval[e];
-            sumA += av;
+            if (MODE == 0) sumA += av;
             double xc[6];
             ld6(a.x + (int64_t)NCMP * col, xc);
 #pragma unroll
@@ -211,32 +95,19 @@ __global__ void __launch_bounds__(KBLK) k_residual(KryArgs<T> a, int mode) {
             a.r0[(int64_t)NCMP * r + c] = rt;
             q[c] += fabs(res);
             q[6 + c] += (double)rt * (double)rt;
-            if (mode == 0) {
+            if (MODE == 0) {
                 const double pA = sumA * xbar[c];
                 q[12 + c] += fabs(y[c] - pA) + fabs(bb[c] - pA);
             }
         }
     }
-    double sums[18];
-    if (kreduce_last<18>(q, a.partials, a.counters + 0, sums) && threadIdx.x == 0) {
+    double sums[NQ];
+    if (reduce_last<NQ>(q, a.partials, a.counters + MODE, sums) && threadIdx.x == 0) {
         for (int c = 0; c < 6; ++c) {
             if (!on[c]) continue;
             CompState& cs = a.cs[c];
-            if (mode == 0) {
-                cs.normFactor = sums[12 + c] + 1e-20;
-                cs.initRes = sums[c] / cs.normFactor;
-                cs.finalRes = cs.initRes;
-                cs.iter = 0;
-                cs.nRefine = 0;
-                cs.solved = 1;
-                cs.converged = 0;
-                cs.needRefine = 0;
-                if (a.ctl.minIter > 0 || !conv_outer(cs, a.ctl, cs.initRes)) {
-                    start_inner(cs, a.ctl, sums[6 + c]);
-                } else {
-                    cs.active = 0;
-                    cs.converged = 1;
-                }
+            if (MODE == 0) {
+                finish_initial(cs, a.ctl, sums[c], sums[6 + c], sums[12 + c]);
             } else {
                 cs.needRefine = 0;
                 cs.nRefine += 1;
@@ -256,7 +127,7 @@ __global__ void __launch_bounds__(KBLK) k_residual(KryArgs<T> a, int mode) {
 
 // ---- p = r  |  p = r + beta (p - omega v) ---------------------------------
 template <class T>
-__global__ void __launch_bounds__(KBLK) k_kry_p(KryArgs<T> a) {
+__global__ void __launch_bounds__(BLK) k_kry_p(KryArgs<T> a) {
     bool on[6];
     bool first[6];
     T beta[6], omega[6];
@@ -271,7 +142,7 @@ __global__ void __launch_bounds__(KBLK) k_kry_p(KryArgs<T> a) {
     }
     if (!any) return;
     const int64_t N = a.N;
-    for (int64_t i = (int64_t)blockIdx.x * KBLK + threadIdx.x; i < N; i += (int64_t)gridDim.x * KBLK) {
+    for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < N; i += (int64_t)gridDim.x * BLK) {
         T rr[6], pp[6], vv[6];
         ld6(a.r + NCMP * i, rr);
         ld6(a.p + NCMP * i, pp);
@@ -285,7 +156,7 @@ __global__ void __launch_bounds__(KBLK) k_kry_p(KryArgs<T> a) {
 
 // ---- v = A M^-1 p = p + Ahat p ;  sigma = (r0, v) -------------------------
 template <class T>
-__global__ void __launch_bounds__(KBLK) k_kry_v(KryArgs<T> a) {
+__global__ void __launch_bounds__(BLK) k_kry_v(KryArgs<T> a) {
     bool on[6];
     bool any = false;
 #pragma unroll
@@ -294,10 +165,12 @@ __global__ void __launch_bounds__(KBLK) k_kry_v(KryArgs<T> a) {
         any |= on[c];
     }
     if (!any) return;
-    int32_t r0, r1;
-    krows(a.N, r0, r1);
-    double q[6] = {0, 0, 0, 0, 0, 0};
-    for (int32_t r = r0 + threadIdx.x; r < r1; r += KBLK) {
+    T q[6] = {0, 0, 0, 0, 0, 0};
+    for (int64_t it = 0;; ++it) {
+        const int64_t t0 = tile_of(it) * BLK;
+        if (t0 >= a.N) break;
+        const int32_t r = (int32_t)(t0 + threadIdx.x);
+        if (r >= a.N) continue;
         T acc[6];
         ld6(a.p + (int64_t)NCMP * r, acc);
         const int s = r >> 6, lane = r & 63;
@@ -317,10 +190,12 @@ __global__ void __launch_bounds__(KBLK) k_kry_v(KryArgs<T> a) {
         ld6(a.r0 + (int64_t)NCMP * r, rr0);
 #pragma unroll
         for (int c = 0; c < 6; ++c)
-            if (on[c]) q[c] += (double)rr0[c] * (double)acc[c];
+            if (on[c]) q[c] += rr0[c] * acc[c];
     }
-    double sums[6];
-    if (kreduce_last<6>(q, a.partials, a.counters + 1, sums) && threadIdx.x == 0) {
+    double qd[6], sums[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) qd[c] = q[c];
+    if (reduce_last<6>(qd, a.partials, a.counters + 2, sums) && threadIdx.x == 0) {
         for (int c = 0; c < 6; ++c)
             if (on[c]) a.cs[c].alpha = a.cs[c].rho / sums[c];
     }
@@ -328,7 +203,7 @@ __global__ void __launch_bounds__(KBLK) k_kry_v(KryArgs<T> a) {
 
 // ---- s = r - alpha v ; t = A M^-1 s ; |s|, (t,s), (t,t) --------------------
 template <class T>
-__global__ void __launch_bounds__(KBLK) k_kry_t(KryArgs<T> a) {
+__global__ void __launch_bounds__(BLK) k_kry_t(KryArgs<T> a) {
     bool on[6];
     T alpha[6];
     bool any = false;
@@ -339,12 +214,14 @@ __global__ void __launch_bounds__(KBLK) k_kry_t(KryArgs<T> a) {
         any |= on[c];
     }
     if (!any) return;
-    int32_t r0, r1;
-    krows(a.N, r0, r1);
-    double q[18];
+    T q[18];
 #pragma unroll
     for (int i = 0; i < 18; ++i) q[i] = 0;
-    for (int32_t r = r0 + threadIdx.x; r < r1; r += KBLK) {
+    for (int64_t it = 0;; ++it) {
+        const int64_t t0 = tile_of(it) * BLK;
+        if (t0 >= a.N) break;
+        const int32_t r = (int32_t)(t0 + threadIdx.x);
+        if (r >= a.N) continue;
         T sv[6], acc[6], rr[6], vv[6];
         ld6(a.r + (int64_t)NCMP * r, rr);
         ld6(a.v + (int64_t)NCMP * r, vv);
@@ -370,13 +247,15 @@ __global__ void __launch_bounds__(KBLK) k_kry_t(KryArgs<T> a) {
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
             if (!on[c]) continue;
-            q[c] += fabs((double)sv[c]);
-            q[6 + c] += (double)acc[c] * (double)sv[c];
-            q[12 + c] += (double)acc[c] * (double)acc[c];
+            q[c] += fabs(sv[c]);
+            q[6 + c] += acc[c] * sv[c];
+            q[12 + c] += acc[c] * acc[c];
         }
     }
-    double sums[18];
-    if (kreduce_last<18>(q, a.partials, a.counters + 2, sums) && threadIdx.x == 0) {
+    double qd[18], sums[18];
+#pragma unroll
+    for (int i = 0; i < 18; ++i) qd[i] = q[i];
+    if (reduce_last<18>(qd, a.partials, a.counters + 3, sums) && threadIdx.x == 0) {
         for (int c = 0; c < 6; ++c) {
             if (!on[c]) continue;
             CompState& cs = a.cs[c];
@@ -393,7 +272,7 @@ __global__ void __launch_bounds__(KBLK) k_kry_t(KryArgs<T> a) {
 
 // ---- x += alpha M^-1 p + omega M^-1 s ; r = s - omega t ; |r|, (r0, r) ------
 template <class T>
-__global__ void __launch_bounds__(KBLK) k_kry_xr(KryArgs<T> a) {
+__global__ void __launch_bounds__(BLK) k_kry_xr(KryArgs<T> a) {
     bool on[6], ex[6];
     double alpha[6], omega[6];
     bool any = false;
@@ -406,12 +285,14 @@ __global__ void __launch_bounds__(KBLK) k_kry_xr(KryArgs<T> a) {
         any |= on[c];
     }
     if (!any) return;
-    int32_t r0, r1;
-    krows(a.N, r0, r1);
-    double q[12];
+    T q[12];
 #pragma unroll
     for (int i = 0; i < 12; ++i) q[i] = 0;
-    for (int32_t r = r0 + threadIdx.x; r < r1; r += KBLK) {
+    for (int64_t it = 0;; ++it) {
+        const int64_t t0 = tile_of(it) * BLK;
+        if (t0 >= a.N) break;
+        const int32_t r = (int32_t)(t0 + threadIdx.x);
+        if (r >= a.N) continue;
         const double rd = a.rD[r];
         const int64_t j = (int64_t)NCMP * r;
         double xv[6];
@@ -435,14 +316,16 @@ __global__ void __launch_bounds__(KBLK) k_kry_xr(KryArgs<T> a) {
             xv[c] += alpha[c] * y + omega[c] * z;
             const T rn = sv - (T)omega[c] * tt[c];
             rr[c] = rn;
-            q[c] += fabs((double)rn);
-            q[6 + c] += (double)rr0[c] * (double)rn;
+            q[c] += fabs(rn);
+            q[6 + c] += rr0[c] * rn;
         }
         st6(a.x + j, xv);
         st6(a.r + j, rr);
     }
-    double sums[12];
-    if (kreduce_last<12>(q, a.partials, a.counters + 3, sums) && threadIdx.x == 0) {
+    double qd[12], sums[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) qd[i] = q[i];
+    if (reduce_last<12>(qd, a.partials, a.counters + 4, sums) && threadIdx.x == 0) {
         for (int c = 0; c < 6; ++c) {
             if (!on[c]) continue;
             CompState& cs = a.cs[c];
@@ -484,18 +367,57 @@ __global__ void __launch_bounds__(KBLK) k_kry_xr(KryArgs<T> a) {
 
 // ---- Ahat = offdiag(A) D^-1 (column scaled), rD = 1/diag ---------------------
 __global__ void k_rdiag(int32_t N, const double* __restrict__ diag, double* __restrict__ rD) {
-    const int64_t r = (int64_t)blockIdx.x * KBLK + threadIdx.x;
+    const int64_t r = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (r < N) rD[r] = 1.0 / diag[r];
 }
 template <class T>
 __global__ void k_scale(int64_t nEll, const int32_t* __restrict__ col, const double* __restrict__ aval,
                         const double* __restrict__ rD, T* __restrict__ ahat) {
-    const int64_t e = (int64_t)blockIdx.x * KBLK + threadIdx.x;
+    const int64_t e = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (e < nEll) ahat[e] = (T)(aval[e] * rD[col[e]]);
 }
 
+// ---- mean of x per component (normalisation factor's gAverage(psi)) ---------
+__global__ void __launch_bounds__(BLK) k_mean6(int32_t N, const double* __restrict__ x, double* partials,
+                                               int32_t* counter, CompState* cs) {
+    double q[6] = {0, 0, 0, 0, 0, 0};
+    for (int64_t it = 0;; ++it) {
+        const int64_t t0 = tile_of(it) * BLK;
+        if (t0 >= N) break;
+        const int64_t r = t0 + threadIdx.x;
+        if (r >= N) continue;
+        double v[6];
+        ld6(x + NCMP * r, v);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) q[c] += v[c];
+    }
+    double sums[6];
+    if (reduce_last<6>(q, partials, counter, sums) && threadIdx.x == 0)
+        for (int c = 0; c < 6; ++c) cs[c].xbar = sums[c] / N;
+}
+
 // ----------------------------------------------------------------- driver
-static inline unsigned kblk(int64_t n) { return (unsigned)((n + KBLK - 1) / KBLK); }
+static inline unsigned kblk(int64_t n) { return (unsigned)((n + BLK - 1) / BLK); }
+
+void launch_mean6(Ctx* ctx, int32_t N, const double* x, double* partials, int32_t* counter, CompState* cs,
+                  int grid) {
+    ProfScope ps(ctx, "k_mean6", (double)N * 48);
+    hipLaunchKernelGGL(k_mean6, dim3(grid), dim3(BLK), 0, ctx->stream, N, x, partials, counter, cs);
+    RT_HIP(hipGetLastError());
+}
+
+KryCtl kryCtl(const Controls& C, bool mixed) {
+    KryCtl k{};
+    k.tol = C.tolerance;
+    k.relTol = C.relTol;
+    k.innerRelTol = C.innerRelTol;
+    k.maxIter = C.maxIter;
+    k.minIter = C.minIter;
+    k.maxInner = 200;
+    k.maxRefine = C.maxRefine;
+    k.mixed = mixed ? 1 : 0;
+    return k;
+}
 
 void Krylov::ensure(Mesh& m, bool mixed) {
     if (!ctx) ctx = m.ctx;
@@ -523,11 +445,11 @@ void Krylov::ensure(Mesh& m, bool mixed) {
     }
 }
 
-// Resumable solve: begin() launches the initial residual, every advance()
-// reads the device state of the previous round (non-blocking when asked) and
-// launches the next round, so several solves can progress on their own
-// streams at once (the serving loop) or one can run to completion
-// (rt_model_correct / rt_symm_solve).
+// Resumable solve: begin() launches the initial residual (unless the assembly
+// already produced it), every advance() reads the device state of the
+// previous round (non-blocking when asked) and launches the next round, so
+// several solves can progress on their own streams at once (the serving loop)
+// or one can run to completion (rt_model_correct / rt_symm_solve).
 struct SolveRun {
     bool mixed = false;
     KryArgs<float> af{};
@@ -564,14 +486,15 @@ void fill_args(Ctx* ctx, const SolveArgs& sa, SolveRun& R) {
     const int32_t N = m.N;
     SolveState* st = K.state.as<SolveState>();
     double* rD = K.rD.as<double>();
-    {
+    if (!sa.initDone) {
         ProfScope ps(ctx, "k_rdiag", (double)N * 16);
-        hipLaunchKernelGGL(k_rdiag, dim3(kblk(N)), dim3(KBLK), 0, ctx->stream, N, sa.diag, rD);
+        hipLaunchKernelGGL(k_rdiag, dim3(kblk(N)), dim3(BLK), 0, ctx->stream, N, sa.diag, rD);
     }
     T* ahat = mixed ? (T*)K.ahat32.p : (T*)K.ahat64.p;
     if (m.nEll > 0) {
-        ProfScope ps(ctx, "k_scale", (double)m.nEll * (4 + 8 + sizeof(T)) + (double)N * 8);
-        hipLaunchKernelGGL(k_scale<T>, dim3(kblk(m.nEll)), dim3(KBLK), 0, ctx->stream, m.nEll,
+        ProfScope ps(ctx, mixed ? "k_scale<f32>" : "k_scale<f64>",
+                     (double)m.nEll * (4 + 8 + sizeof(T)) + (double)N * 8);
+        hipLaunchKernelGGL(k_scale<T>, dim3(kblk(m.nEll)), dim3(BLK), 0, ctx->stream, m.nEll,
                            m.ellCol.as<int32_t>(), sa.aval, rD, ahat);
     }
     RT_HIP(hipGetLastError());
@@ -596,20 +519,12 @@ void fill_args(Ctx* ctx, const SolveArgs& sa, SolveRun& R) {
     a.counters = st->counter;
     a.cs = st->c;
     for (int c = 0; c < 6; ++c) a.valid[c] = sa.valid[c];
-    const Controls& C = *sa.ctl;
-    a.ctl.tol = C.tolerance;
-    a.ctl.relTol = C.relTol;
-    a.ctl.innerRelTol = C.innerRelTol;
-    a.ctl.maxIter = C.maxIter;
-    a.ctl.minIter = C.minIter;
-    a.ctl.maxInner = 200;
-    a.ctl.maxRefine = C.maxRefine;
-    a.ctl.mixed = mixed ? 1 : 0;
+    a.ctl = kryCtl(*sa.ctl, mixed);
 }
 
 // algorithmic bytes per launch for nc participating components: ELL entries
-// (col + value) once per SpMV, each vector once per pass (gathered vectors
-// counted once: neighbour values are reused from cache)
+// (col + value) once per SpMV, each vector once per pass (interleaved rows
+// move all 6 components; gathered vectors counted once)
 struct Bytes {
     double N, ne, w;
     double res(double nc) const { return N * 8.0 + ne * 12 + nc * N * (8 + 8 + 2 * w); }
@@ -624,7 +539,7 @@ void launch_init(Ctx* ctx, SolveRun& R) {
     const bool mixed = std::is_same<T, float>::value;
     Bytes B{R.N, R.ne, (double)sizeof(T)};
     ProfScope ps(ctx, mixed ? "k_residual<f32>" : "k_residual<f64>", B.res(R.nValid));
-    hipLaunchKernelGGL(k_residual<T>, dim3(R.G), dim3(KBLK), 0, ctx->stream, argsOf<T>(R), 0);
+    hipLaunchKernelGGL((k_residual<T, 0>), dim3(R.G), dim3(BLK), 0, ctx->stream, argsOf<T>(R));
 }
 
 template <class T>
@@ -634,24 +549,24 @@ void launch_round(Ctx* ctx, SolveRun& R, double nAct, double nRef) {
     KryArgs<T>& a = argsOf<T>(R);
     if (nRef > 0) {
         ProfScope ps(ctx, mixed ? "k_residual<f32>" : "k_residual<f64>", B.res(nRef));
-        hipLaunchKernelGGL(k_residual<T>, dim3(R.G), dim3(KBLK), 0, ctx->stream, a, 1);
+        hipLaunchKernelGGL((k_residual<T, 1>), dim3(R.G), dim3(BLK), 0, ctx->stream, a);
         nAct += nRef;  // components restarted by the refinement sweep join this round
     }
     {
         ProfScope ps(ctx, mixed ? "k_kry_p<f32>" : "k_kry_p<f64>", B.p(nAct));
-        hipLaunchKernelGGL(k_kry_p<T>, dim3(R.pgrid), dim3(KBLK), 0, ctx->stream, a);
+        hipLaunchKernelGGL(k_kry_p<T>, dim3(R.pgrid), dim3(BLK), 0, ctx->stream, a);
     }
     {
         ProfScope ps(ctx, mixed ? "k_kry_v<f32>" : "k_kry_v<f64>", B.v(nAct));
-        hipLaunchKernelGGL(k_kry_v<T>, dim3(R.G), dim3(KBLK), 0, ctx->stream, a);
+        hipLaunchKernelGGL(k_kry_v<T>, dim3(R.G), dim3(BLK), 0, ctx->stream, a);
     }
     {
         ProfScope ps(ctx, mixed ? "k_kry_t<f32>" : "k_kry_t<f64>", B.t(nAct));
-        hipLaunchKernelGGL(k_kry_t<T>, dim3(R.G), dim3(KBLK), 0, ctx->stream, a);
+        hipLaunchKernelGGL(k_kry_t<T>, dim3(R.G), dim3(BLK), 0, ctx->stream, a);
     }
     {
         ProfScope ps(ctx, mixed ? "k_kry_xr<f32>" : "k_kry_xr<f64>", B.xr(nAct));
-        hipLaunchKernelGGL(k_kry_xr<T>, dim3(R.G), dim3(KBLK), 0, ctx->stream, a);
+        hipLaunchKernelGGL(k_kry_xr<T>, dim3(R.G), dim3(BLK), 0, ctx->stream, a);
     }
 }
 
@@ -689,10 +604,10 @@ void solve_begin(Ctx* ctx, const SolveArgs& sa) {
     R.rounds = 0;
     if (mixed) {
         fill_args<float>(ctx, sa, R);
-        launch_init<float>(ctx, R);
+        if (!sa.initDone) launch_init<float>(ctx, R);
     } else {
         fill_args<double>(ctx, sa, R);
-        launch_init<double>(ctx, R);
+        if (!sa.initDone) launch_init<double>(ctx, R);
     }
     RT_HIP(hipGetLastError());
     queue_pull(ctx, K, R);
@@ -752,7 +667,7 @@ void solve_lockstep(Ctx* ctx, const SolveArgs& a, rt_solver_perf perf[6]) {
 }
 
 int grid_for(const Ctx* ctx, int32_t N) {
-    const int64_t need = ((int64_t)N + KBLK - 1) / KBLK;
+    const int64_t need = ((int64_t)N + BLK - 1) / BLK;
     const int64_t cap = (int64_t)ctx->numCUs * 8;
     int64_t g = need < cap ? need : cap;
     if (g >= 64) g = g / 8 * 8;

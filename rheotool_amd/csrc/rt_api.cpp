@@ -274,6 +274,10 @@ void modelCorrectBegin(Model& M, double deltaT, const double* gradU) {
         launch_aos_to_soa(ctx, N, 9, st.as<double>(), m.permD.as<int32_t>(), M.gradU.as<double>());
     }
     SolveState* st = M.kry.state.as<SolveState>();
+    const bool mixed = M.ctl.mixed;
+    // gAverage(theta) for the normalisation factor of the initial residual
+    launch_mean6(ctx, N, M.theta.as<double>(), M.kry.partials.as<double>(), st->counter + 6, st->c,
+                 M.kry.grid);
     AssembleArgs a{};
     a.N = N;
     a.ellOff = m.ellOff.as<int64_t>();
@@ -309,9 +313,14 @@ void modelCorrectBegin(Model& M, double deltaT, const double* gradU) {
     a.b = M.b.as<double>();
     a.fOld = M.prm.kind == M_FENEP ? M.fOld.as<double>() : nullptr;
     a.partials = M.kry.partials.as<double>();
-    a.counter = st->counter + 4;
+    a.counter = st->counter + 5;
     a.cs = st->c;
     for (int q = 0; q < 6; ++q) a.valid[q] = m.valid[q];
+    a.r = mixed ? M.kry.rs.p : M.kry.r.p;
+    a.r0 = mixed ? M.kry.r0s.p : M.kry.r0.p;
+    a.rD = M.kry.rD.as<double>();
+    a.ctl = kryCtl(M.ctl, mixed);
+    a.mixed = mixed ? 1 : 0;
     launch_assemble(ctx, a, M.kry.grid);
     if (!M.pendingSolve) M.pendingSolve = new SolveArgs{};
     SolveArgs& sa = *M.pendingSolve;
@@ -325,6 +334,7 @@ void modelCorrectBegin(Model& M, double deltaT, const double* gradU) {
     sa.valid = m.valid;
     sa.ctl = &M.ctl;
     sa.xbarKnown = true;
+    sa.initDone = true;
     solve_begin(ctx, sa);
 }
 
@@ -345,13 +355,12 @@ void modelCorrect(Model& M, double deltaT, const double* gradU) {
     RT_HIP(hipStreamSynchronize(M.ctx->stream));
 }
 
+// device bytes of one model on an existing mesh (state, flow, matrix, Krylov)
 size_t modelBytesEstimate(int32_t N, int32_t F, int32_t NB, int64_t nEll, bool mixed) {
     const double n = N, f = F, nb = NB, ne = (double)nEll;
     double by = 0;
-    // mesh: ELL col+fid, Sf, w, V, perm, boundary lists
-    by += ne * (4 + 4 + 1 + 24 + 8 + 8) + 3 * 8 * nb + 8 * n + 4 * n + 4 * (n + 1) + 12 * nb;
     // model state + flow + matrix: theta tau eigVal eigVec U b diag fOld, Ub, phi, bc, aval
-    by += 8 * n * (6 + 6 + 3 + 9 + 3 + 6 + 1 + 1) + 8 * (3 * nb + f + nb) + 6 * 8 * nb + 8 * ne;
+    by += 8 * n * (6 + 6 + 3 + 9 + 4 + 6 + 1 + 1) + 8 * (3 * nb + f + nb) + 6 * 8 * nb + 8 * ne + 8 * ne;
     // Krylov: 5 vectors x 6 components, rD, ahat, partials, state
     by += 6 * n * 5 * (mixed ? 4 : 8) + 8 * n + ne * (mixed ? 4 : 8) + 2048 * 24 * 8 + 4096;
     // staging for AoS transfers (9 comps)
@@ -466,6 +475,28 @@ int rt_ctx_kernel_stats(rt_ctx* ctx, rt_kernel_stat* out, int cap) {
         }
     });
     return rc == RT_OK ? n : -rc;
+}
+
+int rt_ctx_copy_bench(rt_ctx* ctx, size_t bytes, int reps, double* ms) {
+    if (!ctx || !ms || bytes < 16 || reps <= 0) return RT_E_BAD_ARG;
+    return guarded(&ctx->impl, [&] {
+        Ctx& c = ctx->impl;
+        RT_HIP(hipSetDevice(c.device));
+        bytes = bytes / 16 * 16;
+        DBuf a = c.alloc(bytes), b = c.alloc(bytes);
+        RT_HIP(hipMemsetAsync(a.p, 0, bytes, c.stream));
+        launch_copy(&c, a.p, b.p, bytes);  // warm-up
+        hipEvent_t e0 = c.getEvent(), e1 = c.getEvent();
+        RT_HIP(hipEventRecord(e0, c.stream));
+        for (int i = 0; i < reps; ++i) launch_copy(&c, a.p, b.p, bytes);
+        RT_HIP(hipEventRecord(e1, c.stream));
+        RT_HIP(hipEventSynchronize(e1));
+        float t = 0;
+        RT_HIP(hipEventElapsedTime(&t, e0, e1));
+        c.eventPool.push_back(e0);
+        c.eventPool.push_back(e1);
+        *ms = t / reps;
+    });
 }
 
 void rt_ctx_reset_stats(rt_ctx* ctx) {
@@ -772,6 +803,7 @@ int rt_symm_solve(rt_ctx* ctx, const rt_mesh* mesh, const rt_ldu_symm* sys, doub
         sa.valid = m.valid;
         sa.ctl = &ctl;
         sa.xbarKnown = true;
+        sa.initDone = false;
         solve_lockstep(c, sa, perf);
         launch_unpermute_rows(c, N, 6, 6, dX.as<double>(), m.permD.as<int32_t>(), st.as<double>());
         RT_HIP(hipMemcpyAsync(psi, st.p, (size_t)6 * N * 8, hipMemcpyDeviceToHost, c->stream));

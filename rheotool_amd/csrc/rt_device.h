@@ -1,0 +1,143 @@
+// rt_device.h -- device helpers shared by the cell and Krylov kernels (gfx950).
+#pragma once
+
+#include "rt_internal.h"
+
+namespace rt {
+
+constexpr int BLK = 256;
+
+// Row tiles of BLK rows, interleaved over the workgroups so that at any time
+// the grid works inside one window of gridDim.x tiles: neighbour rows of a
+// tile were touched moments ago by other workgroups and are still in L2/MALL.
+// Inside a window, the tiles of one XCD (workgroups are dealt round-robin to
+// the 8 XCDs) are consecutive, so neighbour reuse mostly stays in one L2.
+__device__ __forceinline__ int64_t tile_of(int64_t i) {
+    const int G = gridDim.x;
+    const int b = blockIdx.x;
+    const int j = (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;
+    return i * G + j;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Block reduction of NQ values per thread into partials[block][NQ]; the last
+// block to finish reduces all blocks in a fixed order (deterministic) and
+// returns true with sums[] filled (thread 0 valid).
+template <int NQ>
+__device__ bool reduce_last(double (&v)[NQ], double* partials, int32_t* counter, double (&sums)[NQ]) {
+    __shared__ double red[NQ][BLK / 64];
+    __shared__ int amLast;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        double s = wave_sum(v[q]);
+        if (lane == 0) red[q][wid] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < NQ) {
+        double s = 0;
+        for (int w = 0; w < BLK / 64; ++w) s += red[threadIdx.x][w];
+        partials[(int64_t)blockIdx.x * NQ + threadIdx.x] = s;
+    }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) amLast = atomicAdd(counter, 1) == (int)gridDim.x - 1;
+    __syncthreads();
+    if (!amLast) return false;
+    __threadfence();
+    const volatile double* vp = partials;
+    double acc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) acc[q] = 0;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += BLK)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) acc[q] += vp[(int64_t)i * NQ + q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        double s = wave_sum(acc[q]);
+        if (lane == 0) red[q][wid] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            double s = 0;
+            for (int w = 0; w < BLK / 64; ++w) s += red[q][w];
+            sums[q] = s;
+        }
+        *counter = 0;
+    }
+    return true;
+}
+
+// interleaved [N][6] rows: three 2-wide vector loads/stores per row
+template <class T> struct Vec2;
+template <> struct Vec2<float> { typedef float2 type; };
+template <> struct Vec2<double> { typedef double2 type; };
+
+template <class T>
+__device__ __forceinline__ void ld6(const T* p, T* v) {
+    typedef typename Vec2<T>::type V2;
+    const V2* q = reinterpret_cast<const V2*>(p);
+    const V2 a = q[0], b = q[1], c = q[2];
+    v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y; v[4] = c.x; v[5] = c.y;
+}
+template <class T>
+__device__ __forceinline__ void st6(T* p, const T* v) {
+    typedef typename Vec2<T>::type V2;
+    V2* q = reinterpret_cast<V2*>(p);
+    V2 a, b, c;
+    a.x = v[0]; a.y = v[1]; b.x = v[2]; b.y = v[3]; c.x = v[4]; c.y = v[5];
+    q[0] = a; q[1] = b; q[2] = c;
+}
+
+// ------------------------------------------------- Krylov control (OpenFOAM)
+// SolverPerformance::checkConvergence: final < tolerance, or relTol > 0 and
+// final < relTol * initial
+__device__ __forceinline__ bool conv_outer(const CompState& c, const KryCtl& k, double res) {
+    return res < k.tol || (k.relTol > 0 && res < k.relTol * c.initRes);
+}
+__device__ __forceinline__ bool conv_inner(const CompState& c, const KryCtl& k, double res) {
+    return k.mixed ? res < c.innerTarget : conv_outer(c, k, res);
+}
+
+// begin an inner Krylov solve on a component whose r, r0 were just set
+__device__ __forceinline__ void start_inner(CompState& c, const KryCtl& k, double rr) {
+    c.innerIter = 0;
+    c.rho = rr;
+    c.exitAtS = 0;
+    if (k.mixed) c.innerTarget = fmax(k.innerRelTol * c.finalRes, 0.3 * k.tol);
+    if (fabs(rr) < 1e-300) {  // singular: nothing to iterate on
+        c.active = 0;
+        c.converged = conv_outer(c, k, c.finalRes);
+        return;
+    }
+    c.active = 1;
+}
+
+// after the initial residual: normalisation factor, initial residual, start
+__device__ __forceinline__ void finish_initial(CompState& cs, const KryCtl& k, double absR, double rr,
+                                               double nrm) {
+    cs.normFactor = nrm + 1e-20;
+    cs.initRes = absR / cs.normFactor;
+    cs.finalRes = cs.initRes;
+    cs.iter = 0;
+    cs.nRefine = 0;
+    cs.solved = 1;
+    cs.converged = 0;
+    cs.needRefine = 0;
+    if (k.minIter > 0 || !conv_outer(cs, k, cs.initRes)) {
+        start_inner(cs, k, rr);
+    } else {
+        cs.active = 0;
+        cs.converged = 1;
+    }
+}
+
+}  // namespace rt

@@ -173,6 +173,13 @@ struct alignas(64) CompState {
 };
 static_assert(sizeof(CompState) % 8 == 0, "CompState layout");
 
+// controls the device side of the solve needs
+struct KryCtl {
+    double tol, relTol, innerRelTol;
+    int32_t maxIter, minIter, maxInner, maxRefine;
+    int32_t mixed;
+};
+
 struct SolveState {
     CompState c[6];
     int32_t anyActive;
@@ -272,10 +279,16 @@ struct AssembleArgs {
     double* diag;
     double* b;            // [N][6]
     double* fOld;         // FENE-P f, nullable
-    double* partials;     // [grid][6] sums of theta (mean for normFactor)
+    double* partials;     // [grid][18] residual sums
     int32_t* counter;
     CompState* cs;
     int32_t valid[6];
+    // initial residual of the solve, fused into the assembly
+    void* r;              // [N][6] Krylov precision
+    void* r0;
+    double* rD;           // [N] 1/diag
+    KryCtl ctl;
+    int32_t mixed;        // r/r0 in FP32
 };
 
 void launch_assemble(Ctx* ctx, const AssembleArgs& a, int grid);
@@ -299,6 +312,7 @@ void launch_eigvals_from_tensor(Ctx* ctx, int32_t N, const double* aos9, const i
 void launch_eigvals_to_tensor(Ctx* ctx, int32_t N, const double* eigVal, const int32_t* perm,
                               double* aos9);
 void launch_fill(Ctx* ctx, double* p, int64_t n, double v);
+void launch_copy(Ctx* ctx, const void* src, void* dst, size_t bytes);
 
 // user-supplied LDU -> ELL values and diagonal (the sparseSolver surface)
 void launch_ldu_to_ell(Ctx* ctx, const Mesh& m, const double* lower, const double* upper,
@@ -313,8 +327,13 @@ struct SolveArgs {
     double* x;           // [N][6] (theta in place)
     const int32_t* valid;
     const Controls* ctl;
-    bool xbarKnown;      // CompState.xbar already set (by the assemble kernel)
+    bool xbarKnown;      // CompState.xbar already set
+    bool initDone;       // r, r0, rD and the initial state already written (by the assembly)
 };
+KryCtl kryCtl(const Controls& C, bool mixed);
+// mean of theta per component -> CompState.xbar (gAverage(psi) of the normalisation)
+void launch_mean6(Ctx* ctx, int32_t N, const double* x, double* partials, int32_t* counter,
+                  CompState* cs, int grid);
 // Lockstep PBiCGStab over the valid components; fills perf.
 void solve_lockstep(Ctx* ctx, const SolveArgs& a, rt_solver_perf perf[6]);
 // The same as a resumable job: begin launches the initial residual;

@@ -19,18 +19,15 @@ namespace rt {
 
 struct Request {
     int64_t ticket = 0;
-    // host copies of the submission
-    std::vector<int32_t> owner, neighbour;
-    std::vector<double> Sf, w, V;
-    std::vector<rt_patch> patches;
-    rt_mesh_desc desc{};
+    // host copies of the submission (the mesh is the caller's uploaded handle)
+    const rt_mesh* mesh = nullptr;
+    int32_t nCells = 0;
     std::string dict, controls;
     std::vector<double> theta, eigVals, eigVecs, U, Ub, phi;
     double deltaT = 0;
     size_t estBytes = 0;
     // runtime
     int slot = -1;
-    rt_mesh* mesh = nullptr;
     rt_model* model = nullptr;
     enum { QUEUED, RUNNING, FINISHING, DONE, FAILED } state = QUEUED;
     hipEvent_t done = nullptr;
@@ -61,9 +58,7 @@ namespace {
 
 void releaseRequest(rt_server* S, Request* r) {
     if (r->model) rt_model_destroy(r->model);
-    if (r->mesh) rt_mesh_destroy(r->mesh);
     r->model = nullptr;
-    r->mesh = nullptr;
     if (r->done) (void)hipEventDestroy(r->done);
     r->done = nullptr;
     if (r->slot >= 0) {
@@ -72,9 +67,7 @@ void releaseRequest(rt_server* S, Request* r) {
         r->slot = -1;
     }
     // the submission copies are no longer needed
-    std::vector<int32_t>().swap(r->owner);
-    std::vector<int32_t>().swap(r->neighbour);
-    for (auto* v : {&r->Sf, &r->w, &r->V, &r->theta, &r->eigVals, &r->eigVecs, &r->U, &r->Ub, &r->phi})
+    for (auto* v : {&r->theta, &r->eigVals, &r->eigVecs, &r->U, &r->Ub, &r->phi})
         std::vector<double>().swap(*v);
 }
 
@@ -92,8 +85,7 @@ void admit(rt_server* S, Request* r, int slot) {
     r->slot = slot;
     S->slotReq[slot] = r;
     S->inflightBytes += r->estBytes;
-    int rc = rt_mesh_create(c, &r->desc, RT_RENUMBER_AUTO, &r->mesh);
-    if (rc == RT_OK) rc = rt_model_create(c, "", r->mesh, r->dict.c_str(), &r->model);
+    int rc = rt_model_create(c, "", r->mesh, r->dict.c_str(), &r->model);
     if (rc == RT_OK) rc = rt_model_set_solver(r->model, r->controls.c_str());
     if (rc == RT_OK)
         rc = rt_model_set_state(r->model, r->theta.data(), r->eigVals.empty() ? nullptr : r->eigVals.data(),
@@ -128,7 +120,7 @@ void advance(rt_server* S, Request* r) {
             hipError_t e = hipEventQuery(r->done);
             if (e == hipErrorNotReady) return;
             RT_HIP(e);
-            const size_t N = (size_t)r->desc.nCells;
+            const size_t N = (size_t)r->nCells;
             r->outTheta.resize(6 * N);
             r->outTau.resize(6 * N);
             int rc = rt_model_get(r->model, RT_THETA, r->outTheta.data());
@@ -194,27 +186,16 @@ void rt_server_destroy(rt_server* S) {
 int rt_server_submit(rt_server* S, const rt_request* q, int64_t* ticket) {
     if (!S || !q || !ticket) return RT_E_BAD_ARG;
     rt_ctx* P = S->parent;
-    const rt_mesh_desc* d = q->mesh;
-    if (!d || !q->dict || !q->theta || !q->U || !q->phi || d->nCells <= 0 || !(q->deltaT > 0) ||
-        (d->nBoundaryFaces > 0 && !q->Ub)) {
+    if (!q->mesh || !q->dict || !q->theta || !q->U || !q->phi || !(q->deltaT > 0) ||
+        (q->mesh->impl.NB > 0 && !q->Ub)) {
         P->impl.lastError = "rt_server_submit: incomplete request";
         return RT_E_BAD_ARG;
     }
     auto r = std::make_unique<Request>();
-    const size_t N = d->nCells, F = d->nInternalFaces, NB = d->nBoundaryFaces, NF = F + NB;
-    r->owner.assign(d->owner, d->owner + NF);
-    r->neighbour.assign(d->neighbour, d->neighbour + F);
-    r->Sf.assign(d->Sf, d->Sf + 3 * NF);
-    r->w.assign(d->weights, d->weights + F);
-    r->V.assign(d->V, d->V + N);
-    r->patches.assign(d->patches, d->patches + d->nPatches);
-    r->desc = *d;
-    r->desc.owner = r->owner.data();
-    r->desc.neighbour = r->neighbour.data();
-    r->desc.Sf = r->Sf.data();
-    r->desc.weights = r->w.data();
-    r->desc.V = r->V.data();
-    r->desc.patches = r->patches.data();
+    const Mesh& mm = q->mesh->impl;
+    const size_t N = mm.N, F = mm.F, NB = mm.NB, NF = F + NB;
+    r->mesh = q->mesh;
+    r->nCells = mm.N;
     r->dict = q->dict;
     r->controls = q->controls ? q->controls : S->defaultControls;
     r->theta.assign(q->theta, q->theta + 6 * N);
@@ -224,9 +205,8 @@ int rt_server_submit(rt_server* S, const rt_request* q, int64_t* ticket) {
     if (NB) r->Ub.assign(q->Ub, q->Ub + 3 * NB);
     r->phi.assign(q->phi, q->phi + NF);
     r->deltaT = q->deltaT;
-    // nEll upper bound: 2F entries plus padding of one slice width per slice
     const bool mixed = r->controls.find("mixed") != std::string::npos;
-    r->estBytes = modelBytesEstimate((int32_t)N, (int32_t)F, (int32_t)NB, (int64_t)(2 * F + 16 * N), mixed);
+    r->estBytes = modelBytesEstimate((int32_t)N, (int32_t)F, (int32_t)NB, mm.nEll, mixed);
     if (S->budget && r->estBytes > S->budget) {
         P->impl.lastError = "rt_server_submit: request needs " + std::to_string(r->estBytes) +
                             " bytes, more than the whole HBM budget";

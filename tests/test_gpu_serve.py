@@ -49,10 +49,11 @@ def test_same_result_alone_and_in_mix(precision):
         alone = [run_alone(ctx, r, controls) for r in reqs]
         srv = abi.Server(ctx, hbm_budget=0, max_inflight=4)
         tickets = {}
+        meshes = [ctx.mesh(r[0]) for r in reqs]
         # staggered arrivals: submit some, progress, submit more while running
         for i, r in enumerate(reqs):
             m, f, model, lam, kw, dt = r
-            tickets[i] = srv.submit(m, f, abi.model_dict(model, lam=lam, **kw), dt, controls)
+            tickets[i] = srv.submit(meshes[i], f, abi.model_dict(model, lam=lam, **kw), dt, controls)
             if i % 3 == 2:
                 for _ in range(3):
                     srv.progress()
@@ -73,7 +74,9 @@ def test_server_results_match_oracle():
     with abi.Context(0) as ctx:
         srv = abi.Server(ctx, max_inflight=3)
         reqs = [make_request(i) for i in range(4)]
-        t = [srv.submit(r[0], r[1], abi.model_dict(r[2], lam=r[3], **r[4]), r[5]) for r in reqs]
+        meshes = [ctx.mesh(r[0]) for r in reqs]
+        t = [srv.submit(dm, r[1], abi.model_dict(r[2], lam=r[3], **r[4]), r[5])
+             for dm, r in zip(meshes, reqs)]
         while srv.progress() > 0:
             pass
         for k, r in zip(t, reqs):
@@ -92,8 +95,32 @@ def test_budget_gates_admission():
     with abi.Context(0) as ctx:
         reqs = [make_request(i) for i in range(3)]
         srv = abi.Server(ctx, hbm_budget=1 << 20, max_inflight=2)  # 1 MiB: nothing fits
+        dm = ctx.mesh(reqs[0][0])
         with pytest.raises(abi.RheoError) as e:
-            srv.submit(reqs[0][0], reqs[0][1], abi.model_dict(reqs[0][2], lam=reqs[0][3], **reqs[0][4]),
+            srv.submit(dm, reqs[0][1], abi.model_dict(reqs[0][2], lam=reqs[0][3], **reqs[0][4]),
                        reqs[0][5])
         assert e.value.code == abi.RT_E_OOM
         srv.close()
+
+
+def test_shared_mesh_requests_and_queueing():
+    """Several requests on one uploaded mesh; two slots admit two at a time,
+    the rest queue and still finish with unchanged results."""
+    with abi.Context(0) as ctx:
+        m, f, model, lam, kw, dt = make_request(1)
+        dm = ctx.mesh(m)
+        d = abi.model_dict(model, lam=lam, **kw)
+        alone = run_alone(ctx, (m, f, model, lam, kw, dt), abi.solver_dict(1e-10, precision="mixed"))
+        probe = abi.Server(ctx, max_inflight=1)
+        t0 = probe.submit(dm, f, d, dt)
+        while probe.progress() > 0:
+            pass
+        assert probe.poll(t0) is not None
+        probe.close()
+        srv = abi.Server(ctx, hbm_budget=200 << 20, max_inflight=2)
+        ts = [srv.submit(dm, f, d, dt) for _ in range(5)]
+        while srv.progress() > 0:
+            pass
+        for t in ts:
+            out = srv.poll(t)
+            assert np.array_equal(out["theta"], alone[0])
