@@ -39,6 +39,11 @@ WORKLOADS = {
                       desc="256^3 box, cells permuted (seed 3), nodes jittered +-10%, FENE-P-Log "
                            "L2=100 Wi=5 beta=0.59 dt=5e-4; FP32 Krylov + FP64 refinement to "
                            "1e-10; FP64 eigen/log/exp"),
+    "precision_natural": dict(n=256, perm_seed=0, jitter=0.1, model="FENE-PLog", kw=dict(L2=100.0),
+                              lam=5.0, dt=5e-4, ln=(-8.0, float(np.log(30.0))), precision="mixed",
+                              desc="256^3 box in natural (lexicographic) cell order, nodes jittered "
+                                   "+-10%, FENE-P-Log L2=100 Wi=5 beta=0.59 dt=5e-4; FP32 Krylov + "
+                                   "FP64 refinement to 1e-10; FP64 eigen/log/exp"),
     "dropin": dict(n=128, perm_seed=0, jitter=0.0, model="GiesekusLog", kw=dict(alpha=0.1),
                    lam=10.0, dt=5e-4, ln=(-9.0, 5.0), precision="double",
                    desc="128^3 box, Taylor-Green + shear, Giesekus-Log alpha=0.1 Wi=10 dt=5e-4; "

@@ -335,11 +335,15 @@ __global__ void __launch_bounds__(BLK) k_assemble(AssembleArgs a) {
         st6(rOut + (int64_t)NCMP * r, rt);
         st6(r0Out + (int64_t)NCMP * r, rt);
     }
+    write_partials<18>(q, a.partials);
+}
+
+__global__ void __launch_bounds__(BLK) k_fin_assemble(AssembleArgs a, int G) {
     double sums[18];
-    if (reduce_last<18>(q, a.partials, a.counter, sums) && threadIdx.x == 0) {
+    sum_partials<18>(a.partials, G, sums);
+    if (threadIdx.x == 0)
         for (int c = 0; c < 6; ++c)
             if (a.valid[c] > 0) finish_initial(a.cs[c], a.ctl, sums[c], sums[6 + c], sums[12 + c]);
-    }
 }
 
 // ----------------------------------------------------------- calcEig + tau
@@ -545,9 +549,12 @@ void launch_assemble(Ctx* ctx, const AssembleArgs& a, int grid) {
     const double w = a.mixed ? 4 : 8;
     const double by = (double)a.N * (48 + 96 + 32 + 8 + 48 + 8 + 8 + 12 * w) +
                       (double)a.nEll * (4 + 1 + 4 + 8 + 8 + (a.gradU ? 0 : 24 + 8));
-    ProfScope ps(ctx, a.mixed ? "k_assemble<f32>" : "k_assemble<f64>", by);
-    if (a.mixed) hipLaunchKernelGGL(k_assemble<float>, dim3(grid), dim3(BLK), 0, ctx->stream, a);
-    else hipLaunchKernelGGL(k_assemble<double>, dim3(grid), dim3(BLK), 0, ctx->stream, a);
+    {
+        ProfScope ps(ctx, a.mixed ? "k_assemble<f32>" : "k_assemble<f64>", by);
+        if (a.mixed) hipLaunchKernelGGL(k_assemble<float>, dim3(grid), dim3(BLK), 0, ctx->stream, a);
+        else hipLaunchKernelGGL(k_assemble<double>, dim3(grid), dim3(BLK), 0, ctx->stream, a);
+    }
+    hipLaunchKernelGGL(k_fin_assemble, dim3(1), dim3(BLK), 0, ctx->stream, a, grid);
     RT_HIP(hipGetLastError());
 }
 void launch_eig_tau(Ctx* ctx, int32_t N, const ModelParams& prm, const double* theta,

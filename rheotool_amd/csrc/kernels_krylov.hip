@@ -36,10 +36,10 @@ struct KryArgs {
     double* x;           // [N][6]
     T *r, *r0, *p, *v, *t;  // [N][6]
     double* partials;
-    int32_t* counters;   // one per kernel kind
     CompState* cs;
     int32_t valid[6];
     KryCtl ctl;
+    int32_t G;           // workgroups of the row kernels (partials rows)
 };
 
 // ---- true residual r = b - A x (FP64): MODE 0 initial, MODE 1 refinement ----
@@ -101,8 +101,23 @@ __global__ void __launch_bounds__(BLK) k_residual(KryArgs<T> a) {
             }
         }
     }
+    write_partials<NQ>(q, a.partials);
+}
+
+template <class T, int MODE>
+__global__ void __launch_bounds__(BLK) k_fin_residual(KryArgs<T> a) {
+    constexpr int NQ = MODE == 0 ? 18 : 12;
+    bool on[6];
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        on[c] = a.valid[c] > 0 && (MODE == 0 || a.cs[c].needRefine);
+        any |= on[c];
+    }
+    if (!any) return;
     double sums[NQ];
-    if (reduce_last<NQ>(q, a.partials, a.counters + MODE, sums) && threadIdx.x == 0) {
+    sum_partials<NQ>(a.partials, a.G, sums);
+    if (threadIdx.x == 0) {
         for (int c = 0; c < 6; ++c) {
             if (!on[c]) continue;
             CompState& cs = a.cs[c];
@@ -192,13 +207,27 @@ __global__ void __launch_bounds__(BLK) k_kry_v(KryArgs<T> a) {
         for (int c = 0; c < 6; ++c)
             if (on[c]) q[c] += rr0[c] * acc[c];
     }
-    double qd[6], sums[6];
+    double qd[6];
 #pragma unroll
     for (int c = 0; c < 6; ++c) qd[c] = q[c];
-    if (reduce_last<6>(qd, a.partials, a.counters + 2, sums) && threadIdx.x == 0) {
+    write_partials<6>(qd, a.partials);
+}
+
+template <class T>
+__global__ void __launch_bounds__(BLK) k_fin_v(KryArgs<T> a) {
+    bool on[6];
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        on[c] = a.cs[c].active;
+        any |= on[c];
+    }
+    if (!any) return;
+    double sums[6];
+    sum_partials<6>(a.partials, a.G, sums);
+    if (threadIdx.x == 0)
         for (int c = 0; c < 6; ++c)
             if (on[c]) a.cs[c].alpha = a.cs[c].rho / sums[c];
-    }
 }
 
 // ---- s = r - alpha v ; t = A M^-1 s ; |s|, (t,s), (t,t) --------------------
@@ -252,10 +281,25 @@ __global__ void __launch_bounds__(BLK) k_kry_t(KryArgs<T> a) {
             q[12 + c] += acc[c] * acc[c];
         }
     }
-    double qd[18], sums[18];
+    double qd[18];
 #pragma unroll
     for (int i = 0; i < 18; ++i) qd[i] = q[i];
-    if (reduce_last<18>(qd, a.partials, a.counters + 3, sums) && threadIdx.x == 0) {
+    write_partials<18>(qd, a.partials);
+}
+
+template <class T>
+__global__ void __launch_bounds__(BLK) k_fin_t(KryArgs<T> a) {
+    bool on[6];
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        on[c] = a.cs[c].active;
+        any |= on[c];
+    }
+    if (!any) return;
+    double sums[18];
+    sum_partials<18>(a.partials, a.G, sums);
+    if (threadIdx.x == 0) {
         for (int c = 0; c < 6; ++c) {
             if (!on[c]) continue;
             CompState& cs = a.cs[c];
@@ -322,10 +366,25 @@ __global__ void __launch_bounds__(BLK) k_kry_xr(KryArgs<T> a) {
         st6(a.x + j, xv);
         st6(a.r + j, rr);
     }
-    double qd[12], sums[12];
+    double qd[12];
 #pragma unroll
     for (int i = 0; i < 12; ++i) qd[i] = q[i];
-    if (reduce_last<12>(qd, a.partials, a.counters + 4, sums) && threadIdx.x == 0) {
+    write_partials<12>(qd, a.partials);
+}
+
+template <class T>
+__global__ void __launch_bounds__(BLK) k_fin_xr(KryArgs<T> a) {
+    bool on[6];
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        on[c] = a.cs[c].active;
+        any |= on[c];
+    }
+    if (!any) return;
+    double sums[12];
+    sum_partials<12>(a.partials, a.G, sums);
+    if (threadIdx.x == 0) {
         for (int c = 0; c < 6; ++c) {
             if (!on[c]) continue;
             CompState& cs = a.cs[c];
@@ -378,8 +437,7 @@ __global__ void k_scale(int64_t nEll, const int32_t* __restrict__ col, const dou
 }
 
 // ---- mean of x per component (normalisation factor's gAverage(psi)) ---------
-__global__ void __launch_bounds__(BLK) k_mean6(int32_t N, const double* __restrict__ x, double* partials,
-                                               int32_t* counter, CompState* cs) {
+__global__ void __launch_bounds__(BLK) k_mean6(int32_t N, const double* __restrict__ x, double* partials) {
     double q[6] = {0, 0, 0, 0, 0, 0};
     for (int64_t it = 0;; ++it) {
         const int64_t t0 = tile_of(it) * BLK;
@@ -391,18 +449,26 @@ __global__ void __launch_bounds__(BLK) k_mean6(int32_t N, const double* __restri
 #pragma unroll
         for (int c = 0; c < 6; ++c) q[c] += v[c];
     }
+    write_partials<6>(q, partials);
+}
+
+__global__ void __launch_bounds__(BLK) k_fin_mean6(int32_t N, int G, const double* partials, CompState* cs) {
     double sums[6];
-    if (reduce_last<6>(q, partials, counter, sums) && threadIdx.x == 0)
+    sum_partials<6>(partials, G, sums);
+    if (threadIdx.x == 0)
         for (int c = 0; c < 6; ++c) cs[c].xbar = sums[c] / N;
 }
 
 // ----------------------------------------------------------------- driver
 static inline unsigned kblk(int64_t n) { return (unsigned)((n + BLK - 1) / BLK); }
 
-void launch_mean6(Ctx* ctx, int32_t N, const double* x, double* partials, int32_t* counter, CompState* cs,
+void launch_mean6(Ctx* ctx, int32_t N, const double* x, double* partials, CompState* cs,
                   int grid) {
-    ProfScope ps(ctx, "k_mean6", (double)N * 48);
-    hipLaunchKernelGGL(k_mean6, dim3(grid), dim3(BLK), 0, ctx->stream, N, x, partials, counter, cs);
+    {
+        ProfScope ps(ctx, "k_mean6", (double)N * 48);
+        hipLaunchKernelGGL(k_mean6, dim3(grid), dim3(BLK), 0, ctx->stream, N, x, partials);
+    }
+    hipLaunchKernelGGL(k_fin_mean6, dim3(1), dim3(BLK), 0, ctx->stream, N, grid, partials, cs);
     RT_HIP(hipGetLastError());
 }
 
@@ -516,10 +582,10 @@ void fill_args(Ctx* ctx, const SolveArgs& sa, SolveRun& R) {
         a.r = (T*)K.r.p; a.r0 = (T*)K.r0.p; a.p = (T*)K.p.p; a.v = (T*)K.v.p; a.t = (T*)K.t.p;
     }
     a.partials = K.partials.as<double>();
-    a.counters = st->counter;
     a.cs = st->c;
     for (int c = 0; c < 6; ++c) a.valid[c] = sa.valid[c];
     a.ctl = kryCtl(*sa.ctl, mixed);
+    a.G = K.grid;
 }
 
 // algorithmic bytes per launch for nc participating components: ELL entries
@@ -538,8 +604,11 @@ template <class T>
 void launch_init(Ctx* ctx, SolveRun& R) {
     const bool mixed = std::is_same<T, float>::value;
     Bytes B{R.N, R.ne, (double)sizeof(T)};
-    ProfScope ps(ctx, mixed ? "k_residual<f32>" : "k_residual<f64>", B.res(R.nValid));
-    hipLaunchKernelGGL((k_residual<T, 0>), dim3(R.G), dim3(BLK), 0, ctx->stream, argsOf<T>(R));
+    {
+        ProfScope ps(ctx, mixed ? "k_residual<f32>" : "k_residual<f64>", B.res(R.nValid));
+        hipLaunchKernelGGL((k_residual<T, 0>), dim3(R.G), dim3(BLK), 0, ctx->stream, argsOf<T>(R));
+    }
+    hipLaunchKernelGGL((k_fin_residual<T, 0>), dim3(1), dim3(BLK), 0, ctx->stream, argsOf<T>(R));
 }
 
 template <class T>
@@ -548,8 +617,11 @@ void launch_round(Ctx* ctx, SolveRun& R, double nAct, double nRef) {
     Bytes B{R.N, R.ne, (double)sizeof(T)};
     KryArgs<T>& a = argsOf<T>(R);
     if (nRef > 0) {
-        ProfScope ps(ctx, mixed ? "k_residual<f32>" : "k_residual<f64>", B.res(nRef));
-        hipLaunchKernelGGL((k_residual<T, 1>), dim3(R.G), dim3(BLK), 0, ctx->stream, a);
+        {
+            ProfScope ps(ctx, mixed ? "k_residual<f32>" : "k_residual<f64>", B.res(nRef));
+            hipLaunchKernelGGL((k_residual<T, 1>), dim3(R.G), dim3(BLK), 0, ctx->stream, a);
+        }
+        hipLaunchKernelGGL((k_fin_residual<T, 1>), dim3(1), dim3(BLK), 0, ctx->stream, a);
         nAct += nRef;  // components restarted by the refinement sweep join this round
     }
     {
@@ -560,14 +632,17 @@ void launch_round(Ctx* ctx, SolveRun& R, double nAct, double nRef) {
         ProfScope ps(ctx, mixed ? "k_kry_v<f32>" : "k_kry_v<f64>", B.v(nAct));
         hipLaunchKernelGGL(k_kry_v<T>, dim3(R.G), dim3(BLK), 0, ctx->stream, a);
     }
+    hipLaunchKernelGGL(k_fin_v<T>, dim3(1), dim3(BLK), 0, ctx->stream, a);
     {
         ProfScope ps(ctx, mixed ? "k_kry_t<f32>" : "k_kry_t<f64>", B.t(nAct));
         hipLaunchKernelGGL(k_kry_t<T>, dim3(R.G), dim3(BLK), 0, ctx->stream, a);
     }
+    hipLaunchKernelGGL(k_fin_t<T>, dim3(1), dim3(BLK), 0, ctx->stream, a);
     {
         ProfScope ps(ctx, mixed ? "k_kry_xr<f32>" : "k_kry_xr<f64>", B.xr(nAct));
         hipLaunchKernelGGL(k_kry_xr<T>, dim3(R.G), dim3(BLK), 0, ctx->stream, a);
     }
+    hipLaunchKernelGGL(k_fin_xr<T>, dim3(1), dim3(BLK), 0, ctx->stream, a);
 }
 
 void queue_pull(Ctx* ctx, Krylov& K, SolveRun& R) {

@@ -276,7 +276,7 @@ void modelCorrectBegin(Model& M, double deltaT, const double* gradU) {
     SolveState* st = M.kry.state.as<SolveState>();
     const bool mixed = M.ctl.mixed;
     // gAverage(theta) for the normalisation factor of the initial residual
-    launch_mean6(ctx, N, M.theta.as<double>(), M.kry.partials.as<double>(), st->counter + 6, st->c,
+    launch_mean6(ctx, N, M.theta.as<double>(), M.kry.partials.as<double>(), st->c,
                  M.kry.grid);
     AssembleArgs a{};
     a.N = N;
@@ -313,7 +313,6 @@ void modelCorrectBegin(Model& M, double deltaT, const double* gradU) {
     a.b = M.b.as<double>();
     a.fOld = M.prm.kind == M_FENEP ? M.fOld.as<double>() : nullptr;
     a.partials = M.kry.partials.as<double>();
-    a.counter = st->counter + 5;
     a.cs = st->c;
     for (int q = 0; q < 6; ++q) a.valid[q] = m.valid[q];
     a.r = mixed ? M.kry.rs.p : M.kry.r.p;

@@ -25,13 +25,15 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Block reduction of NQ values per thread into partials[block][NQ]; the last
-// block to finish reduces all blocks in a fixed order (deterministic) and
-// returns true with sums[] filled (thread 0 valid).
+// Grid reductions in two launches: every workgroup of the row kernel writes
+// its NQ partial sums to partials[block][NQ] (plain stores, no fence), then a
+// one-workgroup finalize kernel on the same stream reduces all partials in a
+// fixed order (deterministic) and updates the solver state.  The kernel
+// boundary makes the partials visible across the XCDs' L2s without per-block
+// agent-scope release fences.
 template <int NQ>
-__device__ bool reduce_last(double (&v)[NQ], double* partials, int32_t* counter, double (&sums)[NQ]) {
+__device__ __forceinline__ void write_partials(const double (&v)[NQ], double* partials) {
     __shared__ double red[NQ][BLK / 64];
-    __shared__ int amLast;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
@@ -44,20 +46,25 @@ __device__ bool reduce_last(double (&v)[NQ], double* partials, int32_t* counter,
         for (int w = 0; w < BLK / 64; ++w) s += red[threadIdx.x][w];
         partials[(int64_t)blockIdx.x * NQ + threadIdx.x] = s;
     }
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) amLast = atomicAdd(counter, 1) == (int)gridDim.x - 1;
-    __syncthreads();
-    if (!amLast) return false;
-    __threadfence();
-    const volatile double* vp = partials;
+}
+
+// In a one-workgroup finalize kernel: sums over the G blocks' partials
+// (thread 0 holds the result).
+template <int NQ>
+__device__ __forceinline__ void sum_partials(const double* partials, int G, double (&sums)[NQ]) {
+    __shared__ double red[NQ][BLK / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     double acc[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) acc[q] = 0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += BLK)
+    for (int i = threadIdx.x; i < G; i += BLK) {
+        const double* row = partials + (int64_t)i * NQ;
+        double tmp[NQ];
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) acc[q] += vp[(int64_t)i * NQ + q];
-    __syncthreads();
+        for (int q = 0; q < NQ; ++q) tmp[q] = row[q];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) acc[q] += tmp[q];
+    }
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
         double s = wave_sum(acc[q]);
@@ -71,9 +78,7 @@ __device__ bool reduce_last(double (&v)[NQ], double* partials, int32_t* counter,
             for (int w = 0; w < BLK / 64; ++w) s += red[q][w];
             sums[q] = s;
         }
-        *counter = 0;
     }
-    return true;
 }
 
 // interleaved [N][6] rows: three 2-wide vector loads/stores per row

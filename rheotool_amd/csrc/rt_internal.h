@@ -183,7 +183,6 @@ struct KryCtl {
 struct SolveState {
     CompState c[6];
     int32_t anyActive;
-    int32_t counter[7];  // ticket counters of the reduction kernels
     double misc[8];
 };
 
@@ -280,7 +279,6 @@ struct AssembleArgs {
     double* b;            // [N][6]
     double* fOld;         // FENE-P f, nullable
     double* partials;     // [grid][18] residual sums
-    int32_t* counter;
     CompState* cs;
     int32_t valid[6];
     // initial residual of the solve, fused into the assembly
@@ -332,7 +330,7 @@ struct SolveArgs {
 };
 KryCtl kryCtl(const Controls& C, bool mixed);
 // mean of theta per component -> CompState.xbar (gAverage(psi) of the normalisation)
-void launch_mean6(Ctx* ctx, int32_t N, const double* x, double* partials, int32_t* counter,
+void launch_mean6(Ctx* ctx, int32_t N, const double* x, double* partials,
                   CompState* cs, int grid);
 // Lockstep PBiCGStab over the valid components; fills perf.
 void solve_lockstep(Ctx* ctx, const SolveArgs& a, rt_solver_perf perf[6]);
