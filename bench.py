@@ -187,7 +187,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="precision", choices=sorted(WORKLOADS) + ["copy", "serve"])
     ap.add_argument("--requests", type=int, default=512)
-    ap.add_argument("--rate", type=float, default=300.0, help="serve: Poisson arrival rate, 1/s")
+    ap.add_argument("--rate", type=float, default=100.0, help="serve: Poisson arrival rate, 1/s")
     ap.add_argument("--max-cells", type=float, default=2e6)
     ap.add_argument("--n", type=int, default=0, help="override the mesh size")
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -252,19 +252,25 @@ typedef struct {
     const double* Ub;     /* [3*nBoundaryFaces] */
     const double* phi;    /* [nInternalFaces + nBoundaryFaces] */
     double deltaT;
+    double* thetaOut;     /* [6N] result, written when the request finishes; may be NULL */
+    double* tauOut;       /* [6N] result, may be NULL */
 } rt_request;
 
 int rt_server_create(rt_ctx* ctx, size_t hbm_budget, int max_inflight, rt_server** out);
 void rt_server_destroy(rt_server* server);
-/* Admits the request when the HBM budget allows, else queues it.  All host
- * arrays are copied before return.  *ticket identifies the request. */
+/* Queues the request; it is admitted when a slot and the HBM budget allow.
+ * The strings are copied; the host arrays are NOT: they are read when the
+ * request is admitted and written (thetaOut, tauOut) when it finishes, so they
+ * must stay valid, and the inputs unchanged, until rt_server_poll reports
+ * the ticket finished or failed.  *ticket identifies the request. */
 int rt_server_submit(rt_server* server, const rt_request* req, int64_t* ticket);
 /* Advances every in-flight request by one batched round of device work and
  * admits queued ones as memory frees up.  Returns the number of requests
  * still in flight or queued. */
 int rt_server_progress(rt_server* server);
-/* 1 when the request is finished (then theta/tau [6N] are written, either
- * may be NULL, and *latency_ms is submit-to-finish); 0 when still pending;
+/* 1 when the request is finished (then the request's thetaOut/tauOut hold the
+ * results, theta/tau [6N] get a copy of them when non-NULL, and *latency_ms is
+ * submit-to-finish); 0 when still pending;
  * -(status code) when the request failed (message in rt_last_error of the
  * server's context). */
 int rt_server_poll(rt_server* server, int64_t ticket, double* theta, double* tau,

@@ -21,7 +21,8 @@ import numpy as np
 
 _LIB = None
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "librheo_gpu.so")
+# RHEO_GPU_LIB selects another build of the library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("RHEO_GPU_LIB") or os.path.join(HERE, "lib", "librheo_gpu.so")
 
 RT_OK, RT_E_UNKNOWN_TYPE, RT_E_BAD_ARG, RT_E_OOM, RT_E_HIP, RT_E_UNSUPPORTED, RT_E_NO_DEVICE = range(7)
 RT_W_NOT_CONVERGED = 0x100
@@ -63,7 +64,7 @@ class Request(C.Structure):
     _fields_ = [("mesh", C.c_void_p), ("dict", C.c_char_p), ("controls", C.c_char_p),
                 ("theta", C.c_void_p), ("eigVals", C.c_void_p), ("eigVecs", C.c_void_p),
                 ("U", C.c_void_p), ("Ub", C.c_void_p), ("phi", C.c_void_p),
-                ("deltaT", C.c_double)]
+                ("deltaT", C.c_double), ("thetaOut", C.c_void_p), ("tauOut", C.c_void_p)]
 
 
 class KernelStat(C.Structure):
@@ -379,18 +380,20 @@ class Server:
         ctx.check(lib().rt_server_create(ctx.h, hbm_budget, max_inflight, C.byref(h)),
                   "rt_server_create")
         self.ctx, self.h = ctx, h
-        self.sizes = {}
+        self.held = {}  # ticket -> (input arrays, theta out, tau out): the server reads/writes them
 
     def submit(self, mesh: "Mesh", fields, dict_text, deltaT, controls=None) -> int:
-        """mesh: an uploaded Mesh (shared by any number of requests)."""
+        """mesh: an uploaded Mesh (shared by any number of requests).  The
+        field arrays are referenced by the server until the ticket finishes."""
         m = mesh.src
         arrs = [_f64(fields.theta), _f64(fields.eigVals), _f64(fields.eigVecs), _f64(fields.U),
                 _f64(fields.Ub) if m.nBoundary else None, _f64(fields.phi)]
+        th_out, tau_out = np.empty((m.nCells, 6)), np.empty((m.nCells, 6))
         q = Request(mesh.h, dict_text.encode(), None if controls is None else controls.encode(),
-                    *[_p(a) for a in arrs], deltaT)
+                    *[_p(a) for a in arrs], deltaT, _p(th_out), _p(tau_out))
         t = C.c_int64()
         self.ctx.check(lib().rt_server_submit(self.h, C.byref(q), C.byref(t)), "rt_server_submit")
-        self.sizes[t.value] = m.nCells
+        self.held[t.value] = (arrs, th_out, tau_out)
         return t.value
 
     def progress(self) -> int:
@@ -401,13 +404,12 @@ class Server:
 
     def poll(self, ticket):
         """None while pending, else dict(theta, tau, perf, latency_ms)."""
-        N = self.sizes[ticket]
-        th, tau = np.empty((N, 6)), np.empty((N, 6))
         perf = (SolverPerf * 6)()
         lat = C.c_double()
-        rc = lib().rt_server_poll(self.h, ticket, _p(th), _p(tau), perf, C.byref(lat))
+        rc = lib().rt_server_poll(self.h, ticket, None, None, perf, C.byref(lat))
         if rc == 0:
             return None
+        _, th, tau = self.held.pop(ticket)
         if rc < 0:
             raise RheoError(-rc, lib().rt_last_error(self.ctx.h).decode())
         return dict(theta=th, tau=tau, perf=[perf[i] for i in range(6)], latency_ms=lat.value)

@@ -64,16 +64,18 @@ __device__ __forceinline__ void invDiag(const double* l, int mask, double* r) {
     if (mask & 4) r[2] -= 1.0;
 }
 
-// Log-conformation right-hand side S (6 comps) of one cell.
-__device__ void cell_rhs(const ModelParams& p, int invMask, const double* L, const double* th6,
-                         const double* lam, const double* V, double* S, double& fOut) {
-    const bool ptt = p.kind == M_PTT_LINEAR || p.kind == M_PTT_EXP;
+// Log-conformation right-hand side S (6 comps) of one cell; KIND is the
+// model (ModelKind), fixed at compile time so only its branch holds registers.
+template <int KIND>
+__device__ __forceinline__ void cell_rhs(const ModelParams& p, int invMask, const double* L, const double* th6,
+                                         const double* lam, const double* V, double* S, double& fOut) {
+    constexpr bool ptt = KIND == M_PTT_LINEAR || KIND == M_PTT_EXP;
     double LT[9];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j) LT[3 * i + j] = L[3 * j + i];
-    if (ptt) {
+    if constexpr (ptt) {
         const double zs[9] = {L[0], 0.5 * (L[1] + L[3]), 0.5 * (L[2] + L[6]),
                               0.5 * (L[1] + L[3]), L[4], 0.5 * (L[5] + L[7]),
                               0.5 * (L[2] + L[6]), 0.5 * (L[5] + L[7]), L[8]};
@@ -115,13 +117,13 @@ __device__ void cell_rhs(const ModelParams& p, int invMask, const double* L, con
     const double rl = 1.0 / p.lambda;
     double X[9];
     fOut = 0.0;
-    if (p.kind == M_OLDROYDB) {
+    if constexpr (KIND == M_OLDROYDB) {
         const double dd[3] = {il[0] - 1.0, il[1] - 1.0, il[2] - 1.0};
         double Y[9];
         vdvt(V, dd, Y);
 #pragma unroll
         for (int q = 0; q < 9; ++q) X[q] = rl * Y[q];
-    } else if (p.kind == M_GIESEKUS) {
+    } else if constexpr (KIND == M_GIESEKUS) {
         const double dd[3] = {il[0] - 1.0, il[1] - 1.0, il[2] - 1.0};
         double trhs[9], C[9], tt[9], CT[9];
         vdvt(V, dd, trhs);
@@ -130,7 +132,7 @@ __device__ void cell_rhs(const ModelParams& p, int invMask, const double* L, con
         dotTT(C, tt, CT);
 #pragma unroll
         for (int q = 0; q < 9; ++q) X[q] = rl * (trhs[q] - p.alpha * CT[q]);
-    } else if (p.kind == M_FENEP) {
+    } else if constexpr (KIND == M_FENEP) {
         double C[9], Y[9];
         vdvt(V, lam, C);
         const double f = p.L2 / (p.L2 - (C[0] + C[4] + C[8]));
@@ -147,7 +149,7 @@ __device__ void cell_rhs(const ModelParams& p, int invMask, const double* L, con
         vdvt(V, lam, C);
         const double k = p.epsilon / (1 - p.zeta);
         const double z = k * (C[0] + C[4] + C[8] - 3.);
-        const double fac = p.kind == M_PTT_LINEAR ? 1. + z : exp(z);
+        const double fac = KIND == M_PTT_LINEAR ? 1. + z : exp(z);
 #pragma unroll
         for (int q = 0; q < 9; ++q) X[q] = (rl * Y[q]) * fac;
     }
@@ -160,11 +162,13 @@ __device__ void cell_rhs(const ModelParams& p, int invMask, const double* L, con
 // column order; wider rows write each value to its slot directly.
 constexpr int STAGE_W = 8;
 
-template <class T>
+template <class T, int KIND>
 __global__ void __launch_bounds__(BLK) k_assemble(AssembleArgs a) {
-    double q[18];
+    // per-thread running sums of the initial residual live in LDS, not in
+    // registers: the row body needs every VGPR it can get for occupancy
+    __shared__ double qs[18][BLK];
 #pragma unroll
-    for (int i = 0; i < 18; ++i) q[i] = 0;
+    for (int i = 0; i < 18; ++i) qs[i][threadIdx.x] = 0;
     double xbar[6];
 #pragma unroll
     for (int c = 0; c < 6; ++c) xbar[c] = a.cs[c].xbar;
@@ -219,10 +223,12 @@ __global__ void __launch_bounds__(BLK) k_assemble(AssembleArgs a) {
             sumOff += fabs(val);
             sumAv += val;
             const int32_t c = a.ellCol[ec];
-            double thc[6];
-            ld6(a.theta + (int64_t)NCMP * c, thc);
+            if (val != 0.0) {  // upwind: only the upstream side of a face couples
+                double thc[6];
+                ld6(a.theta + (int64_t)NCMP * c, thc);
 #pragma unroll
-            for (int q6 = 0; q6 < 6; ++q6) offY[q6] += val * thc[q6];
+                for (int q6 = 0; q6 < 6; ++q6) offY[q6] += val * thc[q6];
+            }
             if (!haveGrad) {
                 const double wf = a.eW[e];
                 const double2 c01 = reinterpret_cast<const double2*>(a.U)[2 * (int64_t)c];
@@ -268,7 +274,7 @@ __global__ void __launch_bounds__(BLK) k_assemble(AssembleArgs a) {
             for (int qq = 0; qq < 3; ++qq) lam[qq] = a.eigVal[(int64_t)qq * N + r];
 #pragma unroll
             for (int qq = 0; qq < 9; ++qq) V[qq] = a.eigVec[(int64_t)qq * N + r];
-            cell_rhs(a.prm, a.invMask, g, th, lam, V, S, f);
+            cell_rhs<KIND>(a.prm, a.invMask, g, th, lam, V, S, f);
         }
         if (a.fOld) a.fOld[r] = f;
         // matrix diagonal: ddt + div, optional relax, then boundary internalCoeffs
@@ -326,15 +332,18 @@ __global__ void __launch_bounds__(BLK) k_assemble(AssembleArgs a) {
             const double res = src[c] - y;
             rt[c] = (T)res;
             if (a.valid[c] > 0) {
-                q[c] += fabs(res);
-                q[6 + c] += (double)rt[c] * (double)rt[c];
+                qs[c][threadIdx.x] += fabs(res);
+                qs[6 + c][threadIdx.x] += (double)rt[c] * (double)rt[c];
                 const double pA = sumA * xbar[c];
-                q[12 + c] += fabs(y - pA) + fabs(src[c] - pA);
+                qs[12 + c][threadIdx.x] += fabs(y - pA) + fabs(src[c] - pA);
             }
         }
         st6(rOut + (int64_t)NCMP * r, rt);
         st6(r0Out + (int64_t)NCMP * r, rt);
     }
+    double q[18];
+#pragma unroll
+    for (int i = 0; i < 18; ++i) q[i] = qs[i][threadIdx.x];
     write_partials<18>(q, a.partials);
 }
 
@@ -542,6 +551,27 @@ __global__ void __launch_bounds__(BLK) k_copy(int64_t n16, const double2* __rest
 // ---------------------------------------------------------------- launchers
 static inline unsigned nblk(int64_t n) { return (unsigned)((n + BLK - 1) / BLK); }
 
+template <class T>
+static void launch_assemble_kind(Ctx* ctx, const AssembleArgs& a, int grid) {
+    switch (a.prm.kind) {
+    case M_OLDROYDB:
+        hipLaunchKernelGGL((k_assemble<T, M_OLDROYDB>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
+        break;
+    case M_GIESEKUS:
+        hipLaunchKernelGGL((k_assemble<T, M_GIESEKUS>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
+        break;
+    case M_FENEP:
+        hipLaunchKernelGGL((k_assemble<T, M_FENEP>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
+        break;
+    case M_PTT_LINEAR:
+        hipLaunchKernelGGL((k_assemble<T, M_PTT_LINEAR>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
+        break;
+    default:
+        hipLaunchKernelGGL((k_assemble<T, M_PTT_EXP>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
+        break;
+    }
+}
+
 void launch_assemble(Ctx* ctx, const AssembleArgs& a, int grid) {
     // algorithmic bytes per cell: theta 48, eigen state 96, U 32, V 8, b 48, diag 8, 1/diag 8,
     // r and r0 (2 x 6 x w); per ELL entry: face id 4, slot 1, column 4, phi 8, value 8
@@ -551,8 +581,8 @@ void launch_assemble(Ctx* ctx, const AssembleArgs& a, int grid) {
                       (double)a.nEll * (4 + 1 + 4 + 8 + 8 + (a.gradU ? 0 : 24 + 8));
     {
         ProfScope ps(ctx, a.mixed ? "k_assemble<f32>" : "k_assemble<f64>", by);
-        if (a.mixed) hipLaunchKernelGGL(k_assemble<float>, dim3(grid), dim3(BLK), 0, ctx->stream, a);
-        else hipLaunchKernelGGL(k_assemble<double>, dim3(grid), dim3(BLK), 0, ctx->stream, a);
+        if (a.mixed) launch_assemble_kind<float>(ctx, a, grid);
+        else launch_assemble_kind<double>(ctx, a, grid);
     }
     hipLaunchKernelGGL(k_fin_assemble, dim3(1), dim3(BLK), 0, ctx->stream, a, grid);
     RT_HIP(hipGetLastError());

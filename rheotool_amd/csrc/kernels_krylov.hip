@@ -79,6 +79,7 @@ __global__ void __launch_bounds__(BLK) k_residual(KryArgs<T> a) {
             const int32_t col = a.ellCol[e];
             const double av = a.aval[e];
             if (MODE == 0) sumA += av;
+            if (av == 0.0) continue;  // upwind: the downwind side of a face has no coupling
             double xc[6];
             ld6(a.x + (int64_t)NCMP * col, xc);
 #pragma unroll
@@ -195,6 +196,7 @@ __global__ void __launch_bounds__(BLK) k_kry_v(KryArgs<T> a) {
             const int64_t e = base + (int64_t)SLICE * k;
             const int32_t col = a.ellCol[e];
             const T av = a.ahat[e];
+            if (av == (T)0) continue;
             T pc[6];
             ld6(a.p + (int64_t)NCMP * col, pc);
 #pragma unroll
@@ -266,6 +268,7 @@ __global__ void __launch_bounds__(BLK) k_kry_t(KryArgs<T> a) {
             const int64_t e = base + (int64_t)SLICE * k;
             const int32_t col = a.ellCol[e];
             const T av = a.ahat[e];
+            if (av == (T)0) continue;
             T rc[6], vc[6];
             ld6(a.r + (int64_t)NCMP * col, rc);
             ld6(a.v + (int64_t)NCMP * col, vc);
@@ -523,12 +526,13 @@ struct SolveRun {
     int G = 1, pgrid = 1;
     double N = 0, ne = 0, nValid = 0;
     int32_t valid[6];
-    CompState* host = nullptr;  // pinned
+    Ctx* ctx = nullptr;
+    CompState* host = nullptr;  // pinned block of ctx
     hipEvent_t ev = nullptr;
     bool pending = false, done = false;
     int rounds = 0;
     ~SolveRun() {
-        if (host) (void)hipHostFree(host);
+        if (host) ctx->givePinnedBlock(host);
         if (ev) (void)hipEventDestroy(ev);
     }
 };
@@ -660,8 +664,10 @@ void solve_begin(Ctx* ctx, const SolveArgs& sa) {
     const bool mixed = sa.ctl->mixed;
     K.ensure(m, mixed);
     if (!K.run) {
+        static_assert(sizeof(CompState) * 6 <= Ctx::kPinnedBlock, "status block too small");
         K.run.reset(new SolveRun);
-        RT_HIP(hipHostMalloc((void**)&K.run->host, sizeof(CompState) * 6, hipHostMallocDefault));
+        K.run->ctx = ctx;
+        K.run->host = static_cast<CompState*>(ctx->takePinnedBlock());
         RT_HIP(hipEventCreateWithFlags(&K.run->ev, hipEventDisableTiming));
     }
     SolveRun& R = *K.run;
@@ -711,6 +717,10 @@ bool solve_advance(Ctx* ctx, Krylov& K, bool block) {
         R.done = true;
         return true;
     }
+    // A refinement sweep costs a full FP64 pass whatever the number of
+    // components in it: components whose inner solve stopped wait (idle, no
+    // state change) until every inner solve has stopped, then refine together.
+    if (nAct > 0) nRef = 0;
     if (R.mixed) launch_round<float>(ctx, R, nAct, nRef);
     else launch_round<double>(ctx, R, nAct, nRef);
     RT_HIP(hipGetLastError());

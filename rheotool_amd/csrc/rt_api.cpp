@@ -30,30 +30,89 @@ DBuf& DBuf::operator=(DBuf&& o) noexcept {
 }
 void DBuf::release() {
     if (p) {
-        (void)hipFree(p);
-        if (ctx) ctx->inUse -= bytes;
+        if (ctx) {
+            ctx->inUse -= bytes;
+            ctx->recycle(p, bytes);
+        } else {
+            (void)hipFree(p);
+        }
     }
     p = nullptr;
     bytes = 0;
     ctx = nullptr;
 }
 
+namespace {
+// 256-byte granularity up to 1 MiB, then 8 classes per power of two (at most
+// 12.5% slack), so that buffers of similar meshes share a class
+size_t sizeClass(size_t b) {
+    b = (b + 255) / 256 * 256;
+    if (b <= (size_t(1) << 20)) return b;
+    const int k = 63 - __builtin_clzll(b);
+    const size_t step = size_t(1) << (k - 3);
+    return (b + step - 1) / step * step;
+}
+}  // namespace
+
 DBuf Ctx::alloc(size_t bytes) {
     if (bytes == 0) bytes = 8;
-    bytes = (bytes + 255) / 256 * 256;
+    bytes = sizeClass(bytes);
     if (budget && inUse + bytes > budget)
         throw Error{RT_E_OOM, "HBM budget exceeded: " + std::to_string(inUse + bytes) + " > " +
                                   std::to_string(budget) + " bytes"};
     DBuf b;
-    hipError_t e = hipMalloc(&b.p, bytes);
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        throw Error{RT_E_OOM, "hipMalloc(" + std::to_string(bytes) + ") failed: " + hipGetErrorString(e)};
+    auto it = cache.find(bytes);
+    if (it != cache.end()) {
+        b.p = it->second;
+        cachedBytes -= bytes;
+        cache.erase(it);
+    } else {
+        hipError_t e = hipMalloc(&b.p, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            trimCache(0);  // give the cached memory back and try once more
+            e = hipMalloc(&b.p, bytes);
+        }
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            throw Error{RT_E_OOM, "hipMalloc(" + std::to_string(bytes) + ") failed: " + hipGetErrorString(e)};
+        }
     }
     b.ctx = this;
     b.bytes = bytes;
     inUse += bytes;
     return b;
+}
+
+void Ctx::recycle(void* p, size_t bytes) {
+    cache.emplace(bytes, p);
+    cachedBytes += bytes;
+    if (cachedBytes > cacheCap) trimCache(cacheCap / 2);
+}
+
+void Ctx::trimCache(size_t keep) {
+    // largest first: they are the least likely to be reused exactly
+    while (cachedBytes > keep && !cache.empty()) {
+        auto it = std::prev(cache.end());
+        (void)hipFree(it->second);
+        cachedBytes -= it->first;
+        cache.erase(it);
+    }
+}
+
+void* Ctx::takePinnedBlock() {
+    if (!pinnedBlocks.empty()) {
+        void* p = pinnedBlocks.back();
+        pinnedBlocks.pop_back();
+        return p;
+    }
+    void* p = nullptr;
+    RT_HIP(hipHostMalloc(&p, kPinnedBlock, hipHostMallocDefault));
+    return p;
+}
+
+void Ctx::givePinnedBlock(void* p) {
+    if (p) pinnedBlocks.push_back(p);
 }
 
 void* Ctx::pinnedHost(size_t bytes) {
@@ -391,6 +450,29 @@ int guarded(rt::Ctx* ctx, F&& f) {
 
 }  // namespace
 
+namespace rt {
+static void ctxTeardown(rt_ctx* ctx) {
+    Ctx& x = ctx->impl;
+    (void)hipSetDevice(x.device);
+    (void)hipStreamSynchronize(x.stream);
+    for (auto& p : x.pending) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    for (auto e : x.eventPool) (void)hipEventDestroy(e);
+    if (x.pinned) (void)hipHostFree(x.pinned);
+    x.stage.release();
+    x.trimCache(0);
+    for (void* p : x.pinnedBlocks) (void)hipHostFree(p);
+    if (x.stream) (void)hipStreamDestroy(x.stream);
+    delete ctx;
+}
+
+void ctxUnref(Ctx* c) {
+    if (--c->refs == 0) ctxTeardown(c->handle);
+}
+}  // namespace rt
+
 extern "C" {
 
 int rt_device_count(void) {
@@ -408,6 +490,7 @@ int rt_ctx_create(int device, size_t hbm_budget, rt_ctx** out) {
     int n = rt_device_count();
     if (n <= 0 || device < 0 || device >= n) return RT_E_NO_DEVICE;
     auto* c = new rt_ctx;
+    c->impl.handle = c;
     int rc = guarded(&c->impl, [&] {
         Ctx& x = c->impl;
         x.device = device;
@@ -429,20 +512,9 @@ int rt_ctx_create(int device, size_t hbm_budget, rt_ctx** out) {
     return RT_OK;
 }
 
+
 void rt_ctx_destroy(rt_ctx* ctx) {
-    if (!ctx) return;
-    Ctx& x = ctx->impl;
-    (void)hipSetDevice(x.device);
-    (void)hipStreamSynchronize(x.stream);
-    for (auto& p : x.pending) {
-        (void)hipEventDestroy(p.a);
-        (void)hipEventDestroy(p.b);
-    }
-    for (auto e : x.eventPool) (void)hipEventDestroy(e);
-    if (x.pinned) (void)hipHostFree(x.pinned);
-    x.stage.release();
-    if (x.stream) (void)hipStreamDestroy(x.stream);
-    delete ctx;
+    if (ctx) ctxUnref(&ctx->impl);
 }
 
 const char* rt_last_error(const rt_ctx* ctx) { return ctx ? ctx->impl.lastError.c_str() : "NULL context"; }
@@ -519,11 +591,17 @@ int rt_mesh_create(rt_ctx* ctx, const rt_mesh_desc* desc, int renumber, rt_mesh*
         delete m;
         return rc;
     }
+    ctxRef(&ctx->impl);
     *out = m;
     return RT_OK;
 }
 
-void rt_mesh_destroy(rt_mesh* mesh) { delete mesh; }
+void rt_mesh_destroy(rt_mesh* mesh) {
+    if (!mesh) return;
+    Ctx* c = mesh->impl.ctx;
+    delete mesh;
+    if (c) ctxUnref(c);
+}
 
 int rt_mesh_valid_components(const rt_mesh* mesh, int32_t out[6]) {
     if (!mesh || !out) return RT_E_BAD_ARG;
@@ -578,14 +656,17 @@ int rt_model_create(rt_ctx* ctx, const char* name, const rt_mesh* mesh, const ch
         delete mo;
         return rc;
     }
+    ctxRef(&ctx->impl);
     *out = mo;
     return RT_OK;
 }
 
 void rt_model_destroy(rt_model* model) {
     if (!model) return;
+    Ctx* c = model->impl.ctx;
     delete model->impl.pendingSolve;
     delete model;
+    if (c) ctxUnref(c);
 }
 
 const char* rt_model_type(const rt_model* model) { return model ? model->impl.typeName.c_str() : ""; }

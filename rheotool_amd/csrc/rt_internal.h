@@ -51,6 +51,11 @@ struct KernelStat {
 };
 
 struct Ctx {
+    // The handle and every mesh, model and server made on this context hold a
+    // reference; the context is torn down when the last one goes, so objects
+    // destroyed after rt_ctx_destroy still release their memory safely.
+    int refs = 1;
+    rt_ctx* handle = nullptr;
     int device = 0;
     int numCUs = 256;
     hipStream_t stream = nullptr;
@@ -71,8 +76,22 @@ struct Ctx {
     size_t pinnedBytes = 0;
     // device staging buffer for host AoS uploads/downloads
     DBuf stage;
+    // Released device buffers are cached by size class and reused by later
+    // allocations on this context: hipFree synchronises the whole device, which
+    // would stall every other stream (the serving loop destroys a model per
+    // request).  Buffers are reused only on this context's stream, in order.
+    std::multimap<size_t, void*> cache;
+    size_t cachedBytes = 0;
+    size_t cacheCap = size_t(8) << 30;
+    // pinned 4 KiB blocks for the per-solve status read-back, reused likewise
+    std::vector<void*> pinnedBlocks;
+    static constexpr size_t kPinnedBlock = 4096;
 
     DBuf alloc(size_t bytes);
+    void recycle(void* p, size_t bytes);
+    void trimCache(size_t keep);  // frees cached buffers until at most keep bytes remain
+    void* takePinnedBlock();
+    void givePinnedBlock(void* p);
     void* pinnedHost(size_t bytes);
     hipEvent_t getEvent();
     void flushProfile();  // synchronises and folds pending events into stats
@@ -358,6 +377,10 @@ size_t modelBytesEstimate(int32_t N, int32_t F, int32_t NB, int64_t nEll, bool m
 struct rt_ctx {
     rt::Ctx impl;
 };
+namespace rt {
+inline void ctxRef(Ctx* c) { ++c->refs; }
+void ctxUnref(Ctx* c);  // tears the context down at the last reference
+}  // namespace rt
 struct rt_mesh {
     rt::Mesh impl;
 };

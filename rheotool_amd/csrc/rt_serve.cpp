@@ -19,12 +19,15 @@ namespace rt {
 
 struct Request {
     int64_t ticket = 0;
-    // host copies of the submission (the mesh is the caller's uploaded handle)
+    // the submission: strings copied, host arrays referenced (the caller keeps
+    // them alive until the ticket is finished), the mesh is an uploaded handle
     const rt_mesh* mesh = nullptr;
     int32_t nCells = 0;
     std::string dict, controls;
-    std::vector<double> theta, eigVals, eigVecs, U, Ub, phi;
+    const double *theta = nullptr, *eigVals = nullptr, *eigVecs = nullptr, *U = nullptr, *Ub = nullptr,
+                 *phi = nullptr;
     double deltaT = 0;
+    double *thetaOut = nullptr, *tauOut = nullptr;
     size_t estBytes = 0;
     // runtime
     int slot = -1;
@@ -32,7 +35,7 @@ struct Request {
     enum { QUEUED, RUNNING, FINISHING, DONE, FAILED } state = QUEUED;
     hipEvent_t done = nullptr;
     std::chrono::steady_clock::time_point tSubmit, tDone;
-    std::vector<double> outTheta, outTau;
+    std::vector<double> ownTheta, ownTau;  // results when the caller gave no output arrays
     rt_solver_perf perf[6];
     int rc = RT_OK;
     std::string err;
@@ -66,9 +69,8 @@ void releaseRequest(rt_server* S, Request* r) {
         S->inflightBytes -= r->estBytes;
         r->slot = -1;
     }
-    // the submission copies are no longer needed
-    for (auto* v : {&r->theta, &r->eigVals, &r->eigVecs, &r->U, &r->Ub, &r->phi})
-        std::vector<double>().swap(*v);
+    // the caller's input arrays are no longer read
+    r->theta = r->eigVals = r->eigVecs = r->U = r->Ub = r->phi = nullptr;
 }
 
 void fail(rt_server* S, Request* r, int rc, const std::string& msg) {
@@ -87,11 +89,8 @@ void admit(rt_server* S, Request* r, int slot) {
     S->inflightBytes += r->estBytes;
     int rc = rt_model_create(c, "", r->mesh, r->dict.c_str(), &r->model);
     if (rc == RT_OK) rc = rt_model_set_solver(r->model, r->controls.c_str());
-    if (rc == RT_OK)
-        rc = rt_model_set_state(r->model, r->theta.data(), r->eigVals.empty() ? nullptr : r->eigVals.data(),
-                                r->eigVecs.empty() ? nullptr : r->eigVecs.data(), nullptr);
-    if (rc == RT_OK)
-        rc = rt_model_set_flow(r->model, r->U.data(), r->Ub.empty() ? nullptr : r->Ub.data(), r->phi.data());
+    if (rc == RT_OK) rc = rt_model_set_state(r->model, r->theta, r->eigVals, r->eigVecs, nullptr);
+    if (rc == RT_OK) rc = rt_model_set_flow(r->model, r->U, r->Ub, r->phi);
     if (rc != RT_OK) {
         fail(S, r, rc, rt_last_error(c));
         return;
@@ -121,10 +120,18 @@ void advance(rt_server* S, Request* r) {
             if (e == hipErrorNotReady) return;
             RT_HIP(e);
             const size_t N = (size_t)r->nCells;
-            r->outTheta.resize(6 * N);
-            r->outTau.resize(6 * N);
-            int rc = rt_model_get(r->model, RT_THETA, r->outTheta.data());
-            if (rc == RT_OK) rc = rt_model_get(r->model, RT_TAU, r->outTau.data());
+            double* th = r->thetaOut;
+            double* ta = r->tauOut;
+            if (!th) {
+                r->ownTheta.resize(6 * N);
+                th = r->ownTheta.data();
+            }
+            if (!ta) {
+                r->ownTau.resize(6 * N);
+                ta = r->ownTau.data();
+            }
+            int rc = rt_model_get(r->model, RT_THETA, th);
+            if (rc == RT_OK) rc = rt_model_get(r->model, RT_TAU, ta);
             if (rc != RT_OK) {
                 fail(S, r, rc, rt_last_error(c));
                 return;
@@ -168,6 +175,7 @@ int rt_server_create(rt_ctx* ctx, size_t hbm_budget, int max_inflight, rt_server
         S->slots.push_back(c);
         S->slotReq.push_back(nullptr);
     }
+    ctxRef(&ctx->impl);
     *out = S;
     return RT_OK;
 }
@@ -180,7 +188,9 @@ void rt_server_destroy(rt_server* S) {
             releaseRequest(S, kv.second.get());
         }
     for (auto* c : S->slots) rt_ctx_destroy(c);
+    Ctx* parent = &S->parent->impl;
     delete S;
+    ctxUnref(parent);
 }
 
 int rt_server_submit(rt_server* S, const rt_request* q, int64_t* ticket) {
@@ -193,18 +203,20 @@ int rt_server_submit(rt_server* S, const rt_request* q, int64_t* ticket) {
     }
     auto r = std::make_unique<Request>();
     const Mesh& mm = q->mesh->impl;
-    const size_t N = mm.N, F = mm.F, NB = mm.NB, NF = F + NB;
+    const size_t N = mm.N, F = mm.F, NB = mm.NB;
     r->mesh = q->mesh;
     r->nCells = mm.N;
     r->dict = q->dict;
     r->controls = q->controls ? q->controls : S->defaultControls;
-    r->theta.assign(q->theta, q->theta + 6 * N);
-    if (q->eigVals) r->eigVals.assign(q->eigVals, q->eigVals + 9 * N);
-    if (q->eigVecs) r->eigVecs.assign(q->eigVecs, q->eigVecs + 9 * N);
-    r->U.assign(q->U, q->U + 3 * N);
-    if (NB) r->Ub.assign(q->Ub, q->Ub + 3 * NB);
-    r->phi.assign(q->phi, q->phi + NF);
+    r->theta = q->theta;
+    r->eigVals = q->eigVals;
+    r->eigVecs = q->eigVecs;
+    r->U = q->U;
+    r->Ub = NB ? q->Ub : nullptr;
+    r->phi = q->phi;
     r->deltaT = q->deltaT;
+    r->thetaOut = q->thetaOut;
+    r->tauOut = q->tauOut;
     const bool mixed = r->controls.find("mixed") != std::string::npos;
     r->estBytes = modelBytesEstimate((int32_t)N, (int32_t)F, (int32_t)NB, mm.nEll, mixed);
     if (S->budget && r->estBytes > S->budget) {
@@ -263,9 +275,11 @@ int rt_server_poll(rt_server* S, int64_t ticket, double* theta, double* tau, rt_
         return -r->rc;
     }
     if (r->state != Request::DONE) return 0;
-    const size_t n = r->outTheta.size();
-    if (theta) std::copy(r->outTheta.begin(), r->outTheta.end(), theta);
-    if (tau) std::copy(r->outTau.begin(), r->outTau.begin() + n, tau);
+    const size_t n = (size_t)6 * r->nCells;
+    const double* th = r->thetaOut ? r->thetaOut : r->ownTheta.data();
+    const double* ta = r->tauOut ? r->tauOut : r->ownTau.data();
+    if (theta && theta != th) std::copy(th, th + n, theta);
+    if (tau && tau != ta) std::copy(ta, ta + n, tau);
     if (perf)
         for (int q = 0; q < 6; ++q) perf[q] = r->perf[q];
     if (latency_ms)

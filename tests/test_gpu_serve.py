@@ -124,3 +124,26 @@ def test_shared_mesh_requests_and_queueing():
         for t in ts:
             out = srv.poll(t)
             assert np.array_equal(out["theta"], alone[0])
+
+
+def test_objects_outlive_their_context():
+    """A mesh, a model and a server released after rt_ctx_destroy still free
+    their memory safely (the context is torn down at its last reference), and
+    a model keeps working until it is destroyed."""
+    m, f, model, lam, kw, dt = make_request(2)
+    ref = None
+    with abi.Context(0) as ctx0:
+        ref = run_alone(ctx0, (m, f, model, lam, kw, dt), abi.solver_dict(1e-10, precision="mixed"))
+    ctx = abi.Context(0)
+    dm = ctx.mesh(m)
+    eq = ctx.constitutiveEq("", dm, abi.model_dict(model, lam=lam, **kw))
+    eq.set_solver(abi.solver_dict(1e-10, precision="mixed"))
+    eq.set_state(f.theta, f.eigVals, f.eigVecs)
+    eq.set_flow(f.U, f.Ub, f.phi)
+    srv = abi.Server(ctx, max_inflight=2)
+    ctx.close()
+    eq.correct(dt)
+    assert np.array_equal(eq.theta(), ref[0])
+    srv.close()
+    eq.close()
+    dm.close()
