@@ -80,6 +80,25 @@ def new_model(ctx, dm, w, f, precision):
     return eq
 
 
+def min_pass_bytes(N, F, perf, mixed):
+    """Bytes of one step under the minimum-pass accounting of a fully fused
+    PBiCGStab: the shared matrix read twice per lockstep iteration plus the
+    Jacobi diagonal, 17 vector passes per component-iteration, one FP64
+    residual-and-correction pass per refinement sweep, and the per-cell pre-
+    and post-solve passes (assembly inputs and outputs; eigen state and tau)."""
+    w = 4.0 if mixed else 8.0
+    solved = [p for p in perf if p.solved]
+    nc = len(solved)
+    K = max((p.nIterations for p in solved), default=0)
+    iters = sum(p.nIterations for p in solved)
+    nref = max((p.nRefinements for p in solved), default=0)
+    b_a = lambda wa: N * wa + 2 * F * wa + 8 * F
+    b_pre = N * (48 + 24 + 72 + 24 + 8 + 6 * w + w) + F * (8 + 24 + 8 + 8 + 2 * w)
+    b_post = N * (48 + 24 + 72 + 48)
+    b_ref = b_a(8.0) + nc * N * (8 + 8 + 8 + 4 + 8)
+    return b_pre + K * (2 * b_a(w) + N * w) + 17 * N * w * iters + nref * b_ref + b_post
+
+
 def cpu_baseline(w, n_sample):
     """The oracle (reference algorithm: PBiCGStab + DILU, FP64, one thread) on a
     bounded sample of the same workload."""
@@ -260,6 +279,7 @@ def main():
     stats = ctx.kernel_stats()
     ctx.set_profiling(False)
     iters = [p.nIterations for p in eq.solver_perf()]
+    mp_bytes = min_pass_bytes(m.nCells, m.nInternal, eq.solver_perf(), w["precision"] == "mixed")
     log(f"[bench] last step perf {eq.solver_perf()}")
     value = m.nCells * args.steps / elapsed
     dom = max(stats.items(), key=lambda kv: kv[1]["ms"])
@@ -272,7 +292,7 @@ def main():
     tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
     if os.path.exists(tpath):
         try:
-            traffic = json.load(open(tpath)).get(name)
+            traffic = json.load(open(tpath)).get("kernels", {}).get(name)
         except Exception:
             traffic = None
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
@@ -310,6 +330,8 @@ def main():
         gpu_ms_per_step=round(step_gpu_ms, 3),
         step_algorithmic_GB=round(step_bytes / 1e9, 3),
         step_roofline_frac=round(step_bytes / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
+        step_min_pass_GB=round(mp_bytes / 1e9, 3),
+        step_min_pass_roofline_frac=round(mp_bytes / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
         iterations=iters,
         fp64_dropin_ms_per_step=None if fp64_ms is None else round(fp64_ms, 3),
         speedup_vs_fp64_dropin=None if fp64_ms is None else round(fp64_ms / (1e3 * elapsed / args.steps), 3),

@@ -95,7 +95,9 @@ int rt_ctx_copy_bench(rt_ctx* ctx, size_t bytes, int reps, double* ms);
  * (sparseMatrixSolvers/segregated/eigenSolver/eigenSolver.C:600-641).
  * Geometry: face area vectors Sf (out of the owner), linear weights w,
  * cell volumes V -- the inputs of fvc::grad(U) Gauss linear
- * (constitutiveEqs/utils/boilerLog.H:1) and of fvm::ddt. */
+ * (constitutiveEqs/utils/boilerLog.H:1) and of fvm::ddt -- and the cell
+ * centres C, which only the limited GaussDefCmpw schemes read
+ * (gaussDefCmpwConvectionScheme.C:251-266, mesh.C()). */
 enum rt_patch_kind { RT_PATCH_PATCH = 0, RT_PATCH_WALL = 1, RT_PATCH_EMPTY = 2 };
 
 typedef struct {
@@ -115,6 +117,8 @@ typedef struct {
     const double* weights;    /* [nInternalFaces] */
     const double* V;          /* [nCells] */
     const rt_patch* patches;  /* [nPatches] */
+    const double* C;          /* [3*nCells] cell centres; may be NULL (then only the
+                                 upwind and none schemes are available) */
 } rt_mesh_desc;
 
 enum rt_renumber { RT_RENUMBER_AUTO = 0, RT_RENUMBER_NONE = 1, RT_RENUMBER_RCM = 2 };
@@ -137,8 +141,8 @@ int rt_mesh_valid_components(const rt_mesh* mesh, int32_t out[6]);
  * (e.g. tutorials/rheoFoam/Channel/Oldroyd-BLog/system/fvSolution:29-42),
  * as OpenFOAM dictionary text:
  *   solver PBiCGStab;  (PBiCG accepted: solved with PBiCGStab)
- *   preconditioner DILU;  (DILU -> multicolour ILU0; DIC/diagonal -> Jacobi;
- *                          none; or the explicit names multicolourILU0, Jacobi)
+ *   preconditioner DILU;  (DILU, DIC, diagonal, Jacobi, multicolourILU0 ->
+ *                          right Jacobi on the device; none)
  *   tolerance 1e-10; relTol 0; maxIter 1000; minIter 0;
  *   precision double;  (or "mixed": FP32 Krylov + FP64 iterative refinement)
  * Unknown solver / preconditioner keywords -> RT_E_UNKNOWN_TYPE listing the
@@ -197,6 +201,17 @@ int rt_model_set_solver(rt_model* model, const char* controls);
  * Oldroyd_BLog.C:166): <= 0 (default) means no relaxationFactors entry. */
 int rt_model_set_relax(rt_model* model, double factor);
 
+/* The fvSchemes entry div(phi,theta) (e.g. tutorials/rheoFoam/Channel/
+ * Oldroyd-BLog/system/fvSchemes:36 "GaussDefCmpw cubista"): the text after
+ * the key, "GaussDefCmpw <limiter>" or the limiter alone, one of upwind
+ * (default), cubista, minmod, smart, waceb, superbee, none
+ * (gaussDefCmpwConvectionScheme.H:68 scheme_, limiters.H:38-100).  The
+ * limited schemes add the explicit deferred correction of
+ * gaussDefCmpwConvectionScheme.C:120-169 and need the mesh's cell centres;
+ * none drops the convection term (:90).  Unknown names -> RT_E_UNKNOWN_TYPE
+ * with the reference's list (limiters.H:88-97). */
+int rt_model_set_scheme(rt_model* model, const char* divScheme);
+
 /* constitutiveEq::correct(alpha, gradU)  (constitutiveEq.H:346-350,
  * Oldroyd_BLog.C:127-179): decompose grad(U)^T, assemble theta's
  * ddt + upwind div == source, solve the valid components, eigen-decompose
@@ -254,6 +269,8 @@ typedef struct {
     double deltaT;
     double* thetaOut;     /* [6N] result, written when the request finishes; may be NULL */
     double* tauOut;       /* [6N] result, may be NULL */
+    const double* gradU;  /* [9N] prescribed grad(U) (the gradU argument of correct(),
+                             constitutiveEq.H:346-350); NULL: Gauss linear from U */
 } rt_request;
 
 int rt_server_create(rt_ctx* ctx, size_t hbm_budget, int max_inflight, rt_server** out);

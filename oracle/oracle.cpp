@@ -603,19 +603,24 @@ void orc_rhs(const orc_params* p, int32_t n, const double* gradU, const double* 
     }
 }
 
-void orc_assemble(const orc_mesh* m, const double* phi, const double* theta, const double* S,
-                  double deltaT, const int32_t* bcType, const double* bcValue, double* diag,
-                  double* lower, double* upper, double* source, double* iC, double* bC) {
+// fvm::ddt(theta) + fvm::div(phi, theta) == S (Oldroyd_BLog.C:141-164).
+// scheme: the GaussDefCmpw limiter; souT (NULL for upwind/none) the explicit
+// correction, entering the div matrix's source as -souT (:166).
+static void assembleImpl(const orc_mesh* m, const double* phi, const double* theta, const double* S,
+                         double deltaT, const int32_t* bcType, const double* bcValue, int scheme,
+                         const double* souT, double* diag, double* lower, double* upper,
+                         double* source, double* iC, double* bC) {
     const int N = m->nCells, F = m->nInternal;
     const double rDeltaT = 1.0 / deltaT;
+    const bool conv = scheme != ORC_DC_NONE;  // "none": fvm.diag() = 0, nothing else (:90)
     // fvm::ddt Euler
     std::vector<double> ddtDiag(N), divDiag(N, 0.0);
     for (int c = 0; c < N; ++c) ddtDiag[c] = rDeltaT * m->V[c];
     // fvm::div upwind (gaussDefCmpwConvectionScheme.C:93-100); pos(0) = 1
     for (int f = 0; f < F; ++f) {
         const double upw = phi[f] >= 0 ? 1.0 : 0.0;
-        lower[f] = -upw * phi[f];
-        upper[f] = (1.0 - upw) * phi[f];
+        lower[f] = conv ? -upw * phi[f] : 0.0;
+        upper[f] = conv ? (1.0 - upw) * phi[f] : 0.0;
     }
     for (int f = 0; f < F; ++f) {  // negSumDiag
         divDiag[m->owner[f]] -= lower[f];
@@ -629,7 +634,8 @@ void orc_assemble(const orc_mesh* m, const double* phi, const double* theta, con
             const double pphi = phi[(int64_t)F + b];
             for (int q = 0; q < 6; ++q) {
                 double ic = 0, bc = 0;
-                if (bcType[p] == ORC_BC_ZEROGRADIENT) {
+                if (!conv) {
+                } else if (bcType[p] == ORC_BC_ZEROGRADIENT) {
                     ic = pphi * 1.0;
                     bc = -pphi * 0.0;
                 } else if (bcType[p] == ORC_BC_FIXEDVALUE) {
@@ -640,11 +646,140 @@ void orc_assemble(const orc_mesh* m, const double* phi, const double* theta, con
                 bC[6 * (size_t)b + q] = bc;
             }
         }
-    // sources: ddt (rDeltaT*theta_old*V) + (== S): V*S
+    // sources: ddt (rDeltaT*theta_old*V) [+ div: -souT] + (== S): V*S
     for (int c = 0; c < N; ++c)
         for (int q = 0; q < 6; ++q) {
             const double sd = rDeltaT * theta[6 * (size_t)c + q] * m->V[c];
-            source[6 * (size_t)c + q] = sd + m->V[c] * S[6 * (size_t)c + q];
+            const double vs = m->V[c] * S[6 * (size_t)c + q];
+            source[6 * (size_t)c + q] = souT ? (sd + (-souT[6 * (size_t)c + q])) + vs : sd + vs;
+        }
+}
+
+void orc_assemble(const orc_mesh* m, const double* phi, const double* theta, const double* S,
+                  double deltaT, const int32_t* bcType, const double* bcValue, double* diag,
+                  double* lower, double* upper, double* source, double* iC, double* bC) {
+    assembleImpl(m, phi, theta, S, deltaT, bcType, bcValue, ORC_DC_UPWIND, nullptr, diag, lower, upper,
+                 source, iC, bC);
+}
+
+void orc_grad_gauss_symm(const orc_mesh* m, const double* th, const double* thB, double* g) {
+    // [OF9] gaussGrad::gradf on interpolate(vfc), linear weights, per component
+    const int N = m->nCells, F = m->nInternal;
+    std::fill(g, g + 18 * (size_t)N, 0.0);
+    for (int f = 0; f < F; ++f) {
+        const int o = m->owner[f], n = m->neighbour[f];
+        for (int q = 0; q < 6; ++q) {
+            const double tf = m->w[f] * (th[6 * (size_t)o + q] - th[6 * (size_t)n + q]) + th[6 * (size_t)n + q];
+            for (int i = 0; i < 3; ++i) {
+                const double v = m->Sf[3 * (size_t)f + i] * tf;
+                g[18 * (size_t)o + 3 * q + i] += v;
+                g[18 * (size_t)n + 3 * q + i] -= v;
+            }
+        }
+    }
+    for (int p = 0; p < m->nPatches; ++p) {
+        if (m->patchEmpty[p]) continue;
+        for (int k = 0; k < m->patchSize[p]; ++k) {
+            const int b = m->patchStart[p] + k;
+            const int64_t f = (int64_t)F + b;
+            const int c = m->owner[f];
+            for (int q = 0; q < 6; ++q)
+                for (int i = 0; i < 3; ++i)
+                    g[18 * (size_t)c + 3 * q + i] += m->Sf[3 * f + i] * thB[6 * (size_t)b + q];
+        }
+    }
+    for (int c = 0; c < N; ++c)
+        for (int q = 0; q < 18; ++q) g[18 * (size_t)c + q] /= m->V[c];
+}
+
+// limiters.H:38-100: alpha, beta per NVD segment and the segment bounds
+static int limiterTable(int scheme, double al[3], double be[3], double bl[2]) {
+    switch (scheme) {
+    case ORC_DC_CUBISTA:
+        al[0] = 7. / 4.; al[1] = 3. / 4.; al[2] = 1. / 4.;
+        be[0] = 0.; be[1] = 3. / 8.; be[2] = 3. / 4.;
+        bl[0] = 3. / 8.; bl[1] = 3. / 4.;
+        return 1;
+    case ORC_DC_MINMOD:
+        al[0] = 1.5; al[1] = .5; al[2] = .5;
+        be[0] = 0.; be[1] = .5; be[2] = .5;
+        bl[0] = .5; bl[1] = 1.;
+        return 1;
+    case ORC_DC_SMART:
+        al[0] = 3.; al[1] = 3. / 4.; al[2] = 0.;
+        be[0] = 0.; be[1] = 3. / 8.; be[2] = 1.;
+        bl[0] = 1. / 6.; bl[1] = 5. / 6.;
+        return 1;
+    case ORC_DC_WACEB:
+        al[0] = 2.; al[1] = 3. / 4.; al[2] = 0.;
+        be[0] = 0.; be[1] = 3. / 8.; be[2] = 1.;
+        bl[0] = 3. / 10.; bl[1] = 5. / 6.;
+        return 1;
+    case ORC_DC_SUPERBEE:
+        al[0] = 0.5; al[1] = 1.5; al[2] = 0.;
+        be[0] = 0.5; be[1] = 0.; be[2] = 1.;
+        bl[0] = 1. / 2.; bl[1] = 2. / 3.;
+        return 1;
+    default:
+        return 0;  // upwind / none: no correction
+    }
+}
+
+void orc_defc_faces(const orc_mesh* m, int32_t scheme, const double* phi, const double* th,
+                    const double* g, const double* C, double* dc) {
+    const int F = m->nInternal;
+    double al[3], be[3], bl[2];
+    if (!limiterTable(scheme, al, be, bl)) {
+        std::fill(dc, dc + 6 * (size_t)F, 0.0);
+        return;
+    }
+    const double swit = 1.0;  // onlyDCphi
+    for (int f = 0; f < F; ++f) {
+        const int o = m->owner[f], n = m->neighbour[f];
+        const double upw = phi[f] >= 0 ? 1.0 : 0.0;
+        const double d[3] = {C[3 * (size_t)n] - C[3 * (size_t)o], C[3 * (size_t)n + 1] - C[3 * (size_t)o + 1],
+                             C[3 * (size_t)n + 2] - C[3 * (size_t)o + 2]};
+        for (int q = 0; q < 6; ++q) {
+            const double* go = g + 18 * (size_t)o + 3 * q;
+            const double* gn = g + 18 * (size_t)n + 3 * q;
+            const double vo = th[6 * (size_t)o + q], vn = th[6 * (size_t)n + q];
+            double gu[3];
+            for (int i = 0; i < 3; ++i) gu[i] = go[i] * upw + (1.0 - upw) * gn[i];
+            const double gud = gu[0] * d[0] + gu[1] * d[1] + gu[2] * d[2];
+            const double god = go[0] * d[0] + go[1] * d[1] + go[2] * d[2];
+            const double gnd = gn[0] * d[0] + gn[1] * d[1] + gn[2] * d[2];
+            // :251-255 the normalised variable of the upwind cell, :257-260 the segment
+            const double phitc = 1.0 - ((vn - vo) / (2.0 * gud + 1e-18));
+            double alpha, beta;
+            if (phitc <= 0. || phitc >= 1.) {
+                alpha = 1.;
+                beta = 0.;
+            } else if (phitc < bl[0]) {
+                alpha = al[0];
+                beta = be[0];
+            } else if (phitc < bl[1]) {
+                alpha = al[1];
+                beta = be[1];
+            } else {
+                alpha = al[2];
+                beta = be[2];
+            }
+            // :262-266 limited face value minus the upwind value
+            dc[6 * (size_t)f + q] = (1.0 - alpha - beta) * (vn - 2.0 * god) * upw +
+                                    (1.0 - alpha - beta) * (vo + 2.0 * gnd) * (1.0 - upw) +
+                                    ((alpha - 1.0 * swit) * upw + beta * (1.0 - upw)) * vo +
+                                    (beta * upw + (alpha - 1.0 * swit) * (1.0 - upw)) * vn;
+        }
+    }
+}
+
+void orc_defc_sout(const orc_mesh* m, const double* phi, const double* dc, double* souT) {
+    const int N = m->nCells, F = m->nInternal;
+    std::fill(souT, souT + 6 * (size_t)N, 0.0);
+    for (int f = 0; f < F; ++f)  // :145-149
+        for (int q = 0; q < 6; ++q) {
+            souT[6 * (size_t)m->owner[f] + q] += dc[6 * (size_t)f + q] * phi[f];
+            souT[6 * (size_t)m->neighbour[f] + q] -= dc[6 * (size_t)f + q] * phi[f];
         }
 }
 
@@ -793,6 +928,15 @@ void orc_correct(const orc_mesh* m, const orc_params* p, const double* U, const 
                  const double* bcValue, const int32_t valid[6], const orc_controls* c,
                  double relaxFactor, double* theta, double* eigVals, double* eigVecs, double* tau,
                  orc_perf perf[6]) {
+    orc_correct_scheme(m, p, U, Ub, phi, gradU, deltaT, bcType, bcValue, valid, c, relaxFactor,
+                       ORC_DC_UPWIND, nullptr, theta, eigVals, eigVecs, tau, perf);
+}
+
+void orc_correct_scheme(const orc_mesh* m, const orc_params* p, const double* U, const double* Ub,
+                        const double* phi, const double* gradU, double deltaT, const int32_t* bcType,
+                        const double* bcValue, const int32_t valid[6], const orc_controls* c,
+                        double relaxFactor, int32_t scheme, const double* C, double* theta,
+                        double* eigVals, double* eigVecs, double* tau, orc_perf perf[6]) {
     const int N = m->nCells, F = m->nInternal, NB = m->nBoundary;
     std::vector<double> L;
     if (!gradU) {
@@ -804,8 +948,28 @@ void orc_correct(const orc_mesh* m, const orc_params* p, const double* U, const 
     orc_rhs(p, N, gradU, theta, eigVals, eigVecs, S.data(), f.data());
     std::vector<double> diag(N), lower(F), upper(F), source(6 * (size_t)N), iC(6 * (size_t)NB),
         bC(6 * (size_t)NB);
-    orc_assemble(m, phi, theta, S.data(), deltaT, bcType, bcValue, diag.data(), lower.data(),
-                 upper.data(), source.data(), iC.data(), bC.data());
+    std::vector<double> souT;
+    double al[3], be[3], bl[2];
+    if (limiterTable(scheme, al, be, bl)) {
+        // explicit correction from the old theta: patch values (zeroGradient:
+        // the cell value; fixedValue: the given value), Gauss gradients, face
+        // corrections, souT
+        std::vector<double> thB(6 * (size_t)NB, 0.0), g(18 * (size_t)N), dc(6 * (size_t)F);
+        for (int pp = 0; pp < m->nPatches; ++pp)
+            for (int k = 0; k < m->patchSize[pp]; ++k) {
+                const int b = m->patchStart[pp] + k;
+                const int cell = m->owner[(int64_t)F + b];
+                for (int q = 0; q < 6; ++q)
+                    thB[6 * (size_t)b + q] = bcType[pp] == ORC_BC_FIXEDVALUE ? bcValue[6 * (size_t)b + q]
+                                                                              : theta[6 * (size_t)cell + q];
+            }
+        orc_grad_gauss_symm(m, theta, thB.data(), g.data());
+        orc_defc_faces(m, scheme, phi, theta, g.data(), C, dc.data());
+        souT.resize(6 * (size_t)N);
+        orc_defc_sout(m, phi, dc.data(), souT.data());
+    }
+    assembleImpl(m, phi, theta, S.data(), deltaT, bcType, bcValue, scheme, souT.empty() ? nullptr : souT.data(),
+                 diag.data(), lower.data(), upper.data(), source.data(), iC.data(), bC.data());
     if (relaxFactor > 0)
         orc_relax(m, bcType, relaxFactor, theta, diag.data(), lower.data(), upper.data(),
                   source.data(), iC.data());

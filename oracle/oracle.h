@@ -85,6 +85,29 @@ void orc_assemble(const orc_mesh* m, const double* phi, const double* theta, con
                   double* lower, double* upper, double* source, double* internalCoeffs,
                   double* boundaryCoeffs);
 
+/* GaussDefCmpw limited schemes (gaussDefCmpwConvectionScheme.H:68 scheme_,
+ * limiters.H:38-100). */
+enum { ORC_DC_UPWIND = 0, ORC_DC_CUBISTA = 1, ORC_DC_MINMOD = 2, ORC_DC_SMART = 3, ORC_DC_WACEB = 4,
+       ORC_DC_SUPERBEE = 5, ORC_DC_NONE = 6 };
+
+/* [OF9] fvc::grad of each symmTensor component, Gauss linear, as phifDefC
+ * takes it (gaussDefCmpwConvectionScheme.C:243-250): grad[18*N], per cell the
+ * 6 components (xx xy xz yy yz zz) x 3 derivatives (d/dx d/dy d/dz).
+ * thetaB[6*nBoundary]: patch values of theta on boundary faces (empty
+ * patches are skipped). */
+void orc_grad_gauss_symm(const orc_mesh* m, const double* theta, const double* thetaB, double* grad);
+
+/* Deferred-correction face values of phifDefC with onlyDCphi = true
+ * (gaussDefCmpwConvectionScheme.C:238-283): dc[6*nInternal] = limited HRS
+ * face value minus the upwind value, per component.  C[3N] cell centres. */
+void orc_defc_faces(const orc_mesh* m, int32_t scheme, const double* phi, const double* theta,
+                    const double* grad, const double* C, double* dc);
+
+/* The explicit correction source of fvmDiv (gaussDefCmpwConvectionScheme.C:120-169):
+ * souT[6N] = sum over faces of +-dc*phi (owner +, neighbour -); the matrix
+ * source is -souT.  No coupled patches (they are out of scope). */
+void orc_defc_sout(const orc_mesh* m, const double* phi, const double* dc, double* souT);
+
 /* [OF9] fvMatrix::relax(1) on the assembled system (Oldroyd_BLog.C:166). */
 void orc_relax(const orc_mesh* m, const int32_t* bcType, double alpha, const double* psi,
                double* diag, const double* lower, const double* upper, double* source,
@@ -129,6 +152,15 @@ void orc_correct(const orc_mesh* m, const orc_params* p, const double* U, const 
                  const double* bcValue, const int32_t validComponents[6],
                  const orc_controls* c, double relaxFactor, double* theta, double* eigVals,
                  double* eigVecs, double* tau, orc_perf perf[6]);
+
+/* orc_correct with the div(phi,theta) scheme of fvSchemes: scheme is an
+ * ORC_DC_* value; C[3N] cell centres (needed unless the scheme is upwind or
+ * none).  ORC_DC_NONE drops the convection term (fvmDiv :90). */
+void orc_correct_scheme(const orc_mesh* m, const orc_params* p, const double* U, const double* Ub,
+                        const double* phi, const double* gradU, double deltaT, const int32_t* bcType,
+                        const double* bcValue, const int32_t validComponents[6],
+                        const orc_controls* c, double relaxFactor, int32_t scheme, const double* C,
+                        double* theta, double* eigVals, double* eigVecs, double* tau, orc_perf perf[6]);
 
 #ifdef __cplusplus
 }

@@ -74,6 +74,11 @@ def lib():
         L.orc_tau.argtypes = [vp, C.c_int32, vp, vp, vp, vp]
         L.orc_correct.argtypes = [vp, vp, vp, vp, vp, vp, C.c_double, vp, vp, vp, vp,
                                   C.c_double, vp, vp, vp, vp, vp]
+        L.orc_correct_scheme.argtypes = [vp, vp, vp, vp, vp, vp, C.c_double, vp, vp, vp, vp,
+                                         C.c_double, C.c_int32, vp, vp, vp, vp, vp, vp]
+        L.orc_grad_gauss_symm.argtypes = [vp] * 4
+        L.orc_defc_faces.argtypes = [vp, C.c_int32] + [vp] * 5
+        L.orc_defc_sout.argtypes = [vp] * 4
         _LIB = L
     return _LIB
 
@@ -119,6 +124,36 @@ def grad(m, U, Ub):
     Ub = np.ascontiguousarray(Ub) if m.nBoundary else np.zeros((1, 3))
     lib().orc_grad_gauss(om.ref, _p(np.ascontiguousarray(U)), _p(Ub), _p(g))
     return g
+
+
+# GaussDefCmpw limited schemes (limiters.H:38-100)
+SCHEMES = {"upwind": 0, "cubista": 1, "minmod": 2, "smart": 3, "waceb": 4, "superbee": 5, "none": 6}
+
+
+def grad_symm(m, theta, thetaB):
+    """Gauss-linear gradient of each theta component: [N, 6, 3]."""
+    om = OracleMesh(m)
+    g = np.empty((m.nCells, 6, 3))
+    tb = np.ascontiguousarray(thetaB) if m.nBoundary else np.zeros((1, 6))
+    lib().orc_grad_gauss_symm(om.ref, _p(np.ascontiguousarray(theta)), _p(tb), _p(g))
+    return g
+
+
+def defc_faces(m, scheme, phi, theta, g, C=None):
+    """Deferred-correction face values (limited minus upwind), [nInternal, 6]."""
+    om = OracleMesh(m)
+    dc = np.empty((m.nInternal, 6))
+    Cc = np.ascontiguousarray(m.C if C is None else C)
+    lib().orc_defc_faces(om.ref, SCHEMES[scheme], _p(np.ascontiguousarray(phi)),
+                         _p(np.ascontiguousarray(theta)), _p(np.ascontiguousarray(g)), _p(Cc), _p(dc))
+    return dc
+
+
+def defc_sout(m, phi, dc):
+    om = OracleMesh(m)
+    s = np.empty((m.nCells, 6))
+    lib().orc_defc_sout(om.ref, _p(np.ascontiguousarray(phi)), _p(np.ascontiguousarray(dc)), _p(s))
+    return s
 
 
 def rhs(p, gradU, theta, eigVals, eigVecs):
@@ -188,8 +223,9 @@ def tau(p, eigVals, eigVecs, fOld=None):
 
 
 def correct(m, p, fields, dt, ctl=None, bc=None, bcval=None, gradU=None, relax=0.0,
-            use_grad=False):
-    """One constitutiveEq::correct(); returns (theta, eigVals, eigVecs, tau, perf)."""
+            use_grad=False, scheme="upwind"):
+    """One constitutiveEq::correct(); returns (theta, eigVals, eigVecs, tau, perf).
+    scheme: the GaussDefCmpw limiter of div(phi,theta)."""
     om = OracleMesh(m)
     bc = default_bc(m) if bc is None else bc
     NB = m.nBoundary
@@ -202,8 +238,9 @@ def correct(m, p, fields, dt, ctl=None, bc=None, bcval=None, gradU=None, relax=0
     perf = (Perf * 6)()
     g = gradU if gradU is not None else (fields.gradU if use_grad else None)
     Ub = fields.Ub if NB else np.zeros((1, 3))
-    lib().orc_correct(om.ref, C.byref(p), _p(fields.U), _p(Ub), _p(fields.phi),
-                      _p(None if g is None else np.ascontiguousarray(g)), dt, _p(bc), _p(bcval),
-                      _p(m.validComponents), C.byref(ctl), relax, _p(th), _p(ev), _p(vec), _p(t),
-                      perf)
+    Cc = np.ascontiguousarray(m.C)
+    lib().orc_correct_scheme(om.ref, C.byref(p), _p(fields.U), _p(Ub), _p(fields.phi),
+                             _p(None if g is None else np.ascontiguousarray(g)), dt, _p(bc),
+                             _p(bcval), _p(m.validComponents), C.byref(ctl), relax, SCHEMES[scheme],
+                             _p(Cc), _p(th), _p(ev), _p(vec), _p(t), perf)
     return th, ev, vec, t, [perf[i] for i in range(6)]

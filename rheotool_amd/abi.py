@@ -46,7 +46,8 @@ class MeshDesc(C.Structure):
     _fields_ = [("nCells", C.c_int32), ("nInternalFaces", C.c_int32),
                 ("nBoundaryFaces", C.c_int32), ("nPatches", C.c_int32),
                 ("owner", C.c_void_p), ("neighbour", C.c_void_p), ("Sf", C.c_void_p),
-                ("weights", C.c_void_p), ("V", C.c_void_p), ("patches", C.c_void_p)]
+                ("weights", C.c_void_p), ("V", C.c_void_p), ("patches", C.c_void_p),
+                ("C", C.c_void_p)]
 
 
 class SolverPerf(C.Structure):
@@ -64,7 +65,8 @@ class Request(C.Structure):
     _fields_ = [("mesh", C.c_void_p), ("dict", C.c_char_p), ("controls", C.c_char_p),
                 ("theta", C.c_void_p), ("eigVals", C.c_void_p), ("eigVecs", C.c_void_p),
                 ("U", C.c_void_p), ("Ub", C.c_void_p), ("phi", C.c_void_p),
-                ("deltaT", C.c_double), ("thetaOut", C.c_void_p), ("tauOut", C.c_void_p)]
+                ("deltaT", C.c_double), ("thetaOut", C.c_void_p), ("tauOut", C.c_void_p),
+                ("gradU", C.c_void_p)]
 
 
 class KernelStat(C.Structure):
@@ -106,6 +108,7 @@ def lib():
             "rt_model_set_flow": (i32, [vp, vp, vp, vp]),
             "rt_model_set_solver": (i32, [vp, C.c_char_p]),
             "rt_model_set_relax": (i32, [vp, C.c_double]),
+            "rt_model_set_scheme": (i32, [vp, C.c_char_p]),
             "rt_model_correct": (i32, [vp, C.c_double, vp]),
             "rt_model_get": (i32, [vp, i32, vp]),
             "rt_model_solver_perf": (i32, [vp, vp]),
@@ -220,17 +223,7 @@ class Mesh:
     def __init__(self, ctx: Context, m, renumber=RT_RENUMBER_AUTO):
         self.ctx = ctx
         self.src = m
-        self._keep = dict(owner=np.ascontiguousarray(m.owner, np.int32),
-                          nbr=np.ascontiguousarray(m.neighbour, np.int32),
-                          Sf=_f64(m.Sf), w=_f64(m.w), V=_f64(m.V))
-        pt = (Patch * max(1, len(m.patchStart)))()
-        for i in range(len(m.patchStart)):
-            pt[i] = Patch(int(m.patchStart[i]), int(m.patchSize[i]), int(m.patchKind[i]))
-        self._pt = pt
-        k = self._keep
-        d = MeshDesc(m.nCells, m.nInternal, m.nBoundary, len(m.patchStart), _p(k["owner"]),
-                     _p(k["nbr"]), _p(k["Sf"]), _p(k["w"]), _p(k["V"]),
-                     C.cast(pt, C.c_void_p))
+        d, self._keep = mesh_desc(m)
         h = C.c_void_p()
         ctx.check(lib().rt_mesh_create(ctx.h, C.byref(d), renumber, C.byref(h)), "rt_mesh_create")
         self.h = h
@@ -289,6 +282,10 @@ class ConstitutiveEq:
 
     def set_relax(self, factor: float):
         self.ctx.check(lib().rt_model_set_relax(self.h, factor), "set_relax")
+
+    def set_scheme(self, div_scheme: str):
+        """fvSchemes div(phi,theta): "GaussDefCmpw cubista", "upwind", ..."""
+        self.ctx.check(lib().rt_model_set_scheme(self.h, div_scheme.encode()), "set_scheme")
 
     def correct(self, deltaT, gradU=None):
         g = None if gradU is None else _f64(gradU, (self.N, 3, 3))
@@ -361,14 +358,14 @@ def mesh_desc(m):
     """rt_mesh_desc for a cases.Mesh; returns (desc, keepalive)."""
     keep = dict(owner=np.ascontiguousarray(m.owner, np.int32),
                 nbr=np.ascontiguousarray(m.neighbour, np.int32), Sf=_f64(m.Sf), w=_f64(m.w),
-                V=_f64(m.V))
+                V=_f64(m.V), C=_f64(m.C) if getattr(m, "C", None) is not None else None)
     pt = (Patch * max(1, len(m.patchStart)))()
     for i in range(len(m.patchStart)):
         pt[i] = Patch(int(m.patchStart[i]), int(m.patchSize[i]), int(m.patchKind[i]))
     keep["pt"] = pt
     d = MeshDesc(m.nCells, m.nInternal, m.nBoundary, len(m.patchStart), _p(keep["owner"]),
                  _p(keep["nbr"]), _p(keep["Sf"]), _p(keep["w"]), _p(keep["V"]),
-                 C.cast(pt, C.c_void_p))
+                 C.cast(pt, C.c_void_p), _p(keep["C"]))
     return d, keep
 
 
@@ -382,18 +379,21 @@ class Server:
         self.ctx, self.h = ctx, h
         self.held = {}  # ticket -> (input arrays, theta out, tau out): the server reads/writes them
 
-    def submit(self, mesh: "Mesh", fields, dict_text, deltaT, controls=None) -> int:
+    def submit(self, mesh: "Mesh", fields, dict_text, deltaT, controls=None, gradU=None) -> int:
         """mesh: an uploaded Mesh (shared by any number of requests).  The
-        field arrays are referenced by the server until the ticket finishes."""
+        field arrays are referenced by the server until the ticket finishes.
+        gradU: optional prescribed velocity gradient [N, 3, 3]."""
         m = mesh.src
         arrs = [_f64(fields.theta), _f64(fields.eigVals), _f64(fields.eigVecs), _f64(fields.U),
                 _f64(fields.Ub) if m.nBoundary else None, _f64(fields.phi)]
+        g = None if gradU is None else _f64(gradU)
         th_out, tau_out = np.empty((m.nCells, 6)), np.empty((m.nCells, 6))
         q = Request(mesh.h, dict_text.encode(), None if controls is None else controls.encode(),
-                    *[_p(a) for a in arrs], deltaT, _p(th_out), _p(tau_out))
+                    *[_p(a) for a in arrs], deltaT, _p(th_out), _p(tau_out), _p(g))
+        arrs.append(g)
         t = C.c_int64()
         self.ctx.check(lib().rt_server_submit(self.h, C.byref(q), C.byref(t)), "rt_server_submit")
-        self.held[t.value] = (arrs, th_out, tau_out)
+        self.held[t.value] = (arrs, th_out, tau_out, mesh)  # the mesh must outlive the request
         return t.value
 
     def progress(self) -> int:
@@ -409,7 +409,7 @@ class Server:
         rc = lib().rt_server_poll(self.h, ticket, None, None, perf, C.byref(lat))
         if rc == 0:
             return None
-        _, th, tau = self.held.pop(ticket)
+        _, th, tau, _ = self.held.pop(ticket)
         if rc < 0:
             raise RheoError(-rc, lib().rt_last_error(self.ctx.h).decode())
         return dict(theta=th, tau=tau, perf=[perf[i] for i in range(6)], latency_ms=lat.value)

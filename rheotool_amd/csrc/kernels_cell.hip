@@ -163,12 +163,12 @@ __device__ __forceinline__ void cell_rhs(const ModelParams& p, int invMask, cons
 constexpr int STAGE_W = 8;
 
 template <class T, int KIND>
-__global__ void __launch_bounds__(BLK) k_assemble(AssembleArgs a) {
+__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3, 8))) k_assemble(AssembleArgs a) {
     // per-thread running sums of the initial residual live in LDS, not in
     // registers: the row body needs every VGPR it can get for occupancy
-    __shared__ double qs[18][BLK];
+    __shared__ double qs[19][BLK];
 #pragma unroll
-    for (int i = 0; i < 18; ++i) qs[i][threadIdx.x] = 0;
+    for (int i = 0; i < 19; ++i) qs[i][threadIdx.x] = 0;
     double xbar[6];
 #pragma unroll
     for (int c = 0; c < 6; ++c) xbar[c] = a.cs[c].xbar;
@@ -176,8 +176,9 @@ __global__ void __launch_bounds__(BLK) k_assemble(AssembleArgs a) {
     T* rOut = static_cast<T*>(a.r);
     T* r0Out = static_cast<T*>(a.r0);
     for (int64_t it = 0;; ++it) {
-        const int64_t t0 = tile_of(it) * BLK;
-        if (t0 >= a.N) break;
+        const int64_t tl = tile_of(it, a.N);
+        if (tl < 0) break;
+        const int64_t t0 = tl * BLK;
         const int32_t r = (int32_t)(t0 + threadIdx.x);
         if (r >= a.N) continue;
         const int32_t N = a.N;
@@ -207,11 +208,12 @@ __global__ void __launch_bounds__(BLK) k_assemble(AssembleArgs a) {
             }
             const bool own = fid >= 0;
             const int64_t ec = base + (int64_t)SLICE * slot;  // column-order slot
-            const double ph = a.ePhi[e];
+            const double ph = a.noConv ? 0.0 : a.ePhi[e];
             const double upw = ph >= 0 ? 1.0 : 0.0;
             const double lower = -upw * ph;
             const double upper = (1.0 - upw) * ph;
             const double val = own ? upper : lower;
+            const int32_t c = a.ellCol[ec];
             if (staged) {
 #pragma unroll
                 for (int j = 0; j < STAGE_W; ++j)
@@ -222,7 +224,6 @@ __global__ void __launch_bounds__(BLK) k_assemble(AssembleArgs a) {
             divDiag -= own ? lower : upper;
             sumOff += fabs(val);
             sumAv += val;
-            const int32_t c = a.ellCol[ec];
             if (val != 0.0) {  // upwind: only the upstream side of a face couples
                 double thc[6];
                 ld6(a.theta + (int64_t)NCMP * c, thc);
@@ -247,8 +248,29 @@ __global__ void __launch_bounds__(BLK) k_assemble(AssembleArgs a) {
                     for (int j = 0; j < 3; ++j) g[3 * i + j] += sf[i] * uf[j];
             }
         }
-        if (staged)
-            for (int k = 0; k < width; ++k) a.aval[base + (int64_t)SLICE * k] = av[k];
+        // Upwinding leaves one nonzero per face, on the downstream row, so many
+        // column-order slots are zero in every row of a slice (the 64 rows are
+        // the lanes of this wavefront): bit k of the slice's live-slot mask is
+        // set when slot k holds a nonzero in any row, and the solve's kernels
+        // skip the others without loading them.  Slots past 31 count as live.
+        if (staged) {
+            uint32_t mask = 0;
+            for (int k = 0; k < width; ++k) {
+                double v = 0.0;
+#pragma unroll
+                for (int j = 0; j < STAGE_W; ++j)
+                    if (j == k) v = av[j];
+                a.aval[base + (int64_t)SLICE * k] = v;
+                if (__ballot(v != 0.0) != 0) mask |= 1u << k;
+            }
+            if (lane == 0) {
+                a.slotMask[s] = mask;
+                qs[18][threadIdx.x] += (double)(SLICE * __popc(mask));
+            }
+        } else if (lane == 0) {
+            a.slotMask[s] = 0xffffffffu;
+            qs[18][threadIdx.x] += (double)(SLICE * width);
+        }
         const int32_t b0 = a.bPtr[r], b1 = a.bPtr[r + 1];
         const int64_t NB = a.NB > 0 ? a.NB : 1;
         if (!haveGrad) {
@@ -280,15 +302,23 @@ __global__ void __launch_bounds__(BLK) k_assemble(AssembleArgs a) {
         // matrix diagonal: ddt + div, optional relax, then boundary internalCoeffs
         double D = a.rDeltaT * vol + divDiag;
         double src[6];
+        if (a.souT) {
+            // fvm::ddt + fvm::div (source -souT) == S, summed in that order
+            double so[6];
+            ld6(a.souT + (int64_t)NCMP * r, so);
 #pragma unroll
-        for (int qq = 0; qq < 6; ++qq) src[qq] = a.rDeltaT * th[qq] * vol + vol * S[qq];
+            for (int qq = 0; qq < 6; ++qq) src[qq] = (a.rDeltaT * th[qq] * vol + (-so[qq])) + vol * S[qq];
+        } else {
+#pragma unroll
+            for (int qq = 0; qq < 6; ++qq) src[qq] = a.rDeltaT * th[qq] * vol + vol * S[qq];
+        }
         if (a.relax > 0) {
             const double D0 = D;
             for (int32_t qb = b0; qb < b1; ++qb) {
                 const int32_t bf = a.bFace[qb];
                 const int t = a.bcType[a.bPatch[bf]];
                 if (t == RT_BC_EMPTY) continue;
-                const double ph = a.phiB[bf];
+                const double ph = a.noConv ? 0.0 : a.phiB[bf];
                 D += t == RT_BC_ZEROGRADIENT ? fabs(ph * 1.0) : fabs(ph * 0.0);
             }
             D = fmax(fabs(D), sumOff);
@@ -297,7 +327,7 @@ __global__ void __launch_bounds__(BLK) k_assemble(AssembleArgs a) {
                 const int32_t bf = a.bFace[qb];
                 const int t = a.bcType[a.bPatch[bf]];
                 if (t == RT_BC_EMPTY) continue;
-                const double ph = a.phiB[bf];
+                const double ph = a.noConv ? 0.0 : a.phiB[bf];
                 D -= t == RT_BC_ZEROGRADIENT ? ph * 1.0 : ph * 0.0;
             }
 #pragma unroll
@@ -307,7 +337,7 @@ __global__ void __launch_bounds__(BLK) k_assemble(AssembleArgs a) {
             const int32_t bf = a.bFace[qb];
             const int t = a.bcType[a.bPatch[bf]];
             if (t == RT_BC_EMPTY) continue;
-            const double ph = a.phiB[bf];
+            const double ph = a.noConv ? 0.0 : a.phiB[bf];
             if (t == RT_BC_ZEROGRADIENT) {
                 D += ph * 1.0;
             } else {
@@ -341,18 +371,20 @@ __global__ void __launch_bounds__(BLK) k_assemble(AssembleArgs a) {
         st6(rOut + (int64_t)NCMP * r, rt);
         st6(r0Out + (int64_t)NCMP * r, rt);
     }
-    double q[18];
+    double q[19];
 #pragma unroll
-    for (int i = 0; i < 18; ++i) q[i] = qs[i][threadIdx.x];
-    write_partials<18>(q, a.partials);
+    for (int i = 0; i < 19; ++i) q[i] = qs[i][threadIdx.x];
+    write_partials<19>(q, a.partials);
 }
 
 __global__ void __launch_bounds__(BLK) k_fin_assemble(AssembleArgs a, int G) {
-    double sums[18];
-    sum_partials<18>(a.partials, G, sums);
-    if (threadIdx.x == 0)
+    double sums[19];
+    sum_partials<19>(a.partials, G, sums);
+    if (threadIdx.x == 0) {
         for (int c = 0; c < 6; ++c)
             if (a.valid[c] > 0) finish_initial(a.cs[c], a.ctl, sums[c], sums[6 + c], sums[12 + c]);
+        *a.nnzOut = sums[18];
+    }
 }
 
 // ----------------------------------------------------------- calcEig + tau
@@ -550,6 +582,186 @@ __global__ void __launch_bounds__(BLK) k_copy(int64_t n16, const double2* __rest
 
 // ---------------------------------------------------------------- launchers
 static inline unsigned nblk(int64_t n) { return (unsigned)((n + BLK - 1) / BLK); }
+
+// ------------------------------------------------ GaussDefCmpw correction
+// Gauss-linear gradient of each theta component ([OF9] gaussGrad::gradf on
+// the linear interpolate, as phifDefC takes it, gaussDefCmpwConvectionScheme.C:
+// 243-250): per row, faces in increasing face id (the reference's scatter
+// order), then the row's boundary faces (zeroGradient: the cell value,
+// fixedValue: the patch value), then / V.  gradTh[r][3q + i] = d theta_q / dx_i.
+__global__ void __launch_bounds__(BLK) k_grad_theta(AssembleArgs a, double* __restrict__ gradTh) {
+    const int64_t NB = a.NB > 0 ? a.NB : 1;
+    for (int64_t it = 0;; ++it) {
+        const int64_t tl = tile_of(it, a.N);
+        if (tl < 0) break;
+        const int32_t r = (int32_t)(tl * BLK + threadIdx.x);
+        if (r >= a.N) continue;
+        double th[6];
+        ld6(a.theta + (int64_t)NCMP * r, th);
+        double g[18];
+#pragma unroll
+        for (int q = 0; q < 18; ++q) g[q] = 0.0;
+        const int s = r >> 6, lane = r & 63;
+        const int64_t base = a.ellOff[s] + lane;
+        const int width = a.ellWidth[s];
+        for (int k = 0; k < width; ++k) {
+            const int64_t e = base + (int64_t)SLICE * k;
+            const int32_t fid = a.ellFid[e];
+            if (fid == RT_FID_PAD) continue;
+            const bool own = fid >= 0;
+            const int32_t c = a.ellCol[base + (int64_t)SLICE * a.ellSlot[e]];
+            double thc[6];
+            ld6(a.theta + (int64_t)NCMP * c, thc);
+            const double wf = a.eW[e];
+            const double sf[3] = {a.eSf[e], a.eSf[a.nEll + e], a.eSf[2 * a.nEll + e]};
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                const double vo = own ? th[q] : thc[q], vn = own ? thc[q] : th[q];
+                const double tf = wf * (vo - vn) + vn;
+#pragma unroll
+                for (int i = 0; i < 3; ++i) g[3 * q + i] += sf[i] * tf;
+            }
+        }
+        const int32_t b0 = a.bPtr[r], b1 = a.bPtr[r + 1];
+        for (int32_t qb = b0; qb < b1; ++qb) {
+            const int32_t bf = a.bFace[qb];
+            const int t = a.bcType[a.bPatch[bf]];
+            if (t == RT_BC_EMPTY) continue;
+            const double sf[3] = {a.bSf[bf], a.bSf[NB + bf], a.bSf[2 * NB + bf]};
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                const double tb = t == RT_BC_FIXEDVALUE ? a.bcVal[(int64_t)q * NB + bf] : th[q];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) g[3 * q + i] += sf[i] * tb;
+            }
+        }
+        const double vol = a.V[r];
+        double2* out = reinterpret_cast<double2*>(gradTh + (int64_t)18 * r);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) out[q] = make_double2(g[2 * q] / vol, g[2 * q + 1] / vol);
+    }
+}
+
+// limiters.H:38-100: NVD segments (alpha, beta) and their bounds
+template <int S> struct Limiter;
+template <> struct Limiter<DS_CUBISTA> {
+    static constexpr double al[3] = {7. / 4., 3. / 4., 1. / 4.}, be[3] = {0., 3. / 8., 3. / 4.},
+                            bl[2] = {3. / 8., 3. / 4.};
+};
+template <> struct Limiter<DS_MINMOD> {
+    static constexpr double al[3] = {1.5, .5, .5}, be[3] = {0., .5, .5}, bl[2] = {.5, 1.};
+};
+template <> struct Limiter<DS_SMART> {
+    static constexpr double al[3] = {3., 3. / 4., 0.}, be[3] = {0., 3. / 8., 1.}, bl[2] = {1. / 6., 5. / 6.};
+};
+template <> struct Limiter<DS_WACEB> {
+    static constexpr double al[3] = {2., 3. / 4., 0.}, be[3] = {0., 3. / 8., 1.}, bl[2] = {3. / 10., 5. / 6.};
+};
+template <> struct Limiter<DS_SUPERBEE> {
+    static constexpr double al[3] = {0.5, 1.5, 0.}, be[3] = {0.5, 0., 1.}, bl[2] = {1. / 2., 2. / 3.};
+};
+
+// Explicit correction of fvmDiv (gaussDefCmpwConvectionScheme.C:120-169) with
+// the face values of phifDefC, onlyDCphi (:238-266): per row, in face order,
+// souT += dc*phi on the owner side and -= on the neighbour side.  Only the
+// upwind cell's gradient enters a face (the other one is multiplied by 0), so
+// a row gathers a neighbour's gradient only where the neighbour is upwind.
+template <int S>
+__global__ void __launch_bounds__(BLK) k_defc(AssembleArgs a, const double* __restrict__ C4,
+                                              const double* __restrict__ gradTh, double* __restrict__ souT) {
+    using L = Limiter<S>;
+    for (int64_t it = 0;; ++it) {
+        const int64_t tl = tile_of(it, a.N);
+        if (tl < 0) break;
+        const int32_t r = (int32_t)(tl * BLK + threadIdx.x);
+        if (r >= a.N) continue;
+        double th[6];
+        ld6(a.theta + (int64_t)NCMP * r, th);
+        const double2 cr01 = reinterpret_cast<const double2*>(C4)[2 * (int64_t)r];
+        const double Cr[3] = {cr01.x, cr01.y, C4[4 * (int64_t)r + 2]};
+        double so[6] = {0, 0, 0, 0, 0, 0};
+        const int s = r >> 6, lane = r & 63;
+        const int64_t base = a.ellOff[s] + lane;
+        const int width = a.ellWidth[s];
+        for (int k = 0; k < width; ++k) {
+            const int64_t e = base + (int64_t)SLICE * k;
+            const int32_t fid = a.ellFid[e];
+            if (fid == RT_FID_PAD) continue;
+            const bool own = fid >= 0;
+            const int32_t c = a.ellCol[base + (int64_t)SLICE * a.ellSlot[e]];
+            const double ph = a.ePhi[e];
+            const double upw = ph >= 0 ? 1.0 : 0.0;
+            const double2 cc01 = reinterpret_cast<const double2*>(C4)[2 * (int64_t)c];
+            const double Cc[3] = {cc01.x, cc01.y, C4[4 * (int64_t)c + 2]};
+            double d[3];  // C[neighbour] - C[owner]
+#pragma unroll
+            for (int i = 0; i < 3; ++i) d[i] = own ? Cc[i] - Cr[i] : Cr[i] - Cc[i];
+            double thc[6];
+            ld6(a.theta + (int64_t)NCMP * c, thc);
+            const bool rowUp = own == (upw == 1.0);
+            const double* gU = gradTh + (int64_t)18 * (rowUp ? r : c);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                const double vo = own ? th[q] : thc[q], vn = own ? thc[q] : th[q];
+                const double gud = gU[3 * q] * d[0] + gU[3 * q + 1] * d[1] + gU[3 * q + 2] * d[2];
+                const double phitc = 1.0 - ((vn - vo) / (2.0 * gud + 1e-18));
+                double alpha, beta;
+                if (phitc <= 0. || phitc >= 1.) {
+                    alpha = 1.;
+                    beta = 0.;
+                } else if (phitc < L::bl[0]) {
+                    alpha = L::al[0];
+                    beta = L::be[0];
+                } else if (phitc < L::bl[1]) {
+                    alpha = L::al[1];
+                    beta = L::be[1];
+                } else {
+                    alpha = L::al[2];
+                    beta = L::be[2];
+                }
+                const double god = upw == 1.0 ? gud : 0.0, gnd = upw == 1.0 ? 0.0 : gud;
+                const double dc = (1.0 - alpha - beta) * (vn - 2.0 * god) * upw +
+                                  (1.0 - alpha - beta) * (vo + 2.0 * gnd) * (1.0 - upw) +
+                                  ((alpha - 1.0) * upw + beta * (1.0 - upw)) * vo +
+                                  (beta * upw + (alpha - 1.0) * (1.0 - upw)) * vn;
+                const double fl = dc * ph;
+                so[q] = own ? so[q] + fl : so[q] - fl;
+            }
+        }
+        st6(souT + (int64_t)NCMP * r, so);
+    }
+}
+
+void launch_defc(Ctx* ctx, const AssembleArgs& a, int scheme, const double* C4, double* gradTh, double* souT,
+                 int grid) {
+    const double N = a.N, ne = (double)a.nEll;
+    {
+        // theta own + gathers (counted once), face data, boundary, gradient out
+        ProfScope ps(ctx, "k_grad_theta", N * (48 + 8 + 144) + ne * (4 + 1 + 4 + 32));
+        hipLaunchKernelGGL(k_grad_theta, dim3(grid), dim3(BLK), 0, ctx->stream, a, gradTh);
+    }
+    ProfScope ps(ctx, "k_defc", N * (48 + 32 + 144 + 48) + ne * (4 + 1 + 4 + 8));
+    switch (scheme) {
+    case DS_CUBISTA:
+        hipLaunchKernelGGL(k_defc<DS_CUBISTA>, dim3(grid), dim3(BLK), 0, ctx->stream, a, C4, gradTh, souT);
+        break;
+    case DS_MINMOD:
+        hipLaunchKernelGGL(k_defc<DS_MINMOD>, dim3(grid), dim3(BLK), 0, ctx->stream, a, C4, gradTh, souT);
+        break;
+    case DS_SMART:
+        hipLaunchKernelGGL(k_defc<DS_SMART>, dim3(grid), dim3(BLK), 0, ctx->stream, a, C4, gradTh, souT);
+        break;
+    case DS_WACEB:
+        hipLaunchKernelGGL(k_defc<DS_WACEB>, dim3(grid), dim3(BLK), 0, ctx->stream, a, C4, gradTh, souT);
+        break;
+    case DS_SUPERBEE:
+        hipLaunchKernelGGL(k_defc<DS_SUPERBEE>, dim3(grid), dim3(BLK), 0, ctx->stream, a, C4, gradTh, souT);
+        break;
+    default:
+        throw Error{RT_E_BAD_ARG, "launch_defc: not a limited scheme"};
+    }
+    RT_HIP(hipGetLastError());
+}
 
 template <class T>
 static void launch_assemble_kind(Ctx* ctx, const AssembleArgs& a, int grid) {

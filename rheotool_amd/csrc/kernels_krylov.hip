@@ -29,6 +29,7 @@ struct KryArgs {
     const int32_t* ellWidth;
     const int32_t* ellCol;
     const double* aval;  // FP64 matrix (true residual)
+    const uint32_t* slotMask;  // [nSlices] live column-order slots, NULL: all live
     const double* diag;
     const T* ahat;       // A D^-1 off-diagonal values
     const double* rD;    // 1 / diag
@@ -61,8 +62,9 @@ __global__ void __launch_bounds__(BLK) k_residual(KryArgs<T> a) {
 #pragma unroll
     for (int c = 0; c < 6; ++c) xbar[c] = MODE == 0 ? a.cs[c].xbar : 0.0;
     for (int64_t it = 0;; ++it) {
-        const int64_t t0 = tile_of(it) * BLK;
-        if (t0 >= a.N) break;
+        const int64_t tl = tile_of(it, a.N);
+        if (tl < 0) break;
+        const int64_t t0 = tl * BLK;
         const int32_t r = (int32_t)(t0 + threadIdx.x);
         if (r >= a.N) continue;
         const double d = a.diag[r];
@@ -74,7 +76,9 @@ __global__ void __launch_bounds__(BLK) k_residual(KryArgs<T> a) {
         const int s = r >> 6, lane = r & 63;
         const int64_t base = a.ellOff[s] + lane;
         const int width = a.ellWidth[s];
+        const uint32_t live = a.slotMask ? a.slotMask[s] : 0xffffffffu;
         for (int k = 0; k < width; ++k) {
+            if (k < 32 && !((live >> k) & 1u)) continue;  // zero in every row of the slice
             const int64_t e = base + (int64_t)SLICE * k;
             const int32_t col = a.ellCol[e];
             const double av = a.aval[e];
@@ -183,8 +187,9 @@ __global__ void __launch_bounds__(BLK) k_kry_v(KryArgs<T> a) {
     if (!any) return;
     T q[6] = {0, 0, 0, 0, 0, 0};
     for (int64_t it = 0;; ++it) {
-        const int64_t t0 = tile_of(it) * BLK;
-        if (t0 >= a.N) break;
+        const int64_t tl = tile_of(it, a.N);
+        if (tl < 0) break;
+        const int64_t t0 = tl * BLK;
         const int32_t r = (int32_t)(t0 + threadIdx.x);
         if (r >= a.N) continue;
         T acc[6];
@@ -192,7 +197,9 @@ __global__ void __launch_bounds__(BLK) k_kry_v(KryArgs<T> a) {
         const int s = r >> 6, lane = r & 63;
         const int64_t base = a.ellOff[s] + lane;
         const int width = a.ellWidth[s];
+        const uint32_t live = a.slotMask ? a.slotMask[s] : 0xffffffffu;
         for (int k = 0; k < width; ++k) {
+            if (k < 32 && !((live >> k) & 1u)) continue;  // zero in every row of the slice
             const int64_t e = base + (int64_t)SLICE * k;
             const int32_t col = a.ellCol[e];
             const T av = a.ahat[e];
@@ -249,8 +256,9 @@ __global__ void __launch_bounds__(BLK) k_kry_t(KryArgs<T> a) {
 #pragma unroll
     for (int i = 0; i < 18; ++i) q[i] = 0;
     for (int64_t it = 0;; ++it) {
-        const int64_t t0 = tile_of(it) * BLK;
-        if (t0 >= a.N) break;
+        const int64_t tl = tile_of(it, a.N);
+        if (tl < 0) break;
+        const int64_t t0 = tl * BLK;
         const int32_t r = (int32_t)(t0 + threadIdx.x);
         if (r >= a.N) continue;
         T sv[6], acc[6], rr[6], vv[6];
@@ -264,7 +272,9 @@ __global__ void __launch_bounds__(BLK) k_kry_t(KryArgs<T> a) {
         const int s = r >> 6, lane = r & 63;
         const int64_t base = a.ellOff[s] + lane;
         const int width = a.ellWidth[s];
+        const uint32_t live = a.slotMask ? a.slotMask[s] : 0xffffffffu;
         for (int k = 0; k < width; ++k) {
+            if (k < 32 && !((live >> k) & 1u)) continue;  // zero in every row of the slice
             const int64_t e = base + (int64_t)SLICE * k;
             const int32_t col = a.ellCol[e];
             const T av = a.ahat[e];
@@ -336,8 +346,9 @@ __global__ void __launch_bounds__(BLK) k_kry_xr(KryArgs<T> a) {
 #pragma unroll
     for (int i = 0; i < 12; ++i) q[i] = 0;
     for (int64_t it = 0;; ++it) {
-        const int64_t t0 = tile_of(it) * BLK;
-        if (t0 >= a.N) break;
+        const int64_t tl = tile_of(it, a.N);
+        if (tl < 0) break;
+        const int64_t t0 = tl * BLK;
         const int32_t r = (int32_t)(t0 + threadIdx.x);
         if (r >= a.N) continue;
         const double rd = a.rD[r];
@@ -433,18 +444,29 @@ __global__ void k_rdiag(int32_t N, const double* __restrict__ diag, double* __re
     if (r < N) rD[r] = 1.0 / diag[r];
 }
 template <class T>
-__global__ void k_scale(int64_t nEll, const int32_t* __restrict__ col, const double* __restrict__ aval,
-                        const double* __restrict__ rD, T* __restrict__ ahat) {
-    const int64_t e = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    if (e < nEll) ahat[e] = (T)(aval[e] * rD[col[e]]);
+__global__ void k_scale(int32_t N, const int64_t* __restrict__ off, const int32_t* __restrict__ width,
+                        const uint32_t* __restrict__ slotMask, const int32_t* __restrict__ col,
+                        const double* __restrict__ aval, const double* __restrict__ rD, T* __restrict__ ahat) {
+    const int64_t r = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (r >= N) return;
+    const int s = (int)(r >> 6), lane = (int)(r & 63);
+    const int64_t base = off[s] + lane;
+    const int w = width[s];
+    const uint32_t live = slotMask ? slotMask[s] : 0xffffffffu;
+    for (int k = 0; k < w; ++k) {
+        if (k < 32 && !((live >> k) & 1u)) continue;
+        const int64_t e = base + (int64_t)SLICE * k;
+        ahat[e] = (T)(aval[e] * rD[col[e]]);
+    }
 }
 
 // ---- mean of x per component (normalisation factor's gAverage(psi)) ---------
 __global__ void __launch_bounds__(BLK) k_mean6(int32_t N, const double* __restrict__ x, double* partials) {
     double q[6] = {0, 0, 0, 0, 0, 0};
     for (int64_t it = 0;; ++it) {
-        const int64_t t0 = tile_of(it) * BLK;
-        if (t0 >= N) break;
+        const int64_t tl = tile_of(it, N);
+        if (tl < 0) break;
+        const int64_t t0 = tl * BLK;
         const int64_t r = t0 + threadIdx.x;
         if (r >= N) continue;
         double v[6];
@@ -527,12 +549,15 @@ struct SolveRun {
     double N = 0, ne = 0, nValid = 0;
     int32_t valid[6];
     Ctx* ctx = nullptr;
-    CompState* host = nullptr;  // pinned block of ctx
+    SolveState* hostState = nullptr;  // pinned block of ctx
+    CompState* host = nullptr;        // = hostState->c
+    bool neFromDevice = false;        // matrix entry count comes with the first status pull
+    bool needScale = false;           // Ahat not built yet
     hipEvent_t ev = nullptr;
     bool pending = false, done = false;
     int rounds = 0;
     ~SolveRun() {
-        if (host) ctx->givePinnedBlock(host);
+        if (hostState) ctx->givePinnedBlock(hostState);
         if (ev) (void)hipEventDestroy(ev);
     }
 };
@@ -561,12 +586,7 @@ void fill_args(Ctx* ctx, const SolveArgs& sa, SolveRun& R) {
         hipLaunchKernelGGL(k_rdiag, dim3(kblk(N)), dim3(BLK), 0, ctx->stream, N, sa.diag, rD);
     }
     T* ahat = mixed ? (T*)K.ahat32.p : (T*)K.ahat64.p;
-    if (m.nEll > 0) {
-        ProfScope ps(ctx, mixed ? "k_scale<f32>" : "k_scale<f64>",
-                     (double)m.nEll * (4 + 8 + sizeof(T)) + (double)N * 8);
-        hipLaunchKernelGGL(k_scale<T>, dim3(kblk(m.nEll)), dim3(BLK), 0, ctx->stream, m.nEll,
-                           m.ellCol.as<int32_t>(), sa.aval, rD, ahat);
-    }
+    R.needScale = m.nEll > 0;  // launched with the first round, once the entry count is known
     RT_HIP(hipGetLastError());
     KryArgs<T>& a = argsOf<T>(R);
     a = KryArgs<T>{};
@@ -575,6 +595,7 @@ void fill_args(Ctx* ctx, const SolveArgs& sa, SolveRun& R) {
     a.ellWidth = m.ellWidth.as<int32_t>();
     a.ellCol = m.ellCol.as<int32_t>();
     a.aval = sa.aval;
+    a.slotMask = sa.slotMask;
     a.diag = sa.diag;
     a.ahat = ahat;
     a.rD = rD;
@@ -620,6 +641,12 @@ void launch_round(Ctx* ctx, SolveRun& R, double nAct, double nRef) {
     const bool mixed = std::is_same<T, float>::value;
     Bytes B{R.N, R.ne, (double)sizeof(T)};
     KryArgs<T>& a = argsOf<T>(R);
+    if (R.needScale) {
+        ProfScope ps(ctx, mixed ? "k_scale<f32>" : "k_scale<f64>", R.ne * (4 + 8 + sizeof(T)) + R.N * 8);
+        hipLaunchKernelGGL(k_scale<T>, dim3(kblk((int64_t)R.N)), dim3(BLK), 0, ctx->stream, a.N, a.ellOff,
+                           a.ellWidth, a.slotMask, a.ellCol, a.aval, a.rD, const_cast<T*>(a.ahat));
+        R.needScale = false;
+    }
     if (nRef > 0) {
         {
             ProfScope ps(ctx, mixed ? "k_residual<f32>" : "k_residual<f64>", B.res(nRef));
@@ -651,7 +678,7 @@ void launch_round(Ctx* ctx, SolveRun& R, double nAct, double nRef) {
 
 void queue_pull(Ctx* ctx, Krylov& K, SolveRun& R) {
     SolveState* st = K.state.as<SolveState>();
-    RT_HIP(hipMemcpyAsync(R.host, st->c, sizeof(CompState) * 6, hipMemcpyDeviceToHost, ctx->stream));
+    RT_HIP(hipMemcpyAsync(R.hostState, st, sizeof(SolveState), hipMemcpyDeviceToHost, ctx->stream));
     RT_HIP(hipEventRecord(R.ev, ctx->stream));
     R.pending = true;
 }
@@ -664,10 +691,11 @@ void solve_begin(Ctx* ctx, const SolveArgs& sa) {
     const bool mixed = sa.ctl->mixed;
     K.ensure(m, mixed);
     if (!K.run) {
-        static_assert(sizeof(CompState) * 6 <= Ctx::kPinnedBlock, "status block too small");
+        static_assert(sizeof(SolveState) <= Ctx::kPinnedBlock, "status block too small");
         K.run.reset(new SolveRun);
         K.run->ctx = ctx;
-        K.run->host = static_cast<CompState*>(ctx->takePinnedBlock());
+        K.run->hostState = static_cast<SolveState*>(ctx->takePinnedBlock());
+        K.run->host = K.run->hostState->c;
         RT_HIP(hipEventCreateWithFlags(&K.run->ev, hipEventDisableTiming));
     }
     SolveRun& R = *K.run;
@@ -676,6 +704,7 @@ void solve_begin(Ctx* ctx, const SolveArgs& sa) {
     R.pgrid = std::min<int>(K.grid, (int)kblk(m.N));
     R.N = m.N;
     R.ne = (double)m.nEll;
+    R.neFromDevice = sa.slotMask != nullptr;
     R.nValid = 0;
     for (int c = 0; c < 6; ++c) {
         R.valid[c] = sa.valid[c];
@@ -706,6 +735,10 @@ bool solve_advance(Ctx* ctx, Krylov& K, bool block) {
             RT_HIP(e);
         }
         R.pending = false;
+        if (R.neFromDevice) {
+            R.ne = R.hostState->misc[0];
+            R.neFromDevice = false;
+        }
     }
     double nAct = 0, nRef = 0;
     for (int c = 0; c < 6; ++c) {

@@ -4,6 +4,7 @@
 // handles, the constitutiveEq run-time selection from dictionary text, the
 // correct() sequence and the sparseSolver<symmTensor>::solve surface.
 #include <algorithm>
+#include <cctype>
 #include <cmath>
 #include <cstring>
 #include <set>
@@ -322,6 +323,7 @@ void modelCorrectBegin(Model& M, double deltaT, const double* gradU) {
     if (!M.diag.p) {
         M.diag = ctx->alloc((size_t)N * 8);
         M.aval = ctx->alloc((size_t)std::max<int64_t>(1, m.nEll) * 8);
+        M.slotMask = ctx->alloc((size_t)std::max<int32_t>(1, m.nSlices) * 4);
         M.b = ctx->alloc((size_t)6 * N * 8);
         RT_HIP(hipMemsetAsync(M.b.p, 0, (size_t)6 * N * 8, ctx->stream));
     }
@@ -368,17 +370,27 @@ void modelCorrectBegin(Model& M, double deltaT, const double* gradU) {
     a.prm = M.prm;
     a.invMask = modelInvMask(M);
     a.aval = M.aval.as<double>();
+    a.slotMask = M.slotMask.as<uint32_t>();
     a.diag = M.diag.as<double>();
     a.b = M.b.as<double>();
     a.fOld = M.prm.kind == M_FENEP ? M.fOld.as<double>() : nullptr;
     a.partials = M.kry.partials.as<double>();
     a.cs = st->c;
+    a.nnzOut = &st->misc[0];
     for (int q = 0; q < 6; ++q) a.valid[q] = m.valid[q];
     a.r = mixed ? M.kry.rs.p : M.kry.r.p;
     a.r0 = mixed ? M.kry.r0s.p : M.kry.r0.p;
     a.rD = M.kry.rD.as<double>();
     a.ctl = kryCtl(M.ctl, mixed);
     a.mixed = mixed ? 1 : 0;
+    a.noConv = M.scheme == DS_NONE ? 1 : 0;
+    a.souT = nullptr;
+    if (M.scheme != DS_UPWIND && M.scheme != DS_NONE) {
+        if (!M.gradTh.p) M.gradTh = ctx->alloc((size_t)18 * N * 8);
+        if (!M.souT.p) M.souT = ctx->alloc((size_t)6 * N * 8);
+        launch_defc(ctx, a, M.scheme, m.C4.as<double>(), M.gradTh.as<double>(), M.souT.as<double>(), M.kry.grid);
+        a.souT = M.souT.as<double>();
+    }
     launch_assemble(ctx, a, M.kry.grid);
     if (!M.pendingSolve) M.pendingSolve = new SolveArgs{};
     SolveArgs& sa = *M.pendingSolve;
@@ -387,6 +399,7 @@ void modelCorrectBegin(Model& M, double deltaT, const double* gradU) {
     sa.kry = &M.kry;
     sa.diag = M.diag.as<double>();
     sa.aval = M.aval.as<double>();
+    sa.slotMask = M.slotMask.as<uint32_t>();
     sa.b = M.b.as<double>();
     sa.x = M.theta.as<double>();
     sa.valid = m.valid;
@@ -418,7 +431,8 @@ size_t modelBytesEstimate(int32_t N, int32_t F, int32_t NB, int64_t nEll, bool m
     const double n = N, f = F, nb = NB, ne = (double)nEll;
     double by = 0;
     // model state + flow + matrix: theta tau eigVal eigVec U b diag fOld, Ub, phi, bc, aval
-    by += 8 * n * (6 + 6 + 3 + 9 + 4 + 6 + 1 + 1) + 8 * (3 * nb + f + nb) + 6 * 8 * nb + 8 * ne + 8 * ne;
+    by += 8 * n * (6 + 6 + 3 + 9 + 4 + 6 + 1 + 1) + 8 * (3 * nb + f + nb) + 6 * 8 * nb + 8 * ne + 8 * ne +
+          4 * n / 64;
     // Krylov: 5 vectors x 6 components, rD, ahat, partials, state
     by += 6 * n * 5 * (mixed ? 4 : 8) + 8 * n + ne * (mixed ? 4 : 8) + 2048 * 24 * 8 + 4096;
     // staging for AoS transfers (9 comps)
@@ -766,6 +780,52 @@ int rt_model_set_solver(rt_model* model, const char* controls) {
 int rt_model_set_relax(rt_model* model, double factor) {
     if (!model) return RT_E_BAD_ARG;
     model->impl.relax = factor;
+    return RT_OK;
+}
+
+int rt_model_set_scheme(rt_model* model, const char* divScheme) {
+    if (!model) return RT_E_BAD_ARG;
+    Model& M = model->impl;
+    std::vector<std::string> tok;
+    {
+        std::string cur;
+        for (const char* c = divScheme ? divScheme : ""; *c; ++c) {
+            if (std::isspace((unsigned char)*c) || *c == ';') {
+                if (!cur.empty()) tok.push_back(cur);
+                cur.clear();
+            } else {
+                cur += *c;
+            }
+        }
+        if (!cur.empty()) tok.push_back(cur);
+    }
+    if (tok.size() == 2 && tok[0] == "GaussDefCmpw") tok.erase(tok.begin());
+    if (tok.size() != 1) {
+        M.ctx->lastError = std::string("rt_model_set_scheme: expected \"GaussDefCmpw <limiter>\", got \"") +
+                           (divScheme ? divScheme : "") + "\" (only the GaussDefCmpw convection scheme is supported)";
+        return tok.size() == 2 ? RT_E_UNSUPPORTED : RT_E_BAD_ARG;
+    }
+    static const std::pair<const char*, int> names[] = {{"upwind", DS_UPWIND},   {"cubista", DS_CUBISTA},
+                                                        {"minmod", DS_MINMOD},   {"smart", DS_SMART},
+                                                        {"waceb", DS_WACEB},     {"superbee", DS_SUPERBEE},
+                                                        {"none", DS_NONE}};
+    int scheme = -1;
+    for (const auto& n : names)
+        if (tok[0] == n.first) scheme = n.second;
+    if (scheme < 0) {
+        // limiters.H:88-97
+        M.ctx->lastError = "Error in div(phi, theta" + M.name +
+                           ")\n\nThe deferred limited scheme is not specified or does not exist.\n\n"
+                           "Valid schemes are:\n\n. upwind\n. cubista\n. minmod\n. smart\n. waceb\n"
+                           ". superbee\n. none (no convection)";
+        return RT_E_UNKNOWN_TYPE;
+    }
+    if (scheme != DS_UPWIND && scheme != DS_NONE && !M.mesh->C4.p) {
+        M.ctx->lastError = "rt_model_set_scheme: the " + tok[0] +
+                           " correction needs cell centres; the mesh was created without them";
+        return RT_E_BAD_ARG;
+    }
+    M.scheme = scheme;
     return RT_OK;
 }
 

@@ -7,16 +7,26 @@ namespace rt {
 
 constexpr int BLK = 256;
 
-// Row tiles of BLK rows, interleaved over the workgroups so that at any time
-// the grid works inside one window of gridDim.x tiles: neighbour rows of a
-// tile were touched moments ago by other workgroups and are still in L2/MALL.
-// Inside a window, the tiles of one XCD (workgroups are dealt round-robin to
-// the 8 XCDs) are consecutive, so neighbour reuse mostly stays in one L2.
-__device__ __forceinline__ int64_t tile_of(int64_t i) {
+// Row tiles of BLK rows.  Workgroups are dealt round-robin to the 8 XCDs
+// (XCD = blockIdx.x % 8), each with its own L2.  XCD x owns the contiguous
+// slab of tiles [x*S, (x+1)*S), S = ceil(T/8), and its G/8 workgroups walk
+// the slab together: tile i*(G/8) + blockIdx.x/8 of the slab at step i.  The
+// rows a tile gathers (bandwidth-reduced numbering keeps them within a level
+// of the row) were touched moments before by the same XCD, so they are still
+// in its L2.  Returns -1 once the workgroup's share is done.
+__device__ __forceinline__ int64_t tile_of(int64_t i, int64_t N) {
     const int G = gridDim.x;
     const int b = blockIdx.x;
-    const int j = (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;
-    return i * G + j;
+    const int64_t T = (N + BLK - 1) / BLK;
+    if (G % 8 != 0) {
+        const int64_t t = i * G + b;
+        return t < T ? t : -1;
+    }
+    const int64_t S = (T + 7) / 8;
+    const int64_t local = i * (G / 8) + b / 8;
+    if (local >= S) return -1;
+    const int64_t t = (int64_t)(b % 8) * S + local;
+    return t < T ? t : -1;
 }
 
 __device__ __forceinline__ double wave_sum(double v) {

@@ -160,6 +160,7 @@ struct Mesh {
     DBuf bPatch;    // int32 [NB]    patch of every boundary face
     DBuf bSf;       // double [3][NB] boundary face area vectors
     DBuf V;         // double [N] internal order
+    DBuf C4;        // double [N][4] cell centres (x y z pad); empty when not given
     DBuf permD;     // int32 [N] internal row -> external cell
     double meanAbsBandwidth = 0;
 };
@@ -202,7 +203,7 @@ struct KryCtl {
 struct SolveState {
     CompState c[6];
     int32_t anyActive;
-    double misc[8];
+    double misc[8];  // misc[0]: ELL entries in live slots of the solve's matrix (set by the assembly)
 };
 
 struct SolveRun;  // host side of a resumable solve (kernels_krylov.hip)
@@ -227,6 +228,9 @@ struct Krylov {
 };
 
 // ------------------------------------------------------------------- model
+// GaussDefCmpw limiters (limiters.H:38-100); DS_NONE drops the convection term
+enum DivScheme { DS_UPWIND = 0, DS_CUBISTA = 1, DS_MINMOD = 2, DS_SMART = 3, DS_WACEB = 4, DS_SUPERBEE = 5,
+                 DS_NONE = 6 };
 enum ModelKind { M_OLDROYDB = 0, M_GIESEKUS = 1, M_FENEP = 2, M_PTT_LINEAR = 3, M_PTT_EXP = 4 };
 
 struct ModelParams {
@@ -253,6 +257,10 @@ struct Model {
     DBuf ePhi;              // [nEll] face flux per ELL entry (face order)
     DBuf gradU;             // [9][N] when supplied
     DBuf diag, aval, b, fOld;  // matrix (column order) / sources [N][6] of the step
+    DBuf slotMask;             // uint32 [nSlices] live slots of aval
+    int scheme = 0;            // DivScheme of div(phi,theta)
+    DBuf gradTh;               // double [N][18] component gradients (limited schemes)
+    DBuf souT;                 // double [N][6] explicit correction source
     DBuf bcType;            // int32 [nPatches]
     DBuf bcVal;             // double [6][NB]
     std::vector<int32_t> hBcType;
@@ -293,12 +301,16 @@ struct AssembleArgs {
     double relax;         // <= 0: none
     ModelParams prm;
     int32_t invMask;      // [OF9] inv() first-cell shift bits (ofInvQuirk)
-    double* aval;         // column order
+    double* aval;         // the solve's matrix, column order
+    uint32_t* slotMask;   // [nSlices] live column-order slots of aval
     double* diag;
     double* b;            // [N][6]
     double* fOld;         // FENE-P f, nullable
-    double* partials;     // [grid][18] residual sums
+    const double* souT;   // [N][6] deferred-correction source, NULL for none
+    int32_t noConv;       // the "none" scheme: no convection term
+    double* partials;     // [grid][19] residual sums, entries in live slots
     CompState* cs;
+    double* nnzOut;       // SolveState::misc[0]
     int32_t valid[6];
     // initial residual of the solve, fused into the assembly
     void* r;              // [N][6] Krylov precision
@@ -309,6 +321,10 @@ struct AssembleArgs {
 };
 
 void launch_assemble(Ctx* ctx, const AssembleArgs& a, int grid);
+// Gauss-linear gradients of the 6 theta components ([N][18]) and the
+// deferred correction source souT ([N][6]) of a limited GaussDefCmpw scheme
+void launch_defc(Ctx* ctx, const AssembleArgs& a, int scheme, const double* C4, double* gradTh, double* souT,
+                 int grid);
 void launch_eig_tau(Ctx* ctx, int32_t N, const ModelParams& prm, const double* theta,
                     double* eigVal, double* eigVec, double* tau, const double* fOld);
 void launch_aos_to_soa(Ctx* ctx, int32_t N, int ncomp, const double* aos, const int32_t* perm,
@@ -340,6 +356,7 @@ struct SolveArgs {
     Krylov* kry;
     const double* diag;  // [N]
     const double* aval;  // ELL values
+    const uint32_t* slotMask;  // [nSlices] live slots of aval; NULL: all live
     const double* b;     // [N][6]
     double* x;           // [N][6] (theta in place)
     const int32_t* valid;

@@ -276,6 +276,12 @@ void buildMesh(Ctx* ctx, const rt_mesh_desc* d, int renumber, Mesh& m) {
     upload(ctx, m.bPatch, bpatch);
     upload(ctx, m.bSf, bsf);
     upload(ctx, m.V, V);
+    if (d->C) {
+        std::vector<double> c4(4 * (size_t)N, 0.0);
+        for (int32_t r = 0; r < N; ++r)
+            for (int q = 0; q < 3; ++q) c4[4 * (size_t)r + q] = d->C[3 * (size_t)m.iperm[r] + q];
+        upload(ctx, m.C4, c4);
+    }
     upload(ctx, m.permD, m.iperm);
     RT_HIP(hipStreamSynchronize(ctx->stream));
 }

@@ -25,7 +25,7 @@ struct Request {
     int32_t nCells = 0;
     std::string dict, controls;
     const double *theta = nullptr, *eigVals = nullptr, *eigVecs = nullptr, *U = nullptr, *Ub = nullptr,
-                 *phi = nullptr;
+                 *phi = nullptr, *gradU = nullptr;
     double deltaT = 0;
     double *thetaOut = nullptr, *tauOut = nullptr;
     size_t estBytes = 0;
@@ -70,7 +70,7 @@ void releaseRequest(rt_server* S, Request* r) {
         r->slot = -1;
     }
     // the caller's input arrays are no longer read
-    r->theta = r->eigVals = r->eigVecs = r->U = r->Ub = r->phi = nullptr;
+    r->theta = r->eigVals = r->eigVecs = r->U = r->Ub = r->phi = r->gradU = nullptr;
 }
 
 void fail(rt_server* S, Request* r, int rc, const std::string& msg) {
@@ -97,7 +97,7 @@ void admit(rt_server* S, Request* r, int slot) {
     }
     try {
         RT_HIP(hipSetDevice(c->impl.device));
-        modelCorrectBegin(r->model->impl, r->deltaT, nullptr);
+        modelCorrectBegin(r->model->impl, r->deltaT, r->gradU);
         RT_HIP(hipEventCreateWithFlags(&r->done, hipEventDisableTiming));
         r->state = Request::RUNNING;
     } catch (const Error& e) {
@@ -217,6 +217,7 @@ int rt_server_submit(rt_server* S, const rt_request* q, int64_t* ticket) {
     r->deltaT = q->deltaT;
     r->thetaOut = q->thetaOut;
     r->tauOut = q->tauOut;
+    r->gradU = q->gradU;
     const bool mixed = r->controls.find("mixed") != std::string::npos;
     r->estBytes = modelBytesEstimate((int32_t)N, (int32_t)F, (int32_t)NB, mm.nEll, mixed);
     if (S->budget && r->estBytes > S->budget) {

@@ -183,7 +183,8 @@ const rt_mesh* fvMesh::device() const {
         std::vector<rt_patch> p;
         for (auto& b : boundary) p.push_back(rt_patch{b.start, b.size, b.kind});
         rt_mesh_desc d{nCells, nInternalFaces, nBoundaryFaces, (int32_t)p.size(), owner.data(),
-                       neighbour.data(), Sf.data(), weights.data(), V.data(), p.data()};
+                       neighbour.data(), Sf.data(), weights.data(), V.data(), p.data(),
+                       C.empty() ? nullptr : C.data()};
         gpuContext& c = gpuContext::get();
         c.check(rt_mesh_create(c.ptr(), &d, RT_RENUMBER_AUTO, &dev_), "fvMesh");
     }
@@ -278,6 +279,10 @@ void logConformationEq::correct(const volScalarField*, const volTensorField* gra
     const dictionary rf = mesh.fvSolution.subDict("relaxationFactors");
     if (!lookupScalar(rf.subDict("equations").text(), "theta" + name(), f)) lookupScalar(rf.text(), "theta" + name(), f);
     c.check(rt_model_set_relax(model_, f), "relax");
+    // fvSchemes divSchemes { div(phi,theta) GaussDefCmpw <limiter>; }
+    std::string ds;
+    if (topEntry(mesh.fvSchemes.subDict("divSchemes").text(), "div(phi,theta" + name() + ")", ds))
+        c.check(rt_model_set_scheme(model_, ds.c_str()), "fvSchemes div(phi,theta)");
     c.check(rt_model_correct(model_, mesh.time.deltaTValue(), gradU ? gradU->internal.data() : nullptr),
             "constitutiveEq::correct");
     rt_solver_perf p[6];

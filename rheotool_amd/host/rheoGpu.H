@@ -86,8 +86,11 @@ public:
     std::vector<label> owner, neighbour;
     std::vector<scalar> Sf, weights, V;
     std::vector<patchInfo> boundary;
+    std::vector<scalar> C;  // cell centres [3*nCells]; set before first use when the
+                            // div(phi,theta) scheme is a limited GaussDefCmpw one
     Time time;
     dictionary fvSolution;
+    dictionary fvSchemes;
 
     fvMesh(label nCells, label nInternalFaces, label nBoundaryFaces, std::vector<label> owner,
            std::vector<label> neighbour, std::vector<scalar> Sf, std::vector<scalar> weights,

@@ -147,3 +147,48 @@ def test_objects_outlive_their_context():
     srv.close()
     eq.close()
     dm.close()
+
+
+def test_random_graph_requests_with_prescribed_gradu():
+    """Random-face-graph requests with an analytic grad U (the gradU argument
+    of correct()) mixed with channel requests: each result equals the model
+    run alone and the graph ones agree with the oracle to 1e-9.  With
+    eigenvalues of C up to 1e6 (|theta| up to 14), tau = exp(theta)-based
+    amplifies a theta difference by |theta|, so both sides solve to 1e-12
+    here to keep the solver tolerance itself out of the comparison."""
+    controls = abi.solver_dict(1e-12, precision="mixed")
+    with abi.Context(0) as ctx:
+        graphs = []
+        for i in range(3):
+            m = cases.random_graph(4000 + 1500 * i, kNear=5, sigma=0.5, seed=20 + i)
+            f = cases.make_fields(m, cases.FIELDS_TG, seed=30 + i, lnLo=-9.0, lnHi=float(np.log(1e6)))
+            graphs.append((m, f))
+        d = abi.model_dict("GiesekusLog", lam=50.0, alpha=0.01)
+        alone = []
+        for m, f in graphs:
+            dm = ctx.mesh(m)
+            eq = ctx.constitutiveEq("", dm, d)
+            eq.set_solver(controls)
+            eq.set_state(f.theta, f.eigVals, f.eigVecs)
+            eq.set_flow(f.U, f.Ub, f.phi)
+            eq.correct(1e-3, f.gradU)
+            alone.append((eq.theta(), eq.tau()))
+        srv = abi.Server(ctx, max_inflight=3)
+        meshes = [ctx.mesh(m) for m, _ in graphs]
+        others = [make_request(i) for i in range(2)]
+        ts = []
+        for k, ((m, f), dm) in enumerate(zip(graphs, meshes)):
+            ts.append(srv.submit(dm, f, d, 1e-3, controls, gradU=f.gradU))
+            if k < len(others):
+                r = others[k]
+                srv.submit(ctx.mesh(r[0]), r[1], abi.model_dict(r[2], lam=r[3], **r[4]), r[5], controls)
+        while srv.progress() > 0:
+            pass
+        p = O.params("GiesekusLog", etaS=0.59, etaP=0.41, lam=50.0, alpha=0.01)
+        for t, (m, f), a in zip(ts, graphs, alone):
+            out = srv.poll(t)
+            assert np.array_equal(out["theta"], a[0]) and np.array_equal(out["tau"], a[1])
+            th, _, _, tau, _ = O.correct(m, p, f, 1e-3, gradU=f.gradU, ctl=O.controls(tol=1e-12))
+            assert np.linalg.norm(out["theta"] - th) / np.linalg.norm(th) < 1e-9
+            assert np.linalg.norm(out["tau"] - tau) / np.linalg.norm(tau) < 1e-9
+        srv.close()
