@@ -212,6 +212,19 @@ int rt_model_set_relax(rt_model* model, double factor);
  * with the reference's list (limiters.H:88-97). */
 int rt_model_set_scheme(rt_model* model, const char* divScheme);
 
+/* ------------------------------------------- consumers of tau (momentum eq.)
+ * The explicit stress term constitutiveEq::divTau hands to the momentum
+ * equation, fvc::div(tau()/rho(), "div(tau)") with Gauss linear
+ * (constitutiveEquations/constitutiveEqs/constitutiveEq/constitutiveEq.C:
+ * 72-132; the fvm::laplacian(etaS/rho, U) part is the momentum solver's own
+ * matrix).  Reads the model's current tau on the device; out [3N].
+ * tauB [6*nBoundaryFaces]: tau's patch values, NULL for zeroGradient. */
+int rt_model_div_tau(rt_model* model, const double* tauB, double* out);
+/* constitutiveEq::tauTotal() (constitutiveEq.C:534-547):
+ * tau + etaS (symm(L + L^T) - 2/3 tr(L) I), L = fvc::grad(U) Gauss linear
+ * from the flow last set.  out [6N]. */
+int rt_model_tau_total(rt_model* model, double* out);
+
 /* constitutiveEq::correct(alpha, gradU)  (constitutiveEq.H:346-350,
  * Oldroyd_BLog.C:127-179): decompose grad(U)^T, assemble theta's
  * ddt + upwind div == source, solve the valid components, eigen-decompose

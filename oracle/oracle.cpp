@@ -783,6 +783,62 @@ void orc_defc_sout(const orc_mesh* m, const double* phi, const double* dc, doubl
         }
 }
 
+void orc_div_tau(const orc_mesh* m, const double* tau, const double* tauB, double rho, double* out) {
+    const int N = m->nCells, F = m->nInternal;
+    std::vector<double> t(6 * (size_t)N);
+    for (size_t k = 0; k < 6 * (size_t)N; ++k) t[k] = tau[k] / rho;  // tau()/rho()
+    std::fill(out, out + 3 * (size_t)N, 0.0);
+    // Sf & T for a symmTensor T (xx xy xz yy yz zz)
+    auto dot = [](const double* sf, const double* T, double* v) {
+        v[0] = sf[0] * T[0] + sf[1] * T[1] + sf[2] * T[2];
+        v[1] = sf[0] * T[1] + sf[1] * T[3] + sf[2] * T[4];
+        v[2] = sf[0] * T[2] + sf[1] * T[4] + sf[2] * T[5];
+    };
+    for (int f = 0; f < F; ++f) {  // linear interpolate, surfaceIntegrate
+        const int o = m->owner[f], n = m->neighbour[f];
+        double tf[6], v[3];
+        for (int q = 0; q < 6; ++q) tf[q] = m->w[f] * (t[6 * (size_t)o + q] - t[6 * (size_t)n + q]) + t[6 * (size_t)n + q];
+        dot(m->Sf + 3 * (size_t)f, tf, v);
+        for (int j = 0; j < 3; ++j) {
+            out[3 * (size_t)o + j] += v[j];
+            out[3 * (size_t)n + j] -= v[j];
+        }
+    }
+    for (int p = 0; p < m->nPatches; ++p) {
+        if (m->patchEmpty[p]) continue;
+        for (int k = 0; k < m->patchSize[p]; ++k) {
+            const int b = m->patchStart[p] + k;
+            const int64_t f = (int64_t)F + b;
+            const int c = m->owner[f];
+            double tb[6], v[3];
+            for (int q = 0; q < 6; ++q) tb[q] = tauB ? tauB[6 * (size_t)b + q] / rho : t[6 * (size_t)c + q];
+            dot(m->Sf + 3 * f, tb, v);
+            for (int j = 0; j < 3; ++j) out[3 * (size_t)c + j] += v[j];
+        }
+    }
+    for (int c = 0; c < N; ++c)
+        for (int j = 0; j < 3; ++j) out[3 * (size_t)c + j] /= m->V[c];
+}
+
+void orc_tau_total(const orc_mesh* m, const double* U, const double* Ub, const double* tau, double etaS,
+                   double* out) {
+    const int N = m->nCells;
+    std::vector<double> L(9 * (size_t)N);
+    orc_grad_gauss(m, U, Ub, L.data());
+    for (int c = 0; c < N; ++c) {
+        const double* l = L.data() + 9 * (size_t)c;
+        double T[9];  // L + L.T()
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) T[3 * i + j] = l[3 * i + j] + l[3 * j + i];
+        // symm(T)
+        const double s[6] = {T[0], 0.5 * (T[1] + T[3]), 0.5 * (T[2] + T[6]), T[4], 0.5 * (T[5] + T[7]), T[8]};
+        const double tr = l[0] + l[4] + l[8];
+        const double d = (2. / 3.) * tr;
+        const double I[6] = {d, 0, 0, d, 0, d};  // (2/3) tr(L) * I
+        for (int q = 0; q < 6; ++q) out[6 * (size_t)c + q] = tau[6 * (size_t)c + q] + etaS * (s[q] - I[q]);
+    }
+}
+
 void orc_relax(const orc_mesh* m, const int32_t* bcType, double alpha, const double* psi,
                double* D, const double* lower, const double* upper, double* S,
                const double* iC) {

@@ -108,6 +108,19 @@ void orc_defc_faces(const orc_mesh* m, int32_t scheme, const double* phi, const 
  * source is -souT.  No coupled patches (they are out of scope). */
 void orc_defc_sout(const orc_mesh* m, const double* phi, const double* dc, double* souT);
 
+/* fvc::div(tau/rho), Gauss linear: the explicit stress term that
+ * constitutiveEq::divTau hands to the momentum equation
+ * (constitutiveEq.C:88-92 "fvc::div(tau()/rho(), "div(tau)")").  tauB
+ * [6*nBoundary] patch values of tau (NULL: zeroGradient, the cell value).
+ * out [3N]. */
+void orc_div_tau(const orc_mesh* m, const double* tau, const double* tauB, double rho, double* out);
+
+/* constitutiveEq::tauTotal() (constitutiveEq.C:534-547):
+ * tau + etaS*(symm(L + L^T) - (2/3) tr(L) I), L = fvc::grad(U) Gauss linear.
+ * out [6N]. */
+void orc_tau_total(const orc_mesh* m, const double* U, const double* Ub, const double* tau, double etaS,
+                   double* out);
+
 /* [OF9] fvMatrix::relax(1) on the assembled system (Oldroyd_BLog.C:166). */
 void orc_relax(const orc_mesh* m, const int32_t* bcType, double alpha, const double* psi,
                double* diag, const double* lower, const double* upper, double* source,

@@ -79,6 +79,8 @@ def lib():
         L.orc_grad_gauss_symm.argtypes = [vp] * 4
         L.orc_defc_faces.argtypes = [vp, C.c_int32] + [vp] * 5
         L.orc_defc_sout.argtypes = [vp] * 4
+        L.orc_div_tau.argtypes = [vp, vp, vp, C.c_double, vp]
+        L.orc_tau_total.argtypes = [vp, vp, vp, vp, C.c_double, vp]
         _LIB = L
     return _LIB
 
@@ -154,6 +156,24 @@ def defc_sout(m, phi, dc):
     s = np.empty((m.nCells, 6))
     lib().orc_defc_sout(om.ref, _p(np.ascontiguousarray(phi)), _p(np.ascontiguousarray(dc)), _p(s))
     return s
+
+
+def div_tau(m, tau, tauB=None, rho=1.0):
+    """fvc::div(tau/rho), Gauss linear: [N, 3]."""
+    om = OracleMesh(m)
+    out = np.empty((m.nCells, 3))
+    tb = None if tauB is None else np.ascontiguousarray(tauB)
+    lib().orc_div_tau(om.ref, _p(np.ascontiguousarray(tau)), _p(tb), rho, _p(out))
+    return out
+
+
+def tau_total(m, U, Ub, tau, etaS):
+    om = OracleMesh(m)
+    out = np.empty((m.nCells, 6))
+    Ub = np.ascontiguousarray(Ub) if m.nBoundary else np.zeros((1, 3))
+    lib().orc_tau_total(om.ref, _p(np.ascontiguousarray(U)), _p(Ub), _p(np.ascontiguousarray(tau)), etaS,
+                        _p(out))
+    return out
 
 
 def rhs(p, gradU, theta, eigVals, eigVecs):

@@ -109,6 +109,8 @@ def lib():
             "rt_model_set_solver": (i32, [vp, C.c_char_p]),
             "rt_model_set_relax": (i32, [vp, C.c_double]),
             "rt_model_set_scheme": (i32, [vp, C.c_char_p]),
+            "rt_model_div_tau": (i32, [vp, vp, vp]),
+            "rt_model_tau_total": (i32, [vp, vp]),
             "rt_model_correct": (i32, [vp, C.c_double, vp]),
             "rt_model_get": (i32, [vp, i32, vp]),
             "rt_model_solver_perf": (i32, [vp, vp]),
@@ -286,6 +288,18 @@ class ConstitutiveEq:
     def set_scheme(self, div_scheme: str):
         """fvSchemes div(phi,theta): "GaussDefCmpw cubista", "upwind", ..."""
         self.ctx.check(lib().rt_model_set_scheme(self.h, div_scheme.encode()), "set_scheme")
+
+    def div_tau(self, tauB=None):
+        """fvc::div(tau/rho) of the current tau: [N, 3]."""
+        out = np.empty((self.N, 3))
+        tb = None if tauB is None else _f64(tauB)
+        self.ctx.check(lib().rt_model_div_tau(self.h, _p(tb), _p(out)), "div_tau")
+        return out
+
+    def tau_total(self):
+        out = np.empty((self.N, 6))
+        self.ctx.check(lib().rt_model_tau_total(self.h, _p(out)), "tau_total")
+        return out
 
     def correct(self, deltaT, gradU=None):
         g = None if gradU is None else _f64(gradU, (self.N, 3, 3))

@@ -763,6 +763,146 @@ void launch_defc(Ctx* ctx, const AssembleArgs& a, int scheme, const double* C4, 
     RT_HIP(hipGetLastError());
 }
 
+// ------------------------------------------- stress terms of the momentum eq.
+// fvc::div(tau/rho), Gauss linear (constitutiveEq.C:88-92): per row, faces in
+// increasing face id then the row's boundary faces (patch value tauB [6][NB],
+// or the cell value when tauB is NULL: zeroGradient), then / V.
+__global__ void __launch_bounds__(BLK) k_div_tau(AssembleArgs a, const double* __restrict__ tau,
+                                                 const double* __restrict__ tauB, double rho,
+                                                 double* __restrict__ out) {
+    const int64_t NB = a.NB > 0 ? a.NB : 1;
+    for (int64_t it = 0;; ++it) {
+        const int64_t tl = tile_of(it, a.N);
+        if (tl < 0) break;
+        const int32_t r = (int32_t)(tl * BLK + threadIdx.x);
+        if (r >= a.N) continue;
+        double t[6];
+        ld6(tau + (int64_t)NCMP * r, t);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) t[q] = t[q] / rho;
+        double acc[3] = {0, 0, 0};
+        const int s = r >> 6, lane = r & 63;
+        const int64_t base = a.ellOff[s] + lane;
+        const int width = a.ellWidth[s];
+        for (int k = 0; k < width; ++k) {
+            const int64_t e = base + (int64_t)SLICE * k;
+            const int32_t fid = a.ellFid[e];
+            if (fid == RT_FID_PAD) continue;
+            const bool own = fid >= 0;
+            const int32_t c = a.ellCol[base + (int64_t)SLICE * a.ellSlot[e]];
+            double tc[6];
+            ld6(tau + (int64_t)NCMP * c, tc);
+            const double wf = a.eW[e];
+            double tf[6];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                const double tcq = tc[q] / rho;
+                const double vo = own ? t[q] : tcq, vn = own ? tcq : t[q];
+                tf[q] = wf * (vo - vn) + vn;
+            }
+            // (+-Sf) & tf, the sign carried by the entry
+            const double sf[3] = {a.eSf[e], a.eSf[a.nEll + e], a.eSf[2 * a.nEll + e]};
+            acc[0] += sf[0] * tf[0] + sf[1] * tf[1] + sf[2] * tf[2];
+            acc[1] += sf[0] * tf[1] + sf[1] * tf[3] + sf[2] * tf[4];
+            acc[2] += sf[0] * tf[2] + sf[1] * tf[4] + sf[2] * tf[5];
+        }
+        const int32_t b0 = a.bPtr[r], b1 = a.bPtr[r + 1];
+        for (int32_t qb = b0; qb < b1; ++qb) {
+            const int32_t bf = a.bFace[qb];
+            double tb[6];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) tb[q] = tauB ? tauB[(int64_t)q * NB + bf] / rho : t[q];
+            const double sf[3] = {a.bSf[bf], a.bSf[NB + bf], a.bSf[2 * NB + bf]};
+            acc[0] += sf[0] * tb[0] + sf[1] * tb[1] + sf[2] * tb[2];
+            acc[1] += sf[0] * tb[1] + sf[1] * tb[3] + sf[2] * tb[4];
+            acc[2] += sf[0] * tb[2] + sf[1] * tb[4] + sf[2] * tb[5];
+        }
+        const double vol = a.V[r];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) out[3 * (int64_t)r + j] = acc[j] / vol;
+    }
+}
+
+// tauTotal() = tau + etaS (symm(L + L^T) - 2/3 tr(L) I), L = fvc::grad(U) Gauss
+// linear, the same gather as the assembly's (constitutiveEq.C:534-547).
+__global__ void __launch_bounds__(BLK) k_tau_total(AssembleArgs a, const double* __restrict__ tau, double etaS,
+                                                   double* __restrict__ out) {
+    const int64_t NB = a.NB > 0 ? a.NB : 1;
+    const int64_t nE = a.nEll;
+    for (int64_t it = 0;; ++it) {
+        const int64_t tl = tile_of(it, a.N);
+        if (tl < 0) break;
+        const int32_t r = (int32_t)(tl * BLK + threadIdx.x);
+        if (r >= a.N) continue;
+        const double2 u01 = reinterpret_cast<const double2*>(a.U)[2 * (int64_t)r];
+        const double Ui[3] = {u01.x, u01.y, a.U[4 * (int64_t)r + 2]};
+        double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        const int s = r >> 6, lane = r & 63;
+        const int64_t base = a.ellOff[s] + lane;
+        const int width = a.ellWidth[s];
+        for (int k = 0; k < width; ++k) {
+            const int64_t e = base + (int64_t)SLICE * k;
+            const int32_t fid = a.ellFid[e];
+            if (fid == RT_FID_PAD) continue;
+            const bool own = fid >= 0;
+            const int32_t c = a.ellCol[base + (int64_t)SLICE * a.ellSlot[e]];
+            const double wf = a.eW[e];
+            const double2 c01 = reinterpret_cast<const double2*>(a.U)[2 * (int64_t)c];
+            const double Uc[3] = {c01.x, c01.y, a.U[4 * (int64_t)c + 2]};
+            double uf[3];
+#pragma unroll
+            for (int q3 = 0; q3 < 3; ++q3) {
+                const double uo = own ? Ui[q3] : Uc[q3], un = own ? Uc[q3] : Ui[q3];
+                uf[q3] = wf * (uo - un) + un;
+            }
+            const double sf[3] = {a.eSf[e], a.eSf[nE + e], a.eSf[2 * nE + e]};
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) g[3 * i + j] += sf[i] * uf[j];
+        }
+        const int32_t b0 = a.bPtr[r], b1 = a.bPtr[r + 1];
+        for (int32_t qb = b0; qb < b1; ++qb) {
+            const int32_t bf = a.bFace[qb];
+            const double sf[3] = {a.bSf[bf], a.bSf[NB + bf], a.bSf[2 * NB + bf]};
+            const double ub[3] = {a.Ub[bf], a.Ub[NB + bf], a.Ub[2 * NB + bf]};
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) g[3 * i + j] += sf[i] * ub[j];
+        }
+        const double vol = a.V[r];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) g[q] /= vol;
+        double T[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) T[3 * i + j] = g[3 * i + j] + g[3 * j + i];
+        const double sy[6] = {T[0], 0.5 * (T[1] + T[3]), 0.5 * (T[2] + T[6]), T[4], 0.5 * (T[5] + T[7]), T[8]};
+        const double d = (2. / 3.) * (g[0] + g[4] + g[8]);
+        const double I[6] = {d, 0.0, 0.0, d, 0.0, d};
+        double t[6];
+        ld6(tau + (int64_t)NCMP * r, t);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) t[q] = t[q] + etaS * (sy[q] - I[q]);
+        st6(out + (int64_t)NCMP * r, t);
+    }
+}
+
+void launch_div_tau(Ctx* ctx, const AssembleArgs& a, const double* tau, const double* tauB, double rho,
+                    double* out, int grid) {
+    ProfScope ps(ctx, "k_div_tau", (double)a.N * (48 + 8 + 24) + (double)a.nEll * (4 + 1 + 4 + 32));
+    hipLaunchKernelGGL(k_div_tau, dim3(grid), dim3(BLK), 0, ctx->stream, a, tau, tauB, rho, out);
+    RT_HIP(hipGetLastError());
+}
+
+void launch_tau_total(Ctx* ctx, const AssembleArgs& a, const double* tau, double etaS, double* out, int grid) {
+    ProfScope ps(ctx, "k_tau_total", (double)a.N * (32 + 8 + 48 + 48) + (double)a.nEll * (4 + 1 + 4 + 32));
+    hipLaunchKernelGGL(k_tau_total, dim3(grid), dim3(BLK), 0, ctx->stream, a, tau, etaS, out);
+    RT_HIP(hipGetLastError());
+}
+
 template <class T>
 static void launch_assemble_kind(Ctx* ctx, const AssembleArgs& a, int grid) {
     switch (a.prm.kind) {

@@ -312,6 +312,34 @@ int modelInvMask(Model& M) {
     return mask;
 }
 
+// The mesh and flow part of the per-row kernels' arguments.
+AssembleArgs meshArgs(Model& M) {
+    Mesh& m = *M.mesh;
+    AssembleArgs a{};
+    a.N = m.N;
+    a.ellOff = m.ellOff.as<int64_t>();
+    a.ellWidth = m.ellWidth.as<int32_t>();
+    a.ellCol = m.ellCol.as<int32_t>();
+    a.ellFid = m.ellFid.as<int32_t>();
+    a.ellSlot = m.ellSlot.as<uint8_t>();
+    a.eSf = m.eSf.as<double>();
+    a.eW = m.eW.as<double>();
+    a.ePhi = M.ePhi.as<double>();
+    a.nEll = std::max<int64_t>(1, m.nEll);
+    a.bPtr = m.bPtr.as<int32_t>();
+    a.bFace = m.bFace.as<int32_t>();
+    a.bPatch = m.bPatch.as<int32_t>();
+    a.bcType = M.bcType.as<int32_t>();
+    a.bcVal = M.bcVal.as<double>();
+    a.NB = m.NB;
+    a.bSf = m.bSf.as<double>();
+    a.phiB = M.phi.as<double>() + m.F;
+    a.V = m.V.as<double>();
+    a.U = M.U.as<double>();
+    a.Ub = M.Ub.as<double>();
+    return a;
+}
+
 void modelCorrectBegin(Model& M, double deltaT, const double* gradU) {
     if (!M.hasState) throw Error{RT_E_BAD_ARG, "correct(): theta was never set (rt_model_set_state)"};
     if (!M.hasFlow) throw Error{RT_E_BAD_ARG, "correct(): U/phi were never set (rt_model_set_flow)"};
@@ -339,28 +367,7 @@ void modelCorrectBegin(Model& M, double deltaT, const double* gradU) {
     // gAverage(theta) for the normalisation factor of the initial residual
     launch_mean6(ctx, N, M.theta.as<double>(), M.kry.partials.as<double>(), st->c,
                  M.kry.grid);
-    AssembleArgs a{};
-    a.N = N;
-    a.ellOff = m.ellOff.as<int64_t>();
-    a.ellWidth = m.ellWidth.as<int32_t>();
-    a.ellCol = m.ellCol.as<int32_t>();
-    a.ellFid = m.ellFid.as<int32_t>();
-    a.ellSlot = m.ellSlot.as<uint8_t>();
-    a.eSf = m.eSf.as<double>();
-    a.eW = m.eW.as<double>();
-    a.ePhi = M.ePhi.as<double>();
-    a.nEll = std::max<int64_t>(1, m.nEll);
-    a.bPtr = m.bPtr.as<int32_t>();
-    a.bFace = m.bFace.as<int32_t>();
-    a.bPatch = m.bPatch.as<int32_t>();
-    a.bcType = M.bcType.as<int32_t>();
-    a.bcVal = M.bcVal.as<double>();
-    a.NB = m.NB;
-    a.bSf = m.bSf.as<double>();
-    a.phiB = M.phi.as<double>() + m.F;
-    a.V = m.V.as<double>();
-    a.U = M.U.as<double>();
-    a.Ub = M.Ub.as<double>();
+    AssembleArgs a = meshArgs(M);
     a.gradU = gradU ? M.gradU.as<double>() : nullptr;
     a.theta = M.theta.as<double>();
     a.eigVal = M.eigVal.as<double>();
@@ -861,6 +868,51 @@ int rt_model_get(const rt_model* model, int field, double* out) {
         else if (field == RT_EIGVALS) launch_eigvals_to_tensor(c, N, M.eigVal.as<double>(), perm, st.as<double>());
         else launch_soa_to_aos(c, N, 9, M.eigVec.as<double>(), perm, st.as<double>());
         RT_HIP(hipMemcpyAsync(out, st.p, bytes, hipMemcpyDeviceToHost, c->stream));
+        RT_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int rt_model_div_tau(rt_model* model, const double* tauB, double* out) {
+    if (!model || !out) return RT_E_BAD_ARG;
+    Model& M = model->impl;
+    return guarded(M.ctx, [&] {
+        Ctx* c = M.ctx;
+        Mesh& m = *M.mesh;
+        const int32_t N = m.N;
+        RT_HIP(hipSetDevice(c->device));
+        DBuf tb;
+        if (tauB && m.NB > 0) {
+            std::vector<double> soa((size_t)6 * m.NB);
+            for (int32_t b = 0; b < m.NB; ++b)
+                for (int q = 0; q < 6; ++q) soa[(size_t)q * m.NB + b] = tauB[6 * (size_t)b + q];
+            tb = c->alloc(soa.size() * 8);
+            RT_HIP(hipMemcpyAsync(tb.p, soa.data(), soa.size() * 8, hipMemcpyHostToDevice, c->stream));
+        }
+        DBuf d = c->alloc((size_t)3 * N * 8);
+        launch_div_tau(c, meshArgs(M), M.tau.as<double>(), tb.p ? tb.as<double>() : nullptr, M.prm.rho,
+                       d.as<double>(), grid_for(c, N));
+        DBuf& st = staging(c).buf;
+        if (st.bytes < (size_t)3 * N * 8) st = c->alloc((size_t)3 * N * 8);
+        launch_unpermute_rows(c, N, 3, 3, d.as<double>(), m.permD.as<int32_t>(), st.as<double>());
+        RT_HIP(hipMemcpyAsync(out, st.p, (size_t)3 * N * 8, hipMemcpyDeviceToHost, c->stream));
+        RT_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int rt_model_tau_total(rt_model* model, double* out) {
+    if (!model || !out) return RT_E_BAD_ARG;
+    Model& M = model->impl;
+    return guarded(M.ctx, [&] {
+        Ctx* c = M.ctx;
+        Mesh& m = *M.mesh;
+        const int32_t N = m.N;
+        RT_HIP(hipSetDevice(c->device));
+        DBuf d = c->alloc((size_t)6 * N * 8);
+        launch_tau_total(c, meshArgs(M), M.tau.as<double>(), M.prm.etaS, d.as<double>(), grid_for(c, N));
+        DBuf& st = staging(c).buf;
+        if (st.bytes < (size_t)6 * N * 8) st = c->alloc((size_t)6 * N * 8);
+        launch_unpermute_rows(c, N, 6, 6, d.as<double>(), m.permD.as<int32_t>(), st.as<double>());
+        RT_HIP(hipMemcpyAsync(out, st.p, (size_t)6 * N * 8, hipMemcpyDeviceToHost, c->stream));
         RT_HIP(hipStreamSynchronize(c->stream));
     });
 }

@@ -325,6 +325,11 @@ void launch_assemble(Ctx* ctx, const AssembleArgs& a, int grid);
 // deferred correction source souT ([N][6]) of a limited GaussDefCmpw scheme
 void launch_defc(Ctx* ctx, const AssembleArgs& a, int scheme, const double* C4, double* gradTh, double* souT,
                  int grid);
+// fvc::div(tau/rho) -> out [N][3]; tauB [6][NB] patch values or NULL (zeroGradient)
+void launch_div_tau(Ctx* ctx, const AssembleArgs& a, const double* tau, const double* tauB, double rho,
+                    double* out, int grid);
+// tauTotal() -> out [N][6]
+void launch_tau_total(Ctx* ctx, const AssembleArgs& a, const double* tau, double etaS, double* out, int grid);
 void launch_eig_tau(Ctx* ctx, int32_t N, const ModelParams& prm, const double* theta,
                     double* eigVal, double* eigVec, double* tau, const double* fOld);
 void launch_aos_to_soa(Ctx* ctx, int32_t N, int ncomp, const double* aos, const int32_t* perm,

@@ -309,6 +309,18 @@ volSymmTensorField logConformationEq::tau() const { return getSymm(RT_TAU, "tau"
 volSymmTensorField logConformationEq::theta() const { return getSymm(RT_THETA, "theta" + name()); }
 volTensorField logConformationEq::eigVals() const { return getTensor(RT_EIGVALS, "eigVals" + name()); }
 volTensorField logConformationEq::eigVecs() const { return getTensor(RT_EIGVECS, "eigVecs" + name()); }
+volVectorField logConformationEq::fvcDivTau(const std::vector<scalar>& tauBoundary) const {
+    volVectorField f("div(tau" + name() + ")", *U().mesh);
+    gpuContext::get().check(
+        rt_model_div_tau(model_, tauBoundary.empty() ? nullptr : tauBoundary.data(), f.internal.data()),
+        "divTau");
+    return f;
+}
+volSymmTensorField logConformationEq::tauTotal() const {
+    volSymmTensorField f("tauTotal" + name(), *U().mesh);
+    gpuContext::get().check(rt_model_tau_total(model_, f.internal.data()), "tauTotal");
+    return f;
+}
 
 constitutiveModel::constitutiveModel(const volVectorField& U, const surfaceScalarField& phi,
                                      const dictionary& constitutiveProperties) {

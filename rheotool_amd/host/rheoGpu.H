@@ -205,6 +205,12 @@ public:
     volSymmTensorField theta() const;
     volTensorField eigVals() const;
     volTensorField eigVecs() const;
+    //- the explicit part of divTau(U) (constitutiveEq.C:88-92): fvc::div(tau()/rho()),
+    //  Gauss linear; tau's patch values from tauBoundary ([6*nBoundaryFaces]) or
+    //  zeroGradient when it is empty
+    volVectorField fvcDivTau(const std::vector<scalar>& tauBoundary = std::vector<scalar>()) const;
+    //- tauTotal() (constitutiveEq.C:534-547)
+    volSymmTensorField tauTotal() const;
     const std::vector<solverPerformance>& solverPerf() const { return perf_; }
     const word& type() const { return type_; }
 

@@ -290,3 +290,67 @@ def test_random_graph_analytic_gradu_mixed(ctx):
     assert np.all(np.isfinite(g["theta"])) and np.all(np.isfinite(g["tau"]))
     lam = np.diagonal(g["ev"], axis1=1, axis2=2)
     assert np.all(lam > 0)
+
+
+@pytest.mark.parametrize("relax", [0.0, 1.0, 0.7])
+@pytest.mark.parametrize("scheme", ["upwind", "cubista"])
+def test_fixed_value_inlet_and_relax(ctx, relax, scheme):
+    """fixedValue theta on the inlet (boundaryCoeffs = -phi_b theta_b,
+    gaussDefCmpwConvectionScheme.C:102-116), zeroGradient elsewhere, with and
+    without the equation relaxation factor (Oldroyd_BLog.C:166, [OF9]
+    fvMatrix::relax) and the cubista correction (whose gradients take the
+    inlet values): converged theta and tau within 1e-9 of the oracle."""
+    m, f = cases.channel2d(32)
+    rng = np.random.default_rng(9)
+    NB = m.nBoundary
+    bcval = np.zeros((NB, 6))
+    s, n = int(m.patchStart[0]), int(m.patchSize[0])
+    inlet_cells = m.owner[m.nInternal + s:m.nInternal + s + n]
+    bcval[s:s + n] = f.theta[inlet_cells] + 0.05 * rng.normal(size=(n, 6))
+    bc = O.default_bc(m)
+    bc[0] = O.BC_FIXEDVALUE
+    print(f"[case] channel fixedValue inlet relax={relax} scheme={scheme}", flush=True)
+    dm = ctx.mesh(m)
+    eq = ctx.constitutiveEq("", dm, abi.model_dict("Oldroyd-BLog", lam=1.0))
+    eq.set_solver(abi.solver_dict(1e-10))
+    if scheme != "upwind":
+        eq.set_scheme("GaussDefCmpw " + scheme)
+    eq.set_state(f.theta, f.eigVals, f.eigVecs)
+    eq.set_flow(f.U, f.Ub, f.phi)
+    eq.set_bc(0, abi.RT_BC_FIXEDVALUE, bcval[s:s + n])
+    eq.set_relax(relax)
+    assert eq.correct(1e-3) == abi.RT_OK
+    g = dict(theta=eq.theta(), tau=eq.tau(), perf=eq.solver_perf())
+    p = oracle_params("Oldroyd-BLog", 1.0, {})
+    th, ev, vec, tau, perf = O.correct(m, p, f, 1e-3, bc=bc, bcval=bcval, relax=relax, scheme=scheme)
+    check_step(g, dict(theta=th, tau=tau, perf=perf), m.validComponents)
+    # the inlet values matter: the zeroGradient step differs
+    th_zg = O.correct(m, p, f, 1e-3, relax=relax, scheme=scheme)[0]
+    assert rel_l2(th, th_zg) > 1e-8
+
+
+@pytest.mark.parametrize("with_patch_values", [False, True])
+def test_div_tau_and_tau_total_match_oracle(ctx, with_patch_values):
+    """The momentum-equation consumers of tau after a step:
+    fvc::div(tau/rho) (constitutiveEq.C:88-92) and tauTotal()
+    (constitutiveEq.C:534-547), on a permuted jittered box with rho != 1.
+    Explicit per-row arithmetic in the oracle's operation order: 1e-13."""
+    m = cases.cartesian(12, 10, 8, jitter=0.1, perm_seed=2)
+    f = cases.make_fields(m, cases.FIELDS_TG, seed=4, lnLo=-3.0, lnHi=2.0)
+    dm = ctx.mesh(m)
+    d = abi.model_dict("GiesekusLog", lam=2.0, alpha=0.1, rho=1.3, etaS=0.3, etaP=0.7)
+    eq = ctx.constitutiveEq("", dm, d)
+    eq.set_solver(abi.solver_dict(1e-10))
+    eq.set_state(f.theta, f.eigVals, f.eigVecs)
+    eq.set_flow(f.U, f.Ub, f.phi)
+    assert eq.correct(1e-3) == abi.RT_OK
+    tau = eq.tau()
+    tauB = None
+    if with_patch_values:
+        tauB = np.random.default_rng(5).normal(size=(m.nBoundary, 6))
+    dv = eq.div_tau(tauB)
+    want = O.div_tau(m, tau, tauB, 1.3)
+    assert rel_l2(dv, want) < 1e-13
+    tt = eq.tau_total()
+    want_t = O.tau_total(m, f.U, f.Ub, tau, 0.3)
+    assert rel_l2(tt, want_t) < 1e-13
