@@ -585,7 +585,7 @@ void orc_rhs(const orc_params* p, int32_t n, const double* gradU, const double* 
             dotTT(V, D, t1);
             dotTT(t1, VT, Y);
             for (int q = 0; q < 9; ++q) X[q] = rl * Y[q];
-        } else {  // PTT linear / exponential
+        } else if (p->model == ORC_PTT_LINEAR || p->model == ORC_PTT_EXP) {
             T9 VT, t1, C, D, Y;
             transp(V, VT);
             for (int q = 0; q < 9; ++q) D[q] = invL[q] - I9[q];
@@ -597,6 +597,76 @@ void orc_rhs(const orc_params* p, int32_t n, const double* gradU, const double* 
             const double z = k * (C[0] + C[4] + C[8] - 3.);
             const double fac = p->model == ORC_PTT_LINEAR ? 1. + z : std::exp(z);
             for (int q = 0; q < 9; ++q) X[q] = (rl * Y[q]) * fac;
+        } else if (p->model == ORC_FENECR) {
+            // FENE_CRLog.C:142-164: f = L2/(L2 - tr(V Lam V^T)), X = (1/lambda) f V (inv(Lam) - I) V^T
+            T9 VT, t1, C, D, Y;
+            transp(V, VT);
+            dotTT(V, Lam, t1);
+            dotTT(t1, VT, C);
+            const double f = p->L2 / (p->L2 - (C[0] + C[4] + C[8]));
+            if (fOut) fOut[c] = f;
+            for (int q = 0; q < 9; ++q) D[q] = invL[q] - I9[q];
+            dotTT(V, D, t1);
+            dotTT(t1, VT, Y);
+            for (int q = 0; q < 9; ++q) X[q] = (rl * f) * Y[q];
+        } else if (p->model == ORC_WMCY) {
+            // WhiteMetznerCYLog.C:154-187: shear-rate dependent etaP and lambda
+            double sL[6];
+            symmT(L, sL);
+            const double magS = std::sqrt(sL[0] * sL[0] + 2 * (sL[1] * sL[1]) + 2 * (sL[2] * sL[2]) + sL[3] * sL[3] +
+                                          2 * (sL[4] * sL[4]) + sL[5] * sL[5]);
+            const double etaP =
+                p->etaP * std::pow(1 + std::pow(p->wmK * std::sqrt(2.0) * magS, p->wmA), (p->wmN - 1) / p->wmA);
+            const double lam =
+                p->lambda * std::pow(1 + std::pow(p->wmL * std::sqrt(2.0) * magS, p->wmB), (p->wmM - 1) / p->wmB);
+            if (fOut) fOut[c] = etaP / lam;
+            T9 VT, t1, D, Y;
+            transp(V, VT);
+            for (int q = 0; q < 9; ++q) D[q] = invL[q] - I9[q];
+            dotTT(V, D, t1);
+            dotTT(t1, VT, Y);
+            const double rlc = 1.0 / lam;
+            for (int q = 0; q < 9; ++q) X[q] = rlc * Y[q];
+        } else if (p->model == ORC_ROLIEPOLY || p->model == ORC_XPP) {
+            // A = symm(V Lam V^T); X = -(1/lambda_) (V inv(Lam) V^T) & Z(A)
+            T9 VT, t1, C, A, Ci, Z;
+            transp(V, VT);
+            dotTT(V, Lam, t1);
+            dotTT(t1, VT, C);
+            double A6[6];
+            symmT(C, A6);
+            expandSymm(A6, A);
+            const double trA = A[0] + A[4] + A[8];
+            dotTT(V, invL, t1);
+            dotTT(t1, VT, Ci);
+            double rlm;
+            if (p->model == ORC_ROLIEPOLY) {  // RoliePolyLog.C:140-185
+                double M1 = 2. * (1. - std::sqrt(3. / trA)) / p->lambdaR;
+                if (p->chiMax > 1.) {
+                    const double c2 = p->chiMax * p->chiMax;
+                    M1 *= (3. - (trA / 3.) / c2) * (1. - 1. / c2) / ((1. - (trA / 3.) / c2) * (3. - 1. / c2));
+                }
+                const double bp = p->beta * std::pow(trA / 3., p->delta);
+                for (int q = 0; q < 9; ++q) {
+                    const double AmI = A[q] - I9[q];
+                    Z[q] = AmI + (M1 * p->lambdaD) * (A[q] + bp * AmI);
+                }
+                rlm = 1.0 / p->lambdaD;
+            } else {  // XPomPomLog.C:140-185
+                const double lb = std::sqrt(trA / 3.);
+                T9 AA;
+                dotTT(A, A, AA);
+                const double trAA = AA[0] + AA[4] + AA[8];
+                const double g = p->nXPP == 0 ? (1. - 1. / lb) : (1. - 1. / std::pow(lb, p->nXPP + 1));
+                const double f = 2. * (p->lambdaB / p->lambdaS) * std::exp((2. / p->q) * (lb - 1.)) * g +
+                                 (1. / (lb * lb)) * (1. - p->alpha - (p->alpha / 3.) * (trAA - 2. * trA));
+                for (int q = 0; q < 9; ++q)
+                    Z[q] = A[q] * (f - 2. * p->alpha) + p->alpha * AA[q] + I9[q] * (p->alpha - 1.);
+                rlm = 1.0 / p->lambdaB;
+            }
+            T9 P;
+            dotTT(Ci, Z, P);
+            for (int q = 0; q < 9; ++q) X[q] = -(rlm * P[q]);
         }
         for (int q = 0; q < 9; ++q) R[q] = R[q] + X[q];
         symmT(R, S + 6 * (size_t)c);
@@ -962,6 +1032,7 @@ void orc_tau(const orc_params* p, int32_t n, const double* eigVals, const double
         dotTT(V, Lam, t1);
         dotTT(t1, VT, C);
         double coef;
+        double post = 1.0;  // Rolie-Poly finite-extensibility factor
         if (p->model == ORC_FENEP) {
             const double a = p->L2 / (p->L2 - 3.);
             const double f = fOld[c];
@@ -969,13 +1040,31 @@ void orc_tau(const orc_params* p, int32_t n, const double* eigVals, const double
             coef = p->etaP / p->lambda;
         } else {
             for (int q = 0; q < 9; ++q) D[q] = C[q] - I9[q];
-            coef = (p->model == ORC_PTT_LINEAR || p->model == ORC_PTT_EXP)
-                       ? p->etaP / (p->lambda * (1 - p->zeta))
-                       : p->etaP / p->lambda;
+            if (p->model == ORC_PTT_LINEAR || p->model == ORC_PTT_EXP)
+                coef = p->etaP / (p->lambda * (1 - p->zeta));
+            else if (p->model == ORC_FENECR)  // FENE_CRLog.C:176
+                coef = (p->etaP / p->lambda) * fOld[c];
+            else if (p->model == ORC_WMCY)  // WhiteMetznerCYLog.C:187, etaP/lambda of the old L
+                coef = fOld[c];
+            else if (p->model == ORC_ROLIEPOLY) {  // RoliePolyLog.C:171-183
+                coef = p->etaP / p->lambdaD;
+                if (p->chiMax > 1.) {
+                    double s6[6];
+                    symmT(C, s6);
+                    const double trA = s6[0] + s6[3] + s6[5];
+                    const double c2 = p->chiMax * p->chiMax;
+                    post = (3. - (trA / 3.) / c2) * (1. - 1. / c2) / ((1. - (trA / 3.) / c2) * (3. - 1. / c2));
+                }
+            } else if (p->model == ORC_XPP)  // XPomPomLog.C:180
+                coef = p->etaP / p->lambdaB;
+            else
+                coef = p->etaP / p->lambda;
         }
         double s[6];
         symmT(D, s);
-        for (int q = 0; q < 6; ++q) tau[6 * (size_t)c + q] = coef * s[q];
+        for (int q = 0; q < 6; ++q) tau[6 * (size_t)c + q] = p->model == ORC_ROLIEPOLY && p->chiMax > 1.
+                                                                 ? (coef * s[q]) * post
+                                                                 : coef * s[q];
     }
 }
 

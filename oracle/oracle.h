@@ -37,7 +37,8 @@ typedef struct {
     const int32_t* patchEmpty; /* 1: empty patch (no faces in the fvPatch sense) */
 } orc_mesh;
 
-enum { ORC_OLDROYDB = 0, ORC_GIESEKUS = 1, ORC_FENEP = 2, ORC_PTT_LINEAR = 3, ORC_PTT_EXP = 4 };
+enum { ORC_OLDROYDB = 0, ORC_GIESEKUS = 1, ORC_FENEP = 2, ORC_PTT_LINEAR = 3, ORC_PTT_EXP = 4,
+       ORC_FENECR = 5, ORC_WMCY = 6, ORC_ROLIEPOLY = 7, ORC_XPP = 8 };
 
 typedef struct {
     int32_t model;
@@ -47,6 +48,12 @@ typedef struct {
     double epsilon; /* PTT */
     double zeta;    /* PTT */
     int32_t ofInvQuirk; /* 1: apply [OF9] inv(tensorField) first-cell component shift */
+    /* Rolie-Poly (RoliePolyLog.C:114-121) */
+    double lambdaR, lambdaD, beta, delta, chiMax;
+    /* eXtended Pom-Pom (XPomPomLog.C:115-122); alpha above */
+    double lambdaS, lambdaB, q, nXPP;
+    /* White-Metzner Carreau-Yasuda (WhiteMetznerCYLog.C:115-124): m n K L a b */
+    double wmM, wmN, wmK, wmL, wmA, wmB;
 } orc_params;
 
 enum { ORC_BC_ZEROGRADIENT = 0, ORC_BC_FIXEDVALUE = 1, ORC_BC_EMPTY = 2 };
@@ -71,7 +78,10 @@ void orc_grad_gauss(const orc_mesh* m, const double* U, const double* Ub, double
 /* Per-cell right-hand side S = symm(Omega.theta - theta.Omega + 2B + X)
  * of the model, from the old eigen state (boilerLog.H, constitutiveEq.C:323-358,
  * Oldroyd_BLog.C:141-164, GiesekusLog.C:142-160, FENE_PLog.C:142-166,
- * PTTLog.C:184-196,241-253).  fOut (may be NULL) receives FENE-P's f. */
+ * PTTLog.C:184-196,241-253, FENE_CRLog.C:138-166, WhiteMetznerCYLog.C:150-190,
+ * RoliePolyLog.C:140-185, XPomPomLog.C:140-185).  fOut (may be NULL) receives
+ * the per-cell factor tau needs from the old state: f of FENE-P and FENE-CR,
+ * etaP/lambda of White-Metzner. */
 void orc_rhs(const orc_params* p, int32_t n, const double* gradU, const double* theta,
              const double* eigVals, const double* eigVecs, double* S, double* fOut);
 

@@ -20,8 +20,10 @@ BC_ZEROGRADIENT, BC_FIXEDVALUE, BC_EMPTY = 0, 1, 2
 PBICGSTAB, PBICG = 0, 1
 PC_NONE, PC_DIAGONAL, PC_DILU = 0, 1, 2
 
+FENECR, WMCY, ROLIEPOLY, XPP = 5, 6, 7, 8
 MODEL_IDS = {"Oldroyd-BLog": OLDROYDB, "GiesekusLog": GIESEKUS, "FENE-PLog": FENEP,
-             "PTTLog": PTT_LINEAR}
+             "PTTLog": PTT_LINEAR, "FENE-CRLog": FENECR, "WhiteMetznerCYLog": WMCY,
+             "Rolie-PolyLog": ROLIEPOLY, "XPomPomLog": XPP}
 
 
 class Mesh(C.Structure):
@@ -36,7 +38,13 @@ class Params(C.Structure):
     _fields_ = [("model", C.c_int32), ("rho", C.c_double), ("etaS", C.c_double),
                 ("etaP", C.c_double), ("lambda_", C.c_double), ("alpha", C.c_double),
                 ("L2", C.c_double), ("epsilon", C.c_double), ("zeta", C.c_double),
-                ("ofInvQuirk", C.c_int32)]
+                ("ofInvQuirk", C.c_int32),
+                ("lambdaR", C.c_double), ("lambdaD", C.c_double), ("beta", C.c_double),
+                ("delta", C.c_double), ("chiMax", C.c_double),
+                ("lambdaS", C.c_double), ("lambdaB", C.c_double), ("q", C.c_double),
+                ("nXPP", C.c_double),
+                ("wmM", C.c_double), ("wmN", C.c_double), ("wmK", C.c_double), ("wmL", C.c_double),
+                ("wmA", C.c_double), ("wmB", C.c_double)]
 
 
 class Controls(C.Structure):
@@ -105,11 +113,18 @@ class OracleMesh:
 
 
 def params(model="Oldroyd-BLog", etaS=0.59, etaP=0.41, lam=1.0, alpha=0.0, L2=100.0,
-           epsilon=0.0, zeta=0.0, rho=1.0, ptt_exp=False, of_inv_quirk=False):
+           epsilon=0.0, zeta=0.0, rho=1.0, ptt_exp=False, of_inv_quirk=False,
+           lambdaR=1.0, lambdaD=1.0, beta=0.0, delta=-0.5, chiMax=0.0,
+           lambdaS=1.0, lambdaB=1.0, q=1.0, n=0.0, m=1.0, K=0.0, Lc=0.0, a=2.0, b=2.0,
+           wm_n=1.0):
+    """Model constants.  Rolie-Poly: lambdaR lambdaD beta delta chiMax; XPP:
+    lambdaS lambdaB alpha q n; White-Metzner CY: lam m wm_n K Lc(=L) a b."""
     mid = MODEL_IDS[model]
     if model == "PTTLog" and ptt_exp:
         mid = PTT_EXP
-    return Params(mid, rho, etaS, etaP, lam, alpha, L2, epsilon, zeta, int(of_inv_quirk))
+    return Params(mid, rho, etaS, etaP, lam, alpha, L2, epsilon, zeta, int(of_inv_quirk),
+                  lambdaR, lambdaD, beta, delta, chiMax, lambdaS, lambdaB, q, n,
+                  m, wm_n, K, Lc, a, b)
 
 
 def controls(solver=PBICGSTAB, precond=PC_DILU, tol=1e-10, relTol=0.0, maxIter=1000, minIter=0):

@@ -339,12 +339,23 @@ class ConstitutiveEq:
             pass
 
 
+# dimensions of the further models' constants (RoliePolyLog.C:117-121,
+# XPomPomLog.C:118-122, WhiteMetznerCYLog.C:119-124)
+_DIMS = {"lambdaR": "[0 0 1 0 0 0 0]", "lambdaD": "[0 0 1 0 0 0 0]", "lambdaS": "[0 0 1 0 0 0 0]",
+         "lambdaB": "[0 0 1 0 0 0 0]", "K": "[0 0 1 0 0 0 0]", "L": "[0 0 1 0 0 0 0]"}
+
+
 def model_dict(model="Oldroyd-BLog", etaS=0.59, etaP=0.41, lam=1.0, alpha=None, L2=None,
-               epsilon=None, zeta=None, destruction="linear", rho=1.0):
-    """constitutiveProperties `parameters` text in OpenFOAM syntax."""
+               epsilon=None, zeta=None, destruction="linear", rho=1.0, **extra):
+    """constitutiveProperties `parameters` text in OpenFOAM syntax.  extra:
+    further constants by their dictionary keys (lambdaR, lambdaD, beta, delta,
+    chiMax, lambdaS, lambdaB, q, n, m, K, L, a, b)."""
     lines = [f"type {model};", f"rho rho [1 -3 0 0 0 0 0] {rho!r};",
-             f"etaS etaS [1 -1 -1 0 0 0 0] {etaS!r};", f"etaP etaP [1 -1 -1 0 0 0 0] {etaP!r};",
-             f"lambda lambda [0 0 1 0 0 0 0] {lam!r};"]
+             f"etaS etaS [1 -1 -1 0 0 0 0] {etaS!r};", f"etaP etaP [1 -1 -1 0 0 0 0] {etaP!r};"]
+    if model not in ("Rolie-PolyLog", "XPomPomLog"):
+        lines.append(f"lambda lambda [0 0 1 0 0 0 0] {lam!r};")
+    for k, v in extra.items():
+        lines.append(f"{k} {k} {_DIMS.get(k, '[0 0 0 0 0 0 0]')} {v!r};")
     if alpha is not None:
         lines.append(f"alpha alpha [0 0 0 0 0 0 0] {alpha!r};")
     if L2 is not None:

@@ -142,7 +142,7 @@ __device__ __forceinline__ void cell_rhs(const ModelParams& p, int invMask, cons
         vdvt(V, dd, Y);
 #pragma unroll
         for (int q = 0; q < 9; ++q) X[q] = rl * Y[q];
-    } else {
+    } else if constexpr (KIND == M_PTT_LINEAR || KIND == M_PTT_EXP) {
         const double dd[3] = {il[0] - 1.0, il[1] - 1.0, il[2] - 1.0};
         double Y[9], C[9];
         vdvt(V, dd, Y);
@@ -152,6 +152,76 @@ __device__ __forceinline__ void cell_rhs(const ModelParams& p, int invMask, cons
         const double fac = KIND == M_PTT_LINEAR ? 1. + z : exp(z);
 #pragma unroll
         for (int q = 0; q < 9; ++q) X[q] = (rl * Y[q]) * fac;
+    } else if constexpr (KIND == M_FENECR) {
+        // FENE_CRLog.C:142-164: X = (1/lambda) f V (inv(Lam) - I) V^T, f = L2/(L2 - tr C)
+        double C[9], Y[9];
+        vdvt(V, lam, C);
+        const double f = p.L2 / (p.L2 - (C[0] + C[4] + C[8]));
+        fOut = f;
+        const double dd[3] = {il[0] - 1.0, il[1] - 1.0, il[2] - 1.0};
+        vdvt(V, dd, Y);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) X[q] = (rl * f) * Y[q];
+    } else if constexpr (KIND == M_WMCY) {
+        // WhiteMetznerCYLog.C:154-187: etaP(gd), lambda(gd), gd = sqrt(2)|symm(L)|
+        double sL[6];
+        symmT(L, sL);
+        const double magS = sqrt(sL[0] * sL[0] + 2 * (sL[1] * sL[1]) + 2 * (sL[2] * sL[2]) + sL[3] * sL[3] +
+                                 2 * (sL[4] * sL[4]) + sL[5] * sL[5]);
+        const double etaP = p.etaP * pow(1 + pow(p.wmK * sqrt(2.0) * magS, p.wmA), (p.wmN - 1) / p.wmA);
+        const double lamc = p.lambda * pow(1 + pow(p.wmL * sqrt(2.0) * magS, p.wmB), (p.wmM - 1) / p.wmB);
+        fOut = etaP / lamc;
+        const double dd[3] = {il[0] - 1.0, il[1] - 1.0, il[2] - 1.0};
+        double Y[9];
+        vdvt(V, dd, Y);
+        const double rlc = 1.0 / lamc;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) X[q] = rlc * Y[q];
+    } else {
+        // Rolie-Poly (RoliePolyLog.C:140-165) and XPP (XPomPomLog.C:140-172):
+        // X = -(1/lambda_) (V inv(Lam) V^T) & Z(A), A = symm(V Lam V^T)
+        double C[9], Ci[9], A[9], Z[9];
+        vdvt(V, lam, C);
+        vdvt(V, il, Ci);
+        {
+            double a6[6];
+            symmT(C, a6);
+            const double a9[9] = {a6[0], a6[1], a6[2], a6[1], a6[3], a6[4], a6[2], a6[4], a6[5]};
+#pragma unroll
+            for (int q = 0; q < 9; ++q) A[q] = a9[q];
+        }
+        const double trA = A[0] + A[4] + A[8];
+        const double I9[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+        double rlm;
+        if constexpr (KIND == M_ROLIEPOLY) {
+            double M1 = 2. * (1. - sqrt(3. / trA)) / p.lambdaR;
+            if (p.chiMax > 1.) {
+                const double c2 = p.chiMax * p.chiMax;
+                M1 *= (3. - (trA / 3.) / c2) * (1. - 1. / c2) / ((1. - (trA / 3.) / c2) * (3. - 1. / c2));
+            }
+            const double bp = p.beta * pow(trA / 3., p.delta);
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                const double AmI = A[q] - I9[q];
+                Z[q] = AmI + (M1 * p.lambdaD) * (A[q] + bp * AmI);
+            }
+            rlm = 1.0 / p.lambdaD;
+        } else {
+            const double lb = sqrt(trA / 3.);
+            double AA[9];
+            dotTT(A, A, AA);
+            const double trAA = AA[0] + AA[4] + AA[8];
+            const double g = p.nXPP == 0 ? (1. - 1. / lb) : (1. - 1. / pow(lb, p.nXPP + 1));
+            const double f = 2. * (p.lambdaB / p.lambdaS) * exp((2. / p.q) * (lb - 1.)) * g +
+                             (1. / (lb * lb)) * (1. - p.alpha - (p.alpha / 3.) * (trAA - 2. * trA));
+#pragma unroll
+            for (int q = 0; q < 9; ++q) Z[q] = A[q] * (f - 2. * p.alpha) + p.alpha * AA[q] + I9[q] * (p.alpha - 1.);
+            rlm = 1.0 / p.lambdaB;
+        }
+        double P[9];
+        dotTT(Ci, Z, P);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) X[q] = -(rlm * P[q]);
     }
 #pragma unroll
     for (int q = 0; q < 9; ++q) R[q] = R[q] + X[q];
@@ -466,24 +536,43 @@ __global__ void __launch_bounds__(BLK) k_eig_tau(int32_t N, ModelParams p, const
     double C[9], D[9];
     vdvt(V, lam, C);
     double coef;
+    double post = 1.0;
+    const double I9[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     if (p.kind == M_FENEP) {
         const double a = p.L2 / (p.L2 - 3.);
         const double f = fOld[r];
-        const double I9[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
 #pragma unroll
         for (int q = 0; q < 9; ++q) D[q] = f * C[q] - a * I9[q];
         coef = p.etaP / p.lambda;
     } else {
-        const double I9[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
 #pragma unroll
         for (int q = 0; q < 9; ++q) D[q] = C[q] - I9[q];
-        coef = (p.kind == M_PTT_LINEAR || p.kind == M_PTT_EXP) ? p.etaP / (p.lambda * (1 - p.zeta))
-                                                               : p.etaP / p.lambda;
+        if (p.kind == M_PTT_LINEAR || p.kind == M_PTT_EXP) {
+            coef = p.etaP / (p.lambda * (1 - p.zeta));
+        } else if (p.kind == M_FENECR) {  // FENE_CRLog.C:176, f of the old state
+            coef = (p.etaP / p.lambda) * fOld[r];
+        } else if (p.kind == M_WMCY) {  // WhiteMetznerCYLog.C:187, etaP/lambda of the old L
+            coef = fOld[r];
+        } else if (p.kind == M_ROLIEPOLY) {  // RoliePolyLog.C:171-183
+            coef = p.etaP / p.lambdaD;
+            if (p.chiMax > 1.) {
+                double s6[6];
+                symmT(C, s6);
+                const double trA = s6[0] + s6[3] + s6[5];
+                const double c2 = p.chiMax * p.chiMax;
+                post = (3. - (trA / 3.) / c2) * (1. - 1. / c2) / ((1. - (trA / 3.) / c2) * (3. - 1. / c2));
+            }
+        } else if (p.kind == M_XPP) {  // XPomPomLog.C:180
+            coef = p.etaP / p.lambdaB;
+        } else {
+            coef = p.etaP / p.lambda;
+        }
     }
     double s[6];
     symmT(D, s);
+    const bool chi = p.kind == M_ROLIEPOLY && p.chiMax > 1.;
 #pragma unroll
-    for (int q = 0; q < 6; ++q) s[q] = coef * s[q];
+    for (int q = 0; q < 6; ++q) s[q] = chi ? (coef * s[q]) * post : coef * s[q];
     st6(tau + (int64_t)NCMP * r, s);
 }
 
@@ -918,8 +1007,20 @@ static void launch_assemble_kind(Ctx* ctx, const AssembleArgs& a, int grid) {
     case M_PTT_LINEAR:
         hipLaunchKernelGGL((k_assemble<T, M_PTT_LINEAR>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
         break;
-    default:
+    case M_PTT_EXP:
         hipLaunchKernelGGL((k_assemble<T, M_PTT_EXP>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
+        break;
+    case M_FENECR:
+        hipLaunchKernelGGL((k_assemble<T, M_FENECR>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
+        break;
+    case M_WMCY:
+        hipLaunchKernelGGL((k_assemble<T, M_WMCY>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
+        break;
+    case M_ROLIEPOLY:
+        hipLaunchKernelGGL((k_assemble<T, M_ROLIEPOLY>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
+        break;
+    default:
+        hipLaunchKernelGGL((k_assemble<T, M_XPP>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
         break;
     }
 }
@@ -941,7 +1042,7 @@ void launch_assemble(Ctx* ctx, const AssembleArgs& a, int grid) {
 }
 void launch_eig_tau(Ctx* ctx, int32_t N, const ModelParams& prm, const double* theta,
                     double* eigVal, double* eigVec, double* tau, const double* fOld) {
-    ProfScope ps(ctx, "k_eig_tau", (double)N * (48 + 24 + 72 + 48 + (prm.kind == M_FENEP ? 8 : 0)));
+    ProfScope ps(ctx, "k_eig_tau", (double)N * (48 + 24 + 72 + 48 + (modelKeepsFactor(prm.kind) ? 8 : 0)));
     hipLaunchKernelGGL(k_eig_tau, dim3(nblk(N)), dim3(BLK), 0, ctx->stream, N, prm, theta, eigVal,
                        eigVec, tau, fOld);
     RT_HIP(hipGetLastError());

@@ -210,7 +210,8 @@ Controls parseControls(const char* text) {
 // ------------------------------------------------------------ model setup
 namespace {
 
-const char* kTypes[] = {"FENE-PLog", "GiesekusLog", "Oldroyd-BLog", "PTTLog"};
+const char* kTypes[] = {"FENE-CRLog", "FENE-PLog", "GiesekusLog", "Oldroyd-BLog", "PTTLog",
+                        "Rolie-PolyLog", "WhiteMetznerCYLog", "XPomPomLog"};
 
 double need(const Dict& d, const char* key, const std::string& dictName) {
     double v;
@@ -231,6 +232,10 @@ void parseModel(Model& M, const char* text) {
     else if (type == "GiesekusLog") p.kind = M_GIESEKUS;
     else if (type == "FENE-PLog") p.kind = M_FENEP;
     else if (type == "PTTLog") p.kind = M_PTT_LINEAR;
+    else if (type == "FENE-CRLog") p.kind = M_FENECR;
+    else if (type == "WhiteMetznerCYLog") p.kind = M_WMCY;
+    else if (type == "Rolie-PolyLog") p.kind = M_ROLIEPOLY;
+    else if (type == "XPomPomLog") p.kind = M_XPP;
     else {
         std::string msg = "Unknown constitutiveEq type " + type + "\n\nValid constitutiveEq types are :\n" +
                           std::to_string(sizeof(kTypes) / sizeof(kTypes[0])) + "\n(\n";
@@ -245,9 +250,32 @@ void parseModel(Model& M, const char* text) {
         p.epsilon = need(d, "epsilon", dn);
         p.zeta = need(d, "zeta", dn);
     }
-    p.lambda = need(d, "lambda", dn);
+    // the keys each constructor looks up (e.g. RoliePolyLog.C:114-121, XPomPomLog.C:115-122)
+    if (p.kind == M_ROLIEPOLY) {
+        p.lambdaR = need(d, "lambdaR", dn);
+        p.lambdaD = need(d, "lambdaD", dn);
+        p.beta = need(d, "beta", dn);
+        p.delta = need(d, "delta", dn);
+        p.chiMax = need(d, "chiMax", dn);
+    } else if (p.kind == M_XPP) {
+        p.lambdaS = need(d, "lambdaS", dn);
+        p.lambdaB = need(d, "lambdaB", dn);
+        p.alpha = need(d, "alpha", dn);
+        p.q = need(d, "q", dn);
+        p.nXPP = need(d, "n", dn);
+    } else {
+        p.lambda = need(d, "lambda", dn);
+    }
+    if (p.kind == M_WMCY) {  // WhiteMetznerCYLog.C:119-124
+        p.wmM = need(d, "m", dn);
+        p.wmN = need(d, "n", dn);
+        p.wmK = need(d, "K", dn);
+        p.wmL = need(d, "L", dn);
+        p.wmA = need(d, "a", dn);
+        p.wmB = need(d, "b", dn);
+    }
     if (p.kind == M_GIESEKUS) p.alpha = need(d, "alpha", dn);
-    if (p.kind == M_FENEP) p.L2 = need(d, "L2", dn);
+    if (p.kind == M_FENEP || p.kind == M_FENECR) p.L2 = need(d, "L2", dn);
     if (p.kind == M_PTT_LINEAR) {
         std::string f;
         if (!d.word("destructionFunctionType", f))
@@ -266,7 +294,8 @@ void parseModel(Model& M, const char* text) {
     if (stab != "none" && stab != "BSD" && stab != "coupling")
         throw Error{RT_E_BAD_ARG, stab + " is not in enumeration: 3(none BSD coupling)"};
     // thermo functions: only the isothermal default (Constant) is on this path
-    for (const char* tf : {"thermoLambda", "thermoEta"}) {
+    for (const char* tf : {"thermoLambda", "thermoEta", "thermoLambdaR", "thermoLambdaD", "thermoLambdaS",
+                           "thermoLambdaB"}) {
         auto it = d.subDicts.find(tf);
         if (it != d.subDicts.end()) {
             std::string t = "Constant";
@@ -355,7 +384,7 @@ void modelCorrectBegin(Model& M, double deltaT, const double* gradU) {
         M.b = ctx->alloc((size_t)6 * N * 8);
         RT_HIP(hipMemsetAsync(M.b.p, 0, (size_t)6 * N * 8, ctx->stream));
     }
-    if (M.prm.kind == M_FENEP && !M.fOld.p) M.fOld = ctx->alloc((size_t)N * 8);
+    if (modelKeepsFactor(M.prm.kind) && !M.fOld.p) M.fOld = ctx->alloc((size_t)N * 8);
     if (gradU) {
         if (!M.gradU.p) M.gradU = ctx->alloc((size_t)9 * N * 8);
         DBuf& st = staging(ctx).buf;
@@ -380,7 +409,7 @@ void modelCorrectBegin(Model& M, double deltaT, const double* gradU) {
     a.slotMask = M.slotMask.as<uint32_t>();
     a.diag = M.diag.as<double>();
     a.b = M.b.as<double>();
-    a.fOld = M.prm.kind == M_FENEP ? M.fOld.as<double>() : nullptr;
+    a.fOld = modelKeepsFactor(M.prm.kind) ? M.fOld.as<double>() : nullptr;
     a.partials = M.kry.partials.as<double>();
     a.cs = st->c;
     a.nnzOut = &st->misc[0];
@@ -422,7 +451,7 @@ bool modelCorrectAdvance(Model& M, bool block) {
     solve_finish(M.kry, M.perf);
     launch_eig_tau(ctx, M.mesh->N, M.prm, M.theta.as<double>(), M.eigVal.as<double>(),
                    M.eigVec.as<double>(), M.tau.as<double>(),
-                   M.prm.kind == M_FENEP ? M.fOld.as<double>() : nullptr);
+                   modelKeepsFactor(M.prm.kind) ? M.fOld.as<double>() : nullptr);
     return true;
 }
 

@@ -231,13 +231,19 @@ struct Krylov {
 // GaussDefCmpw limiters (limiters.H:38-100); DS_NONE drops the convection term
 enum DivScheme { DS_UPWIND = 0, DS_CUBISTA = 1, DS_MINMOD = 2, DS_SMART = 3, DS_WACEB = 4, DS_SUPERBEE = 5,
                  DS_NONE = 6 };
-enum ModelKind { M_OLDROYDB = 0, M_GIESEKUS = 1, M_FENEP = 2, M_PTT_LINEAR = 3, M_PTT_EXP = 4 };
+enum ModelKind { M_OLDROYDB = 0, M_GIESEKUS = 1, M_FENEP = 2, M_PTT_LINEAR = 3, M_PTT_EXP = 4,
+                 M_FENECR = 5, M_WMCY = 6, M_ROLIEPOLY = 7, M_XPP = 8 };
+// models whose tau needs a per-cell factor of the old state (Model::fOld)
+inline bool modelKeepsFactor(int kind) { return kind == M_FENEP || kind == M_FENECR || kind == M_WMCY; }
 
 struct ModelParams {
     int kind = M_OLDROYDB;
     double rho = 1, etaS = 0, etaP = 0, lambda = 1;
     double alpha = 0, L2 = 0, epsilon = 0, zeta = 0;
     int ofInvQuirk = 0;
+    double lambdaR = 1, lambdaD = 1, beta = 0, delta = 0, chiMax = 0;  // Rolie-Poly
+    double lambdaS = 1, lambdaB = 1, q = 1, nXPP = 0;                 // XPP (+ alpha)
+    double wmM = 1, wmN = 1, wmK = 0, wmL = 0, wmA = 1, wmB = 1;      // White-Metzner CY
 };
 
 struct Model {

@@ -219,6 +219,10 @@ static constitutiveEq::adddictionaryConstructorToTable<Oldroyd_BLog> addOB("Oldr
 static constitutiveEq::adddictionaryConstructorToTable<GiesekusLog> addGiesekus("GiesekusLog");
 static constitutiveEq::adddictionaryConstructorToTable<FENE_PLog> addFENEP("FENE-PLog");
 static constitutiveEq::adddictionaryConstructorToTable<PTTLog> addPTT("PTTLog");
+static constitutiveEq::adddictionaryConstructorToTable<FENE_CRLog> addFENECR("FENE-CRLog");
+static constitutiveEq::adddictionaryConstructorToTable<WhiteMetznerCYLog> addWMCY("WhiteMetznerCYLog");
+static constitutiveEq::adddictionaryConstructorToTable<RoliePolyLog> addRP("Rolie-PolyLog");
+static constitutiveEq::adddictionaryConstructorToTable<XPomPomLog> addXPP("XPomPomLog");
 
 std::unique_ptr<constitutiveEq> constitutiveEq::New(const word& name, const volVectorField& U,
                                                     const surfaceScalarField& phi, const dictionary& dict) {

@@ -236,6 +236,10 @@ RHEOGPU_LOG_MODEL(Oldroyd_BLog, "Oldroyd-BLog")
 RHEOGPU_LOG_MODEL(GiesekusLog, "GiesekusLog")
 RHEOGPU_LOG_MODEL(FENE_PLog, "FENE-PLog")
 RHEOGPU_LOG_MODEL(PTTLog, "PTTLog")
+RHEOGPU_LOG_MODEL(FENE_CRLog, "FENE-CRLog")
+RHEOGPU_LOG_MODEL(WhiteMetznerCYLog, "WhiteMetznerCYLog")
+RHEOGPU_LOG_MODEL(RoliePolyLog, "Rolie-PolyLog")
+RHEOGPU_LOG_MODEL(XPomPomLog, "XPomPomLog")
 
 //- IOdictionary wrapper reading constitutiveProperties (constitutiveModel.C:41-59)
 class constitutiveModel {

@@ -264,10 +264,96 @@ def make_ldu(rng):
                                  s["perm"], s["kind"]], dtype=np.float64))
 
 
+# --------------------------------------------- the further log models
+def model_rhs_more(model, C, gradU, p):
+    """Conformation-tensor relaxation rate R(C) of dC/dt = kC + Ck^T + R(C),
+    written from each model's definition in conformation form (A = C):
+    FENE-CR  R = -(f/lambda)(C - I), f = L2/(L2 - tr C);
+    White-Metzner Carreau-Yasuda  R = -(C - I)/lambda(gd),
+        lambda(gd) = lambda (1 + (Lc gd)^b)^((m-1)/b), gd = sqrt(2)|sym(gradU)|;
+    Rolie-Poly  R = -(C - I)/lambdaD - M1 (C + beta (trC/3)^delta (C - I)),
+        M1 = 2 (1 - sqrt(3/trC))/lambdaR [x the chiMax factor];
+    XPP  R = -(C (f - 2 alpha) + alpha C^2 + (alpha - 1) I)/lambdaB."""
+    I = np.eye(3)
+    trC = np.trace(C)
+    if model == "FENE-CRLog":
+        f = p["L2"] / (p["L2"] - trC)
+        return -(f / p["lambda"]) * (C - I)
+    if model == "WhiteMetznerCYLog":
+        S = 0.5 * (gradU + gradU.T)
+        gd = np.sqrt(2.0) * np.sqrt(np.sum(S * S))
+        lam = p["lambda"] * (1 + (p["Lc"] * gd) ** p["b"]) ** ((p["m"] - 1) / p["b"])
+        return -(C - I) / lam
+    if model == "Rolie-PolyLog":
+        M1 = 2 * (1 - np.sqrt(3 / trC)) / p["lambdaR"]
+        if p["chiMax"] > 1:
+            c2 = p["chiMax"] ** 2
+            M1 *= (3 - trC / 3 / c2) * (1 - 1 / c2) / ((1 - trC / 3 / c2) * (3 - 1 / c2))
+        return -(C - I) / p["lambdaD"] - M1 * (C + p["beta"] * (trC / 3) ** p["delta"] * (C - I))
+    if model == "XPomPomLog":
+        a = p["alpha"]
+        lb = np.sqrt(trC / 3)
+        g = (1 - 1 / lb) if p["n"] == 0 else (1 - 1 / lb ** (p["n"] + 1))
+        f = 2 * p["lambdaB"] / p["lambdaS"] * np.exp(2 / p["q"] * (lb - 1)) * g + \
+            1 / lb ** 2 * (1 - a - a / 3 * (np.trace(C @ C) - 2 * trC))
+        return -(C * (f - 2 * a) + a * C @ C + (a - 1) * I) / p["lambdaB"]
+    raise KeyError(model)
+
+
+MODELS_MORE = {
+    "FENE-CRLog": dict(lam=1.5, L2=50.0),
+    "WhiteMetznerCYLog": dict(lam=1.2, m=0.6, wm_n=0.5, K=0.8, Lc=1.1, a=2.0, b=1.5),
+    "Rolie-PolyLog": dict(lambdaR=0.3, lambdaD=2.0, beta=0.5, delta=-0.5, chiMax=3.0),
+    "Rolie-PolyLog-nochi": dict(lambdaR=0.4, lambdaD=1.5, beta=0.3, delta=-0.5, chiMax=0.0),
+    "XPomPomLog": dict(lambdaS=0.4, lambdaB=1.6, alpha=0.1, q=3.0, n=0.0),
+    "XPomPomLog-n2": dict(lambdaS=0.5, lambdaB=1.2, alpha=0.15, q=2.0, n=2.0),
+}
+
+
+def make_rhs_more():
+    rng = np.random.default_rng(20261203)
+    out = {}
+    for key, prm in MODELS_MORE.items():
+        model = key.split("-nochi")[0].split("-n2")[0]
+        p = dict(prm)
+        p["lambda"] = p.pop("lam", 1.0)
+        if "wm_n" in p:
+            p["n"] = p["wm_n"]
+        ths, evs, vecs, Ls, S = [], [], [], [], []
+        while len(ths) < 60:
+            R = rand_rot(rng)
+            ln = np.sort(rng.uniform(-1.5, 1.5, 3))
+            if np.min(np.diff(ln)) < 0.05:
+                continue
+            C = R @ np.diag(np.exp(ln)) @ R.T
+            th = R @ np.diag(ln) @ R.T
+            L = rng.standard_normal((3, 3))
+            w, V = np.linalg.eigh(th)
+            kap = L.T
+            E = kap @ C + C @ kap.T + model_rhs_more(model, C, L, p)
+            ths.append(sym6(th))
+            evs.append(np.diag(np.exp(w)))
+            vecs.append(V)
+            Ls.append(L)
+            S.append(sym6(dlog(C, E)))
+        k = key.replace("-", "_")
+        out[k + "_theta"] = np.array(ths)
+        out[k + "_eigVals"] = np.array(evs)
+        out[k + "_eigVecs"] = np.array(vecs)
+        out[k + "_gradU"] = np.array(Ls)
+        out[k + "_S"] = np.array(S)
+    np.savez_compressed(os.path.join(HERE, "rhs_more.npz"), **out)
+
+
 if __name__ == "__main__":
-    rng = np.random.default_rng(20261202)
-    make_eig(rng)
-    make_rhs(rng)
-    make_steady()
-    make_ldu(rng)
+    # python tests/golden/make_golden.py [eig rhs steady ldu rhs_more]
+    which = sys.argv[1:] or ["eig", "rhs", "steady", "ldu", "rhs_more"]
+    if set(which) & {"eig", "rhs", "steady", "ldu"}:
+        rng = np.random.default_rng(20261202)  # one stream through these four, in this order
+        make_eig(rng)
+        make_rhs(rng)
+        make_steady()
+        make_ldu(rng)
+    if "rhs_more" in which:
+        make_rhs_more()
     print("golden fixtures written to", HERE)

@@ -354,3 +354,52 @@ def test_div_tau_and_tau_total_match_oracle(ctx, with_patch_values):
     tt = eq.tau_total()
     want_t = O.tau_total(m, f.U, f.Ub, tau, 0.3)
     assert rel_l2(tt, want_t) < 1e-13
+
+
+# the further log models (FENE_CRLog.C, WhiteMetznerCYLog.C, RoliePolyLog.C,
+# XPomPomLog.C): device keys, oracle keyword names
+MORE = {
+    "FENE-CRLog": (dict(L2=50.0), dict(L2=50.0)),
+    "WhiteMetznerCYLog": (dict(m=0.6, n=0.5, K=0.8, L=1.1, a=2.0, b=1.5),
+                          dict(m=0.6, wm_n=0.5, K=0.8, Lc=1.1, a=2.0, b=1.5)),
+    "Rolie-PolyLog": (dict(lambdaR=0.3, lambdaD=2.0, beta=0.5, delta=-0.5, chiMax=3.0),
+                      dict(lambdaR=0.3, lambdaD=2.0, beta=0.5, delta=-0.5, chiMax=3.0)),
+    "XPomPomLog": (dict(lambdaS=0.4, lambdaB=1.6, alpha=0.1, q=3.0, n=0.0),
+                   dict(lambdaS=0.4, lambdaB=1.6, alpha=0.1, q=3.0, n=0.0)),
+}
+
+
+def run_more(ctx, model, m, f, dt, precision="double"):
+    dev, orc = MORE[model]
+    print(f"[case] {model} N={m.nCells} precision={precision}", flush=True)
+    dm = ctx.mesh(m)
+    eq = ctx.constitutiveEq("", dm, abi.model_dict(model, lam=1.2, **dev))
+    eq.set_solver(abi.solver_dict(1e-10, precision=precision))
+    eq.set_state(f.theta, f.eigVals, f.eigVecs)
+    eq.set_flow(f.U, f.Ub, f.phi)
+    assert eq.correct(dt) == abi.RT_OK, eq.solver_perf()
+    g = dict(theta=eq.theta(), tau=eq.tau(), perf=eq.solver_perf())
+    p = O.params(model, etaS=0.59, etaP=0.41, lam=1.2, **orc)
+    th, ev, vec, tau, perf = O.correct(m, p, f, dt)
+    return g, dict(theta=th, tau=tau, perf=perf)
+
+
+@pytest.mark.parametrize("model", list(MORE))
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_further_log_models_match_oracle(ctx, model, precision):
+    m = cases.cartesian(12, 10, 9, jitter=0.1, perm_seed=4)
+    f = cases.make_fields(m, cases.FIELDS_TG, seed=8, lnLo=-1.5, lnHi=1.5)
+    g, o = run_more(ctx, model, m, f, 2e-3, precision)
+    check_step(g, o, m.validComponents)
+
+
+@pytest.mark.parametrize("model", list(MORE))
+def test_further_log_models_explicit_step(ctx, model):
+    """phi = 0: theta_new = theta_old + dt S.  The device's pow/exp/sqrt may
+    differ from the host library's in the last ulp, so the bar is 1e-12."""
+    m, f = cases.box3d(10, lnLo=-1.5, lnHi=1.5)
+    f.phi[:] = 0.0
+    g, o = run_more(ctx, model, m, f, 1e-2)
+    d = np.abs(g["theta"] - o["theta"]).max() / np.abs(o["theta"]).max()
+    assert d < 1e-12, (model, d)
+    assert rel_l2(g["tau"], o["tau"]) < 1e-12

@@ -1,0 +1,106 @@
+"""Oracle pinning for FENE-CR, White-Metzner Carreau-Yasuda, Rolie-Poly and
+XPP (the further log-conformation models: FENE_CRLog.C, WhiteMetznerCYLog.C,
+RoliePolyLog.C, XPomPomLog.C).  The fixture holds the log-conformation
+right-hand side obtained from each model's conformation-tensor equation
+through the Daleckii-Krein derivative of the matrix logarithm (an
+independent formulation), written by tests/golden/make_golden.py."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.golden.make_golden import MODELS_MORE
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def base(key):
+    return key.split("-nochi")[0].split("-n2")[0]
+
+
+def params(key, **over):
+    prm = dict(MODELS_MORE[key])
+    prm.update(over)
+    return O.params(base(key), etaP=0.41, **prm)
+
+
+@pytest.mark.parametrize("key", list(MODELS_MORE))
+def test_rhs_matches_daleckii_krein(key):
+    d = np.load(os.path.join(HERE, "rhs_more.npz"))
+    k = key.replace("-", "_")
+    S, _ = O.rhs(params(key), d[k + "_gradU"], d[k + "_theta"], d[k + "_eigVals"], d[k + "_eigVecs"])
+    ref = d[k + "_S"]
+    err = np.abs(S - ref).max(1) / (1 + np.abs(ref).max(1))
+    assert err.max() < 1e-10, err.max()
+
+
+@pytest.mark.parametrize("key,over", [("FENE-CRLog", dict(L2=60.0)),
+                                      ("WhiteMetznerCYLog", dict(m=0.7)),
+                                      ("Rolie-PolyLog", dict(beta=0.4)),
+                                      ("XPomPomLog", dict(q=2.5))])
+def test_planted_parameter_mistake_is_caught(key, over):
+    d = np.load(os.path.join(HERE, "rhs_more.npz"))
+    k = key.replace("-", "_")
+    S, _ = O.rhs(params(key, **over), d[k + "_gradU"], d[k + "_theta"], d[k + "_eigVals"],
+                 d[k + "_eigVecs"])
+    assert np.abs(S - d[k + "_S"]).max() > 1e-6
+
+
+def _state(rng, n=20):
+    th, ev, vec = [], [], []
+    for _ in range(n):
+        A = rng.normal(size=(3, 3))
+        Q, _ = np.linalg.qr(A)
+        ln = rng.uniform(-1, 1.5, 3)
+        C = Q @ np.diag(np.exp(ln)) @ Q.T
+        w, V = np.linalg.eigh(C)
+        t = V @ np.diag(np.log(w)) @ V.T
+        th.append([t[0, 0], t[0, 1], t[0, 2], t[1, 1], t[1, 2], t[2, 2]])
+        ev.append(np.diag(w))
+        vec.append(V)
+    return np.array(th), np.array(ev), np.array(vec)
+
+
+def _sym(M):
+    return np.stack([M[:, 0, 0], M[:, 0, 1], M[:, 0, 2], M[:, 1, 1], M[:, 1, 2], M[:, 2, 2]], 1)
+
+
+def test_tau_closed_forms():
+    """tau from the new eigen state (and, where the model says so, the old
+    state's factor): FENE-CR (etaP/lambda) f (C - I); White-Metzner
+    (etaP(gd)/lambda(gd)) (C - I); Rolie-Poly (etaP/lambdaD)(C - I) x the
+    chiMax factor of the new tr C; XPP (etaP/lambdaB)(C - I)."""
+    rng = np.random.default_rng(6)
+    th, ev, vec = _state(rng)
+    C = np.einsum("nik,nkl,njl->nij", vec, ev, vec)
+    I = np.eye(3)[None]
+    trC = np.trace(C, axis1=1, axis2=2)
+    L = rng.normal(size=(len(th), 3, 3))
+    # FENE-CR: f of the (old) state given to rhs
+    p = params("FENE-CRLog")
+    _, f = O.rhs(p, L, th, ev, vec)
+    assert np.allclose(f, 50.0 / (50.0 - trC), rtol=1e-13)
+    t = O.tau(p, ev, vec, f)
+    assert np.allclose(t, _sym(0.41 / 1.5 * f[:, None, None] * (C - I)), rtol=1e-12, atol=1e-13)
+    # White-Metzner: etaP(gd)/lambda(gd) of the old gradient
+    p = params("WhiteMetznerCYLog")
+    _, k = O.rhs(p, L, th, ev, vec)
+    S = 0.5 * (L + np.swapaxes(L, 1, 2))
+    gd = np.sqrt(2.0) * np.sqrt((S * S).sum((1, 2)))
+    pr = MODELS_MORE["WhiteMetznerCYLog"]
+    eta = 0.41 * (1 + (pr["K"] * gd) ** pr["a"]) ** ((pr["wm_n"] - 1) / pr["a"])
+    lam = pr["lam"] * (1 + (pr["Lc"] * gd) ** pr["b"]) ** ((pr["m"] - 1) / pr["b"])
+    assert np.allclose(k, eta / lam, rtol=1e-13)
+    t = O.tau(p, ev, vec, k)
+    assert np.allclose(t, _sym((eta / lam)[:, None, None] * (C - I)), rtol=1e-12, atol=1e-13)
+    # Rolie-Poly with finite extensibility
+    p = params("Rolie-PolyLog")
+    c2 = 9.0
+    fac = (3 - trC / 3 / c2) * (1 - 1 / c2) / ((1 - trC / 3 / c2) * (3 - 1 / c2))
+    t = O.tau(p, ev, vec, None)
+    assert np.allclose(t, _sym(0.41 / 2.0 * fac[:, None, None] * (C - I)), rtol=1e-12, atol=1e-13)
+    # XPP
+    p = params("XPomPomLog")
+    t = O.tau(p, ev, vec, None)
+    assert np.allclose(t, _sym(0.41 / 1.6 * (C - I)), rtol=1e-12, atol=1e-13)
