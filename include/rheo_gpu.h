@@ -212,6 +212,20 @@ int rt_model_set_relax(rt_model* model, double factor);
  * with the reference's list (limiters.H:88-97). */
 int rt_model_set_scheme(rt_model* model, const char* divScheme);
 
+/* multiMode (constitutiveEqs/multiMode/multiMode.C): a model created from
+ * "type multiMode; models ( M1 { type ...; ... } M2 { ... } );" owns one
+ * model per entry, named name + keyword (:43-83).  Its setters (bc, flow,
+ * solver, relax, scheme) apply to every mode, correct() runs the modes in
+ * turn (:250-260), RT_TAU and rt_model_div_tau / rt_model_tau_total are the
+ * sums over the modes (:141-151, :219-227; etaS summed :80), and
+ * rt_model_solver_perf reports the worst mode per component.  The state of
+ * each mode (theta MUST_READ etc.) is set and read through its handle.
+ * rt_model_mode_count: the number of modes (0 for a single model).
+ * rt_model_mode: mode i, owned by the multiMode model (valid until it is
+ * destroyed; never pass it to rt_model_destroy). */
+int rt_model_mode_count(const rt_model* model);
+rt_model* rt_model_mode(rt_model* model, int i);
+
 /* ------------------------------------------- consumers of tau (momentum eq.)
  * The explicit stress term constitutiveEq::divTau hands to the momentum
  * equation, fvc::div(tau()/rho(), "div(tau)") with Gauss linear

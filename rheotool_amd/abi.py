@@ -109,6 +109,8 @@ def lib():
             "rt_model_set_solver": (i32, [vp, C.c_char_p]),
             "rt_model_set_relax": (i32, [vp, C.c_double]),
             "rt_model_set_scheme": (i32, [vp, C.c_char_p]),
+            "rt_model_mode_count": (i32, [vp]),
+            "rt_model_mode": (vp, [vp, i32]),
             "rt_model_div_tau": (i32, [vp, vp, vp]),
             "rt_model_tau_total": (i32, [vp, vp]),
             "rt_model_correct": (i32, [vp, C.c_double, vp]),
@@ -252,13 +254,23 @@ class Mesh:
 class ConstitutiveEq:
     """constitutiveEq selected at run time from a constitutiveProperties dictionary."""
 
-    def __init__(self, ctx: Context, name, mesh: Mesh, dict_text):
+    def __init__(self, ctx: Context, name, mesh: Mesh, dict_text, _handle=None, _owner=None):
         self.ctx, self.mesh = ctx, mesh
-        h = C.c_void_p()
-        ctx.check(lib().rt_model_create(ctx.h, name.encode(), mesh.h, dict_text.encode(),
-                                        C.byref(h)), "constitutiveEq::New")
-        self.h = h
+        self._owner = _owner  # a mode keeps its multiMode model alive and is not destroyed itself
+        if _handle is not None:
+            self.h = _handle
+        else:
+            h = C.c_void_p()
+            ctx.check(lib().rt_model_create(ctx.h, name.encode(), mesh.h, dict_text.encode(),
+                                            C.byref(h)), "constitutiveEq::New")
+            self.h = h
         self.N = mesh.nCells
+
+    def modes(self):
+        """The modes of a multiMode model (empty for a single model)."""
+        n = lib().rt_model_mode_count(self.h)
+        return [ConstitutiveEq(self.ctx, None, self.mesh, None, _handle=C.c_void_p(lib().rt_model_mode(self.h, i)),
+                               _owner=self) for i in range(n)]
 
     @property
     def type(self):
@@ -329,7 +341,8 @@ class ConstitutiveEq:
 
     def close(self):
         if getattr(self, "h", None):
-            lib().rt_model_destroy(self.h)
+            if self._owner is None:
+                lib().rt_model_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -612,6 +612,11 @@ __global__ void k_eigvals_to_tensor(int32_t N, const double* __restrict__ ev,
     t[5] = t[6] = t[7] = 0.0;
     t[8] = ev[2 * (int64_t)N + r];
 }
+__global__ void k_add(int64_t n, const double* __restrict__ in, double* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i < n) out[i] = out[i] + in[i];
+}
+
 __global__ void k_fill(double* p, int64_t n, double v) {
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (i < n) p[i] = v;
@@ -1070,6 +1075,13 @@ void launch_copy(Ctx* ctx, const void* src, void* dst, size_t bytes) {
     hipLaunchKernelGGL(k_copy, dim3(grid), dim3(BLK), 0, ctx->stream, n16, (const double2*)src, (double2*)dst);
     RT_HIP(hipGetLastError());
 }
+void launch_add(Ctx* ctx, int64_t n, const double* in, double* out) {
+    if (n <= 0) return;
+    ProfScope ps(ctx, "k_add", (double)n * 24);
+    hipLaunchKernelGGL(k_add, dim3((unsigned)((n + BLK - 1) / BLK)), dim3(BLK), 0, ctx->stream, n, in, out);
+    RT_HIP(hipGetLastError());
+}
+
 void launch_fill(Ctx* ctx, double* p, int64_t n, double v) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_fill, dim3(nblk(n)), dim3(BLK), 0, ctx->stream, p, n, v);

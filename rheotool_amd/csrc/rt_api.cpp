@@ -210,8 +210,8 @@ Controls parseControls(const char* text) {
 // ------------------------------------------------------------ model setup
 namespace {
 
-const char* kTypes[] = {"FENE-CRLog", "FENE-PLog", "GiesekusLog", "Oldroyd-BLog", "PTTLog",
-                        "Rolie-PolyLog", "WhiteMetznerCYLog", "XPomPomLog"};
+const char* kTypes[] = {"FENE-CRLog", "FENE-PLog",          "GiesekusLog", "Oldroyd-BLog", "PTTLog",
+                        "Rolie-PolyLog", "WhiteMetznerCYLog", "XPomPomLog", "multiMode"};
 
 double need(const Dict& d, const char* key, const std::string& dictName) {
     double v;
@@ -220,11 +220,7 @@ double need(const Dict& d, const char* key, const std::string& dictName) {
     return v;
 }
 
-void parseModel(Model& M, const char* text) {
-    if (!text) throw Error{RT_E_BAD_ARG, "rt_model_create: NULL dictionary"};
-    Dict full = parseDict(text);
-    const Dict& d = parametersOf(full);
-    const std::string dn = "constitutiveProperties.parameters";
+void parseModel(Model& M, const Dict& d, const std::string& dn) {
     std::string type;
     if (!d.word("type", type)) throw Error{RT_E_BAD_ARG, "keyword type is undefined in dictionary " + dn};
     ModelParams& p = M.prm;
@@ -669,14 +665,85 @@ int rt_model_create(rt_ctx* ctx, const char* name, const rt_mesh* mesh, const ch
     }
     auto* mo = new rt_model;
     int rc = guarded(&ctx->impl, [&] {
-        Model& M = mo->impl;
-        Ctx* c = &ctx->impl;
-        RT_HIP(hipSetDevice(c->device));
-        M.ctx = c;
-        M.mesh = const_cast<Mesh*>(&mesh->impl);
-        M.name = name ? name : "";
-        parseModel(M, dict);
-        const int32_t N = M.mesh->N, NB = M.mesh->NB;
+        if (!dict) throw Error{RT_E_BAD_ARG, "rt_model_create: NULL dictionary"};
+        RT_HIP(hipSetDevice(ctx->impl.device));
+        Dict full = parseDict(dict);
+        createModel(mo->impl, &ctx->impl, const_cast<Mesh*>(&mesh->impl), name ? name : "",
+                     parametersOf(full), "constitutiveProperties.parameters");
+    });
+    if (rc != RT_OK) {
+        destroyModes(mo->impl);
+        delete mo;
+        return rc;
+    }
+    ctxRef(&ctx->impl);
+    *out = mo;
+    return RT_OK;
+}
+
+void rt_model_destroy(rt_model* model) {
+    if (!model) return;
+    Ctx* c = model->impl.ctx;
+    destroyModes(model->impl);
+    delete model->impl.pendingSolve;
+    delete model;
+    if (c) ctxUnref(c);
+}
+
+int rt_model_mode_count(const rt_model* model) {
+    return model ? (int)model->impl.modes.size() : 0;
+}
+
+rt_model* rt_model_mode(rt_model* model, int i) {
+    if (!model || i < 0 || i >= (int)model->impl.modes.size()) return nullptr;
+    return model->impl.modes[i];
+}
+
+}  // extern "C"
+
+namespace rt {
+
+void destroyModes(Model& M) {
+    for (rt_model* x : M.modes) {
+        delete x->impl.pendingSolve;
+        delete x;
+    }
+    M.modes.clear();
+}
+
+// A model on ctx/mesh from its parameters dictionary.  multiMode
+// (multiMode.C:43-83): one model per entry of `models`, named name + keyword,
+// and only the tau sum held here.
+void createModel(Model& M, Ctx* c, Mesh* mesh, const std::string& name, const Dict& d, const std::string& dn) {
+    M.ctx = c;
+    M.mesh = mesh;
+    M.name = name;
+    const int32_t N = M.mesh->N, NB = M.mesh->NB;
+    std::string type;
+    if (d.word("type", type) && type == "multiMode") {
+        std::vector<std::pair<std::string, Dict>> list;
+        if (!d.dictList("models", list) || list.empty())
+            throw Error{RT_E_BAD_ARG, "keyword models is undefined in dictionary " + dn};
+        M.typeName = "multiMode";
+        double etaS = 0, rho = 0;
+        for (auto& e : list) {
+            auto* sub = new rt_model;
+            M.modes.push_back(sub);
+            createModel(sub->impl, c, mesh, name + e.first, e.second, dn + ".models." + e.first);
+            if (sub->impl.isMulti()) throw Error{RT_E_BAD_ARG, "multiMode: a mode cannot itself be multiMode"};
+            etaS += sub->impl.prm.etaS;  // multiMode.C:80
+            rho += sub->impl.prm.rho;
+        }
+        M.prm.etaS = etaS;
+        M.prm.rho = rho / list.size();  // multiMode::rho() averages
+        M.tau = c->alloc((size_t)6 * N * 8);
+        RT_HIP(hipMemsetAsync(M.tau.p, 0, M.tau.bytes, c->stream));
+        M.ctl = parseControls(nullptr);
+        RT_HIP(hipStreamSynchronize(c->stream));
+        return;
+    }
+    parseModel(M, d, dn);
+    {
         M.theta = c->alloc((size_t)6 * N * 8);
         M.tau = c->alloc((size_t)6 * N * 8);
         M.eigVal = c->alloc((size_t)3 * N * 8);
@@ -701,30 +768,38 @@ int rt_model_create(rt_ctx* ctx, const char* name, const rt_mesh* mesh, const ch
         for (int q : {0, 4, 8}) launch_fill(c, M.eigVec.as<double>() + (int64_t)q * N, N, 1.0);
         M.ctl = parseControls(nullptr);
         RT_HIP(hipStreamSynchronize(c->stream));
-    });
-    if (rc != RT_OK) {
-        delete mo;
-        return rc;
     }
-    ctxRef(&ctx->impl);
-    *out = mo;
-    return RT_OK;
 }
 
-void rt_model_destroy(rt_model* model) {
-    if (!model) return;
-    Ctx* c = model->impl.ctx;
-    delete model->impl.pendingSolve;
-    delete model;
-    if (c) ctxUnref(c);
-}
+}  // namespace rt
+
+extern "C" {
 
 const char* rt_model_type(const rt_model* model) { return model ? model->impl.typeName.c_str() : ""; }
+
+// multiMode forwards a setter to every mode (first failure wins)
+#define RT_FORWARD_TO_MODES(call)                                   \
+    if (model->impl.isMulti()) {                                    \
+        for (rt_model* mode_ : model->impl.modes) {                 \
+            rt_model* model = mode_;                                \
+            int rc_ = (call);                                       \
+            if (rc_ != RT_OK) {                                     \
+                mode_->impl.ctx->lastError = "mode " + mode_->impl.name + ": " + mode_->impl.ctx->lastError; \
+                return rc_;                                         \
+            }                                                       \
+        }                                                           \
+        return RT_OK;                                               \
+    }
 
 int rt_model_set_state(rt_model* model, const double* theta, const double* eigVals,
                        const double* eigVecs, const double* tau) {
     if (!model) return RT_E_BAD_ARG;
     Model& M = model->impl;
+    if (M.isMulti()) {
+        M.ctx->lastError = "rt_model_set_state: a multiMode model has no state of its own; set each mode's "
+                           "(rt_model_mode)";
+        return RT_E_BAD_ARG;
+    }
     return guarded(M.ctx, [&] {
         if (!theta) throw Error{RT_E_BAD_ARG, "rt_model_set_state: theta is required (MUST_READ)"};
         Ctx* c = M.ctx;
@@ -757,6 +832,7 @@ int rt_model_set_state(rt_model* model, const double* theta, const double* eigVa
 
 int rt_model_set_bc(rt_model* model, int patch, int bcType, const double* values) {
     if (!model) return RT_E_BAD_ARG;
+    RT_FORWARD_TO_MODES(rt_model_set_bc(model, patch, bcType, values));
     Model& M = model->impl;
     return guarded(M.ctx, [&] {
         Mesh& m = *M.mesh;
@@ -783,6 +859,7 @@ int rt_model_set_bc(rt_model* model, int patch, int bcType, const double* values
 
 int rt_model_set_flow(rt_model* model, const double* U, const double* Ub, const double* phi) {
     if (!model) return RT_E_BAD_ARG;
+    RT_FORWARD_TO_MODES(rt_model_set_flow(model, U, Ub, phi));
     Model& M = model->impl;
     return guarded(M.ctx, [&] {
         Mesh& m = *M.mesh;
@@ -809,18 +886,21 @@ int rt_model_set_flow(rt_model* model, const double* U, const double* Ub, const 
 
 int rt_model_set_solver(rt_model* model, const char* controls) {
     if (!model) return RT_E_BAD_ARG;
+    RT_FORWARD_TO_MODES(rt_model_set_solver(model, controls));
     Model& M = model->impl;
     return guarded(M.ctx, [&] { M.ctl = parseControls(controls); });
 }
 
 int rt_model_set_relax(rt_model* model, double factor) {
     if (!model) return RT_E_BAD_ARG;
+    RT_FORWARD_TO_MODES(rt_model_set_relax(model, factor));
     model->impl.relax = factor;
     return RT_OK;
 }
 
 int rt_model_set_scheme(rt_model* model, const char* divScheme) {
     if (!model) return RT_E_BAD_ARG;
+    RT_FORWARD_TO_MODES(rt_model_set_scheme(model, divScheme));
     Model& M = model->impl;
     std::vector<std::string> tok;
     {
@@ -868,6 +948,25 @@ int rt_model_set_scheme(rt_model* model, const char* divScheme) {
 int rt_model_correct(rt_model* model, double deltaT, const double* gradU) {
     if (!model) return RT_E_BAD_ARG;
     Model& M = model->impl;
+    if (M.isMulti()) {
+        // multiMode::correct (multiMode.C:250-260): each mode in turn, then
+        // tau = sum of the modes' tau (multiMode.C:219-227)
+        int worst = RT_OK;
+        for (rt_model* mode : M.modes) {
+            const int rc = rt_model_correct(mode, deltaT, gradU);
+            if (rc != RT_OK && rc != RT_W_NOT_CONVERGED) {
+                M.ctx->lastError = "mode " + mode->impl.name + ": " + mode->impl.ctx->lastError;
+                return rc;
+            }
+            if (rc == RT_W_NOT_CONVERGED) worst = rc;
+        }
+        const int rc = guarded(M.ctx, [&] {
+            const int64_t n = (int64_t)6 * M.mesh->N;
+            launch_fill(M.ctx, M.tau.as<double>(), n, 0.0);
+            for (rt_model* mode : M.modes) launch_add(M.ctx, n, mode->impl.tau.as<double>(), M.tau.as<double>());
+        });
+        return rc != RT_OK ? rc : worst;
+    }
     int rc = guarded(M.ctx, [&] {
         RT_HIP(hipSetDevice(M.ctx->device));
         modelCorrect(M, deltaT, gradU);
@@ -889,6 +988,9 @@ int rt_model_get(const rt_model* model, int field, double* out) {
         DBuf& st = staging(c).buf;
         int nc = field == RT_THETA || field == RT_TAU ? 6 : 9;
         if (field < RT_THETA || field > RT_EIGVECS) throw Error{RT_E_BAD_ARG, "rt_model_get: unknown field"};
+        if (M.isMulti() && field != RT_TAU)
+            throw Error{RT_E_BAD_ARG, "rt_model_get: a multiMode model holds only the summed tau; read the "
+                                      "modes' fields (rt_model_mode)"};
         const size_t bytes = (size_t)nc * N * 8;
         if (st.bytes < bytes) st = c->alloc(bytes);
         const int32_t* perm = m.permD.as<int32_t>();
@@ -904,6 +1006,18 @@ int rt_model_get(const rt_model* model, int field, double* out) {
 int rt_model_div_tau(rt_model* model, const double* tauB, double* out) {
     if (!model || !out) return RT_E_BAD_ARG;
     Model& M = model->impl;
+    if (M.isMulti()) {
+        // multiMode::divTau sums the modes' terms (multiMode.C:141-151)
+        const size_t n = (size_t)3 * M.mesh->N;
+        std::vector<double> part(n);
+        for (size_t k = 0; k < n; ++k) out[k] = 0.0;
+        for (rt_model* mode : M.modes) {
+            const int rc = rt_model_div_tau(mode, tauB, part.data());
+            if (rc != RT_OK) return rc;
+            for (size_t k = 0; k < n; ++k) out[k] = out[k] + part[k];
+        }
+        return RT_OK;
+    }
     return guarded(M.ctx, [&] {
         Ctx* c = M.ctx;
         Mesh& m = *M.mesh;
@@ -937,7 +1051,9 @@ int rt_model_tau_total(rt_model* model, double* out) {
         const int32_t N = m.N;
         RT_HIP(hipSetDevice(c->device));
         DBuf d = c->alloc((size_t)6 * N * 8);
-        launch_tau_total(c, meshArgs(M), M.tau.as<double>(), M.prm.etaS, d.as<double>(), grid_for(c, N));
+        // multiMode: sum of the modes' tau and of their etaS; U from the first mode
+        Model& F = M.isMulti() ? M.modes.front()->impl : M;
+        launch_tau_total(c, meshArgs(F), M.tau.as<double>(), M.prm.etaS, d.as<double>(), grid_for(c, N));
         DBuf& st = staging(c).buf;
         if (st.bytes < (size_t)6 * N * 8) st = c->alloc((size_t)6 * N * 8);
         launch_unpermute_rows(c, N, 6, 6, d.as<double>(), m.permD.as<int32_t>(), st.as<double>());
@@ -948,7 +1064,25 @@ int rt_model_tau_total(rt_model* model, double* out) {
 
 int rt_model_solver_perf(const rt_model* model, rt_solver_perf perf[6]) {
     if (!model || !perf) return RT_E_BAD_ARG;
-    for (int q = 0; q < 6; ++q) perf[q] = model->impl.perf[q];
+    const Model& M = model->impl;
+    if (M.isMulti()) {  // the worst mode per component: most iterations, largest residuals
+        for (int q = 0; q < 6; ++q) perf[q] = rt_solver_perf{0, 0, 0, 0, 1, 0};
+        for (const rt_model* mode : M.modes)
+            for (int q = 0; q < 6; ++q) {
+                const rt_solver_perf& s = mode->impl.perf[q];
+                rt_solver_perf& t = perf[q];
+                t.initialResidual = std::max(t.initialResidual, s.initialResidual);
+                t.finalResidual = std::max(t.finalResidual, s.finalResidual);
+                t.nIterations = std::max(t.nIterations, s.nIterations);
+                t.nRefinements = std::max(t.nRefinements, s.nRefinements);
+                t.converged = t.converged && (s.converged || !s.solved);
+                t.solved = t.solved || s.solved;
+            }
+        for (int q = 0; q < 6; ++q)
+            if (!perf[q].solved) perf[q].converged = 0;
+        return RT_OK;
+    }
+    for (int q = 0; q < 6; ++q) perf[q] = M.perf[q];
     return RT_OK;
 }
 

@@ -109,6 +109,25 @@ bool Dict::word(const std::string& k, std::string& out) const {
     return true;
 }
 
+bool Dict::dictList(const std::string& k, std::vector<std::pair<std::string, Dict>>& out) const {
+    auto it = entries.find(k);
+    if (it == entries.end()) return false;
+    const std::vector<std::string>& t = it->second;
+    out.clear();
+    size_t i = 0;
+    if (i >= t.size() || t[i] != "(") return false;
+    ++i;
+    while (i < t.size() && t[i] != ")") {
+        std::string name = t[i++];
+        if (i >= t.size() || t[i] != "{") return false;
+        ++i;
+        Dict sub;
+        parseInto(t, i, sub, false);  // consumes the closing '}'
+        out.emplace_back(name, std::move(sub));
+    }
+    return i < t.size();
+}
+
 std::vector<std::string> Dict::keys() const {
     std::vector<std::string> k;
     for (auto& e : entries) k.push_back(e.first);

@@ -20,6 +20,9 @@ struct Dict {
     // word: the first token of the entry
     bool word(const std::string& k, std::string& out) const;
     std::vector<std::string> keys() const;
+    // a list of named sub-dictionaries, `key ( name1 { ... } name2 { ... } );`
+    // (the `models` entry of multiMode); false when absent or malformed
+    bool dictList(const std::string& k, std::vector<std::pair<std::string, Dict>>& out) const;
 };
 
 // Throws rt::Error(RT_E_BAD_ARG) on malformed text.

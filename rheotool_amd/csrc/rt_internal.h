@@ -274,6 +274,10 @@ struct Model {
     Krylov kry;
     rt_solver_perf perf[6];
     struct SolveArgs* pendingSolve = nullptr;  // set between correct begin/advance
+    // multiMode (multiMode.C): the modes own their state; this model holds the
+    // sum of their tau and forwards the setters
+    std::vector<rt_model*> modes;
+    bool isMulti() const { return !modes.empty(); }
 };
 
 // ---------------------------------------------------------------- kernels
@@ -355,6 +359,8 @@ void launch_eigvals_from_tensor(Ctx* ctx, int32_t N, const double* aos9, const i
                                 double* eigVal);
 void launch_eigvals_to_tensor(Ctx* ctx, int32_t N, const double* eigVal, const int32_t* perm,
                               double* aos9);
+// out[i] += in[i], i < n
+void launch_add(Ctx* ctx, int64_t n, const double* in, double* out);
 void launch_fill(Ctx* ctx, double* p, int64_t n, double v);
 void launch_copy(Ctx* ctx, const void* src, void* dst, size_t bytes);
 
@@ -377,6 +383,10 @@ struct SolveArgs {
 };
 KryCtl kryCtl(const Controls& C, bool mixed);
 // mean of theta per component -> CompState.xbar (gAverage(psi) of the normalisation)
+struct Dict;
+// a model (or a multiMode and its modes) from its parameters dictionary
+void createModel(Model& M, Ctx* c, Mesh* mesh, const std::string& name, const Dict& d, const std::string& dn);
+void destroyModes(Model& M);
 void launch_mean6(Ctx* ctx, int32_t N, const double* x, double* partials,
                   CompState* cs, int grid);
 // Lockstep PBiCGStab over the valid components; fills perf.

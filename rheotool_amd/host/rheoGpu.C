@@ -223,6 +223,67 @@ static constitutiveEq::adddictionaryConstructorToTable<FENE_CRLog> addFENECR("FE
 static constitutiveEq::adddictionaryConstructorToTable<WhiteMetznerCYLog> addWMCY("WhiteMetznerCYLog");
 static constitutiveEq::adddictionaryConstructorToTable<RoliePolyLog> addRP("Rolie-PolyLog");
 static constitutiveEq::adddictionaryConstructorToTable<XPomPomLog> addXPP("XPomPomLog");
+static constitutiveEq::adddictionaryConstructorToTable<multiMode> addMulti("multiMode");
+
+multiMode::multiMode(const word& name, const volVectorField& U, const surfaceScalarField& phi,
+                     const dictionary& dict)
+    : constitutiveEq(name, U, phi) {
+    gpuContext& c = gpuContext::get();
+    c.check(rt_model_create(c.ptr(), name.c_str(), U.mesh->device(), dict.text().c_str(), &model_),
+            "constitutiveEq::New");
+    for (int i = 0; i < rt_model_mode_count(model_); ++i) modes_.push_back(rt_model_mode(model_, i));
+    // multiMode::rho() averages the modes' densities (multiMode.C:229-242)
+    std::vector<double> rhos;
+    std::regex re("rho\\s+(rho\\s+)?(\\[[^\\]]*\\]\\s*)?([-+0-9.eE]+)\\s*;");
+    for (std::sregex_iterator it(dict.text().begin(), dict.text().end(), re), e; it != e; ++it)
+        rhos.push_back(std::stod((*it)[3]));
+    if (!rhos.empty()) {
+        rho_ = 0;
+        for (double r : rhos) rho_ += r;
+        rho_ /= rhos.size();
+    }
+    volSymmTensorField th("theta", *U.mesh);
+    for (rt_model* m : modes_) c.check(rt_model_set_state(m, th.internal.data(), nullptr, nullptr, nullptr), "theta");
+}
+multiMode::~multiMode() {
+    if (model_) rt_model_destroy(model_);
+}
+void multiMode::setState(label mode, const volSymmTensorField& theta, const volTensorField* eigVals,
+                         const volTensorField* eigVecs) {
+    gpuContext::get().check(rt_model_set_state(modes_.at(mode), theta.internal.data(),
+                                               eigVals ? eigVals->internal.data() : nullptr,
+                                               eigVecs ? eigVecs->internal.data() : nullptr, nullptr),
+                            "multiMode::setState");
+}
+volSymmTensorField multiMode::tau() const {
+    volSymmTensorField f("tau" + name(), *U().mesh);
+    gpuContext::get().check(rt_model_get(model_, RT_TAU, f.internal.data()), "tau");
+    return f;
+}
+volSymmTensorField multiMode::theta(label mode) const {
+    volSymmTensorField f("theta" + name() + std::to_string(mode), *U().mesh);
+    gpuContext::get().check(rt_model_get(modes_.at(mode), RT_THETA, f.internal.data()), "theta");
+    return f;
+}
+volSymmTensorField multiMode::tau(label mode) const {
+    volSymmTensorField f("tau" + name(), *U().mesh);
+    gpuContext::get().check(rt_model_get(modes_.at(mode), RT_TAU, f.internal.data()), "tau");
+    return f;
+}
+void multiMode::correct(const volScalarField*, const volTensorField* gradU) {
+    gpuContext& c = gpuContext::get();
+    const fvMesh& mesh = *U().mesh;
+    c.check(rt_model_set_flow(model_, U().internal.data(), U().boundary.empty() ? nullptr : U().boundary.data(),
+                              phi().values.data()),
+            "multiMode::correct (U, phi)");
+    const dictionary sd = mesh.solverDict("theta" + name());
+    if (!sd.text().empty()) c.check(rt_model_set_solver(model_, sd.text().c_str()), "fvSolution theta");
+    std::string ds;
+    if (topEntry(mesh.fvSchemes.subDict("divSchemes").text(), "div(phi,theta)", ds))
+        c.check(rt_model_set_scheme(model_, ds.c_str()), "fvSchemes div(phi,theta)");
+    c.check(rt_model_correct(model_, mesh.time.deltaTValue(), gradU ? gradU->internal.data() : nullptr),
+            "multiMode::correct");
+}
 
 std::unique_ptr<constitutiveEq> constitutiveEq::New(const word& name, const volVectorField& U,
                                                     const surfaceScalarField& phi, const dictionary& dict) {

@@ -223,6 +223,29 @@ private:
     volTensorField getTensor(int which, const word& n) const;
 };
 
+//- multiMode (multiMode.C): one log model per entry of `models`; tau() is the
+//  sum over the modes, whose state is set and read per mode
+class multiMode : public constitutiveEq {
+public:
+    static const char* typeName_() { return "multiMode"; }
+    multiMode(const word& name, const volVectorField& U, const surfaceScalarField& phi, const dictionary& dict);
+    ~multiMode() override;
+    volSymmTensorField tau() const override;
+    scalar rho() const override { return rho_; }
+    bool isGNF() const override { return false; }
+    void correct(const volScalarField* alpha = nullptr, const volTensorField* gradU = nullptr) override;
+    label size() const { return (label)modes_.size(); }
+    void setState(label mode, const volSymmTensorField& theta, const volTensorField* eigVals,
+                  const volTensorField* eigVecs);
+    volSymmTensorField theta(label mode) const;
+    volSymmTensorField tau(label mode) const;
+
+private:
+    rt_model* model_ = nullptr;
+    std::vector<rt_model*> modes_;  // owned by model_
+    scalar rho_ = 1;
+};
+
 #define RHEOGPU_LOG_MODEL(Class, TypeNameStr)                                                   \
     class Class : public logConformationEq {                                                 \
     public:                                                                                  \

@@ -403,3 +403,50 @@ def test_further_log_models_explicit_step(ctx, model):
     d = np.abs(g["theta"] - o["theta"]).max() / np.abs(o["theta"]).max()
     assert d < 1e-12, (model, d)
     assert rel_l2(g["tau"], o["tau"]) < 1e-12
+
+
+def _mode_entry(name, model, **kw):
+    body = abi.model_dict(model, **kw).split("{", 1)[1].rsplit("}", 1)[0]
+    return f"{name} {{{body}}}"
+
+
+def test_multimode_matches_per_mode_oracle(ctx):
+    """multiMode (multiMode.C): two modes with their own state share the flow;
+    each mode's theta and tau match the oracle run on that mode, and the
+    model's tau, div(tau/rho) and tauTotal are the sums over the modes."""
+    import dataclasses
+    m = cases.cartesian(10, 9, 8, jitter=0.1, perm_seed=3)
+    f1 = cases.make_fields(m, cases.FIELDS_TG, seed=21, lnLo=-2.0, lnHi=2.0)
+    g2 = cases.make_fields(m, cases.FIELDS_TG, seed=22, lnLo=-2.0, lnHi=2.0)
+    f2 = dataclasses.replace(f1, theta=g2.theta, eigVals=g2.eigVals, eigVecs=g2.eigVecs)
+    spec = [("M1", "Oldroyd-BLog", dict(lam=0.8, etaS=0.1, etaP=0.3, rho=1.1)),
+            ("M2", "GiesekusLog", dict(lam=2.5, alpha=0.2, etaS=0.05, etaP=0.55, rho=1.1))]
+    text = ("parameters\n{\n    type multiMode;\n    models\n    (\n" +
+            "\n".join(_mode_entry(n, mdl, **kw) for n, mdl, kw in spec) + "\n    );\n}\n")
+    dm = ctx.mesh(m)
+    eq = ctx.constitutiveEq("", dm, text)
+    assert eq.type == "multiMode"
+    modes = eq.modes()
+    assert len(modes) == 2
+    for md, f in zip(modes, (f1, f2)):
+        md.set_state(f.theta, f.eigVals, f.eigVecs)
+    with pytest.raises(abi.RheoError):
+        eq.set_state(f1.theta)
+    eq.set_solver(abi.solver_dict(1e-10))
+    eq.set_flow(f1.U, f1.Ub, f1.phi)
+    assert eq.correct(2e-3) == abi.RT_OK
+    taus = []
+    for md, f, (n, mdl, kw) in zip(modes, (f1, f2), spec):
+        p = O.params(mdl, etaS=kw["etaS"], etaP=kw["etaP"], lam=kw["lam"], alpha=kw.get("alpha", 0.0),
+                     rho=kw["rho"])
+        th, ev, vec, tau, perf = O.correct(m, p, f, 2e-3)
+        assert rel_l2(md.theta(), th) < THETA_TOL
+        assert rel_l2(md.tau(), tau) < THETA_TOL
+        taus.append(md.tau())
+    assert rel_l2(eq.tau(), taus[0] + taus[1]) < 1e-15
+    want = O.div_tau(m, taus[0], None, 1.1) + O.div_tau(m, taus[1], None, 1.1)
+    assert rel_l2(eq.div_tau(), want) < 1e-13
+    want_t = O.tau_total(m, f1.U, f1.Ub, taus[0] + taus[1], 0.1 + 0.05)
+    assert rel_l2(eq.tau_total(), want_t) < 1e-13
+    perf = eq.solver_perf()
+    assert all(q.converged for q in perf if q.solved)
