@@ -246,6 +246,12 @@ int rt_model_tau_total(rt_model* model, double* out);
  * is ignored, as by every single-phase *Log model.  deltaT is
  * runTime.deltaT(). */
 int rt_model_correct(rt_model* model, double deltaT, const double* gradU);
+/* correct() as an inner iteration of a time step: rheoTestFoam calls
+ * correct() nInIter times per step (solvers/rheoTestFoam/rheoTestFoam.C:64-68),
+ * each solve's ddt using theta.oldTime() of the step while the explicit terms
+ * use the latest iterate.  iteration 0 starts the step, 1, 2, ... repeat it;
+ * a negative iteration is rt_model_correct. */
+int rt_model_correct_iter(rt_model* model, double deltaT, const double* gradU, int iteration);
 
 enum rt_field { RT_THETA = 0, RT_TAU = 1, RT_EIGVALS = 2, RT_EIGVECS = 3 };
 /* Copies a model field to the host: theta/tau [6N], eigVals/eigVecs [9N]. */

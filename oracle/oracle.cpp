@@ -676,7 +676,7 @@ void orc_rhs(const orc_params* p, int32_t n, const double* gradU, const double* 
 // fvm::ddt(theta) + fvm::div(phi, theta) == S (Oldroyd_BLog.C:141-164).
 // scheme: the GaussDefCmpw limiter; souT (NULL for upwind/none) the explicit
 // correction, entering the div matrix's source as -souT (:166).
-static void assembleImpl(const orc_mesh* m, const double* phi, const double* theta, const double* S,
+static void assembleImpl(const orc_mesh* m, const double* phi, const double* thetaOld, const double* S,
                          double deltaT, const int32_t* bcType, const double* bcValue, int scheme,
                          const double* souT, double* diag, double* lower, double* upper,
                          double* source, double* iC, double* bC) {
@@ -719,7 +719,7 @@ static void assembleImpl(const orc_mesh* m, const double* phi, const double* the
     // sources: ddt (rDeltaT*theta_old*V) [+ div: -souT] + (== S): V*S
     for (int c = 0; c < N; ++c)
         for (int q = 0; q < 6; ++q) {
-            const double sd = rDeltaT * theta[6 * (size_t)c + q] * m->V[c];
+            const double sd = rDeltaT * thetaOld[6 * (size_t)c + q] * m->V[c];  // theta.oldTime()
             const double vs = m->V[c] * S[6 * (size_t)c + q];
             source[6 * (size_t)c + q] = souT ? (sd + (-souT[6 * (size_t)c + q])) + vs : sd + vs;
         }
@@ -1082,6 +1082,15 @@ void orc_correct_scheme(const orc_mesh* m, const orc_params* p, const double* U,
                         const double* bcValue, const int32_t valid[6], const orc_controls* c,
                         double relaxFactor, int32_t scheme, const double* C, double* theta,
                         double* eigVals, double* eigVecs, double* tau, orc_perf perf[6]) {
+    orc_correct_iter(m, p, U, Ub, phi, gradU, deltaT, bcType, bcValue, valid, c, relaxFactor, scheme, C, nullptr,
+                     theta, eigVals, eigVecs, tau, perf);
+}
+
+void orc_correct_iter(const orc_mesh* m, const orc_params* p, const double* U, const double* Ub,
+                      const double* phi, const double* gradU, double deltaT, const int32_t* bcType,
+                      const double* bcValue, const int32_t valid[6], const orc_controls* c,
+                      double relaxFactor, int32_t scheme, const double* C, const double* thetaOld,
+                      double* theta, double* eigVals, double* eigVecs, double* tau, orc_perf perf[6]) {
     const int N = m->nCells, F = m->nInternal, NB = m->nBoundary;
     std::vector<double> L;
     if (!gradU) {
@@ -1113,7 +1122,8 @@ void orc_correct_scheme(const orc_mesh* m, const orc_params* p, const double* U,
         souT.resize(6 * (size_t)N);
         orc_defc_sout(m, phi, dc.data(), souT.data());
     }
-    assembleImpl(m, phi, theta, S.data(), deltaT, bcType, bcValue, scheme, souT.empty() ? nullptr : souT.data(),
+    assembleImpl(m, phi, thetaOld ? thetaOld : theta, S.data(), deltaT, bcType, bcValue, scheme,
+                 souT.empty() ? nullptr : souT.data(),
                  diag.data(), lower.data(), upper.data(), source.data(), iC.data(), bC.data());
     if (relaxFactor > 0)
         orc_relax(m, bcType, relaxFactor, theta, diag.data(), lower.data(), upper.data(),

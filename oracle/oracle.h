@@ -185,6 +185,17 @@ void orc_correct_scheme(const orc_mesh* m, const orc_params* p, const double* U,
                         const orc_controls* c, double relaxFactor, int32_t scheme, const double* C,
                         double* theta, double* eigVals, double* eigVecs, double* tau, orc_perf perf[6]);
 
+/* An inner iteration of a time step (rheoTestFoam.C:64-68 calls correct()
+ * nInIter times per step): the ddt source uses thetaOld, theta.oldTime() of
+ * the step, while the explicit terms use the current theta and eigen state.
+ * thetaOld NULL: theta itself (a new step). */
+void orc_correct_iter(const orc_mesh* m, const orc_params* p, const double* U, const double* Ub,
+                      const double* phi, const double* gradU, double deltaT, const int32_t* bcType,
+                      const double* bcValue, const int32_t validComponents[6],
+                      const orc_controls* c, double relaxFactor, int32_t scheme, const double* C,
+                      const double* thetaOld, double* theta, double* eigVals, double* eigVecs, double* tau,
+                      orc_perf perf[6]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -84,6 +84,8 @@ def lib():
                                   C.c_double, vp, vp, vp, vp, vp]
         L.orc_correct_scheme.argtypes = [vp, vp, vp, vp, vp, vp, C.c_double, vp, vp, vp, vp,
                                          C.c_double, C.c_int32, vp, vp, vp, vp, vp, vp]
+        L.orc_correct_iter.argtypes = [vp, vp, vp, vp, vp, vp, C.c_double, vp, vp, vp, vp,
+                                       C.c_double, C.c_int32, vp, vp, vp, vp, vp, vp, vp]
         L.orc_grad_gauss_symm.argtypes = [vp] * 4
         L.orc_defc_faces.argtypes = [vp, C.c_int32] + [vp] * 5
         L.orc_defc_sout.argtypes = [vp] * 4
@@ -258,9 +260,10 @@ def tau(p, eigVals, eigVecs, fOld=None):
 
 
 def correct(m, p, fields, dt, ctl=None, bc=None, bcval=None, gradU=None, relax=0.0,
-            use_grad=False, scheme="upwind"):
+            use_grad=False, scheme="upwind", theta_old=None):
     """One constitutiveEq::correct(); returns (theta, eigVals, eigVecs, tau, perf).
-    scheme: the GaussDefCmpw limiter of div(phi,theta)."""
+    scheme: the GaussDefCmpw limiter of div(phi,theta).  theta_old: the
+    step's theta.oldTime() for an inner iteration (None: a new step)."""
     om = OracleMesh(m)
     bc = default_bc(m) if bc is None else bc
     NB = m.nBoundary
@@ -274,8 +277,9 @@ def correct(m, p, fields, dt, ctl=None, bc=None, bcval=None, gradU=None, relax=0
     g = gradU if gradU is not None else (fields.gradU if use_grad else None)
     Ub = fields.Ub if NB else np.zeros((1, 3))
     Cc = np.ascontiguousarray(m.C)
-    lib().orc_correct_scheme(om.ref, C.byref(p), _p(fields.U), _p(Ub), _p(fields.phi),
-                             _p(None if g is None else np.ascontiguousarray(g)), dt, _p(bc),
-                             _p(bcval), _p(m.validComponents), C.byref(ctl), relax, SCHEMES[scheme],
-                             _p(Cc), _p(th), _p(ev), _p(vec), _p(t), perf)
+    to = None if theta_old is None else np.ascontiguousarray(theta_old, dtype=np.float64)
+    lib().orc_correct_iter(om.ref, C.byref(p), _p(fields.U), _p(Ub), _p(fields.phi),
+                           _p(None if g is None else np.ascontiguousarray(g)), dt, _p(bc),
+                           _p(bcval), _p(m.validComponents), C.byref(ctl), relax, SCHEMES[scheme],
+                           _p(Cc), _p(to), _p(th), _p(ev), _p(vec), _p(t), perf)
     return th, ev, vec, t, [perf[i] for i in range(6)]

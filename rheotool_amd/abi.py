@@ -114,6 +114,7 @@ def lib():
             "rt_model_div_tau": (i32, [vp, vp, vp]),
             "rt_model_tau_total": (i32, [vp, vp]),
             "rt_model_correct": (i32, [vp, C.c_double, vp]),
+            "rt_model_correct_iter": (i32, [vp, C.c_double, vp, i32]),
             "rt_model_get": (i32, [vp, i32, vp]),
             "rt_model_solver_perf": (i32, [vp, vp]),
             "rt_symm_solve": (i32, [vp, vp, vp, vp, C.c_char_p, vp]),
@@ -313,9 +314,13 @@ class ConstitutiveEq:
         self.ctx.check(lib().rt_model_tau_total(self.h, _p(out)), "tau_total")
         return out
 
-    def correct(self, deltaT, gradU=None):
+    def correct(self, deltaT, gradU=None, iteration=None):
+        """One step; iteration 0, 1, ... makes it an inner iteration of a step
+        (the ddt keeps theta.oldTime() of iteration 0)."""
         g = None if gradU is None else _f64(gradU, (self.N, 3, 3))
-        return self.ctx.check(lib().rt_model_correct(self.h, deltaT, _p(g)), "correct")
+        if iteration is None:
+            return self.ctx.check(lib().rt_model_correct(self.h, deltaT, _p(g)), "correct")
+        return self.ctx.check(lib().rt_model_correct_iter(self.h, deltaT, _p(g), int(iteration)), "correct")
 
     def _get(self, which, shape):
         out = np.empty(shape)

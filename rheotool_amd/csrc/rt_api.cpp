@@ -365,7 +365,7 @@ AssembleArgs meshArgs(Model& M) {
     return a;
 }
 
-void modelCorrectBegin(Model& M, double deltaT, const double* gradU) {
+void modelCorrectBegin(Model& M, double deltaT, const double* gradU, int iteration) {
     if (!M.hasState) throw Error{RT_E_BAD_ARG, "correct(): theta was never set (rt_model_set_state)"};
     if (!M.hasFlow) throw Error{RT_E_BAD_ARG, "correct(): U/phi were never set (rt_model_set_flow)"};
     if (!(deltaT > 0)) throw Error{RT_E_BAD_ARG, "correct(): deltaT must be positive"};
@@ -381,6 +381,19 @@ void modelCorrectBegin(Model& M, double deltaT, const double* gradU) {
         RT_HIP(hipMemsetAsync(M.b.p, 0, (size_t)6 * N * 8, ctx->stream));
     }
     if (modelKeepsFactor(M.prm.kind) && !M.fOld.p) M.fOld = ctx->alloc((size_t)N * 8);
+    // inner iterations of a step keep theta.oldTime() of the step's first one
+    const double* thetaOld = nullptr;
+    if (iteration > 0) {
+        if (!M.iterating)
+            throw Error{RT_E_BAD_ARG, "correct(): an inner iteration needs a preceding iteration 0 of its step"};
+        thetaOld = M.thetaOld.as<double>();
+    } else if (iteration == 0) {
+        if (!M.thetaOld.p) M.thetaOld = ctx->alloc((size_t)6 * N * 8);
+        launch_copy(ctx, M.theta.p, M.thetaOld.p, (size_t)6 * N * 8);
+        M.iterating = true;
+    } else {
+        M.iterating = false;  // a plain step
+    }
     if (gradU) {
         if (!M.gradU.p) M.gradU = ctx->alloc((size_t)9 * N * 8);
         DBuf& st = staging(ctx).buf;
@@ -394,6 +407,7 @@ void modelCorrectBegin(Model& M, double deltaT, const double* gradU) {
                  M.kry.grid);
     AssembleArgs a = meshArgs(M);
     a.gradU = gradU ? M.gradU.as<double>() : nullptr;
+    a.thetaOld = thetaOld;
     a.theta = M.theta.as<double>();
     a.eigVal = M.eigVal.as<double>();
     a.eigVec = M.eigVec.as<double>();
@@ -412,6 +426,7 @@ void modelCorrectBegin(Model& M, double deltaT, const double* gradU) {
     for (int q = 0; q < 6; ++q) a.valid[q] = m.valid[q];
     a.r = mixed ? M.kry.rs.p : M.kry.r.p;
     a.r0 = mixed ? M.kry.r0s.p : M.kry.r0.p;
+    a.p = mixed ? M.kry.ps.p : M.kry.p.p;
     a.rD = M.kry.rD.as<double>();
     a.ctl = kryCtl(M.ctl, mixed);
     a.mixed = mixed ? 1 : 0;
@@ -451,8 +466,8 @@ bool modelCorrectAdvance(Model& M, bool block) {
     return true;
 }
 
-void modelCorrect(Model& M, double deltaT, const double* gradU) {
-    modelCorrectBegin(M, deltaT, gradU);
+void modelCorrect(Model& M, double deltaT, const double* gradU, int iteration) {
+    modelCorrectBegin(M, deltaT, gradU, iteration);
     while (!modelCorrectAdvance(M, true)) {
     }
     RT_HIP(hipStreamSynchronize(M.ctx->stream));
@@ -946,6 +961,10 @@ int rt_model_set_scheme(rt_model* model, const char* divScheme) {
 }
 
 int rt_model_correct(rt_model* model, double deltaT, const double* gradU) {
+    return rt_model_correct_iter(model, deltaT, gradU, -1);
+}
+
+int rt_model_correct_iter(rt_model* model, double deltaT, const double* gradU, int iteration) {
     if (!model) return RT_E_BAD_ARG;
     Model& M = model->impl;
     if (M.isMulti()) {
@@ -953,7 +972,7 @@ int rt_model_correct(rt_model* model, double deltaT, const double* gradU) {
         // tau = sum of the modes' tau (multiMode.C:219-227)
         int worst = RT_OK;
         for (rt_model* mode : M.modes) {
-            const int rc = rt_model_correct(mode, deltaT, gradU);
+            const int rc = rt_model_correct_iter(mode, deltaT, gradU, iteration);
             if (rc != RT_OK && rc != RT_W_NOT_CONVERGED) {
                 M.ctx->lastError = "mode " + mode->impl.name + ": " + mode->impl.ctx->lastError;
                 return rc;
@@ -969,7 +988,7 @@ int rt_model_correct(rt_model* model, double deltaT, const double* gradU) {
     }
     int rc = guarded(M.ctx, [&] {
         RT_HIP(hipSetDevice(M.ctx->device));
-        modelCorrect(M, deltaT, gradU);
+        modelCorrect(M, deltaT, gradU, iteration);
     });
     if (rc != RT_OK) return rc;
     for (int q = 0; q < 6; ++q)

@@ -185,7 +185,8 @@ Controls parseControls(const char* text);
 struct alignas(64) CompState {
     double rho, rhoOld, alpha, omega, beta;
     double normFactor, initRes, finalRes, xbar;
-    double tmp0, tmp1, tmp2;
+    double sigma;       // (r0, v) of the current iteration
+    double tmp1, tmp2;
     int32_t iter, active, converged, solved;
     int32_t exitAtS, needRefine, nRefine, innerIter;
     double innerTarget;  // inner (FP32) residual target per sweep
@@ -264,6 +265,8 @@ struct Model {
     DBuf gradU;             // [9][N] when supplied
     DBuf diag, aval, b, fOld;  // matrix (column order) / sources [N][6] of the step
     DBuf slotMask;             // uint32 [nSlices] live slots of aval
+    DBuf thetaOld;             // [N][6] theta at the start of the step (inner iterations)
+    bool iterating = false;    // thetaOld holds the current step's oldTime
     int scheme = 0;            // DivScheme of div(phi,theta)
     DBuf gradTh;               // double [N][18] component gradients (limited schemes)
     DBuf souT;                 // double [N][6] explicit correction source
@@ -305,6 +308,7 @@ struct AssembleArgs {
     const double* Ub;     // [3][NB]
     const double* gradU;  // [9][N], nullable
     const double* theta;  // [N][6]
+    const double* thetaOld;  // [N][6] theta.oldTime() for inner iterations, NULL: theta
     const double* eigVal;
     const double* eigVec;
     double rDeltaT;
@@ -325,6 +329,7 @@ struct AssembleArgs {
     // initial residual of the solve, fused into the assembly
     void* r;              // [N][6] Krylov precision
     void* r0;
+    void* p;
     double* rD;           // [N] 1/diag
     KryCtl ctl;
     int32_t mixed;        // r/r0 in FP32
@@ -404,9 +409,12 @@ int grid_for(const Ctx* ctx, int32_t N);
 // correct() as a resumable job (the serving loop interleaves several):
 // begin launches the assembly and the initial residual; advance returns true
 // once the solve has finished and the eigen/tau kernel has been launched.
-void modelCorrectBegin(Model& M, double deltaT, const double* gradU);
+// iteration < 0: a plain step (theta.oldTime() is theta); 0 starts a step with
+// inner iterations (saves theta.oldTime()); > 0 repeats the step from the
+// current iterate with the saved oldTime
+void modelCorrectBegin(Model& M, double deltaT, const double* gradU, int iteration = -1);
 bool modelCorrectAdvance(Model& M, bool block);
-void modelCorrect(Model& M, double deltaT, const double* gradU);
+void modelCorrect(Model& M, double deltaT, const double* gradU, int iteration = -1);
 // device bytes a model of this size allocates (for admission control)
 size_t modelBytesEstimate(int32_t N, int32_t F, int32_t NB, int64_t nEll, bool mixed);
 

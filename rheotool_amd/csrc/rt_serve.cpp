@@ -88,6 +88,10 @@ void admit(rt_server* S, Request* r, int slot) {
     S->slotReq[slot] = r;
     S->inflightBytes += r->estBytes;
     int rc = rt_model_create(c, "", r->mesh, r->dict.c_str(), &r->model);
+    if (rc == RT_OK && rt_model_mode_count(r->model) > 0) {
+        c->impl.lastError = "rt_server: multiMode requests are not supported (submit one request per mode)";
+        rc = RT_E_UNSUPPORTED;
+    }
     if (rc == RT_OK) rc = rt_model_set_solver(r->model, r->controls.c_str());
     if (rc == RT_OK) rc = rt_model_set_state(r->model, r->theta, r->eigVals, r->eigVecs, nullptr);
     if (rc == RT_OK) rc = rt_model_set_flow(r->model, r->U, r->Ub, r->phi);

@@ -147,6 +147,37 @@ def test_prescribed_grad_u(ctx):
     check_step(g, o, m.validComponents)
 
 
+@pytest.mark.parametrize("scheme", ["upwind", "cubista"])
+def test_inner_iterations_keep_old_time(ctx, scheme):
+    """Three inner iterations of one step (rheoTestFoam.C:64-70, nInIter): the
+    ddt keeps theta.oldTime() of iteration 0; each iteration matches the
+    oracle given the same oldTime."""
+    import dataclasses
+    m = cases.cartesian(12, 10, 8, jitter=0.1)
+    f = cases.make_fields(m, cases.FIELDS_TG, seed=4, lnLo=-2, lnHi=2)
+    print(f"[case] inner iterations N={m.nCells} scheme={scheme}", flush=True)
+    dm = ctx.mesh(m)
+    eq = ctx.constitutiveEq("", dm, abi.model_dict("GiesekusLog", lam=0.5, alpha=0.2))
+    eq.set_solver(abi.solver_dict(1e-10))
+    if scheme != "upwind":
+        eq.set_scheme("GaussDefCmpw " + scheme)
+    eq.set_state(f.theta, f.eigVals, f.eigVecs)
+    eq.set_flow(f.U, f.Ub, f.phi)
+    p = oracle_params("GiesekusLog", 0.5, dict(alpha=0.2))
+    cur, seq = f, []
+    for it in range(3):
+        assert eq.correct(5e-3, iteration=it) == abi.RT_OK
+        th, ev, vec, tau, _ = O.correct(m, p, cur, 5e-3, scheme=scheme,
+                                        theta_old=f.theta if it else None)
+        g = eq.theta()
+        assert rel_l2(g, th) < THETA_TOL and rel_l2(eq.tau(), tau) < THETA_TOL
+        seq.append((cur, g))
+        cur = dataclasses.replace(cur, theta=th, eigVals=ev, eigVecs=vec)
+    # iteration 1 differs from a new step started at iteration 0's result
+    plain = O.correct(m, p, seq[1][0], 5e-3, scheme=scheme)[0]
+    assert rel_l2(seq[1][1], plain) > 1e-4
+
+
 def test_explicit_step_exact(ctx):
     """phi = 0: the matrix is diagonal, theta_new = theta_old + dt S; the GPU's
     per-cell algebra follows the reference's operation order, so the result
