@@ -209,7 +209,12 @@ int rt_model_set_relax(rt_model* model, double factor);
  * limited schemes add the explicit deferred correction of
  * gaussDefCmpwConvectionScheme.C:120-169 and need the mesh's cell centres;
  * none drops the convection term (:90).  Unknown names -> RT_E_UNKNOWN_TYPE
- * with the reference's list (limiters.H:88-97). */
+ * with the reference's list (limiters.H:88-97).  A leading "bounded"
+ * (e.g. tutorials/rheoFilmFoam/UCM/system/fvSchemes:35
+ * "bounded GaussDefCmpw cubista") applies OpenFOAM's bounded
+ * convection wrapper: the matrix diagonal loses the cell's net outflux
+ * (fvm::Sp(fvc::surfaceIntegrate(phi), theta)).  "Gauss upwind" is accepted
+ * as the upwind scheme (the same matrix). */
 int rt_model_set_scheme(rt_model* model, const char* divScheme);
 
 /* multiMode (constitutiveEqs/multiMode/multiMode.C): a model created from

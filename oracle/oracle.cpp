@@ -682,6 +682,8 @@ static void assembleImpl(const orc_mesh* m, const double* phi, const double* the
                          double* source, double* iC, double* bC) {
     const int N = m->nCells, F = m->nInternal;
     const double rDeltaT = 1.0 / deltaT;
+    const bool bounded = (scheme & ORC_DC_BOUNDED) != 0;
+    scheme &= ~ORC_DC_BOUNDED;
     const bool conv = scheme != ORC_DC_NONE;  // "none": fvm.diag() = 0, nothing else (:90)
     // fvm::ddt Euler
     std::vector<double> ddtDiag(N), divDiag(N, 0.0);
@@ -695,6 +697,24 @@ static void assembleImpl(const orc_mesh* m, const double* phi, const double* the
     for (int f = 0; f < F; ++f) {  // negSumDiag
         divDiag[m->owner[f]] -= lower[f];
         divDiag[m->neighbour[f]] -= upper[f];
+    }
+    if (bounded) {
+        // [OF9] boundedConvectionScheme::fvmDiv: scheme - fvm::Sp(surfaceIntegrate(phi)):
+        // surfaceIntegrate sums phi out of each cell (internal faces in face
+        // order, then the non-empty patches) over V; Sp's diagonal is V*sp
+        std::vector<double> net(N, 0.0);
+        for (int f = 0; f < F; ++f) {
+            net[m->owner[f]] += phi[f];
+            net[m->neighbour[f]] -= phi[f];
+        }
+        for (int p = 0; p < m->nPatches; ++p) {
+            if (m->patchEmpty[p]) continue;
+            for (int k = 0; k < m->patchSize[p]; ++k) {
+                const int b = m->patchStart[p] + k;
+                net[m->owner[(int64_t)F + b]] += phi[(int64_t)F + b];
+            }
+        }
+        for (int c = 0; c < N; ++c) divDiag[c] -= m->V[c] * (net[c] / m->V[c]);
     }
     for (int c = 0; c < N; ++c) diag[c] = ddtDiag[c] + divDiag[c];
     // patch coefficients with plim = 1 (:102-116)
@@ -1104,7 +1124,7 @@ void orc_correct_iter(const orc_mesh* m, const orc_params* p, const double* U, c
         bC(6 * (size_t)NB);
     std::vector<double> souT;
     double al[3], be[3], bl[2];
-    if (limiterTable(scheme, al, be, bl)) {
+    if (limiterTable(scheme & ~ORC_DC_BOUNDED, al, be, bl)) {
         // explicit correction from the old theta: patch values (zeroGradient:
         // the cell value; fixedValue: the given value), Gauss gradients, face
         // corrections, souT
@@ -1118,7 +1138,7 @@ void orc_correct_iter(const orc_mesh* m, const orc_params* p, const double* U, c
                                                                               : theta[6 * (size_t)cell + q];
             }
         orc_grad_gauss_symm(m, theta, thB.data(), g.data());
-        orc_defc_faces(m, scheme, phi, theta, g.data(), C, dc.data());
+        orc_defc_faces(m, scheme & ~ORC_DC_BOUNDED, phi, theta, g.data(), C, dc.data());
         souT.resize(6 * (size_t)N);
         orc_defc_sout(m, phi, dc.data(), souT.data());
     }

@@ -99,6 +99,10 @@ void orc_assemble(const orc_mesh* m, const double* phi, const double* theta, con
  * limiters.H:38-100). */
 enum { ORC_DC_UPWIND = 0, ORC_DC_CUBISTA = 1, ORC_DC_MINMOD = 2, ORC_DC_SMART = 3, ORC_DC_WACEB = 4,
        ORC_DC_SUPERBEE = 5, ORC_DC_NONE = 6 };
+/* Or-ed into a scheme: OpenFOAM's `bounded` convection wrapper, which
+ * subtracts fvm::Sp(fvc::surfaceIntegrate(phi), theta) from the scheme's
+ * matrix ([OF9] boundedConvectionScheme.C fvmDiv). */
+enum { ORC_DC_BOUNDED = 16 };
 
 /* [OF9] fvc::grad of each symmTensor component, Gauss linear, as phifDefC
  * takes it (gaussDefCmpwConvectionScheme.C:243-250): grad[18*N], per cell the

@@ -147,6 +147,15 @@ def grad(m, U, Ub):
 
 # GaussDefCmpw limited schemes (limiters.H:38-100)
 SCHEMES = {"upwind": 0, "cubista": 1, "minmod": 2, "smart": 3, "waceb": 4, "superbee": 5, "none": 6}
+BOUNDED = 16  # ORC_DC_BOUNDED
+
+
+def scheme_id(scheme):
+    """'cubista' or 'bounded cubista' -> the ORC_DC_* value."""
+    w = scheme.split()
+    if w[0] == "bounded":
+        return BOUNDED | SCHEMES[w[1]]
+    return SCHEMES[w[0]]
 
 
 def grad_symm(m, theta, thetaB):
@@ -262,7 +271,8 @@ def tau(p, eigVals, eigVecs, fOld=None):
 def correct(m, p, fields, dt, ctl=None, bc=None, bcval=None, gradU=None, relax=0.0,
             use_grad=False, scheme="upwind", theta_old=None):
     """One constitutiveEq::correct(); returns (theta, eigVals, eigVecs, tau, perf).
-    scheme: the GaussDefCmpw limiter of div(phi,theta).  theta_old: the
+    scheme: the GaussDefCmpw limiter of div(phi,theta), optionally prefixed by
+    "bounded " (OpenFOAM's bounded convection wrapper).  theta_old: the
     step's theta.oldTime() for an inner iteration (None: a new step)."""
     om = OracleMesh(m)
     bc = default_bc(m) if bc is None else bc
@@ -280,6 +290,6 @@ def correct(m, p, fields, dt, ctl=None, bc=None, bcval=None, gradU=None, relax=0
     to = None if theta_old is None else np.ascontiguousarray(theta_old, dtype=np.float64)
     lib().orc_correct_iter(om.ref, C.byref(p), _p(fields.U), _p(Ub), _p(fields.phi),
                            _p(None if g is None else np.ascontiguousarray(g)), dt, _p(bc),
-                           _p(bcval), _p(m.validComponents), C.byref(ctl), relax, SCHEMES[scheme],
+                           _p(bcval), _p(m.validComponents), C.byref(ctl), relax, scheme_id(scheme),
                            _p(Cc), _p(to), _p(th), _p(ev), _p(vec), _p(t), perf)
     return th, ev, vec, t, [perf[i] for i in range(6)]

@@ -36,6 +36,10 @@ def lib():
                                                                                C.c_uint64]
         L.rtc_mesh_random_graph.restype = C.c_void_p
         L.rtc_mesh_random_graph.argtypes = [C.c_int, C.c_int, C.c_double, C.c_uint64]
+        L.rtc_mesh_from_poly.restype = C.c_void_p
+        L.rtc_mesh_from_poly.argtypes = [C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_int32,
+                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_char_p]
         L.rtc_mesh_free.argtypes = [C.c_void_p]
         L.rtc_mesh_sizes.argtypes = [C.c_void_p, C.c_void_p]
         L.rtc_mesh_export.argtypes = [C.c_void_p] + [C.c_void_p] * 11
@@ -139,6 +143,28 @@ def cartesian(nx, ny, nz, L=(1.0, 1.0, 1.0), twoD=False, channel=False, jitter=0
                                  float(jitter), jitter_seed, perm_seed)
     if not h:
         raise ValueError("invalid Cartesian mesh request")
+    return _export(h)
+
+
+def from_poly(points, faces, owner, neighbour, nCells, patches) -> Mesh:
+    """A Mesh from polyMesh data: points [nPoints, 3], faces (list of point
+    label lists), owner [nFaces], neighbour [nInternal], and patches as
+    (name, kind, start, size) with start counted from the first boundary face."""
+    pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    fs = np.zeros(len(faces) + 1, np.int32)
+    fs[1:] = np.cumsum([len(f) for f in faces])
+    fl = np.ascontiguousarray(np.concatenate([np.asarray(f, np.int32) for f in faces]) if faces
+                              else np.zeros(0, np.int32))
+    own = np.ascontiguousarray(owner, np.int32)
+    nbr = np.ascontiguousarray(neighbour, np.int32)
+    ps = np.array([p[2] for p in patches], np.int32)
+    pz = np.array([p[3] for p in patches], np.int32)
+    pk = np.array([p[1] for p in patches], np.int32)
+    names = b"".join(p[0].encode() + b"\0" for p in patches)
+    h = lib().rtc_mesh_from_poly(len(pts), _p(pts), len(faces), _p(fs), _p(fl), _p(own), len(nbr),
+                                 _p(nbr), int(nCells), len(patches), _p(ps), _p(pz), _p(pk), names)
+    if not h:
+        raise ValueError("inconsistent polyMesh description")
     return _export(h)
 
 

@@ -458,6 +458,62 @@ rtc_mesh* rtc_mesh_random_graph(int nCells, int kNear, double sigma, uint64_t se
     return m;
 }
 
+rtc_mesh* rtc_mesh_from_poly(int32_t nPoints, const double* points, int32_t nFaces,
+                             const int32_t* faceStart, const int32_t* faceLabels,
+                             const int32_t* owner, int32_t nInternal, const int32_t* neighbour,
+                             int32_t nCells, int32_t nPatches, const int32_t* patchStart,
+                             const int32_t* patchSize, const int32_t* patchKind,
+                             const char* patchNames) {
+    if (nPoints < 1 || nCells < 1 || nInternal < 0 || nFaces < nInternal || nPatches < 0 || !points ||
+        !faceStart || !faceLabels || !owner || (nInternal && !neighbour))
+        return nullptr;
+    // patches tile [nInternal, nFaces)
+    int32_t next = 0;
+    for (int32_t p = 0; p < nPatches; ++p) {
+        if (patchStart[p] != next || patchSize[p] < 0) return nullptr;
+        next += patchSize[p];
+    }
+    if (next != nFaces - nInternal) return nullptr;
+    for (int32_t f = 0; f < nFaces; ++f) {
+        if (faceStart[f + 1] - faceStart[f] < 3 || owner[f] < 0 || owner[f] >= nCells) return nullptr;
+        if (f < nInternal && (neighbour[f] <= owner[f] || neighbour[f] >= nCells)) return nullptr;
+        for (int32_t k = faceStart[f]; k < faceStart[f + 1]; ++k)
+            if (faceLabels[k] < 0 || faceLabels[k] >= nPoints) return nullptr;
+    }
+    auto* m = new rtc_mesh;
+    m->nCells = nCells;
+    m->nInternal = nInternal;
+    m->nBoundary = nFaces - nInternal;
+    m->owner.assign(owner, owner + nFaces);
+    m->neighbour.assign(neighbour, neighbour + nInternal);
+    m->Sf.resize(3 * (size_t)nFaces);
+    m->Cf.resize(3 * (size_t)nFaces);
+    std::vector<V3> pts;
+    for (int32_t f = 0; f < nFaces; ++f) {
+        pts.clear();
+        for (int32_t k = faceStart[f]; k < faceStart[f + 1]; ++k) {
+            const double* x = points + 3 * (size_t)faceLabels[k];
+            pts.push_back(V3{x[0], x[1], x[2]});
+        }
+        V3 cf, sf;
+        faceGeom(pts.data(), (int)pts.size(), cf, sf);
+        m->Cf[3 * f] = cf.x, m->Cf[3 * f + 1] = cf.y, m->Cf[3 * f + 2] = cf.z;
+        m->Sf[3 * f] = sf.x, m->Sf[3 * f + 1] = sf.y, m->Sf[3 * f + 2] = sf.z;
+    }
+    const char* nm = patchNames;
+    for (int32_t p = 0; p < nPatches; ++p) {
+        m->pStart.push_back(patchStart[p]);
+        m->pSize.push_back(patchSize[p]);
+        m->pKind.push_back(patchKind[p]);
+        m->pName.push_back(nm ? std::string(nm) : "patch" + std::to_string(p));
+        if (nm) nm += std::strlen(nm) + 1;
+    }
+    m->origId.resize(nCells);
+    for (int32_t c = 0; c < nCells; ++c) m->origId[c] = c;
+    finishGeometry(*m);
+    return m;
+}
+
 void rtc_mesh_free(rtc_mesh* m) { delete m; }
 
 void rtc_mesh_sizes(const rtc_mesh* m, int32_t out[4]) {

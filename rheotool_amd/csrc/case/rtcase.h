@@ -55,6 +55,22 @@ rtc_mesh* rtc_mesh_cartesian(int nx, int ny, int nz, double Lx, double Ly, doubl
  * No boundary faces. */
 rtc_mesh* rtc_mesh_random_graph(int nCells, int kNear, double sigma, uint64_t seed);
 
+/* A mesh from OpenFOAM polyMesh data (constant/polyMesh points, faces,
+ * owner, neighbour, boundary).  Face f has the points
+ * faceLabels[faceStart[f] .. faceStart[f+1]); faces [0, nInternal) are
+ * internal (neighbour[f] given), the rest boundary faces in patch order with
+ * patch p covering [patchStart[p], patchStart[p] + patchSize[p]) counted from
+ * face nInternal.  patchNames holds nPatches NUL-terminated names back to
+ * back.  Geometry as the conventions above.  Returns NULL on an inconsistent
+ * description (an index out of range, owner >= neighbour, a face with fewer
+ * than 3 points, patches that do not tile the boundary faces). */
+rtc_mesh* rtc_mesh_from_poly(int32_t nPoints, const double* points, int32_t nFaces,
+                             const int32_t* faceStart, const int32_t* faceLabels,
+                             const int32_t* owner, int32_t nInternal, const int32_t* neighbour,
+                             int32_t nCells, int32_t nPatches, const int32_t* patchStart,
+                             const int32_t* patchSize, const int32_t* patchKind,
+                             const char* patchNames);
+
 void rtc_mesh_free(rtc_mesh* m);
 
 /* out[0..3] = nCells, nInternalFaces, nBoundaryFaces, nPatches */

@@ -267,7 +267,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3, 8))
 #pragma unroll
         for (int k = 0; k < STAGE_W; ++k) av[k] = 0.0;
         double offY[6] = {0, 0, 0, 0, 0, 0};
-        double divDiag = 0.0, sumOff = 0.0, sumAv = 0.0;
+        double divDiag = 0.0, sumOff = 0.0, sumAv = 0.0, netPhi = 0.0;
         const bool haveGrad = a.gradU != nullptr;
         for (int k = 0; k < width; ++k) {
             const int64_t e = base + (int64_t)SLICE * k;  // face-order entry
@@ -279,7 +279,9 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3, 8))
             }
             const bool own = fid >= 0;
             const int64_t ec = base + (int64_t)SLICE * slot;  // column-order slot
-            const double ph = a.noConv ? 0.0 : a.ePhi[e];
+            const double phRaw = a.ePhi[e];
+            netPhi += own ? phRaw : -phRaw;
+            const double ph = a.noConv ? 0.0 : phRaw;
             const double upw = ph >= 0 ? 1.0 : 0.0;
             const double lower = -upw * ph;
             const double upper = (1.0 - upw) * ph;
@@ -371,6 +373,15 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3, 8))
         }
         if (a.fOld) a.fOld[r] = f;
         // matrix diagonal: ddt + div, optional relax, then boundary internalCoeffs
+        if (a.bounded) {
+            // bounded wrapper: div - fvm::Sp(surfaceIntegrate(phi)); the net
+            // outflux adds the non-empty patch faces after the internal ones
+            for (int32_t qb = b0; qb < b1; ++qb) {
+                const int32_t bf = a.bFace[qb];
+                if (a.bcType[a.bPatch[bf]] != RT_BC_EMPTY) netPhi += a.phiB[bf];
+            }
+            divDiag -= vol * (netPhi / vol);
+        }
         double D = a.rDeltaT * vol + divDiag;
         double src[6];
         double tho[6];  // theta.oldTime() of the step: theta unless this is an inner iteration

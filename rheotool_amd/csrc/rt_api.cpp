@@ -431,6 +431,7 @@ void modelCorrectBegin(Model& M, double deltaT, const double* gradU, int iterati
     a.ctl = kryCtl(M.ctl, mixed);
     a.mixed = mixed ? 1 : 0;
     a.noConv = M.scheme == DS_NONE ? 1 : 0;
+    a.bounded = M.bounded ? 1 : 0;
     a.souT = nullptr;
     if (M.scheme != DS_UPWIND && M.scheme != DS_NONE) {
         if (!M.gradTh.p) M.gradTh = ctx->alloc((size_t)18 * N * 8);
@@ -930,9 +931,14 @@ int rt_model_set_scheme(rt_model* model, const char* divScheme) {
         }
         if (!cur.empty()) tok.push_back(cur);
     }
+    // [OF9] convectionScheme::New: an optional `bounded` wrapper, then the
+    // scheme; "Gauss upwind" builds the same matrix as "GaussDefCmpw upwind"
+    const bool bounded = !tok.empty() && tok[0] == "bounded";
+    if (bounded) tok.erase(tok.begin());
+    if (tok.size() == 2 && tok[0] == "Gauss" && tok[1] == "upwind") tok.erase(tok.begin());
     if (tok.size() == 2 && tok[0] == "GaussDefCmpw") tok.erase(tok.begin());
     if (tok.size() != 1) {
-        M.ctx->lastError = std::string("rt_model_set_scheme: expected \"GaussDefCmpw <limiter>\", got \"") +
+        M.ctx->lastError = std::string("rt_model_set_scheme: expected \"[bounded] GaussDefCmpw <limiter>\", got \"") +
                            (divScheme ? divScheme : "") + "\" (only the GaussDefCmpw convection scheme is supported)";
         return tok.size() == 2 ? RT_E_UNSUPPORTED : RT_E_BAD_ARG;
     }
@@ -957,6 +963,7 @@ int rt_model_set_scheme(rt_model* model, const char* divScheme) {
         return RT_E_BAD_ARG;
     }
     M.scheme = scheme;
+    M.bounded = bounded;
     return RT_OK;
 }
 

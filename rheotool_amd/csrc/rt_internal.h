@@ -268,6 +268,7 @@ struct Model {
     DBuf thetaOld;             // [N][6] theta at the start of the step (inner iterations)
     bool iterating = false;    // thetaOld holds the current step's oldTime
     int scheme = 0;            // DivScheme of div(phi,theta)
+    bool bounded = false;      // the `bounded` wrapper: diagonal minus the cell's net outflux
     DBuf gradTh;               // double [N][18] component gradients (limited schemes)
     DBuf souT;                 // double [N][6] explicit correction source
     DBuf bcType;            // int32 [nPatches]
@@ -322,6 +323,7 @@ struct AssembleArgs {
     double* fOld;         // FENE-P f, nullable
     const double* souT;   // [N][6] deferred-correction source, NULL for none
     int32_t noConv;       // the "none" scheme: no convection term
+    int32_t bounded;      // subtract V*(net outflux/V) from the diagonal
     double* partials;     // [grid][19] residual sums, entries in live slots
     CompState* cs;
     double* nnzOut;       // SolveState::misc[0]

@@ -4,7 +4,8 @@ Reads the entries the stress-step tools need from `constant/` and `system/`
 files and `0/` fields: `key value;`, `key name [dims] value;`, nested
 `name { ... }` dictionaries, quoted regex keys ("tau.*|A.*"), lists
 `( ... )` of numbers, vectors/tensors and named dictionaries (multiMode's
-`models`), `uniform` field values, and C/C++ comments.  The FoamFile header
+`models`), `uniform` field values, C/C++ comments, and `#{ ... #}` code
+blocks (kept as the single word `#codeBlock`, never run).  The FoamFile header
 is read like any other sub-dictionary.
 
 A dictionary is a plain dict: key -> list of tokens (strings, numbers, nested
@@ -15,10 +16,15 @@ from __future__ import annotations
 
 import re
 
-_TOKEN = re.compile(r'"[^"]*"|[{}()\[\];]|[^\s{}()\[\];"]+')
+# A word that starts with a letter keeps balanced parentheses written without
+# spaces (OpenFOAM keywords such as div(phi,theta) or div(etaP,grad(U))).
+_WORD = r'[A-Za-z_][^\s{}()\[\];"]*(?:\((?:[^\s(){};"]|\([^\s(){};"]*\))*\)[^\s{}()\[\];"]*)*'
+_TOKEN = re.compile(r'"[^"]*"|' + _WORD + r'|[{}()\[\];]|[^\s{}()\[\];"]+')
 
 
 def _strip_comments(s: str) -> str:
+    """Drops comments; a verbatim `#{ ... #}` code block becomes one word."""
+    s = re.sub(r"#\{.*?#\}", " #codeBlock ", s, flags=re.S)
     s = re.sub(r"/\*.*?\*/", " ", s, flags=re.S)
     return re.sub(r"//[^\n]*", " ", s)
 

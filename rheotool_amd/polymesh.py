@@ -1,0 +1,552 @@
+"""OpenFOAM case geometry and fields for the stress step: constant/polyMesh
+(ASCII) read and write, a blockMesh subset, and volField files.
+
+polyMesh follows OpenFOAM's file layout (points, faces, owner, neighbour,
+boundary; internal faces first in upper-triangular order, boundary faces
+grouped by patch, face points ordered so that the right-hand normal leaves
+the owner).  The geometry (face centres and areas, cell centres and volumes,
+interpolation weights) is computed by the case library's `rtc_mesh_from_poly`
+(rheotool_amd/csrc/case/rtcase.cpp).
+
+The blockMesh subset builds what tutorial cases of the stress step use:
+hex blocks with straight edges, `simpleGrading` with an expansion ratio or
+multi-grading sections per direction, `convertToMeters`/`scale`, block faces
+merged by position, patches listed in `boundary` (or the older `patches`
+form), and the unlisted faces in `defaultPatch` (`defaultFaces`, empty).
+Curved `edges`, `edgeGrading`, `mergePatchPairs` and projection are not
+built: a blockMeshDict that uses them raises NotImplementedError.
+"""
+from __future__ import annotations
+
+import os
+import re
+
+import numpy as np
+
+from . import cases, foamdict
+
+# hex model faces (OpenFOAM cellModel "hex"), right-hand normals outward
+HEX_FACES = np.array([[0, 4, 7, 3], [1, 2, 6, 5], [0, 1, 5, 4], [3, 7, 6, 2], [0, 3, 2, 1],
+                      [4, 5, 6, 7]], dtype=np.int64)
+
+KINDS = {"empty": cases.EMPTY, "wall": cases.WALL}
+
+
+class PolyMesh:
+    """polyMesh data: points, faces (list of int arrays), owner, neighbour,
+    patches [(name, type, startFace, nFaces)] with startFace absolute."""
+
+    def __init__(self, points, faces, owner, neighbour, patches):
+        self.points = np.asarray(points, dtype=np.float64).reshape(-1, 3)
+        self.faces = faces
+        self.owner = np.asarray(owner, dtype=np.int64)
+        self.neighbour = np.asarray(neighbour, dtype=np.int64)
+        self.patches = list(patches)
+
+    @property
+    def nCells(self):
+        return int(max(self.owner.max(initial=-1), self.neighbour.max(initial=-1))) + 1
+
+    @property
+    def nInternal(self):
+        return len(self.neighbour)
+
+    def patch_index(self, name):
+        for i, p in enumerate(self.patches):
+            if p[0] == name:
+                return i
+        raise KeyError(f"patch {name} not found")
+
+    def to_mesh(self) -> cases.Mesh:
+        nI = self.nInternal
+        plist = [(n, KINDS.get(t, cases.PATCH), s - nI, z) for n, t, s, z in self.patches]
+        return cases.from_poly(self.points, self.faces, self.owner, self.neighbour, self.nCells, plist)
+
+
+# --------------------------------------------------------------------- files
+_HEADER = """FoamFile
+{{
+    format      ascii;
+    class       {cls};
+    location    "{loc}";
+    object      {obj};
+}}
+"""
+
+
+def _strip(text):
+    text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", " ", text)
+    return re.sub(r"FoamFile\s*\{[^}]*\}", " ", text, count=1)
+
+
+def _check_ascii(text, path):
+    m = re.search(r"format\s+(\w+)\s*;", text)
+    if m and m.group(1) != "ascii":
+        raise NotImplementedError(f"{path}: only ascii polyMesh files are read")
+
+
+def _list_body(text, path):
+    """`N ( ... )` -> (N, body text between the outer parentheses)."""
+    m = re.search(r"(\d+)\s*\(", text)
+    if not m:
+        raise ValueError(f"{path}: no list found")
+    n = int(m.group(1))
+    end = text.rindex(")")
+    return n, text[m.end():end]
+
+
+def _read_list_file(path):
+    with open(path) as fh:
+        raw = fh.read()
+    _check_ascii(raw, path)
+    return _list_body(_strip(raw), path)
+
+
+def read_points(path):
+    n, body = _read_list_file(path)
+    pts = np.array(body.replace("(", " ").replace(")", " ").split(), dtype=np.float64).reshape(-1, 3)
+    if len(pts) != n:
+        raise ValueError(f"{path}: {len(pts)} points, header says {n}")
+    return pts
+
+
+def read_labels(path):
+    n, body = _read_list_file(path)
+    lab = np.array(body.split(), dtype=np.int64)
+    if len(lab) != n:
+        raise ValueError(f"{path}: {len(lab)} labels, header says {n}")
+    return lab
+
+
+def read_faces(path):
+    n, body = _read_list_file(path)
+    tok = np.array(body.replace("(", " ").replace(")", " ").split(), dtype=np.int64)
+    if n and len(tok) == 5 * n and np.all(tok[::5] == 4):  # all quads
+        return list(tok.reshape(n, 5)[:, 1:])
+    faces, i = [], 0
+    while i < len(tok):
+        k = int(tok[i])
+        faces.append(tok[i + 1:i + 1 + k])
+        i += 1 + k
+    if len(faces) != n:
+        raise ValueError(f"{path}: {len(faces)} faces, header says {n}")
+    return faces
+
+
+def read_boundary(path):
+    with open(path) as fh:
+        raw = fh.read()
+    d = foamdict.parse(_strip(raw))
+    (key, val), = [(k, v) for k, v in d.items()]
+    items = val[0]
+    out = []
+    for name, sub in items:
+        out.append((name, sub.word("type"), int(sub.scalar("startFace")), int(sub.scalar("nFaces"))))
+    if len(out) != int(float(key)):
+        raise ValueError(f"{path}: {len(out)} patches, header says {key}")
+    return out
+
+
+def read_polymesh(case):
+    d = os.path.join(case, "constant", "polyMesh")
+    return PolyMesh(read_points(os.path.join(d, "points")), read_faces(os.path.join(d, "faces")),
+                    read_labels(os.path.join(d, "owner")), read_labels(os.path.join(d, "neighbour")),
+                    read_boundary(os.path.join(d, "boundary")))
+
+
+def write_polymesh(case, pm: PolyMesh):
+    d = os.path.join(case, "constant", "polyMesh")
+    os.makedirs(d, exist_ok=True)
+    loc = "constant/polyMesh"
+
+    def put(name, cls, n, rows):
+        with open(os.path.join(d, name), "w") as fh:
+            fh.write(_HEADER.format(cls=cls, loc=loc, obj=name))
+            fh.write(f"\n{n}\n(\n")
+            fh.writelines(rows)
+            fh.write(")\n")
+
+    put("points", "vectorField", len(pm.points),
+        (f"({p[0]!r} {p[1]!r} {p[2]!r})\n" for p in pm.points.tolist()))
+    put("faces", "faceList", len(pm.faces),
+        (f"{len(f)}({' '.join(str(int(x)) for x in f)})\n" for f in pm.faces))
+    put("owner", "labelList", len(pm.owner), (f"{int(x)}\n" for x in pm.owner))
+    put("neighbour", "labelList", len(pm.neighbour), (f"{int(x)}\n" for x in pm.neighbour))
+    rows = []
+    for n, t, s, z in pm.patches:
+        rows.append(f"    {n}\n    {{\n        type            {t};\n"
+                    f"        nFaces          {z};\n        startFace       {s};\n    }}\n")
+    put("boundary", "polyBoundaryMesh", len(pm.patches), rows)
+
+
+# ----------------------------------------------------------------- blockMesh
+def _grading_points(n, g):
+    """Normalised positions 0..1 of the n+1 points along one block direction.
+    g: an expansion ratio (last cell / first cell) or a list of sections
+    (length fraction, cell fraction, expansion ratio)."""
+    if not isinstance(g, list):
+        g = [[1.0, 1.0, float(g)]]
+    secs = [list(map(float, s)) for s in g]
+    lf = np.array([s[0] for s in secs])
+    cf = np.array([s[1] for s in secs])
+    lf, cf = lf / lf.sum(), cf / cf.sum()
+    # cells per section: rounded, remainder to the last section
+    nd = [int(c * n + 0.5) for c in cf[:-1]]
+    nd.append(n - sum(nd))
+    if min(nd) < 1 and len(nd) > 1:
+        raise ValueError(f"multi-grading over {n} cells leaves a section empty")
+    pts = [0.0]
+    start = 0.0
+    for (_, _, r), k, L in zip(secs, nd, lf):
+        if k == 1 or abs(r - 1.0) < 1e-14:
+            lam = np.arange(1, k + 1) / k
+        else:
+            q = r ** (1.0 / (k - 1))
+            lam = (1.0 - q ** np.arange(1, k + 1)) / (1.0 - q ** k)
+        pts.extend(start + L * lam)
+        start += L
+    x = np.array(pts)
+    x[-1] = 1.0
+    return x
+
+
+def _blocks(d):
+    toks = d["blocks"][0]
+    blocks, i = [], 0
+    while i < len(toks):
+        if toks[i] != "hex":
+            raise NotImplementedError(f"block shape {toks[i]!r}: only hex blocks are built")
+        i += 1
+        verts = [int(x) for x in toks[i]]
+        i += 1
+        if isinstance(toks[i], str):  # zone name
+            i += 1
+        ncell = [int(x) for x in toks[i]]
+        i += 1
+        kind = toks[i]
+        i += 1
+        if kind == "edgeGrading":
+            raise NotImplementedError("edgeGrading is not built (simpleGrading only)")
+        if kind != "simpleGrading":
+            raise ValueError(f"unknown grading {kind!r}")
+        grad = toks[i]
+        i += 1
+        blocks.append((verts, ncell, grad))
+    return blocks
+
+
+def _patch_specs(d):
+    """[(name, type, [block-face vertex quads])] from `boundary` or `patches`."""
+    out = []
+    if "boundary" in d:
+        for name, sub in d["boundary"][0]:
+            faces = sub["faces"][0] if "faces" in sub else []
+            out.append((name, sub.word("type"), [[int(x) for x in f] for f in faces]))
+    elif "patches" in d:
+        toks = d["patches"][0]
+        for i in range(0, len(toks), 3):
+            out.append((toks[i + 1], toks[i], [[int(x) for x in f] for f in toks[i + 2]]))
+    return out
+
+
+def block_mesh(d: foamdict.FoamDict) -> PolyMesh:
+    """polyMesh of a blockMeshDict (the subset described in the module doc)."""
+    for key in ("edges", "mergePatchPairs"):
+        if key in d and d[key] and d[key][0]:
+            raise NotImplementedError(f"{key} are not built")
+    if "geometry" in d:
+        raise NotImplementedError("projected geometry is not built")
+    scale = d.scalar("convertToMeters", d.scalar("scale", 1.0))
+    V = np.array(d["vertices"][0], dtype=np.float64) * scale
+    blocks = _blocks(d)
+    # point grids per block, then merged by position
+    grids, cell_pts, cell_off = [], [], [0]
+    min_edge = np.inf
+    for verts, (nx, ny, nz), grad in blocks:
+        g = grad if len(grad) == 3 else None
+        if g is None:
+            raise NotImplementedError("simpleGrading needs three entries")
+        lx, ly, lz = (_grading_points(n, gi) for n, gi in zip((nx, ny, nz), g))
+        X = V[verts]  # 8 corners, hex order
+        u = lx[:, None, None, None]
+        v = ly[None, :, None, None]
+        w = lz[None, None, :, None]
+        P = ((1 - u) * (1 - v) * (1 - w) * X[0] + u * (1 - v) * (1 - w) * X[1]
+             + u * v * (1 - w) * X[2] + (1 - u) * v * (1 - w) * X[3]
+             + (1 - u) * (1 - v) * w * X[4] + u * (1 - v) * w * X[5]
+             + u * v * w * X[6] + (1 - u) * v * w * X[7])
+        grids.append(P)
+        for ax in range(3):
+            dd = np.linalg.norm(np.diff(P, axis=ax), axis=-1)
+            if dd.size:
+                min_edge = min(min_edge, dd.min())
+        cell_off.append(cell_off[-1] + nx * ny * nz)
+    allp = np.concatenate([P.reshape(-1, 3) for P in grids])
+    tol = 1e-4 * min_edge
+    from scipy.spatial import cKDTree
+    tree = cKDTree(allp)
+    pairs = tree.query_pairs(tol, output_type="ndarray")
+    parent = np.arange(len(allp))
+    if len(pairs):
+        # union-find with path halving; pairs are few (block interfaces only)
+        def find(a):
+            while parent[a] != a:
+                parent[a] = parent[parent[a]]
+                a = parent[a]
+            return a
+        for a, b in pairs:
+            ra, rb = find(a), find(b)
+            if ra != rb:
+                parent[max(ra, rb)] = min(ra, rb)
+        for a in range(len(allp)):
+            parent[a] = find(a)
+    uniq, gid_all = np.unique(parent, return_inverse=True)
+    points = allp[uniq]
+    # cells: 8 global point ids each, OpenFOAM hex vertex order
+    off = 0
+    for P, (verts, (nx, ny, nz), _) in zip(grids, blocks):
+        gid = gid_all[off:off + P.shape[0] * P.shape[1] * P.shape[2]].reshape(P.shape[:3])
+        off += gid.size
+        a = gid[:-1, :-1, :-1]
+        b = gid[1:, :-1, :-1]
+        c = gid[1:, 1:, :-1]
+        e = gid[:-1, 1:, :-1]
+        a1 = gid[:-1, :-1, 1:]
+        b1 = gid[1:, :-1, 1:]
+        c1 = gid[1:, 1:, 1:]
+        e1 = gid[:-1, 1:, 1:]
+        hexes = np.stack([a, b, c, e, a1, b1, c1, e1], axis=-1)  # [nx, ny, nz, 8]
+        cell_pts.append(hexes.transpose(2, 1, 0, 3).reshape(-1, 8))  # i fastest
+    cells = np.concatenate(cell_pts)
+    nC = len(cells)
+    cf = cells[:, HEX_FACES]  # [nC, 6, 4]
+    flat = cf.reshape(-1, 4)
+    key = np.sort(flat, axis=1)
+    _, inv, cnt = np.unique(key, axis=0, return_inverse=True, return_counts=True)
+    inv = inv.ravel()
+    occ_cell = np.repeat(np.arange(nC), 6)
+    # internal faces: two occurrences; the owner is the lower cell
+    order = np.argsort(inv, kind="stable")
+    inv_s = inv[order]
+    first = np.ones(len(inv_s), bool)
+    first[1:] = inv_s[1:] != inv_s[:-1]
+    starts = np.nonzero(first)[0]
+    internal = cnt[inv_s[starts]] == 2
+    i0 = order[starts[internal]]
+    i1 = order[starts[internal] + 1]
+    c0, c1 = occ_cell[i0], occ_cell[i1]
+    own = np.minimum(c0, c1)
+    nbr = np.maximum(c0, c1)
+    occ_own = np.where(c0 <= c1, i0, i1)
+    srt = np.lexsort((nbr, own))
+    own, nbr, occ_own = own[srt], nbr[srt], occ_own[srt]
+    if np.any(cnt > 2):
+        raise ValueError("blockMesh: a face is shared by more than two cells")
+    # boundary occurrences, assigned to patches by block faces
+    bnd_occ = order[starts[~internal]]
+    bnd_key_of = {int(inv[o]): int(o) for o in bnd_occ}
+    used = set()
+    faces = [flat[o] for o in occ_own]
+    owner = list(own)
+    patches = []
+    nI = len(own)
+    for name, ptype, quads in _patch_specs(d):
+        start = nI + sum(p[3] for p in patches)
+        n_here = 0
+        for q in quads:
+            occs = _block_face_occurrences(q, blocks, cell_off, HEX_FACES)
+            for o in occs:
+                k = int(inv[o])
+                if k not in bnd_key_of:
+                    raise ValueError(f"patch {name}: face {q} is not on the boundary")
+                if k in used:
+                    raise ValueError(f"patch {name}: face {q} is in more than one patch")
+                used.add(k)
+                faces.append(flat[o])
+                owner.append(occ_cell[o])
+                n_here += 1
+        patches.append((name, ptype, start, n_here))
+    rest = [o for k, o in bnd_key_of.items() if k not in used]
+    if rest:
+        dp = d.get("defaultPatch")
+        dname = dp.word("name", "defaultFaces") if isinstance(dp, foamdict.FoamDict) else "defaultFaces"
+        dtype = dp.word("type", "empty") if isinstance(dp, foamdict.FoamDict) else "empty"
+        rest.sort()
+        start = nI + sum(p[3] for p in patches)
+        for o in rest:
+            faces.append(flat[o])
+            owner.append(occ_cell[o])
+        patches.append((dname, dtype, start, len(rest)))
+    return PolyMesh(points, faces, np.array(owner), nbr, patches)
+
+
+def _block_face_occurrences(quad, blocks, cell_off, hexfaces):
+    """Cell-face occurrence indices (cell*6 + face) covering one block face."""
+    qs = set(quad)
+    for b, (verts, (nx, ny, nz), _) in enumerate(blocks):
+        for fi, hf in enumerate(hexfaces):
+            if set(verts[j] for j in hf) == qs:
+                i = np.arange(nx)
+                j = np.arange(ny)
+                k = np.arange(nz)
+                if fi == 0:
+                    i = np.array([0])
+                elif fi == 1:
+                    i = np.array([nx - 1])
+                elif fi == 2:
+                    j = np.array([0])
+                elif fi == 3:
+                    j = np.array([ny - 1])
+                elif fi == 4:
+                    k = np.array([0])
+                else:
+                    k = np.array([nz - 1])
+                K, J, I = np.meshgrid(k, j, i, indexing="ij")
+                cells = cell_off[b] + I + nx * (J + ny * K)
+                return (cells.ravel() * 6 + fi).tolist()
+    raise ValueError(f"boundary face {quad} is not a face of any block")
+
+
+def read_block_mesh(case):
+    return block_mesh(foamdict.read(os.path.join(case, "system", "blockMeshDict")))
+
+
+def case_mesh(case) -> PolyMesh:
+    """constant/polyMesh when present, else the case's system/blockMeshDict."""
+    if os.path.isfile(os.path.join(case, "constant", "polyMesh", "faces")):
+        return read_polymesh(case)
+    return read_block_mesh(case)
+
+
+# -------------------------------------------------------------------- fields
+NCOMP = {"scalar": 1, "vector": 3, "symmTensor": 6, "tensor": 9}
+CLASS = {1: "volScalarField", 3: "volVectorField", 6: "volSymmTensorField", 9: "volTensorField"}
+TYPE = {1: "scalar", 3: "vector", 6: "symmTensor", 9: "tensor"}
+
+
+def _values(tokens, n, nc):
+    """`uniform x` / `nonuniform List<T> N (...)` token lists -> [n, nc]."""
+    if tokens[0] == "uniform":
+        x = tokens[1]
+        x = np.array(x if isinstance(x, list) else [x], dtype=np.float64)
+        return np.broadcast_to(x, (n, nc)).copy()
+    if tokens[0] == "nonuniform":
+        lst = [t for t in tokens[1:] if isinstance(t, list)]
+        a = np.array(lst[0] if lst else [], dtype=np.float64).reshape(-1, nc)
+        if len(a) != n:
+            raise ValueError(f"nonuniform list of {len(a)} values, {n} expected")
+        return a
+    raise ValueError(f"field value {tokens[:2]} not read")
+
+
+def _internal_fast(raw, nc):
+    """internalField of a large nonuniform field, parsed with numpy."""
+    m = re.search(r"internalField\s+nonuniform\s+List<\w+>\s*(\d+)\s*\(", raw)
+    if not m:
+        return None
+    n = int(m.group(1))
+    depth, i = 1, m.end()
+    while depth:
+        ch = raw[i]
+        depth += (ch == "(") - (ch == ")")
+        i += 1
+    body = raw[m.end():i - 1]
+    a = np.array(body.replace("(", " ").replace(")", " ").split(), dtype=np.float64).reshape(-1, nc)
+    if len(a) != n:
+        raise ValueError(f"internalField: {len(a)} values, header says {n}")
+    return a, raw[:m.start()] + "internalField uniform 0;" + raw[i:]
+
+
+class VolField:
+    """A volField file: internal values [nCells, nc] and per-patch entries."""
+
+    def __init__(self, name, internal, patches):
+        self.name, self.internal, self.patches = name, internal, patches  # patches: name -> FoamDict
+
+    def patch_type(self, patch):
+        return self.patches[patch].word("type")
+
+    def patch_values(self, patch, n):
+        nc = self.internal.shape[1]
+        ent = self.patches[patch]
+        return _values(ent["value"], n, nc) if "value" in ent else None
+
+
+def read_field(path, pm: PolyMesh) -> VolField:
+    with open(path) as fh:
+        raw = fh.read()
+    _check_ascii(raw, path)
+    cls = re.search(r"class\s+(\w+)\s*;", raw).group(1)
+    nc = {"volScalarField": 1, "volVectorField": 3, "volSymmTensorField": 6, "volTensorField": 9}[cls]
+    fast = _internal_fast(raw, nc)
+    internal = None
+    if fast is not None:
+        internal, raw = fast
+    d = foamdict.parse(_strip(raw))
+    if internal is None:
+        internal = _values(d["internalField"], pm.nCells, nc)
+    if len(internal) != pm.nCells:
+        raise ValueError(f"{path}: {len(internal)} cell values for {pm.nCells} cells")
+    bf = d.sub("boundaryField")
+    patches = {}
+    for name, _, _, _ in pm.patches:
+        ent = bf.lookup_regex(name)
+        if not isinstance(ent, foamdict.FoamDict):
+            raise KeyError(f"{path}: no boundaryField entry for patch {name}")
+        patches[name] = ent
+    return VolField(os.path.basename(path), internal, patches)
+
+
+def read_surface_field(path, pm: PolyMesh):
+    """A surfaceScalarField (e.g. phi): (internal [nInternal], boundary [nBoundary])."""
+    with open(path) as fh:
+        raw = fh.read()
+    _check_ascii(raw, path)
+    fast = _internal_fast(raw, 1)
+    internal = None
+    if fast is not None:
+        internal, raw = fast
+    d = foamdict.parse(_strip(raw))
+    if internal is None:
+        internal = _values(d["internalField"], pm.nInternal, 1)
+    if len(internal) != pm.nInternal:
+        raise ValueError(f"{path}: {len(internal)} face values for {pm.nInternal} internal faces")
+    nI = pm.nInternal
+    bnd = np.zeros(len(pm.owner) - nI)
+    bf = d.sub("boundaryField")
+    for name, ptype, start, n in pm.patches:
+        ent = bf.lookup_regex(name)
+        if not isinstance(ent, foamdict.FoamDict):
+            raise KeyError(f"{path}: no boundaryField entry for patch {name}")
+        if ent.word("type") == "empty" or "value" not in ent:
+            continue
+        bnd[start - nI:start - nI + n] = _values(ent["value"], n, 1)[:, 0]
+    return internal[:, 0], bnd
+
+
+def _fmt(row):
+    return row[0] if len(row) == 1 else "(" + " ".join(row) + ")"
+
+
+def write_field(path, name, values, pm: PolyMesh, patch_entries, location="0", dims="[0 0 0 0 0 0 0]"):
+    """Writes a volField; patch_entries: patch -> (type, values or None)."""
+    values = np.asarray(values, dtype=np.float64).reshape(pm.nCells, -1)
+    nc = values.shape[1]
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    with open(path, "w") as fh:
+        fh.write(_HEADER.format(cls=CLASS[nc], loc=location, obj=name))
+        fh.write(f"\ndimensions      {dims};\n\n")
+        fh.write(f"internalField   nonuniform List<{TYPE[nc]}> {len(values)}\n(\n")
+        fh.writelines(_fmt([repr(x) for x in r]) + "\n" for r in values.tolist())
+        fh.write(")\n;\n\nboundaryField\n{\n")
+        for pname, _, _, nf in pm.patches:
+            t, v = patch_entries[pname]
+            fh.write(f"    {pname}\n    {{\n        type            {t};\n")
+            if v is not None:
+                v = np.asarray(v, dtype=np.float64).reshape(nf, nc)
+                fh.write(f"        value           nonuniform List<{TYPE[nc]}> {nf}\n(\n")
+                fh.writelines(_fmt([repr(x) for x in r]) + "\n" for r in v.tolist())
+                fh.write(")\n;\n")
+            fh.write("    }\n")
+        fh.write("}\n")

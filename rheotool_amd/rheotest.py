@@ -203,21 +203,28 @@ class Runner:
             entries = [d]
         self.modes = []
         for e in entries:
-            t = e.word("type")
-            kw = dict(etaS=e.scalar("etaS"), etaP=e.scalar("etaP"), rho=e.scalar("rho", 1.0))
-            for k, o in (("lambda", "lam"), ("alpha", "alpha"), ("L2", "L2"), ("epsilon", "epsilon"),
-                         ("zeta", "zeta"), ("lambdaR", "lambdaR"), ("lambdaD", "lambdaD"), ("beta", "beta"),
-                         ("delta", "delta"), ("chiMax", "chiMax"), ("lambdaS", "lambdaS"),
-                         ("lambdaB", "lambdaB"), ("q", "q"), ("m", "m"), ("K", "K"), ("L", "Lc"),
-                         ("a", "a"), ("b", "b")):
-                if k in e and not isinstance(e[k], foamdict.FoamDict):
-                    kw[o] = e.scalar(k)
-            if "n" in e:
-                kw["n" if t == "XPomPomLog" else "wm_n"] = e.scalar("n")
             st = dict(theta=th0.copy(), ev=np.eye(3)[None].copy(), vec=np.eye(3)[None].copy(),
                       tau=np.zeros((1, 6)), old=None)
-            self.modes.append((O.params(t, **kw), kw.get("etaS", 0.0), st))
+            p, etaS = self.oracle_params(e)
+            self.modes.append((p, etaS, st))
         self.ctl = O.controls(tol=1e-10)
+
+    @staticmethod
+    def oracle_params(e):
+        """Oracle parameters and etaS of one model dictionary."""
+        from oracle import oracle as O
+        t = e.word("type")
+        kw = dict(etaS=e.scalar("etaS"), etaP=e.scalar("etaP"), rho=e.scalar("rho", 1.0))
+        for k, o in (("lambda", "lam"), ("alpha", "alpha"), ("L2", "L2"), ("epsilon", "epsilon"),
+                     ("zeta", "zeta"), ("lambdaR", "lambdaR"), ("lambdaD", "lambdaD"), ("beta", "beta"),
+                     ("delta", "delta"), ("chiMax", "chiMax"), ("lambdaS", "lambdaS"),
+                     ("lambdaB", "lambdaB"), ("q", "q"), ("m", "m"), ("K", "K"), ("L", "Lc"),
+                     ("a", "a"), ("b", "b")):
+            if k in e and not isinstance(e[k], foamdict.FoamDict):
+                kw[o] = e.scalar(k)
+        if "n" in e:
+            kw["n" if t == "XPomPomLog" else "wm_n"] = e.scalar("n")
+        return O.params(t, **kw), kw.get("etaS", 0.0)
 
     def _step_oracle(self, dt, iteration):
         import dataclasses

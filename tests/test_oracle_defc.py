@@ -171,3 +171,48 @@ def test_limited_step_converges_and_differs_from_upwind(scheme):
     assert np.all(np.isfinite(th))
     rel = np.linalg.norm(th - th_up) / np.linalg.norm(th_up - f.theta)
     assert 1e-6 < rel < 1.0
+
+
+@pytest.mark.parametrize("scheme", ["upwind", "cubista", "none"])
+def test_bounded_wrapper_leaves_uniform_fields_to_the_source(scheme):
+    """bounded: div(phi theta) - theta div(phi).  For a uniform theta the two
+    cancel for any flux, so with a non-solenoidal flux the bounded step of a
+    uniform field equals the step without convection, while the plain scheme
+    does not (its convection term is theta * net outflux).  With "none" only
+    the -theta div(phi) part is left: per cell
+    theta_new = V (theta/dt + S) / (V/dt - net outflux)."""
+    m = cases.cartesian(5, 4, 3, jitter=0.1)
+    f = cases.make_fields(m, cases.FIELDS_TG, seed=5, lnLo=-1, lnHi=1)
+    rng = np.random.default_rng(4)
+    R, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    lam = np.array([-0.5, 0.2, 0.9])
+    T = R @ np.diag(lam) @ R.T
+    f.theta[:] = T[[0, 0, 0, 1, 1, 2], [0, 1, 2, 1, 2, 2]]
+    f.eigVals[:] = np.diag(np.exp(lam))
+    f.eigVecs[:] = R
+    f.phi = rng.normal(size=f.phi.shape)  # far from solenoidal
+    p = O.params("Oldroyd-BLog", lam=0.5)
+    g0 = np.zeros((m.nCells, 3, 3))
+    b = O.correct(m, p, f, 1e-2, gradU=g0, scheme="bounded " + scheme)[0]
+    f0 = cases.make_fields(m, cases.FIELDS_TG, seed=5, lnLo=-1, lnHi=1)
+    f0.theta, f0.eigVals, f0.eigVecs = f.theta, f.eigVals, f.eigVecs
+    f0.phi = np.zeros_like(f.phi)
+    ref = O.correct(m, p, f0, 1e-2, gradU=g0)[0]
+    if scheme == "none":
+        net = np.zeros(m.nCells)
+        nI = m.nInternal
+        np.add.at(net, m.owner[:nI], f.phi[:nI])
+        np.subtract.at(net, m.neighbour, f.phi[:nI])
+        live = np.ones(m.nBoundary, bool)
+        for pi in range(m.nPatches):
+            if m.patchKind[pi] == cases.EMPTY:
+                live[m.patchStart[pi]:m.patchStart[pi] + m.patchSize[pi]] = False
+        np.add.at(net, m.owner[nI:][live], f.phi[nI:][live])
+        S = (ref - f.theta) / 1e-2
+        want = m.V[:, None] * (f.theta / 1e-2 + S) / (m.V / 1e-2 - net)[:, None]
+        assert np.allclose(b, want, rtol=1e-12, atol=1e-13), np.abs(b - want).max()
+        return
+    assert np.allclose(b, ref, rtol=0, atol=1e-11), np.abs(b - ref).max()
+    if scheme != "none":
+        plain = O.correct(m, p, f, 1e-2, gradU=g0, scheme=scheme)[0]
+        assert np.abs(plain - ref).max() > 1e-3
