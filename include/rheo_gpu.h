@@ -151,7 +151,8 @@ typedef struct {
     double initialResidual;
     double finalResidual;
     int32_t nIterations;  /* Krylov iterations (inner iterations for "mixed") */
-    int32_t nRefinements; /* FP64 refinement sweeps ("mixed"), 0 otherwise */
+    int32_t nRefinements; /* FP64 true-residual sweeps: refinements ("mixed")
+                             and restarts after a PBiCGStab breakdown */
     int32_t converged;
     int32_t solved;       /* 0 for a frozen (invalid) component */
 } rt_solver_perf;
@@ -177,8 +178,8 @@ void rt_model_destroy(rt_model* model);
 const char* rt_model_type(const rt_model* model);
 
 /* State: theta [6N] (MUST_READ); eigVals/eigVecs [9N] each (READ_IF_PRESENT:
- * NULL leaves them at the identity tensor, as the reference's default);
- * tau [6N] may be NULL. */
+ * NULL leaves them as they are, which is the identity tensor until they are
+ * first set, as the reference's default); tau [6N] may be NULL. */
 int rt_model_set_state(rt_model* model, const double* theta, const double* eigVals,
                        const double* eigVecs, const double* tau);
 

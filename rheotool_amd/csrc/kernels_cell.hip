@@ -245,7 +245,6 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3, 8))
     const int64_t nE = a.nEll;
     T* rOut = static_cast<T*>(a.r);
     T* r0Out = static_cast<T*>(a.r0);
-    T* pOut = static_cast<T*>(a.p);
     for (int64_t it = 0;; ++it) {
         const int64_t tl = tile_of(it, a.N);
         if (tl < 0) break;
@@ -458,7 +457,6 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3, 8))
         }
         st6(rOut + (int64_t)NCMP * r, rt);
         st6(r0Out + (int64_t)NCMP * r, rt);
-        st6(pOut + (int64_t)NCMP * r, rt);  // the first search direction
     }
     double q[19];
 #pragma unroll
@@ -1051,10 +1049,10 @@ static void launch_assemble_kind(Ctx* ctx, const AssembleArgs& a, int grid) {
 
 void launch_assemble(Ctx* ctx, const AssembleArgs& a, int grid) {
     // algorithmic bytes per cell: theta 48, eigen state 96, U 32, V 8, b 48, diag 8, 1/diag 8,
-    // r, r0 and p (3 x 6 x w); per ELL entry: face id 4, slot 1, column 4, phi 8, value 8
+    // r and r0 (2 x 6 x w), theta.oldTime() 48 more in inner iterations; per ELL entry: face id 4, slot 1, column 4, phi 8, value 8
     // (+ Sf 24 and w 8 for the gradient)
     const double w = a.mixed ? 4 : 8;
-    const double by = (double)a.N * (48 + 96 + 32 + 8 + 48 + 8 + 8 + 18 * w) +
+    const double by = (double)a.N * (48 + 96 + 32 + 8 + 48 + 8 + 8 + 12 * w + (a.thetaOld ? 48 : 0)) +
                       (double)a.nEll * (4 + 1 + 4 + 8 + 8 + (a.gradU ? 0 : 24 + 8));
     {
         ProfScope ps(ctx, a.mixed ? "k_assemble<f32>" : "k_assemble<f64>", by);

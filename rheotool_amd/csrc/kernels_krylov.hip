@@ -98,7 +98,6 @@ __global__ void __launch_bounds__(BLK) k_residual(KryArgs<T> a) {
             const T rt = (T)res;
             a.r[(int64_t)NCMP * r + c] = rt;
             a.r0[(int64_t)NCMP * r + c] = rt;
-            a.p[(int64_t)NCMP * r + c] = rt;  // the first search direction
             q[c] += fabs(res);
             q[6 + c] += (double)rt * (double)rt;
             if (MODE == 0) {
@@ -143,6 +142,35 @@ __global__ void __launch_bounds__(BLK) k_fin_residual(KryArgs<T> a) {
                 }
             }
         }
+    }
+}
+
+// ---- p = r  |  p = r + beta (p - omega v) ---------------------------------
+template <class T>
+__global__ void __launch_bounds__(BLK) k_kry_p(KryArgs<T> a) {
+    bool on[6];
+    bool first[6];
+    T beta[6], omega[6];
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        on[c] = a.cs[c].active;
+        first[c] = a.cs[c].innerIter == 0;
+        beta[c] = (T)a.cs[c].beta;
+        omega[c] = (T)a.cs[c].omega;
+        any |= on[c];
+    }
+    if (!any) return;
+    const int64_t N = a.N;
+    for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < N; i += (int64_t)gridDim.x * BLK) {
+        T rr[6], pp[6], vv[6];
+        ld6(a.r + NCMP * i, rr);
+        ld6(a.p + NCMP * i, pp);
+        ld6(a.v + NCMP * i, vv);
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+            if (on[c]) pp[c] = first[c] ? rr[c] : rr[c] + beta[c] * (pp[c] - omega[c] * vv[c]);
+        st6(a.p + NCMP * i, pp);
     }
 }
 
@@ -208,13 +236,10 @@ __global__ void __launch_bounds__(BLK) k_fin_v(KryArgs<T> a) {
     sum_partials<6>(a.partials, a.G, sums);
     if (threadIdx.x == 0)
         for (int c = 0; c < 6; ++c)
-            if (on[c]) {
-                a.cs[c].sigma = sums[c];
-                a.cs[c].alpha = a.cs[c].rho / sums[c];
-            }
+            if (on[c]) a.cs[c].alpha = a.cs[c].rho / sums[c];
 }
 
-// ---- s = r - alpha v ; t = A M^-1 s ; |s|, (t,s), (t,t), (r0,t) -------------
+// ---- s = r - alpha v ; t = A M^-1 s ; |s|, (t,s), (t,t) --------------------
 template <class T>
 __global__ void __launch_bounds__(BLK) k_kry_t(KryArgs<T> a) {
     bool on[6];
@@ -227,16 +252,16 @@ __global__ void __launch_bounds__(BLK) k_kry_t(KryArgs<T> a) {
         any |= on[c];
     }
     if (!any) return;
-    T q[24];
+    T q[18];
 #pragma unroll
-    for (int i = 0; i < 24; ++i) q[i] = 0;
+    for (int i = 0; i < 18; ++i) q[i] = 0;
     for (int64_t it = 0;; ++it) {
         const int64_t tl = tile_of(it, a.N);
         if (tl < 0) break;
         const int64_t t0 = tl * BLK;
         const int32_t r = (int32_t)(t0 + threadIdx.x);
         if (r >= a.N) continue;
-        T sv[6], acc[6], rr[6], vv[6], rr0[6];
+        T sv[6], acc[6], rr[6], vv[6];
         ld6(a.r + (int64_t)NCMP * r, rr);
         ld6(a.v + (int64_t)NCMP * r, vv);
 #pragma unroll
@@ -261,20 +286,18 @@ __global__ void __launch_bounds__(BLK) k_kry_t(KryArgs<T> a) {
             for (int c = 0; c < 6; ++c) acc[c] += av * (rc[c] - alpha[c] * vc[c]);
         }
         st6(a.t + (int64_t)NCMP * r, acc);
-        ld6(a.r0 + (int64_t)NCMP * r, rr0);
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
             if (!on[c]) continue;
             q[c] += fabs(sv[c]);
             q[6 + c] += acc[c] * sv[c];
             q[12 + c] += acc[c] * acc[c];
-            q[18 + c] += rr0[c] * acc[c];
         }
     }
-    double qd[24];
+    double qd[18];
 #pragma unroll
-    for (int i = 0; i < 24; ++i) qd[i] = q[i];
-    write_partials<24>(qd, a.partials);
+    for (int i = 0; i < 18; ++i) qd[i] = q[i];
+    write_partials<18>(qd, a.partials);
 }
 
 template <class T>
@@ -287,8 +310,8 @@ __global__ void __launch_bounds__(BLK) k_fin_t(KryArgs<T> a) {
         any |= on[c];
     }
     if (!any) return;
-    double sums[24];
-    sum_partials<24>(a.partials, a.G, sums);
+    double sums[18];
+    sum_partials<18>(a.partials, a.G, sums);
     if (threadIdx.x == 0) {
         for (int c = 0; c < 6; ++c) {
             if (!on[c]) continue;
@@ -299,23 +322,16 @@ __global__ void __launch_bounds__(BLK) k_fin_t(KryArgs<T> a) {
                 cs.exitAtS = 1;
             } else {
                 cs.omega = sums[6 + c] / sums[12 + c];
-                // the next search direction is formed in the x/r pass, so beta
-                // is needed now: rho_new = (r0, s - omega t)
-                //                        = rho - alpha sigma - omega (r0, t)
-                const double rhoNew = (cs.rho - cs.alpha * cs.sigma) - cs.omega * sums[18 + c];
-                cs.beta = (fabs(cs.omega) < 1e-300) ? 0.0 : (rhoNew / cs.rho) * (cs.alpha / cs.omega);
             }
         }
     }
 }
 
-// ---- x += alpha M^-1 p + omega M^-1 s ; r = s - omega t ; p = r + beta (p - omega v)
-//      |r|, (r0, r) ------------------------------------------------------------
+// ---- x += alpha M^-1 p + omega M^-1 s ; r = s - omega t ; |r|, (r0, r) ------
 template <class T>
 __global__ void __launch_bounds__(BLK) k_kry_xr(KryArgs<T> a) {
     bool on[6], ex[6];
     double alpha[6], omega[6];
-    T beta[6];
     bool any = false;
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
@@ -323,7 +339,6 @@ __global__ void __launch_bounds__(BLK) k_kry_xr(KryArgs<T> a) {
         ex[c] = a.cs[c].exitAtS;
         alpha[c] = a.cs[c].alpha;
         omega[c] = a.cs[c].omega;
-        beta[c] = (T)a.cs[c].beta;
         any |= on[c];
     }
     if (!any) return;
@@ -359,13 +374,11 @@ __global__ void __launch_bounds__(BLK) k_kry_xr(KryArgs<T> a) {
             xv[c] += alpha[c] * y + omega[c] * z;
             const T rn = sv - (T)omega[c] * tt[c];
             rr[c] = rn;
-            pp[c] = rn + beta[c] * (pp[c] - (T)omega[c] * vv[c]);
             q[c] += fabs(rn);
             q[6 + c] += rr0[c] * rn;
         }
         st6(a.x + j, xv);
         st6(a.r + j, rr);
-        st6(a.p + j, pp);
     }
     double qd[12];
 #pragma unroll
@@ -391,7 +404,7 @@ __global__ void __launch_bounds__(BLK) k_fin_xr(KryArgs<T> a) {
             CompState& cs = a.cs[c];
             cs.iter += 1;
             cs.innerIter += 1;
-            bool stop;
+            bool stop, breakdown = false;
             if (cs.exitAtS) {
                 cs.exitAtS = 0;
                 stop = true;
@@ -404,16 +417,26 @@ __global__ void __launch_bounds__(BLK) k_fin_xr(KryArgs<T> a) {
                 if (a.ctl.mixed && cs.innerIter >= a.ctl.maxInner) cont = false;
                 stop = !cont;
                 if (cont) {
-                    // p was already advanced with beta from the recurrence; the
-                    // direct (r0, r) is the rho of the next iteration's alpha
                     cs.rhoOld = cs.rho;
                     cs.rho = sums[6 + c];
-                    if (fabs(cs.rho) < 1e-300 || fabs(cs.omega) < 1e-300) stop = true;
+                    if (fabs(cs.rho) < 1e-300 || fabs(cs.omega) < 1e-300) {
+                        stop = true;
+                        breakdown = true;
+                    } else {
+                        cs.beta = (cs.rho / cs.rhoOld) * (cs.alpha / cs.omega);
+                    }
                 }
             }
             if (stop) {
                 cs.active = 0;
-                if (a.ctl.mixed) {
+                // A breakdown ((r0, r) or omega vanishing before convergence)
+                // restarts from the true residual with r0 = r, through the same
+                // pass as a refinement sweep (at most maxRefine times); [OF9]
+                // PBiCGStab stops unconverged there instead (checkSingularity), but its
+                // DILU preconditioner meets breakdowns in other places than
+                // right Jacobi does, so stopping would leave results that differ
+                // from it by more than the tolerance.
+                if (a.ctl.mixed || breakdown) {
                     cs.needRefine = 1;
                 } else {
                     cs.converged = conv_outer(cs, a.ctl, cs.finalRes);
@@ -530,7 +553,7 @@ struct SolveRun {
     bool mixed = false;
     KryArgs<float> af{};
     KryArgs<double> ad{};
-    int G = 1;
+    int G = 1, pgrid = 1;
     double N = 0, ne = 0, nValid = 0;
     int32_t valid[6];
     Ctx* ctx = nullptr;
@@ -603,10 +626,11 @@ void fill_args(Ctx* ctx, const SolveArgs& sa, SolveRun& R) {
 // move all 6 components; gathered vectors counted once)
 struct Bytes {
     double N, ne, w;
-    double res(double nc) const { return N * 8.0 + ne * 12 + nc * N * (8 + 8 + 3 * w); }
+    double res(double nc) const { return N * 8.0 + ne * 12 + nc * N * (8 + 8 + 2 * w); }
+    double p(double nc) const { return nc * N * 4 * w; }
     double v(double nc) const { return ne * (4 + w) + nc * N * 3 * w; }
-    double t(double nc) const { return ne * (4 + w) + nc * N * 4 * w; }
-    double xr(double nc) const { return N * 8.0 + nc * N * (16 + 7 * w); }
+    double t(double nc) const { return ne * (4 + w) + nc * N * 3 * w; }
+    double xr(double nc) const { return N * 8.0 + nc * N * (16 + 6 * w); }
 };
 
 template <class T>
@@ -638,6 +662,10 @@ void launch_round(Ctx* ctx, SolveRun& R, double nAct, double nRef) {
         }
         hipLaunchKernelGGL((k_fin_residual<T, 1>), dim3(1), dim3(BLK), 0, ctx->stream, a);
         nAct += nRef;  // components restarted by the refinement sweep join this round
+    }
+    {
+        ProfScope ps(ctx, mixed ? "k_kry_p<f32>" : "k_kry_p<f64>", B.p(nAct));
+        hipLaunchKernelGGL(k_kry_p<T>, dim3(R.pgrid), dim3(BLK), 0, ctx->stream, a);
     }
     {
         ProfScope ps(ctx, mixed ? "k_kry_v<f32>" : "k_kry_v<f64>", B.v(nAct));
@@ -681,6 +709,7 @@ void solve_begin(Ctx* ctx, const SolveArgs& sa) {
     SolveRun& R = *K.run;
     R.mixed = mixed;
     R.G = K.grid;
+    R.pgrid = std::min<int>(K.grid, (int)kblk(m.N));
     R.N = m.N;
     R.ne = (double)m.nEll;
     R.neFromDevice = sa.slotMask != nullptr;
@@ -750,7 +779,7 @@ void solve_finish(Krylov& K, rt_solver_perf perf[6]) {
         pf.initialResidual = R.host[c].initRes;
         pf.finalResidual = R.host[c].finalRes;
         pf.nIterations = R.host[c].iter;
-        pf.nRefinements = R.mixed ? R.host[c].nRefine : 0;
+        pf.nRefinements = R.host[c].nRefine;
         pf.converged = R.host[c].converged;
         pf.solved = 1;
     }

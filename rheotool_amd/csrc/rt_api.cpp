@@ -426,7 +426,6 @@ void modelCorrectBegin(Model& M, double deltaT, const double* gradU, int iterati
     for (int q = 0; q < 6; ++q) a.valid[q] = m.valid[q];
     a.r = mixed ? M.kry.rs.p : M.kry.r.p;
     a.r0 = mixed ? M.kry.r0s.p : M.kry.r0.p;
-    a.p = mixed ? M.kry.ps.p : M.kry.p.p;
     a.rD = M.kry.rD.as<double>();
     a.ctl = kryCtl(M.ctl, mixed);
     a.mixed = mixed ? 1 : 0;

@@ -185,8 +185,7 @@ Controls parseControls(const char* text);
 struct alignas(64) CompState {
     double rho, rhoOld, alpha, omega, beta;
     double normFactor, initRes, finalRes, xbar;
-    double sigma;       // (r0, v) of the current iteration
-    double tmp1, tmp2;
+    double tmp0, tmp1, tmp2;
     int32_t iter, active, converged, solved;
     int32_t exitAtS, needRefine, nRefine, innerIter;
     double innerTarget;  // inner (FP32) residual target per sweep
@@ -331,7 +330,6 @@ struct AssembleArgs {
     // initial residual of the solve, fused into the assembly
     void* r;              // [N][6] Krylov precision
     void* r0;
-    void* p;
     double* rD;           // [N] 1/diag
     KryCtl ctl;
     int32_t mixed;        // r/r0 in FP32

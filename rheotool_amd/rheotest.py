@@ -224,6 +224,12 @@ class Runner:
                 kw[o] = e.scalar(k)
         if "n" in e:
             kw["n" if t == "XPomPomLog" else "wm_n"] = e.scalar("n")
+        if t == "PTTLog":
+            # PTTLog.C:139 reads destructionFunctionType (no default)
+            fn = e.word("destructionFunctionType")
+            if fn not in ("linear", "exponential"):
+                raise NotImplementedError(f"PTT destructionFunctionType {fn}")
+            kw["ptt_exp"] = fn == "exponential"
         return O.params(t, **kw), kw.get("etaS", 0.0)
 
     def _step_oracle(self, dt, iteration):

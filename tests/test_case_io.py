@@ -154,7 +154,10 @@ def test_field_files_round_trip(tmp_path):
     assert np.array_equal(f.patch_values("inlet", 10), inlet)
     assert f.patch_type("walls") == "zeroGradient" and f.patch_values("walls", 24) is None
     g = polymesh.read_field(os.path.join(GOLDEN, "stresscase_channel", "0", "thetaM1"), pm)
-    assert np.all(g.internal == 0) and np.allclose(g.patch_values("inlet", 10), [0.1, 0.02, 0, 0.05, 0, 0])
+    assert np.all(g.internal == [0.3, 0, 0, -0.2, 0, 0.1])
+    assert np.allclose(g.patch_values("inlet", 10), [0.4, 0.05, 0, -0.1, 0, 0.15])
+    e = polymesh.read_field(os.path.join(GOLDEN, "stresscase_channel", "0", "eigValsM1"), pm)
+    assert e.internal.shape == (pm.nCells, 9) and np.allclose(e.internal[0, [0, 4, 8]], np.exp([0.3, -0.2, 0.1]))
 
 
 def test_code_blocks_are_kept_as_one_word():

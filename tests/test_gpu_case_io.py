@@ -1,8 +1,13 @@
 """Stress-only runs of OpenFOAM case directories (blockMesh or polyMesh, 0/
 fields, fvSchemes/fvSolution/controlDict) on the device path against the same
-runs through the oracle: theta and tau of every mode within 1e-9 relative L2
-after ten steps, and the files written at each write time read back equal to
-the device state."""
+runs through the oracle, and the files written at each write time read back
+equal to the device state.
+
+Bar: each step's solve stops at the normalised residual 1e-10 with a different
+preconditioner on each side (right Jacobi against DILU), so the two runs'
+fields differ by up to ~1e-10 per step and the differences add up over a run;
+the single-step bar of 1e-9 relative L2 is applied per step taken (1e-8 after
+ten steps)."""
 import os
 
 import numpy as np
@@ -31,10 +36,11 @@ def test_case_run_device_matches_oracle(tmp_path, name, vel):
     orc.run(steps=10, write=False)
     d, dtau = dev.state()
     o, otau = orc.state()
+    bar = 10 * 1e-9
     for k in o:
-        assert rel(d[k]["theta"], o[k]["theta"]) < 1e-9, k
-        assert rel(d[k]["tau"], o[k]["tau"]) < 1e-9, k
-    assert rel(dtau, otau) < 1e-9
+        assert rel(d[k]["theta"], o[k]["theta"]) < bar, k
+        assert rel(d[k]["tau"], o[k]["tau"]) < bar, k
+    assert rel(dtau, otau) < bar
     last = written[-1]
     for md in dev.modes:
         f = polymesh.read_field(os.path.join(last, md.field), dev.pm)
@@ -53,3 +59,39 @@ def test_written_polymesh_gives_the_same_run(tmp_path):
     rb = stressfoam.StressCase(b)
     rb.run(steps=3, write=False)
     assert np.array_equal(ra.state()[1], rb.state()[1])
+
+
+def test_breakdown_restarts_instead_of_stopping(tmp_path):
+    """theta = 0 everywhere but a fixedValue inlet: the right-Jacobi
+    PBiCGStab of the xy component meets (r0, r) = 0 after one iteration (the
+    residual moves off the inlet cells where r0 lives).  The solve restarts
+    from the true residual and converges to the same step as the oracle."""
+    print("[case] PBiCGStab breakdown restart", flush=True)
+    c = prepare(tmp_path / "dev", "stresscase_channel", channel_velocity)
+    o = prepare(tmp_path / "orc", "stresscase_channel", channel_velocity)
+    runs = []
+    for path, oracle in ((c, False), (o, True)):
+        sc = stressfoam.StressCase(path, oracle=oracle)
+        for md in sc.modes:
+            md.theta[:] = 0.0
+            vals = np.tile([0.1, 0.02, 0.0, 0.05, 0.0, 0.0], (10, 1))
+            md.bc = [(p, t, vals if t == stressfoam.BC_FV else v) for p, t, v in md.bc]
+            if oracle:
+                md.ev[:] = np.eye(3)
+                md.vec[:] = np.eye(3)
+                md.obcval[:10] = vals
+            else:
+                eye = np.broadcast_to(np.eye(3), (len(md.theta), 3, 3))
+                md.unit.set_state(md.theta, eye, eye)  # the fixture's eigen files were loaded
+                for patch, t, v in md.bc:
+                    md.unit.set_bc(patch, t, v)
+        sc.step()
+        runs.append(sc)
+    d, o_ = runs[0].state()[0], runs[1].state()[0]
+    restarts = 0
+    for md in runs[0].modes:
+        perf = md.unit.solver_perf()
+        assert all(p.converged == 1 for p in perf if p.solved), perf
+        restarts += perf[1].nRefinements
+        assert rel(d[md.name]["theta"], o_[md.name]["theta"]) < 1e-9
+    assert restarts >= 1  # an xy solve broke down and was restarted
