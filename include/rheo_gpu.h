@@ -162,7 +162,12 @@ typedef struct {
  *   constitutiveEq::New(name, U, phi, dict)   newConstitutiveEq.C:32-61
  * keyed by dict.lookup("type"): "Oldroyd-BLog" (Oldroyd_BLog.H:132),
  * "GiesekusLog" (GiesekusLog.H:133), "FENE-PLog" (FENE_PLog.H:133),
- * "PTTLog" (PTTLog.H:167, destructionFunctionType linear|exponential).
+ * "PTTLog" (PTTLog.H:167, destructionFunctionType linear|exponential),
+ * "FENE-CRLog" (FENE_CRLog.H:134), "WhiteMetznerCYLog" (WhiteMetznerCYLog.H:148),
+ * "Rolie-PolyLog" (RoliePolyLog.H:146), "XPomPomLog" (XPomPomLog.H:146),
+ * "SaramitoLog" (SaramitoLog.H:137; its plasticity factor reads the model's
+ * tau before each step, so give tau to rt_model_set_state, as the reference
+ * reads it MUST_READ), "multiMode" (multiMode.C:37).
  * `dict` is the text of the `parameters { ... }` sub-dictionary of
  * constant/constitutiveProperties in OpenFOAM syntax, dimension sets
  * allowed (e.g. "type Oldroyd-BLog; rho rho [1 -3 0 0 0 0 0] 1; etaS ...;

@@ -513,6 +513,31 @@ void orc_grad_gauss(const orc_mesh* m, const double* U, const double* Ub, double
 
 void orc_rhs(const orc_params* p, int32_t n, const double* gradU, const double* theta,
              const double* eigVals, const double* eigVecs, double* S, double* fOut) {
+    orc_rhs_tau(p, n, gradU, theta, eigVals, eigVecs, nullptr, S, fOut);
+}
+
+// SaramitoLog.C:181-212: the plasticity factor from the second invariant of
+// the deviatoric part of tau (over the valid dimensions)
+static double saramitoFac(const orc_params* p, const double* tau) {
+    double t[6] = {0, 0, 0, 0, 0, 0};
+    if (tau)
+        for (int q = 0; q < 6; ++q) t[q] = tau[q];
+    const double nDims = p->dimX + p->dimY + p->dimZ;
+    const double tr = t[0] + t[3] + t[5];
+    double d[6] = {t[0] - (p->dimX * tr) / nDims, t[1] - (0.0 * tr) / nDims, t[2] - (0.0 * tr) / nDims,
+                   t[3] - (p->dimY * tr) / nDims, t[4] - (0.0 * tr) / nDims, t[5] - (p->dimZ * tr) / nDims};
+    const double magSqr = d[0] * d[0] + 2 * (d[1] * d[1]) + 2 * (d[2] * d[2]) + d[3] * d[3] + 2 * (d[4] * d[4]) +
+                          d[5] * d[5];
+    const double tauDMag = std::sqrt(magSqr) / std::sqrt(2.);
+    const double small = 1e-16;
+    if (p->sarN == 1.) return std::max(0., (tauDMag - p->tau0) / (p->sarK * tauDMag + small));
+    return std::pow(std::max(0., (tauDMag - p->tau0) / (p->sarK * std::pow(tauDMag, p->sarN) * 1. + small)),
+                    1. / p->sarN);
+}
+
+void orc_rhs_tau(const orc_params* p, int32_t n, const double* gradU, const double* theta,
+                 const double* eigVals, const double* eigVecs, const double* tauOld, double* S,
+                 double* fOut) {
     bool mask[3] = {false, false, false};
     if (p->ofInvQuirk && n > 0) invShiftMask(eigVals, mask);
     const bool ptt = p->model == ORC_PTT_LINEAR || p->model == ORC_PTT_EXP;
@@ -556,7 +581,25 @@ void orc_rhs(const orc_params* p, int32_t n, const double* gradU, const double* 
         T9 invL, X;
         invField(Lam, mask, invL);
         const double rl = 1.0 / p->lambda;
-        if (p->model == ORC_OLDROYDB) {
+        if (p->model == ORC_SARAMITO) {
+            // SaramitoLog.C:231-275: (fac etaP/lambda) [PTT g(tr C)] V (inv(Lam) - I) V^T
+            const double fac = saramitoFac(p, tauOld ? tauOld + 6 * (size_t)c : nullptr);
+            if (fOut) fOut[c] = fac;
+            T9 D, Y;
+            for (int q = 0; q < 9; ++q) D[q] = invL[q] - I9[q];
+            innerP(V, D, false, Y);
+            double coef = (fac * p->etaP) / p->lambda;
+            if (p->sarPTT != 0) {
+                T9 VT, t1, C;
+                transp(V, VT);
+                dotTT(V, Lam, t1);
+                dotTT(t1, VT, C);
+                const double trC = C[0] + C[4] + C[8];
+                const double z = (p->epsilon / (1 - p->zeta)) * (trC - 3.);
+                coef = coef * (p->sarPTT == 1 ? 1. + z : std::exp(z));
+            }
+            for (int q = 0; q < 9; ++q) X[q] = coef * Y[q];
+        } else if (p->model == ORC_OLDROYDB) {
             T9 D, Y;
             for (int q = 0; q < 9; ++q) D[q] = invL[q] - I9[q];
             innerP(V, D, false, Y);
@@ -1060,8 +1103,8 @@ void orc_tau(const orc_params* p, int32_t n, const double* eigVals, const double
             coef = p->etaP / p->lambda;
         } else {
             for (int q = 0; q < 9; ++q) D[q] = C[q] - I9[q];
-            if (p->model == ORC_PTT_LINEAR || p->model == ORC_PTT_EXP)
-                coef = p->etaP / (p->lambda * (1 - p->zeta));
+            if (p->model == ORC_PTT_LINEAR || p->model == ORC_PTT_EXP || p->model == ORC_SARAMITO)
+                coef = p->etaP / (p->lambda * (1 - p->zeta));  // PTTLog.C, SaramitoLog.C:292
             else if (p->model == ORC_FENECR)  // FENE_CRLog.C:176
                 coef = (p->etaP / p->lambda) * fOld[c];
             else if (p->model == ORC_WMCY)  // WhiteMetznerCYLog.C:187, etaP/lambda of the old L
@@ -1119,7 +1162,7 @@ void orc_correct_iter(const orc_mesh* m, const orc_params* p, const double* U, c
         gradU = L.data();
     }
     std::vector<double> S(6 * (size_t)N), f(N, 0.0);
-    orc_rhs(p, N, gradU, theta, eigVals, eigVecs, S.data(), f.data());
+    orc_rhs_tau(p, N, gradU, theta, eigVals, eigVecs, tau, S.data(), f.data());
     std::vector<double> diag(N), lower(F), upper(F), source(6 * (size_t)N), iC(6 * (size_t)NB),
         bC(6 * (size_t)NB);
     std::vector<double> souT;

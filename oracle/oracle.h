@@ -38,7 +38,7 @@ typedef struct {
 } orc_mesh;
 
 enum { ORC_OLDROYDB = 0, ORC_GIESEKUS = 1, ORC_FENEP = 2, ORC_PTT_LINEAR = 3, ORC_PTT_EXP = 4,
-       ORC_FENECR = 5, ORC_WMCY = 6, ORC_ROLIEPOLY = 7, ORC_XPP = 8 };
+       ORC_FENECR = 5, ORC_WMCY = 6, ORC_ROLIEPOLY = 7, ORC_XPP = 8, ORC_SARAMITO = 9 };
 
 typedef struct {
     int32_t model;
@@ -54,6 +54,11 @@ typedef struct {
     double lambdaS, lambdaB, q, nXPP;
     /* White-Metzner Carreau-Yasuda (WhiteMetznerCYLog.C:115-124): m n K L a b */
     double wmM, wmN, wmK, wmL, wmA, wmB;
+    /* Saramito (SaramitoLog.C:117-170): yield stress tau0, power n, consistency
+     * k (etaP when n = 1), the valid dimensions, PTT function 0 none / 1
+     * linear / 2 exponential (n = 1 only); epsilon, zeta above */
+    double tau0, sarN, sarK, dimX, dimY, dimZ;
+    int32_t sarPTT;
 } orc_params;
 
 enum { ORC_BC_ZEROGRADIENT = 0, ORC_BC_FIXEDVALUE = 1, ORC_BC_EMPTY = 2 };
@@ -84,6 +89,13 @@ void orc_grad_gauss(const orc_mesh* m, const double* U, const double* Ub, double
  * etaP/lambda of White-Metzner. */
 void orc_rhs(const orc_params* p, int32_t n, const double* gradU, const double* theta,
              const double* eigVals, const double* eigVecs, double* S, double* fOut);
+
+/* orc_rhs with the model's current tau [6n] (SaramitoLog.C:181-212 reads it
+ * for the plasticity factor; NULL reads as zero).  fOut receives Saramito's
+ * factor fac. */
+void orc_rhs_tau(const orc_params* p, int32_t n, const double* gradU, const double* theta,
+                 const double* eigVals, const double* eigVecs, const double* tauOld, double* S,
+                 double* fOut);
 
 /* Euler ddt + upwind GaussDefCmpw div + (== S) assembly
  * (GDC/gaussDefCmpwConvectionScheme.C:70-118, [OF9] EulerDdtScheme, fvMatrix
@@ -173,7 +185,9 @@ void orc_tau(const orc_params* p, int32_t n, const double* eigVals, const double
 
 /* One full constitutiveEq::correct() of a *Log model (Oldroyd_BLog.C:127-179).
  * gradU may be NULL (then Gauss linear from U/Ub).  theta/eigVals/eigVecs
- * are updated in place, tau written.  relaxFactor <= 0 disables relax(). */
+ * are updated in place; tau holds the model's current tau on entry (read by
+ * Saramito's plasticity factor) and the new tau on return.  relaxFactor <= 0
+ * disables relax(). */
 void orc_correct(const orc_mesh* m, const orc_params* p, const double* U, const double* Ub,
                  const double* phi, const double* gradU, double deltaT, const int32_t* bcType,
                  const double* bcValue, const int32_t validComponents[6],

@@ -20,10 +20,11 @@ BC_ZEROGRADIENT, BC_FIXEDVALUE, BC_EMPTY = 0, 1, 2
 PBICGSTAB, PBICG = 0, 1
 PC_NONE, PC_DIAGONAL, PC_DILU = 0, 1, 2
 
-FENECR, WMCY, ROLIEPOLY, XPP = 5, 6, 7, 8
+FENECR, WMCY, ROLIEPOLY, XPP, SARAMITO = 5, 6, 7, 8, 9
 MODEL_IDS = {"Oldroyd-BLog": OLDROYDB, "GiesekusLog": GIESEKUS, "FENE-PLog": FENEP,
              "PTTLog": PTT_LINEAR, "FENE-CRLog": FENECR, "WhiteMetznerCYLog": WMCY,
-             "Rolie-PolyLog": ROLIEPOLY, "XPomPomLog": XPP}
+             "Rolie-PolyLog": ROLIEPOLY, "XPomPomLog": XPP, "SaramitoLog": SARAMITO}
+SARAMITO_PTT = {"none": 0, "linear": 1, "exponential": 2}
 
 
 class Mesh(C.Structure):
@@ -44,7 +45,10 @@ class Params(C.Structure):
                 ("lambdaS", C.c_double), ("lambdaB", C.c_double), ("q", C.c_double),
                 ("nXPP", C.c_double),
                 ("wmM", C.c_double), ("wmN", C.c_double), ("wmK", C.c_double), ("wmL", C.c_double),
-                ("wmA", C.c_double), ("wmB", C.c_double)]
+                ("wmA", C.c_double), ("wmB", C.c_double),
+                ("tau0", C.c_double), ("sarN", C.c_double), ("sarK", C.c_double),
+                ("dimX", C.c_double), ("dimY", C.c_double), ("dimZ", C.c_double),
+                ("sarPTT", C.c_int32)]
 
 
 class Controls(C.Structure):
@@ -71,6 +75,7 @@ def lib():
         vp = C.c_void_p
         L.orc_grad_gauss.argtypes = [vp] * 4
         L.orc_rhs.argtypes = [vp, C.c_int32] + [vp] * 6
+        L.orc_rhs_tau.argtypes = [vp, C.c_int32] + [vp] * 7
         L.orc_assemble.argtypes = [vp, vp, vp, vp, C.c_double] + [vp] * 8
         L.orc_relax.argtypes = [vp, vp, C.c_double] + [vp] * 6
         L.orc_solve_segregated.argtypes = [vp] * 12
@@ -118,15 +123,19 @@ def params(model="Oldroyd-BLog", etaS=0.59, etaP=0.41, lam=1.0, alpha=0.0, L2=10
            epsilon=0.0, zeta=0.0, rho=1.0, ptt_exp=False, of_inv_quirk=False,
            lambdaR=1.0, lambdaD=1.0, beta=0.0, delta=-0.5, chiMax=0.0,
            lambdaS=1.0, lambdaB=1.0, q=1.0, n=0.0, m=1.0, K=0.0, Lc=0.0, a=2.0, b=2.0,
-           wm_n=1.0):
+           wm_n=1.0, tau0=0.0, sar_n=1.0, sar_k=None, dims=(1.0, 1.0, 1.0), ptt_function="none"):
     """Model constants.  Rolie-Poly: lambdaR lambdaD beta delta chiMax; XPP:
-    lambdaS lambdaB alpha q n; White-Metzner CY: lam m wm_n K Lc(=L) a b."""
+    lambdaS lambdaB alpha q n; White-Metzner CY: lam m wm_n K Lc(=L) a b;
+    Saramito: tau0 sar_n sar_k (etaP when sar_n = 1) dims ptt_function
+    (n = 1 only) epsilon zeta."""
     mid = MODEL_IDS[model]
     if model == "PTTLog" and ptt_exp:
         mid = PTT_EXP
+    k = etaP if (sar_k is None or sar_n == 1.0) else sar_k
+    ptt = SARAMITO_PTT[ptt_function] if sar_n == 1.0 else 0
     return Params(mid, rho, etaS, etaP, lam, alpha, L2, epsilon, zeta, int(of_inv_quirk),
                   lambdaR, lambdaD, beta, delta, chiMax, lambdaS, lambdaB, q, n,
-                  m, wm_n, K, Lc, a, b)
+                  m, wm_n, K, Lc, a, b, tau0, sar_n, k, *[float(x) for x in dims], ptt)
 
 
 def controls(solver=PBICGSTAB, precond=PC_DILU, tol=1e-10, relTol=0.0, maxIter=1000, minIter=0):
@@ -202,13 +211,16 @@ def tau_total(m, U, Ub, tau, etaS):
     return out
 
 
-def rhs(p, gradU, theta, eigVals, eigVecs):
+def rhs(p, gradU, theta, eigVals, eigVecs, tau_old=None):
+    """The log-form source S and the per-cell factor; tau_old: the model's
+    current tau (Saramito's plasticity factor), None reads as zero."""
     n = theta.shape[0]
     S = np.empty((n, 6))
     f = np.zeros(n)
-    lib().orc_rhs(C.byref(p), n, _p(np.ascontiguousarray(gradU)), _p(np.ascontiguousarray(theta)),
-                  _p(np.ascontiguousarray(eigVals)), _p(np.ascontiguousarray(eigVecs)), _p(S),
-                  _p(f))
+    t = None if tau_old is None else np.ascontiguousarray(tau_old, dtype=np.float64)
+    lib().orc_rhs_tau(C.byref(p), n, _p(np.ascontiguousarray(gradU)), _p(np.ascontiguousarray(theta)),
+                      _p(np.ascontiguousarray(eigVals)), _p(np.ascontiguousarray(eigVecs)), _p(t), _p(S),
+                      _p(f))
     return S, f
 
 
@@ -269,11 +281,12 @@ def tau(p, eigVals, eigVecs, fOld=None):
 
 
 def correct(m, p, fields, dt, ctl=None, bc=None, bcval=None, gradU=None, relax=0.0,
-            use_grad=False, scheme="upwind", theta_old=None):
+            use_grad=False, scheme="upwind", theta_old=None, tau_old=None):
     """One constitutiveEq::correct(); returns (theta, eigVals, eigVecs, tau, perf).
     scheme: the GaussDefCmpw limiter of div(phi,theta), optionally prefixed by
     "bounded " (OpenFOAM's bounded convection wrapper).  theta_old: the
-    step's theta.oldTime() for an inner iteration (None: a new step)."""
+    step's theta.oldTime() for an inner iteration (None: a new step).
+    tau_old: the model's tau before the step (Saramito reads it; None: zero)."""
     om = OracleMesh(m)
     bc = default_bc(m) if bc is None else bc
     NB = m.nBoundary
@@ -282,7 +295,7 @@ def correct(m, p, fields, dt, ctl=None, bc=None, bcval=None, gradU=None, relax=0
     th = fields.theta.copy()
     ev = fields.eigVals.copy()
     vec = fields.eigVecs.copy()
-    t = np.empty_like(th)
+    t = np.zeros_like(th) if tau_old is None else np.array(tau_old, dtype=np.float64, order="C")
     perf = (Perf * 6)()
     g = gradU if gradU is not None else (fields.gradU if use_grad else None)
     Ub = fields.Ub if NB else np.zeros((1, 3))

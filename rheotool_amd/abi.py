@@ -360,7 +360,8 @@ class ConstitutiveEq:
 # dimensions of the further models' constants (RoliePolyLog.C:117-121,
 # XPomPomLog.C:118-122, WhiteMetznerCYLog.C:119-124)
 _DIMS = {"lambdaR": "[0 0 1 0 0 0 0]", "lambdaD": "[0 0 1 0 0 0 0]", "lambdaS": "[0 0 1 0 0 0 0]",
-         "lambdaB": "[0 0 1 0 0 0 0]", "K": "[0 0 1 0 0 0 0]", "L": "[0 0 1 0 0 0 0]"}
+         "lambdaB": "[0 0 1 0 0 0 0]", "K": "[0 0 1 0 0 0 0]", "L": "[0 0 1 0 0 0 0]",
+         "tau0": "[1 -1 -2 0 0 0 0]", "k": "[1 -1 -1 0 0 0 0]"}
 
 
 def model_dict(model="Oldroyd-BLog", etaS=0.59, etaP=0.41, lam=1.0, alpha=None, L2=None,
@@ -373,7 +374,12 @@ def model_dict(model="Oldroyd-BLog", etaS=0.59, etaP=0.41, lam=1.0, alpha=None, 
     if model not in ("Rolie-PolyLog", "XPomPomLog"):
         lines.append(f"lambda lambda [0 0 1 0 0 0 0] {lam!r};")
     for k, v in extra.items():
-        lines.append(f"{k} {k} {_DIMS.get(k, '[0 0 0 0 0 0 0]')} {v!r};")
+        if isinstance(v, str):  # a word, e.g. Saramito's PTTfunction
+            lines.append(f"{k} {v};")
+        elif isinstance(v, (tuple, list)):  # a vector, e.g. Saramito's dims
+            lines.append(f"{k} ({' '.join(repr(float(x)) for x in v)});")
+        else:
+            lines.append(f"{k} {k} {_DIMS.get(k, '[0 0 0 0 0 0 0]')} {v!r};")
     if alpha is not None:
         lines.append(f"alpha alpha [0 0 0 0 0 0 0] {alpha!r};")
     if L2 is not None:

@@ -68,7 +68,8 @@ __device__ __forceinline__ void invDiag(const double* l, int mask, double* r) {
 // model (ModelKind), fixed at compile time so only its branch holds registers.
 template <int KIND>
 __device__ __forceinline__ void cell_rhs(const ModelParams& p, int invMask, const double* L, const double* th6,
-                                         const double* lam, const double* V, double* S, double& fOut) {
+                                         const double* lam, const double* V, const double* tau6, double* S,
+                                         double& fOut) {
     constexpr bool ptt = KIND == M_PTT_LINEAR || KIND == M_PTT_EXP;
     double LT[9];
 #pragma unroll
@@ -152,6 +153,33 @@ __device__ __forceinline__ void cell_rhs(const ModelParams& p, int invMask, cons
         const double fac = KIND == M_PTT_LINEAR ? 1. + z : exp(z);
 #pragma unroll
         for (int q = 0; q < 9; ++q) X[q] = (rl * Y[q]) * fac;
+    } else if constexpr (KIND == M_SARAMITO) {
+        // SaramitoLog.C:181-275: plasticity factor from the second invariant of
+        // the deviator (over the valid dimensions) of the tau before the step,
+        // then (fac etaP/lambda) [PTT g(tr C)] V (inv(Lam) - I) V^T
+        const double nD = p.dimX + p.dimY + p.dimZ;
+        const double tr = tau6[0] + tau6[3] + tau6[5];
+        const double d0 = tau6[0] - (p.dimX * tr) / nD, d3 = tau6[3] - (p.dimY * tr) / nD,
+                     d5 = tau6[5] - (p.dimZ * tr) / nD;
+        const double magSqr = d0 * d0 + 2 * (tau6[1] * tau6[1]) + 2 * (tau6[2] * tau6[2]) + d3 * d3 +
+                              2 * (tau6[4] * tau6[4]) + d5 * d5;
+        const double tauDMag = sqrt(magSqr) / sqrt(2.);
+        const double fac = p.sarN == 1. ? fmax(0., (tauDMag - p.tau0) / (p.sarK * tauDMag + 1e-16))
+                                        : pow(fmax(0., (tauDMag - p.tau0) / (p.sarK * pow(tauDMag, p.sarN) * 1. + 1e-16)),
+                                              1. / p.sarN);
+        fOut = fac;
+        double coef = (fac * p.etaP) / p.lambda;
+        if (p.sarPTT != 0) {
+            double C[9];
+            vdvt(V, lam, C);
+            const double z = (p.epsilon / (1 - p.zeta)) * (C[0] + C[4] + C[8] - 3.);
+            coef = coef * (p.sarPTT == 1 ? 1. + z : exp(z));
+        }
+        const double dd[3] = {il[0] - 1.0, il[1] - 1.0, il[2] - 1.0};
+        double Y[9];
+        vdvt(V, dd, Y);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) X[q] = coef * Y[q];
     } else if constexpr (KIND == M_FENECR) {
         // FENE_CRLog.C:142-164: X = (1/lambda) f V (inv(Lam) - I) V^T, f = L2/(L2 - tr C)
         double C[9], Y[9];
@@ -368,7 +396,11 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3, 8))
             for (int qq = 0; qq < 3; ++qq) lam[qq] = a.eigVal[(int64_t)qq * N + r];
 #pragma unroll
             for (int qq = 0; qq < 9; ++qq) V[qq] = a.eigVec[(int64_t)qq * N + r];
-            cell_rhs<KIND>(a.prm, a.invMask, g, th, lam, V, S, f);
+            double to6[6] = {0, 0, 0, 0, 0, 0};
+            if constexpr (KIND == M_SARAMITO) {
+                if (a.tauOld) ld6(a.tauOld + (int64_t)NCMP * r, to6);
+            }
+            cell_rhs<KIND>(a.prm, a.invMask, g, th, lam, V, to6, S, f);
         }
         if (a.fOld) a.fOld[r] = f;
         // matrix diagonal: ddt + div, optional relax, then boundary internalCoeffs
@@ -564,8 +596,8 @@ __global__ void __launch_bounds__(BLK) k_eig_tau(int32_t N, ModelParams p, const
     } else {
 #pragma unroll
         for (int q = 0; q < 9; ++q) D[q] = C[q] - I9[q];
-        if (p.kind == M_PTT_LINEAR || p.kind == M_PTT_EXP) {
-            coef = p.etaP / (p.lambda * (1 - p.zeta));
+        if (p.kind == M_PTT_LINEAR || p.kind == M_PTT_EXP || p.kind == M_SARAMITO) {
+            coef = p.etaP / (p.lambda * (1 - p.zeta));  // PTTLog.C, SaramitoLog.C:292
         } else if (p.kind == M_FENECR) {  // FENE_CRLog.C:176, f of the old state
             coef = (p.etaP / p.lambda) * fOld[r];
         } else if (p.kind == M_WMCY) {  // WhiteMetznerCYLog.C:187, etaP/lambda of the old L
@@ -1041,6 +1073,9 @@ static void launch_assemble_kind(Ctx* ctx, const AssembleArgs& a, int grid) {
     case M_ROLIEPOLY:
         hipLaunchKernelGGL((k_assemble<T, M_ROLIEPOLY>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
         break;
+    case M_SARAMITO:
+        hipLaunchKernelGGL((k_assemble<T, M_SARAMITO>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
+        break;
     default:
         hipLaunchKernelGGL((k_assemble<T, M_XPP>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
         break;
@@ -1049,10 +1084,12 @@ static void launch_assemble_kind(Ctx* ctx, const AssembleArgs& a, int grid) {
 
 void launch_assemble(Ctx* ctx, const AssembleArgs& a, int grid) {
     // algorithmic bytes per cell: theta 48, eigen state 96, U 32, V 8, b 48, diag 8, 1/diag 8,
-    // r and r0 (2 x 6 x w), theta.oldTime() 48 more in inner iterations; per ELL entry: face id 4, slot 1, column 4, phi 8, value 8
+    // r and r0 (2 x 6 x w), theta.oldTime() 48 more in inner iterations, the old tau 48 for
+    // Saramito; per ELL entry: face id 4, slot 1, column 4, phi 8, value 8
     // (+ Sf 24 and w 8 for the gradient)
     const double w = a.mixed ? 4 : 8;
-    const double by = (double)a.N * (48 + 96 + 32 + 8 + 48 + 8 + 8 + 12 * w + (a.thetaOld ? 48 : 0)) +
+    const double by = (double)a.N * (48 + 96 + 32 + 8 + 48 + 8 + 8 + 12 * w + (a.thetaOld ? 48 : 0) +
+                                     (a.tauOld ? 48 : 0)) +
                       (double)a.nEll * (4 + 1 + 4 + 8 + 8 + (a.gradU ? 0 : 24 + 8));
     {
         ProfScope ps(ctx, a.mixed ? "k_assemble<f32>" : "k_assemble<f64>", by);

@@ -211,7 +211,7 @@ Controls parseControls(const char* text) {
 namespace {
 
 const char* kTypes[] = {"FENE-CRLog", "FENE-PLog",          "GiesekusLog", "Oldroyd-BLog", "PTTLog",
-                        "Rolie-PolyLog", "WhiteMetznerCYLog", "XPomPomLog", "multiMode"};
+                        "Rolie-PolyLog", "SaramitoLog", "WhiteMetznerCYLog", "XPomPomLog", "multiMode"};
 
 double need(const Dict& d, const char* key, const std::string& dictName) {
     double v;
@@ -232,6 +232,7 @@ void parseModel(Model& M, const Dict& d, const std::string& dn) {
     else if (type == "WhiteMetznerCYLog") p.kind = M_WMCY;
     else if (type == "Rolie-PolyLog") p.kind = M_ROLIEPOLY;
     else if (type == "XPomPomLog") p.kind = M_XPP;
+    else if (type == "SaramitoLog") p.kind = M_SARAMITO;
     else {
         std::string msg = "Unknown constitutiveEq type " + type + "\n\nValid constitutiveEq types are :\n" +
                           std::to_string(sizeof(kTypes) / sizeof(kTypes[0])) + "\n(\n";
@@ -242,9 +243,30 @@ void parseModel(Model& M, const Dict& d, const std::string& dn) {
     p.rho = need(d, "rho", dn);
     p.etaS = need(d, "etaS", dn);
     p.etaP = need(d, "etaP", dn);
-    if (p.kind == M_PTT_LINEAR) {
+    if (p.kind == M_PTT_LINEAR || p.kind == M_SARAMITO) {
         p.epsilon = need(d, "epsilon", dn);
         p.zeta = need(d, "zeta", dn);
+    }
+    if (p.kind == M_SARAMITO) {  // SaramitoLog.C:117-170
+        p.tau0 = need(d, "tau0", dn);
+        p.sarN = need(d, "n", dn);
+        p.sarK = p.sarN == 1. ? need(d, "etaP", dn) : need(d, "k", dn);
+        std::vector<double> dims;
+        if (!d.vector("dims", dims) || dims.size() != 3)
+            throw Error{RT_E_BAD_ARG, "keyword dims is undefined in dictionary " + dn};
+        p.dimX = dims[0], p.dimY = dims[1], p.dimZ = dims[2];
+        p.sarPTT = 0;
+        if (p.sarN == 1.) {
+            std::string f;
+            if (!d.word("PTTfunction", f))
+                throw Error{RT_E_BAD_ARG, "keyword PTTfunction is undefined in dictionary " + dn};
+            if (f == "none") p.sarPTT = 0;
+            else if (f == "linear") p.sarPTT = 1;
+            else if (f == "exponential") p.sarPTT = 2;
+            else
+                throw Error{RT_E_BAD_ARG, "\nThe PTT function specified does not exist.\n\nAvailable PTT functions "
+                                          "are:\n\n. none\n. linear\n. exponential"};
+        }
     }
     // the keys each constructor looks up (e.g. RoliePolyLog.C:114-121, XPomPomLog.C:115-122)
     if (p.kind == M_ROLIEPOLY) {
@@ -411,6 +433,7 @@ void modelCorrectBegin(Model& M, double deltaT, const double* gradU, int iterati
     a.theta = M.theta.as<double>();
     a.eigVal = M.eigVal.as<double>();
     a.eigVec = M.eigVec.as<double>();
+    a.tauOld = M.prm.kind == M_SARAMITO ? M.tau.as<double>() : nullptr;
     a.rDeltaT = 1.0 / deltaT;
     a.relax = M.relax;
     a.prm = M.prm;

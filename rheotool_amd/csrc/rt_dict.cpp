@@ -109,6 +109,26 @@ bool Dict::word(const std::string& k, std::string& out) const {
     return true;
 }
 
+bool Dict::vector(const std::string& k, std::vector<double>& out) const {
+    auto it = entries.find(k);
+    if (it == entries.end()) return false;
+    out.clear();
+    bool open = false;
+    for (const auto& tok : it->second) {
+        if (tok == "(") {
+            if (open) return false;  // nested lists are not vectors
+            open = true;
+        } else if (tok == ")") {
+            return open;
+        } else if (open) {
+            double v;
+            if (!isNumber(tok, v)) return false;
+            out.push_back(v);
+        }
+    }
+    return false;
+}
+
 bool Dict::dictList(const std::string& k, std::vector<std::pair<std::string, Dict>>& out) const {
     auto it = entries.find(k);
     if (it == entries.end()) return false;

@@ -19,6 +19,8 @@ struct Dict {
     bool scalar(const std::string& k, double& out) const;
     // word: the first token of the entry
     bool word(const std::string& k, std::string& out) const;
+    // vector: the numbers of the entry's first parenthesised list, `key (x y z);`
+    bool vector(const std::string& k, std::vector<double>& out) const;
     std::vector<std::string> keys() const;
     // a list of named sub-dictionaries, `key ( name1 { ... } name2 { ... } );`
     // (the `models` entry of multiMode); false when absent or malformed

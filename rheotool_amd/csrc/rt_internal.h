@@ -232,7 +232,7 @@ struct Krylov {
 enum DivScheme { DS_UPWIND = 0, DS_CUBISTA = 1, DS_MINMOD = 2, DS_SMART = 3, DS_WACEB = 4, DS_SUPERBEE = 5,
                  DS_NONE = 6 };
 enum ModelKind { M_OLDROYDB = 0, M_GIESEKUS = 1, M_FENEP = 2, M_PTT_LINEAR = 3, M_PTT_EXP = 4,
-                 M_FENECR = 5, M_WMCY = 6, M_ROLIEPOLY = 7, M_XPP = 8 };
+                 M_FENECR = 5, M_WMCY = 6, M_ROLIEPOLY = 7, M_XPP = 8, M_SARAMITO = 9 };
 // models whose tau needs a per-cell factor of the old state (Model::fOld)
 inline bool modelKeepsFactor(int kind) { return kind == M_FENEP || kind == M_FENECR || kind == M_WMCY; }
 
@@ -244,6 +244,9 @@ struct ModelParams {
     double lambdaR = 1, lambdaD = 1, beta = 0, delta = 0, chiMax = 0;  // Rolie-Poly
     double lambdaS = 1, lambdaB = 1, q = 1, nXPP = 0;                 // XPP (+ alpha)
     double wmM = 1, wmN = 1, wmK = 0, wmL = 0, wmA = 1, wmB = 1;      // White-Metzner CY
+    // Saramito: yield stress, power, consistency, valid dimensions, PTT function
+    double tau0 = 0, sarN = 1, sarK = 0, dimX = 1, dimY = 1, dimZ = 1;
+    int sarPTT = 0;  // 0 none, 1 linear, 2 exponential
 };
 
 struct Model {
@@ -311,6 +314,7 @@ struct AssembleArgs {
     const double* thetaOld;  // [N][6] theta.oldTime() for inner iterations, NULL: theta
     const double* eigVal;
     const double* eigVec;
+    const double* tauOld;  // [N][6] the model's tau before the step (Saramito), else NULL
     double rDeltaT;
     double relax;         // <= 0: none
     ModelParams prm;

@@ -223,6 +223,7 @@ static constitutiveEq::adddictionaryConstructorToTable<FENE_CRLog> addFENECR("FE
 static constitutiveEq::adddictionaryConstructorToTable<WhiteMetznerCYLog> addWMCY("WhiteMetznerCYLog");
 static constitutiveEq::adddictionaryConstructorToTable<RoliePolyLog> addRP("Rolie-PolyLog");
 static constitutiveEq::adddictionaryConstructorToTable<XPomPomLog> addXPP("XPomPomLog");
+static constitutiveEq::adddictionaryConstructorToTable<SaramitoLog> addSaramito("SaramitoLog");
 static constitutiveEq::adddictionaryConstructorToTable<multiMode> addMulti("multiMode");
 
 multiMode::multiMode(const word& name, const volVectorField& U, const surfaceScalarField& phi,

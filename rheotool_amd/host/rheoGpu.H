@@ -263,6 +263,7 @@ RHEOGPU_LOG_MODEL(FENE_CRLog, "FENE-CRLog")
 RHEOGPU_LOG_MODEL(WhiteMetznerCYLog, "WhiteMetznerCYLog")
 RHEOGPU_LOG_MODEL(RoliePolyLog, "Rolie-PolyLog")
 RHEOGPU_LOG_MODEL(XPomPomLog, "XPomPomLog")
+RHEOGPU_LOG_MODEL(SaramitoLog, "SaramitoLog")
 
 //- IOdictionary wrapper reading constitutiveProperties (constitutiveModel.C:41-59)
 class constitutiveModel {

@@ -222,7 +222,15 @@ class Runner:
                      ("a", "a"), ("b", "b")):
             if k in e and not isinstance(e[k], foamdict.FoamDict):
                 kw[o] = e.scalar(k)
-        if "n" in e:
+        if t == "SaramitoLog":  # SaramitoLog.C:117-170
+            kw.update(tau0=e.scalar("tau0"), sar_n=e.scalar("n"),
+                      dims=tuple(float(x) for x in e["dims"][0]))
+            kw.pop("K", None)
+            if kw["sar_n"] == 1.0:
+                kw["ptt_function"] = e.word("PTTfunction")
+            else:
+                kw["sar_k"] = e.scalar("k")
+        elif "n" in e:
             kw["n" if t == "XPomPomLog" else "wm_n"] = e.scalar("n")
         if t == "PTTLog":
             # PTTLog.C:139 reads destructionFunctionType (no default)

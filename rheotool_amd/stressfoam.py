@@ -13,6 +13,7 @@ path, on a case directory:
   writeControl/writeInterval, timePrecision);
 - <start>/theta<mode> (MUST_READ) with zeroGradient, fixedValue and empty
   patches; eigVals<mode>/eigVecs<mode> when present (READ_IF_PRESENT);
+  tau<mode> for SaramitoLog (MUST_READ: its plasticity factor reads it);
 - U and, when present, phi from the velocity time directory (default: the
   start time).  Patch velocities: the patch's `value` entry, the face cell's
   value for zeroGradient, zero for noSlip.  Without a phi file the flux is
@@ -149,6 +150,12 @@ class StressCase:
             md.bc = self._theta_bc(fld)
             md.eigVals = self._opt_tensor(t0, "eigVals" + md.name)
             md.eigVecs = self._opt_tensor(t0, "eigVecs" + md.name)
+            # SaramitoLog reads its tau (MUST_READ, SaramitoLog.C:51-62) for the
+            # plasticity factor of the first step
+            md.tau0 = None
+            tp = os.path.join(t0, "tau" + md.name)
+            if md.params.word("type") == "SaramitoLog":
+                md.tau0 = polymesh.read_field(tp, self.pm).internal.copy()
         ut = os.path.join(path, u_time if u_time is not None else self.ctl.startName)
         self.U, self.Ub = self._velocity(ut)
         self.phi = self._flux(ut)
@@ -231,7 +238,7 @@ class StressCase:
         units = self.eq.modes() or [self.eq]
         for md, u in zip(self.modes, units):
             md.unit = u
-            u.set_state(md.theta, md.eigVals, md.eigVecs)
+            u.set_state(md.theta, md.eigVals, md.eigVecs, md.tau0)
             if md.solver:
                 u.set_solver(md.solver)
             u.set_relax(md.relax)
@@ -263,7 +270,7 @@ class StressCase:
                     s = self.m.patchStart[patch]
                     bcval[s:s + len(vals)] = vals
             md.obc, md.obcval = bc, bcval
-            md.tau = np.zeros((N, 6))
+            md.tau = np.zeros((N, 6)) if md.tau0 is None else md.tau0.copy()
         self.fields = Fields(U=self.U, Ub=self.Ub, phi=self.phi, gradU=None,
                              theta=None, eigVals=None, eigVecs=None)
         self.octl = O.controls()
@@ -274,7 +281,7 @@ class StressCase:
             f = dataclasses.replace(self.fields, theta=md.theta, eigVals=md.ev, eigVecs=md.vec)
             th, ev, vec, tau, _ = self.O.correct(self.m, md.oparams, f, dt, ctl=self.octl,
                                                  bc=md.obc, bcval=md.obcval, relax=md.relax,
-                                                 scheme=_oracle_scheme(md.scheme))
+                                                 scheme=_oracle_scheme(md.scheme), tau_old=md.tau)
             md.theta, md.ev, md.vec, md.tau = th, ev, vec, tau
         return 0
 

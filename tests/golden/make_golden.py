@@ -273,9 +273,29 @@ def model_rhs_more(model, C, gradU, p):
         lambda(gd) = lambda (1 + (Lc gd)^b)^((m-1)/b), gd = sqrt(2)|sym(gradU)|;
     Rolie-Poly  R = -(C - I)/lambdaD - M1 (C + beta (trC/3)^delta (C - I)),
         M1 = 2 (1 - sqrt(3/trC))/lambdaR [x the chiMax factor];
-    XPP  R = -(C (f - 2 alpha) + alpha C^2 + (alpha - 1) I)/lambdaB."""
+    XPP  R = -(C (f - 2 alpha) + alpha C^2 + (alpha - 1) I)/lambdaB;
+    Saramito (elastoviscoplastic; Saramito 2007, Herschel-Bulkley form 2009)
+        R = -(G/eta_p) F g(tr C) (C - I) with G = etaP/lambda and the
+        plasticity factor F = (max(0, |tau_d| - tau0) / (k |tau_d|^n))^(1/n)
+        of the previous stress (|tau_d| = sqrt(tau_d:tau_d / 2), tau_d the
+        deviator over the valid dimensions), k = etaP for n = 1, and g the
+        optional PTT function 1 + eps/(1-zeta)(tr C - 3) or its exponential."""
     I = np.eye(3)
     trC = np.trace(C)
+    if model == "SaramitoLog":
+        dims = np.array(p["dims"], dtype=float)
+        tau = p["tauOld"]
+        td = tau - np.diag(dims) * np.trace(tau) / dims.sum()
+        mag = np.sqrt(np.sum(td * td) / 2)
+        k = p["etaP"] if p["sar_n"] == 1 else p["sar_k"]
+        F = (max(0.0, mag - p["tau0"]) / (k * mag ** p["sar_n"] + 1e-300)) ** (1 / p["sar_n"])
+        g = 1.0
+        z = p.get("epsilon", 0.0) / (1 - p.get("zeta", 0.0)) * (trC - 3)
+        if p.get("ptt_function", "none") == "linear":
+            g = 1 + z
+        elif p.get("ptt_function", "none") == "exponential":
+            g = np.exp(z)
+        return -(p["etaP"] / p["lambda"]) * F * g * (C - I)
     if model == "FENE-CRLog":
         f = p["L2"] / (p["L2"] - trC)
         return -(f / p["lambda"]) * (C - I)
@@ -307,19 +327,33 @@ MODELS_MORE = {
     "Rolie-PolyLog-nochi": dict(lambdaR=0.4, lambdaD=1.5, beta=0.3, delta=-0.5, chiMax=0.0),
     "XPomPomLog": dict(lambdaS=0.4, lambdaB=1.6, alpha=0.1, q=3.0, n=0.0),
     "XPomPomLog-n2": dict(lambdaS=0.5, lambdaB=1.2, alpha=0.15, q=2.0, n=2.0),
+    "SaramitoLog": dict(lam=1.3, tau0=0.3, sar_n=1.0, dims=(1, 1, 1)),
+    "SaramitoLog-hb2d": dict(lam=0.8, tau0=0.25, sar_n=0.75, sar_k=1.5, dims=(1, 1, 0)),
+    "SaramitoLog-lin": dict(lam=1.1, tau0=0.2, sar_n=1.0, dims=(1, 1, 1), ptt_function="linear",
+                            epsilon=0.2, zeta=0.1),
+    "SaramitoLog-exp": dict(lam=0.9, tau0=0.35, sar_n=1.0, dims=(1, 1, 1), ptt_function="exponential",
+                            epsilon=0.15),
 }
+
+
+def base_model(key):
+    """The model type of a fixture key (keys carry a variant suffix)."""
+    if key.startswith("SaramitoLog"):
+        return "SaramitoLog"
+    return key.split("-nochi")[0].split("-n2")[0]
 
 
 def make_rhs_more():
     rng = np.random.default_rng(20261203)
     out = {}
     for key, prm in MODELS_MORE.items():
-        model = key.split("-nochi")[0].split("-n2")[0]
+        model = base_model(key)
         p = dict(prm)
         p["lambda"] = p.pop("lam", 1.0)
+        p.setdefault("etaP", 0.41)  # the value the tests give O.params
         if "wm_n" in p:
             p["n"] = p["wm_n"]
-        ths, evs, vecs, Ls, S = [], [], [], [], []
+        ths, evs, vecs, Ls, S, taus = [], [], [], [], [], []
         while len(ths) < 60:
             R = rand_rot(rng)
             ln = np.sort(rng.uniform(-1.5, 1.5, 3))
@@ -328,6 +362,11 @@ def make_rhs_more():
             C = R @ np.diag(np.exp(ln)) @ R.T
             th = R @ np.diag(ln) @ R.T
             L = rng.standard_normal((3, 3))
+            if model == "SaramitoLog":
+                # previous stress: about a third below the yield stress
+                A = rng.standard_normal((3, 3)) * p["tau0"] * rng.uniform(0.2, 3.0)
+                p["tauOld"] = 0.5 * (A + A.T)
+                taus.append(sym6(p["tauOld"]))
             w, V = np.linalg.eigh(th)
             kap = L.T
             E = kap @ C + C @ kap.T + model_rhs_more(model, C, L, p)
@@ -342,6 +381,8 @@ def make_rhs_more():
         out[k + "_eigVecs"] = np.array(vecs)
         out[k + "_gradU"] = np.array(Ls)
         out[k + "_S"] = np.array(S)
+        if taus:
+            out[k + "_tauOld"] = np.array(taus)
     np.savez_compressed(os.path.join(HERE, "rhs_more.npz"), **out)
 
 

@@ -462,6 +462,96 @@ def test_further_log_models_explicit_step(ctx, model):
     assert rel_l2(g["tau"], o["tau"]) < 1e-12
 
 
+SARAMITO = {
+    "bingham": (dict(tau0=0.3, n=1.0, epsilon=0.0, zeta=0.0, PTTfunction="none", dims=(1, 1, 1)),
+                dict(tau0=0.3, sar_n=1.0, dims=(1, 1, 1))),
+    "hb2d": (dict(tau0=0.25, n=0.75, k=1.5, epsilon=0.0, zeta=0.0, dims=(1, 1, 0)),
+             dict(tau0=0.25, sar_n=0.75, sar_k=1.5, dims=(1, 1, 0))),
+    "pttlin": (dict(tau0=0.2, n=1.0, epsilon=0.2, zeta=0.1, PTTfunction="linear", dims=(1, 1, 1)),
+               dict(tau0=0.2, sar_n=1.0, epsilon=0.2, zeta=0.1, ptt_function="linear", dims=(1, 1, 1))),
+    "pttexp": (dict(tau0=0.35, n=1.0, epsilon=0.15, zeta=0.0, PTTfunction="exponential", dims=(1, 1, 1)),
+               dict(tau0=0.35, sar_n=1.0, epsilon=0.15, ptt_function="exponential", dims=(1, 1, 1))),
+}
+
+
+def _saramito_tau(n, seed):
+    """A previous stress with about a third of the cells below the yield stress."""
+    rng = np.random.default_rng(seed)
+    A = rng.normal(size=(n, 3, 3)) * rng.uniform(0.05, 0.9, size=(n, 1, 1))
+    A = 0.5 * (A + np.swapaxes(A, 1, 2))
+    return np.stack([A[:, 0, 0], A[:, 0, 1], A[:, 0, 2], A[:, 1, 1], A[:, 1, 2], A[:, 2, 2]], 1)
+
+
+@pytest.mark.parametrize("variant", list(SARAMITO))
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_saramito_matches_oracle_over_steps(ctx, variant, precision):
+    """SaramitoLog (SaramitoLog.C:181-294): the plasticity factor reads the
+    model's tau before the step (given on the first step, the step's own
+    result afterwards).  Three steps on a permuted jittered box, theta and tau
+    within 1e-9 per step of the oracle fed the same tau."""
+    import dataclasses
+    dev, orc = SARAMITO[variant]
+    m = cases.cartesian(12, 10, 9, jitter=0.1, perm_seed=4)
+    f = cases.make_fields(m, cases.FIELDS_TG, seed=8, lnLo=-1.5, lnHi=1.5)
+    tau0 = _saramito_tau(m.nCells, 3)
+    print(f"[case] SaramitoLog {variant} N={m.nCells} precision={precision}", flush=True)
+    dm = ctx.mesh(m)
+    eq = ctx.constitutiveEq("", dm, abi.model_dict("SaramitoLog", lam=1.2, **dev))
+    eq.set_solver(abi.solver_dict(1e-10, precision=precision))
+    eq.set_state(f.theta, f.eigVals, f.eigVecs, tau0)
+    eq.set_flow(f.U, f.Ub, f.phi)
+    p = O.params("SaramitoLog", etaS=0.59, etaP=0.41, lam=1.2, **orc)
+    cur, tau = f, tau0
+    for step in range(3):
+        assert eq.correct(2e-3) == abi.RT_OK, eq.solver_perf()
+        th, ev, vec, tau, perf = O.correct(m, p, cur, 2e-3, tau_old=tau)
+        assert rel_l2(eq.theta(), th) < THETA_TOL * (step + 1)
+        assert rel_l2(eq.tau(), tau) < THETA_TOL * (step + 1)
+        cur = dataclasses.replace(cur, theta=th, eigVals=ev, eigVecs=vec)
+    # the factor matters: a model with tau0 far above every stress is Oldroyd-B-like with
+    # no relaxation at all, and differs
+    p_hi = O.params("SaramitoLog", etaS=0.59, etaP=0.41, lam=1.2, **dict(orc, tau0=1e3))
+    th_hi = O.correct(m, p_hi, f, 2e-3, tau_old=tau0)[0]
+    th_1 = O.correct(m, p, f, 2e-3, tau_old=tau0)[0]
+    assert rel_l2(th_1 - f.theta, th_hi - f.theta) > 1e-4
+
+
+@pytest.mark.parametrize("variant", ["bingham", "hb2d", "pttexp"])
+def test_saramito_explicit_step(ctx, variant):
+    """phi = 0: theta_new = theta_old + dt S, to 1e-12 (device pow/exp may
+    differ from the host library's in the last ulp)."""
+    dev, orc = SARAMITO[variant]
+    m, f = cases.box3d(10, lnLo=-1.5, lnHi=1.5)
+    f.phi[:] = 0.0
+    tau0 = _saramito_tau(m.nCells, 5)
+    dm = ctx.mesh(m)
+    eq = ctx.constitutiveEq("", dm, abi.model_dict("SaramitoLog", lam=1.2, **dev))
+    eq.set_solver(abi.solver_dict(1e-10))
+    eq.set_state(f.theta, f.eigVals, f.eigVecs, tau0)
+    eq.set_flow(f.U, f.Ub, f.phi)
+    assert eq.correct(1e-2) == abi.RT_OK
+    p = O.params("SaramitoLog", etaS=0.59, etaP=0.41, lam=1.2, **orc)
+    th, ev, vec, tau, _ = O.correct(m, p, f, 1e-2, tau_old=tau0)
+    g = eq.theta()
+    assert np.abs(g - th).max() / np.abs(th).max() < 1e-12
+    assert rel_l2(eq.tau(), tau) < 1e-12
+
+
+def test_saramito_dictionary_errors(ctx):
+    m, f = cases.channel2d(8)
+    dm = ctx.mesh(m)
+    base = dict(tau0=0.3, n=1.0, epsilon=0.0, zeta=0.0, dims=(1, 1, 0))
+    with pytest.raises(abi.RheoError) as e:
+        ctx.constitutiveEq("", dm, abi.model_dict("SaramitoLog", **base, PTTfunction="quadratic"))
+    assert "PTT function specified does not exist" in e.value.msg
+    with pytest.raises(abi.RheoError) as e:  # n = 1 needs PTTfunction
+        ctx.constitutiveEq("", dm, abi.model_dict("SaramitoLog", **base))
+    assert "PTTfunction" in e.value.msg
+    with pytest.raises(abi.RheoError) as e:  # n != 1 needs k
+        ctx.constitutiveEq("", dm, abi.model_dict("SaramitoLog", **dict(base, n=0.5)))
+    assert "keyword k" in e.value.msg
+
+
 def _mode_entry(name, model, **kw):
     body = abi.model_dict(model, **kw).split("{", 1)[1].rsplit("}", 1)[0]
     return f"{name} {{{body}}}"

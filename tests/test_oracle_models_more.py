@@ -1,6 +1,6 @@
-"""Oracle pinning for FENE-CR, White-Metzner Carreau-Yasuda, Rolie-Poly and
-XPP (the further log-conformation models: FENE_CRLog.C, WhiteMetznerCYLog.C,
-RoliePolyLog.C, XPomPomLog.C).  The fixture holds the log-conformation
+"""Oracle pinning for FENE-CR, White-Metzner Carreau-Yasuda, Rolie-Poly, XPP
+and Saramito (the further log-conformation models: FENE_CRLog.C,
+WhiteMetznerCYLog.C, RoliePolyLog.C, XPomPomLog.C, SaramitoLog.C).  The fixture holds the log-conformation
 right-hand side obtained from each model's conformation-tensor equation
 through the Daleckii-Krein derivative of the matrix logarithm (an
 independent formulation), written by tests/golden/make_golden.py."""
@@ -10,26 +10,27 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
-from tests.golden.make_golden import MODELS_MORE
+from tests.golden.make_golden import MODELS_MORE, base_model
 
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-
-
-def base(key):
-    return key.split("-nochi")[0].split("-n2")[0]
 
 
 def params(key, **over):
     prm = dict(MODELS_MORE[key])
     prm.update(over)
-    return O.params(base(key), etaP=0.41, **prm)
+    return O.params(base_model(key), etaP=0.41, **prm)
+
+
+def tau_old(d, k):
+    return d[k + "_tauOld"] if k + "_tauOld" in d.files else None
 
 
 @pytest.mark.parametrize("key", list(MODELS_MORE))
 def test_rhs_matches_daleckii_krein(key):
     d = np.load(os.path.join(HERE, "rhs_more.npz"))
     k = key.replace("-", "_")
-    S, _ = O.rhs(params(key), d[k + "_gradU"], d[k + "_theta"], d[k + "_eigVals"], d[k + "_eigVecs"])
+    S, _ = O.rhs(params(key), d[k + "_gradU"], d[k + "_theta"], d[k + "_eigVals"], d[k + "_eigVecs"],
+                 tau_old(d, k))
     ref = d[k + "_S"]
     err = np.abs(S - ref).max(1) / (1 + np.abs(ref).max(1))
     assert err.max() < 1e-10, err.max()
@@ -38,13 +39,31 @@ def test_rhs_matches_daleckii_krein(key):
 @pytest.mark.parametrize("key,over", [("FENE-CRLog", dict(L2=60.0)),
                                       ("WhiteMetznerCYLog", dict(m=0.7)),
                                       ("Rolie-PolyLog", dict(beta=0.4)),
-                                      ("XPomPomLog", dict(q=2.5))])
+                                      ("XPomPomLog", dict(q=2.5)),
+                                      ("SaramitoLog", dict(tau0=0.33)),
+                                      ("SaramitoLog-hb2d", dict(dims=(1, 1, 1))),
+                                      ("SaramitoLog-hb2d", dict(sar_n=0.8)),
+                                      ("SaramitoLog-lin", dict(ptt_function="exponential")),
+                                      ("SaramitoLog-exp", dict(zeta=0.1))])
 def test_planted_parameter_mistake_is_caught(key, over):
     d = np.load(os.path.join(HERE, "rhs_more.npz"))
     k = key.replace("-", "_")
     S, _ = O.rhs(params(key, **over), d[k + "_gradU"], d[k + "_theta"], d[k + "_eigVals"],
-                 d[k + "_eigVecs"])
+                 d[k + "_eigVecs"], tau_old(d, k))
     assert np.abs(S - d[k + "_S"]).max() > 1e-6
+
+
+def test_saramito_fixture_spans_both_sides_of_the_yield_stress():
+    """Some samples are below tau0 (the factor vanishes and S is the upper-
+    convected part alone), some above; the factor is returned per cell."""
+    d = np.load(os.path.join(HERE, "rhs_more.npz"))
+    S, fac = O.rhs(params("SaramitoLog"), d["SaramitoLog_gradU"], d["SaramitoLog_theta"],
+                   d["SaramitoLog_eigVals"], d["SaramitoLog_eigVecs"], d["SaramitoLog_tauOld"])
+    assert (fac == 0).sum() >= 5 and (fac > 0).sum() >= 5
+    # below the yield stress the model term is absent: zero tau -> the same S
+    z, fz = O.rhs(params("SaramitoLog"), d["SaramitoLog_gradU"], d["SaramitoLog_theta"],
+                  d["SaramitoLog_eigVals"], d["SaramitoLog_eigVecs"], None)
+    assert np.all(fz == 0) and np.array_equal(S[fac == 0], z[fac == 0])
 
 
 def _state(rng, n=20):
@@ -104,3 +123,7 @@ def test_tau_closed_forms():
     p = params("XPomPomLog")
     t = O.tau(p, ev, vec, None)
     assert np.allclose(t, _sym(0.41 / 1.6 * (C - I)), rtol=1e-12, atol=1e-13)
+    # Saramito: etaP / (lambda (1 - zeta)) (C - I), with and without zeta
+    for key, lam, zeta in (("SaramitoLog", 1.3, 0.0), ("SaramitoLog-lin", 1.1, 0.1)):
+        t = O.tau(params(key), ev, vec, None)
+        assert np.allclose(t, _sym(0.41 / (lam * (1 - zeta)) * (C - I)), rtol=1e-12, atol=1e-13)
