@@ -237,6 +237,17 @@ int rt_model_set_scheme(rt_model* model, const char* divScheme);
 int rt_model_mode_count(const rt_model* model);
 rt_model* rt_model_mode(rt_model* model, int i);
 
+/* BMPLog's fluidity equation (BMPLog.C:98-111 Phi<name> MUST_READ, 151-163:
+ * ddt + div(phi,Phi) == -Sp(1/lambda) + Phi0/lambda + k (PhiInf - Phi)(tau &&
+ * symm(L))), solved before theta in every rt_model_correct, as a handle of its
+ * own for its fvSolution/fvSchemes entries (`Phi`, `div(phi,Phi)`): it takes
+ * rt_model_set_state (theta = Phi [N]; eigen state and tau ignored),
+ * rt_model_set_bc (fixedValue values [n], one per face), rt_model_set_solver,
+ * rt_model_set_scheme, rt_model_set_relax, rt_model_get(RT_THETA) -> Phi [N]
+ * and rt_model_solver_perf (component 0).  The flow comes from the BMP model.
+ * NULL for other models; owned by the BMP model. */
+rt_model* rt_model_fluidity(rt_model* model);
+
 /* ------------------------------------------- consumers of tau (momentum eq.)
  * The explicit stress term constitutiveEq::divTau hands to the momentum
  * equation, fvc::div(tau()/rho(), "div(tau)") with Gauss linear

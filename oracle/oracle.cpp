@@ -513,7 +513,7 @@ void orc_grad_gauss(const orc_mesh* m, const double* U, const double* Ub, double
 
 void orc_rhs(const orc_params* p, int32_t n, const double* gradU, const double* theta,
              const double* eigVals, const double* eigVecs, double* S, double* fOut) {
-    orc_rhs_tau(p, n, gradU, theta, eigVals, eigVecs, nullptr, S, fOut);
+    orc_rhs_tau(p, n, gradU, theta, eigVals, eigVecs, nullptr, nullptr, S, fOut);
 }
 
 // SaramitoLog.C:181-212: the plasticity factor from the second invariant of
@@ -536,13 +536,28 @@ static double saramitoFac(const orc_params* p, const double* tau) {
 }
 
 void orc_rhs_tau(const orc_params* p, int32_t n, const double* gradU, const double* theta,
-                 const double* eigVals, const double* eigVecs, const double* tauOld, double* S,
-                 double* fOut) {
+                 const double* eigVals, const double* eigVecs, const double* tauOld, const double* aux,
+                 double* S, double* fOut) {
     bool mask[3] = {false, false, false};
     if (p->ofInvQuirk && n > 0) invShiftMask(eigVals, mask);
     const bool ptt = p->model == ORC_PTT_LINEAR || p->model == ORC_PTT_EXP;
     for (int c = 0; c < n; ++c) {
         const double* L = gradU + 9 * (size_t)c;
+        if (p->model == ORC_FLUIDITY) {
+            // BMPLog.C:152-160: Phi0/lambda + k (PhiInf - Phi) (tau && symm(L)); the
+            // implicit -Sp(1/lambda, Phi) goes to the diagonal
+            double t[6] = {0, 0, 0, 0, 0, 0}, d[6];
+            if (tauOld)
+                for (int q = 0; q < 6; ++q) t[q] = tauOld[6 * (size_t)c + q];
+            symmT(L, d);
+            const double tdd = t[0] * d[0] + 2 * (t[1] * d[1]) + 2 * (t[2] * d[2]) + t[3] * d[3] +
+                               2 * (t[4] * d[4]) + t[5] * d[5];
+            const double Phi = theta[6 * (size_t)c];
+            double* s = S + 6 * (size_t)c;
+            s[0] = p->Phi0 / p->lambda + (p->kBMP * (p->PhiInf - Phi)) * tdd;
+            for (int q = 1; q < 6; ++q) s[q] = 0.0;
+            continue;
+        }
         const double* V = eigVecs + 9 * (size_t)c;
         const double* Lam = eigVals + 9 * (size_t)c;
         // boilerLog.H:25-31: M = V^T (L^T [- zeta symm(L)]) V
@@ -581,7 +596,15 @@ void orc_rhs_tau(const orc_params* p, int32_t n, const double* gradU, const doub
         T9 invL, X;
         invField(Lam, mask, invL);
         const double rl = 1.0 / p->lambda;
-        if (p->model == ORC_SARAMITO) {
+        if (p->model == ORC_BMP) {
+            // BMPLog.C:166-187: (Phi G0) V (inv(Lam) - I) V^T with the step's fluidity
+            const double Phi = aux ? aux[6 * (size_t)c] : 0.0;
+            T9 D, Y;
+            for (int q = 0; q < 9; ++q) D[q] = invL[q] - I9[q];
+            innerP(V, D, false, Y);
+            const double coef = Phi * p->G0;
+            for (int q = 0; q < 9; ++q) X[q] = coef * Y[q];
+        } else if (p->model == ORC_SARAMITO) {
             // SaramitoLog.C:231-275: (fac etaP/lambda) [PTT g(tr C)] V (inv(Lam) - I) V^T
             const double fac = saramitoFac(p, tauOld ? tauOld + 6 * (size_t)c : nullptr);
             if (fOut) fOut[c] = fac;
@@ -1105,6 +1128,8 @@ void orc_tau(const orc_params* p, int32_t n, const double* eigVals, const double
             for (int q = 0; q < 9; ++q) D[q] = C[q] - I9[q];
             if (p->model == ORC_PTT_LINEAR || p->model == ORC_PTT_EXP || p->model == ORC_SARAMITO)
                 coef = p->etaP / (p->lambda * (1 - p->zeta));  // PTTLog.C, SaramitoLog.C:292
+            else if (p->model == ORC_BMP)  // BMPLog.C:199
+                coef = p->G0;
             else if (p->model == ORC_FENECR)  // FENE_CRLog.C:176
                 coef = (p->etaP / p->lambda) * fOld[c];
             else if (p->model == ORC_WMCY)  // WhiteMetznerCYLog.C:187, etaP/lambda of the old L
@@ -1154,6 +1179,16 @@ void orc_correct_iter(const orc_mesh* m, const orc_params* p, const double* U, c
                       const double* bcValue, const int32_t valid[6], const orc_controls* c,
                       double relaxFactor, int32_t scheme, const double* C, const double* thetaOld,
                       double* theta, double* eigVals, double* eigVecs, double* tau, orc_perf perf[6]) {
+    orc_correct_aux(m, p, U, Ub, phi, gradU, deltaT, bcType, bcValue, valid, c, relaxFactor, scheme, C, thetaOld,
+                    nullptr, theta, eigVals, eigVecs, tau, perf);
+}
+
+void orc_correct_aux(const orc_mesh* m, const orc_params* p, const double* U, const double* Ub,
+                     const double* phi, const double* gradU, double deltaT, const int32_t* bcType,
+                     const double* bcValue, const int32_t valid[6], const orc_controls* c,
+                     double relaxFactor, int32_t scheme, const double* C, const double* thetaOld,
+                     const double* aux, double* theta, double* eigVals, double* eigVecs, double* tau,
+                     orc_perf perf[6]) {
     const int N = m->nCells, F = m->nInternal, NB = m->nBoundary;
     std::vector<double> L;
     if (!gradU) {
@@ -1162,7 +1197,7 @@ void orc_correct_iter(const orc_mesh* m, const orc_params* p, const double* U, c
         gradU = L.data();
     }
     std::vector<double> S(6 * (size_t)N), f(N, 0.0);
-    orc_rhs_tau(p, N, gradU, theta, eigVals, eigVecs, tau, S.data(), f.data());
+    orc_rhs_tau(p, N, gradU, theta, eigVals, eigVecs, tau, aux, S.data(), f.data());
     std::vector<double> diag(N), lower(F), upper(F), source(6 * (size_t)N), iC(6 * (size_t)NB),
         bC(6 * (size_t)NB);
     std::vector<double> souT;
@@ -1188,11 +1223,14 @@ void orc_correct_iter(const orc_mesh* m, const orc_params* p, const double* U, c
     assembleImpl(m, phi, thetaOld ? thetaOld : theta, S.data(), deltaT, bcType, bcValue, scheme,
                  souT.empty() ? nullptr : souT.data(),
                  diag.data(), lower.data(), upper.data(), source.data(), iC.data(), bC.data());
+    if (p->model == ORC_FLUIDITY)  // == -fvm::Sp(1/lambda, Phi) (BMPLog.C:157)
+        for (int cc = 0; cc < N; ++cc) diag[cc] = diag[cc] - (-(m->V[cc] * (1.0 / p->lambda)));
     if (relaxFactor > 0)
         orc_relax(m, bcType, relaxFactor, theta, diag.data(), lower.data(), upper.data(),
                   source.data(), iC.data());
     orc_solve_segregated(m, diag.data(), lower.data(), upper.data(), source.data(), iC.data(),
                          bC.data(), bcType, valid, c, theta, perf);
+    if (p->model == ORC_FLUIDITY) return;
     orc_calc_eig(N, theta, eigVals, eigVecs, 0);
     orc_tau(p, N, eigVals, eigVecs, f.data(), tau);
 }

@@ -38,7 +38,10 @@ typedef struct {
 } orc_mesh;
 
 enum { ORC_OLDROYDB = 0, ORC_GIESEKUS = 1, ORC_FENEP = 2, ORC_PTT_LINEAR = 3, ORC_PTT_EXP = 4,
-       ORC_FENECR = 5, ORC_WMCY = 6, ORC_ROLIEPOLY = 7, ORC_XPP = 8, ORC_SARAMITO = 9 };
+       ORC_FENECR = 5, ORC_WMCY = 6, ORC_ROLIEPOLY = 7, ORC_XPP = 8, ORC_SARAMITO = 9,
+       /* BMPLog (BMPLog.C:142-202): the theta equation, and its scalar fluidity
+        * equation, held in component xx of a [6n] field (the others unused) */
+       ORC_BMP = 10, ORC_FLUIDITY = 11 };
 
 typedef struct {
     int32_t model;
@@ -59,6 +62,9 @@ typedef struct {
      * linear / 2 exponential (n = 1 only); epsilon, zeta above */
     double tau0, sarN, sarK, dimX, dimY, dimZ;
     int32_t sarPTT;
+    /* BMP (BMPLog.C:113-119): modulus, rate constant, fluidity at rest and at
+     * infinite shear; lambda above */
+    double G0, kBMP, Phi0, PhiInf;
 } orc_params;
 
 enum { ORC_BC_ZEROGRADIENT = 0, ORC_BC_FIXEDVALUE = 1, ORC_BC_EMPTY = 2 };
@@ -91,11 +97,12 @@ void orc_rhs(const orc_params* p, int32_t n, const double* gradU, const double* 
              const double* eigVals, const double* eigVecs, double* S, double* fOut);
 
 /* orc_rhs with the model's current tau [6n] (SaramitoLog.C:181-212 reads it
- * for the plasticity factor; NULL reads as zero).  fOut receives Saramito's
- * factor fac. */
+ * for the plasticity factor, the BMP fluidity equation for its source; NULL
+ * reads as zero) and aux [6n] (BMP: the step's fluidity in component xx).
+ * fOut receives Saramito's factor fac. */
 void orc_rhs_tau(const orc_params* p, int32_t n, const double* gradU, const double* theta,
-                 const double* eigVals, const double* eigVecs, const double* tauOld, double* S,
-                 double* fOut);
+                 const double* eigVals, const double* eigVecs, const double* tauOld, const double* aux,
+                 double* S, double* fOut);
 
 /* Euler ddt + upwind GaussDefCmpw div + (== S) assembly
  * (GDC/gaussDefCmpwConvectionScheme.C:70-118, [OF9] EulerDdtScheme, fvMatrix
@@ -213,6 +220,18 @@ void orc_correct_iter(const orc_mesh* m, const orc_params* p, const double* U, c
                       const orc_controls* c, double relaxFactor, int32_t scheme, const double* C,
                       const double* thetaOld, double* theta, double* eigVals, double* eigVecs, double* tau,
                       orc_perf perf[6]);
+
+/* orc_correct_iter with aux [6N] (BMP: the step's fluidity in component xx).
+ * ORC_FLUIDITY solves BMP's fluidity equation held in component xx of theta
+ * (pass validComponents {1,-1,-1,-1,-1,-1}): the implicit 1/lambda joins the
+ * diagonal, tau is the BMP model's tau (read only), and eigVals/eigVecs are
+ * neither read nor written. */
+void orc_correct_aux(const orc_mesh* m, const orc_params* p, const double* U, const double* Ub,
+                     const double* phi, const double* gradU, double deltaT, const int32_t* bcType,
+                     const double* bcValue, const int32_t validComponents[6],
+                     const orc_controls* c, double relaxFactor, int32_t scheme, const double* C,
+                     const double* thetaOld, const double* aux, double* theta, double* eigVals,
+                     double* eigVecs, double* tau, orc_perf perf[6]);
 
 #ifdef __cplusplus
 }

@@ -20,10 +20,10 @@ BC_ZEROGRADIENT, BC_FIXEDVALUE, BC_EMPTY = 0, 1, 2
 PBICGSTAB, PBICG = 0, 1
 PC_NONE, PC_DIAGONAL, PC_DILU = 0, 1, 2
 
-FENECR, WMCY, ROLIEPOLY, XPP, SARAMITO = 5, 6, 7, 8, 9
+FENECR, WMCY, ROLIEPOLY, XPP, SARAMITO, BMP, FLUIDITY = 5, 6, 7, 8, 9, 10, 11
 MODEL_IDS = {"Oldroyd-BLog": OLDROYDB, "GiesekusLog": GIESEKUS, "FENE-PLog": FENEP,
              "PTTLog": PTT_LINEAR, "FENE-CRLog": FENECR, "WhiteMetznerCYLog": WMCY,
-             "Rolie-PolyLog": ROLIEPOLY, "XPomPomLog": XPP, "SaramitoLog": SARAMITO}
+             "Rolie-PolyLog": ROLIEPOLY, "XPomPomLog": XPP, "SaramitoLog": SARAMITO, "BMPLog": BMP}
 SARAMITO_PTT = {"none": 0, "linear": 1, "exponential": 2}
 
 
@@ -48,7 +48,8 @@ class Params(C.Structure):
                 ("wmA", C.c_double), ("wmB", C.c_double),
                 ("tau0", C.c_double), ("sarN", C.c_double), ("sarK", C.c_double),
                 ("dimX", C.c_double), ("dimY", C.c_double), ("dimZ", C.c_double),
-                ("sarPTT", C.c_int32)]
+                ("sarPTT", C.c_int32),
+                ("G0", C.c_double), ("kBMP", C.c_double), ("Phi0", C.c_double), ("PhiInf", C.c_double)]
 
 
 class Controls(C.Structure):
@@ -75,7 +76,7 @@ def lib():
         vp = C.c_void_p
         L.orc_grad_gauss.argtypes = [vp] * 4
         L.orc_rhs.argtypes = [vp, C.c_int32] + [vp] * 6
-        L.orc_rhs_tau.argtypes = [vp, C.c_int32] + [vp] * 7
+        L.orc_rhs_tau.argtypes = [vp, C.c_int32] + [vp] * 8
         L.orc_assemble.argtypes = [vp, vp, vp, vp, C.c_double] + [vp] * 8
         L.orc_relax.argtypes = [vp, vp, C.c_double] + [vp] * 6
         L.orc_solve_segregated.argtypes = [vp] * 12
@@ -91,6 +92,8 @@ def lib():
                                          C.c_double, C.c_int32, vp, vp, vp, vp, vp, vp]
         L.orc_correct_iter.argtypes = [vp, vp, vp, vp, vp, vp, C.c_double, vp, vp, vp, vp,
                                        C.c_double, C.c_int32, vp, vp, vp, vp, vp, vp, vp]
+        L.orc_correct_aux.argtypes = [vp, vp, vp, vp, vp, vp, C.c_double, vp, vp, vp, vp,
+                                      C.c_double, C.c_int32, vp, vp, vp, vp, vp, vp, vp, vp]
         L.orc_grad_gauss_symm.argtypes = [vp] * 4
         L.orc_defc_faces.argtypes = [vp, C.c_int32] + [vp] * 5
         L.orc_defc_sout.argtypes = [vp] * 4
@@ -123,11 +126,12 @@ def params(model="Oldroyd-BLog", etaS=0.59, etaP=0.41, lam=1.0, alpha=0.0, L2=10
            epsilon=0.0, zeta=0.0, rho=1.0, ptt_exp=False, of_inv_quirk=False,
            lambdaR=1.0, lambdaD=1.0, beta=0.0, delta=-0.5, chiMax=0.0,
            lambdaS=1.0, lambdaB=1.0, q=1.0, n=0.0, m=1.0, K=0.0, Lc=0.0, a=2.0, b=2.0,
-           wm_n=1.0, tau0=0.0, sar_n=1.0, sar_k=None, dims=(1.0, 1.0, 1.0), ptt_function="none"):
+           wm_n=1.0, tau0=0.0, sar_n=1.0, sar_k=None, dims=(1.0, 1.0, 1.0), ptt_function="none",
+           G0=1.0, k_bmp=0.0, Phi0=0.0, PhiInf=0.0):
     """Model constants.  Rolie-Poly: lambdaR lambdaD beta delta chiMax; XPP:
     lambdaS lambdaB alpha q n; White-Metzner CY: lam m wm_n K Lc(=L) a b;
     Saramito: tau0 sar_n sar_k (etaP when sar_n = 1) dims ptt_function
-    (n = 1 only) epsilon zeta."""
+    (n = 1 only) epsilon zeta; BMP: lam G0 k_bmp Phi0 PhiInf."""
     mid = MODEL_IDS[model]
     if model == "PTTLog" and ptt_exp:
         mid = PTT_EXP
@@ -135,7 +139,8 @@ def params(model="Oldroyd-BLog", etaS=0.59, etaP=0.41, lam=1.0, alpha=0.0, L2=10
     ptt = SARAMITO_PTT[ptt_function] if sar_n == 1.0 else 0
     return Params(mid, rho, etaS, etaP, lam, alpha, L2, epsilon, zeta, int(of_inv_quirk),
                   lambdaR, lambdaD, beta, delta, chiMax, lambdaS, lambdaB, q, n,
-                  m, wm_n, K, Lc, a, b, tau0, sar_n, k, *[float(x) for x in dims], ptt)
+                  m, wm_n, K, Lc, a, b, tau0, sar_n, k, *[float(x) for x in dims], ptt,
+                  G0, k_bmp, Phi0, PhiInf)
 
 
 def controls(solver=PBICGSTAB, precond=PC_DILU, tol=1e-10, relTol=0.0, maxIter=1000, minIter=0):
@@ -211,15 +216,17 @@ def tau_total(m, U, Ub, tau, etaS):
     return out
 
 
-def rhs(p, gradU, theta, eigVals, eigVecs, tau_old=None):
+def rhs(p, gradU, theta, eigVals, eigVecs, tau_old=None, aux=None):
     """The log-form source S and the per-cell factor; tau_old: the model's
-    current tau (Saramito's plasticity factor), None reads as zero."""
+    current tau (Saramito's plasticity factor, BMP's fluidity source), None
+    reads as zero; aux [n, 6]: BMP's fluidity in column 0."""
     n = theta.shape[0]
     S = np.empty((n, 6))
     f = np.zeros(n)
     t = None if tau_old is None else np.ascontiguousarray(tau_old, dtype=np.float64)
     lib().orc_rhs_tau(C.byref(p), n, _p(np.ascontiguousarray(gradU)), _p(np.ascontiguousarray(theta)),
-                      _p(np.ascontiguousarray(eigVals)), _p(np.ascontiguousarray(eigVecs)), _p(t), _p(S),
+                      _p(np.ascontiguousarray(eigVals)), _p(np.ascontiguousarray(eigVecs)), _p(t),
+                      _p(None if aux is None else np.ascontiguousarray(aux, dtype=np.float64)), _p(S),
                       _p(f))
     return S, f
 
@@ -281,12 +288,14 @@ def tau(p, eigVals, eigVecs, fOld=None):
 
 
 def correct(m, p, fields, dt, ctl=None, bc=None, bcval=None, gradU=None, relax=0.0,
-            use_grad=False, scheme="upwind", theta_old=None, tau_old=None):
+            use_grad=False, scheme="upwind", theta_old=None, tau_old=None, aux=None, valid=None):
     """One constitutiveEq::correct(); returns (theta, eigVals, eigVecs, tau, perf).
     scheme: the GaussDefCmpw limiter of div(phi,theta), optionally prefixed by
     "bounded " (OpenFOAM's bounded convection wrapper).  theta_old: the
     step's theta.oldTime() for an inner iteration (None: a new step).
-    tau_old: the model's tau before the step (Saramito reads it; None: zero)."""
+    tau_old: the model's tau before the step (Saramito reads it; None: zero).
+    aux [N, 6]: BMP's fluidity of the step in column 0.  valid: the solved
+    components (default: the mesh's validComponents)."""
     om = OracleMesh(m)
     bc = default_bc(m) if bc is None else bc
     NB = m.nBoundary
@@ -301,8 +310,40 @@ def correct(m, p, fields, dt, ctl=None, bc=None, bcval=None, gradU=None, relax=0
     Ub = fields.Ub if NB else np.zeros((1, 3))
     Cc = np.ascontiguousarray(m.C)
     to = None if theta_old is None else np.ascontiguousarray(theta_old, dtype=np.float64)
-    lib().orc_correct_iter(om.ref, C.byref(p), _p(fields.U), _p(Ub), _p(fields.phi),
-                           _p(None if g is None else np.ascontiguousarray(g)), dt, _p(bc),
-                           _p(bcval), _p(m.validComponents), C.byref(ctl), relax, scheme_id(scheme),
-                           _p(Cc), _p(to), _p(th), _p(ev), _p(vec), _p(t), perf)
+    va = np.ascontiguousarray(m.validComponents if valid is None else valid, dtype=np.int32)
+    ax = None if aux is None else np.ascontiguousarray(aux, dtype=np.float64)
+    lib().orc_correct_aux(om.ref, C.byref(p), _p(fields.U), _p(Ub), _p(fields.phi),
+                          _p(None if g is None else np.ascontiguousarray(g)), dt, _p(bc),
+                          _p(bcval), _p(va), C.byref(ctl), relax, scheme_id(scheme),
+                          _p(Cc), _p(to), _p(ax), _p(th), _p(ev), _p(vec), _p(t), perf)
     return th, ev, vec, t, [perf[i] for i in range(6)]
+
+
+def bmp_correct(m, p, fields, dt, Phi, tau_old, ctl=None, bc=None, bcval=None, relax=0.0,
+                scheme="upwind", bc_phi=None, bcval_phi=None, relax_phi=0.0, scheme_phi="upwind",
+                ctl_phi=None, gradU=None, theta_old=None, Phi_old=None):
+    """BMPLog::correct (BMPLog.C:142-202): the fluidity equation, then the
+    theta equation with the new fluidity.  Phi [N], bcval_phi [NB] (patch
+    values of fixedValue patches).  Returns (theta, eigVals, eigVecs, tau,
+    perf, Phi_new, perf_phi)."""
+    import dataclasses
+    N, NB = m.nCells, m.nBoundary
+    pf = Params.from_buffer_copy(p)
+    pf.model = FLUIDITY
+    P6 = np.zeros((N, 6))
+    P6[:, 0] = Phi
+    I = np.broadcast_to(np.eye(3), (N, 3, 3)).copy()
+    ff = dataclasses.replace(fields, theta=P6, eigVals=I, eigVecs=I)
+    bv = np.zeros((max(NB, 1), 6))
+    if bcval_phi is not None:
+        bv[:NB, 0] = bcval_phi
+    po = None
+    if Phi_old is not None:
+        po = np.zeros((N, 6))
+        po[:, 0] = Phi_old
+    Pn, _, _, _, perf_phi = correct(m, pf, ff, dt, ctl=ctl_phi, bc=bc_phi, bcval=bv, gradU=gradU,
+                                    relax=relax_phi, scheme=scheme_phi, theta_old=po, tau_old=tau_old,
+                                    valid=[1, -1, -1, -1, -1, -1])
+    th, ev, vec, t, perf = correct(m, p, fields, dt, ctl=ctl, bc=bc, bcval=bcval, gradU=gradU, relax=relax,
+                                   scheme=scheme, theta_old=theta_old, tau_old=tau_old, aux=Pn)
+    return th, ev, vec, t, perf, Pn[:, 0].copy(), perf_phi

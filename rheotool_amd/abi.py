@@ -111,6 +111,7 @@ def lib():
             "rt_model_set_scheme": (i32, [vp, C.c_char_p]),
             "rt_model_mode_count": (i32, [vp]),
             "rt_model_mode": (vp, [vp, i32]),
+            "rt_model_fluidity": (vp, [vp]),
             "rt_model_div_tau": (i32, [vp, vp, vp]),
             "rt_model_tau_total": (i32, [vp, vp]),
             "rt_model_correct": (i32, [vp, C.c_double, vp]),
@@ -273,6 +274,12 @@ class ConstitutiveEq:
         return [ConstitutiveEq(self.ctx, None, self.mesh, None, _handle=C.c_void_p(lib().rt_model_mode(self.h, i)),
                                _owner=self) for i in range(n)]
 
+    def fluidity(self):
+        """BMPLog's fluidity equation (None for other models): scalar state
+        Phi [N], one fixedValue value per face, its own solver/scheme/relax."""
+        h = lib().rt_model_fluidity(self.h)
+        return FluidityEq(self.ctx, None, self.mesh, None, _handle=C.c_void_p(h), _owner=self) if h else None
+
     @property
     def type(self):
         return lib().rt_model_type(self.h).decode()
@@ -362,6 +369,21 @@ class ConstitutiveEq:
 _DIMS = {"lambdaR": "[0 0 1 0 0 0 0]", "lambdaD": "[0 0 1 0 0 0 0]", "lambdaS": "[0 0 1 0 0 0 0]",
          "lambdaB": "[0 0 1 0 0 0 0]", "K": "[0 0 1 0 0 0 0]", "L": "[0 0 1 0 0 0 0]",
          "tau0": "[1 -1 -2 0 0 0 0]", "k": "[1 -1 -1 0 0 0 0]"}
+
+
+class FluidityEq(ConstitutiveEq):
+    """The fluidity equation of a BMPLog model (owned by it)."""
+
+    def set_state(self, Phi, eigVals=None, eigVecs=None, tau=None):
+        keep = _f64(Phi, (self.N,))
+        self.ctx.check(lib().rt_model_set_state(self.h, _p(keep), None, None, None), "set_state")
+
+    def Phi(self):
+        out = np.empty(self.N)
+        self.ctx.check(lib().rt_model_get(self.h, RT_THETA, _p(out)), "get")
+        return out
+
+    theta = Phi
 
 
 def model_dict(model="Oldroyd-BLog", etaS=0.59, etaP=0.41, lam=1.0, alpha=None, L2=None,

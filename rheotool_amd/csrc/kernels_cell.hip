@@ -68,9 +68,22 @@ __device__ __forceinline__ void invDiag(const double* l, int mask, double* r) {
 // model (ModelKind), fixed at compile time so only its branch holds registers.
 template <int KIND>
 __device__ __forceinline__ void cell_rhs(const ModelParams& p, int invMask, const double* L, const double* th6,
-                                         const double* lam, const double* V, const double* tau6, double* S,
-                                         double& fOut) {
+                                         const double* lam, const double* V, const double* tau6, double aux0,
+                                         double* S, double& fOut) {
     constexpr bool ptt = KIND == M_PTT_LINEAR || KIND == M_PTT_EXP;
+    if constexpr (KIND == M_FLUIDITY) {
+        // BMPLog.C:152-160: Phi0/lambda + k (PhiInf - Phi)(tau && symm(L)); the
+        // implicit -Sp(1/lambda) is on the diagonal
+        double d6[6];
+        symmT(L, d6);
+        const double tdd = tau6[0] * d6[0] + 2 * (tau6[1] * d6[1]) + 2 * (tau6[2] * d6[2]) + tau6[3] * d6[3] +
+                           2 * (tau6[4] * d6[4]) + tau6[5] * d6[5];
+        S[0] = p.Phi0 / p.lambda + (p.kBMP * (p.PhiInf - th6[0])) * tdd;
+#pragma unroll
+        for (int q = 1; q < 6; ++q) S[q] = 0.0;
+        fOut = 0.0;
+        return;
+    }
     double LT[9];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -153,6 +166,14 @@ __device__ __forceinline__ void cell_rhs(const ModelParams& p, int invMask, cons
         const double fac = KIND == M_PTT_LINEAR ? 1. + z : exp(z);
 #pragma unroll
         for (int q = 0; q < 9; ++q) X[q] = (rl * Y[q]) * fac;
+    } else if constexpr (KIND == M_BMP) {
+        // BMPLog.C:166-187: (Phi G0) V (inv(Lam) - I) V^T with the step's fluidity
+        const double dd[3] = {il[0] - 1.0, il[1] - 1.0, il[2] - 1.0};
+        double Y[9];
+        vdvt(V, dd, Y);
+        const double coef = aux0 * p.G0;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) X[q] = coef * Y[q];
     } else if constexpr (KIND == M_SARAMITO) {
         // SaramitoLog.C:181-275: plasticity factor from the second invariant of
         // the deviator (over the valid dimensions) of the tau before the step,
@@ -397,10 +418,12 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3, 8))
 #pragma unroll
             for (int qq = 0; qq < 9; ++qq) V[qq] = a.eigVec[(int64_t)qq * N + r];
             double to6[6] = {0, 0, 0, 0, 0, 0};
-            if constexpr (KIND == M_SARAMITO) {
+            if constexpr (KIND == M_SARAMITO || KIND == M_FLUIDITY) {
                 if (a.tauOld) ld6(a.tauOld + (int64_t)NCMP * r, to6);
             }
-            cell_rhs<KIND>(a.prm, a.invMask, g, th, lam, V, to6, S, f);
+            double aux0 = 0.0;
+            if constexpr (KIND == M_BMP) aux0 = a.aux[(int64_t)NCMP * r];
+            cell_rhs<KIND>(a.prm, a.invMask, g, th, lam, V, to6, aux0, S, f);
         }
         if (a.fOld) a.fOld[r] = f;
         // matrix diagonal: ddt + div, optional relax, then boundary internalCoeffs
@@ -414,6 +437,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3, 8))
             divDiag -= vol * (netPhi / vol);
         }
         double D = a.rDeltaT * vol + divDiag;
+        if constexpr (KIND == M_FLUIDITY) D = D - (-(vol * (1.0 / a.prm.lambda)));  // == -fvm::Sp(1/lambda, Phi)
         double src[6];
         double tho[6];  // theta.oldTime() of the step: theta unless this is an inner iteration
         if (a.thetaOld) ld6(a.thetaOld + (int64_t)NCMP * r, tho);
@@ -613,6 +637,8 @@ __global__ void __launch_bounds__(BLK) k_eig_tau(int32_t N, ModelParams p, const
             }
         } else if (p.kind == M_XPP) {  // XPomPomLog.C:180
             coef = p.etaP / p.lambdaB;
+        } else if (p.kind == M_BMP) {  // BMPLog.C:199
+            coef = p.G0;
         } else {
             coef = p.etaP / p.lambda;
         }
@@ -1076,6 +1102,12 @@ static void launch_assemble_kind(Ctx* ctx, const AssembleArgs& a, int grid) {
     case M_SARAMITO:
         hipLaunchKernelGGL((k_assemble<T, M_SARAMITO>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
         break;
+    case M_BMP:
+        hipLaunchKernelGGL((k_assemble<T, M_BMP>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
+        break;
+    case M_FLUIDITY:
+        hipLaunchKernelGGL((k_assemble<T, M_FLUIDITY>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
+        break;
     default:
         hipLaunchKernelGGL((k_assemble<T, M_XPP>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
         break;
@@ -1089,7 +1121,7 @@ void launch_assemble(Ctx* ctx, const AssembleArgs& a, int grid) {
     // (+ Sf 24 and w 8 for the gradient)
     const double w = a.mixed ? 4 : 8;
     const double by = (double)a.N * (48 + 96 + 32 + 8 + 48 + 8 + 8 + 12 * w + (a.thetaOld ? 48 : 0) +
-                                     (a.tauOld ? 48 : 0)) +
+                                     (a.tauOld ? 48 : 0) + (a.aux ? 8 : 0)) +
                       (double)a.nEll * (4 + 1 + 4 + 8 + 8 + (a.gradU ? 0 : 24 + 8));
     {
         ProfScope ps(ctx, a.mixed ? "k_assemble<f32>" : "k_assemble<f64>", by);

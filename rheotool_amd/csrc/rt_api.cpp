@@ -211,7 +211,8 @@ Controls parseControls(const char* text) {
 namespace {
 
 const char* kTypes[] = {"FENE-CRLog", "FENE-PLog",          "GiesekusLog", "Oldroyd-BLog", "PTTLog",
-                        "Rolie-PolyLog", "SaramitoLog", "WhiteMetznerCYLog", "XPomPomLog", "multiMode"};
+                        "Rolie-PolyLog", "SaramitoLog", "WhiteMetznerCYLog", "XPomPomLog", "BMPLog",
+                        "multiMode"};
 
 double need(const Dict& d, const char* key, const std::string& dictName) {
     double v;
@@ -233,6 +234,7 @@ void parseModel(Model& M, const Dict& d, const std::string& dn) {
     else if (type == "Rolie-PolyLog") p.kind = M_ROLIEPOLY;
     else if (type == "XPomPomLog") p.kind = M_XPP;
     else if (type == "SaramitoLog") p.kind = M_SARAMITO;
+    else if (type == "BMPLog") p.kind = M_BMP;
     else {
         std::string msg = "Unknown constitutiveEq type " + type + "\n\nValid constitutiveEq types are :\n" +
                           std::to_string(sizeof(kTypes) / sizeof(kTypes[0])) + "\n(\n";
@@ -283,6 +285,12 @@ void parseModel(Model& M, const Dict& d, const std::string& dn) {
         p.nXPP = need(d, "n", dn);
     } else {
         p.lambda = need(d, "lambda", dn);
+    }
+    if (p.kind == M_BMP) {  // BMPLog.C:113-119
+        p.G0 = need(d, "G0", dn);
+        p.kBMP = need(d, "k", dn);
+        p.Phi0 = need(d, "Phi0", dn);
+        p.PhiInf = need(d, "PhiInf", dn);
     }
     if (p.kind == M_WMCY) {  // WhiteMetznerCYLog.C:119-124
         p.wmM = need(d, "m", dn);
@@ -433,7 +441,10 @@ void modelCorrectBegin(Model& M, double deltaT, const double* gradU, int iterati
     a.theta = M.theta.as<double>();
     a.eigVal = M.eigVal.as<double>();
     a.eigVec = M.eigVec.as<double>();
-    a.tauOld = M.prm.kind == M_SARAMITO ? M.tau.as<double>() : nullptr;
+    a.tauOld = M.prm.kind == M_SARAMITO ? M.tau.as<double>()
+               : M.scalar()                ? M.parentTau->as<double>()
+                                           : nullptr;
+    a.aux = M.prm.kind == M_BMP ? M.fluid->impl.theta.as<double>() : nullptr;
     a.rDeltaT = 1.0 / deltaT;
     a.relax = M.relax;
     a.prm = M.prm;
@@ -446,7 +457,8 @@ void modelCorrectBegin(Model& M, double deltaT, const double* gradU, int iterati
     a.partials = M.kry.partials.as<double>();
     a.cs = st->c;
     a.nnzOut = &st->misc[0];
-    for (int q = 0; q < 6; ++q) a.valid[q] = m.valid[q];
+    for (int q = 0; q < 6; ++q) M.valid6[q] = M.scalar() ? (q == 0 ? 1 : -1) : m.valid[q];
+    for (int q = 0; q < 6; ++q) a.valid[q] = M.valid6[q];
     a.r = mixed ? M.kry.rs.p : M.kry.r.p;
     a.r0 = mixed ? M.kry.r0s.p : M.kry.r0.p;
     a.rD = M.kry.rD.as<double>();
@@ -472,7 +484,7 @@ void modelCorrectBegin(Model& M, double deltaT, const double* gradU, int iterati
     sa.slotMask = M.slotMask.as<uint32_t>();
     sa.b = M.b.as<double>();
     sa.x = M.theta.as<double>();
-    sa.valid = m.valid;
+    sa.valid = M.valid6;
     sa.ctl = &M.ctl;
     sa.xbarKnown = true;
     sa.initDone = true;
@@ -483,6 +495,7 @@ bool modelCorrectAdvance(Model& M, bool block) {
     Ctx* ctx = M.ctx;
     if (!solve_advance(ctx, M.kry, block)) return false;
     solve_finish(M.kry, M.perf);
+    if (M.scalar()) return true;  // the fluidity has no eigen state or tau
     launch_eig_tau(ctx, M.mesh->N, M.prm, M.theta.as<double>(), M.eigVal.as<double>(),
                    M.eigVec.as<double>(), M.tau.as<double>(),
                    modelKeepsFactor(M.prm.kind) ? M.fOld.as<double>() : nullptr);
@@ -728,6 +741,8 @@ void rt_model_destroy(rt_model* model) {
     if (c) ctxUnref(c);
 }
 
+rt_model* rt_model_fluidity(rt_model* model) { return model ? model->impl.fluid : nullptr; }
+
 int rt_model_mode_count(const rt_model* model) {
     return model ? (int)model->impl.modes.size() : 0;
 }
@@ -743,10 +758,16 @@ namespace rt {
 
 void destroyModes(Model& M) {
     for (rt_model* x : M.modes) {
+        destroyModes(x->impl);
         delete x->impl.pendingSolve;
         delete x;
     }
     M.modes.clear();
+    if (M.fluid) {
+        delete M.fluid->impl.pendingSolve;
+        delete M.fluid;
+        M.fluid = nullptr;
+    }
 }
 
 // A model on ctx/mesh from its parameters dictionary.  multiMode
@@ -781,6 +802,28 @@ void createModel(Model& M, Ctx* c, Mesh* mesh, const std::string& name, const Di
         return;
     }
     parseModel(M, d, dn);
+    allocModelState(M);
+    if (M.prm.kind == M_BMP) {
+        // the fluidity equation Phi<name> (BMPLog.C:98-111, 151-163)
+        auto* fl = new rt_model;
+        M.fluid = fl;
+        Model& F = fl->impl;
+        F.ctx = c;
+        F.mesh = mesh;
+        F.name = name;
+        F.typeName = "BMPLog fluidity";
+        F.prm = M.prm;
+        F.prm.kind = M_FLUIDITY;
+        F.parentTau = &M.tau;
+        allocModelState(F);
+    }
+}
+
+// device state of a (non-multiMode) model: fields, flow, boundary conditions,
+// with the reference's defaults (zero tau, identity eigen state, zeroGradient)
+void allocModelState(Model& M) {
+    Ctx* c = M.ctx;
+    const int32_t N = M.mesh->N, NB = M.mesh->NB;
     {
         M.theta = c->alloc((size_t)6 * N * 8);
         M.tau = c->alloc((size_t)6 * N * 8);
@@ -846,8 +889,16 @@ int rt_model_set_state(rt_model* model, const double* theta, const double* eigVa
         RT_HIP(hipSetDevice(c->device));
         DBuf& st = staging(c).buf;
         const int32_t* perm = m.permD.as<int32_t>();
+        std::vector<double> phi6;
+        if (M.scalar()) {  // the fluidity: Phi [N] into component xx
+            phi6.assign((size_t)6 * N, 0.0);
+            for (int32_t i = 0; i < N; ++i) phi6[6 * (size_t)i] = theta[i];
+            theta = phi6.data();
+            eigVals = eigVecs = tau = nullptr;
+        }
         uploadAos(c, st, theta, (int64_t)6 * N);
         launch_permute_rows(c, N, 6, 6, st.as<double>(), perm, M.theta.as<double>());
+        if (!phi6.empty()) RT_HIP(hipStreamSynchronize(c->stream));  // phi6 is local
         if (eigVals) {
             RT_HIP(hipStreamSynchronize(c->stream));
             uploadAos(c, st, eigVals, (int64_t)9 * N);
@@ -886,8 +937,9 @@ int rt_model_set_bc(rt_model* model, int patch, int bcType, const double* values
             if (!values) throw Error{RT_E_BAD_ARG, "rt_model_set_bc: fixedValue needs values"};
             const rt_patch& p = m.patches[patch];
             std::vector<double> col(p.size);
+            const int nc = M.scalar() ? 1 : 6;  // the fluidity takes one value per face
             for (int q = 0; q < 6; ++q) {
-                for (int k = 0; k < p.size; ++k) col[k] = values[6 * (size_t)k + q];
+                for (int k = 0; k < p.size; ++k) col[k] = q < nc ? values[nc * (size_t)k + q] : 0.0;
                 RT_HIP(hipMemcpy(M.bcVal.as<double>() + (int64_t)q * m.NB + p.start, col.data(),
                                  col.size() * 8, hipMemcpyHostToDevice));
             }
@@ -919,6 +971,10 @@ int rt_model_set_flow(rt_model* model, const double* U, const double* Ub, const 
         launch_face_to_entry(c, m, M.phi.as<double>(), M.ePhi.as<double>());
         RT_HIP(hipStreamSynchronize(c->stream));
         M.hasFlow = true;
+        if (M.fluid) {
+            const int rc = rt_model_set_flow(M.fluid, U, Ub, phi);
+            if (rc != RT_OK) throw Error{rc, "fluidity: " + c->lastError};
+        }
     });
 }
 
@@ -1017,11 +1073,14 @@ int rt_model_correct_iter(rt_model* model, double deltaT, const double* gradU, i
     }
     int rc = guarded(M.ctx, [&] {
         RT_HIP(hipSetDevice(M.ctx->device));
+        // BMPLog::correct (BMPLog.C:151-191): the fluidity, then theta with it
+        if (M.fluid) modelCorrect(M.fluid->impl, deltaT, gradU, iteration);
         modelCorrect(M, deltaT, gradU, iteration);
     });
     if (rc != RT_OK) return rc;
     for (int q = 0; q < 6; ++q)
         if (M.perf[q].solved && !M.perf[q].converged) return RT_W_NOT_CONVERGED;
+    if (M.fluid && M.fluid->impl.perf[0].solved && !M.fluid->impl.perf[0].converged) return RT_W_NOT_CONVERGED;
     return RT_OK;
 }
 
@@ -1039,6 +1098,8 @@ int rt_model_get(const rt_model* model, int field, double* out) {
         if (M.isMulti() && field != RT_TAU)
             throw Error{RT_E_BAD_ARG, "rt_model_get: a multiMode model holds only the summed tau; read the "
                                       "modes' fields (rt_model_mode)"};
+        if (M.scalar() && field != RT_THETA)
+            throw Error{RT_E_BAD_ARG, "rt_model_get: the fluidity equation holds only Phi (RT_THETA, [N])"};
         const size_t bytes = (size_t)nc * N * 8;
         if (st.bytes < bytes) st = c->alloc(bytes);
         const int32_t* perm = m.permD.as<int32_t>();
@@ -1046,6 +1107,13 @@ int rt_model_get(const rt_model* model, int field, double* out) {
         else if (field == RT_TAU) launch_unpermute_rows(c, N, 6, 6, M.tau.as<double>(), perm, st.as<double>());
         else if (field == RT_EIGVALS) launch_eigvals_to_tensor(c, N, M.eigVal.as<double>(), perm, st.as<double>());
         else launch_soa_to_aos(c, N, 9, M.eigVec.as<double>(), perm, st.as<double>());
+        if (M.scalar()) {  // Phi is component xx
+            std::vector<double> all((size_t)6 * N);
+            RT_HIP(hipMemcpyAsync(all.data(), st.p, bytes, hipMemcpyDeviceToHost, c->stream));
+            RT_HIP(hipStreamSynchronize(c->stream));
+            for (int32_t i = 0; i < N; ++i) out[i] = all[6 * (size_t)i];
+            return;
+        }
         RT_HIP(hipMemcpyAsync(out, st.p, bytes, hipMemcpyDeviceToHost, c->stream));
         RT_HIP(hipStreamSynchronize(c->stream));
     });

@@ -232,7 +232,10 @@ struct Krylov {
 enum DivScheme { DS_UPWIND = 0, DS_CUBISTA = 1, DS_MINMOD = 2, DS_SMART = 3, DS_WACEB = 4, DS_SUPERBEE = 5,
                  DS_NONE = 6 };
 enum ModelKind { M_OLDROYDB = 0, M_GIESEKUS = 1, M_FENEP = 2, M_PTT_LINEAR = 3, M_PTT_EXP = 4,
-                 M_FENECR = 5, M_WMCY = 6, M_ROLIEPOLY = 7, M_XPP = 8, M_SARAMITO = 9 };
+                 M_FENECR = 5, M_WMCY = 6, M_ROLIEPOLY = 7, M_XPP = 8, M_SARAMITO = 9,
+                 // BMPLog: the theta equation, and its scalar fluidity equation (an
+                 // owned sub-model holding Phi in component xx of its theta)
+                 M_BMP = 10, M_FLUIDITY = 11 };
 // models whose tau needs a per-cell factor of the old state (Model::fOld)
 inline bool modelKeepsFactor(int kind) { return kind == M_FENEP || kind == M_FENECR || kind == M_WMCY; }
 
@@ -247,6 +250,8 @@ struct ModelParams {
     // Saramito: yield stress, power, consistency, valid dimensions, PTT function
     double tau0 = 0, sarN = 1, sarK = 0, dimX = 1, dimY = 1, dimZ = 1;
     int sarPTT = 0;  // 0 none, 1 linear, 2 exponential
+    // BMP: modulus, rate constant, fluidity at rest and at infinite shear
+    double G0 = 1, kBMP = 0, Phi0 = 0, PhiInf = 0;
 };
 
 struct Model {
@@ -284,6 +289,12 @@ struct Model {
     // sum of their tau and forwards the setters
     std::vector<rt_model*> modes;
     bool isMulti() const { return !modes.empty(); }
+    // BMPLog (BMPLog.C:142-163): the fluidity equation, solved before theta each
+    // step; in the fluidity sub-model, the BMP model's tau its source reads
+    rt_model* fluid = nullptr;
+    const DBuf* parentTau = nullptr;
+    bool scalar() const { return prm.kind == M_FLUIDITY; }
+    int32_t valid6[6] = {0, 0, 0, 0, 0, 0};  // components solved in the last step
 };
 
 // ---------------------------------------------------------------- kernels
@@ -314,7 +325,8 @@ struct AssembleArgs {
     const double* thetaOld;  // [N][6] theta.oldTime() for inner iterations, NULL: theta
     const double* eigVal;
     const double* eigVec;
-    const double* tauOld;  // [N][6] the model's tau before the step (Saramito), else NULL
+    const double* tauOld;  // [N][6] the model's tau before the step (Saramito, BMP fluidity), else NULL
+    const double* aux;     // [N][6] BMP: the step's fluidity in component xx, else NULL
     double rDeltaT;
     double relax;         // <= 0: none
     ModelParams prm;
@@ -396,6 +408,7 @@ struct Dict;
 // a model (or a multiMode and its modes) from its parameters dictionary
 void createModel(Model& M, Ctx* c, Mesh* mesh, const std::string& name, const Dict& d, const std::string& dn);
 void destroyModes(Model& M);
+void allocModelState(Model& M);
 void launch_mean6(Ctx* ctx, int32_t N, const double* x, double* partials,
                   CompState* cs, int grid);
 // Lockstep PBiCGStab over the valid components; fills perf.

@@ -92,6 +92,11 @@ void admit(rt_server* S, Request* r, int slot) {
         c->impl.lastError = "rt_server: multiMode requests are not supported (submit one request per mode)";
         rc = RT_E_UNSUPPORTED;
     }
+    if (rc == RT_OK && (rt_model_fluidity(r->model) || r->model->impl.prm.kind == M_SARAMITO)) {
+        // their extra state (BMP's fluidity, Saramito's previous tau) is not part of a request
+        c->impl.lastError = "rt_server: BMPLog and SaramitoLog requests are not supported";
+        rc = RT_E_UNSUPPORTED;
+    }
     if (rc == RT_OK) rc = rt_model_set_solver(r->model, r->controls.c_str());
     if (rc == RT_OK) rc = rt_model_set_state(r->model, r->theta, r->eigVals, r->eigVecs, nullptr);
     if (rc == RT_OK) rc = rt_model_set_flow(r->model, r->U, r->Ub, r->phi);

@@ -224,6 +224,7 @@ static constitutiveEq::adddictionaryConstructorToTable<WhiteMetznerCYLog> addWMC
 static constitutiveEq::adddictionaryConstructorToTable<RoliePolyLog> addRP("Rolie-PolyLog");
 static constitutiveEq::adddictionaryConstructorToTable<XPomPomLog> addXPP("XPomPomLog");
 static constitutiveEq::adddictionaryConstructorToTable<SaramitoLog> addSaramito("SaramitoLog");
+static constitutiveEq::adddictionaryConstructorToTable<BMPLog> addBMP("BMPLog");
 static constitutiveEq::adddictionaryConstructorToTable<multiMode> addMulti("multiMode");
 
 multiMode::multiMode(const word& name, const volVectorField& U, const surfaceScalarField& phi,
@@ -319,10 +320,10 @@ logConformationEq::~logConformationEq() {
 }
 
 void logConformationEq::setState(const volSymmTensorField& theta, const volTensorField* eigVals,
-                                 const volTensorField* eigVecs) {
+                                 const volTensorField* eigVecs, const volSymmTensorField* tau) {
     gpuContext& c = gpuContext::get();
     c.check(rt_model_set_state(model_, theta.internal.data(), eigVals ? eigVals->internal.data() : nullptr,
-                               eigVecs ? eigVecs->internal.data() : nullptr, nullptr),
+                               eigVecs ? eigVecs->internal.data() : nullptr, tau ? tau->internal.data() : nullptr),
             "setState");
     for (size_t p = 0; p < theta.bcType.size(); ++p)
         if (theta.bcType[p] != zeroGradient && theta.bcType[p] != empty) {
@@ -332,9 +333,42 @@ void logConformationEq::setState(const volSymmTensorField& theta, const volTenso
         }
 }
 
+void logConformationEq::setFluidity(const volScalarField& Phi) {
+    gpuContext& c = gpuContext::get();
+    rt_model* fl = rt_model_fluidity(model_);
+    if (!fl) throw FatalError(RT_E_BAD_ARG, "setFluidity: " + type_ + " has no fluidity equation");
+    c.check(rt_model_set_state(fl, Phi.internal.data(), nullptr, nullptr, nullptr), "Phi");
+    for (size_t p = 0; p < Phi.bcType.size(); ++p)
+        if (Phi.bcType[p] != zeroGradient && Phi.bcType[p] != empty) {
+            const patchInfo& P = Phi.mesh->boundary[p];
+            c.check(rt_model_set_bc(fl, (int)p, Phi.bcType[p], Phi.boundary.data() + (size_t)P.start),
+                    "Phi boundary condition");
+        }
+}
+
+volScalarField logConformationEq::fluidity() const {
+    volScalarField f("Phi" + name(), *U().mesh);
+    rt_model* fl = rt_model_fluidity(model_);
+    if (fl) gpuContext::get().check(rt_model_get(fl, RT_THETA, f.internal.data()), "get Phi");
+    return f;
+}
+
 void logConformationEq::correct(const volScalarField*, const volTensorField* gradU) {
     gpuContext& c = gpuContext::get();
     const fvMesh& mesh = *U().mesh;
+    if (rt_model* fl = rt_model_fluidity(model_)) {
+        // BMPLog's fluidity equation: fvSolution Phi, relaxationFactors, div(phi,Phi)
+        const dictionary fd = mesh.solverDict("Phi" + name());
+        if (!fd.text().empty()) c.check(rt_model_set_solver(fl, fd.text().c_str()), "fvSolution Phi");
+        double fr = 0;
+        const dictionary frd = mesh.fvSolution.subDict("relaxationFactors");
+        if (!lookupScalar(frd.subDict("equations").text(), "Phi" + name(), fr))
+            lookupScalar(frd.text(), "Phi" + name(), fr);
+        c.check(rt_model_set_relax(fl, fr), "relax Phi");
+        std::string fs;
+        if (topEntry(mesh.fvSchemes.subDict("divSchemes").text(), "div(phi,Phi" + name() + ")", fs))
+            c.check(rt_model_set_scheme(fl, fs.c_str()), "fvSchemes div(phi,Phi)");
+    }
     c.check(rt_model_set_flow(model_, U().internal.data(), U().boundary.empty() ? nullptr : U().boundary.data(),
                               phi().values.data()),
             "constitutiveEq::correct (U, phi)");

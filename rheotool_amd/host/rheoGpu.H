@@ -199,9 +199,13 @@ public:
     scalar rho() const override { return rho_; }
     bool isGNF() const override { return false; }
     void correct(const volScalarField* alpha = nullptr, const volTensorField* gradU = nullptr) override;
-    //- theta (MUST_READ) and the eigen state (READ_IF_PRESENT) for restarts
+    //- theta (MUST_READ) and the eigen state (READ_IF_PRESENT) for restarts; tau
+    //  for the models that read it (SaramitoLog, BMPLog: MUST_READ)
     void setState(const volSymmTensorField& theta, const volTensorField* eigVals,
-                  const volTensorField* eigVecs);
+                  const volTensorField* eigVecs, const volSymmTensorField* tau = nullptr);
+    //- BMPLog's fluidity Phi<name> (MUST_READ, BMPLog.C:98-111), with its patch conditions
+    void setFluidity(const volScalarField& Phi);
+    volScalarField fluidity() const;
     volSymmTensorField theta() const;
     volTensorField eigVals() const;
     volTensorField eigVecs() const;
@@ -264,6 +268,7 @@ RHEOGPU_LOG_MODEL(WhiteMetznerCYLog, "WhiteMetznerCYLog")
 RHEOGPU_LOG_MODEL(RoliePolyLog, "Rolie-PolyLog")
 RHEOGPU_LOG_MODEL(XPomPomLog, "XPomPomLog")
 RHEOGPU_LOG_MODEL(SaramitoLog, "SaramitoLog")
+RHEOGPU_LOG_MODEL(BMPLog, "BMPLog")
 
 //- IOdictionary wrapper reading constitutiveProperties (constitutiveModel.C:41-59)
 class constitutiveModel {

@@ -222,7 +222,9 @@ class Runner:
                      ("a", "a"), ("b", "b")):
             if k in e and not isinstance(e[k], foamdict.FoamDict):
                 kw[o] = e.scalar(k)
-        if t == "SaramitoLog":  # SaramitoLog.C:117-170
+        if t == "BMPLog":  # BMPLog.C:113-119
+            kw.update(G0=e.scalar("G0"), k_bmp=e.scalar("k"), Phi0=e.scalar("Phi0"), PhiInf=e.scalar("PhiInf"))
+        elif t == "SaramitoLog":  # SaramitoLog.C:117-170
             kw.update(tau0=e.scalar("tau0"), sar_n=e.scalar("n"),
                       dims=tuple(float(x) for x in e["dims"][0]))
             kw.pop("K", None)

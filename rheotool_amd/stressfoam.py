@@ -13,13 +13,16 @@ path, on a case directory:
   writeControl/writeInterval, timePrecision);
 - <start>/theta<mode> (MUST_READ) with zeroGradient, fixedValue and empty
   patches; eigVals<mode>/eigVecs<mode> when present (READ_IF_PRESENT);
-  tau<mode> for SaramitoLog (MUST_READ: its plasticity factor reads it);
+  tau<mode> for SaramitoLog and BMPLog (MUST_READ: their sources read it);
+  for BMPLog also Phi<mode> (MUST_READ) with its patches, `div(phi,Phi<mode>)`,
+  the Phi solver and relaxation entries;
 - U and, when present, phi from the velocity time directory (default: the
   start time).  Patch velocities: the patch's `value` entry, the face cell's
   value for zeroGradient, zero for noSlip.  Without a phi file the flux is
   interpolate(U) & Sf with linear weights (fvc::flux).
 Every write time gets theta<mode>, tau<mode>, eigVals<mode>, eigVecs<mode>
-(and the summed tau of multiMode), in OpenFOAM's ASCII format.
+(Phi<mode> for BMPLog, and the summed tau of multiMode), in OpenFOAM's ASCII
+format.
 
     python -m rheotool_amd.stressfoam CASE [--U-time T] [--steps N] [--no-write]
 
@@ -143,6 +146,11 @@ class StressCase:
             md.scheme = _scheme_text(divs, md.field)
             md.solver = _solver_text(fs, md.field)
             md.relax = _relax(fs, md.field)
+            md.bmp = md.params.word("type") == "BMPLog"
+            if md.bmp:  # the fluidity equation's entries
+                md.fscheme = _scheme_text(divs, "Phi" + md.name)
+                md.fsolver = _solver_text(fs, "Phi" + md.name)
+                md.frelax = _relax(fs, "Phi" + md.name)
         t0 = os.path.join(path, self.ctl.startName)
         for md in self.modes:
             fld = polymesh.read_field(os.path.join(t0, md.field), self.pm)
@@ -154,8 +162,12 @@ class StressCase:
             # plasticity factor of the first step
             md.tau0 = None
             tp = os.path.join(t0, "tau" + md.name)
-            if md.params.word("type") == "SaramitoLog":
+            if md.params.word("type") in ("SaramitoLog", "BMPLog"):
                 md.tau0 = polymesh.read_field(tp, self.pm).internal.copy()
+            if md.bmp:
+                pf = polymesh.read_field(os.path.join(t0, "Phi" + md.name), self.pm)
+                md.Phi = pf.internal[:, 0].copy()
+                md.fbc = [(p, t, None if v is None else v[:, 0].copy()) for p, t, v in self._theta_bc(pf)]
         ut = os.path.join(path, u_time if u_time is not None else self.ctl.startName)
         self.U, self.Ub = self._velocity(ut)
         self.phi = self._flux(ut)
@@ -245,6 +257,16 @@ class StressCase:
             u.set_scheme(md.scheme)
             for patch, t, vals in md.bc:
                 u.set_bc(patch, t, vals)
+            if md.bmp:
+                fl = u.fluidity()
+                md.funit = fl
+                fl.set_state(md.Phi)
+                if md.fsolver:
+                    fl.set_solver(md.fsolver)
+                fl.set_relax(md.frelax)
+                fl.set_scheme(md.fscheme)
+                for patch, t, vals in md.fbc:
+                    fl.set_bc(patch, t, vals)
         self.eq.set_flow(self.U, self.Ub, self.phi)
 
     def _step_device(self, dt):
@@ -271,6 +293,15 @@ class StressCase:
                     bcval[s:s + len(vals)] = vals
             md.obc, md.obcval = bc, bcval
             md.tau = np.zeros((N, 6)) if md.tau0 is None else md.tau0.copy()
+            if md.bmp:
+                fb = np.zeros(self.m.nPatches, np.int32)
+                fv = np.zeros(max(self.m.nBoundary, 1))
+                for patch, t, vals in md.fbc:
+                    fb[patch] = t
+                    if vals is not None:
+                        s = self.m.patchStart[patch]
+                        fv[s:s + len(vals)] = vals
+                md.ofbc, md.ofbcval = fb, fv
         self.fields = Fields(U=self.U, Ub=self.Ub, phi=self.phi, gradU=None,
                              theta=None, eigVals=None, eigVecs=None)
         self.octl = O.controls()
@@ -279,9 +310,16 @@ class StressCase:
         import dataclasses
         for md in self.modes:
             f = dataclasses.replace(self.fields, theta=md.theta, eigVals=md.ev, eigVecs=md.vec)
-            th, ev, vec, tau, _ = self.O.correct(self.m, md.oparams, f, dt, ctl=self.octl,
-                                                 bc=md.obc, bcval=md.obcval, relax=md.relax,
-                                                 scheme=_oracle_scheme(md.scheme), tau_old=md.tau)
+            if md.bmp:
+                th, ev, vec, tau, _, md.Phi, _ = self.O.bmp_correct(
+                    self.m, md.oparams, f, dt, md.Phi, md.tau, ctl=self.octl, bc=md.obc, bcval=md.obcval,
+                    relax=md.relax, scheme=_oracle_scheme(md.scheme), bc_phi=md.ofbc,
+                    bcval_phi=md.ofbcval, relax_phi=md.frelax, scheme_phi=_oracle_scheme(md.fscheme),
+                    ctl_phi=self.octl)
+            else:
+                th, ev, vec, tau, _ = self.O.correct(self.m, md.oparams, f, dt, ctl=self.octl,
+                                                     bc=md.obc, bcval=md.obcval, relax=md.relax,
+                                                     scheme=_oracle_scheme(md.scheme), tau_old=md.tau)
             md.theta, md.ev, md.vec, md.tau = th, ev, vec, tau
         return 0
 
@@ -293,14 +331,18 @@ class StressCase:
         return rc
 
     def state(self):
-        """{mode: dict(theta, tau, eigVals, eigVecs)} and the summed tau."""
+        """{mode: dict(theta, tau, eigVals, eigVecs[, Phi])} and the summed tau."""
         out = {}
         for md in self.modes:
             if self.oracle:
                 out[md.name] = dict(theta=md.theta, tau=md.tau, eigVals=md.ev, eigVecs=md.vec)
+                if md.bmp:
+                    out[md.name]["Phi"] = md.Phi
             else:
                 u = md.unit
                 out[md.name] = dict(theta=u.theta(), tau=u.tau(), eigVals=u.eigVals(), eigVecs=u.eigVecs())
+                if md.bmp:
+                    out[md.name]["Phi"] = md.funit.Phi()
         tau = sum(s["tau"] for s in out.values())
         return out, tau
 
@@ -325,6 +367,13 @@ class StressCase:
                 v = s[fname].reshape(-1, 9)
                 zg = {n: ("empty", None) if th_bc[n][0] == "empty" else ("zeroGradient", None) for n in th_bc}
                 polymesh.write_field(os.path.join(tdir, fname + md.name), fname + md.name, v, pm, zg, tname)
+            if md.bmp:
+                f_bc = {}
+                for patch, t, vals in md.fbc:
+                    f_bc[pm.patches[patch][0]] = ("empty", None) if t == BC_EMPTY else (
+                        ("zeroGradient", None) if t == BC_ZG else ("fixedValue", vals))
+                polymesh.write_field(os.path.join(tdir, "Phi" + md.name), "Phi" + md.name, s["Phi"], pm, f_bc,
+                                     tname, dims="[-1 1 1 0 0 0 0]")
         if len(self.modes) > 1:
             calc = {p[0]: ("empty", None) if p[1] == "empty" else ("calculated", _face_vals(pm, tau, p[0]))
                     for p in pm.patches}

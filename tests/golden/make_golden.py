@@ -279,9 +279,12 @@ def model_rhs_more(model, C, gradU, p):
         plasticity factor F = (max(0, |tau_d| - tau0) / (k |tau_d|^n))^(1/n)
         of the previous stress (|tau_d| = sqrt(tau_d:tau_d / 2), tau_d the
         deviator over the valid dimensions), k = etaP for n = 1, and g the
-        optional PTT function 1 + eps/(1-zeta)(tr C - 3) or its exponential."""
+        optional PTT function 1 + eps/(1-zeta)(tr C - 3) or its exponential;
+    BMP (Bautista-Manero-Puig)  R = -Phi G0 (C - I), Phi the fluidity."""
     I = np.eye(3)
     trC = np.trace(C)
+    if model == "BMPLog":
+        return -(p["Phi"] * p["G0"]) * (C - I)
     if model == "SaramitoLog":
         dims = np.array(p["dims"], dtype=float)
         tau = p["tauOld"]
@@ -333,6 +336,7 @@ MODELS_MORE = {
                             epsilon=0.2, zeta=0.1),
     "SaramitoLog-exp": dict(lam=0.9, tau0=0.35, sar_n=1.0, dims=(1, 1, 1), ptt_function="exponential",
                             epsilon=0.15),
+    "BMPLog": dict(lam=1.0, G0=0.05, k_bmp=1.0, Phi0=2.0, PhiInf=200.0),
 }
 
 
@@ -362,6 +366,9 @@ def make_rhs_more():
             C = R @ np.diag(np.exp(ln)) @ R.T
             th = R @ np.diag(ln) @ R.T
             L = rng.standard_normal((3, 3))
+            if model == "BMPLog":
+                p["Phi"] = rng.uniform(0.5, 50.0)
+                taus.append(p["Phi"])
             if model == "SaramitoLog":
                 # previous stress: about a third below the yield stress
                 A = rng.standard_normal((3, 3)) * p["tau0"] * rng.uniform(0.2, 3.0)
@@ -381,7 +388,9 @@ def make_rhs_more():
         out[k + "_eigVecs"] = np.array(vecs)
         out[k + "_gradU"] = np.array(Ls)
         out[k + "_S"] = np.array(S)
-        if taus:
+        if taus and model == "BMPLog":
+            out[k + "_Phi"] = np.array(taus)
+        elif taus:
             out[k + "_tauOld"] = np.array(taus)
     np.savez_compressed(os.path.join(HERE, "rhs_more.npz"), **out)
 

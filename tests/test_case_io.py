@@ -204,7 +204,8 @@ def prepare(tmp_path, name, vel):
 
 
 @pytest.mark.parametrize("name,vel", [("stresscase_cavity", cavity_velocity),
-                                      ("stresscase_channel", channel_velocity)])
+                                      ("stresscase_channel", channel_velocity),
+                                      ("stresscase_bmp", channel_velocity)])
 def test_runner_restart_matches_one_run(tmp_path, name, vel):
     a = prepare(tmp_path / "a", name, vel)
     b = prepare(tmp_path / "b", name, vel)
@@ -213,7 +214,7 @@ def test_runner_restart_matches_one_run(tmp_path, name, vel):
     assert written and all(os.path.isdir(d) for d in written)
     full, tau = sa.state()
     sb = stressfoam.StressCase(b, oracle=True)
-    n1 = 5 if name == "stresscase_channel" else 4
+    n1 = 4 if name == "stresscase_cavity" else 5
     sb.run(steps=n1)
     # restart from the latest written time
     with open(os.path.join(b, "system", "controlDict")) as fh:
@@ -225,10 +226,26 @@ def test_runner_restart_matches_one_run(tmp_path, name, vel):
     sc.run(steps=10 - n1)
     again, tau2 = sc.state()
     for k in full:
-        for fld in ("theta", "tau"):
-            assert np.allclose(again[k][fld], full[k][fld], rtol=0, atol=1e-13), (k, fld)
+        for fld in ("theta", "tau", "Phi"):
+            if fld in full[k]:
+                assert np.allclose(again[k][fld], full[k][fld], rtol=0, atol=1e-13), (k, fld)
     assert np.abs(tau).max() > 1e-3
     # the summed stress of multiMode is written next to the modes' fields
     if name == "stresscase_channel":
         t = polymesh.read_field(os.path.join(written[-1], "tau"), sa.pm)
         assert np.array_equal(t.internal, tau)
+
+
+def test_oracle_parameters_of_the_case_models():
+    """The case runner's oracle reads every key each model's constructor reads."""
+    from rheotool_amd.rheotest import Runner
+    d = foamdict.read(os.path.join(GOLDEN, "stresscase_bmp", "constant", "constitutiveProperties"))
+    p, etaS = Runner.oracle_params(d.sub("parameters"))
+    assert (p.G0, p.kBMP, p.Phi0, p.PhiInf, p.lambda_, etaS) == (0.5, 2.0, 2.0, 50.0, 0.5, 0.01)
+    s = foamdict.parse("parameters { type SaramitoLog; rho rho [1 -3 0 0 0 0 0] 1; etaS etaS [1 -1 -1 0 0 0 0] 0; "
+                       "etaP etaP [1 -1 -1 0 0 0 0] 1; lambda lambda [0 0 1 0 0 0 0] 0.1; "
+                       "tau0 tau0 [1 -1 -2 0 0 0 0] 2.5; k k [1 -1 -1 0 0 0 0] 1.5; n n [0 0 0 0 0 0 0] 0.75; "
+                       "zeta zeta [0 0 0 0 0 0 0] 0; PTTfunction none; epsilon epsilon [0 0 0 0 0 0 0] 0; "
+                       "dims (1 1 0); stabilization coupling; }").sub("parameters")
+    p, _ = Runner.oracle_params(s)
+    assert (p.tau0, p.sarN, p.sarK, p.dimX, p.dimY, p.dimZ, p.sarPTT) == (2.5, 0.75, 1.5, 1.0, 1.0, 0.0, 0)

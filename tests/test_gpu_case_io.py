@@ -25,7 +25,8 @@ def rel(a, b):
 
 
 @pytest.mark.parametrize("name,vel", [("stresscase_cavity", cavity_velocity),
-                                      ("stresscase_channel", channel_velocity)])
+                                      ("stresscase_channel", channel_velocity),
+                                      ("stresscase_bmp", channel_velocity)])
 def test_case_run_device_matches_oracle(tmp_path, name, vel):
     print(f"[case] stress-only case run {name}, 10 steps", flush=True)
     dev_case = prepare(tmp_path / "dev", name, vel)
@@ -40,6 +41,8 @@ def test_case_run_device_matches_oracle(tmp_path, name, vel):
     for k in o:
         assert rel(d[k]["theta"], o[k]["theta"]) < bar, k
         assert rel(d[k]["tau"], o[k]["tau"]) < bar, k
+        if "Phi" in o[k]:
+            assert rel(d[k]["Phi"], o[k]["Phi"]) < bar, k
     assert rel(dtau, otau) < bar
     last = written[-1]
     for md in dev.modes:

@@ -552,6 +552,81 @@ def test_saramito_dictionary_errors(ctx):
     assert "keyword k" in e.value.msg
 
 
+BMP_DEV = dict(G0=0.05, k=1.0, Phi0=2.0, PhiInf=200.0)
+BMP_ORC = dict(G0=0.05, k_bmp=1.0, Phi0=2.0, PhiInf=200.0)
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+@pytest.mark.parametrize("scheme", ["upwind", "cubista"])
+def test_bmp_matches_oracle_over_steps(ctx, precision, scheme):
+    """BMPLog (BMPLog.C:142-202): each step solves the fluidity equation
+    (its source reads the previous tau and the velocity gradient), then theta
+    with the new fluidity.  Three steps on a permuted jittered box: theta, tau
+    and Phi within 1e-9 per step of the oracle."""
+    import dataclasses
+    m = cases.cartesian(12, 10, 9, jitter=0.1, perm_seed=4)
+    f = cases.make_fields(m, cases.FIELDS_TG, seed=8, lnLo=-1.5, lnHi=1.5)
+    rng = np.random.default_rng(11)
+    Phi = rng.uniform(1.0, 20.0, size=m.nCells)
+    tau = _saramito_tau(m.nCells, 6)
+    print(f"[case] BMPLog N={m.nCells} precision={precision} scheme={scheme}", flush=True)
+    dm = ctx.mesh(m)
+    eq = ctx.constitutiveEq("", dm, abi.model_dict("BMPLog", lam=1.0, etaP=0.01, **BMP_DEV))
+    fl = eq.fluidity()
+    assert fl is not None and ctx.constitutiveEq("", dm, abi.model_dict("Oldroyd-BLog")).fluidity() is None
+    for e in (eq, fl):
+        e.set_solver(abi.solver_dict(1e-10, precision=precision))
+        if scheme != "upwind":
+            e.set_scheme("GaussDefCmpw " + scheme)
+    eq.set_state(f.theta, f.eigVals, f.eigVecs, tau)
+    fl.set_state(Phi)
+    eq.set_flow(f.U, f.Ub, f.phi)
+    p = O.params("BMPLog", etaS=0.59, etaP=0.01, lam=1.0, **BMP_ORC)
+    cur = f
+    for step in range(3):
+        assert eq.correct(2e-3) == abi.RT_OK, (eq.solver_perf(), fl.solver_perf())
+        th, ev, vec, tau, perf, Phi, pphi = O.bmp_correct(m, p, cur, 2e-3, Phi, tau, scheme=scheme,
+                                                           scheme_phi=scheme)
+        assert rel_l2(fl.Phi(), Phi) < THETA_TOL * (step + 1)
+        assert rel_l2(eq.theta(), th) < THETA_TOL * (step + 1)
+        assert rel_l2(eq.tau(), tau) < THETA_TOL * (step + 1)
+        assert fl.solver_perf()[0].converged == 1 and fl.solver_perf()[1].solved == 0
+        cur = dataclasses.replace(cur, theta=th, eigVals=ev, eigVecs=vec)
+    assert np.abs(Phi - 10.5).max() > 1e-3  # the fluidity moved
+
+
+def test_bmp_fluidity_inlet_relax_and_errors(ctx):
+    """fixedValue fluidity on the channel inlet, a relaxation factor on the
+    fluidity equation; missing BMP keys are named."""
+    m, f = cases.channel2d(32)
+    dm = ctx.mesh(m)
+    eq = ctx.constitutiveEq("", dm, abi.model_dict("BMPLog", lam=1.0, etaP=0.01, **BMP_DEV))
+    fl = eq.fluidity()
+    eq.set_solver(abi.solver_dict(1e-10))
+    fl.set_solver(abi.solver_dict(1e-10))
+    Phi = np.full(m.nCells, 2.0)
+    s, n = int(m.patchStart[0]), int(m.patchSize[0])
+    fl.set_state(Phi)
+    fl.set_bc(0, abi.RT_BC_FIXEDVALUE, np.full(n, 5.0))
+    fl.set_relax(0.8)
+    eq.set_state(f.theta, f.eigVals, f.eigVecs)
+    eq.set_flow(f.U, f.Ub, f.phi)
+    assert eq.correct(1e-3) == abi.RT_OK
+    bc = O.default_bc(m)
+    bc[0] = O.BC_FIXEDVALUE
+    bv = np.zeros(m.nBoundary)
+    bv[s:s + n] = 5.0
+    p = O.params("BMPLog", etaS=0.59, etaP=0.01, lam=1.0, **BMP_ORC)
+    out = O.bmp_correct(m, p, f, 1e-3, Phi, np.zeros((m.nCells, 6)), bc_phi=bc, bcval_phi=bv, relax_phi=0.8)
+    assert rel_l2(fl.Phi(), out[5]) < THETA_TOL
+    assert rel_l2(eq.theta(), out[0]) < THETA_TOL
+    inlet_cells = m.owner[m.nInternal + s:m.nInternal + s + n]
+    assert np.all(out[5][inlet_cells] > 2.0)  # the inlet value enters
+    with pytest.raises(abi.RheoError) as e:
+        ctx.constitutiveEq("", dm, abi.model_dict("BMPLog", lam=1.0, G0=0.05, k=1.0, Phi0=2.0))
+    assert "PhiInf" in e.value.msg
+
+
 def _mode_entry(name, model, **kw):
     body = abi.model_dict(model, **kw).split("{", 1)[1].rsplit("}", 1)[0]
     return f"{name} {{{body}}}"

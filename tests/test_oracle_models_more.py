@@ -1,6 +1,6 @@
 """Oracle pinning for FENE-CR, White-Metzner Carreau-Yasuda, Rolie-Poly, XPP
-and Saramito (the further log-conformation models: FENE_CRLog.C,
-WhiteMetznerCYLog.C, RoliePolyLog.C, XPomPomLog.C, SaramitoLog.C).  The fixture holds the log-conformation
+Saramito and BMP (the further log-conformation models: FENE_CRLog.C,
+WhiteMetznerCYLog.C, RoliePolyLog.C, XPomPomLog.C, SaramitoLog.C, BMPLog.C).  The fixture holds the log-conformation
 right-hand side obtained from each model's conformation-tensor equation
 through the Daleckii-Krein derivative of the matrix logarithm (an
 independent formulation), written by tests/golden/make_golden.py."""
@@ -25,12 +25,21 @@ def tau_old(d, k):
     return d[k + "_tauOld"] if k + "_tauOld" in d.files else None
 
 
+def aux(d, k):
+    """BMP's fluidity per sample, in column 0 of a [n, 6] field."""
+    if k + "_Phi" not in d.files:
+        return None
+    a = np.zeros((len(d[k + "_Phi"]), 6))
+    a[:, 0] = d[k + "_Phi"]
+    return a
+
+
 @pytest.mark.parametrize("key", list(MODELS_MORE))
 def test_rhs_matches_daleckii_krein(key):
     d = np.load(os.path.join(HERE, "rhs_more.npz"))
     k = key.replace("-", "_")
     S, _ = O.rhs(params(key), d[k + "_gradU"], d[k + "_theta"], d[k + "_eigVals"], d[k + "_eigVecs"],
-                 tau_old(d, k))
+                 tau_old(d, k), aux(d, k))
     ref = d[k + "_S"]
     err = np.abs(S - ref).max(1) / (1 + np.abs(ref).max(1))
     assert err.max() < 1e-10, err.max()
@@ -44,12 +53,13 @@ def test_rhs_matches_daleckii_krein(key):
                                       ("SaramitoLog-hb2d", dict(dims=(1, 1, 1))),
                                       ("SaramitoLog-hb2d", dict(sar_n=0.8)),
                                       ("SaramitoLog-lin", dict(ptt_function="exponential")),
-                                      ("SaramitoLog-exp", dict(zeta=0.1))])
+                                      ("SaramitoLog-exp", dict(zeta=0.1)),
+                                      ("BMPLog", dict(G0=0.051))])
 def test_planted_parameter_mistake_is_caught(key, over):
     d = np.load(os.path.join(HERE, "rhs_more.npz"))
     k = key.replace("-", "_")
     S, _ = O.rhs(params(key, **over), d[k + "_gradU"], d[k + "_theta"], d[k + "_eigVals"],
-                 d[k + "_eigVecs"], tau_old(d, k))
+                 d[k + "_eigVecs"], tau_old(d, k), aux(d, k))
     assert np.abs(S - d[k + "_S"]).max() > 1e-6
 
 
@@ -123,7 +133,41 @@ def test_tau_closed_forms():
     p = params("XPomPomLog")
     t = O.tau(p, ev, vec, None)
     assert np.allclose(t, _sym(0.41 / 1.6 * (C - I)), rtol=1e-12, atol=1e-13)
+    # BMP: G0 (C - I)
+    t = O.tau(params("BMPLog"), ev, vec, None)
+    assert np.allclose(t, _sym(0.05 * (C - I)), rtol=1e-12, atol=1e-13)
     # Saramito: etaP / (lambda (1 - zeta)) (C - I), with and without zeta
     for key, lam, zeta in (("SaramitoLog", 1.3, 0.0), ("SaramitoLog-lin", 1.1, 0.1)):
         t = O.tau(params(key), ev, vec, None)
         assert np.allclose(t, _sym(0.41 / (lam * (1 - zeta)) * (C - I)), rtol=1e-12, atol=1e-13)
+
+
+@pytest.mark.parametrize("scheme", ["upwind", "cubista"])
+def test_bmp_fluidity_equation_closed_form(scheme):
+    """BMPLog.C:151-163 with no convection (phi = 0), a prescribed uniform
+    gradient and a uniform previous stress: per cell
+    (1/dt + 1/lambda) Phi_new = Phi/dt + Phi0/lambda + k (PhiInf - Phi)(tau : D)."""
+    from rheotool_amd import cases
+    m = cases.cartesian(4, 3, 2, jitter=0.1)
+    f = cases.make_fields(m, cases.FIELDS_TG, seed=2, lnLo=-1, lnHi=1)
+    f.phi = np.zeros_like(f.phi)
+    rng = np.random.default_rng(3)
+    L = rng.normal(size=(3, 3))
+    g = np.broadcast_to(L, (m.nCells, 3, 3)).copy()
+    T = rng.normal(size=(3, 3))
+    T = 0.5 * (T + T.T)
+    tau = np.tile([T[0, 0], T[0, 1], T[0, 2], T[1, 1], T[1, 2], T[2, 2]], (m.nCells, 1))
+    Phi = rng.uniform(1.0, 5.0, size=m.nCells)
+    p = O.params("BMPLog", lam=0.7, G0=0.05, k_bmp=0.3, Phi0=2.0, PhiInf=20.0, etaP=0.01)
+    dt = 1e-2
+    out = O.bmp_correct(m, p, f, dt, Phi, tau, gradU=g, scheme_phi=scheme)
+    D = 0.5 * (L + L.T)
+    tdd = np.sum(T * D)
+    want = (Phi / dt + 2.0 / 0.7 + 0.3 * (20.0 - Phi) * tdd) / (1 / dt + 1 / 0.7)
+    assert np.allclose(out[5], want, rtol=1e-12)
+    assert out[6][0].converged == 1 and all(q.solved == 0 for q in out[6][1:])
+    # the theta step used the new fluidity: the same as the theta equation fed it directly
+    Pn = np.zeros((m.nCells, 6))
+    Pn[:, 0] = out[5]
+    th = O.correct(m, p, f, dt, gradU=g, tau_old=tau, aux=Pn)[0]
+    assert np.array_equal(th, out[0])
