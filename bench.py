@@ -210,6 +210,8 @@ def main():
     ap.add_argument("--max-cells", type=float, default=2e6)
     ap.add_argument("--n", type=int, default=0, help="override the mesh size")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--renumber", default="auto", choices=["auto", "none", "rcm", "sfc"],
+                    help="device cell order (auto: RCM when the input order has poor locality)")
     ap.add_argument("--no-fp64-compare", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=64)
     args = ap.parse_args()
@@ -237,7 +239,8 @@ def main():
         f"generated in {time.perf_counter() - t0:.1f} s")
     ctx = abi.Context(0)
     t0 = time.perf_counter()
-    dm = ctx.mesh(m)
+    dm = ctx.mesh(m, {"auto": abi.RT_RENUMBER_AUTO, "none": abi.RT_RENUMBER_NONE,
+                      "rcm": abi.RT_RENUMBER_RCM, "sfc": abi.RT_RENUMBER_SFC}[args.renumber])
     log(f"[bench] mesh uploaded in {time.perf_counter() - t0:.1f} s")
     eq = new_model(ctx, dm, w, f, w["precision"])
     log(f"[bench] device bytes in use {ctx.bytes_in_use / 1e9:.2f} GB")

@@ -27,7 +27,7 @@ LIB_PATH = os.environ.get("RHEO_GPU_LIB") or os.path.join(HERE, "lib", "librheo_
 RT_OK, RT_E_UNKNOWN_TYPE, RT_E_BAD_ARG, RT_E_OOM, RT_E_HIP, RT_E_UNSUPPORTED, RT_E_NO_DEVICE = range(7)
 RT_W_NOT_CONVERGED = 0x100
 RT_THETA, RT_TAU, RT_EIGVALS, RT_EIGVECS = range(4)
-RT_RENUMBER_AUTO, RT_RENUMBER_NONE, RT_RENUMBER_RCM = range(3)
+RT_RENUMBER_AUTO, RT_RENUMBER_NONE, RT_RENUMBER_RCM, RT_RENUMBER_SFC = range(4)
 RT_BC_ZEROGRADIENT, RT_BC_FIXEDVALUE, RT_BC_EMPTY = range(3)
 
 

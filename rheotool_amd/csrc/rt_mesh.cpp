@@ -21,6 +21,43 @@ namespace rt {
 namespace {
 
 // Reverse Cuthill-McKee on the cell graph (faces as edges).
+// spreads the low 21 bits of v to every third bit
+uint64_t spread3(uint64_t v) {
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffULL;
+    v = (v | v << 16) & 0x1f0000ff0000ffULL;
+    v = (v | v << 8) & 0x100f00f00f00f00fULL;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+    v = (v | v << 2) & 0x1249249249249249ULL;
+    return v;
+}
+
+// cells sorted by the Morton key of their centre, quantised to 21 bits per
+// axis over the bounding box; ties keep the input order
+std::vector<int32_t> mortonOrder(int32_t N, const double* C) {
+    double lo[3] = {C[0], C[1], C[2]}, hi[3] = {C[0], C[1], C[2]};
+    for (int32_t c = 0; c < N; ++c)
+        for (int q = 0; q < 3; ++q) {
+            lo[q] = std::min(lo[q], C[3 * (size_t)c + q]);
+            hi[q] = std::max(hi[q], C[3 * (size_t)c + q]);
+        }
+    const double span = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-300});
+    const double scale = ((1 << 21) - 1) / span;  // one scale: cubic cells stay cubic
+    std::vector<std::pair<uint64_t, int32_t>> key(N);
+    for (int32_t c = 0; c < N; ++c) {
+        uint64_t k = 0;
+        for (int q = 0; q < 3; ++q) {
+            const uint64_t x = (uint64_t)((C[3 * (size_t)c + q] - lo[q]) * scale);
+            k |= spread3(x) << q;
+        }
+        key[c] = {k, c};
+    }
+    std::sort(key.begin(), key.end());
+    std::vector<int32_t> order(N);
+    for (int32_t r = 0; r < N; ++r) order[r] = key[r].second;
+    return order;
+}
+
 std::vector<int32_t> rcmOrder(int32_t N, const std::vector<int64_t>& ptr,
                               const std::vector<int32_t>& adj) {
     std::vector<int32_t> deg(N);
@@ -164,13 +201,18 @@ void buildMesh(Ctx* ctx, const rt_mesh_desc* d, int renumber, Mesh& m) {
     for (int32_t f = 0; f < F; ++f) band += (double)(d->neighbour[f] - d->owner[f]);
     m.meanAbsBandwidth = F ? band / F : 0.0;
     bool doRcm = renumber == RT_RENUMBER_RCM;
+    bool doSfc = renumber == RT_RENUMBER_SFC;
+    if (doSfc && !d->C) throw Error{RT_E_BAD_ARG, "rt_mesh_create: SFC renumbering needs the cell centres"};
     if (renumber == RT_RENUMBER_AUTO) {
         const double ref = std::pow((double)N, 2.0 / 3.0);
         doRcm = F > 0 && m.meanAbsBandwidth > 8.0 * ref;
     }
     m.iperm.resize(N);
     m.perm.resize(N);
-    if (doRcm) {
+    if (doSfc) {
+        m.iperm = mortonOrder(N, d->C);
+        m.renumbered = true;
+    } else if (doRcm) {
         m.iperm = rcmOrder(N, rptr, rcol);
         m.renumbered = true;
     } else {

@@ -132,6 +132,17 @@ def test_permuted_jittered_fenep(ctx, precision):
     check_step(g, o, m.validComponents)
 
 
+@pytest.mark.parametrize("scheme", ["upwind", "cubista"])
+def test_space_filling_curve_renumbering(ctx, scheme):
+    """The Morton-curve device order (RT_RENUMBER_SFC) on a permuted jittered
+    box: the results come back in the caller's numbering, within 1e-9 of the
+    oracle."""
+    m, f = cases.box3d(20, lnLo=-3.0, lnHi=2.0, perm_seed=7, jitter=0.1)
+    g, o = run_pair(ctx, m, f, "GiesekusLog", 2.0, 1e-3, dict(alpha=0.1), renumber=abi.RT_RENUMBER_SFC,
+                    scheme=scheme)
+    check_step(g, o, m.validComponents)
+
+
 @pytest.mark.parametrize("model", ["Oldroyd-BLog", "GiesekusLog"])
 def test_mixed_precision_matches_fp64_oracle(ctx, model):
     m, f = cases.box3d(24, lnLo=-9.0, lnHi=5.0)
