@@ -151,7 +151,7 @@ __global__ void __launch_bounds__(BLK) k_kry_p(KryArgs<T> a) {
     bool on[6];
     bool first[6];
     T beta[6], omega[6];
-    bool any = false;
+    bool any = false, update = false;
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
         on[c] = a.cs[c].active;
@@ -159,9 +159,20 @@ __global__ void __launch_bounds__(BLK) k_kry_p(KryArgs<T> a) {
         beta[c] = (T)a.cs[c].beta;
         omega[c] = (T)a.cs[c].omega;
         any |= on[c];
+        update |= on[c] && !first[c];
     }
     if (!any) return;
     const int64_t N = a.N;
+    if (!update) {
+        // every active component starts an inner solve: p = r.  The other
+        // components' p is dead until their own restart sets it.
+        for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < N; i += (int64_t)gridDim.x * BLK) {
+            T rr[6];
+            ld6(a.r + NCMP * i, rr);
+            st6(a.p + NCMP * i, rr);
+        }
+        return;
+    }
     for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < N; i += (int64_t)gridDim.x * BLK) {
         T rr[6], pp[6], vv[6];
         ld6(a.r + NCMP * i, rr);
@@ -627,7 +638,7 @@ void fill_args(Ctx* ctx, const SolveArgs& sa, SolveRun& R) {
 struct Bytes {
     double N, ne, w;
     double res(double nc) const { return N * 8.0 + ne * 12 + nc * N * (8 + 8 + 2 * w); }
-    double p(double nc) const { return nc * N * 4 * w; }
+    double p(double nc, bool start) const { return nc * N * (start ? 2 : 4) * w; }
     double v(double nc) const { return ne * (4 + w) + nc * N * 3 * w; }
     double t(double nc) const { return ne * (4 + w) + nc * N * 3 * w; }
     double xr(double nc) const { return N * 8.0 + nc * N * (16 + 6 * w); }
@@ -645,7 +656,7 @@ void launch_init(Ctx* ctx, SolveRun& R) {
 }
 
 template <class T>
-void launch_round(Ctx* ctx, SolveRun& R, double nAct, double nRef) {
+void launch_round(Ctx* ctx, SolveRun& R, double nAct, double nRef, bool start) {
     const bool mixed = std::is_same<T, float>::value;
     Bytes B{R.N, R.ne, (double)sizeof(T)};
     KryArgs<T>& a = argsOf<T>(R);
@@ -664,7 +675,7 @@ void launch_round(Ctx* ctx, SolveRun& R, double nAct, double nRef) {
         nAct += nRef;  // components restarted by the refinement sweep join this round
     }
     {
-        ProfScope ps(ctx, mixed ? "k_kry_p<f32>" : "k_kry_p<f64>", B.p(nAct));
+        ProfScope ps(ctx, mixed ? "k_kry_p<f32>" : "k_kry_p<f64>", B.p(nAct, start));
         hipLaunchKernelGGL(k_kry_p<T>, dim3(R.pgrid), dim3(BLK), 0, ctx->stream, a);
     }
     {
@@ -749,10 +760,12 @@ bool solve_advance(Ctx* ctx, Krylov& K, bool block) {
         }
     }
     double nAct = 0, nRef = 0;
+    bool start = true;  // every component of the round begins an inner solve (p = r)
     for (int c = 0; c < 6; ++c) {
         if (R.valid[c] <= 0) continue;
         nAct += R.host[c].active != 0;
         nRef += R.host[c].needRefine != 0;
+        if (R.host[c].active && R.host[c].innerIter != 0) start = false;
     }
     if ((nAct == 0 && nRef == 0) || R.rounds >= 100000) {
         R.done = true;
@@ -762,8 +775,8 @@ bool solve_advance(Ctx* ctx, Krylov& K, bool block) {
     // components in it: components whose inner solve stopped wait (idle, no
     // state change) until every inner solve has stopped, then refine together.
     if (nAct > 0) nRef = 0;
-    if (R.mixed) launch_round<float>(ctx, R, nAct, nRef);
-    else launch_round<double>(ctx, R, nAct, nRef);
+    if (R.mixed) launch_round<float>(ctx, R, nAct, nRef, start);
+    else launch_round<double>(ctx, R, nAct, nRef, start);
     RT_HIP(hipGetLastError());
     ++R.rounds;
     queue_pull(ctx, K, R);
