@@ -1,0 +1,111 @@
+"""The stress step at the benchmark configurations' full sizes.
+
+- 128^3 drop-in (2.1 M cells, Oldroyd-B and Giesekus, FP64): the whole step
+  against the oracle, which finishes in seconds at this size, within 1e-9.
+- 256^3 precision form (16.8 M cells, permuted, jittered, FENE-P): the mixed
+  precision step against the FP64 step on the same inputs, within 1e-9
+  relative per field (the oracle would take minutes here); the explicit
+  (phi = 0) step against the oracle's source on a sample of cells, which is
+  a per-cell identity at any size.
+- 4 M-cell random face graph (Giesekus alpha = 0.01, Wi = 50, prescribed
+  grad U): mixed against FP64 within 5e-9, theta finite, C positive definite.
+  The two paths stop on different residuals: the FP64 path on its
+  recursively updated one, as the reference's PBiCGStab does, and the mixed
+  path on the true FP64 residual of each refinement sweep.  At Wi = 50 on
+  this graph the fields differ by 1.0e-9 at the 1e-10 tolerance (1.8e-9 when
+  both solve to 1e-11, the recursive residual's drift), hence the bar.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from rheotool_amd import abi, cases
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = abi.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def box256():
+    """The 256^3 precision form (built once: about half a minute on the host)."""
+    return cases.box3d(256, lnLo=-8.0, lnHi=float(np.log(30.0)), perm_seed=3, jitter=0.1)
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def step(ctx, dm, f, model, lam, dt, kw, precision, gradU=None, tol=1e-10):
+    eq = ctx.constitutiveEq("", dm, abi.model_dict(model, lam=lam, **kw))
+    eq.set_solver(abi.solver_dict(tol, precision=precision))
+    eq.set_state(f.theta, f.eigVals, f.eigVecs)
+    eq.set_flow(f.U, f.Ub, f.phi)
+    assert eq.correct(dt, gradU) == abi.RT_OK, eq.solver_perf()
+    out = dict(theta=eq.theta(), tau=eq.tau(), ev=eq.eigVals(), perf=eq.solver_perf())
+    eq.close()
+    return out
+
+
+@pytest.mark.parametrize("model,kw", [("Oldroyd-BLog", {}), ("GiesekusLog", dict(alpha=0.1))])
+def test_dropin_128_fp64_matches_oracle(ctx, model, kw):
+    print(f"[case] 128^3 drop-in {model} FP64 vs oracle", flush=True)
+    m, f = cases.box3d(128, lnLo=-9.0, lnHi=5.0)
+    dm = ctx.mesh(m)
+    g = step(ctx, dm, f, model, 10.0, 5e-4, kw, "double")
+    p = O.params(model, etaS=0.59, etaP=0.41, lam=10.0, alpha=kw.get("alpha", 0.0))
+    th, ev, vec, tau, perf = O.correct(m, p, f, 5e-4)
+    assert rel(g["theta"], th) < 1e-9 and rel(g["tau"], tau) < 1e-9
+    assert all(q.converged for q in g["perf"] if q.solved)
+
+
+def test_precision_256_mixed_matches_fp64(ctx, box256):
+    print("[case] 256^3 precision form, mixed vs FP64", flush=True)
+    m, f = box256
+    dm = ctx.mesh(m)
+    kw = dict(L2=100.0)
+    a = step(ctx, dm, f, "FENE-PLog", 5.0, 5e-4, kw, "double")
+    b = step(ctx, dm, f, "FENE-PLog", 5.0, 5e-4, kw, "mixed")
+    for k in ("theta", "tau"):
+        assert np.all(np.isfinite(b[k]))
+        assert rel(b[k], a[k]) < 1e-9, (k, rel(b[k], a[k]))
+    assert all(q.nRefinements >= 1 for q in b["perf"] if q.solved)
+
+
+def test_precision_256_explicit_step_matches_oracle_source(ctx, box256):
+    """phi = 0 and a prescribed gradient: every cell's step is
+    theta + dt S(theta, grad U, eigen state) exactly; 20000 sampled cells
+    against the oracle's source."""
+    import dataclasses
+    print("[case] 256^3 explicit step vs oracle source sample", flush=True)
+    m, f = box256
+    f = dataclasses.replace(f, phi=np.zeros_like(f.phi))
+    dm = ctx.mesh(m)
+    dt = 1e-3
+    g = step(ctx, dm, f, "FENE-PLog", 5.0, dt, dict(L2=100.0), "double", gradU=f.gradU)
+    idx = np.random.default_rng(0).choice(m.nCells, 20000, replace=False)
+    p = O.params("FENE-PLog", etaS=0.59, etaP=0.41, lam=5.0, L2=100.0)
+    S, _ = O.rhs(p, f.gradU[idx], f.theta[idx], f.eigVals[idx], f.eigVecs[idx])
+    want = f.theta[idx] + dt * S
+    err = np.abs(g["theta"][idx] - want).max() / np.abs(want).max()
+    assert err < 1e-13, err
+
+
+def test_random_graph_4m_mixed_matches_fp64(ctx):
+    print("[case] 4M-cell random face graph, mixed vs FP64", flush=True)
+    m = cases.random_graph(4_000_000, kNear=5, sigma=0.5, seed=3)
+    f = cases.make_fields(m, cases.FIELDS_TG, seed=11, lnLo=-9.0, lnHi=float(np.log(1e6)))
+    dm = ctx.mesh(m)
+    kw = dict(alpha=0.01)
+    a = step(ctx, dm, f, "GiesekusLog", 50.0, 1e-3, kw, "double", gradU=f.gradU)
+    b = step(ctx, dm, f, "GiesekusLog", 50.0, 1e-3, kw, "mixed", gradU=f.gradU)
+    for k in ("theta", "tau"):
+        assert np.all(np.isfinite(b[k]))
+        assert rel(b[k], a[k]) < 5e-9, (k, rel(b[k], a[k]))
+    lam = np.diagonal(b["ev"], axis1=1, axis2=2)
+    assert np.all(lam > 0)
