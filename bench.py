@@ -216,8 +216,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported by this build "
-              "(the stress step shards by cell partition; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the stress step "
+              "would shard by cell partition, one rank per GPU, with halo exchanges per SpMV; "
+              "only the one-GPU path is built)", file=sys.stderr)
         sys.exit(2)
 
     from rheotool_amd import abi

@@ -533,9 +533,33 @@ __global__ void __launch_bounds__(BLK) k_fin_assemble(AssembleArgs a, int G) {
 // ----------------------------------------------------------- calcEig + tau
 // Symmetric 3x3 eigen-decomposition by cyclic Jacobi rotations (Rutishauser's
 // stable update), ascending order; column eigenvectors in V (row-major).
-__device__ void jacobi3(const double* t6, double* ev, double* V) {
+// W (may be NULL): an orthonormal basis to start from (columns), normally the
+// previous step's eigenvectors.  The rotations then act on W^T t W, which is
+// nearly diagonal when theta changed little, and converge in one or two
+// sweeps instead of four to six; the result is the same decomposition (to
+// rounding, and up to the order and signs of the vectors, which are sorted
+// and do not enter the model terms).
+__device__ void jacobi3(const double* t6, const double* W, double* ev, double* V) {
     double a00 = t6[0], a01 = t6[1], a02 = t6[2], a11 = t6[3], a12 = t6[4], a22 = t6[5];
     double v[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    if (W) {
+        // A = W^T T W
+        const double T[9] = {a00, a01, a02, a01, a11, a12, a02, a12, a22};
+        double TW[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) TW[3 * i + j] = T[3 * i] * W[j] + T[3 * i + 1] * W[3 + j] + T[3 * i + 2] * W[6 + j];
+        double A[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) A[3 * i + j] = W[i] * TW[j] + W[3 + i] * TW[3 + j] + W[6 + i] * TW[6 + j];
+        a00 = A[0], a11 = A[4], a22 = A[8];
+        a01 = 0.5 * (A[1] + A[3]), a02 = 0.5 * (A[2] + A[6]), a12 = 0.5 * (A[5] + A[7]);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) v[q] = W[q];
+    }
     for (int sweep = 0; sweep < 12; ++sweep) {
         const double off = fabs(a01) + fabs(a02) + fabs(a12);
         const double dia = fabs(a00) + fabs(a11) + fabs(a22);
@@ -590,14 +614,36 @@ __device__ void jacobi3(const double* t6, double* ev, double* V) {
     }
 }
 
+// Gram-Schmidt on the columns of W (row-major) in place: false when W holds a
+// non-finite or (nearly) dependent set, which is then not used as a start.
+__device__ __forceinline__ bool orthonormalize3(double* W) {
+    double n0 = W[0] * W[0] + W[3] * W[3] + W[6] * W[6];
+    if (!(n0 > 1e-12) || !isfinite(n0)) return false;
+    n0 = 1.0 / sqrt(n0);
+    W[0] *= n0, W[3] *= n0, W[6] *= n0;
+    const double d = W[0] * W[1] + W[3] * W[4] + W[6] * W[7];
+    W[1] -= d * W[0], W[4] -= d * W[3], W[7] -= d * W[6];
+    double n1 = W[1] * W[1] + W[4] * W[4] + W[7] * W[7];
+    if (!(n1 > 1e-12) || !isfinite(n1)) return false;
+    n1 = 1.0 / sqrt(n1);
+    W[1] *= n1, W[4] *= n1, W[7] *= n1;
+    // third column: the cross product of the first two
+    W[2] = W[3] * W[7] - W[6] * W[4];
+    W[5] = W[6] * W[1] - W[0] * W[7];
+    W[8] = W[0] * W[4] - W[3] * W[1];
+    return true;
+}
+
 __global__ void __launch_bounds__(BLK) k_eig_tau(int32_t N, ModelParams p, const double* __restrict__ theta,
                                                  double* __restrict__ eigVal, double* __restrict__ eigVec,
                                                  double* __restrict__ tau, const double* __restrict__ fOld) {
     const int64_t r = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (r >= N) return;
-    double th[6], ev[3], V[9];
+    double th[6], ev[3], V[9], W[9];
     ld6(theta + (int64_t)NCMP * r, th);
-    jacobi3(th, ev, V);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) W[q] = eigVec[q * (int64_t)N + r];
+    jacobi3(th, orthonormalize3(W) ? W : nullptr, ev, V);
     double lam[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
