@@ -1179,7 +1179,8 @@ void launch_assemble(Ctx* ctx, const AssembleArgs& a, int grid) {
 }
 void launch_eig_tau(Ctx* ctx, int32_t N, const ModelParams& prm, const double* theta,
                     double* eigVal, double* eigVec, double* tau, const double* fOld) {
-    ProfScope ps(ctx, "k_eig_tau", (double)N * (48 + 24 + 72 + 48 + (modelKeepsFactor(prm.kind) ? 8 : 0)));
+    // theta 48 + previous eigenvectors 72 in; eigenvalues 24, eigenvectors 72, tau 48 out
+    ProfScope ps(ctx, "k_eig_tau", (double)N * (48 + 72 + 24 + 72 + 48 + (modelKeepsFactor(prm.kind) ? 8 : 0)));
     hipLaunchKernelGGL(k_eig_tau, dim3(nblk(N)), dim3(BLK), 0, ctx->stream, N, prm, theta, eigVal,
                        eigVec, tau, fOld);
     RT_HIP(hipGetLastError());
