@@ -244,6 +244,19 @@ def assemble(m, phi, theta, S, dt, bc=None, bcval=None):
     return out
 
 
+def relax(m, sys, alpha, psi, bc=None):
+    """[OF9] fvMatrix::relax on an assembled system (copies): returns the
+    relaxed diag and source."""
+    om = OracleMesh(m)
+    bc = default_bc(m) if bc is None else bc
+    D = np.array(sys["diag"], dtype=np.float64)
+    S = np.array(sys["source"], dtype=np.float64, order="C")
+    lib().orc_relax(om.ref, _p(np.ascontiguousarray(bc, dtype=np.int32)), alpha,
+                    _p(np.ascontiguousarray(psi, dtype=np.float64)), _p(D), _p(sys["lower"]), _p(sys["upper"]),
+                    _p(S), _p(sys["iC"]))
+    return D, S
+
+
 def solve_segregated(m, sys, psi, ctl=None, bc=None, valid=None):
     om = OracleMesh(m)
     bc = default_bc(m) if bc is None else bc
