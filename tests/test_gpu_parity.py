@@ -189,6 +189,28 @@ def test_inner_iterations_keep_old_time(ctx, scheme):
     assert rel_l2(seq[1][1], plain) > 1e-4
 
 
+def test_inverse_quirk_matches_oracle(ctx):
+    """ofInvQuirk 1 ([OF9] inv(tensorField)'s treatment of a first tensor with
+    zero diagonal components) on the device, the first cell of the caller's
+    numbering holding a zero eigenvalue: the explicit step agrees with the
+    oracle's to 1e-12 (the cells are permuted on upload, so the device must
+    find the caller's first cell)."""
+    m, f = cases.box3d(10, lnLo=-1.5, lnHi=1.5, perm_seed=2)
+    f.phi[:] = 0.0
+    f.eigVals[0] = np.diag([0.0, 2.0, 4.0])
+    f.eigVecs[0] = np.eye(3)
+    dm = ctx.mesh(m, abi.RT_RENUMBER_RCM)
+    eq = ctx.constitutiveEq("", dm, abi.model_dict("Oldroyd-BLog", lam=2.0, ofInvQuirk=1))
+    eq.set_solver(abi.solver_dict(1e-10))
+    eq.set_state(f.theta, f.eigVals, f.eigVecs)
+    eq.set_flow(f.U, f.Ub, f.phi)
+    assert eq.correct(1e-2) == abi.RT_OK
+    p = O.params("Oldroyd-BLog", etaS=0.59, etaP=0.41, lam=2.0, of_inv_quirk=True)
+    th = O.correct(m, p, f, 1e-2)[0]
+    g = eq.theta()
+    assert np.all(np.isfinite(th)) and np.abs(g - th).max() / np.abs(th).max() < 1e-12
+
+
 def test_explicit_step_exact(ctx):
     """phi = 0: the matrix is diagonal, theta_new = theta_old + dt S; the GPU's
     per-cell algebra follows the reference's operation order, so the result

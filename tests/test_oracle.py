@@ -271,3 +271,28 @@ def test_one_step_explicit_limit():
     dt = 1e-3
     th, *_ = O.correct(m, p, f, dt, gradU=f.gradU)
     assert np.abs(th - (f.theta + dt * S)).max() < 1e-12
+
+
+def test_inverse_quirk_closed_form():
+    """[OF9] inv(tensorField) adds 1 to the diagonal components that are zero
+    in the field's first tensor before inverting every tensor, and subtracts
+    it after (its treatment of 2D fields).  With V = I and no velocity
+    gradient the Oldroyd-B source is (inv(Lambda) - I)/lambda, so the quirk's
+    inverse is read off directly: the first cell's diag(0, b, c) inverts to
+    diag(0, 1/b, 1/c), and every other cell's xx to 1/(a + 1) - 1.  Without
+    the quirk (the default) the other cells invert plainly."""
+    lam = 2.0
+    ev = np.zeros((3, 3, 3))
+    ev[0] = np.diag([0.0, 2.0, 4.0])
+    ev[1] = np.diag([0.5, 2.0, 4.0])
+    ev[2] = np.diag([3.0, 0.25, 1.0])
+    vec = np.broadcast_to(np.eye(3), (3, 3, 3)).copy()
+    th = np.zeros((3, 6))
+    g = np.zeros((3, 3, 3))
+    S, _ = O.rhs(O.params("Oldroyd-BLog", lam=lam, of_inv_quirk=True), g, th, ev, vec)
+    inv = np.array([[0.0, 0.5, 0.25], [1 / 1.5 - 1, 0.5, 0.25], [1 / 4.0 - 1, 4.0, 1.0]])
+    want = (inv - 1.0) / lam
+    assert np.allclose(S[:, [0, 3, 5]], want, rtol=1e-15, atol=1e-16)
+    assert np.all(S[:, [1, 2, 4]] == 0)
+    S0, _ = O.rhs(O.params("Oldroyd-BLog", lam=lam), g[1:], th[1:], ev[1:], vec[1:])
+    assert np.allclose(S0[:, [0, 3, 5]], (1 / np.diagonal(ev[1:], axis1=1, axis2=2) - 1) / lam, rtol=1e-15)
