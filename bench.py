@@ -162,6 +162,7 @@ def serve_bench(args):
     srv = abi.Server(ctx, hbm_budget=200 << 30, max_inflight=16)
     arrivals = np.cumsum(rng.exponential(1.0 / args.rate, nreq))
     tickets = [None] * nreq
+    index_of = {}
     done = [None] * nreq
     cells = 0
     t_start = time.perf_counter()
@@ -173,14 +174,14 @@ def serve_bench(args):
             m, f, dm, dt = meshes[key]
             tickets[i] = srv.submit(dm, f, abi.model_dict(model, lam=lam, **kwmap.get(model, {})), dt,
                                     controls)
+            index_of[tickets[i]] = i
             i += 1
         live = srv.progress()
-        for j in range(i):
-            if done[j] is None and tickets[j] is not None:
-                out = srv.poll(tickets[j])
-                if out is not None:
-                    done[j] = out["latency_ms"]
-                    cells += meshes[specs[j][0]][0].nCells
+        for t in srv.finished():  # poll only the requests that finished
+            j = index_of[t]
+            out = srv.poll(t)
+            done[j] = out["latency_ms"]
+            cells += meshes[specs[j][0]][0].nCells
         if i == nreq and live == 0 and all(d is not None for d in done):
             break
     elapsed = time.perf_counter() - t_start

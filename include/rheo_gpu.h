@@ -350,6 +350,10 @@ int rt_server_progress(rt_server* server);
  * server's context). */
 int rt_server_poll(rt_server* server, int64_t ticket, double* theta, double* tau,
                    rt_solver_perf perf[6], double* latency_ms);
+/* Writes up to max tickets that finished or failed since the last call,
+ * oldest first, and returns how many; a caller then polls only those instead
+ * of every pending ticket. */
+int rt_server_finished(rt_server* server, int64_t* tickets, int max);
 
 #ifdef __cplusplus
 }

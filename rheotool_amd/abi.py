@@ -124,6 +124,7 @@ def lib():
             "rt_server_submit": (i32, [vp, vp, vp]),
             "rt_server_progress": (i32, [vp]),
             "rt_server_poll": (i32, [vp, C.c_int64, vp, vp, vp, vp]),
+            "rt_server_finished": (i32, [vp, vp, i32]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -472,6 +473,18 @@ class Server:
         if n < 0:
             raise RheoError(-n, lib().rt_last_error(self.ctx.h).decode())
         return n
+
+    def finished(self, max_n=256):
+        """Tickets that finished or failed since the last call."""
+        buf = (C.c_int64 * max_n)()
+        out = []
+        while True:
+            n = lib().rt_server_finished(self.h, buf, max_n)
+            if n < 0:
+                raise RheoError(-n, lib().rt_last_error(self.ctx.h).decode())
+            out.extend(buf[i] for i in range(n))
+            if n < max_n:
+                return out
 
     def poll(self, ticket):
         """None while pending, else dict(theta, tau, perf, latency_ms)."""

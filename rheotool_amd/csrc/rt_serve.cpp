@@ -51,6 +51,7 @@ struct rt_server {
     std::vector<rt_ctx*> slots;
     std::vector<Request*> slotReq;
     std::deque<Request*> queue;
+    std::deque<int64_t> finished;  // tickets done or failed, not yet reported by rt_server_finished
     std::map<int64_t, std::unique_ptr<Request>> all;
     int64_t nextTicket = 1;
     std::string defaultControls = "solver PBiCGStab; preconditioner DILU; tolerance 1e-10; relTol 0; "
@@ -69,6 +70,7 @@ void releaseRequest(rt_server* S, Request* r) {
         S->inflightBytes -= r->estBytes;
         r->slot = -1;
     }
+    S->finished.push_back(r->ticket);
     // the caller's input arrays are no longer read
     r->theta = r->eigVals = r->eigVecs = r->U = r->Ub = r->phi = r->gradU = nullptr;
 }
@@ -269,6 +271,16 @@ int rt_server_progress(rt_server* S) {
         if (S->slotReq[i]) ++live;
     }
     return live;
+}
+
+int rt_server_finished(rt_server* S, int64_t* tickets, int max) {
+    if (!S || (max > 0 && !tickets)) return -RT_E_BAD_ARG;
+    int n = 0;
+    while (n < max && !S->finished.empty()) {
+        tickets[n++] = S->finished.front();
+        S->finished.pop_front();
+    }
+    return n;
 }
 
 int rt_server_poll(rt_server* S, int64_t ticket, double* theta, double* tau, rt_solver_perf perf[6],

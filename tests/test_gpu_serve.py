@@ -119,8 +119,11 @@ def test_shared_mesh_requests_and_queueing():
         probe.close()
         srv = abi.Server(ctx, hbm_budget=200 << 20, max_inflight=2)
         ts = [srv.submit(dm, f, d, dt) for _ in range(5)]
+        seen = []
         while srv.progress() > 0:
-            pass
+            seen += srv.finished()
+        seen += srv.finished()
+        assert sorted(seen) == sorted(ts) and srv.finished() == []  # each finish reported once
         for t in ts:
             out = srv.poll(t)
             assert np.array_equal(out["theta"], alone[0])
