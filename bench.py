@@ -99,21 +99,89 @@ def min_pass_bytes(N, F, perf, mixed):
     return b_pre + K * (2 * b_a(w) + N * w) + 17 * N * w * iters + nref * b_ref + b_post
 
 
-def cpu_baseline(w, n_sample):
-    """The oracle (reference algorithm: PBiCGStab + DILU, FP64, one thread) on a
-    bounded sample of the same workload."""
+def cpu_threads():
+    """Threads of the all-cores baseline: OMP_NUM_THREADS when set (the host
+    cores assigned to this GPU), else the CPUs this process may use, at most
+    16."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def median_step(run, reps):
+    """Median of reps calls after one warm-up call; run() returns seconds."""
+    ts = [run() for _ in range(reps + 1)]
+    return float(np.median(ts[1:]))
+
+
+def cpu_baseline(w, n_single, n_all, reps=5):
+    """The oracle (the reference's algorithm: PBiCGStab + DILU, FP64) on
+    bounded samples of the same workload, each the median of reps steps
+    after one warm-up, every step from the same state:
+    (i) one thread, as one rank of the reference runs, on the n_single^3
+        version;
+    (ii) all cores on the n_all^3 version, decomposed into one "simple" slab
+        per thread as decomposePar does, DILU per slab and the processor
+        interfaces inside Amul.
+    Returns the single-thread sample's mesh, fields and result (the accuracy
+    check runs on it) and the cpu_baseline record."""
     from oracle import oracle as O
-    m, f = make_case(w, n_sample)
     p = O.params(w["model"], etaS=0.59, etaP=0.41, lam=w["lam"], alpha=w["kw"].get("alpha", 0.0),
                  L2=w["kw"].get("L2", 100.0))
-    t0 = time.perf_counter()
-    th, ev, vec, tau, perf = O.correct(m, p, f, w["dt"])
-    dt = time.perf_counter() - t0
-    return m, f, (th, tau), dict(value=m.nCells / dt, unit="cell-updates/s", cores=1, kind="port",
-                                 sample=f"one full stress step (oracle, PBiCGStab+DILU, FP64) on the "
-                                        f"{n_sample}^3 version of the same workload "
-                                        f"({m.nCells} cells), {dt:.2f} s",
-                                 iterations=[int(q.nIterations) for q in perf])
+    m, f = make_case(w, n_single)
+    out = {}
+
+    def serial():
+        t0 = time.perf_counter()
+        out["r"] = O.correct(m, p, f, w["dt"])
+        return time.perf_counter() - t0
+
+    def label(n, mesh):
+        return ("the 2D case itself" if mesh.emptyDir >= 0 else
+                f"the {n}^3 version of the same workload") + f" ({mesh.nCells} cells)"
+
+    t1 = median_step(serial, reps)
+    th, ev, vec, tau, perf = out["r"]
+    single = dict(value=m.nCells / t1, unit="cell-updates/s", cores=1, kind="port",
+                  sample=f"one stress step (oracle, PBiCGStab+DILU, FP64) on {label(n_single, m)}, "
+                         f"median of {reps} after a warm-up: {t1:.3f} s",
+                  iterations=[int(q.nIterations) for q in perf])
+    nt = cpu_threads()
+    mA, fA = (m, f) if n_all == n_single else make_case(w, n_all)
+    ext = np.ptp(mA.C, axis=0)
+    axis = 2 if ext[2] >= 0.99 * ext.max() else int(np.argmax(ext))
+    dd = O.Decomposed(mA, O.simple_domains(mA, nt, axis))
+
+    def decomposed():
+        dd.set_fields(fA)
+        t0 = time.perf_counter()
+        out["p"] = dd.correct(p, w["dt"])
+        return time.perf_counter() - t0
+
+    try:
+        tP = median_step(decomposed, reps)
+    finally:
+        dd.close()
+    cpu = dict(value=mA.nCells / tP, unit="cell-updates/s", cores=nt, kind="port",
+               sample=f"one stress step (oracle, PBiCGStab+DILU, FP64) on {label(n_all, mA)} decomposed "
+                      f"into {nt} slabs along axis {axis}, one thread each, median of {reps} after a "
+                      f"warm-up: {tP:.3f} s",
+               iterations=[int(q.nIterations) for q in out["p"]],
+               cpu=cpu_model(), omp_num_threads=os.environ.get("OMP_NUM_THREADS"),
+               single_thread=single)
+    return m, f, (th, tau), cpu
 
 
 SERVE_DESC = ("serving stream: {n} stress-step requests, Poisson arrivals at {rate}/s, 2D/3D "
@@ -214,7 +282,10 @@ def main():
     ap.add_argument("--renumber", default="auto", choices=["auto", "none", "rcm", "sfc"],
                     help="device cell order (auto: RCM when the input order has poor locality)")
     ap.add_argument("--no-fp64-compare", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=64)
+    ap.add_argument("--cpu-sample", type=int, default=64,
+                    help="cells per side of the one-thread CPU sample")
+    ap.add_argument("--cpu-sample-all", type=int, default=128,
+                    help="cells per side of the all-cores (decomposed) CPU sample")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the stress step "
@@ -312,14 +383,16 @@ def main():
     if not args.no_cpu_baseline and w["model"] != "Oldroyd-BLog" or (
             not args.no_cpu_baseline and args.workload == "channel"):
         ns = args.cpu_sample if args.workload != "channel" else n
-        ms, fs, (th_o, tau_o), cpu = cpu_baseline(w, ns)
+        na = args.cpu_sample_all if args.workload != "channel" else n
+        ms, fs, (th_o, tau_o), cpu = cpu_baseline(w, ns, na)
         dms = ctx.mesh(ms)
         es = new_model(ctx, dms, w, fs, w["precision"])
         es.correct(w["dt"])
         err_sample = dict(theta=rel_l2(es.theta(), th_o), tau=rel_l2(es.tau(), tau_o))
         es.close()
         dms.close()
-        log(f"[bench] cpu baseline {cpu['value']:.3e} cell-updates/s; GPU vs oracle on the sample "
+        log(f"[bench] cpu baseline {cpu['value']:.3e} cell-updates/s on {cpu['cores']} threads, "
+            f"{cpu['single_thread']['value']:.3e} on one; GPU vs oracle on the sample "
             f"{err_sample}")
 
     line = dict(

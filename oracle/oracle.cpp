@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <omp.h>
 #include <vector>
 
 namespace {
@@ -797,6 +798,10 @@ static void assembleImpl(const orc_mesh* m, const double* phi, const double* the
                 } else if (bcType[p] == ORC_BC_FIXEDVALUE) {
                     ic = pphi * 0.0;
                     bc = -pphi * bcValue[6 * (size_t)b + q];
+                } else if (bcType[p] == ORC_BC_COUPLED) {
+                    // [OF9] coupled upwind: weights pos0(phi); the boundary
+                    // coefficient becomes the interface coefficient, not a source
+                    ic = pphi * (pphi >= 0 ? 1.0 : 0.0);
                 }
                 iC[6 * (size_t)b + q] = ic;
                 bC[6 * (size_t)b + q] = bc;
@@ -1233,6 +1238,446 @@ void orc_correct_aux(const orc_mesh* m, const orc_params* p, const double* U, co
     if (p->model == ORC_FLUIDITY) return;
     orc_calc_eig(N, theta, eigVals, eigVecs, 0);
     orc_tau(p, N, eigVals, eigVecs, f.data(), tau);
+}
+
+}  // extern "C"
+
+// ---------------- the step of a decomposed run ---------------------------
+// [OF9] decomposePar + processor patches: one thread per domain stands for
+// one MPI rank.  Threads share memory, so the halo "exchange" of a processor
+// patch is a read of the other domain's values after a barrier, and gSum is
+// a sum of per-domain partials in domain order (every thread forms the same
+// total, as every rank does after the all-reduce).
+namespace {
+
+struct Domain {
+    int n = 0, off = 0;          // cells [off, off + n) of the domain-major order
+    int Fi = 0, Nb = 0, Np = 0;  // internal, boundary and processor faces
+    std::vector<int32_t> owner, neighbour;  // local cells; owner [Fi + Nb + Np]
+    std::vector<double> Sf, w, V;           // Sf [3 (Fi + Nb + Np)], outward for processor faces
+    std::vector<int32_t> patchStart, patchSize, patchEmpty;  // the mesh's patches, then the processor patch
+    std::vector<int64_t> intFace, bndFace;  // original internal face / boundary-face index
+    // processor faces: original owner and neighbour (domain-major), linear
+    // weight, the other domain's cell (domain-major), the original face and
+    // whether the local cell is its owner
+    std::vector<int32_t> pO, pN, pRemote;
+    std::vector<int64_t> pFace;
+    std::vector<int8_t> pOwnerSide;
+    std::vector<double> pW;
+    std::vector<double> Ub, phi, bcValue;  // [Nb + Np] face values (processor part of bcValue unused)
+    std::vector<int32_t> bcType;           // [nPatches + 1]
+    orc_mesh mesh{};
+};
+
+// gSum over domains of up to four values; two slot banks alternate so a
+// bank is rewritten only after every thread has passed the barrier that
+// follows its last read.
+struct Reducer {
+    double* slots;
+    int P, s, k;
+    void sum(double* v, int m) {
+        double* bank = slots + (size_t)(k++ & 1) * P * 4;
+        for (int i = 0; i < m; ++i) bank[4 * s + i] = v[i];
+#pragma omp barrier
+        for (int i = 0; i < m; ++i) {
+            double a = 0;
+            for (int r = 0; r < P; ++r) a += bank[4 * r + i];
+            v[i] = a;
+        }
+    }
+    double sum1(double x) {
+        sum(&x, 1);
+        return x;
+    }
+};
+
+// [OF9] lduMatrix::Amul: the domain's own product, then
+// updateMatrixInterfaces: result[faceCell] -= interfaceBouCoeff * neighbour value.
+void amulDD(const Ldu& A, const Domain& D, const double* cc, const double* xAll, double* y) {
+    Amul(A, xAll + D.off, y);
+    const int32_t* fc = D.owner.data() + D.Fi + D.Nb;
+    for (int k = 0; k < D.Np; ++k) y[fc[k]] -= cc[k] * xAll[D.pRemote[k]];
+}
+
+// [OF9] lduMatrix::solver::normFactor with sumA's interface terms
+// (sumA[faceCell] -= interfaceBouCoeff), gAverage and gSum.
+double normFactorDD(const Ldu& A, const Domain& D, const double* cc, const double* psi, const double* b,
+                    const double* Apsi, int nTotal, Reducer& R) {
+    std::vector<double> t(A.N);
+    sumA(A, t.data());
+    const int32_t* fc = D.owner.data() + D.Fi + D.Nb;
+    for (int k = 0; k < D.Np; ++k) t[fc[k]] -= cc[k];
+    double avg = 0;
+    for (int c = 0; c < A.N; ++c) avg += psi[c];
+    avg = R.sum1(avg);
+    avg /= nTotal;
+    for (int c = 0; c < A.N; ++c) t[c] *= avg;
+    double s = 0;
+    for (int c = 0; c < A.N; ++c) s += std::fabs(Apsi[c] - t[c]) + std::fabs(b[c] - t[c]);
+    return R.sum1(s) + 1e-20;
+}
+
+// pbicgstab() above on one domain: the same recurrence, with Amul across
+// the interfaces, DILU on the domain's own matrix and global reductions.
+// psi, yA and zA live in the shared domain-major vectors xAll/yAll/zAll
+// because Amul reads them across the interfaces.
+orc_perf pbicgstabDD(const Ldu& A, const Domain& D, const double* cc, double* xAll, double* yAll,
+                     double* zAll, const double* b, const orc_controls* ctl, int nTotal, Reducer& R) {
+    const int N = A.N;
+    double* psi = xAll + D.off;
+    double* yA = yAll + D.off;
+    double* zA = zAll + D.off;
+    orc_perf perf{0, 0, 0, 0, 1};
+    std::vector<double> pA(N), rA(N);
+#pragma omp barrier
+    amulDD(A, D, cc, xAll, yA);
+    for (int c = 0; c < N; ++c) rA[c] = b[c] - yA[c];
+    const double nf = normFactorDD(A, D, cc, psi, b, yA, nTotal, R);
+    perf.initialResidual = R.sum1(sumMag(rA)) / nf;
+    perf.finalResidual = perf.initialResidual;
+    if (ctl->minIter > 0 || !converged(perf.finalResidual, perf.initialResidual, ctl)) {
+        std::vector<double> AyA(N), sA(N), tA(N), rA0(rA);
+        double rA0rA = 0, alpha = 0, omega = 0;
+        Precond P(A, ctl->precond);
+        do {
+            const double rA0rAold = rA0rA;
+            rA0rA = R.sum1(sumProd(rA0.data(), rA.data(), N));
+            if (std::fabs(rA0rA) < 1e-300) break;
+            if (perf.nIterations == 0) {
+                pA = rA;
+            } else {
+                if (std::fabs(omega) < 1e-300) break;
+                const double beta = (rA0rA / rA0rAold) * (alpha / omega);
+                for (int c = 0; c < N; ++c) pA[c] = rA[c] + beta * (pA[c] - omega * AyA[c]);
+            }
+            P.apply(yA, pA.data());
+#pragma omp barrier
+            amulDD(A, D, cc, yAll, AyA.data());
+            const double rA0AyA = R.sum1(sumProd(rA0.data(), AyA.data(), N));
+            alpha = rA0rA / rA0AyA;
+            for (int c = 0; c < N; ++c) sA[c] = rA[c] - alpha * AyA[c];
+            perf.finalResidual = R.sum1(sumMag(sA)) / nf;
+            if (converged(perf.finalResidual, perf.initialResidual, ctl)) {
+                for (int c = 0; c < N; ++c) psi[c] += alpha * yA[c];
+                perf.nIterations++;
+                perf.converged = 1;
+                return perf;
+            }
+            P.apply(zA, sA.data());
+#pragma omp barrier
+            amulDD(A, D, cc, zAll, tA.data());
+            double tt[2] = {sumProd(tA.data(), tA.data(), N), sumProd(tA.data(), sA.data(), N)};
+            R.sum(tt, 2);
+            omega = tt[1] / tt[0];
+            for (int c = 0; c < N; ++c) {
+                psi[c] += alpha * yA[c] + omega * zA[c];
+                rA[c] = sA[c] - omega * tA[c];
+            }
+            perf.finalResidual = R.sum1(sumMag(rA)) / nf;
+        } while ((++perf.nIterations < ctl->maxIter &&
+                  !converged(perf.finalResidual, perf.initialResidual, ctl)) ||
+                 perf.nIterations < ctl->minIter);
+    }
+    perf.converged = converged(perf.finalResidual, perf.initialResidual, ctl) ? 1 : 0;
+    return perf;
+}
+
+}  // namespace
+
+struct orc_decomp {
+    int P = 0, N = 0, nPatches = 0, F = 0;
+    std::vector<int32_t> order, pos;  // domain-major position <-> original cell
+    std::vector<Domain> dom;
+    // domain-major fields and work vectors, each page first written by the
+    // thread of the domain that owns it
+    std::vector<double*> bufs;
+    double *U = nullptr, *theta = nullptr, *eigVals = nullptr, *eigVecs = nullptr, *tau = nullptr;
+    double *g = nullptr, *S = nullptr, *f = nullptr, *xh = nullptr, *yh = nullptr, *zh = nullptr;
+    std::vector<double> red;
+    ~orc_decomp() {
+        for (double* b : bufs) delete[] b;
+    }
+};
+
+namespace {
+
+// one domain's step: orc_correct_aux's phases on the domain's cells
+void domainStep(orc_decomp* d, int s, const orc_params* p, double deltaT, const int32_t valid[6],
+                const orc_controls* ctl, int scheme, orc_perf perf[6]) {
+    Domain& D = d->dom[s];
+    const int n = D.n, off = D.off, Fi = D.Fi, NB = D.Nb + D.Np;
+    Reducer R{d->red.data(), d->P, s, 0};
+    // coupled patch: the linear interpolate of U from both sides, formed
+    // as the undecomposed face loop forms it
+    for (int k = 0; k < D.Np; ++k) {
+        const int o = D.pO[k], nb = D.pN[k];
+        for (int q = 0; q < 3; ++q)
+            D.Ub[3 * (size_t)(D.Nb + k) + q] = D.pW[k] * (d->U[3 * (size_t)o + q] - d->U[3 * (size_t)nb + q]) +
+                                               d->U[3 * (size_t)nb + q];
+    }
+    double* g = d->g + 9 * (size_t)off;
+    double* th = d->theta + 6 * (size_t)off;
+    double* ev = d->eigVals + 9 * (size_t)off;
+    double* vec = d->eigVecs + 9 * (size_t)off;
+    double* tau = d->tau + 6 * (size_t)off;
+    double* S = d->S + 6 * (size_t)off;
+    double* f = d->f + off;
+    orc_grad_gauss(&D.mesh, d->U + 3 * (size_t)off, D.Ub.data(), g);
+    std::fill(f, f + n, 0.0);
+    orc_rhs_tau(p, n, g, th, ev, vec, tau, nullptr, S, f);
+    std::vector<double> diag(n), lower(Fi), upper(Fi), source(6 * (size_t)n), iC(6 * (size_t)NB),
+        bC(6 * (size_t)NB);
+    assembleImpl(&D.mesh, D.phi.data(), th, S, deltaT, D.bcType.data(), D.bcValue.data(), scheme, nullptr,
+                 diag.data(), lower.data(), upper.data(), source.data(), iC.data(), bC.data());
+    // interfaceBouCoeffs: the coupled patch's boundaryCoeffs, -phi_b (1 - pos0(phi_b))
+    std::vector<double> cc(D.Np);
+    for (int k = 0; k < D.Np; ++k) {
+        const double pphi = D.phi[(size_t)Fi + D.Nb + k];
+        cc[k] = -pphi * (1.0 - (pphi >= 0 ? 1.0 : 0.0));
+    }
+    // orc_solve_segregated on the domain.  [OF9] solveSegregated adds the
+    // coupled patches' boundaryCoeffs x neighbour value to the source and
+    // the interface update takes the same product off again; for these
+    // component-wise coefficients the two cancel and both are left out.
+    for (int pp = 0; pp < D.mesh.nPatches; ++pp) {
+        if (D.patchEmpty[pp] || D.bcType[pp] == ORC_BC_EMPTY || D.bcType[pp] == ORC_BC_COUPLED) continue;
+        for (int k = 0; k < D.patchSize[pp]; ++k) {
+            const int b = D.patchStart[pp] + k;
+            const int cell = D.owner[(size_t)Fi + b];
+            for (int q = 0; q < 6; ++q) source[6 * (size_t)cell + q] += bC[6 * (size_t)b + q];
+        }
+    }
+    std::vector<double> dg(n), bb(n);
+    double* x = d->xh + off;
+    for (int q = 0; q < 6; ++q) {
+        orc_perf pf{0, 0, 0, 0, 0};
+        if (valid[q] != -1) {
+            std::copy(diag.begin(), diag.end(), dg.begin());
+            for (int pp = 0; pp < D.mesh.nPatches; ++pp) {
+                if (D.patchEmpty[pp] || D.bcType[pp] == ORC_BC_EMPTY) continue;
+                for (int k = 0; k < D.patchSize[pp]; ++k) {
+                    const int b = D.patchStart[pp] + k;
+                    dg[D.owner[(size_t)Fi + b]] += iC[6 * (size_t)b + q];
+                }
+            }
+            for (int c = 0; c < n; ++c) {
+                x[c] = th[6 * (size_t)c + q];
+                bb[c] = source[6 * (size_t)c + q];
+            }
+            Ldu A{n, Fi, D.owner.data(), D.neighbour.data(), dg.data(),
+                  lower.data(), upper.data()};
+            pf = pbicgstabDD(A, D, cc.data(), d->xh, d->yh, d->zh, bb.data(), ctl, d->N, R);
+            for (int c = 0; c < n; ++c) th[6 * (size_t)c + q] = x[c];
+        }
+        if (s == 0) perf[q] = pf;
+    }
+    orc_calc_eig(n, th, ev, vec, 0);
+    orc_tau(p, n, ev, vec, f, tau);
+}
+
+}  // namespace
+
+extern "C" {
+
+orc_decomp* orc_decomp_create(const orc_mesh* m, int32_t P, const int32_t* domainOf) {
+    const int N = m->nCells, F = m->nInternal;
+    if (P < 1) return nullptr;
+    for (int c = 0; c < N; ++c)
+        if (domainOf[c] < 0 || domainOf[c] >= P) return nullptr;
+    auto* d = new orc_decomp;
+    d->P = P;
+    d->N = N;
+    d->F = F;
+    d->nPatches = m->nPatches;
+    d->dom.resize(P);
+    std::vector<int32_t> start(P + 1, 0);
+    for (int c = 0; c < N; ++c) start[domainOf[c] + 1]++;
+    for (int r = 0; r < P; ++r) start[r + 1] += start[r];
+    d->order.resize(N);
+    d->pos.resize(N);
+    {
+        std::vector<int32_t> next(start.begin(), start.end() - 1);
+        for (int c = 0; c < N; ++c) {
+            const int k = next[domainOf[c]]++;
+            d->order[k] = c;
+            d->pos[c] = k;
+        }
+    }
+    // faces of each domain, in the undecomposed face order
+    std::vector<std::vector<int64_t>> iF(P), pF(P);
+    std::vector<std::vector<std::vector<int64_t>>> bF(P, std::vector<std::vector<int64_t>>(m->nPatches));
+    for (int f = 0; f < F; ++f) {
+        const int a = domainOf[m->owner[f]], b = domainOf[m->neighbour[f]];
+        if (a == b) {
+            iF[a].push_back(f);
+        } else {
+            pF[a].push_back(2 * (int64_t)f + 1);
+            pF[b].push_back(2 * (int64_t)f);
+        }
+    }
+    for (int pp = 0; pp < m->nPatches; ++pp)
+        for (int k = 0; k < m->patchSize[pp]; ++k) {
+            const int b = m->patchStart[pp] + k;
+            bF[domainOf[m->owner[(int64_t)F + b]]][pp].push_back(b);
+        }
+    d->red.assign(2 * (size_t)P * 4, 0.0);
+    d->bufs.assign(11, nullptr);
+#pragma omp parallel for schedule(static, 1) num_threads(P)
+    for (int r = 0; r < P; ++r) {
+        Domain& D = d->dom[r];
+        D.off = start[r];
+        D.n = start[r + 1] - start[r];
+        D.Fi = (int)iF[r].size();
+        D.Np = (int)pF[r].size();
+        D.Nb = 0;
+        for (int pp = 0; pp < m->nPatches; ++pp) D.Nb += (int)bF[r][pp].size();
+        const size_t nf = (size_t)D.Fi + D.Nb + D.Np;
+        D.owner.resize(nf);
+        D.neighbour.resize(D.Fi);
+        D.Sf.resize(3 * nf);
+        D.w.resize(D.Fi);
+        D.V.resize(D.n);
+        for (int c = 0; c < D.n; ++c) D.V[c] = m->V[d->order[D.off + c]];
+        D.intFace = iF[r];
+        for (int k = 0; k < D.Fi; ++k) {
+            const int64_t f = iF[r][k];
+            D.owner[k] = d->pos[m->owner[f]] - D.off;
+            D.neighbour[k] = d->pos[m->neighbour[f]] - D.off;
+            D.w[k] = m->w[f];
+            for (int q = 0; q < 3; ++q) D.Sf[3 * (size_t)k + q] = m->Sf[3 * f + q];
+        }
+        D.patchStart.resize(m->nPatches + 1);
+        D.patchSize.resize(m->nPatches + 1);
+        D.patchEmpty.resize(m->nPatches + 1);
+        int run = 0;
+        for (int pp = 0; pp < m->nPatches; ++pp) {
+            D.patchStart[pp] = run;
+            D.patchSize[pp] = (int)bF[r][pp].size();
+            D.patchEmpty[pp] = m->patchEmpty[pp];
+            for (int64_t b : bF[r][pp]) {
+                const size_t lf = (size_t)D.Fi + run;
+                D.owner[lf] = d->pos[m->owner[(int64_t)F + b]] - D.off;
+                for (int q = 0; q < 3; ++q) D.Sf[3 * lf + q] = m->Sf[3 * ((int64_t)F + b) + q];
+                D.bndFace.push_back(b);
+                ++run;
+            }
+        }
+        D.patchStart[m->nPatches] = run;
+        D.patchSize[m->nPatches] = D.Np;
+        D.patchEmpty[m->nPatches] = 0;
+        D.pO.resize(D.Np);
+        D.pN.resize(D.Np);
+        D.pRemote.resize(D.Np);
+        D.pFace.resize(D.Np);
+        D.pOwnerSide.resize(D.Np);
+        D.pW.resize(D.Np);
+        for (int k = 0; k < D.Np; ++k) {
+            const int64_t f = pF[r][k] / 2;
+            const bool own = (pF[r][k] & 1) != 0;
+            const int o = d->pos[m->owner[f]], nb = d->pos[m->neighbour[f]];
+            const size_t lf = (size_t)D.Fi + D.Nb + k;
+            D.owner[lf] = (own ? o : nb) - D.off;
+            const double sg = own ? 1.0 : -1.0;
+            for (int q = 0; q < 3; ++q) D.Sf[3 * lf + q] = sg * m->Sf[3 * f + q];
+            D.pO[k] = o;
+            D.pN[k] = nb;
+            D.pRemote[k] = own ? nb : o;
+            D.pFace[k] = f;
+            D.pOwnerSide[k] = own ? 1 : 0;
+            D.pW[k] = m->w[f];
+        }
+        D.Ub.assign(3 * (size_t)(D.Nb + D.Np), 0.0);
+        D.phi.assign(nf, 0.0);
+        D.bcValue.assign(6 * (size_t)(D.Nb + D.Np), 0.0);
+        D.bcType.assign(m->nPatches + 1, ORC_BC_COUPLED);
+        D.mesh = orc_mesh{D.n,
+                          D.Fi,
+                          D.Nb + D.Np,
+                          m->nPatches + 1,
+                          D.owner.data(),
+                          D.neighbour.data(),
+                          D.Sf.data(),
+                          D.w.data(),
+                          D.V.data(),
+                          D.patchStart.data(),
+                          D.patchSize.data(),
+                          D.patchEmpty.data()};
+    }
+    const size_t widths[11] = {3, 6, 9, 9, 6, 9, 6, 1, 1, 1, 1};
+    for (int i = 0; i < 11; ++i) d->bufs[i] = new double[widths[i] * (size_t)N];
+#pragma omp parallel for schedule(static, 1) num_threads(P)
+    for (int r = 0; r < P; ++r)
+        for (int i = 0; i < 11; ++i)
+            std::fill(d->bufs[i] + widths[i] * (size_t)start[r], d->bufs[i] + widths[i] * (size_t)start[r + 1],
+                      0.0);
+    d->U = d->bufs[0];
+    d->theta = d->bufs[1];
+    d->eigVals = d->bufs[2];
+    d->eigVecs = d->bufs[3];
+    d->tau = d->bufs[4];
+    d->g = d->bufs[5];
+    d->S = d->bufs[6];
+    d->f = d->bufs[7];
+    d->xh = d->bufs[8];
+    d->yh = d->bufs[9];
+    d->zh = d->bufs[10];
+    return d;
+}
+
+void orc_decomp_destroy(orc_decomp* d) { delete d; }
+
+void orc_decomp_set_fields(orc_decomp* d, const double* U, const double* Ub, const double* phi,
+                           const double* theta, const double* eigVals, const double* eigVecs,
+                           const double* tau, const double* bcValue) {
+    const int N = d->N, F = d->F;
+    for (int k = 0; k < N; ++k) {
+        const int c = d->order[k];
+        std::copy(U + 3 * (size_t)c, U + 3 * (size_t)c + 3, d->U + 3 * (size_t)k);
+        std::copy(theta + 6 * (size_t)c, theta + 6 * (size_t)c + 6, d->theta + 6 * (size_t)k);
+        std::copy(eigVals + 9 * (size_t)c, eigVals + 9 * (size_t)c + 9, d->eigVals + 9 * (size_t)k);
+        std::copy(eigVecs + 9 * (size_t)c, eigVecs + 9 * (size_t)c + 9, d->eigVecs + 9 * (size_t)k);
+        for (int q = 0; q < 6; ++q) d->tau[6 * (size_t)k + q] = tau ? tau[6 * (size_t)c + q] : 0.0;
+    }
+    for (Domain& D : d->dom) {
+        for (int k = 0; k < D.Fi; ++k) D.phi[k] = phi[D.intFace[k]];
+        for (int k = 0; k < D.Nb; ++k) {
+            const int64_t b = D.bndFace[k];
+            D.phi[(size_t)D.Fi + k] = phi[F + b];
+            for (int q = 0; q < 3; ++q) D.Ub[3 * (size_t)k + q] = Ub ? Ub[3 * b + q] : 0.0;
+            for (int q = 0; q < 6; ++q) D.bcValue[6 * (size_t)k + q] = bcValue ? bcValue[6 * b + q] : 0.0;
+        }
+        for (int k = 0; k < D.Np; ++k)
+            D.phi[(size_t)D.Fi + D.Nb + k] = D.pOwnerSide[k] ? phi[D.pFace[k]] : -phi[D.pFace[k]];
+    }
+}
+
+int32_t orc_decomp_correct(orc_decomp* d, const orc_params* p, double deltaT, const int32_t* bcType,
+                           const int32_t valid[6], const orc_controls* c, int32_t scheme, orc_perf perf[6]) {
+    if ((scheme & ~ORC_DC_BOUNDED) != ORC_DC_UPWIND || c->solver != ORC_PBICGSTAB ||
+        p->model == ORC_BMP || p->model == ORC_FLUIDITY)
+        return -1;
+    for (Domain& D : d->dom)
+        for (int pp = 0; pp < d->nPatches; ++pp) D.bcType[pp] = bcType[pp];
+    int bad = 0;
+    const int P = d->P;
+#pragma omp parallel num_threads(P)
+    {
+        if (omp_get_num_threads() == P)
+            domainStep(d, omp_get_thread_num(), p, deltaT, valid, c, scheme, perf);
+        else if (omp_get_thread_num() == 0)
+            bad = 1;
+    }
+    return bad ? -1 : 0;
+}
+
+void orc_decomp_get(const orc_decomp* d, double* theta, double* eigVals, double* eigVecs, double* tau) {
+    for (int k = 0; k < d->N; ++k) {
+        const int c = d->order[k];
+        std::copy(d->theta + 6 * (size_t)k, d->theta + 6 * (size_t)k + 6, theta + 6 * (size_t)c);
+        std::copy(d->eigVals + 9 * (size_t)k, d->eigVals + 9 * (size_t)k + 9, eigVals + 9 * (size_t)c);
+        std::copy(d->eigVecs + 9 * (size_t)k, d->eigVecs + 9 * (size_t)k + 9, eigVecs + 9 * (size_t)c);
+        std::copy(d->tau + 6 * (size_t)k, d->tau + 6 * (size_t)k + 6, tau + 6 * (size_t)c);
+    }
 }
 
 }  // extern "C"

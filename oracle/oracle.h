@@ -67,7 +67,11 @@ typedef struct {
     double G0, kBMP, Phi0, PhiInf;
 } orc_params;
 
-enum { ORC_BC_ZEROGRADIENT = 0, ORC_BC_FIXEDVALUE = 1, ORC_BC_EMPTY = 2 };
+/* ORC_BC_COUPLED: a processor patch of a decomposed run.  The upwind
+ * convection puts phi_b*pos0(phi_b) on the diagonal (internalCoeffs) and
+ * nothing in the source; -phi_b*(1 - pos0(phi_b)) multiplies the
+ * neighbouring domain's value inside Amul (interfaceBouCoeffs). */
+enum { ORC_BC_ZEROGRADIENT = 0, ORC_BC_FIXEDVALUE = 1, ORC_BC_EMPTY = 2, ORC_BC_COUPLED = 3 };
 
 enum { ORC_PBICGSTAB = 0, ORC_PBICG = 1 };
 enum { ORC_PC_NONE = 0, ORC_PC_DIAGONAL = 1, ORC_PC_DILU = 2 };
@@ -232,6 +236,42 @@ void orc_correct_aux(const orc_mesh* m, const orc_params* p, const double* U, co
                      const orc_controls* c, double relaxFactor, int32_t scheme, const double* C,
                      const double* thetaOld, const double* aux, double* theta, double* eigVals,
                      double* eigVecs, double* tau, orc_perf perf[6]);
+
+/* ---- The stress step as a decomposed (parallel) OpenFOAM run executes it ----
+ * decomposePar splits the cells into nDomains domains; each domain holds its
+ * cells in their original relative order, its internal faces, its share of
+ * every boundary patch and one processor patch with the faces it shares with
+ * other domains.  A step runs one thread per domain (one MPI rank each in
+ * the reference) through the same phases as orc_correct, with:
+ *   - the processor faces' U interpolated linearly from both sides
+ *     (coupled patch, [OF9] processorFvPatchField);
+ *   - their upwind coefficients split into the diagonal and the interface
+ *     coefficient (ORC_BC_COUPLED), applied inside Amul and sumA
+ *     ([OF9] lduMatrix::Amul, updateMatrixInterfaces, sumA);
+ *   - DILU on each domain's own matrix, the interfaces left out
+ *     ([OF9] DILUPreconditioner works on the local lduAddressing only);
+ *   - residuals and inner products summed over domains (gSum).
+ * With one domain the step is the serial orc_correct operation for
+ * operation.  Only Gauss upwind (optionally bounded), PBiCGStab, no relax()
+ * and Gauss-linear gradU are supported (the configuration timed as the CPU
+ * baseline); the step returns -1 for anything else and for a thread team
+ * smaller than nDomains.
+ *
+ * domainOf[nCells]: the domain of each cell, 0 <= d < nDomains.  Fields are
+ * in the original numbering in set_fields/get; the step works on the
+ * domain-major copies the object holds. */
+typedef struct orc_decomp orc_decomp;
+orc_decomp* orc_decomp_create(const orc_mesh* m, int32_t nDomains, const int32_t* domainOf);
+void orc_decomp_destroy(orc_decomp* d);
+/* U[3N], Ub[3 nBoundary], phi[nInternal + nBoundary], theta[6N],
+ * eigVals/eigVecs[9N], tau[6N] (read by Saramito), bcValue[6 nBoundary] */
+void orc_decomp_set_fields(orc_decomp* d, const double* U, const double* Ub, const double* phi,
+                           const double* theta, const double* eigVals, const double* eigVecs,
+                           const double* tau, const double* bcValue);
+int32_t orc_decomp_correct(orc_decomp* d, const orc_params* p, double deltaT, const int32_t* bcType,
+                           const int32_t validComponents[6], const orc_controls* c, int32_t scheme,
+                           orc_perf perf[6]);
+void orc_decomp_get(const orc_decomp* d, double* theta, double* eigVals, double* eigVecs, double* tau);
 
 #ifdef __cplusplus
 }

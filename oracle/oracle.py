@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
 OLDROYDB, GIESEKUS, FENEP, PTT_LINEAR, PTT_EXP = 0, 1, 2, 3, 4
-BC_ZEROGRADIENT, BC_FIXEDVALUE, BC_EMPTY = 0, 1, 2
+BC_ZEROGRADIENT, BC_FIXEDVALUE, BC_EMPTY, BC_COUPLED = 0, 1, 2, 3
 PBICGSTAB, PBICG = 0, 1
 PC_NONE, PC_DIAGONAL, PC_DILU = 0, 1, 2
 
@@ -99,6 +99,13 @@ def lib():
         L.orc_defc_sout.argtypes = [vp] * 4
         L.orc_div_tau.argtypes = [vp, vp, vp, C.c_double, vp]
         L.orc_tau_total.argtypes = [vp, vp, vp, vp, C.c_double, vp]
+        L.orc_decomp_create.restype = vp
+        L.orc_decomp_create.argtypes = [vp, C.c_int32, vp]
+        L.orc_decomp_destroy.argtypes = [vp]
+        L.orc_decomp_set_fields.argtypes = [vp] * 9
+        L.orc_decomp_correct.restype = C.c_int32
+        L.orc_decomp_correct.argtypes = [vp, vp, C.c_double, vp, vp, vp, C.c_int32, vp]
+        L.orc_decomp_get.argtypes = [vp] * 5
         _LIB = L
     return _LIB
 
@@ -360,3 +367,80 @@ def bmp_correct(m, p, fields, dt, Phi, tau_old, ctl=None, bc=None, bcval=None, r
     th, ev, vec, t, perf = correct(m, p, fields, dt, ctl=ctl, bc=bc, bcval=bcval, gradU=gradU, relax=relax,
                                    scheme=scheme, theta_old=theta_old, tau_old=tau_old, aux=Pn)
     return th, ev, vec, t, perf, Pn[:, 0].copy(), perf_phi
+
+
+def simple_domains(m, n, axis=2):
+    """[OF9] decomposePar's "simple" method with n domains along one axis:
+    cells sorted by centre coordinate (ties by cell number), cut into n
+    chunks of equal count."""
+    order = np.lexsort((np.arange(m.nCells), np.asarray(m.C)[:, axis]))
+    dom = np.empty(m.nCells, dtype=np.int32)
+    dom[order] = (np.arange(m.nCells, dtype=np.int64) * n // m.nCells).astype(np.int32)
+    return dom
+
+
+class Decomposed:
+    """The stress step of a decomposed run, one thread per domain
+    (orc_decomp_*): set_fields() distributes the fields (decomposePar),
+    correct() runs one step, get() gathers the result (reconstructPar)."""
+
+    def __init__(self, m, domains):
+        self.m = m
+        self.om = OracleMesh(m)
+        self.domains = np.ascontiguousarray(domains, dtype=np.int32)
+        self.n = int(self.domains.max()) + 1
+        self.h = lib().orc_decomp_create(self.om.ref, self.n, _p(self.domains))
+        if not self.h:
+            raise ValueError("invalid domain assignment")
+
+    def set_fields(self, fields, tau_old=None, bcval=None):
+        NB = self.m.nBoundary
+        Ub = fields.Ub if NB else np.zeros((1, 3))
+        self._keep = [np.ascontiguousarray(a, dtype=np.float64) for a in
+                      (fields.U, Ub, fields.phi, fields.theta, fields.eigVals, fields.eigVecs)]
+        t = None if tau_old is None else np.ascontiguousarray(tau_old, dtype=np.float64)
+        bv = None if bcval is None else np.ascontiguousarray(bcval, dtype=np.float64)
+        lib().orc_decomp_set_fields(self.h, *[_p(a) for a in self._keep], _p(t), _p(bv))
+
+    def correct(self, p, dt, ctl=None, bc=None, valid=None, scheme="upwind"):
+        bc = default_bc(self.m) if bc is None else np.ascontiguousarray(bc, dtype=np.int32)
+        va = np.ascontiguousarray(self.m.validComponents if valid is None else valid, dtype=np.int32)
+        ctl = controls() if ctl is None else ctl
+        perf = (Perf * 6)()
+        rc = lib().orc_decomp_correct(self.h, C.byref(p), dt, _p(bc), _p(va), C.byref(ctl),
+                                      scheme_id(scheme), perf)
+        if rc != 0:
+            raise ValueError("decomposed step: unsupported scheme, solver or model, "
+                             "or fewer threads than domains")
+        return [perf[i] for i in range(6)]
+
+    def get(self):
+        N = self.m.nCells
+        th, t = np.empty((N, 6)), np.empty((N, 6))
+        ev, vec = np.empty((N, 3, 3)), np.empty((N, 3, 3))
+        lib().orc_decomp_get(self.h, _p(th), _p(ev), _p(vec), _p(t))
+        return th, ev, vec, t
+
+    def close(self):
+        if self.h:
+            lib().orc_decomp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def decomposed_correct(m, p, fields, dt, n_domains, ctl=None, bc=None, bcval=None, tau_old=None,
+                       valid=None, scheme="upwind", domains=None):
+    """correct() as a run decomposed into n_domains "simple" domains along z
+    executes it; returns (theta, eigVals, eigVecs, tau, perf)."""
+    d = Decomposed(m, simple_domains(m, n_domains) if domains is None else domains)
+    try:
+        d.set_fields(fields, tau_old=tau_old, bcval=bcval)
+        perf = d.correct(p, dt, ctl=ctl, bc=bc, valid=valid, scheme=scheme)
+        return (*d.get(), perf)
+    finally:
+        d.close()
