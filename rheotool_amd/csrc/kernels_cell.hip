@@ -788,11 +788,23 @@ __global__ void k_unpermute_rows(int32_t N, int ncIn, int nc, const double* __re
     for (int q = 0; q < nc; ++q) out[c * nc + q] = in[r * ncIn + q];
 }
 
-// plain streaming copy (16-byte vectors), the HBM calibration kernel
+// plain streaming copy, the HBM calibration kernel: each lane moves COPY_U
+// 16-byte vectors, all loads issued before the stores
+constexpr int COPY_U = 4;
 __global__ void __launch_bounds__(BLK) k_copy(int64_t n16, const double2* __restrict__ src,
                                               double2* __restrict__ dst) {
-    for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n16; i += (int64_t)gridDim.x * BLK)
-        dst[i] = src[i];
+    const int64_t base = (int64_t)blockIdx.x * BLK * COPY_U + threadIdx.x;
+    double2 v[COPY_U];
+#pragma unroll
+    for (int u = 0; u < COPY_U; ++u) {
+        const int64_t i = base + (int64_t)u * BLK;
+        if (i < n16) v[u] = src[i];
+    }
+#pragma unroll
+    for (int u = 0; u < COPY_U; ++u) {
+        const int64_t i = base + (int64_t)u * BLK;
+        if (i < n16) dst[i] = v[u];
+    }
 }
 
 // ---------------------------------------------------------------- launchers
@@ -1203,7 +1215,7 @@ void launch_eigvals_to_tensor(Ctx* ctx, int32_t N, const double* ev, const int32
 }
 void launch_copy(Ctx* ctx, const void* src, void* dst, size_t bytes) {
     const int64_t n16 = (int64_t)(bytes / 16);
-    const unsigned grid = (unsigned)std::min<int64_t>(nblk(n16), (int64_t)ctx->numCUs * 16);
+    const unsigned grid = (unsigned)((n16 + (int64_t)BLK * COPY_U - 1) / ((int64_t)BLK * COPY_U));
     ProfScope ps(ctx, "k_copy", 2.0 * (double)n16 * 16);
     hipLaunchKernelGGL(k_copy, dim3(grid), dim3(BLK), 0, ctx->stream, n16, (const double2*)src, (double2*)dst);
     RT_HIP(hipGetLastError());

@@ -91,20 +91,24 @@ __global__ void __launch_bounds__(BLK) k_residual(KryArgs<T> a) {
         }
         double bb[6];
         ld6(a.b + (int64_t)NCMP * r, bb);
+        // r and r0 are written whole (three 2-wide stores each): the entries of
+        // components that are off belong to finished or frozen solves, which
+        // no later kernel reads
+        T rt[6];
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
-            if (!on[c]) continue;
             const double res = bb[c] - y[c];
-            const T rt = (T)res;
-            a.r[(int64_t)NCMP * r + c] = rt;
-            a.r0[(int64_t)NCMP * r + c] = rt;
+            rt[c] = (T)res;
+            if (!on[c]) continue;
             q[c] += fabs(res);
-            q[6 + c] += (double)rt * (double)rt;
+            q[6 + c] += (double)rt[c] * (double)rt[c];
             if (MODE == 0) {
                 const double pA = sumA * xbar[c];
                 q[12 + c] += fabs(y[c] - pA) + fabs(bb[c] - pA);
             }
         }
+        st6(a.r + (int64_t)NCMP * r, rt);
+        st6(a.r0 + (int64_t)NCMP * r, rt);
     }
     write_partials<NQ>(q, a.partials);
 }
