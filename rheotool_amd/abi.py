@@ -143,8 +143,12 @@ def _p(a):
 
 
 def _f64(a, shape=None):
+    """C-contiguous float64 copy or view; with shape, the array must hold
+    exactly that many values (the C library reads that many)."""
     a = np.ascontiguousarray(a, dtype=np.float64)
     if shape is not None:
+        if a.size != int(np.prod(shape)):
+            raise ValueError(f"expected {int(np.prod(shape))} values {tuple(shape)}, got shape {a.shape}")
         a = a.reshape(shape)
     return a
 
@@ -212,11 +216,12 @@ class Context:
     def symm_solve(self, mesh, diag, lower, upper, source, psi, controls, internalCoeffs=None,
                    boundaryCoeffs=None):
         """sparseSolver<symmTensor>::solve: returns (psi, perf[6])."""
-        keep = [_f64(diag), _f64(lower), _f64(upper), _f64(source),
-                None if internalCoeffs is None else _f64(internalCoeffs),
-                None if boundaryCoeffs is None else _f64(boundaryCoeffs)]
+        N, F, NB = mesh.nCells, mesh.nInternal, mesh.nBoundary
+        keep = [_f64(diag, (N,)), _f64(lower, (F,)), _f64(upper, (F,)), _f64(source, (N, 6)),
+                None if internalCoeffs is None else _f64(internalCoeffs, (NB, 6)),
+                None if boundaryCoeffs is None else _f64(boundaryCoeffs, (NB, 6))]
         s = LduSymm(*[_p(k) for k in keep])
-        x = _f64(psi).copy()
+        x = _f64(psi, (N, 6)).copy()
         perf = (SolverPerf * 6)()
         rc = lib().rt_symm_solve(self.h, mesh.h, C.byref(s), _p(x), controls.encode(), perf)
         self.check(rc, "rt_symm_solve")
@@ -235,6 +240,9 @@ class Mesh:
         ctx.check(lib().rt_mesh_create(ctx.h, C.byref(d), renumber, C.byref(h)), "rt_mesh_create")
         self.h = h
         self.nCells = m.nCells
+        self.nInternal = int(m.nInternal)
+        self.nBoundary = int(m.nBoundary)
+        self.patchSize = [int(x) for x in m.patchSize]
 
     @property
     def validComponents(self):
@@ -293,11 +301,19 @@ class ConstitutiveEq:
         self.ctx.check(lib().rt_model_set_state(self.h, *[_p(k) for k in keep]), "set_state")
 
     def set_flow(self, U, Ub, phi):
-        keep = [_f64(U), _f64(Ub) if Ub is not None and np.size(Ub) else None, _f64(phi)]
+        NB = self.mesh.nBoundary
+        keep = [_f64(U, (self.N, 3)), _f64(Ub, (NB, 3)) if Ub is not None and np.size(Ub) else None,
+                _f64(phi, (self.mesh.nInternal + NB,))]
         self.ctx.check(lib().rt_model_set_flow(self.h, *[_p(k) for k in keep]), "set_flow")
 
+    _BC_WIDTH = 6  # values per face of a fixedValue patch
+
     def set_bc(self, patch, bc_type, values=None):
-        v = None if values is None else _f64(values)
+        v = None
+        if values is not None:
+            if not 0 <= patch < len(self.mesh.patchSize):
+                raise ValueError(f"patch {patch} out of range")
+            v = _f64(values, (self.mesh.patchSize[patch] * self._BC_WIDTH,))
         self.ctx.check(lib().rt_model_set_bc(self.h, patch, bc_type, _p(v)), "set_bc")
 
     def set_solver(self, controls: str):
@@ -313,7 +329,7 @@ class ConstitutiveEq:
     def div_tau(self, tauB=None):
         """fvc::div(tau/rho) of the current tau: [N, 3]."""
         out = np.empty((self.N, 3))
-        tb = None if tauB is None else _f64(tauB)
+        tb = None if tauB is None else _f64(tauB, (self.mesh.nBoundary, 6))
         self.ctx.check(lib().rt_model_div_tau(self.h, _p(tb), _p(out)), "div_tau")
         return out
 
@@ -374,6 +390,8 @@ _DIMS = {"lambdaR": "[0 0 1 0 0 0 0]", "lambdaD": "[0 0 1 0 0 0 0]", "lambdaS": 
 
 class FluidityEq(ConstitutiveEq):
     """The fluidity equation of a BMPLog model (owned by it)."""
+
+    _BC_WIDTH = 1
 
     def set_state(self, Phi, eigVals=None, eigVecs=None, tau=None):
         keep = _f64(Phi, (self.N,))
@@ -456,9 +474,11 @@ class Server:
         field arrays are referenced by the server until the ticket finishes.
         gradU: optional prescribed velocity gradient [N, 3, 3]."""
         m = mesh.src
-        arrs = [_f64(fields.theta), _f64(fields.eigVals), _f64(fields.eigVecs), _f64(fields.U),
-                _f64(fields.Ub) if m.nBoundary else None, _f64(fields.phi)]
-        g = None if gradU is None else _f64(gradU)
+        N, NB = m.nCells, m.nBoundary
+        arrs = [_f64(fields.theta, (N, 6)), _f64(fields.eigVals, (N, 3, 3)), _f64(fields.eigVecs, (N, 3, 3)),
+                _f64(fields.U, (N, 3)), _f64(fields.Ub, (NB, 3)) if NB else None,
+                _f64(fields.phi, (m.nInternal + NB,))]
+        g = None if gradU is None else _f64(gradU, (N, 3, 3))
         th_out, tau_out = np.empty((m.nCells, 6)), np.empty((m.nCells, 6))
         q = Request(mesh.h, dict_text.encode(), None if controls is None else controls.encode(),
                     *[_p(a) for a in arrs], deltaT, _p(th_out), _p(tau_out), _p(g))

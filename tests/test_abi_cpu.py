@@ -34,3 +34,16 @@ def test_no_device_reports_cleanly():
     with pytest.raises(abi.RheoError) as e:
         abi.Context(0)
     assert e.value.code == abi.RT_E_NO_DEVICE
+
+
+def test_arrays_of_the_wrong_size_are_refused_before_the_library_reads_them():
+    """The C entry points read a fixed number of values from each pointer
+    (the sizes in include/rheo_gpu.h); the Python side checks them first."""
+    import numpy as np
+
+    assert abi._f64(np.arange(6.0), (2, 3)).shape == (2, 3)
+    assert abi._f64([[1, 2, 3]], (3,)).dtype == np.float64
+    with pytest.raises(ValueError):
+        abi._f64(np.zeros(5), (2, 3))
+    with pytest.raises(ValueError):
+        abi._f64(np.zeros((4, 6)), (3, 6))

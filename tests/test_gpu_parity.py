@@ -585,6 +585,29 @@ def test_saramito_dictionary_errors(ctx):
     assert "keyword k" in e.value.msg
 
 
+
+def test_wrong_sized_arrays_are_refused(ctx):
+    """Every setter checks the array sizes the C entry point will read
+    (include/rheo_gpu.h) before it is called; the model stays usable."""
+    m, f = cases.channel2d(8)
+    dm = ctx.mesh(m)
+    eq = ctx.constitutiveEq("", dm, abi.model_dict("Oldroyd-BLog", lam=1.0))
+    with pytest.raises(ValueError):
+        eq.set_state(f.theta[:-1], f.eigVals, f.eigVecs)
+    with pytest.raises(ValueError):
+        eq.set_flow(f.U, f.Ub, f.phi[:-1])
+    with pytest.raises(ValueError):
+        eq.set_flow(f.U[:-1], f.Ub, f.phi)
+    with pytest.raises(ValueError):
+        eq.set_bc(0, abi.RT_BC_FIXEDVALUE, np.zeros(int(m.patchSize[0]) * 6 - 1))
+    with pytest.raises(ValueError):
+        eq.set_bc(len(m.patchSize), abi.RT_BC_FIXEDVALUE, np.zeros(6))
+    with pytest.raises(ValueError):
+        eq.correct(1e-3, gradU=np.zeros((m.nCells - 1, 3, 3)))
+    eq.set_state(f.theta, f.eigVals, f.eigVecs)
+    eq.set_flow(f.U, f.Ub, f.phi)
+    assert eq.correct(1e-3) == abi.RT_OK
+
 BMP_DEV = dict(G0=0.05, k=1.0, Phi0=2.0, PhiInf=200.0)
 BMP_ORC = dict(G0=0.05, k_bmp=1.0, Phi0=2.0, PhiInf=200.0)
 
