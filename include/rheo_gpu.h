@@ -355,6 +355,16 @@ int rt_server_poll(rt_server* server, int64_t ticket, double* theta, double* tau
  * of every pending ticket. */
 int rt_server_finished(rt_server* server, int64_t* tickets, int max);
 
+/* Page-locked host memory (hipHostMalloc on the context's device) for a
+ * serving caller's request and result arrays.  Transfers from and to it run
+ * asynchronously on the device's DMA engines, so the server's uploads at
+ * admission and its result downloads overlap the kernels of other requests;
+ * from pageable arrays each transfer holds the calling thread until it is
+ * done.  No counterpart in the reference (an OpenFOAM rank's fields live in
+ * pageable memory and never cross a bus).  rt_host_free(NULL) does nothing. */
+int rt_host_alloc(rt_ctx* ctx, size_t bytes, void** out);
+void rt_host_free(void* p);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,6 +16,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -125,6 +126,8 @@ def lib():
             "rt_server_progress": (i32, [vp]),
             "rt_server_poll": (i32, [vp, C.c_int64, vp, vp, vp, vp]),
             "rt_server_finished": (i32, [vp, vp, i32]),
+            "rt_host_alloc": (i32, [vp, C.c_size_t, vp]),
+            "rt_host_free": (None, [vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -206,6 +209,19 @@ class Context:
         ms = C.c_double()
         self.check(lib().rt_ctx_copy_bench(self.h, nbytes, reps, C.byref(ms)), "copy_bench")
         return ms.value
+
+    def pinned_empty(self, shape, dtype=np.float64):
+        """Uninitialised numpy array in page-locked host memory
+        (rt_host_alloc): copies from and to it run asynchronously, so a
+        server's uploads and downloads overlap the device's work.  The memory
+        is freed when the last array using it is collected."""
+        n = max(1, int(np.prod(shape)) * np.dtype(dtype).itemsize)
+        p = C.c_void_p()
+        self.check(lib().rt_host_alloc(self.h, n, C.byref(p)), "rt_host_alloc")
+        buf = (C.c_char * n).from_address(p.value)
+        fin = weakref.finalize(buf, lib().rt_host_free, C.c_void_p(p.value))
+        fin.atexit = False  # the process's end returns it anyway
+        return np.frombuffer(buf, dtype=dtype, count=int(np.prod(shape))).reshape(shape)
 
     def mesh(self, m, renumber=RT_RENUMBER_AUTO) -> "Mesh":
         return Mesh(self, m, renumber)
@@ -469,17 +485,27 @@ class Server:
         self.ctx, self.h = ctx, h
         self.held = {}  # ticket -> (input arrays, theta out, tau out): the server reads/writes them
 
-    def submit(self, mesh: "Mesh", fields, dict_text, deltaT, controls=None, gradU=None) -> int:
+    def submit(self, mesh: "Mesh", fields, dict_text, deltaT, controls=None, gradU=None, out=None) -> int:
         """mesh: an uploaded Mesh (shared by any number of requests).  The
         field arrays are referenced by the server until the ticket finishes.
-        gradU: optional prescribed velocity gradient [N, 3, 3]."""
+        gradU: optional prescribed velocity gradient [N, 3, 3].  out: optional
+        (theta, tau) result arrays, float64 [N, 6] C-contiguous, written in
+        place (page-locked ones, Context.pinned_empty, let the download
+        overlap other work)."""
         m = mesh.src
         N, NB = m.nCells, m.nBoundary
         arrs = [_f64(fields.theta, (N, 6)), _f64(fields.eigVals, (N, 3, 3)), _f64(fields.eigVecs, (N, 3, 3)),
                 _f64(fields.U, (N, 3)), _f64(fields.Ub, (NB, 3)) if NB else None,
                 _f64(fields.phi, (m.nInternal + NB,))]
         g = None if gradU is None else _f64(gradU, (N, 3, 3))
-        th_out, tau_out = np.empty((m.nCells, 6)), np.empty((m.nCells, 6))
+        if out is None:
+            th_out, tau_out = np.empty((N, 6)), np.empty((N, 6))
+        else:
+            th_out, tau_out = out
+            for a in (th_out, tau_out):
+                if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.shape == (N, 6)
+                        and a.flags.c_contiguous and a.flags.writeable):
+                    raise ValueError(f"out arrays must be writable C-contiguous float64 ({N}, 6)")
         q = Request(mesh.h, dict_text.encode(), None if controls is None else controls.encode(),
                     *[_p(a) for a in arrs], deltaT, _p(th_out), _p(tau_out), _p(g))
         arrs.append(g)

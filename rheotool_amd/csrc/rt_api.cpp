@@ -645,6 +645,25 @@ int rt_ctx_kernel_stats(rt_ctx* ctx, rt_kernel_stat* out, int cap) {
     return rc == RT_OK ? n : -rc;
 }
 
+int rt_host_alloc(rt_ctx* ctx, size_t bytes, void** out) {
+    if (!ctx || !out || bytes == 0) return RT_E_BAD_ARG;
+    *out = nullptr;
+    return guarded(&ctx->impl, [&] {
+        RT_HIP(hipSetDevice(ctx->impl.device));
+        void* p = nullptr;
+        if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess || !p) {
+            (void)hipGetLastError();
+            throw Error{RT_E_OOM, "rt_host_alloc: page-locked allocation of " + std::to_string(bytes) +
+                                      " bytes failed"};
+        }
+        *out = p;
+    });
+}
+
+void rt_host_free(void* p) {
+    if (p) (void)hipHostFree(p);
+}
+
 int rt_ctx_copy_bench(rt_ctx* ctx, size_t bytes, int reps, double* ms) {
     if (!ctx || !ms || bytes < 16 || reps <= 0) return RT_E_BAD_ARG;
     return guarded(&ctx->impl, [&] {

@@ -4,7 +4,9 @@
 // stress step) are admitted while a slot and the HBM budget allow, then
 // advanced together: every progress() call moves each in-flight request one
 // round further (assembly, Krylov rounds, eigen/tau, download) without
-// blocking on any of them.  Each slot owns a context with its own stream, so
+// blocking on any of them (results go home with asynchronous copies, which
+// overlap other requests' kernels when the output arrays are page-locked,
+// rt_host_alloc).  Each slot owns a context with its own stream, so
 // the rounds of different requests overlap on the device, and requests that
 // arrive while others run are admitted as soon as a slot and memory free up.
 // A request's kernels depend only on its own data (fixed grids, per-request
@@ -34,6 +36,7 @@ struct Request {
     rt_model* model = nullptr;
     enum { QUEUED, RUNNING, FINISHING, DONE, FAILED } state = QUEUED;
     hipEvent_t done = nullptr;
+    DBuf dlTheta, dlTau;  // theta and tau in the caller's row order, on their way to the host
     std::chrono::steady_clock::time_point tSubmit, tDone;
     std::vector<double> ownTheta, ownTau;  // results when the caller gave no output arrays
     rt_solver_perf perf[6];
@@ -65,6 +68,8 @@ void releaseRequest(rt_server* S, Request* r) {
     r->model = nullptr;
     if (r->done) (void)hipEventDestroy(r->done);
     r->done = nullptr;
+    r->dlTheta.release();
+    r->dlTau.release();
     if (r->slot >= 0) {
         S->slotReq[r->slot] = nullptr;
         S->inflightBytes -= r->estBytes;
@@ -80,6 +85,9 @@ void fail(rt_server* S, Request* r, int rc, const std::string& msg) {
     r->err = msg;
     r->state = Request::FAILED;
     r->tDone = std::chrono::steady_clock::now();
+    // work already queued for the request (kernels, copies into the caller's
+    // arrays) ends before its buffers and the caller's arrays are let go
+    if (r->slot >= 0) (void)hipStreamSynchronize(S->slots[r->slot]->impl.stream);
     releaseRequest(S, r);
 }
 
@@ -116,12 +124,41 @@ void admit(rt_server* S, Request* r, int slot) {
     }
 }
 
+// Enqueues the results' way home on the slot's stream: theta and tau back
+// to the caller's row order (what rt_model_get does), then device-to-host
+// copies into the output arrays.  Nothing here waits: from page-locked
+// output arrays the copies run while other requests compute.
+void beginDownload(rt_ctx* c, Request* r) {
+    Model& M = r->model->impl;
+    const Mesh& m = *M.mesh;
+    Ctx* x = &c->impl;
+    const size_t N = (size_t)m.N, bytes = 6 * N * sizeof(double);
+    double* th = r->thetaOut;
+    double* ta = r->tauOut;
+    if (!th) {
+        r->ownTheta.resize(6 * N);
+        th = r->ownTheta.data();
+    }
+    if (!ta) {
+        r->ownTau.resize(6 * N);
+        ta = r->ownTau.data();
+    }
+    r->dlTheta = x->alloc(bytes);
+    r->dlTau = x->alloc(bytes);
+    const int32_t* perm = m.permD.as<int32_t>();
+    launch_unpermute_rows(x, m.N, 6, 6, M.theta.as<double>(), perm, r->dlTheta.as<double>());
+    launch_unpermute_rows(x, m.N, 6, 6, M.tau.as<double>(), perm, r->dlTau.as<double>());
+    RT_HIP(hipMemcpyAsync(th, r->dlTheta.p, bytes, hipMemcpyDeviceToHost, x->stream));
+    RT_HIP(hipMemcpyAsync(ta, r->dlTau.p, bytes, hipMemcpyDeviceToHost, x->stream));
+}
+
 void advance(rt_server* S, Request* r) {
     rt_ctx* c = S->slots[r->slot];
     try {
         RT_HIP(hipSetDevice(c->impl.device));
         if (r->state == Request::RUNNING) {
             if (modelCorrectAdvance(r->model->impl, false)) {
+                beginDownload(c, r);
                 RT_HIP(hipEventRecord(r->done, c->impl.stream));
                 r->state = Request::FINISHING;
             }
@@ -130,23 +167,6 @@ void advance(rt_server* S, Request* r) {
             hipError_t e = hipEventQuery(r->done);
             if (e == hipErrorNotReady) return;
             RT_HIP(e);
-            const size_t N = (size_t)r->nCells;
-            double* th = r->thetaOut;
-            double* ta = r->tauOut;
-            if (!th) {
-                r->ownTheta.resize(6 * N);
-                th = r->ownTheta.data();
-            }
-            if (!ta) {
-                r->ownTau.resize(6 * N);
-                ta = r->ownTau.data();
-            }
-            int rc = rt_model_get(r->model, RT_THETA, th);
-            if (rc == RT_OK) rc = rt_model_get(r->model, RT_TAU, ta);
-            if (rc != RT_OK) {
-                fail(S, r, rc, rt_last_error(c));
-                return;
-            }
             rt_model_solver_perf(r->model, r->perf);
             r->rc = RT_OK;
             for (int q = 0; q < 6; ++q)

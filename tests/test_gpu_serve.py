@@ -70,6 +70,50 @@ def test_same_result_alone_and_in_mix(precision):
         srv.close()
 
 
+
+def test_page_locked_requests_give_the_same_bits():
+    """Requests whose inputs and result arrays are page-locked
+    (Context.pinned_empty, rt_host_alloc): the uploads and the asynchronous
+    downloads land the same bits as the pageable path, written in place into
+    the caller's arrays."""
+    import dataclasses
+    controls = abi.solver_dict(1e-10, precision="mixed")
+    reqs = [make_request(i) for i in range(6)]
+    with abi.Context(0) as ctx:
+        alone = [run_alone(ctx, r, controls) for r in reqs]
+
+        def pin(a):
+            p = ctx.pinned_empty(a.shape)
+            p[...] = a
+            return p
+
+        srv = abi.Server(ctx, hbm_budget=0, max_inflight=3)
+        meshes = [ctx.mesh(r[0]) for r in reqs]
+        outs, tickets = [], []
+        for i, r in enumerate(reqs):
+            m, f, model, lam, kw, dt = r
+            pf = dataclasses.replace(f, **{k: pin(getattr(f, k)) for k in
+                                           ("theta", "eigVals", "eigVecs", "U", "Ub", "phi")})
+            o = (ctx.pinned_empty((m.nCells, 6)), ctx.pinned_empty((m.nCells, 6)))
+            outs.append(o)
+            tickets.append(srv.submit(meshes[i], pf, abi.model_dict(model, lam=lam, **kw), dt, controls,
+                                      out=o))
+            srv.progress()
+        with pytest.raises(ValueError):  # result arrays of the wrong shape are refused
+            srv.submit(meshes[0], reqs[0][1], abi.model_dict("Oldroyd-BLog"), 1e-3, controls,
+                       out=(np.empty((3, 6)), np.empty((3, 6))))
+        spins = 0
+        while srv.progress() > 0:
+            spins += 1
+            assert spins < 2_000_000
+        for i in range(len(reqs)):
+            out = srv.poll(tickets[i])
+            assert out["theta"] is outs[i][0] and out["tau"] is outs[i][1]
+            assert np.array_equal(outs[i][0], alone[i][0]), i
+            assert np.array_equal(outs[i][1], alone[i][1]), i
+        srv.close()
+        del outs
+
 def test_server_results_match_oracle():
     with abi.Context(0) as ctx:
         srv = abi.Server(ctx, max_inflight=3)
