@@ -351,6 +351,68 @@ struct StagingRef {
 };
 static StagingRef staging(Ctx* c) { return StagingRef{c->stage}; }
 
+// Uploads of a model's state and flow.  Each host array goes through the
+// context's staging buffer and a reorder kernel; consecutive uploads share
+// the buffer safely because the copies and kernels run in stream order.
+// wait = true (the public setters): return once the caller's arrays have
+// been read, since the caller may change them at once.  wait = false (the
+// server): return at once; the server's requests keep their arrays
+// unchanged until the ticket finishes, and from page-locked arrays the
+// copies then overlap other work.
+void modelUploadState(Model& M, const double* theta, const double* eigVals, const double* eigVecs,
+                      const double* tau, bool wait) {
+    if (!theta) throw Error{RT_E_BAD_ARG, "rt_model_set_state: theta is required (MUST_READ)"};
+    Ctx* c = M.ctx;
+    Mesh& m = *M.mesh;
+    const int32_t N = m.N;
+    RT_HIP(hipSetDevice(c->device));
+    DBuf& st = staging(c).buf;
+    const int32_t* perm = m.permD.as<int32_t>();
+    std::vector<double> phi6;
+    if (M.scalar()) {  // the fluidity: Phi [N] into component xx
+        phi6.assign((size_t)6 * N, 0.0);
+        for (int32_t i = 0; i < N; ++i) phi6[6 * (size_t)i] = theta[i];
+        theta = phi6.data();
+        eigVals = eigVecs = tau = nullptr;
+        wait = true;  // phi6 is local
+    }
+    uploadAos(c, st, theta, (int64_t)6 * N);
+    launch_permute_rows(c, N, 6, 6, st.as<double>(), perm, M.theta.as<double>());
+    if (eigVals) {
+        uploadAos(c, st, eigVals, (int64_t)9 * N);
+        launch_eigvals_from_tensor(c, N, st.as<double>(), perm, M.eigVal.as<double>());
+    }
+    if (eigVecs) {
+        uploadAos(c, st, eigVecs, (int64_t)9 * N);
+        launch_aos_to_soa(c, N, 9, st.as<double>(), perm, M.eigVec.as<double>());
+    }
+    if (tau) {
+        uploadAos(c, st, tau, (int64_t)6 * N);
+        launch_permute_rows(c, N, 6, 6, st.as<double>(), perm, M.tau.as<double>());
+    }
+    if (wait) RT_HIP(hipStreamSynchronize(c->stream));
+    M.hasState = true;
+}
+
+void modelUploadFlow(Model& M, const double* U, const double* Ub, const double* phi, bool wait) {
+    Mesh& m = *M.mesh;
+    if (!U || !phi || (m.NB > 0 && !Ub))
+        throw Error{RT_E_BAD_ARG, "rt_model_set_flow: U, Ub and phi are required"};
+    Ctx* c = M.ctx;
+    RT_HIP(hipSetDevice(c->device));
+    DBuf& st = staging(c).buf;
+    uploadAos(c, st, U, (int64_t)3 * m.N);
+    launch_permute_rows(c, m.N, 3, 4, st.as<double>(), m.permD.as<int32_t>(), M.U.as<double>());
+    if (m.NB > 0) {  // [NB][3] -> [3][NB]
+        uploadAos(c, st, Ub, (int64_t)3 * m.NB);
+        launch_aos_to_soa(c, m.NB, 3, st.as<double>(), nullptr, M.Ub.as<double>());
+    }
+    RT_HIP(hipMemcpyAsync(M.phi.p, phi, ((size_t)m.F + m.NB) * 8, hipMemcpyHostToDevice, c->stream));
+    launch_face_to_entry(c, m, M.phi.as<double>(), M.ePhi.as<double>());
+    if (wait) RT_HIP(hipStreamSynchronize(c->stream));
+    M.hasFlow = true;
+}
+
 int modelInvMask(Model& M) {
     if (!M.prm.ofInvQuirk) return 0;
     const Mesh& m = *M.mesh;
@@ -900,44 +962,8 @@ int rt_model_set_state(rt_model* model, const double* theta, const double* eigVa
                            "(rt_model_mode)";
         return RT_E_BAD_ARG;
     }
-    return guarded(M.ctx, [&] {
-        if (!theta) throw Error{RT_E_BAD_ARG, "rt_model_set_state: theta is required (MUST_READ)"};
-        Ctx* c = M.ctx;
-        Mesh& m = *M.mesh;
-        const int32_t N = m.N;
-        RT_HIP(hipSetDevice(c->device));
-        DBuf& st = staging(c).buf;
-        const int32_t* perm = m.permD.as<int32_t>();
-        std::vector<double> phi6;
-        if (M.scalar()) {  // the fluidity: Phi [N] into component xx
-            phi6.assign((size_t)6 * N, 0.0);
-            for (int32_t i = 0; i < N; ++i) phi6[6 * (size_t)i] = theta[i];
-            theta = phi6.data();
-            eigVals = eigVecs = tau = nullptr;
-        }
-        uploadAos(c, st, theta, (int64_t)6 * N);
-        launch_permute_rows(c, N, 6, 6, st.as<double>(), perm, M.theta.as<double>());
-        if (!phi6.empty()) RT_HIP(hipStreamSynchronize(c->stream));  // phi6 is local
-        if (eigVals) {
-            RT_HIP(hipStreamSynchronize(c->stream));
-            uploadAos(c, st, eigVals, (int64_t)9 * N);
-            launch_eigvals_from_tensor(c, N, st.as<double>(), perm, M.eigVal.as<double>());
-        }
-        if (eigVecs) {
-            RT_HIP(hipStreamSynchronize(c->stream));
-            uploadAos(c, st, eigVecs, (int64_t)9 * N);
-            launch_aos_to_soa(c, N, 9, st.as<double>(), perm, M.eigVec.as<double>());
-        }
-        if (tau) {
-            RT_HIP(hipStreamSynchronize(c->stream));
-            uploadAos(c, st, tau, (int64_t)6 * N);
-            launch_permute_rows(c, N, 6, 6, st.as<double>(), perm, M.tau.as<double>());
-        }
-        RT_HIP(hipStreamSynchronize(c->stream));
-        M.hasState = true;
-    });
+    return guarded(M.ctx, [&] { modelUploadState(M, theta, eigVals, eigVecs, tau, true); });
 }
-
 int rt_model_set_bc(rt_model* model, int patch, int bcType, const double* values) {
     if (!model) return RT_E_BAD_ARG;
     RT_FORWARD_TO_MODES(rt_model_set_bc(model, patch, bcType, values));
@@ -971,28 +997,10 @@ int rt_model_set_flow(rt_model* model, const double* U, const double* Ub, const 
     RT_FORWARD_TO_MODES(rt_model_set_flow(model, U, Ub, phi));
     Model& M = model->impl;
     return guarded(M.ctx, [&] {
-        Mesh& m = *M.mesh;
-        if (!U || !phi || (m.NB > 0 && !Ub))
-            throw Error{RT_E_BAD_ARG, "rt_model_set_flow: U, Ub and phi are required"};
-        Ctx* c = M.ctx;
-        RT_HIP(hipSetDevice(c->device));
-        DBuf& st = staging(c).buf;
-        uploadAos(c, st, U, (int64_t)3 * m.N);
-        launch_permute_rows(c, m.N, 3, 4, st.as<double>(), m.permD.as<int32_t>(), M.U.as<double>());
-        if (m.NB > 0) {
-            std::vector<double> ub((size_t)3 * m.NB);
-            for (int32_t b = 0; b < m.NB; ++b)
-                for (int q = 0; q < 3; ++q) ub[(size_t)q * m.NB + b] = Ub[3 * (size_t)b + q];
-            RT_HIP(hipMemcpyAsync(M.Ub.p, ub.data(), ub.size() * 8, hipMemcpyHostToDevice, c->stream));
-            RT_HIP(hipStreamSynchronize(c->stream));
-        }
-        RT_HIP(hipMemcpyAsync(M.phi.p, phi, ((size_t)m.F + m.NB) * 8, hipMemcpyHostToDevice, c->stream));
-        launch_face_to_entry(c, m, M.phi.as<double>(), M.ePhi.as<double>());
-        RT_HIP(hipStreamSynchronize(c->stream));
-        M.hasFlow = true;
+        modelUploadFlow(M, U, Ub, phi, true);
         if (M.fluid) {
             const int rc = rt_model_set_flow(M.fluid, U, Ub, phi);
-            if (rc != RT_OK) throw Error{rc, "fluidity: " + c->lastError};
+            if (rc != RT_OK) throw Error{rc, "fluidity: " + M.ctx->lastError};
         }
     });
 }

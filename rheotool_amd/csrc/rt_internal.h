@@ -407,6 +407,11 @@ KryCtl kryCtl(const Controls& C, bool mixed);
 struct Dict;
 // a model (or a multiMode and its modes) from its parameters dictionary
 void createModel(Model& M, Ctx* c, Mesh* mesh, const std::string& name, const Dict& d, const std::string& dn);
+// state and flow uploads (rt_model_set_state / rt_model_set_flow); wait =
+// false returns before the copies have read the host arrays
+void modelUploadState(Model& M, const double* theta, const double* eigVals, const double* eigVecs,
+                      const double* tau, bool wait);
+void modelUploadFlow(Model& M, const double* U, const double* Ub, const double* phi, bool wait);
 void destroyModes(Model& M);
 void allocModelState(Model& M);
 void launch_mean6(Ctx* ctx, int32_t N, const double* x, double* partials,

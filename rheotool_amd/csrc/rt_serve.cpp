@@ -108,14 +108,16 @@ void admit(rt_server* S, Request* r, int slot) {
         rc = RT_E_UNSUPPORTED;
     }
     if (rc == RT_OK) rc = rt_model_set_solver(r->model, r->controls.c_str());
-    if (rc == RT_OK) rc = rt_model_set_state(r->model, r->theta, r->eigVals, r->eigVecs, nullptr);
-    if (rc == RT_OK) rc = rt_model_set_flow(r->model, r->U, r->Ub, r->phi);
     if (rc != RT_OK) {
         fail(S, r, rc, rt_last_error(c));
         return;
     }
     try {
         RT_HIP(hipSetDevice(c->impl.device));
+        // the request's arrays stay unchanged until its ticket finishes, so
+        // the uploads are queued without waiting for them
+        modelUploadState(r->model->impl, r->theta, r->eigVals, r->eigVecs, nullptr, false);
+        modelUploadFlow(r->model->impl, r->U, r->Ub, r->phi, false);
         modelCorrectBegin(r->model->impl, r->deltaT, r->gradU);
         RT_HIP(hipEventCreateWithFlags(&r->done, hipEventDisableTiming));
         r->state = Request::RUNNING;
