@@ -261,7 +261,7 @@ def serve_bench(args):
             key, model, lam = specs[i]
             m, f, dm, dt = meshes[key]
             tickets[i] = srv.submit(dm, f, abi.model_dict(model, lam=lam, **kwmap.get(model, {})), dt,
-                                    controls, out=outs[i])
+                                    controls, out=outs[i], eigen=args.upload_eigen)
             index_of[tickets[i]] = i
             i += 1
         live = srv.progress()
@@ -281,7 +281,8 @@ def serve_bench(args):
                 data="synthetic",
                 config=dict(workload=SERVE_DESC.format(n=nreq, rate=args.rate, mx=args.max_cells),
                             requests=nreq, total_cells=int(cells), distinct_meshes=len(meshes),
-                            host_buffers="pageable" if args.pageable else "page-locked"),
+                            host_buffers="pageable" if args.pageable else "page-locked",
+                            eigen_state="uploaded" if args.upload_eigen else "computed from theta on the device"),
                 p50_ms=round(float(np.percentile(lat, 50)), 3),
                 p99_ms=round(float(np.percentile(lat, 99)), 3),
                 max_ms=round(float(lat.max()), 3), wall_s=round(elapsed, 3))
@@ -300,6 +301,8 @@ def main():
     ap.add_argument("--max-cells", type=float, default=2e6)
     ap.add_argument("--pageable", action="store_true",
                     help="serve: requests in pageable host arrays instead of page-locked ones")
+    ap.add_argument("--upload-eigen", action="store_true",
+                    help="serve: send each request's eigen state instead of computing it from theta")
     ap.add_argument("--n", type=int, default=0, help="override the mesh size")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--renumber", default="auto", choices=["auto", "none", "rcm", "sfc"],

@@ -319,8 +319,8 @@ typedef struct {
     const char* dict;     /* constitutiveProperties parameters text */
     const char* controls; /* fvSolution text, NULL for the server default */
     const double* theta;  /* [6N] */
-    const double* eigVals;
-    const double* eigVecs;
+    const double* eigVals; /* [9N] each; both NULL: computed from theta on the device */
+    const double* eigVecs; /* (calcEig, constitutiveEq.C:360-428, as the previous step left them) */
     const double* U;      /* [3N] */
     const double* Ub;     /* [3*nBoundaryFaces] */
     const double* phi;    /* [nInternalFaces + nBoundaryFaces] */

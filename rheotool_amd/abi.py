@@ -485,16 +485,19 @@ class Server:
         self.ctx, self.h = ctx, h
         self.held = {}  # ticket -> (input arrays, theta out, tau out): the server reads/writes them
 
-    def submit(self, mesh: "Mesh", fields, dict_text, deltaT, controls=None, gradU=None, out=None) -> int:
+    def submit(self, mesh: "Mesh", fields, dict_text, deltaT, controls=None, gradU=None, out=None,
+               eigen=True) -> int:
         """mesh: an uploaded Mesh (shared by any number of requests).  The
         field arrays are referenced by the server until the ticket finishes.
         gradU: optional prescribed velocity gradient [N, 3, 3].  out: optional
         (theta, tau) result arrays, float64 [N, 6] C-contiguous, written in
         place (page-locked ones, Context.pinned_empty, let the download
-        overlap other work)."""
+        overlap other work).  eigen=False sends no eigen state: the server
+        computes it from theta on the device (calcEig)."""
         m = mesh.src
         N, NB = m.nCells, m.nBoundary
-        arrs = [_f64(fields.theta, (N, 6)), _f64(fields.eigVals, (N, 3, 3)), _f64(fields.eigVecs, (N, 3, 3)),
+        arrs = [_f64(fields.theta, (N, 6)), _f64(fields.eigVals, (N, 3, 3)) if eigen else None,
+                _f64(fields.eigVecs, (N, 3, 3)) if eigen else None,
                 _f64(fields.U, (N, 3)), _f64(fields.Ub, (NB, 3)) if NB else None,
                 _f64(fields.phi, (m.nInternal + NB,))]
         g = None if gradU is None else _f64(gradU, (N, 3, 3))

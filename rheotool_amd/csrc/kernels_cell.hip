@@ -652,6 +652,7 @@ __global__ void __launch_bounds__(BLK) k_eig_tau(int32_t N, ModelParams p, const
     }
 #pragma unroll
     for (int q = 0; q < 9; ++q) eigVec[q * (int64_t)N + r] = V[q];
+    if (!tau) return;  // calcEig only (the eigen state of an uploaded theta)
     double C[9], D[9];
     vdvt(V, lam, C);
     double coef;
@@ -1192,7 +1193,9 @@ void launch_assemble(Ctx* ctx, const AssembleArgs& a, int grid) {
 void launch_eig_tau(Ctx* ctx, int32_t N, const ModelParams& prm, const double* theta,
                     double* eigVal, double* eigVec, double* tau, const double* fOld) {
     // theta 48 + previous eigenvectors 72 in; eigenvalues 24, eigenvectors 72, tau 48 out
-    ProfScope ps(ctx, "k_eig_tau", (double)N * (48 + 72 + 24 + 72 + 48 + (modelKeepsFactor(prm.kind) ? 8 : 0)));
+    // (tau NULL: the eigen state only)
+    const double tauBytes = tau ? 48 + (modelKeepsFactor(prm.kind) ? 8 : 0) : 0;
+    ProfScope ps(ctx, "k_eig_tau", (double)N * (48 + 72 + 24 + 72 + tauBytes));
     hipLaunchKernelGGL(k_eig_tau, dim3(nblk(N)), dim3(BLK), 0, ctx->stream, N, prm, theta, eigVal,
                        eigVec, tau, fOld);
     RT_HIP(hipGetLastError());

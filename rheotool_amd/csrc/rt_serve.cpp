@@ -116,8 +116,12 @@ void admit(rt_server* S, Request* r, int slot) {
         RT_HIP(hipSetDevice(c->impl.device));
         // the request's arrays stay unchanged until its ticket finishes, so
         // the uploads are queued without waiting for them
-        modelUploadState(r->model->impl, r->theta, r->eigVals, r->eigVecs, nullptr, false);
-        modelUploadFlow(r->model->impl, r->U, r->Ub, r->phi, false);
+        Model& M = r->model->impl;
+        modelUploadState(M, r->theta, r->eigVals, r->eigVecs, nullptr, false);
+        if (!r->eigVals && !r->eigVecs)  // the eigen state of theta, computed here instead of uploaded
+            launch_eig_tau(M.ctx, M.mesh->N, M.prm, M.theta.as<double>(), M.eigVal.as<double>(),
+                           M.eigVec.as<double>(), nullptr, nullptr);
+        modelUploadFlow(M, r->U, r->Ub, r->phi, false);
         modelCorrectBegin(r->model->impl, r->deltaT, r->gradU);
         RT_HIP(hipEventCreateWithFlags(&r->done, hipEventDisableTiming));
         r->state = Request::RUNNING;

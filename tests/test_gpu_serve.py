@@ -114,6 +114,40 @@ def test_page_locked_requests_give_the_same_bits():
         srv.close()
         del outs
 
+
+def test_requests_without_eigen_state():
+    """A request may leave out eigVals/eigVecs: the server computes them from
+    theta on the device (calcEig) before the step.  The results match the
+    FP64 oracle fed the host's eigen state of the same theta, and are the
+    same bits alone and in a mix."""
+    controls = abi.solver_dict(1e-10, precision="mixed")
+    reqs = [make_request(i) for i in range(5)]
+    with abi.Context(0) as ctx:
+        meshes = [ctx.mesh(r[0]) for r in reqs]
+
+        def serve(idx, inflight):
+            srv = abi.Server(ctx, max_inflight=inflight)
+            t = {i: srv.submit(meshes[i], reqs[i][1], abi.model_dict(reqs[i][2], lam=reqs[i][3], **reqs[i][4]),
+                               reqs[i][5], controls, eigen=False) for i in idx}
+            while srv.progress() > 0:
+                pass
+            out = {i: srv.poll(k) for i, k in t.items()}
+            srv.close()
+            return out
+
+        mix = serve(range(len(reqs)), 3)
+        for i, r in enumerate(reqs):
+            alone = serve([i], 1)[i]
+            assert np.array_equal(alone["theta"], mix[i]["theta"]), i
+            assert np.array_equal(alone["tau"], mix[i]["tau"]), i
+            m, f, model, lam, kw, dt = r
+            p = O.params(model, etaS=0.59, etaP=0.41, lam=lam, alpha=kw.get("alpha", 0.0),
+                         epsilon=kw.get("epsilon", 0.0), zeta=kw.get("zeta", 0.0))
+            th, _, _, tau, _ = O.correct(m, p, f, dt)
+            v = m.validComponents > 0
+            assert np.linalg.norm(mix[i]["theta"][:, v] - th[:, v]) / np.linalg.norm(th[:, v]) < 1e-9, i
+            assert np.linalg.norm(mix[i]["tau"] - tau) / np.linalg.norm(tau) < 1e-9, i
+
 def test_server_results_match_oracle():
     with abi.Context(0) as ctx:
         srv = abi.Server(ctx, max_inflight=3)
