@@ -62,8 +62,13 @@ DBuf Ctx::alloc(size_t bytes) {
         throw Error{RT_E_OOM, "HBM budget exceeded: " + std::to_string(inUse + bytes) + " > " +
                                   std::to_string(budget) + " bytes"};
     DBuf b;
-    auto it = cache.find(bytes);
-    if (it != cache.end()) {
+    // best fit: the smallest cached block that holds the request, if it
+    // wastes at most half the request (the serving loop sees many sizes, and
+    // a miss costs a hipMalloc)
+    auto it = cache.lower_bound(bytes);
+    if (it != cache.end() && it->first <= bytes + bytes / 2 &&
+        !(budget && inUse + it->first > budget)) {
+        bytes = it->first;
         b.p = it->second;
         cachedBytes -= bytes;
         cache.erase(it);
