@@ -17,6 +17,9 @@
   G (exp(theta) - I) turns an absolute theta difference into the same
   relative tau difference, and theta reaches ln(1e6) here, so tau's bar is
   5e-9 (1 + max |theta|).
+- The same graph as a serving request beside a 256^3 request, with the
+  device memory filled to about 80 %: the same bars, and the 256^3 result
+  bit-equal to the direct step.
 """
 import numpy as np
 import pytest
@@ -113,3 +116,54 @@ def test_random_graph_4m_mixed_matches_fp64(ctx):
         assert rel(b[k], a[k]) < bars[k], (k, rel(b[k], a[k]), bars[k])
     lam = np.diagonal(b["ev"], axis1=1, axis2=2)
     assert np.all(lam > 0)
+
+
+def test_random_graph_4m_in_the_server_beside_256_at_80_percent_hbm(ctx, box256):
+    """The 4 M-cell graph as a serving request, next to a 256^3 precision-form
+    request, with the device's memory filled to about 80 % by other
+    allocations: no failure for want of memory, the graph's fields within
+    the bars of the direct mixed-vs-FP64 test above, and the 256^3 result the
+    same bits as the same step run directly."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")  # the runtime the library already loaded
+
+    def mem_info():
+        free, total = ctypes.c_size_t(), ctypes.c_size_t()
+        assert hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) == 0
+        return free.value, total.value
+
+    print("[case] 4M graph + 256^3 in the server at 80% HBM", flush=True)
+    g = cases.random_graph(4_000_000, kNear=5, sigma=0.5, seed=3)
+    fg = cases.make_fields(g, cases.FIELDS_TG, seed=11, lnLo=-9.0, lnHi=float(np.log(1e6)))
+    mb, fb = box256
+    dg, db = ctx.mesh(g), ctx.mesh(mb)
+    kwg, kwb = dict(alpha=0.01), dict(L2=100.0)
+    ref_g = step(ctx, dg, fg, "GiesekusLog", 50.0, 1e-3, kwg, "double", gradU=fg.gradU)
+    ref_b = step(ctx, db, fb, "FENE-PLog", 5.0, 5e-4, kwb, "mixed")
+    free, total = mem_info()
+    # both requests need about 20 GB; the rest of 80 % is taken by ballast
+    ballast_bytes = int(0.8 * total) - (total - free) - (20 << 30)
+    assert ballast_bytes > 0
+    ballast = ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(ballast), ctypes.c_size_t(ballast_bytes)) == 0
+    try:
+        srv = abi.Server(ctx, hbm_budget=0, max_inflight=2)
+        controls = abi.solver_dict(1e-10, precision="mixed")
+        tg = srv.submit(dg, fg, abi.model_dict("GiesekusLog", lam=50.0, **kwg), 1e-3, controls, gradU=fg.gradU)
+        tb = srv.submit(db, fb, abi.model_dict("FENE-PLog", lam=5.0, **kwb), 5e-4, controls)
+        peak = 0.0
+        while srv.progress() > 0:
+            f_now, _ = mem_info()
+            peak = max(peak, 1 - f_now / total)
+        out_g, out_b = srv.poll(tg), srv.poll(tb)
+        srv.close()
+    finally:
+        assert hip.hipFree(ballast) == 0
+    print(f"[case] peak device memory in use {peak:.2f} of {total / 2**30:.0f} GiB", flush=True)
+    assert peak > 0.75
+    bars = dict(theta=5e-9, tau=5e-9 * (1 + np.abs(ref_g["theta"]).max()))
+    for k in ("theta", "tau"):
+        assert np.all(np.isfinite(out_g[k]))
+        assert rel(out_g[k], ref_g[k]) < bars[k], (k, rel(out_g[k], ref_g[k]), bars[k])
+        assert np.array_equal(out_b[k], ref_b[k]), k
+    assert all(q.converged for q in out_g["perf"] if q.solved)
