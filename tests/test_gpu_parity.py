@@ -728,3 +728,74 @@ def test_multimode_matches_per_mode_oracle(ctx):
     assert rel_l2(eq.tau_total(), want_t) < 1e-13
     perf = eq.solver_perf()
     assert all(q.converged for q in perf if q.solved)
+
+
+INV_MODELS = [("Oldroyd-BLog", dict(lam=1.7)), ("GiesekusLog", dict(lam=0.8, alpha=0.2)),
+              ("FENE-PLog", dict(lam=2.3, L2=100.0)),
+              ("PTTLog", dict(lam=1.1, epsilon=0.25, zeta=0.1)),
+              ("FENE-CRLog", dict(lam=1.5, L2=50.0)),
+              ("XPomPomLog", dict(lambdaS=0.4, lambdaB=1.6, alpha=0.1, q=3.0, n=2.0)),
+              ("SaramitoLog", dict(lam=1.3, tau0=0.3, n=1.0, dims=(1, 1, 1), PTTfunction="linear",
+                                   epsilon=0.2, zeta=0.1))]
+
+
+@pytest.mark.parametrize("model,kw", INV_MODELS)
+def test_device_step_is_frame_indifferent_and_basis_free(ctx, model, kw):
+    """phi = 0 and a prescribed gradient make each cell's step independent, so
+    every cell may be rotated by its own R: the device's new theta and tau
+    must rotate with the inputs (theta, grad U, eigenvectors, the previous
+    tau), and must not depend on the signs and order of the eigenpairs
+    handed in."""
+    import dataclasses
+    m = cases.cartesian(6, 5, 4)
+    f = cases.make_fields(m, cases.FIELDS_TG, seed=5, lnLo=-2.0, lnHi=2.0)
+    f = dataclasses.replace(f, phi=np.zeros_like(f.phi))
+    n = m.nCells
+    rng = np.random.default_rng(3)
+    IJ = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+
+    def full(s6):
+        t = np.empty((n, 3, 3))
+        for k, (i, j) in enumerate(IJ):
+            t[:, i, j] = t[:, j, i] = s6[:, k]
+        return t
+
+    def six(t):
+        return np.stack([t[:, i, j] for i, j in IJ], axis=-1)
+
+    q, r = np.linalg.qr(rng.normal(size=(n, 3, 3)))
+    R = q * np.sign(np.diagonal(r, axis1=1, axis2=2))[:, None, :]
+    R[np.linalg.det(R) < 0, :, 0] *= -1
+    Rt = R.transpose(0, 2, 1)
+    th = full(f.theta)
+    w, V = np.linalg.eigh(th)
+    ev = np.zeros((n, 3, 3))
+    ev[:, [0, 1, 2], [0, 1, 2]] = np.exp(w)
+    T = rng.normal(size=(n, 3, 3))
+    tau0 = 0.5 * (T + T.transpose(0, 2, 1))
+    L = 0.5 * rng.normal(size=(n, 3, 3))
+    dm = ctx.mesh(m)
+
+    def run(th_, ev_, V_, L_, tau_):
+        eq = ctx.constitutiveEq("", dm, abi.model_dict(model, **kw))
+        eq.set_solver(abi.solver_dict(1e-12))
+        eq.set_state(six(th_), ev_, V_, six(tau_) if model == "SaramitoLog" else None)
+        eq.set_flow(f.U, f.Ub, f.phi)
+        assert eq.correct(1e-2, gradU=L_) == abi.RT_OK
+        out = full(eq.theta()), full(eq.tau())
+        eq.close()
+        return out
+
+    print(f"[case] frame indifference {model}", flush=True)
+    base = run(th, ev, V, L, tau0)
+    rot = run(R @ th @ Rt, ev, R @ V, R @ L @ Rt, R @ tau0 @ Rt)
+    perm = np.array([rng.permutation(3) for _ in range(n)])
+    signs = rng.choice([-1.0, 1.0], size=(n, 3))
+    V2 = np.take_along_axis(V, perm[:, None, :], axis=2) * signs[:, None, :]
+    ev2 = np.zeros_like(ev)
+    ev2[:, [0, 1, 2], [0, 1, 2]] = np.take_along_axis(np.exp(w), perm, axis=1)
+    basis = run(th, ev2, V2, L, tau0)
+    for k in range(2):
+        scale = np.abs(base[k]).max()
+        assert np.abs(rot[k] - R @ base[k] @ Rt).max() / scale < 1e-11, (k, model)
+        assert np.abs(basis[k] - base[k]).max() / scale < 1e-11, (k, model)
