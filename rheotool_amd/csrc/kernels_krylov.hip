@@ -44,6 +44,16 @@ struct KryArgs {
 };
 
 // ---- true residual r = b - A x (FP64): MODE 0 initial, MODE 1 refinement ----
+// A refinement sweep (MODE 1) waits until no inner solve is running; the
+// gate is read on the device so a sweep may be queued ahead of knowing.
+template <class T>
+__device__ __forceinline__ bool inner_running(const KryArgs<T>& a) {
+    bool run = false;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) run |= a.valid[c] > 0 && a.cs[c].active;
+    return run;
+}
+
 template <class T, int MODE>
 __global__ void __launch_bounds__(BLK) k_residual(KryArgs<T> a) {
     constexpr int NQ = MODE == 0 ? 18 : 12;
@@ -54,7 +64,7 @@ __global__ void __launch_bounds__(BLK) k_residual(KryArgs<T> a) {
         on[c] = a.valid[c] > 0 && (MODE == 0 || a.cs[c].needRefine);
         any |= on[c];
     }
-    if (!any) return;
+    if (!any || (MODE == 1 && inner_running(a))) return;
     double q[NQ];
 #pragma unroll
     for (int i = 0; i < NQ; ++i) q[i] = 0;
@@ -123,7 +133,7 @@ __global__ void __launch_bounds__(BLK) k_fin_residual(KryArgs<T> a) {
         on[c] = a.valid[c] > 0 && (MODE == 0 || a.cs[c].needRefine);
         any |= on[c];
     }
-    if (!any) return;
+    if (!any || (MODE == 1 && inner_running(a))) return;
     double sums[NQ];
     sum_partials<NQ>(a.partials, a.G, sums);
     if (threadIdx.x == 0) {
@@ -699,6 +709,9 @@ void launch_round(Ctx* ctx, SolveRun& R, double nAct, double nRef, bool start) {
     hipLaunchKernelGGL(k_fin_xr<T>, dim3(1), dim3(BLK), 0, ctx->stream, a);
 }
 
+// rounds a non-blocking advance queues per host round trip
+constexpr int kRoundsAhead = 3;
+
 void queue_pull(Ctx* ctx, Krylov& K, SolveRun& R) {
     SolveState* st = K.state.as<SolveState>();
     RT_HIP(hipMemcpyAsync(R.hostState, st, sizeof(SolveState), hipMemcpyDeviceToHost, ctx->stream));
@@ -781,8 +794,22 @@ bool solve_advance(Ctx* ctx, Krylov& K, bool block) {
     if (nAct > 0) nRef = 0;
     if (R.mixed) launch_round<float>(ctx, R, nAct, nRef, start);
     else launch_round<double>(ctx, R, nAct, nRef, start);
-    RT_HIP(hipGetLastError());
     ++R.rounds;
+    // Non-blocking callers (the serving loop) also queue the next rounds
+    // before the state comes back: every kernel of a round reads the
+    // component records on the device and does nothing for finished
+    // components, and the refinement sweep is gated there too, so a round
+    // launched ahead does what the host would have launched.  This hides the
+    // host round trip, which dominates small solves.  Not while profiling:
+    // the byte accounting of a launch needs the host's view of the round.
+    if (!block && !ctx->profiling) {
+        for (int k = 1; k < kRoundsAhead; ++k) {
+            if (R.mixed) launch_round<float>(ctx, R, nAct, 1, false);
+            else launch_round<double>(ctx, R, nAct, 1, false);
+            ++R.rounds;
+        }
+    }
+    RT_HIP(hipGetLastError());
     queue_pull(ctx, K, R);
     return false;
 }
