@@ -36,6 +36,7 @@ struct KryArgs {
     const double* b;     // [N][6]
     double* x;           // [N][6]
     T *r, *r0, *p, *v, *t;  // [N][6]
+    T* e;                // [N][6] mixed: the inner solve's correction to x (NULL in FP64)
     double* partials;
     CompState* cs;
     int32_t valid[6];
@@ -355,13 +356,19 @@ __global__ void __launch_bounds__(BLK) k_fin_t(KryArgs<T> a) {
 // ---- x += alpha M^-1 p + omega M^-1 s ; r = s - omega t ; |r|, (r0, r) ------
 template <class T>
 __global__ void __launch_bounds__(BLK) k_kry_xr(KryArgs<T> a) {
-    bool on[6], ex[6];
+    // FP64: x += alpha M^-1 p + omega M^-1 s in place.  Mixed: the inner
+    // solve accumulates the same update into its FP32 correction e (from 0 at
+    // the solve's first iteration) and k_fold adds e to x in FP64 before the
+    // refinement sweep, so the pass moves 4-byte instead of 8-byte x words.
+    constexpr bool MIX = std::is_same<T, float>::value;
+    bool on[6], ex[6], first[6];
     double alpha[6], omega[6];
     bool any = false;
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
         on[c] = a.cs[c].active;
         ex[c] = a.cs[c].exitAtS;
+        first[c] = a.cs[c].innerIter == 0;
         alpha[c] = a.cs[c].alpha;
         omega[c] = a.cs[c].omega;
         any |= on[c];
@@ -379,8 +386,15 @@ __global__ void __launch_bounds__(BLK) k_kry_xr(KryArgs<T> a) {
         const double rd = a.rD[r];
         const int64_t j = (int64_t)NCMP * r;
         double xv[6];
+        T ev[6];
         T pp[6], rr[6], vv[6], tt[6], rr0[6];
-        ld6(a.x + j, xv);
+        if constexpr (MIX) {
+            ld6(a.e + j, ev);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) xv[c] = first[c] ? 0.0 : (double)ev[c];
+        } else {
+            ld6(a.x + j, xv);
+        }
         ld6(a.p + j, pp);
         ld6(a.r + j, rr);
         ld6(a.v + j, vv);
@@ -402,13 +416,45 @@ __global__ void __launch_bounds__(BLK) k_kry_xr(KryArgs<T> a) {
             q[c] += fabs(rn);
             q[6 + c] += rr0[c] * rn;
         }
-        st6(a.x + j, xv);
+        if constexpr (MIX) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) ev[c] = (T)xv[c];
+            st6(a.e + j, ev);
+        } else {
+            st6(a.x + j, xv);
+        }
         st6(a.r + j, rr);
     }
     double qd[12];
 #pragma unroll
     for (int i = 0; i < 12; ++i) qd[i] = q[i];
     write_partials<12>(qd, a.partials);
+}
+
+// ---- mixed: x += e (FP64) for the components about to be refined ----------
+// Gated like the refinement sweep it precedes (no inner solve running, the
+// component waiting for refinement); the sweep's finalize then clears the
+// wait, so a correction is added once.
+template <class T>
+__global__ void __launch_bounds__(BLK) k_fold(KryArgs<T> a) {
+    bool on[6];
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        on[c] = a.valid[c] > 0 && a.cs[c].needRefine;
+        any |= on[c];
+    }
+    if (!any || inner_running(a)) return;
+    for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < a.N; i += (int64_t)gridDim.x * BLK) {
+        double xv[6];
+        T ev[6];
+        ld6(a.x + NCMP * i, xv);
+        ld6(a.e + NCMP * i, ev);
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+            if (on[c]) xv[c] += (double)ev[c];
+        st6(a.x + NCMP * i, xv);
+    }
 }
 
 template <class T>
@@ -550,7 +596,7 @@ void Krylov::ensure(Mesh& m, bool mixed) {
         N = m.N;
         grid = grid_for(ctx, N);
         r = DBuf(); r0 = DBuf(); p = DBuf(); v = DBuf(); t = DBuf();
-        rs = DBuf(); r0s = DBuf(); ps = DBuf(); vs = DBuf(); ts = DBuf();
+        rs = DBuf(); r0s = DBuf(); ps = DBuf(); vs = DBuf(); ts = DBuf(); es = DBuf();
         ahat32 = DBuf(); ahat64 = DBuf();
         partials = ctx->alloc((size_t)grid * 24 * sizeof(double));
         state = ctx->alloc(sizeof(SolveState));
@@ -564,7 +610,7 @@ void Krylov::ensure(Mesh& m, bool mixed) {
     }
     if (mixed && !rs.p) {
         rs = ctx->alloc(vec * 4); r0s = ctx->alloc(vec * 4); ps = ctx->alloc(vec * 4);
-        vs = ctx->alloc(vec * 4); ts = ctx->alloc(vec * 4);
+        vs = ctx->alloc(vec * 4); ts = ctx->alloc(vec * 4); es = ctx->alloc(vec * 4);
         ahat32 = ctx->alloc(std::max<int64_t>(1, m.nEll) * 4);
     }
 }
@@ -636,8 +682,10 @@ void fill_args(Ctx* ctx, const SolveArgs& sa, SolveRun& R) {
     a.x = sa.x;
     if (mixed) {
         a.r = (T*)K.rs.p; a.r0 = (T*)K.r0s.p; a.p = (T*)K.ps.p; a.v = (T*)K.vs.p; a.t = (T*)K.ts.p;
+        a.e = (T*)K.es.p;
     } else {
         a.r = (T*)K.r.p; a.r0 = (T*)K.r0.p; a.p = (T*)K.p.p; a.v = (T*)K.v.p; a.t = (T*)K.t.p;
+        a.e = nullptr;
     }
     a.partials = K.partials.as<double>();
     a.cs = st->c;
@@ -655,7 +703,9 @@ struct Bytes {
     double p(double nc, bool start) const { return nc * N * (start ? 2 : 4) * w; }
     double v(double nc) const { return ne * (4 + w) + nc * N * 3 * w; }
     double t(double nc) const { return ne * (4 + w) + nc * N * 3 * w; }
-    double xr(double nc) const { return N * 8.0 + nc * N * (16 + 6 * w); }
+    // FP64: x read and written in place; mixed: the FP32 correction instead
+    double xr(double nc) const { return N * 8.0 + nc * N * (w == 4 ? 8 * w : 16 + 6 * w); }
+    double fold(double nc) const { return nc * N * (8 + 8 + w); }
 };
 
 template <class T>
@@ -681,6 +731,10 @@ void launch_round(Ctx* ctx, SolveRun& R, double nAct, double nRef, bool start) {
         R.needScale = false;
     }
     if (nRef > 0) {
+        if (mixed) {
+            ProfScope ps(ctx, "k_fold<f32>", B.fold(nRef));
+            hipLaunchKernelGGL(k_fold<T>, dim3(R.pgrid), dim3(BLK), 0, ctx->stream, a);
+        }
         {
             ProfScope ps(ctx, mixed ? "k_residual<f32>" : "k_residual<f64>", B.res(nRef));
             hipLaunchKernelGGL((k_residual<T, 1>), dim3(R.G), dim3(BLK), 0, ctx->stream, a);

@@ -218,6 +218,7 @@ struct Krylov {
     int grid = 0;  // workgroups of the row kernels
     DBuf r, r0, p, v, t;   // FP64 [N][6]
     DBuf rs, r0s, ps, vs, ts;  // FP32 [N][6] (mixed precision inner solve)
+    DBuf es;               // FP32 [N][6] the inner solve's correction to x (mixed)
     DBuf ahat32;           // float [nEll]  column-scaled off-diagonals A D^-1
     DBuf ahat64;           // double [nEll]
     DBuf rD;               // double [N]    1/diag
