@@ -119,6 +119,9 @@ typedef struct {
     const rt_patch* patches;  /* [nPatches] */
     const double* C;          /* [3*nCells] cell centres; may be NULL (then only the
                                  upwind and none schemes are available) */
+    const double* Cb;         /* [3*nBoundaryFaces] boundary face centres Cf; may be
+                                 NULL (then divTau's BSD and coupling terms, which
+                                 need the patch delta coefficients, are unavailable) */
 } rt_mesh_desc;
 
 enum rt_renumber { RT_RENUMBER_AUTO = 0, RT_RENUMBER_NONE = 1, RT_RENUMBER_RCM = 2, RT_RENUMBER_SFC = 3 };
@@ -252,13 +255,32 @@ rt_model* rt_model_mode(rt_model* model, int i);
 rt_model* rt_model_fluidity(rt_model* model);
 
 /* ------------------------------------------- consumers of tau (momentum eq.)
- * The explicit stress term constitutiveEq::divTau hands to the momentum
- * equation, fvc::div(tau()/rho(), "div(tau)") with Gauss linear
- * (constitutiveEquations/constitutiveEqs/constitutiveEq/constitutiveEq.C:
- * 72-132; the fvm::laplacian(etaS/rho, U) part is the momentum solver's own
- * matrix).  Reads the model's current tau on the device; out [3N].
- * tauB [6*nBoundaryFaces]: tau's patch values, NULL for zeroGradient. */
+ * constitutiveEq::divTau(U) (constitutiveEquations/constitutiveEqs/
+ * constitutiveEq/constitutiveEq.C:72-132, the segregated branch) returns
+ *   explicit(U) + fvm::laplacian(nu, U)
+ * with the model's `stabilization` option:
+ *   none      explicit = fvc::div(tau/rho, "div(tau)")                      nu = etaS/rho
+ *   BSD       explicit = fvc::div(tau/rho) - fvc::laplacian(etaP/rho, U)    nu = (etaP+etaS)/rho
+ *   coupling  explicit = fvc::div(tau/rho) - fvc::div((etaP/rho) grad(U))   nu = (etaP+etaS)/rho
+ * (div(tau) and div(grad(U)) Gauss linear, grad(U) Gauss linear from the flow
+ * last set).  rt_model_div_tau writes the explicit field from the model's
+ * current tau on the device, out [3N] (to be added to the momentum source as
+ * the caller's fvc terms are); rt_model_div_tau_coeff gives nu, the
+ * coefficient of the implicit fvm::laplacian the caller's momentum matrix
+ * holds.  tauB [6*nBoundaryFaces]: tau's patch values, NULL for
+ * zeroGradient.  U's patch values (rt_model_set_flow's Ub) are read as a
+ * fixedValue patch's (snGrad = (Ub - U_P) |n (n & (Cf - C_P))|^-1); a
+ * zeroGradient velocity patch gives Ub = U_P, hence a zero snGrad, as the
+ * reference's.  BSD and coupling need the mesh's cell and boundary face
+ * centres.  multiMode sums the modes' terms and coefficients
+ * (multiMode.C:141-151). */
 int rt_model_div_tau(rt_model* model, const double* tauB, double* out);
+int rt_model_div_tau_coeff(const rt_model* model, double* nu);
+/* The fvSchemes entry laplacian(eta,U) that BSD's explicit laplacian uses
+ * (e.g. tutorials/rheoFoam/Channel/Oldroyd-BLog/system/fvSchemes:44
+ * "Gauss linear corrected"): "Gauss linear corrected" (default),
+ * "Gauss linear orthogonal" or "Gauss linear uncorrected". */
+int rt_model_set_laplacian_scheme(rt_model* model, const char* laplacianScheme);
 /* constitutiveEq::tauTotal() (constitutiveEq.C:534-547):
  * tau + etaS (symm(L + L^T) - 2/3 tr(L) I), L = fvc::grad(U) Gauss linear
  * from the flow last set.  out [6N]. */

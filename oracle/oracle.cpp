@@ -981,6 +981,114 @@ void orc_div_tau(const orc_mesh* m, const double* tau, const double* tauB, doubl
         for (int j = 0; j < 3; ++j) out[3 * (size_t)c + j] /= m->V[c];
 }
 
+void orc_div_tau_stab(const orc_mesh* m, const double* C, const double* Cb, const double* U, const double* Ub,
+                      const double* tau, const double* tauB, double rho, double etaP, int32_t stab,
+                      int32_t lapScheme, double* out) {
+    orc_div_tau(m, tau, tauB, rho, out);  // fvc::div(tau()/rho(), "div(tau)")
+    if (stab == ORC_STAB_NONE) return;
+    const int N = m->nCells, F = m->nInternal;
+    std::vector<double> G(9 * (size_t)N);
+    orc_grad_gauss(m, U, Ub, G.data());  // fvc::grad(U), cell values
+    const double gam = etaP / rho;       // etaP()/rho(), uniform (Constant thermo)
+    std::vector<double> S(3 * (size_t)N, 0.0);
+    auto vdotT = [](const double* v, const double* T, double* o) {  // v & T
+        for (int j = 0; j < 3; ++j) o[j] = v[0] * T[j] + v[1] * T[3 + j] + v[2] * T[6 + j];
+    };
+    auto mag3 = [](const double* v) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); };
+    // patch delta coefficient of a boundary face b of cell c: 1/|n (n & (Cf - C))|
+    auto patchDelta = [&](int64_t f, int b, int c, const double* n) {
+        const double d[3] = {Cb[3 * (size_t)b] - C[3 * (size_t)c], Cb[3 * (size_t)b + 1] - C[3 * (size_t)c + 1],
+                             Cb[3 * (size_t)b + 2] - C[3 * (size_t)c + 2]};
+        const double nd = n[0] * d[0] + n[1] * d[1] + n[2] * d[2];
+        const double dl[3] = {n[0] * nd, n[1] * nd, n[2] * nd};
+        (void)f;
+        return 1.0 / mag3(dl);
+    };
+    if (stab == ORC_STAB_COUPLING) {
+        // fvc::div((etaP/rho)*fvc::grad(U), "div(grad(U))"), Gauss linear
+        std::vector<double> Gp(9 * (size_t)N);
+        for (size_t k = 0; k < 9 * (size_t)N; ++k) Gp[k] = gam * G[k];
+        for (int f = 0; f < F; ++f) {
+            const int o = m->owner[f], n = m->neighbour[f];
+            double gf[9], v[3];
+            for (int q = 0; q < 9; ++q)
+                gf[q] = m->w[f] * (Gp[9 * (size_t)o + q] - Gp[9 * (size_t)n + q]) + Gp[9 * (size_t)n + q];
+            vdotT(m->Sf + 3 * (size_t)f, gf, v);
+            for (int j = 0; j < 3; ++j) {
+                S[3 * (size_t)o + j] += v[j];
+                S[3 * (size_t)n + j] -= v[j];
+            }
+        }
+        for (int p = 0; p < m->nPatches; ++p) {
+            if (m->patchEmpty[p]) continue;
+            for (int k = 0; k < m->patchSize[p]; ++k) {
+                const int b = m->patchStart[p] + k;
+                const int64_t f = (int64_t)F + b;
+                const int c = m->owner[f];
+                const double* sf = m->Sf + 3 * f;
+                const double ms = mag3(sf);
+                const double n[3] = {sf[0] / ms, sf[1] / ms, sf[2] / ms};
+                const double dcb = patchDelta(f, b, c, n);
+                const double* gP = G.data() + 9 * (size_t)c;
+                double sn[3], nG[3], gb[9], gpb[9], v[3];
+                for (int j = 0; j < 3; ++j) sn[j] = dcb * (Ub[3 * (size_t)b + j] - U[3 * (size_t)c + j]);
+                vdotT(n, gP, nG);
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) gb[3 * i + j] = gP[3 * i + j] + n[i] * (sn[j] - nG[j]);
+                for (int q = 0; q < 9; ++q) gpb[q] = gam * gb[q];
+                vdotT(sf, gpb, v);
+                for (int j = 0; j < 3; ++j) S[3 * (size_t)c + j] += v[j];
+            }
+        }
+    } else {
+        // fvc::laplacian(etaP/rho, U, "laplacian(eta,U)"): Gauss linear <snGrad>
+        for (int f = 0; f < F; ++f) {
+            const int o = m->owner[f], n = m->neighbour[f];
+            const double* sf = m->Sf + 3 * (size_t)f;
+            const double ms = mag3(sf);
+            const double nf[3] = {sf[0] / ms, sf[1] / ms, sf[2] / ms};
+            const double d[3] = {C[3 * (size_t)n] - C[3 * (size_t)o], C[3 * (size_t)n + 1] - C[3 * (size_t)o + 1],
+                                 C[3 * (size_t)n + 2] - C[3 * (size_t)o + 2]};
+            double dc;
+            if (lapScheme == ORC_SNGRAD_ORTHOGONAL) dc = 1.0 / mag3(d);
+            else dc = 1.0 / std::max(nf[0] * d[0] + nf[1] * d[1] + nf[2] * d[2], 0.05 * mag3(d));
+            double sn[3];
+            for (int j = 0; j < 3; ++j) sn[j] = dc * (U[3 * (size_t)n + j] - U[3 * (size_t)o + j]);
+            if (lapScheme == ORC_SNGRAD_CORRECTED) {
+                const double kv[3] = {nf[0] - d[0] * dc, nf[1] - d[1] * dc, nf[2] - d[2] * dc};
+                double gf[9], cr[3];
+                for (int q = 0; q < 9; ++q)
+                    gf[q] = m->w[f] * (G[9 * (size_t)o + q] - G[9 * (size_t)n + q]) + G[9 * (size_t)n + q];
+                vdotT(kv, gf, cr);
+                for (int j = 0; j < 3; ++j) sn[j] = sn[j] + cr[j];
+            }
+            const double gms = gam * ms;  // gammaMagSf
+            for (int j = 0; j < 3; ++j) {
+                const double v = gms * sn[j];
+                S[3 * (size_t)o + j] += v;
+                S[3 * (size_t)n + j] -= v;
+            }
+        }
+        for (int p = 0; p < m->nPatches; ++p) {
+            if (m->patchEmpty[p]) continue;
+            for (int k = 0; k < m->patchSize[p]; ++k) {
+                const int b = m->patchStart[p] + k;
+                const int64_t f = (int64_t)F + b;
+                const int c = m->owner[f];
+                const double* sf = m->Sf + 3 * f;
+                const double ms = mag3(sf);
+                const double n[3] = {sf[0] / ms, sf[1] / ms, sf[2] / ms};
+                const double dcb = patchDelta(f, b, c, n);
+                const double gms = gam * ms;
+                for (int j = 0; j < 3; ++j)
+                    S[3 * (size_t)c + j] += gms * (dcb * (Ub[3 * (size_t)b + j] - U[3 * (size_t)c + j]));
+            }
+        }
+    }
+    for (int c = 0; c < N; ++c)
+        for (int j = 0; j < 3; ++j) out[3 * (size_t)c + j] = out[3 * (size_t)c + j] - S[3 * (size_t)c + j] / m->V[c];
+}
+
 void orc_tau_total(const orc_mesh* m, const double* U, const double* Ub, const double* tau, double etaS,
                    double* out) {
     const int N = m->nCells;

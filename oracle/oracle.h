@@ -152,6 +152,29 @@ void orc_defc_sout(const orc_mesh* m, const double* phi, const double* dc, doubl
  * out [3N]. */
 void orc_div_tau(const orc_mesh* m, const double* tau, const double* tauB, double rho, double* out);
 
+/* The explicit part of constitutiveEq::divTau(U) for a stabilization option
+ * (constitutiveEq.C:72-132, segregated branch):
+ *   none      fvc::div(tau/rho)
+ *   BSD       fvc::div(tau/rho) - fvc::laplacian(etaP/rho, U)   "laplacian(eta,U)"
+ *   coupling  fvc::div(tau/rho) - fvc::div((etaP/rho)*fvc::grad(U))  "div(grad(U))"
+ * with Gauss linear div/grad and the laplacian's snGrad scheme lapScheme
+ * (ORC_SNGRAD_*).  [OF9] semantics restated: linear interpolation
+ * w*(a_o - a_n) + a_n; deltaCoeffs 1/|C_n - C_o| (orthogonal), nonOrth
+ * 1/max(n.d, 0.05|d|) with the correction (n - d*dc) & interpolate(grad U)
+ * (corrected); boundary faces: delta = n (n & (Cf - C_P)), patch snGrad
+ * (Ub - U_P)/|delta| (fixedValue; zeroGradient patches give Ub = U_P, hence 0);
+ * grad(U) on a patch face: G_P + n*(snGrad - n & G_P)
+ * (gaussGrad::correctBoundaryConditions).  C [3N] cell centres, Cb [3NB]
+ * boundary face centres, U [3N], Ub [3NB], tau [6N], tauB [6NB] or NULL
+ * (zeroGradient).  out [3N].  The implicit fvm::laplacian(nu, U) part is the
+ * momentum matrix's own: nu = etaS/rho (none), (etaP + etaS)/rho (BSD,
+ * coupling). */
+enum { ORC_STAB_NONE = 0, ORC_STAB_BSD = 1, ORC_STAB_COUPLING = 2 };
+enum { ORC_SNGRAD_CORRECTED = 0, ORC_SNGRAD_ORTHOGONAL = 1, ORC_SNGRAD_UNCORRECTED = 2 };
+void orc_div_tau_stab(const orc_mesh* m, const double* C, const double* Cb, const double* U, const double* Ub,
+                      const double* tau, const double* tauB, double rho, double etaP, int32_t stab,
+                      int32_t lapScheme, double* out);
+
 /* constitutiveEq::tauTotal() (constitutiveEq.C:534-547):
  * tau + etaS*(symm(L + L^T) - (2/3) tr(L) I), L = fvc::grad(U) Gauss linear.
  * out [6N]. */

@@ -99,6 +99,8 @@ def lib():
         L.orc_defc_sout.argtypes = [vp] * 4
         L.orc_div_tau.argtypes = [vp, vp, vp, C.c_double, vp]
         L.orc_tau_total.argtypes = [vp, vp, vp, vp, C.c_double, vp]
+        L.orc_div_tau_stab.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_double, C.c_double, C.c_int32,
+                                       C.c_int32, vp]
         L.orc_decomp_create.restype = vp
         L.orc_decomp_create.argtypes = [vp, C.c_int32, vp]
         L.orc_decomp_destroy.argtypes = [vp]
@@ -211,6 +213,22 @@ def div_tau(m, tau, tauB=None, rho=1.0):
     out = np.empty((m.nCells, 3))
     tb = None if tauB is None else np.ascontiguousarray(tauB)
     lib().orc_div_tau(om.ref, _p(np.ascontiguousarray(tau)), _p(tb), rho, _p(out))
+    return out
+
+
+STAB = {"none": 0, "BSD": 1, "coupling": 2}
+SNGRAD = {"corrected": 0, "orthogonal": 1, "uncorrected": 2}
+
+
+def div_tau_stab(m, U, Ub, tau, tauB=None, rho=1.0, etaP=0.0, stab="none", lap="corrected"):
+    """The explicit part of divTau(U) for a stabilization option: [N, 3]."""
+    om = OracleMesh(m)
+    out = np.empty((m.nCells, 3))
+    tb = None if tauB is None else np.ascontiguousarray(tauB)
+    Cb = np.ascontiguousarray(m.Cf[m.nInternal:]) if m.nBoundary else np.zeros((1, 3))
+    Ub = np.ascontiguousarray(Ub) if m.nBoundary else np.zeros((1, 3))
+    lib().orc_div_tau_stab(om.ref, _p(np.ascontiguousarray(m.C)), _p(Cb), _p(np.ascontiguousarray(U)), _p(Ub),
+                           _p(np.ascontiguousarray(tau)), _p(tb), rho, etaP, STAB[stab], SNGRAD[lap], _p(out))
     return out
 
 

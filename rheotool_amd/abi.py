@@ -48,7 +48,7 @@ class MeshDesc(C.Structure):
                 ("nBoundaryFaces", C.c_int32), ("nPatches", C.c_int32),
                 ("owner", C.c_void_p), ("neighbour", C.c_void_p), ("Sf", C.c_void_p),
                 ("weights", C.c_void_p), ("V", C.c_void_p), ("patches", C.c_void_p),
-                ("C", C.c_void_p)]
+                ("C", C.c_void_p), ("Cb", C.c_void_p)]
 
 
 class SolverPerf(C.Structure):
@@ -115,6 +115,8 @@ def lib():
             "rt_model_fluidity": (vp, [vp]),
             "rt_model_div_tau": (i32, [vp, vp, vp]),
             "rt_model_tau_total": (i32, [vp, vp]),
+            "rt_model_div_tau_coeff": (i32, [vp, vp]),
+            "rt_model_set_laplacian_scheme": (i32, [vp, C.c_char_p]),
             "rt_model_correct": (i32, [vp, C.c_double, vp]),
             "rt_model_correct_iter": (i32, [vp, C.c_double, vp, i32]),
             "rt_model_get": (i32, [vp, i32, vp]),
@@ -343,11 +345,23 @@ class ConstitutiveEq:
         self.ctx.check(lib().rt_model_set_scheme(self.h, div_scheme.encode()), "set_scheme")
 
     def div_tau(self, tauB=None):
-        """fvc::div(tau/rho) of the current tau: [N, 3]."""
+        """The explicit part of divTau(U) for the model's stabilization option
+        (fvc::div(tau/rho), minus the BSD laplacian or the coupling
+        div((etaP/rho) grad U)) from the current tau and flow: [N, 3]."""
         out = np.empty((self.N, 3))
         tb = None if tauB is None else _f64(tauB, (self.mesh.nBoundary, 6))
         self.ctx.check(lib().rt_model_div_tau(self.h, _p(tb), _p(out)), "div_tau")
         return out
+
+    def div_tau_coeff(self) -> float:
+        """nu of divTau's implicit fvm::laplacian(nu, U)."""
+        v = C.c_double()
+        self.ctx.check(lib().rt_model_div_tau_coeff(self.h, C.byref(v)), "div_tau_coeff")
+        return v.value
+
+    def set_laplacian_scheme(self, text: str):
+        """fvSchemes laplacian(eta,U): "Gauss linear corrected|orthogonal|uncorrected"."""
+        self.ctx.check(lib().rt_model_set_laplacian_scheme(self.h, text.encode()), "set_laplacian_scheme")
 
     def tau_total(self):
         out = np.empty((self.N, 6))
@@ -422,7 +436,7 @@ class FluidityEq(ConstitutiveEq):
 
 
 def model_dict(model="Oldroyd-BLog", etaS=0.59, etaP=0.41, lam=1.0, alpha=None, L2=None,
-               epsilon=None, zeta=None, destruction="linear", rho=1.0, **extra):
+               epsilon=None, zeta=None, destruction="linear", rho=1.0, stabilization="coupling", **extra):
     """constitutiveProperties `parameters` text in OpenFOAM syntax.  extra:
     further constants by their dictionary keys (lambdaR, lambdaD, beta, delta,
     chiMax, lambdaS, lambdaB, q, n, m, K, L, a, b)."""
@@ -447,7 +461,7 @@ def model_dict(model="Oldroyd-BLog", etaS=0.59, etaP=0.41, lam=1.0, alpha=None, 
         lines.append(f"zeta zeta [0 0 0 0 0 0 0] {zeta!r};")
     if model == "PTTLog":
         lines.append(f"destructionFunctionType {destruction};")
-    lines.append("stabilization coupling;")
+    lines.append(f"stabilization {stabilization};")
     return "parameters\n{\n    " + "\n    ".join(lines) + "\n}\n"
 
 
@@ -464,14 +478,15 @@ def mesh_desc(m):
     """rt_mesh_desc for a cases.Mesh; returns (desc, keepalive)."""
     keep = dict(owner=np.ascontiguousarray(m.owner, np.int32),
                 nbr=np.ascontiguousarray(m.neighbour, np.int32), Sf=_f64(m.Sf), w=_f64(m.w),
-                V=_f64(m.V), C=_f64(m.C) if getattr(m, "C", None) is not None else None)
+                V=_f64(m.V), C=_f64(m.C) if getattr(m, "C", None) is not None else None,
+                Cb=_f64(m.Cf[m.nInternal:]) if getattr(m, "Cf", None) is not None and m.nBoundary else None)
     pt = (Patch * max(1, len(m.patchStart)))()
     for i in range(len(m.patchStart)):
         pt[i] = Patch(int(m.patchStart[i]), int(m.patchSize[i]), int(m.patchKind[i]))
     keep["pt"] = pt
     d = MeshDesc(m.nCells, m.nInternal, m.nBoundary, len(m.patchStart), _p(keep["owner"]),
                  _p(keep["nbr"]), _p(keep["Sf"]), _p(keep["w"]), _p(keep["V"]),
-                 C.cast(pt, C.c_void_p), _p(keep["C"]))
+                 C.cast(pt, C.c_void_p), _p(keep["C"]), _p(keep["Cb"]))
     return d, keep
 
 

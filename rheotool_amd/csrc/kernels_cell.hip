@@ -992,12 +992,24 @@ void launch_defc(Ctx* ctx, const AssembleArgs& a, int scheme, const double* C4, 
 }
 
 // ------------------------------------------- stress terms of the momentum eq.
-// fvc::div(tau/rho), Gauss linear (constitutiveEq.C:88-92): per row, faces in
-// increasing face id then the row's boundary faces (patch value tauB [6][NB],
-// or the cell value when tauB is NULL: zeroGradient), then / V.
+// The explicit part of constitutiveEq::divTau(U) (constitutiveEq.C:72-132):
+// fvc::div(tau/rho) with Gauss linear, per row the faces in increasing face id
+// then the row's boundary faces (patch value tauB [6][NB], or the cell value
+// when tauB is NULL: zeroGradient), then / V; minus, for STAB,
+//  * BSD (1): fvc::laplacian(etaP/rho, U), Gauss linear with the snGrad scheme
+//    LAP (0 corrected, 1 orthogonal, 2 uncorrected): per face
+//    gamma |Sf| (dc (U_n - U_o) [+ k & interpolate(grad U)]), the boundary
+//    faces' patch snGrad dcB (Ub - U_P);
+//  * coupling (2): fvc::div((etaP/rho) grad(U)), Gauss linear, the patch value
+//    of grad(U) being G_P + n (snGrad - n & G_P).
+// Each face term is evaluated in the row's frame (d = C_other - C_row, Sf
+// signed for the row's side), which negates the owner's value exactly, so
+// both rows of a face see the reference's face value with its sign.
+template <int STAB, int LAP>
 __global__ void __launch_bounds__(BLK) k_div_tau(AssembleArgs a, const double* __restrict__ tau,
-                                                 const double* __restrict__ tauB, double rho,
-                                                 double* __restrict__ out) {
+                                                 const double* __restrict__ tauB, double rho, double gam,
+                                                 const double* __restrict__ G, const double* __restrict__ C4,
+                                                 const double* __restrict__ bDc, double* __restrict__ out) {
     const int64_t NB = a.NB > 0 ? a.NB : 1;
     for (int64_t it = 0;; ++it) {
         const int64_t tl = tile_of(it, a.N);
@@ -1008,7 +1020,14 @@ __global__ void __launch_bounds__(BLK) k_div_tau(AssembleArgs a, const double* _
         ld6(tau + (int64_t)NCMP * r, t);
 #pragma unroll
         for (int q = 0; q < 6; ++q) t[q] = t[q] / rho;
-        double acc[3] = {0, 0, 0};
+        double acc[3] = {0, 0, 0}, st[3] = {0, 0, 0};
+        double Ur[3] = {0, 0, 0}, Cr[3] = {0, 0, 0}, Gr[9];
+        if constexpr (STAB != 0) {
+            Ur[0] = a.U[4 * (int64_t)r], Ur[1] = a.U[4 * (int64_t)r + 1], Ur[2] = a.U[4 * (int64_t)r + 2];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) Gr[q] = G[9 * (int64_t)r + q];
+            if constexpr (STAB == 1) Cr[0] = C4[4 * (int64_t)r], Cr[1] = C4[4 * (int64_t)r + 1], Cr[2] = C4[4 * (int64_t)r + 2];
+        }
         const int s = r >> 6, lane = r & 63;
         const int64_t base = a.ellOff[s] + lane;
         const int width = a.ellWidth[s];
@@ -1033,6 +1052,47 @@ __global__ void __launch_bounds__(BLK) k_div_tau(AssembleArgs a, const double* _
             acc[0] += sf[0] * tf[0] + sf[1] * tf[1] + sf[2] * tf[2];
             acc[1] += sf[0] * tf[1] + sf[1] * tf[3] + sf[2] * tf[4];
             acc[2] += sf[0] * tf[2] + sf[1] * tf[4] + sf[2] * tf[5];
+            if constexpr (STAB != 0) {
+                double Gc[9];
+#pragma unroll
+                for (int q = 0; q < 9; ++q) Gc[q] = G[9 * (int64_t)c + q];
+                if constexpr (STAB == 2) {
+                    double gf[9];
+#pragma unroll
+                    for (int q = 0; q < 9; ++q) {
+                        const double pr = gam * Gr[q], pc = gam * Gc[q];
+                        const double vo = own ? pr : pc, vn = own ? pc : pr;
+                        gf[q] = wf * (vo - vn) + vn;
+                    }
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) st[j] += sf[0] * gf[j] + sf[1] * gf[3 + j] + sf[2] * gf[6 + j];
+                } else {
+                    const double Cc[3] = {C4[4 * (int64_t)c], C4[4 * (int64_t)c + 1], C4[4 * (int64_t)c + 2]};
+                    const double Uc[3] = {a.U[4 * (int64_t)c], a.U[4 * (int64_t)c + 1], a.U[4 * (int64_t)c + 2]};
+                    const double d[3] = {Cc[0] - Cr[0], Cc[1] - Cr[1], Cc[2] - Cr[2]};
+                    const double ms = sqrt(sf[0] * sf[0] + sf[1] * sf[1] + sf[2] * sf[2]);
+                    const double nf[3] = {sf[0] / ms, sf[1] / ms, sf[2] / ms};
+                    const double md = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                    const double dc = LAP == 1 ? 1.0 / md : 1.0 / fmax(nf[0] * d[0] + nf[1] * d[1] + nf[2] * d[2], 0.05 * md);
+                    double sn[3];
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) sn[j] = dc * (Uc[j] - Ur[j]);
+                    if constexpr (LAP == 0) {
+                        const double kv[3] = {nf[0] - d[0] * dc, nf[1] - d[1] * dc, nf[2] - d[2] * dc};
+                        double gf[9];
+#pragma unroll
+                        for (int q = 0; q < 9; ++q) {
+                            const double vo = own ? Gr[q] : Gc[q], vn = own ? Gc[q] : Gr[q];
+                            gf[q] = wf * (vo - vn) + vn;
+                        }
+#pragma unroll
+                        for (int j = 0; j < 3; ++j) sn[j] = sn[j] + (kv[0] * gf[j] + kv[1] * gf[3 + j] + kv[2] * gf[6 + j]);
+                    }
+                    const double gms = gam * ms;
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) st[j] += gms * sn[j];
+                }
+            }
         }
         const int32_t b0 = a.bPtr[r], b1 = a.bPtr[r + 1];
         for (int32_t qb = b0; qb < b1; ++qb) {
@@ -1044,64 +1104,97 @@ __global__ void __launch_bounds__(BLK) k_div_tau(AssembleArgs a, const double* _
             acc[0] += sf[0] * tb[0] + sf[1] * tb[1] + sf[2] * tb[2];
             acc[1] += sf[0] * tb[1] + sf[1] * tb[3] + sf[2] * tb[4];
             acc[2] += sf[0] * tb[2] + sf[1] * tb[4] + sf[2] * tb[5];
+            if constexpr (STAB != 0) {
+                const double ub[3] = {a.Ub[bf], a.Ub[NB + bf], a.Ub[2 * NB + bf]};
+                const double ms = sqrt(sf[0] * sf[0] + sf[1] * sf[1] + sf[2] * sf[2]);
+                const double dcb = bDc[bf];
+                double sn[3];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) sn[j] = dcb * (ub[j] - Ur[j]);
+                if constexpr (STAB == 1) {
+                    const double gms = gam * ms;
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) st[j] += gms * sn[j];
+                } else {
+                    const double n[3] = {sf[0] / ms, sf[1] / ms, sf[2] / ms};
+                    double nG[3], gpb[9];
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) nG[j] = n[0] * Gr[j] + n[1] * Gr[3 + j] + n[2] * Gr[6 + j];
+#pragma unroll
+                    for (int i = 0; i < 3; ++i)
+#pragma unroll
+                        for (int j = 0; j < 3; ++j) gpb[3 * i + j] = gam * (Gr[3 * i + j] + n[i] * (sn[j] - nG[j]));
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) st[j] += sf[0] * gpb[j] + sf[1] * gpb[3 + j] + sf[2] * gpb[6 + j];
+                }
+            }
         }
         const double vol = a.V[r];
 #pragma unroll
-        for (int j = 0; j < 3; ++j) out[3 * (int64_t)r + j] = acc[j] / vol;
+        for (int j = 0; j < 3; ++j) out[3 * (int64_t)r + j] = STAB == 0 ? acc[j] / vol : acc[j] / vol - st[j] / vol;
     }
+}
+
+// fvc::grad(U) Gauss linear of row r (boilerLog.H:1): the row's faces in
+// increasing face id, then its boundary faces with the patch values Ub, then
+// / V -- the same sums, in the same order, as the assembly's gather.
+__device__ __forceinline__ void gauss_grad_u(const AssembleArgs& a, int32_t r, double* g) {
+    const int64_t NB = a.NB > 0 ? a.NB : 1;
+    const int64_t nE = a.nEll;
+    const double2 u01 = reinterpret_cast<const double2*>(a.U)[2 * (int64_t)r];
+    const double Ui[3] = {u01.x, u01.y, a.U[4 * (int64_t)r + 2]};
+#pragma unroll
+    for (int q = 0; q < 9; ++q) g[q] = 0.0;
+    const int s = r >> 6, lane = r & 63;
+    const int64_t base = a.ellOff[s] + lane;
+    const int width = a.ellWidth[s];
+    for (int k = 0; k < width; ++k) {
+        const int64_t e = base + (int64_t)SLICE * k;
+        const int32_t fid = a.ellFid[e];
+        if (fid == RT_FID_PAD) continue;
+        const bool own = fid >= 0;
+        const int32_t c = a.ellCol[base + (int64_t)SLICE * a.ellSlot[e]];
+        const double wf = a.eW[e];
+        const double2 c01 = reinterpret_cast<const double2*>(a.U)[2 * (int64_t)c];
+        const double Uc[3] = {c01.x, c01.y, a.U[4 * (int64_t)c + 2]};
+        double uf[3];
+#pragma unroll
+        for (int q3 = 0; q3 < 3; ++q3) {
+            const double uo = own ? Ui[q3] : Uc[q3], un = own ? Uc[q3] : Ui[q3];
+            uf[q3] = wf * (uo - un) + un;
+        }
+        const double sf[3] = {a.eSf[e], a.eSf[nE + e], a.eSf[2 * nE + e]};
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) g[3 * i + j] += sf[i] * uf[j];
+    }
+    const int32_t b0 = a.bPtr[r], b1 = a.bPtr[r + 1];
+    for (int32_t qb = b0; qb < b1; ++qb) {
+        const int32_t bf = a.bFace[qb];
+        const double sf[3] = {a.bSf[bf], a.bSf[NB + bf], a.bSf[2 * NB + bf]};
+        const double ub[3] = {a.Ub[bf], a.Ub[NB + bf], a.Ub[2 * NB + bf]};
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) g[3 * i + j] += sf[i] * ub[j];
+    }
+    const double vol = a.V[r];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) g[q] /= vol;
 }
 
 // tauTotal() = tau + etaS (symm(L + L^T) - 2/3 tr(L) I), L = fvc::grad(U) Gauss
 // linear, the same gather as the assembly's (constitutiveEq.C:534-547).
 __global__ void __launch_bounds__(BLK) k_tau_total(AssembleArgs a, const double* __restrict__ tau, double etaS,
                                                    double* __restrict__ out) {
-    const int64_t NB = a.NB > 0 ? a.NB : 1;
-    const int64_t nE = a.nEll;
     for (int64_t it = 0;; ++it) {
         const int64_t tl = tile_of(it, a.N);
         if (tl < 0) break;
         const int32_t r = (int32_t)(tl * BLK + threadIdx.x);
         if (r >= a.N) continue;
-        const double2 u01 = reinterpret_cast<const double2*>(a.U)[2 * (int64_t)r];
-        const double Ui[3] = {u01.x, u01.y, a.U[4 * (int64_t)r + 2]};
-        double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        const int s = r >> 6, lane = r & 63;
-        const int64_t base = a.ellOff[s] + lane;
-        const int width = a.ellWidth[s];
-        for (int k = 0; k < width; ++k) {
-            const int64_t e = base + (int64_t)SLICE * k;
-            const int32_t fid = a.ellFid[e];
-            if (fid == RT_FID_PAD) continue;
-            const bool own = fid >= 0;
-            const int32_t c = a.ellCol[base + (int64_t)SLICE * a.ellSlot[e]];
-            const double wf = a.eW[e];
-            const double2 c01 = reinterpret_cast<const double2*>(a.U)[2 * (int64_t)c];
-            const double Uc[3] = {c01.x, c01.y, a.U[4 * (int64_t)c + 2]};
-            double uf[3];
-#pragma unroll
-            for (int q3 = 0; q3 < 3; ++q3) {
-                const double uo = own ? Ui[q3] : Uc[q3], un = own ? Uc[q3] : Ui[q3];
-                uf[q3] = wf * (uo - un) + un;
-            }
-            const double sf[3] = {a.eSf[e], a.eSf[nE + e], a.eSf[2 * nE + e]};
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j) g[3 * i + j] += sf[i] * uf[j];
-        }
-        const int32_t b0 = a.bPtr[r], b1 = a.bPtr[r + 1];
-        for (int32_t qb = b0; qb < b1; ++qb) {
-            const int32_t bf = a.bFace[qb];
-            const double sf[3] = {a.bSf[bf], a.bSf[NB + bf], a.bSf[2 * NB + bf]};
-            const double ub[3] = {a.Ub[bf], a.Ub[NB + bf], a.Ub[2 * NB + bf]};
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j) g[3 * i + j] += sf[i] * ub[j];
-        }
-        const double vol = a.V[r];
-#pragma unroll
-        for (int q = 0; q < 9; ++q) g[q] /= vol;
+        double g[9];
+        gauss_grad_u(a, r, g);
         double T[9];
 #pragma unroll
         for (int i = 0; i < 3; ++i)
@@ -1118,10 +1211,37 @@ __global__ void __launch_bounds__(BLK) k_tau_total(AssembleArgs a, const double*
     }
 }
 
+// fvc::grad(U) of every row -> G [N][9] (the cell values the stabilization
+// terms of divTau interpolate to the faces)
+__global__ void __launch_bounds__(BLK) k_grad_u(AssembleArgs a, double* __restrict__ G) {
+    for (int64_t it = 0;; ++it) {
+        const int64_t tl = tile_of(it, a.N);
+        if (tl < 0) break;
+        const int32_t r = (int32_t)(tl * BLK + threadIdx.x);
+        if (r >= a.N) continue;
+        double g[9];
+        gauss_grad_u(a, r, g);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) G[9 * (int64_t)r + q] = g[q];
+    }
+}
+
 void launch_div_tau(Ctx* ctx, const AssembleArgs& a, const double* tau, const double* tauB, double rho,
-                    double* out, int grid) {
+                    int stab, int lap, double gam, double* G, const double* C4, const double* bDc, double* out,
+                    int grid) {
+    if (stab != 0) {
+        ProfScope ps(ctx, "k_grad_u", (double)a.N * (32 + 8 + 72) + (double)a.nEll * (4 + 1 + 4 + 32));
+        hipLaunchKernelGGL(k_grad_u, dim3(grid), dim3(BLK), 0, ctx->stream, a, G);
+    }
     ProfScope ps(ctx, "k_div_tau", (double)a.N * (48 + 8 + 24) + (double)a.nEll * (4 + 1 + 4 + 32));
-    hipLaunchKernelGGL(k_div_tau, dim3(grid), dim3(BLK), 0, ctx->stream, a, tau, tauB, rho, out);
+#define RT_DIVTAU(S, L) \
+    hipLaunchKernelGGL((k_div_tau<S, L>), dim3(grid), dim3(BLK), 0, ctx->stream, a, tau, tauB, rho, gam, G, C4, bDc, out)
+    if (stab == 0) RT_DIVTAU(0, 0);
+    else if (stab == 2) RT_DIVTAU(2, 0);
+    else if (lap == 0) RT_DIVTAU(1, 0);
+    else if (lap == 1) RT_DIVTAU(1, 1);
+    else RT_DIVTAU(1, 2);
+#undef RT_DIVTAU
     RT_HIP(hipGetLastError());
 }
 

@@ -8,8 +8,10 @@
 // each kernel walks the sliced-ELL rows once and updates all active
 // components (stored interleaved, [N][6]) with per-component convergence
 // masks.  Every scalar of the recurrence (rho, alpha, omega, beta, residuals,
-// flags) stays on the device: the last workgroup of each reduction finalises
-// it, so no host round trip is needed inside an iteration.
+// flags) stays on the device: each row kernel writes per-workgroup partial
+// sums and a one-workgroup finalize kernel on the same stream reduces them in
+// a fixed order and updates the component records, so no host round trip is
+// needed inside an iteration.
 //
 // Preconditioning is right-Jacobi with the column-scaled matrix
 // Ahat = A D^-1 stored explicitly: A M^-1 p = p + offdiag(Ahat) p.
@@ -729,8 +731,27 @@ void launch_init(Ctx* ctx, SolveRun& R) {
     hipLaunchKernelGGL((k_fin_residual<T, 0>), dim3(1), dim3(BLK), 0, ctx->stream, argsOf<T>(R));
 }
 
+// The refinement sweep: mixed-precision fold, then the FP64 true residual and
+// its finalize (which converges, restarts or stops each waiting component).
 template <class T>
-void launch_round(Ctx* ctx, SolveRun& R, double nAct, double nRef, bool start) {
+void launch_refine(Ctx* ctx, SolveRun& R, double nRef) {
+    const bool mixed = std::is_same<T, float>::value;
+    Bytes B{R.N, R.ne, (double)sizeof(T)};
+    KryArgs<T>& a = argsOf<T>(R);
+    if (mixed) {
+        ProfScope ps(ctx, "k_fold<f32>", B.fold(nRef));
+        hipLaunchKernelGGL(k_fold<T>, dim3(R.pgrid), dim3(BLK), 0, ctx->stream, a);
+    }
+    {
+        ProfScope ps(ctx, mixed ? "k_residual<f32>" : "k_residual<f64>", B.res(nRef));
+        hipLaunchKernelGGL((k_residual<T, 1>), dim3(R.G), dim3(BLK), 0, ctx->stream, a);
+    }
+    hipLaunchKernelGGL((k_fin_residual<T, 1>), dim3(1), dim3(BLK), 0, ctx->stream, a);
+}
+
+// One lockstep PBiCGStab iteration of the nAct active components.
+template <class T>
+void launch_iter(Ctx* ctx, SolveRun& R, double nAct, bool start) {
     const bool mixed = std::is_same<T, float>::value;
     Bytes B{R.N, R.ne, (double)sizeof(T)};
     KryArgs<T>& a = argsOf<T>(R);
@@ -739,18 +760,6 @@ void launch_round(Ctx* ctx, SolveRun& R, double nAct, double nRef, bool start) {
         hipLaunchKernelGGL(k_scale<T>, dim3(kblk((int64_t)R.N)), dim3(BLK), 0, ctx->stream, a.N, a.ellOff,
                            a.ellWidth, a.slotMask, a.ellCol, a.aval, a.rD, const_cast<T*>(a.ahat));
         R.needScale = false;
-    }
-    if (nRef > 0) {
-        if (mixed) {
-            ProfScope ps(ctx, "k_fold<f32>", B.fold(nRef));
-            hipLaunchKernelGGL(k_fold<T>, dim3(R.pgrid), dim3(BLK), 0, ctx->stream, a);
-        }
-        {
-            ProfScope ps(ctx, mixed ? "k_residual<f32>" : "k_residual<f64>", B.res(nRef));
-            hipLaunchKernelGGL((k_residual<T, 1>), dim3(R.G), dim3(BLK), 0, ctx->stream, a);
-        }
-        hipLaunchKernelGGL((k_fin_residual<T, 1>), dim3(1), dim3(BLK), 0, ctx->stream, a);
-        nAct += nRef;  // components restarted by the refinement sweep join this round
     }
     {
         ProfScope ps(ctx, mixed ? "k_kry_p<f32>" : "k_kry_p<f64>", B.p(nAct, start));
@@ -856,20 +865,37 @@ bool solve_advance(Ctx* ctx, Krylov& K, bool block) {
     // components in it: components whose inner solve stopped wait (idle, no
     // state change) until every inner solve has stopped, then refine together.
     if (nAct > 0) nRef = 0;
-    if (R.mixed) launch_round<float>(ctx, R, nAct, nRef, start);
-    else launch_round<double>(ctx, R, nAct, nRef, start);
-    ++R.rounds;
-    // Non-blocking callers (the serving loop) also queue the next rounds
-    // before the state comes back: every kernel of a round reads the
-    // component records on the device and does nothing for finished
-    // components, and the refinement sweep is gated there too, so a round
-    // launched ahead does what the host would have launched.  This hides the
-    // host round trip, which dominates small solves.  Not while profiling:
-    // the byte accounting of a launch needs the host's view of the round.
-    if (!block && !ctx->profiling) {
-        for (int k = 1; k < kRoundsAhead; ++k) {
-            if (R.mixed) launch_round<float>(ctx, R, nAct, 1, false);
-            else launch_round<double>(ctx, R, nAct, 1, false);
+    if (block) {
+        // Blocking callers (rt_model_correct, rt_symm_solve, every profiled
+        // run) launch exactly what the host's view of the state asks for: a
+        // round is either one refinement sweep or one Krylov iteration of the
+        // components active now.  A sweep may converge every component, so the
+        // iteration after it is only launched once its state is back; no
+        // kernel is launched (or charged bytes) for a finished solve.
+        if (nAct > 0) {
+            if (R.mixed) launch_iter<float>(ctx, R, nAct, start);
+            else launch_iter<double>(ctx, R, nAct, start);
+        } else {
+            if (R.mixed) launch_refine<float>(ctx, R, nRef);
+            else launch_refine<double>(ctx, R, nRef);
+        }
+        ++R.rounds;
+    } else {
+        // Non-blocking callers (the serving loop) queue kRoundsAhead rounds of
+        // sweep + iteration before the state comes back: every kernel reads
+        // the component records on the device and does nothing for finished
+        // components, and the sweep is gated there on no inner solve running,
+        // so a round launched ahead does what the host would have launched.
+        // This hides the host round trip, which dominates small solves.
+        for (int k = 0; k < kRoundsAhead; ++k) {
+            const double nR = k == 0 ? nRef : 1, nA = k == 0 ? nAct + nRef : nAct;
+            if (R.mixed) {
+                if (nR > 0) launch_refine<float>(ctx, R, nR);
+                launch_iter<float>(ctx, R, nA, k == 0 && start && nRef == 0);
+            } else {
+                if (nR > 0) launch_refine<double>(ctx, R, nR);
+                launch_iter<double>(ctx, R, nA, k == 0 && start && nRef == 0);
+            }
             ++R.rounds;
         }
     }

@@ -324,6 +324,7 @@ void parseModel(Model& M, const Dict& d, const std::string& dn) {
         throw Error{RT_E_BAD_ARG, "keyword stabilization is undefined in dictionary " + dn};
     if (stab != "none" && stab != "BSD" && stab != "coupling")
         throw Error{RT_E_BAD_ARG, stab + " is not in enumeration: 3(none BSD coupling)"};
+    M.stab = stab == "BSD" ? 1 : stab == "coupling" ? 2 : 0;
     // thermo functions: only the isothermal default (Constant) is on this path
     for (const char* tf : {"thermoLambda", "thermoEta", "thermoLambdaR", "thermoLambdaD", "thermoLambdaS",
                            "thermoLambdaB"}) {
@@ -981,19 +982,27 @@ int rt_model_set_bc(rt_model* model, int patch, int bcType, const double* values
         const bool emptyPatch = m.patches[patch].kind == RT_PATCH_EMPTY;
         if (emptyPatch != (bcType == RT_BC_EMPTY))
             throw Error{RT_E_BAD_ARG, "rt_model_set_bc: empty patches take exactly the empty condition"};
+        if (bcType == RT_BC_FIXEDVALUE && !values)
+            throw Error{RT_E_BAD_ARG, "rt_model_set_bc: fixedValue needs values"};
         M.hBcType[patch] = bcType;
-        RT_HIP(hipMemcpy(M.bcType.as<int32_t>() + patch, &bcType, 4, hipMemcpyHostToDevice));
+        // copies on the model's stream, ordered with the work already queued
+        // there; the host arrays are read before the call returns
+        Ctx* c = M.ctx;
+        RT_HIP(hipMemcpyAsync(M.bcType.as<int32_t>() + patch, &M.hBcType[patch], 4, hipMemcpyHostToDevice,
+                              c->stream));
+        std::vector<double> cols;
         if (bcType == RT_BC_FIXEDVALUE) {
-            if (!values) throw Error{RT_E_BAD_ARG, "rt_model_set_bc: fixedValue needs values"};
             const rt_patch& p = m.patches[patch];
-            std::vector<double> col(p.size);
+            cols.resize((size_t)6 * p.size);
             const int nc = M.scalar() ? 1 : 6;  // the fluidity takes one value per face
             for (int q = 0; q < 6; ++q) {
+                double* col = cols.data() + (size_t)q * p.size;
                 for (int k = 0; k < p.size; ++k) col[k] = q < nc ? values[nc * (size_t)k + q] : 0.0;
-                RT_HIP(hipMemcpy(M.bcVal.as<double>() + (int64_t)q * m.NB + p.start, col.data(),
-                                 col.size() * 8, hipMemcpyHostToDevice));
+                RT_HIP(hipMemcpyAsync(M.bcVal.as<double>() + (int64_t)q * m.NB + p.start, col,
+                                      (size_t)p.size * 8, hipMemcpyHostToDevice, c->stream));
             }
         }
+        RT_HIP(hipStreamSynchronize(c->stream));
     });
 }
 
@@ -1161,7 +1170,10 @@ int rt_model_div_tau(rt_model* model, const double* tauB, double* out) {
         for (size_t k = 0; k < n; ++k) out[k] = 0.0;
         for (rt_model* mode : M.modes) {
             const int rc = rt_model_div_tau(mode, tauB, part.data());
-            if (rc != RT_OK) return rc;
+            if (rc != RT_OK) {
+                M.ctx->lastError = "mode " + mode->impl.name + ": " + mode->impl.ctx->lastError;
+                return rc;
+            }
             for (size_t k = 0; k < n; ++k) out[k] = out[k] + part[k];
         }
         return RT_OK;
@@ -1171,6 +1183,14 @@ int rt_model_div_tau(rt_model* model, const double* tauB, double* out) {
         Mesh& m = *M.mesh;
         const int32_t N = m.N;
         RT_HIP(hipSetDevice(c->device));
+        if (M.stab != 0) {
+            if (!M.hasFlow)
+                throw Error{RT_E_BAD_ARG, "rt_model_div_tau: the stabilization term needs U (rt_model_set_flow)"};
+            if (!m.C4.p || (m.NB > 0 && !m.bDc.p))
+                throw Error{RT_E_BAD_ARG, std::string("rt_model_div_tau: stabilization ") +
+                                              (M.stab == 1 ? "BSD" : "coupling") +
+                                              " needs the mesh's cell and boundary face centres (rt_mesh_desc C, Cb)"};
+        }
         DBuf tb;
         if (tauB && m.NB > 0) {
             std::vector<double> soa((size_t)6 * m.NB);
@@ -1180,7 +1200,10 @@ int rt_model_div_tau(rt_model* model, const double* tauB, double* out) {
             RT_HIP(hipMemcpyAsync(tb.p, soa.data(), soa.size() * 8, hipMemcpyHostToDevice, c->stream));
         }
         DBuf d = c->alloc((size_t)3 * N * 8);
-        launch_div_tau(c, meshArgs(M), M.tau.as<double>(), tb.p ? tb.as<double>() : nullptr, M.prm.rho,
+        DBuf G;
+        if (M.stab != 0) G = c->alloc((size_t)9 * N * 8);
+        launch_div_tau(c, meshArgs(M), M.tau.as<double>(), tb.p ? tb.as<double>() : nullptr, M.prm.rho, M.stab,
+                       M.lapScheme, M.prm.etaP / M.prm.rho, G.as<double>(), m.C4.as<double>(), m.bDc.as<double>(),
                        d.as<double>(), grid_for(c, N));
         DBuf& st = staging(c).buf;
         if (st.bytes < (size_t)3 * N * 8) st = c->alloc((size_t)3 * N * 8);
@@ -1188,6 +1211,52 @@ int rt_model_div_tau(rt_model* model, const double* tauB, double* out) {
         RT_HIP(hipMemcpyAsync(out, st.p, (size_t)3 * N * 8, hipMemcpyDeviceToHost, c->stream));
         RT_HIP(hipStreamSynchronize(c->stream));
     });
+}
+
+int rt_model_div_tau_coeff(const rt_model* model, double* nu) {
+    if (!model || !nu) return RT_E_BAD_ARG;
+    const Model& M = model->impl;
+    if (M.isMulti()) {  // the modes' matrices are summed (multiMode.C:141-151)
+        double s = 0;
+        for (const rt_model* mode : M.modes) {
+            double v = 0;
+            const int rc = rt_model_div_tau_coeff(mode, &v);
+            if (rc != RT_OK) return rc;
+            s = s + v;
+        }
+        *nu = s;
+        return RT_OK;
+    }
+    // fvm::laplacian(etaS/rho) (none) or ((etaP + etaS)/rho) (BSD, coupling)
+    *nu = M.stab == 0 ? M.prm.etaS / M.prm.rho : (M.prm.etaP + M.prm.etaS) / M.prm.rho;
+    return RT_OK;
+}
+
+int rt_model_set_laplacian_scheme(rt_model* model, const char* text) {
+    if (!model) return RT_E_BAD_ARG;
+    RT_FORWARD_TO_MODES(rt_model_set_laplacian_scheme(model, text));
+    Model& M = model->impl;
+    std::vector<std::string> tok;
+    {
+        std::string cur;
+        for (const char* c = text ? text : ""; *c; ++c) {
+            if (std::isspace((unsigned char)*c) || *c == ';') {
+                if (!cur.empty()) tok.push_back(cur);
+                cur.clear();
+            } else {
+                cur += *c;
+            }
+        }
+        if (!cur.empty()) tok.push_back(cur);
+    }
+    if (tok.size() != 3 || tok[0] != "Gauss" || tok[1] != "linear" ||
+        (tok[2] != "corrected" && tok[2] != "orthogonal" && tok[2] != "uncorrected")) {
+        M.ctx->lastError = std::string("rt_model_set_laplacian_scheme: expected \"Gauss linear "
+                                       "corrected|orthogonal|uncorrected\", got \"") + (text ? text : "") + "\"";
+        return RT_E_UNSUPPORTED;
+    }
+    M.lapScheme = tok[2] == "corrected" ? 0 : tok[2] == "orthogonal" ? 1 : 2;
+    return RT_OK;
 }
 
 int rt_model_tau_total(rt_model* model, double* out) {

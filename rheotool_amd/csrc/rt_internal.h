@@ -161,6 +161,8 @@ struct Mesh {
     DBuf bSf;       // double [3][NB] boundary face area vectors
     DBuf V;         // double [N] internal order
     DBuf C4;        // double [N][4] cell centres (x y z pad); empty when not given
+    DBuf bDc;       // double [NB] patch delta coefficients 1/|n (n & (Cf - C_P))|; empty
+                    // unless cell and boundary face centres are given
     DBuf permD;     // int32 [N] internal row -> external cell
     double meanAbsBandwidth = 0;
 };
@@ -262,6 +264,11 @@ struct Model {
     ModelParams prm;
     Controls ctl;
     double relax = 0;
+    // the momentum coupling of divTau (constitutiveEq.C:72-132): stabilization
+    // option (0 none, 1 BSD, 2 coupling) and the snGrad scheme of the
+    // laplacian(eta,U) entry (0 corrected, 1 orthogonal, 2 uncorrected)
+    int stab = 0;
+    int lapScheme = 0;
     // device state in internal numbering
     DBuf theta, tau;        // [N][6] interleaved
     DBuf eigVal;            // [3][N] conformation eigenvalues (diagonal of eigVals)
@@ -358,8 +365,13 @@ void launch_assemble(Ctx* ctx, const AssembleArgs& a, int grid);
 void launch_defc(Ctx* ctx, const AssembleArgs& a, int scheme, const double* C4, double* gradTh, double* souT,
                  int grid);
 // fvc::div(tau/rho) -> out [N][3]; tauB [6][NB] patch values or NULL (zeroGradient)
+// with stab (0 none, 1 BSD, 2 coupling) the stabilization term of divTau, lap
+// the snGrad scheme of its laplacian (0 corrected, 1 orthogonal, 2
+// uncorrected), gam = etaP/rho; G [N][9] receives grad(U), C4 cell centres,
+// bDc [NB] the patch delta coefficients
 void launch_div_tau(Ctx* ctx, const AssembleArgs& a, const double* tau, const double* tauB, double rho,
-                    double* out, int grid);
+                    int stab, int lap, double gam, double* G, const double* C4, const double* bDc, double* out,
+                    int grid);
 // tauTotal() -> out [N][6]
 void launch_tau_total(Ctx* ctx, const AssembleArgs& a, const double* tau, double etaS, double* out, int grid);
 void launch_eig_tau(Ctx* ctx, int32_t N, const ModelParams& prm, const double* theta,

@@ -324,6 +324,24 @@ void buildMesh(Ctx* ctx, const rt_mesh_desc* d, int renumber, Mesh& m) {
             for (int q = 0; q < 3; ++q) c4[4 * (size_t)r + q] = d->C[3 * (size_t)m.iperm[r] + q];
         upload(ctx, m.C4, c4);
     }
+    if (d->C && d->Cb && NB > 0) {
+        // [OF9] fvPatch::delta() = n (n & (Cf - C_P)) on non-coupled patches;
+        // the patch snGrad is deltaCoeffs (value - internal), deltaCoeffs =
+        // 1/|delta| (surfaceInterpolation::makeDeltaCoeffs)
+        std::vector<double> dcb(NB);
+        for (int32_t b = 0; b < NB; ++b) {
+            const double* sf = d->Sf + 3 * ((int64_t)F + b);
+            const int32_t c = d->owner[F + b];
+            const double ms = std::sqrt(sf[0] * sf[0] + sf[1] * sf[1] + sf[2] * sf[2]);
+            const double n[3] = {sf[0] / ms, sf[1] / ms, sf[2] / ms};
+            const double dd[3] = {d->Cb[3 * (size_t)b] - d->C[3 * (size_t)c], d->Cb[3 * (size_t)b + 1] - d->C[3 * (size_t)c + 1],
+                                  d->Cb[3 * (size_t)b + 2] - d->C[3 * (size_t)c + 2]};
+            const double nd = n[0] * dd[0] + n[1] * dd[1] + n[2] * dd[2];
+            const double dl[3] = {n[0] * nd, n[1] * nd, n[2] * nd};
+            dcb[b] = 1.0 / std::sqrt(dl[0] * dl[0] + dl[1] * dl[1] + dl[2] * dl[2]);
+        }
+        upload(ctx, m.bDc, dcb);
+    }
     upload(ctx, m.permD, m.iperm);
     RT_HIP(hipStreamSynchronize(ctx->stream));
 }

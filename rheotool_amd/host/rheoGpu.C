@@ -184,7 +184,7 @@ const rt_mesh* fvMesh::device() const {
         for (auto& b : boundary) p.push_back(rt_patch{b.start, b.size, b.kind});
         rt_mesh_desc d{nCells, nInternalFaces, nBoundaryFaces, (int32_t)p.size(), owner.data(),
                        neighbour.data(), Sf.data(), weights.data(), V.data(), p.data(),
-                       C.empty() ? nullptr : C.data()};
+                       C.empty() ? nullptr : C.data(), Cb.empty() ? nullptr : Cb.data()};
         gpuContext& c = gpuContext::get();
         c.check(rt_mesh_create(c.ptr(), &d, RT_RENUMBER_AUTO, &dev_), "fvMesh");
     }
@@ -411,10 +411,19 @@ volTensorField logConformationEq::eigVals() const { return getTensor(RT_EIGVALS,
 volTensorField logConformationEq::eigVecs() const { return getTensor(RT_EIGVECS, "eigVecs" + name()); }
 volVectorField logConformationEq::fvcDivTau(const std::vector<scalar>& tauBoundary) const {
     volVectorField f("div(tau" + name() + ")", *U().mesh);
+    // laplacianSchemes { laplacian(eta,U) Gauss linear corrected; } (BSD's explicit laplacian)
+    std::string ls;
+    if (topEntry(U().mesh->fvSchemes.subDict("laplacianSchemes").text(), "laplacian(eta,U)", ls))
+        gpuContext::get().check(rt_model_set_laplacian_scheme(model_, ls.c_str()), "fvSchemes laplacian(eta,U)");
     gpuContext::get().check(
         rt_model_div_tau(model_, tauBoundary.empty() ? nullptr : tauBoundary.data(), f.internal.data()),
         "divTau");
     return f;
+}
+scalar logConformationEq::divTauCoeff() const {
+    double nu = 0;
+    gpuContext::get().check(rt_model_div_tau_coeff(model_, &nu), "divTau coefficient");
+    return nu;
 }
 volSymmTensorField logConformationEq::tauTotal() const {
     volSymmTensorField f("tauTotal" + name(), *U().mesh);

@@ -88,6 +88,8 @@ public:
     std::vector<patchInfo> boundary;
     std::vector<scalar> C;  // cell centres [3*nCells]; set before first use when the
                             // div(phi,theta) scheme is a limited GaussDefCmpw one
+    std::vector<scalar> Cb; // boundary face centres [3*nBoundaryFaces]; set before first
+                            // use (with C) when divTau's stabilization is BSD or coupling
     Time time;
     dictionary fvSolution;
     dictionary fvSchemes;
@@ -209,10 +211,15 @@ public:
     volSymmTensorField theta() const;
     volTensorField eigVals() const;
     volTensorField eigVecs() const;
-    //- the explicit part of divTau(U) (constitutiveEq.C:88-92): fvc::div(tau()/rho()),
-    //  Gauss linear; tau's patch values from tauBoundary ([6*nBoundaryFaces]) or
-    //  zeroGradient when it is empty
+    //- the explicit part of divTau(U) (constitutiveEq.C:72-132) for the
+    //  stabilization option: fvc::div(tau()/rho()) (none), minus
+    //  fvc::laplacian(etaP/rho, U) with fvSchemes laplacian(eta,U) (BSD), or minus
+    //  fvc::div((etaP/rho) grad(U)) (coupling); tau's patch values from
+    //  tauBoundary ([6*nBoundaryFaces]) or zeroGradient when it is empty
     volVectorField fvcDivTau(const std::vector<scalar>& tauBoundary = std::vector<scalar>()) const;
+    //- nu of the implicit fvm::laplacian(nu, U) divTau(U) holds: etaS/rho (none),
+    //  (etaP + etaS)/rho (BSD, coupling)
+    scalar divTauCoeff() const;
     //- tauTotal() (constitutiveEq.C:534-547)
     volSymmTensorField tauTotal() const;
     const std::vector<solverPerformance>& solverPerf() const { return perf_; }

@@ -428,7 +428,7 @@ def test_div_tau_and_tau_total_match_oracle(ctx, with_patch_values):
     m = cases.cartesian(12, 10, 8, jitter=0.1, perm_seed=2)
     f = cases.make_fields(m, cases.FIELDS_TG, seed=4, lnLo=-3.0, lnHi=2.0)
     dm = ctx.mesh(m)
-    d = abi.model_dict("GiesekusLog", lam=2.0, alpha=0.1, rho=1.3, etaS=0.3, etaP=0.7)
+    d = abi.model_dict("GiesekusLog", lam=2.0, alpha=0.1, rho=1.3, etaS=0.3, etaP=0.7, stabilization="none")
     eq = ctx.constitutiveEq("", dm, d)
     eq.set_solver(abi.solver_dict(1e-10))
     eq.set_state(f.theta, f.eigVals, f.eigVecs)
@@ -444,6 +444,48 @@ def test_div_tau_and_tau_total_match_oracle(ctx, with_patch_values):
     tt = eq.tau_total()
     want_t = O.tau_total(m, f.U, f.Ub, tau, 0.3)
     assert rel_l2(tt, want_t) < 1e-13
+
+
+@pytest.mark.parametrize("stab,lap", [("none", "corrected"), ("BSD", "corrected"), ("BSD", "orthogonal"),
+                                      ("BSD", "uncorrected"), ("coupling", "corrected")])
+def test_div_tau_stabilization_matches_oracle(ctx, stab, lap):
+    """divTau's explicit part per stabilization option (constitutiveEq.C:
+    88-125): fvc::div(tau/rho) minus the BSD laplacian (snGrad scheme of
+    laplacian(eta,U)) or the coupling div((etaP/rho) grad U), on a permuted
+    jittered box (non-orthogonal faces) with tau patch values and a channel
+    velocity whose walls are fixedValue, against the oracle's face loops: the
+    same arithmetic in the same order, 1e-13.  The implicit coefficient is
+    etaS/rho for none and (etaP + etaS)/rho otherwise."""
+    m = cases.cartesian(12, 10, 8, jitter=0.1, perm_seed=2)
+    f = cases.make_fields(m, cases.FIELDS_TG, seed=4, lnLo=-3.0, lnHi=2.0)
+    dm = ctx.mesh(m)
+    d = abi.model_dict("Oldroyd-BLog", lam=2.0, rho=1.3, etaS=0.3, etaP=0.7, stabilization=stab)
+    eq = ctx.constitutiveEq("", dm, d)
+    eq.set_laplacian_scheme(f"Gauss linear {lap}")
+    rng = np.random.default_rng(6)
+    tau = rng.normal(size=(m.nCells, 6))
+    eq.set_state(f.theta, f.eigVals, f.eigVecs, tau)
+    eq.set_flow(f.U, f.Ub, f.phi)
+    tauB = rng.normal(size=(m.nBoundary, 6))
+    want = O.div_tau_stab(m, f.U, f.Ub, tau, tauB, 1.3, 0.7, stab, lap)
+    assert rel_l2(eq.div_tau(tauB), want) < 1e-13
+    assert eq.div_tau_coeff() == (0.3 / 1.3 if stab == "none" else (0.7 + 0.3) / 1.3)
+    if stab != "none":
+        assert rel_l2(want, O.div_tau(m, tau, tauB, 1.3)) > 1e-3  # the term is there
+
+
+def test_div_tau_stabilization_needs_face_centres(ctx):
+    m = cases.cartesian(6, 5, 4)
+    f = cases.make_fields(m, cases.FIELDS_TG, seed=4, lnLo=-3.0, lnHi=2.0)
+    m.C = None  # a mesh uploaded without centres
+    dm = ctx.mesh(m)
+    eq = ctx.constitutiveEq("", dm, abi.model_dict("Oldroyd-BLog", stabilization="BSD"))
+    eq.set_state(f.theta, f.eigVals, f.eigVecs)
+    eq.set_flow(f.U, f.Ub, f.phi)
+    with pytest.raises(abi.RheoError, match="centres"):
+        eq.div_tau()
+    with pytest.raises(abi.RheoError):
+        eq.set_laplacian_scheme("Gauss linear limited 0.5")
 
 
 # the further log models (FENE_CRLog.C, WhiteMetznerCYLog.C, RoliePolyLog.C,
@@ -722,8 +764,11 @@ def test_multimode_matches_per_mode_oracle(ctx):
         assert rel_l2(md.tau(), tau) < THETA_TOL
         taus.append(md.tau())
     assert rel_l2(eq.tau(), taus[0] + taus[1]) < 1e-15
-    want = O.div_tau(m, taus[0], None, 1.1) + O.div_tau(m, taus[1], None, 1.1)
+    # each mode's divTau carries its own coupling term (multiMode.C:141-151)
+    want = sum(O.div_tau_stab(m, f1.U, f1.Ub, t, None, 1.1, kw["etaP"], "coupling")
+               for t, (n, mdl, kw) in zip(taus, spec))
     assert rel_l2(eq.div_tau(), want) < 1e-13
+    assert abs(eq.div_tau_coeff() - sum((kw["etaS"] + kw["etaP"]) / 1.1 for n, mdl, kw in spec)) < 1e-15
     want_t = O.tau_total(m, f1.U, f1.Ub, taus[0] + taus[1], 0.1 + 0.05)
     assert rel_l2(eq.tau_total(), want_t) < 1e-13
     perf = eq.solver_perf()
