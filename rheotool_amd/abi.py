@@ -466,8 +466,8 @@ def model_dict(model="Oldroyd-BLog", etaS=0.59, etaP=0.41, lam=1.0, alpha=None, 
 
 
 def solver_dict(tol=1e-10, precond="DILU", precision="double", maxIter=1000, relTol=0.0,
-                innerRelTol=None):
-    s = (f"solver PBiCGStab; preconditioner {precond}; tolerance {tol!r}; relTol {relTol!r}; "
+                innerRelTol=None, solver="PBiCGStab"):
+    s = (f"solver {solver}; preconditioner {precond}; tolerance {tol!r}; relTol {relTol!r}; "
          f"maxIter {maxIter}; minIter 0; precision {precision};")
     if innerRelTol is not None:
         s += f" innerRelTol {innerRelTol!r};"

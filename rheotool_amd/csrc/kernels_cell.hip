@@ -323,6 +323,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3, 8))
             const int slot = a.ellSlot[e];
             if (fid == RT_FID_PAD) {
                 if (!staged) a.aval[base + (int64_t)SLICE * slot] = 0.0;
+                if (a.atval) a.atval[base + (int64_t)SLICE * slot] = 0.0;
                 continue;
             }
             const bool own = fid >= 0;
@@ -334,6 +335,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3, 8))
             const double lower = -upw * ph;
             const double upper = (1.0 - upw) * ph;
             const double val = own ? upper : lower;
+            if (a.atval) a.atval[ec] = own ? lower : upper;  // A^T: the face's other coefficient
             const int32_t c = a.ellCol[ec];
             if (staged) {
 #pragma unroll

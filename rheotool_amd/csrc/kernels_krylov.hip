@@ -39,6 +39,16 @@ struct KryArgs {
     double* x;           // [N][6]
     T *r, *r0, *p, *v, *t;  // [N][6]
     T* e;                // [N][6] mixed: the inner solve's correction to x (NULL in FP64)
+    // preconditioner M (right preconditioning of PBiCGStab, M^-1 of PBiCG):
+    // rM = M^-1 as a diagonal (1/diag for Jacobi, NULL for none: 1), and
+    // dg = diag(A M^-1) (NULL: 1, Jacobi) -- D itself for none
+    const double* rM;
+    const double* dg;
+    // PBiCG: wT = s, the off-diagonals of A and A^T in T, FP64 A^T for rT
+    T* s;
+    const T* aA;
+    const T* aT;
+    const double* atval;
     double* partials;
     CompState* cs;
     int32_t valid[6];
@@ -69,7 +79,7 @@ __device__ __forceinline__ bool inner_running(const KryArgs<T>& a) {
 
 template <class T, int MODE>
 __global__ void __launch_bounds__(BLK) k_residual(KryArgs<T> a) {
-    constexpr int NQ = MODE == 0 ? 18 : 12;
+    constexpr int NQ = 18;
     bool on[6];
     bool any = false;
 #pragma unroll
@@ -128,6 +138,10 @@ __global__ void __launch_bounds__(BLK) k_residual(KryArgs<T> a) {
             if (MODE == 0) {
                 const double pA = sumA * xbar[c];
                 q[12 + c] += fabs(y[c] - pA) + fabs(bb[c] - pA);
+            } else {
+                // (M^-1 r, r): the rho of a PBiCG restart with rT = r
+                const double rm = a.rM ? a.rM[r] : 1.0;
+                q[12 + c] += (rm * (double)rt[c]) * (double)rt[c];
             }
         }
         st6(a.r + (int64_t)NCMP * r, rt);
@@ -138,7 +152,7 @@ __global__ void __launch_bounds__(BLK) k_residual(KryArgs<T> a) {
 
 template <class T, int MODE>
 __global__ void __launch_bounds__(BLK) k_fin_residual(KryArgs<T> a) {
-    constexpr int NQ = MODE == 0 ? 18 : 12;
+    constexpr int NQ = 18;
     bool on[6];
     bool any = false;
 #pragma unroll
@@ -165,7 +179,7 @@ __global__ void __launch_bounds__(BLK) k_fin_residual(KryArgs<T> a) {
                 } else if (cs.nRefine >= a.ctl.maxRefine || cs.iter >= a.ctl.maxIter) {
                     cs.active = 0;
                 } else {
-                    start_inner(cs, a.ctl, sums[6 + c]);
+                    start_inner(cs, a.ctl, a.ctl.bicg ? sums[12 + c] : sums[6 + c]);
                 }
             }
         }
@@ -232,6 +246,11 @@ __global__ void __launch_bounds__(BLK) k_kry_v(KryArgs<T> a) {
         if (r >= a.N) continue;
         T acc[6];
         ld6(a.p + (int64_t)NCMP * r, acc);
+        if (a.dg) {
+            const T dgr = (T)a.dg[r];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) acc[c] = dgr * acc[c];
+        }
         const int s = r >> 6, lane = r & 63;
         const int64_t base = a.ellOff[s] + lane;
         const int width = a.ellWidth[s];
@@ -302,10 +321,11 @@ __global__ void __launch_bounds__(BLK) k_kry_t(KryArgs<T> a) {
         T sv[6], acc[6], rr[6], vv[6];
         ld6(a.r + (int64_t)NCMP * r, rr);
         ld6(a.v + (int64_t)NCMP * r, vv);
+        const T dgr = a.dg ? (T)a.dg[r] : (T)1;
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
             sv[c] = rr[c] - alpha[c] * vv[c];
-            acc[c] = sv[c];
+            acc[c] = a.dg ? dgr * sv[c] : sv[c];
         }
         const int s = r >> 6, lane = r & 63;
         const int64_t base = a.ellOff[s] + lane;
@@ -395,7 +415,7 @@ __global__ void __launch_bounds__(BLK) k_kry_xr(KryArgs<T> a) {
         const int64_t t0 = tl * BLK;
         const int32_t r = (int32_t)(t0 + threadIdx.x);
         if (r >= a.N) continue;
-        const double rd = a.rD[r];
+        const double rd = a.rM ? a.rM[r] : 1.0;
         const int64_t j = (int64_t)NCMP * r;
         double xv[6];
         T ev[6];
@@ -529,6 +549,320 @@ __global__ void __launch_bounds__(BLK) k_fin_xr(KryArgs<T> a) {
     }
 }
 
+// ================================================= PBiCG ([OF9] PBiCG::solve)
+// The reference's tutorials solve theta with PBiCG (bi-conjugate gradients,
+// e.g. tutorials/rheoFoam/Channel/Oldroyd-BLog/system/fvSolution:32): per
+// iteration wA = M^-1 rA, wT = M^-T rT, wArT = (wA, rT), pA = wA + beta pA,
+// pT = wT + beta pT, wA = A pA, wT = A^T pT, alpha = wArT / (wA, pT),
+// psi += alpha pA, rA -= alpha wA, rT -= alpha wT.  Lockstep over the
+// components like the PBiCGStab above, with M diagonal (rM).  Vectors:
+// rA = r, rT = r0, pA = p, pT = v, wA = t, wT = s.
+
+// rT = b - A^T x (FP64) and the first wArT = (M^-1 rA, rT), after the
+// initial residual has written rA
+template <class T>
+__global__ void __launch_bounds__(BLK) k_bicg_init(KryArgs<T> a) {
+    bool on[6];
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        on[c] = a.cs[c].active;
+        any |= on[c];
+    }
+    if (!any) return;
+    double q[6] = {0, 0, 0, 0, 0, 0};
+    for (int64_t it = 0;; ++it) {
+        const int64_t tl = tile_of(it, a.N);
+        if (tl < 0) break;
+        const int32_t r = (int32_t)(tl * BLK + threadIdx.x);
+        if (r >= a.N) continue;
+        const double d = a.diag[r];
+        double xv[6], y[6];
+        ld6(a.x + (int64_t)NCMP * r, xv);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) y[c] = d * xv[c];
+        const int s = r >> 6, lane = r & 63;
+        const int64_t base = a.ellOff[s] + lane;
+        const int width = a.ellWidth[s];
+        for (int k = 0; k < width; ++k) {
+            const int64_t e = base + (int64_t)SLICE * k;
+            const double av = a.atval[e];
+            if (av == 0.0) continue;
+            const int32_t col = a.ellCol[e];
+            double xc[6];
+            ld6(a.x + (int64_t)NCMP * col, xc);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) y[c] += av * xc[c];
+        }
+        double bb[6];
+        ld6(a.b + (int64_t)NCMP * r, bb);
+        T rA[6], rT[6];
+        ld6(a.r + (int64_t)NCMP * r, rA);
+        const double rm = a.rM ? a.rM[r] : 1.0;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            rT[c] = (T)(bb[c] - y[c]);
+            if (on[c]) q[c] += (rm * (double)rA[c]) * (double)rT[c];
+        }
+        st6(a.r0 + (int64_t)NCMP * r, rT);
+    }
+    write_partials<6>(q, a.partials);
+}
+
+template <class T>
+__global__ void __launch_bounds__(BLK) k_fin_bicg_init(KryArgs<T> a) {
+    bool on[6];
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        on[c] = a.cs[c].active;
+        any |= on[c];
+    }
+    if (!any) return;
+    double sums[6];
+    sum_partials<6>(a.partials, a.G, sums);
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 6; ++c)
+            if (on[c]) a.cs[c].rho = sums[c];
+}
+
+// pA = M^-1 rA (+ beta pA), pT = M^-1 rT (+ beta pT)
+template <class T>
+__global__ void __launch_bounds__(BLK) k_bicg_p(KryArgs<T> a) {
+    bool on[6], first[6];
+    T beta[6];
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        on[c] = a.cs[c].active;
+        first[c] = a.cs[c].innerIter == 0;
+        beta[c] = (T)a.cs[c].beta;
+        any |= on[c];
+    }
+    if (!any) return;
+    for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < a.N; i += (int64_t)gridDim.x * BLK) {
+        const T rm = a.rM ? (T)a.rM[i] : (T)1;
+        T rA[6], rT[6], pA[6], pT[6];
+        ld6(a.r + NCMP * i, rA);
+        ld6(a.r0 + NCMP * i, rT);
+        ld6(a.p + NCMP * i, pA);
+        ld6(a.v + NCMP * i, pT);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            if (!on[c]) continue;
+            const T wA = rm * rA[c], wT = rm * rT[c];
+            pA[c] = first[c] ? wA : wA + beta[c] * pA[c];
+            pT[c] = first[c] ? wT : wT + beta[c] * pT[c];
+        }
+        st6(a.p + NCMP * i, pA);
+        st6(a.v + NCMP * i, pT);
+    }
+}
+
+// wA = A pA, wT = A^T pT; (wA, pT)
+template <class T>
+__global__ void __launch_bounds__(BLK) k_bicg_w(KryArgs<T> a) {
+    bool on[6];
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        on[c] = a.cs[c].active;
+        any |= on[c];
+    }
+    if (!any) return;
+    T q[6] = {0, 0, 0, 0, 0, 0};
+    for (int64_t it = 0;; ++it) {
+        const int64_t tl = tile_of(it, a.N);
+        if (tl < 0) break;
+        const int32_t r = (int32_t)(tl * BLK + threadIdx.x);
+        if (r >= a.N) continue;
+        const T d = (T)a.diag[r];
+        T wA[6], wT[6], pT[6];
+        ld6(a.p + (int64_t)NCMP * r, wA);
+        ld6(a.v + (int64_t)NCMP * r, pT);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            wA[c] = d * wA[c];
+            wT[c] = d * pT[c];
+        }
+        const int s = r >> 6, lane = r & 63;
+        const int64_t base = a.ellOff[s] + lane;
+        const int width = a.ellWidth[s];
+        const uint32_t live = a.slotMask ? a.slotMask[s] : 0xffffffffu;
+        for (int k = 0; k < width; ++k) {
+            const int64_t e = base + (int64_t)SLICE * k;
+            const T av = (k < 32 && !((live >> k) & 1u)) ? (T)0 : ld_stream<T>(a.aA + e);
+            const T at = ld_stream<T>(a.aT + e);
+            if (av == (T)0 && at == (T)0) continue;
+            const int32_t col = ld_stream<T>(a.ellCol + e);
+            if (av != (T)0) {
+                T pc[6];
+                ld6(a.p + (int64_t)NCMP * col, pc);
+#pragma unroll
+                for (int c = 0; c < 6; ++c) wA[c] += av * pc[c];
+            }
+            if (at != (T)0) {
+                T pc[6];
+                ld6(a.v + (int64_t)NCMP * col, pc);
+#pragma unroll
+                for (int c = 0; c < 6; ++c) wT[c] += at * pc[c];
+            }
+        }
+        st6(a.t + (int64_t)NCMP * r, wA);
+        st6(a.s + (int64_t)NCMP * r, wT);
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+            if (on[c]) q[c] += wA[c] * pT[c];
+    }
+    double qd[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) qd[c] = q[c];
+    write_partials<6>(qd, a.partials);
+}
+
+template <class T>
+__global__ void __launch_bounds__(BLK) k_fin_bicg_w(KryArgs<T> a) {
+    bool on[6];
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        on[c] = a.cs[c].active;
+        any |= on[c];
+    }
+    if (!any) return;
+    double sums[6];
+    sum_partials<6>(a.partials, a.G, sums);
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 6; ++c) {
+            if (!on[c]) continue;
+            CompState& cs = a.cs[c];
+            // checkSingularity(mag(wApT)/normFactor): stop; restarted from the
+            // true residual as after a PBiCGStab breakdown
+            if (fabs(sums[c]) / cs.normFactor < 1e-300) {
+                cs.active = 0;
+                cs.needRefine = 1;
+            } else {
+                cs.alpha = cs.rho / sums[c];
+            }
+        }
+}
+
+// psi += alpha pA, rA -= alpha wA, rT -= alpha wT; |rA|, (M^-1 rA, rT)
+template <class T>
+__global__ void __launch_bounds__(BLK) k_bicg_xr(KryArgs<T> a) {
+    constexpr bool MIX = std::is_same<T, float>::value;
+    bool on[6], first[6];
+    double alpha[6];
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        on[c] = a.cs[c].active;
+        first[c] = a.cs[c].innerIter == 0;
+        alpha[c] = a.cs[c].alpha;
+        any |= on[c];
+    }
+    if (!any) return;
+    T q[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) q[i] = 0;
+    for (int64_t it = 0;; ++it) {
+        const int64_t tl = tile_of(it, a.N);
+        if (tl < 0) break;
+        const int32_t r = (int32_t)(tl * BLK + threadIdx.x);
+        if (r >= a.N) continue;
+        const int64_t j = (int64_t)NCMP * r;
+        const T rm = a.rM ? (T)a.rM[r] : (T)1;
+        double xv[6];
+        T ev[6], pA[6], wA[6], wT[6], rA[6], rT[6];
+        if constexpr (MIX) {
+            ld6(a.e + j, ev);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) xv[c] = first[c] ? 0.0 : (double)ev[c];
+        } else {
+            ld6(a.x + j, xv);
+        }
+        ld6(a.p + j, pA);
+        ld6(a.t + j, wA);
+        ld6(a.s + j, wT);
+        ld6(a.r + j, rA);
+        ld6(a.r0 + j, rT);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            if (!on[c]) continue;
+            xv[c] += alpha[c] * (double)pA[c];
+            rA[c] = rA[c] - (T)alpha[c] * wA[c];
+            rT[c] = rT[c] - (T)alpha[c] * wT[c];
+            q[c] += fabs(rA[c]);
+            q[6 + c] += (rm * rA[c]) * rT[c];
+        }
+        if constexpr (MIX) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) ev[c] = (T)xv[c];
+            st6(a.e + j, ev);
+        } else {
+            st6(a.x + j, xv);
+        }
+        st6(a.r + j, rA);
+        st6(a.r0 + j, rT);
+    }
+    double qd[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) qd[i] = q[i];
+    write_partials<12>(qd, a.partials);
+}
+
+template <class T>
+__global__ void __launch_bounds__(BLK) k_fin_bicg_xr(KryArgs<T> a) {
+    bool on[6];
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        on[c] = a.cs[c].active;
+        any |= on[c];
+    }
+    if (!any) return;
+    double sums[12];
+    sum_partials<12>(a.partials, a.G, sums);
+    if (threadIdx.x == 0) {
+        for (int c = 0; c < 6; ++c) {
+            if (!on[c]) continue;
+            CompState& cs = a.cs[c];
+            cs.iter += 1;
+            cs.innerIter += 1;
+            const double res = sums[c] / cs.normFactor;
+            if (!a.ctl.mixed) cs.finalRes = res;
+            const bool conv = conv_inner(cs, a.ctl, res);
+            bool cont = (!conv && cs.iter < a.ctl.maxIter) || (!a.ctl.mixed && cs.iter < a.ctl.minIter);
+            if (a.ctl.mixed && cs.innerIter >= a.ctl.maxInner) cont = false;
+            bool breakdown = false;
+            if (cont) {
+                cs.rhoOld = cs.rho;
+                cs.rho = sums[6 + c];
+                if (fabs(cs.rho) < 1e-300) {
+                    cont = false;
+                    breakdown = true;
+                } else {
+                    cs.beta = cs.rho / cs.rhoOld;
+                }
+            }
+            if (!cont) {
+                cs.active = 0;
+                if (a.ctl.mixed || breakdown) cs.needRefine = 1;
+                else cs.converged = conv_outer(cs, a.ctl, cs.finalRes);
+            }
+        }
+    }
+}
+
+// float copies of the off-diagonals of A and A^T (mixed PBiCG)
+__global__ void k_to_f32(int64_t n, const double* __restrict__ a, const double* __restrict__ b,
+                         float* __restrict__ fa, float* __restrict__ fb) {
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    fa[i] = (float)a[i];
+    fb[i] = (float)b[i];
+}
+
 // ---- Ahat = offdiag(A) D^-1 (column scaled), rD = 1/diag ---------------------
 __global__ void k_rdiag(int32_t N, const double* __restrict__ diag, double* __restrict__ rD) {
     const int64_t r = (int64_t)blockIdx.x * BLK + threadIdx.x;
@@ -547,7 +881,7 @@ __global__ void k_scale(int32_t N, const int64_t* __restrict__ off, const int32_
     for (int k = 0; k < w; ++k) {
         if (k < 32 && !((live >> k) & 1u)) continue;
         const int64_t e = base + (int64_t)SLICE * k;
-        ahat[e] = (T)(aval[e] * rD[col[e]]);
+        ahat[e] = rD ? (T)(aval[e] * rD[col[e]]) : (T)aval[e];
     }
 }
 
@@ -598,10 +932,12 @@ KryCtl kryCtl(const Controls& C, bool mixed) {
     k.maxInner = 200;
     k.maxRefine = C.maxRefine;
     k.mixed = mixed ? 1 : 0;
+    k.bicg = C.solver == SV_PBICG ? 1 : 0;
+    k.precond = C.precond;
     return k;
 }
 
-void Krylov::ensure(Mesh& m, bool mixed) {
+void Krylov::ensure(Mesh& m, bool mixed, bool bicg) {
     if (!ctx) ctx = m.ctx;
     const size_t vec = (size_t)6 * m.N;
     if (N != m.N) {
@@ -625,6 +961,15 @@ void Krylov::ensure(Mesh& m, bool mixed) {
         vs = ctx->alloc(vec * 4); ts = ctx->alloc(vec * 4); es = ctx->alloc(vec * 4);
         ahat32 = ctx->alloc(std::max<int64_t>(1, m.nEll) * 4);
     }
+    if (bicg) {
+        if (!atval.p) atval = ctx->alloc(std::max<int64_t>(1, m.nEll) * 8);
+        if (!mixed && !s64.p) s64 = ctx->alloc(vec * 8);
+        if (mixed && !s32.p) {
+            s32 = ctx->alloc(vec * 4);
+            a32 = ctx->alloc(std::max<int64_t>(1, m.nEll) * 4);
+            at32 = ctx->alloc(std::max<int64_t>(1, m.nEll) * 4);
+        }
+    }
 }
 
 // Resumable solve: begin() launches the initial residual (unless the assembly
@@ -643,7 +988,9 @@ struct SolveRun {
     SolveState* hostState = nullptr;  // pinned block of ctx
     CompState* host = nullptr;        // = hostState->c
     bool neFromDevice = false;        // matrix entry count comes with the first status pull
-    bool needScale = false;           // Ahat not built yet
+    bool needScale = false;           // Ahat (or the float copies of A, A^T) not built yet
+    bool bicg = false;                // PBiCG instead of PBiCGStab
+    int64_t nEll = 0;                 // all ELL entries (the float copies of A, A^T)
     hipEvent_t ev = nullptr;
     bool pending = false, done = false;
     int rounds = 0;
@@ -704,6 +1051,18 @@ void fill_args(Ctx* ctx, const SolveArgs& sa, SolveRun& R) {
     for (int c = 0; c < 6; ++c) a.valid[c] = sa.valid[c];
     a.ctl = kryCtl(*sa.ctl, mixed);
     a.G = K.grid;
+    // the preconditioner: Jacobi M = D (M^-1 = rD, A M^-1 has a unit diagonal)
+    // or none (M^-1 = 1, diag(A M^-1) = D)
+    const bool jac = sa.ctl->precond == PC_JACOBI;
+    a.rM = jac ? rD : nullptr;
+    a.dg = jac ? nullptr : sa.diag;
+    R.bicg = sa.ctl->solver == SV_PBICG;
+    if (R.bicg) {
+        a.atval = sa.atval;
+        a.s = mixed ? (T*)K.s32.p : (T*)K.s64.p;
+        a.aA = mixed ? (const T*)K.a32.p : (const T*)sa.aval;
+        a.aT = mixed ? (const T*)K.at32.p : (const T*)sa.atval;
+    }
 }
 
 // algorithmic bytes per launch for nc participating components: ELL entries
@@ -718,6 +1077,11 @@ struct Bytes {
     // FP64: x read and written in place; mixed: the FP32 correction instead
     double xr(double nc) const { return N * 8.0 + nc * N * (w == 4 ? 8 * w : 16 + 6 * w); }
     double fold(double nc) const { return nc * N * (8 + 8 + w); }
+    // PBiCG: rT = b - A^T x; the p pass; A pA and A^T pT; the x/r pass
+    double binit(double nc) const { return N * 16.0 + ne * 12 + nc * N * (8 + 8 + 2 * w); }
+    double bp(double nc) const { return N * 8.0 + nc * N * 6 * w; }
+    double bw(double nc) const { return N * 8.0 + 2 * ne * (4 + w) + nc * N * 4 * w; }
+    double bxr(double nc) const { return N * 8.0 + nc * N * (w == 4 ? 9 * w : 16 + 7 * w); }
 };
 
 template <class T>
@@ -729,6 +1093,18 @@ void launch_init(Ctx* ctx, SolveRun& R) {
         hipLaunchKernelGGL((k_residual<T, 0>), dim3(R.G), dim3(BLK), 0, ctx->stream, argsOf<T>(R));
     }
     hipLaunchKernelGGL((k_fin_residual<T, 0>), dim3(1), dim3(BLK), 0, ctx->stream, argsOf<T>(R));
+}
+
+// PBiCG: rT = b - A^T x and the first wArT, once the initial residual is in
+template <class T>
+void launch_bicg_init(Ctx* ctx, SolveRun& R) {
+    const bool mixed = std::is_same<T, float>::value;
+    Bytes B{R.N, R.ne, (double)sizeof(T)};
+    {
+        ProfScope ps(ctx, mixed ? "k_bicg_init<f32>" : "k_bicg_init<f64>", B.binit(R.nValid));
+        hipLaunchKernelGGL(k_bicg_init<T>, dim3(R.G), dim3(BLK), 0, ctx->stream, argsOf<T>(R));
+    }
+    hipLaunchKernelGGL(k_fin_bicg_init<T>, dim3(1), dim3(BLK), 0, ctx->stream, argsOf<T>(R));
 }
 
 // The refinement sweep: mixed-precision fold, then the FP64 true residual and
@@ -755,10 +1131,34 @@ void launch_iter(Ctx* ctx, SolveRun& R, double nAct, bool start) {
     const bool mixed = std::is_same<T, float>::value;
     Bytes B{R.N, R.ne, (double)sizeof(T)};
     KryArgs<T>& a = argsOf<T>(R);
+    if (R.bicg) {
+        if (R.needScale && mixed) {
+            const int64_t n = std::max<int64_t>(1, R.nEll);
+            ProfScope ps(ctx, "k_to_f32", (double)n * 24);
+            hipLaunchKernelGGL(k_to_f32, dim3(kblk(n)), dim3(BLK), 0, ctx->stream, n, a.aval, a.atval,
+                               (float*)const_cast<T*>(a.aA), (float*)const_cast<T*>(a.aT));
+        }
+        R.needScale = false;
+        {
+            ProfScope ps(ctx, mixed ? "k_bicg_p<f32>" : "k_bicg_p<f64>", B.bp(nAct));
+            hipLaunchKernelGGL(k_bicg_p<T>, dim3(R.pgrid), dim3(BLK), 0, ctx->stream, a);
+        }
+        {
+            ProfScope ps(ctx, mixed ? "k_bicg_w<f32>" : "k_bicg_w<f64>", B.bw(nAct));
+            hipLaunchKernelGGL(k_bicg_w<T>, dim3(R.G), dim3(BLK), 0, ctx->stream, a);
+        }
+        hipLaunchKernelGGL(k_fin_bicg_w<T>, dim3(1), dim3(BLK), 0, ctx->stream, a);
+        {
+            ProfScope ps(ctx, mixed ? "k_bicg_xr<f32>" : "k_bicg_xr<f64>", B.bxr(nAct));
+            hipLaunchKernelGGL(k_bicg_xr<T>, dim3(R.G), dim3(BLK), 0, ctx->stream, a);
+        }
+        hipLaunchKernelGGL(k_fin_bicg_xr<T>, dim3(1), dim3(BLK), 0, ctx->stream, a);
+        return;
+    }
     if (R.needScale) {
         ProfScope ps(ctx, mixed ? "k_scale<f32>" : "k_scale<f64>", R.ne * (4 + 8 + sizeof(T)) + R.N * 8);
         hipLaunchKernelGGL(k_scale<T>, dim3(kblk((int64_t)R.N)), dim3(BLK), 0, ctx->stream, a.N, a.ellOff,
-                           a.ellWidth, a.slotMask, a.ellCol, a.aval, a.rD, const_cast<T*>(a.ahat));
+                           a.ellWidth, a.slotMask, a.ellCol, a.aval, a.rM, const_cast<T*>(a.ahat));
         R.needScale = false;
     }
     {
@@ -798,7 +1198,9 @@ void solve_begin(Ctx* ctx, const SolveArgs& sa) {
     Mesh& m = *sa.mesh;
     Krylov& K = *sa.kry;
     const bool mixed = sa.ctl->mixed;
-    K.ensure(m, mixed);
+    const bool bicg = sa.ctl->solver == SV_PBICG;
+    K.ensure(m, mixed, bicg);
+    if (bicg && !sa.atval) throw Error{RT_E_BAD_ARG, "PBiCG needs the transposed matrix"};
     if (!K.run) {
         static_assert(sizeof(SolveState) <= Ctx::kPinnedBlock, "status block too small");
         K.run.reset(new SolveRun);
@@ -813,6 +1215,7 @@ void solve_begin(Ctx* ctx, const SolveArgs& sa) {
     R.pgrid = std::min<int>(K.grid, (int)kblk(m.N));
     R.N = m.N;
     R.ne = (double)m.nEll;
+    R.nEll = m.nEll;
     R.neFromDevice = sa.slotMask != nullptr;
     R.nValid = 0;
     for (int c = 0; c < 6; ++c) {
@@ -824,9 +1227,11 @@ void solve_begin(Ctx* ctx, const SolveArgs& sa) {
     if (mixed) {
         fill_args<float>(ctx, sa, R);
         if (!sa.initDone) launch_init<float>(ctx, R);
+        if (bicg) launch_bicg_init<float>(ctx, R);
     } else {
         fill_args<double>(ctx, sa, R);
         if (!sa.initDone) launch_init<double>(ctx, R);
+        if (bicg) launch_bicg_init<double>(ctx, R);
     }
     RT_HIP(hipGetLastError());
     queue_pull(ctx, K, R);

@@ -180,6 +180,7 @@ Controls parseControls(const char* text) {
             throw Error{RT_E_UNKNOWN_TYPE, "Unknown asymmetric matrix solver " + s +
                                                "\n\nValid asymmetric matrix solvers are :\n2\n(\nPBiCG\nPBiCGStab\n)"};
         c.solverName = s;
+        c.solver = s == "PBiCG" ? SV_PBICG : SV_PBICGSTAB;
     }
     std::string pc = "none";
     if (e->subDicts.count("preconditioner")) {
@@ -188,14 +189,25 @@ Controls parseControls(const char* text) {
         e->word("preconditioner", pc);
     }
     c.precondName = pc;
-    if (pc == "DILU" || pc == "multicolourILU0" || pc == "diagonal" || pc == "Jacobi" ||
-        pc == "DIC") {
+    // DILU's sweep is sequential; the device replaces it by the diagonal
+    // (Jacobi) preconditioner, so iterates differ from the reference's and only
+    // converged fields are compared.  diagonal/Jacobi is the same
+    // preconditioner, none is none.  DIC is a symmetric-matrix preconditioner
+    // and the theta matrix is asymmetric ([OF9] rejects it there); the
+    // multicolour ILU0 is not built.  Both are refused rather than replaced.
+    if (pc == "DILU" || pc == "diagonal" || pc == "Jacobi") {
         c.precond = PC_JACOBI;
     } else if (pc == "none") {
         c.precond = PC_NONE;
+    } else if (pc == "DIC" || pc == "FDIC" || pc == "GAMG") {
+        throw Error{RT_E_UNKNOWN_TYPE, "Unknown asymmetric matrix preconditioner " + pc +
+                                           "\n\nValid asymmetric matrix preconditioners are :\n4\n(\nDILU\nJacobi\ndiagonal\nnone\n)"};
+    } else if (pc == "multicolourILU0") {
+        throw Error{RT_E_UNSUPPORTED, "preconditioner multicolourILU0 is not available on the device path; "
+                                      "valid: DILU (run as diagonal), diagonal, Jacobi, none"};
     } else {
         throw Error{RT_E_UNKNOWN_TYPE, "Unknown asymmetric matrix preconditioner " + pc +
-                                           "\n\nValid asymmetric matrix preconditioners are :\n5\n(\nDILU\nJacobi\ndiagonal\nmulticolourILU0\nnone\n)"};
+                                           "\n\nValid asymmetric matrix preconditioners are :\n4\n(\nDILU\nJacobi\ndiagonal\nnone\n)"};
     }
     double v;
     if (e->scalar("tolerance", v)) c.tolerance = v;
@@ -470,7 +482,8 @@ void modelCorrectBegin(Model& M, double deltaT, const double* gradU, int iterati
     Ctx* ctx = M.ctx;
     Mesh& m = *M.mesh;
     const int32_t N = m.N;
-    M.kry.ensure(m, M.ctl.mixed);
+    const bool bicg = M.ctl.solver == SV_PBICG;
+    M.kry.ensure(m, M.ctl.mixed, bicg);
     if (!M.diag.p) {
         M.diag = ctx->alloc((size_t)N * 8);
         M.aval = ctx->alloc((size_t)std::max<int64_t>(1, m.nEll) * 8);
@@ -522,6 +535,7 @@ void modelCorrectBegin(Model& M, double deltaT, const double* gradU, int iterati
     a.diag = M.diag.as<double>();
     a.b = M.b.as<double>();
     a.fOld = modelKeepsFactor(M.prm.kind) ? M.fOld.as<double>() : nullptr;
+    a.atval = bicg ? M.kry.atval.as<double>() : nullptr;
     a.partials = M.kry.partials.as<double>();
     a.cs = st->c;
     a.nnzOut = &st->misc[0];
@@ -550,6 +564,7 @@ void modelCorrectBegin(Model& M, double deltaT, const double* gradU, int iterati
     sa.diag = M.diag.as<double>();
     sa.aval = M.aval.as<double>();
     sa.slotMask = M.slotMask.as<uint32_t>();
+    sa.atval = bicg ? M.kry.atval.as<double>() : nullptr;
     sa.b = M.b.as<double>();
     sa.x = M.theta.as<double>();
     sa.valid = M.valid6;
@@ -1338,7 +1353,8 @@ int rt_symm_solve(rt_ctx* ctx, const rt_mesh* mesh, const rt_ldu_symm* sys, doub
         }
         std::unique_ptr<Krylov> Kp(new Krylov);  // workspace of this call
         Krylov& K = *Kp;
-        K.ensure(m, ctl.mixed);
+        const bool bicg = ctl.solver == SV_PBICG;
+        K.ensure(m, ctl.mixed, bicg);
         DBuf dD = c->alloc((size_t)N * 8), dA = c->alloc((size_t)std::max<int64_t>(1, m.nEll) * 8),
              dB = c->alloc((size_t)6 * N * 8), dX = c->alloc((size_t)6 * N * 8),
              dLo = c->alloc((size_t)std::max(1, F) * 8), dUp = c->alloc((size_t)std::max(1, F) * 8);
@@ -1350,6 +1366,7 @@ int rt_symm_solve(rt_ctx* ctx, const rt_mesh* mesh, const rt_ldu_symm* sys, doub
             RT_HIP(hipMemcpyAsync(dUp.p, sys->upper, (size_t)F * 8, hipMemcpyHostToDevice, c->stream));
         }
         launch_ldu_to_ell(c, m, dLo.as<double>(), dUp.as<double>(), dA.as<double>());
+        if (bicg) launch_ldu_to_ell(c, m, dUp.as<double>(), dLo.as<double>(), K.atval.as<double>());  // A^T
         DBuf& st = staging(c).buf;
         uploadAos(c, st, src.data(), (int64_t)6 * N);
         launch_permute_rows(c, N, 6, 6, st.as<double>(), m.permD.as<int32_t>(), dB.as<double>());
@@ -1370,6 +1387,7 @@ int rt_symm_solve(rt_ctx* ctx, const rt_mesh* mesh, const rt_ldu_symm* sys, doub
         sa.kry = &K;
         sa.diag = dD.as<double>();
         sa.aval = dA.as<double>();
+        sa.atval = bicg ? K.atval.as<double>() : nullptr;
         sa.b = dB.as<double>();
         sa.x = dX.as<double>();
         sa.valid = m.valid;

@@ -169,8 +169,10 @@ struct Mesh {
 
 // --------------------------------------------------------- solver controls
 enum Precond { PC_NONE = 0, PC_JACOBI = 1, PC_MCILU0 = 2 };
+enum Solver { SV_PBICGSTAB = 0, SV_PBICG = 1 };
 struct Controls {
     std::string solverName = "PBiCGStab";
+    int solver = SV_PBICGSTAB;
     std::string precondName = "DILU";
     int precond = PC_JACOBI;
     double tolerance = 1e-6, relTol = 0.0;
@@ -200,6 +202,8 @@ struct KryCtl {
     double tol, relTol, innerRelTol;
     int32_t maxIter, minIter, maxInner, maxRefine;
     int32_t mixed;
+    int32_t bicg;   // PBiCG (1) instead of PBiCGStab (0)
+    int32_t precond;  // PC_NONE or PC_JACOBI
 };
 
 struct SolveState {
@@ -221,13 +225,17 @@ struct Krylov {
     DBuf r, r0, p, v, t;   // FP64 [N][6]
     DBuf rs, r0s, ps, vs, ts;  // FP32 [N][6] (mixed precision inner solve)
     DBuf es;               // FP32 [N][6] the inner solve's correction to x (mixed)
+    // PBiCG: rA = r, rT = r0, pA = p, pT = v, wA = t, wT = s
+    DBuf s64, s32;         // [N][6] wT in FP64 / FP32
+    DBuf atval;            // double [nEll] A^T off-diagonals, column order (written by the assembly)
+    DBuf a32, at32;        // float [nEll] A and A^T off-diagonals of the mixed PBiCG
     DBuf ahat32;           // float [nEll]  column-scaled off-diagonals A D^-1
     DBuf ahat64;           // double [nEll]
     DBuf rD;               // double [N]    1/diag
     DBuf partials;         // double [grid][24]
     DBuf state;            // SolveState
     std::unique_ptr<SolveRun, SolveRunDeleter> run;
-    void ensure(Mesh& m, bool mixed);
+    void ensure(Mesh& m, bool mixed, bool bicg = false);
 };
 
 // ------------------------------------------------------------------- model
@@ -344,6 +352,7 @@ struct AssembleArgs {
     double* diag;
     double* b;            // [N][6]
     double* fOld;         // FENE-P f, nullable
+    double* atval;        // A^T off-diagonals in column order (PBiCG), nullable
     const double* souT;   // [N][6] deferred-correction source, NULL for none
     int32_t noConv;       // the "none" scheme: no convection term
     int32_t bounded;      // subtract V*(net outflux/V) from the diagonal
@@ -408,6 +417,7 @@ struct SolveArgs {
     const double* diag;  // [N]
     const double* aval;  // ELL values
     const uint32_t* slotMask;  // [nSlices] live slots of aval; NULL: all live
+    const double* atval;       // A^T off-diagonals (PBiCG), column order
     const double* b;     // [N][6]
     double* x;           // [N][6] (theta in place)
     const int32_t* valid;

@@ -27,7 +27,7 @@ format.
     python -m rheotool_amd.stressfoam CASE [--U-time T] [--steps N] [--no-write]
 
 The velocity is not solved: the momentum and pressure equations of rheoFoam are
-outside this library.  --oracle runs the CPU oracle instead (test use only).
+outside this library.
 """
 from __future__ import annotations
 
@@ -123,8 +123,11 @@ class Mode:
 
 
 class StressCase:
-    def __init__(self, path, u_time=None, oracle=False):
-        self.path, self.oracle = path, oracle
+    """A case directory's stress step on the device.  Subclasses may replace
+    the backend (_init_backend, _step_backend, _mode_state)."""
+
+    def __init__(self, path, u_time=None):
+        self.path = path
         self.pm = polymesh.case_mesh(path)
         self.m = self.pm.to_mesh()
         self.ctl = Controls(path)
@@ -172,10 +175,7 @@ class StressCase:
         self.U, self.Ub = self._velocity(ut)
         self.phi = self._flux(ut)
         self.time = self.ctl.startTime
-        if oracle:
-            self._init_oracle()
-        else:
-            self._init_device()
+        self._init_backend()
 
     # ----------------------------------------------------------------- input
     def _opt_tensor(self, tdir, name):
@@ -241,7 +241,7 @@ class StressCase:
         return phi
 
     # ---------------------------------------------------------------- device
-    def _init_device(self):
+    def _init_backend(self):
         from . import abi
         self.abi = abi
         self.ctx = abi.Context(0)
@@ -269,80 +269,27 @@ class StressCase:
                     fl.set_bc(patch, t, vals)
         self.eq.set_flow(self.U, self.Ub, self.phi)
 
-    def _step_device(self, dt):
+    def _step_backend(self, dt):
         rc = self.eq.correct(dt)
         return rc
-
-    # ---------------------------------------------------------------- oracle
-    def _init_oracle(self):
-        from oracle import oracle as O
-        from .cases import Fields
-        from .rheotest import Runner
-        self.O = O
-        N = self.m.nCells
-        for md in self.modes:
-            md.oparams, md.etaS = Runner.oracle_params(md.params)
-            md.ev = np.broadcast_to(np.eye(3), (N, 3, 3)).copy() if md.eigVals is None else md.eigVals
-            md.vec = np.broadcast_to(np.eye(3), (N, 3, 3)).copy() if md.eigVecs is None else md.eigVecs
-            bc = np.zeros(self.m.nPatches, np.int32)
-            bcval = np.zeros((max(self.m.nBoundary, 1), 6))
-            for patch, t, vals in md.bc:
-                bc[patch] = t
-                if vals is not None:
-                    s = self.m.patchStart[patch]
-                    bcval[s:s + len(vals)] = vals
-            md.obc, md.obcval = bc, bcval
-            md.tau = np.zeros((N, 6)) if md.tau0 is None else md.tau0.copy()
-            if md.bmp:
-                fb = np.zeros(self.m.nPatches, np.int32)
-                fv = np.zeros(max(self.m.nBoundary, 1))
-                for patch, t, vals in md.fbc:
-                    fb[patch] = t
-                    if vals is not None:
-                        s = self.m.patchStart[patch]
-                        fv[s:s + len(vals)] = vals
-                md.ofbc, md.ofbcval = fb, fv
-        self.fields = Fields(U=self.U, Ub=self.Ub, phi=self.phi, gradU=None,
-                             theta=None, eigVals=None, eigVecs=None)
-        self.octl = O.controls()
-
-    def _step_oracle(self, dt):
-        import dataclasses
-        for md in self.modes:
-            f = dataclasses.replace(self.fields, theta=md.theta, eigVals=md.ev, eigVecs=md.vec)
-            if md.bmp:
-                th, ev, vec, tau, _, md.Phi, _ = self.O.bmp_correct(
-                    self.m, md.oparams, f, dt, md.Phi, md.tau, ctl=self.octl, bc=md.obc, bcval=md.obcval,
-                    relax=md.relax, scheme=_oracle_scheme(md.scheme), bc_phi=md.ofbc,
-                    bcval_phi=md.ofbcval, relax_phi=md.frelax, scheme_phi=_oracle_scheme(md.fscheme),
-                    ctl_phi=self.octl)
-            else:
-                th, ev, vec, tau, _ = self.O.correct(self.m, md.oparams, f, dt, ctl=self.octl,
-                                                     bc=md.obc, bcval=md.obcval, relax=md.relax,
-                                                     scheme=_oracle_scheme(md.scheme), tau_old=md.tau)
-            md.theta, md.ev, md.vec, md.tau = th, ev, vec, tau
-        return 0
 
     # ------------------------------------------------------------------- run
     def step(self):
         dt = self.ctl.deltaT
-        rc = self._step_oracle(dt) if self.oracle else self._step_device(dt)
+        rc = self._step_backend(dt)
         self.time += dt
         return rc
 
+    def _mode_state(self, md):
+        u = md.unit
+        out = dict(theta=u.theta(), tau=u.tau(), eigVals=u.eigVals(), eigVecs=u.eigVecs())
+        if md.bmp:
+            out["Phi"] = md.funit.Phi()
+        return out
+
     def state(self):
         """{mode: dict(theta, tau, eigVals, eigVecs[, Phi])} and the summed tau."""
-        out = {}
-        for md in self.modes:
-            if self.oracle:
-                out[md.name] = dict(theta=md.theta, tau=md.tau, eigVals=md.ev, eigVecs=md.vec)
-                if md.bmp:
-                    out[md.name]["Phi"] = md.Phi
-            else:
-                u = md.unit
-                out[md.name] = dict(theta=u.theta(), tau=u.tau(), eigVals=u.eigVals(), eigVecs=u.eigVecs())
-                if md.bmp:
-                    out[md.name]["Phi"] = md.funit.Phi()
+        out = {md.name: self._mode_state(md) for md in self.modes}
         tau = sum(s["tau"] for s in out.values())
         return out, tau
 
@@ -406,22 +353,14 @@ def _face_vals(pm, cellvals, patch):
     raise KeyError(patch)
 
 
-def _oracle_scheme(text):
-    w = text.split()
-    bounded = w[0] == "bounded"
-    w = [x for x in w if x not in ("bounded", "GaussDefCmpw", "Gauss")]
-    return ("bounded " if bounded else "") + w[0]
-
-
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m rheotool_amd.stressfoam")
     ap.add_argument("case")
     ap.add_argument("--U-time", default=None, help="time directory holding U (and phi)")
     ap.add_argument("--steps", type=int, default=None, help="at most N time steps")
     ap.add_argument("--no-write", action="store_true")
-    ap.add_argument("--oracle", action="store_true", help="run the CPU oracle instead (test use)")
     a = ap.parse_args(argv)
-    sc = StressCase(a.case, u_time=a.U_time, oracle=a.oracle)
+    sc = StressCase(a.case, u_time=a.U_time)
     out = sc.run(steps=a.steps, write=not a.no_write, log=lambda s: print(s, flush=True))
     for d in out:
         print(f"wrote {d}")

@@ -13,6 +13,8 @@ import shutil
 import numpy as np
 import pytest
 
+import oracle_backends
+
 from rheotool_amd import cases, foamdict, polymesh, stressfoam
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
@@ -209,11 +211,11 @@ def prepare(tmp_path, name, vel):
 def test_runner_restart_matches_one_run(tmp_path, name, vel):
     a = prepare(tmp_path / "a", name, vel)
     b = prepare(tmp_path / "b", name, vel)
-    sa = stressfoam.StressCase(a, oracle=True)
+    sa = oracle_backends.OracleStressCase(a)
     written = sa.run(steps=10)
     assert written and all(os.path.isdir(d) for d in written)
     full, tau = sa.state()
-    sb = stressfoam.StressCase(b, oracle=True)
+    sb = oracle_backends.OracleStressCase(b)
     n1 = 4 if name == "stresscase_cavity" else 5
     sb.run(steps=n1)
     # restart from the latest written time
@@ -221,7 +223,7 @@ def test_runner_restart_matches_one_run(tmp_path, name, vel):
         txt = fh.read()
     with open(os.path.join(b, "system", "controlDict"), "w") as fh:
         fh.write(txt.replace("startFrom       startTime;", "startFrom       latestTime;"))
-    sc = stressfoam.StressCase(b, u_time="0", oracle=True)
+    sc = oracle_backends.OracleStressCase(b, u_time="0")
     assert np.isclose(sc.time, n1 * sc.ctl.deltaT)
     sc.run(steps=10 - n1)
     again, tau2 = sc.state()
@@ -238,14 +240,13 @@ def test_runner_restart_matches_one_run(tmp_path, name, vel):
 
 def test_oracle_parameters_of_the_case_models():
     """The case runner's oracle reads every key each model's constructor reads."""
-    from rheotool_amd.rheotest import Runner
     d = foamdict.read(os.path.join(GOLDEN, "stresscase_bmp", "constant", "constitutiveProperties"))
-    p, etaS = Runner.oracle_params(d.sub("parameters"))
+    p, etaS = oracle_backends.oracle_params(d.sub("parameters"))
     assert (p.G0, p.kBMP, p.Phi0, p.PhiInf, p.lambda_, etaS) == (0.5, 2.0, 2.0, 50.0, 0.5, 0.01)
     s = foamdict.parse("parameters { type SaramitoLog; rho rho [1 -3 0 0 0 0 0] 1; etaS etaS [1 -1 -1 0 0 0 0] 0; "
                        "etaP etaP [1 -1 -1 0 0 0 0] 1; lambda lambda [0 0 1 0 0 0 0] 0.1; "
                        "tau0 tau0 [1 -1 -2 0 0 0 0] 2.5; k k [1 -1 -1 0 0 0 0] 1.5; n n [0 0 0 0 0 0 0] 0.75; "
                        "zeta zeta [0 0 0 0 0 0 0] 0; PTTfunction none; epsilon epsilon [0 0 0 0 0 0 0] 0; "
                        "dims (1 1 0); stabilization coupling; }").sub("parameters")
-    p, _ = Runner.oracle_params(s)
+    p, _ = oracle_backends.oracle_params(s)
     assert (p.tau0, p.sarN, p.sarK, p.dimX, p.dimY, p.dimZ, p.sarPTT) == (2.5, 0.75, 1.5, 1.0, 1.0, 0.0, 0)

@@ -13,6 +13,7 @@ import os
 import numpy as np
 import pytest
 
+import oracle_backends
 from rheotool_amd import polymesh, stressfoam
 
 from test_case_io import cavity_velocity, channel_velocity, prepare
@@ -33,7 +34,7 @@ def test_case_run_device_matches_oracle(tmp_path, name, vel):
     orc_case = prepare(tmp_path / "orc", name, vel)
     dev = stressfoam.StressCase(dev_case)
     written = dev.run(steps=10)
-    orc = stressfoam.StressCase(orc_case, oracle=True)
+    orc = oracle_backends.OracleStressCase(orc_case)
     orc.run(steps=10, write=False)
     d, dtau = dev.state()
     o, otau = orc.state()
@@ -74,7 +75,7 @@ def test_breakdown_restarts_instead_of_stopping(tmp_path):
     o = prepare(tmp_path / "orc", "stresscase_channel", channel_velocity)
     runs = []
     for path, oracle in ((c, False), (o, True)):
-        sc = stressfoam.StressCase(path, oracle=oracle)
+        sc = oracle_backends.OracleStressCase(path) if oracle else stressfoam.StressCase(path)
         for md in sc.modes:
             md.theta[:] = 0.0
             vals = np.tile([0.1, 0.02, 0.0, 0.05, 0.0, 0.0], (10, 1))

@@ -291,6 +291,75 @@ def test_symm_solve_matches_spsolve_fixtures(ctx, golden_dir):
                 assert rel_l2(x[:, q], d["x"][:, q]) < 1e-11, (name, prec, q, perf[q])
 
 
+@pytest.mark.parametrize("solver,precond,precision", [
+    ("PBiCG", "DILU", "double"), ("PBiCG", "DILU", "mixed"), ("PBiCG", "none", "double"),
+    ("PBiCG", "diagonal", "double"), ("PBiCGStab", "none", "double"), ("PBiCGStab", "none", "mixed")])
+def test_solver_and_preconditioner_keywords_match_oracle(ctx, solver, precond, precision):
+    """fvSolution's solver and preconditioner keywords select the device
+    algorithm: PBiCG ([OF9] PBiCG::solve, the tutorials' theta solver) and
+    PBiCGStab, with the diagonal preconditioner (DILU's device replacement) or
+    none.  Converged theta/tau against the reference algorithm (oracle,
+    PBiCG/PBiCGStab + DILU) within 1e-9.  With the same preconditioner on both
+    sides (diagonal, none) in FP64 the device runs the oracle's recurrence, so
+    the iteration counts agree too."""
+    m, f = cases.box3d(20, lnLo=-3.0, lnHi=2.0, perm_seed=5, jitter=0.1)
+    print(f"[case] {solver} {precond} {precision}", flush=True)
+    dm = ctx.mesh(m)
+    eq = ctx.constitutiveEq("", dm, abi.model_dict("GiesekusLog", lam=2.0, alpha=0.1))
+    eq.set_solver(abi.solver_dict(1e-10, precond=precond, precision=precision, solver=solver))
+    eq.set_state(f.theta, f.eigVals, f.eigVecs)
+    eq.set_flow(f.U, f.Ub, f.phi)
+    assert eq.correct(1e-3) == abi.RT_OK, eq.solver_perf()
+    g = dict(theta=eq.theta(), tau=eq.tau(), perf=eq.solver_perf())
+    p = oracle_params("GiesekusLog", 2.0, dict(alpha=0.1))
+    sv = O.PBICG if solver == "PBiCG" else O.PBICGSTAB
+    th, ev, vec, tau, perf = O.correct(m, p, f, 1e-3, ctl=O.controls(solver=sv, tol=1e-10))
+    check_step(g, dict(theta=th, tau=tau, perf=perf), m.validComponents)
+    if precision == "double" and precond != "DILU":
+        pc = O.PC_NONE if precond == "none" else O.PC_DIAGONAL
+        perf_same = O.correct(m, p, f, 1e-3, ctl=O.controls(solver=sv, precond=pc, tol=1e-10))[4]
+        for q in range(6):  # rounding may move a stop across the tolerance by one iteration
+            assert abs(g["perf"][q].nIterations - perf_same[q].nIterations) <= 1, (q, g["perf"], perf_same)
+
+
+def test_pbicg_symm_solve_matches_spsolve_fixtures(ctx, golden_dir):
+    """The sparseSolver surface with solver PBiCG (A^T from the swapped
+    lower/upper coefficients) on the golden LDU systems."""
+    for name in ["3x3x1", "3x3x3", "4x4x4j", "5x4x1p"]:
+        d = np.load(f"{golden_dir}/ldu_{name}.npz")
+        nx, ny, nz, twoD, channel, jitter, perm, kind = d["spec"]
+        m = cases.cartesian(int(nx), int(ny), int(nz), twoD=bool(twoD), channel=bool(channel),
+                            jitter=float(jitter), perm_seed=int(perm))
+        sysm = O.assemble(m, d["phi"], d["theta"], d["S"], float(d["dt"]))
+        dm = ctx.mesh(m)
+        for prec in ["double", "mixed"]:
+            x, perf = ctx.symm_solve(dm, sysm["diag"], sysm["lower"], sysm["upper"], sysm["source"],
+                                     d["theta"], abi.solver_dict(1e-13, precision=prec, solver="PBiCG"),
+                                     internalCoeffs=sysm["iC"][:m.nBoundary],
+                                     boundaryCoeffs=sysm["bC"][:m.nBoundary])
+            for q in range(6):
+                if m.validComponents[q] > 0:
+                    assert rel_l2(x[:, q], d["x"][:, q]) < 1e-11, (name, prec, q, perf[q])
+
+
+@pytest.mark.parametrize("pc,code", [("DIC", abi.RT_E_UNKNOWN_TYPE), ("FDIC", abi.RT_E_UNKNOWN_TYPE),
+                                     ("multicolourILU0", abi.RT_E_UNSUPPORTED),
+                                     ("ILU7", abi.RT_E_UNKNOWN_TYPE)])
+def test_unavailable_preconditioners_are_refused(ctx, pc, code):
+    """No silent replacement: DIC/FDIC are symmetric-matrix preconditioners
+    that [OF9] rejects for the asymmetric theta matrix (listing the valid
+    asymmetric ones), multicolour ILU0 is not built, unknown names list the
+    valid ones."""
+    m = cases.cartesian(3, 3, 3)
+    dm = ctx.mesh(m)
+    eq = ctx.constitutiveEq("", dm, abi.model_dict("Oldroyd-BLog"))
+    with pytest.raises(abi.RheoError) as e:
+        eq.set_solver(abi.solver_dict(1e-10, precond=pc))
+    assert e.value.code == code
+    if code == abi.RT_E_UNKNOWN_TYPE:
+        assert f"Unknown asymmetric matrix preconditioner {pc}" in e.value.msg and "DILU" in e.value.msg
+
+
 def test_unknown_model_type_lists_valid_types(ctx):
     m = cases.cartesian(2, 2, 2)
     dm = ctx.mesh(m)

@@ -6,6 +6,7 @@ import os
 import numpy as np
 import pytest
 
+import oracle_backends
 from rheotool_amd import rheotest
 
 pytestmark = pytest.mark.gpu
@@ -21,7 +22,7 @@ def test_ramp_fenecr_device_matches_oracle():
     case = os.path.join(GOLDEN, "rheotest_fenecr")
     print("[case] rheotest ramp FENE-CR 2 rates", flush=True)
     dev = rheotest.run(case, rates=2)
-    orc = rheotest.run(case, rates=2, oracle=True)
+    orc = oracle_backends.rheotest_run(case, rates=2)
     a, b = rows(dev), rows(orc)
     assert a.shape == b.shape == (2, 7)
     assert np.allclose(a, b, rtol=1e-9, atol=1e-12), (a, b)
@@ -32,7 +33,7 @@ def test_transient_multimode_xpp_inner_iterations_device_matches_oracle():
     case = os.path.join(GOLDEN, "rheotest_xpp")
     print("[case] rheotest transient XPP 2 modes 20 steps x 5 inner", flush=True)
     dev = rheotest.run(case, max_steps=20)
-    orc = rheotest.run(case, max_steps=20, oracle=True)
+    orc = oracle_backends.rheotest_run(case, max_steps=20)
     a, b = rows(dev), rows(orc)
     assert a.shape == b.shape == (20, 7)
     assert np.array_equal(a[:, 0], b[:, 0])

@@ -10,6 +10,8 @@ extra stress is etaP/lambda f (C - I) + 2 etaS D."""
 import os
 
 import numpy as np
+
+import oracle_backends
 import pytest
 from scipy.optimize import brentq
 
@@ -100,7 +102,7 @@ def parse_ramp(text):
 
 
 def test_ramp_report_matches_the_closed_form_steady_state():
-    text = rheotest.run(os.path.join(GOLDEN, "rheotest_fenecr"), oracle=True)
+    text = oracle_backends.rheotest_run(os.path.join(GOLDEN, "rheotest_fenecr"))
     assert text.startswith("******* Ramp mode")
     rows = parse_ramp(text)
     assert [r[0] for r in rows] == [0.05, 0.2, 0.45]
@@ -111,7 +113,7 @@ def test_ramp_report_matches_the_closed_form_steady_state():
 
 
 def test_transient_report_format_and_multimode_sum():
-    text = rheotest.run(os.path.join(GOLDEN, "rheotest_xpp"), oracle=True, max_steps=20)
+    text = oracle_backends.rheotest_run(os.path.join(GOLDEN, "rheotest_xpp"), max_steps=20)
     lines = text.strip().splitlines()
     assert lines[0] == "******* Transient mode: fixed ε̇|γ̇ = 0.38 **********"
     assert lines[1].split("\t")[0] == "t" and len(lines) == 22
