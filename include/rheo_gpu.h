@@ -329,6 +329,24 @@ typedef struct {
 int rt_symm_solve(rt_ctx* ctx, const rt_mesh* mesh, const rt_ldu_symm* sys, double* psi,
                   const char* controls, rt_solver_perf perf[6]);
 
+/* rt_symm_solve without allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and kernels as rt_symm_solve, but it calls no hipMalloc,
+ * hipHostMalloc or hipFree: the device copies of the system, the ELL values,
+ * the Krylov vectors, the reduction partials, the solver state and the
+ * staging live in one device array of RT_NOALLOC_SCRATCH_BYTES that the
+ * library holds on each device from the first rt_ctx_create there until the
+ * process ends, and the solver status comes back through pageable memory
+ * (the call waits on the context's stream).  A mesh whose solve needs more
+ * (about 180k cells in FP64, 280k with "precision mixed") is refused with
+ * RT_E_BAD_ARG.  Calls that overlap on one device share the array and race;
+ * every call clears the state it needs.  Kernel profiling
+ * (rt_ctx_set_profiling) creates timing events and is meant to be off. */
+#define RT_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+int rt_symm_solve_noalloc(rt_ctx* ctx, const rt_mesh* mesh, const rt_ldu_symm* sys, double* psi,
+                          const char* controls, rt_solver_perf perf[6]);
+
 /* ------------------------------------------------------------------ serving
  * A device-resident request loop: independent stress steps (different
  * meshes, models, Weissenberg numbers) are admitted against an HBM budget

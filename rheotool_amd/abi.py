@@ -122,6 +122,7 @@ def lib():
             "rt_model_get": (i32, [vp, i32, vp]),
             "rt_model_solver_perf": (i32, [vp, vp]),
             "rt_symm_solve": (i32, [vp, vp, vp, vp, C.c_char_p, vp]),
+            "rt_symm_solve_noalloc": (i32, [vp, vp, vp, vp, C.c_char_p, vp]),
             "rt_server_create": (i32, [vp, C.c_size_t, i32, vp]),
             "rt_server_destroy": (None, [vp]),
             "rt_server_submit": (i32, [vp, vp, vp]),
@@ -232,8 +233,10 @@ class Context:
         return ConstitutiveEq(self, name, mesh, dict_text)
 
     def symm_solve(self, mesh, diag, lower, upper, source, psi, controls, internalCoeffs=None,
-                   boundaryCoeffs=None):
-        """sparseSolver<symmTensor>::solve: returns (psi, perf[6])."""
+                   boundaryCoeffs=None, noalloc=False):
+        """sparseSolver<symmTensor>::solve: returns (psi, perf[6]).  noalloc:
+        through rt_symm_solve_noalloc (no device allocation; meshes up to the
+        library's scratch size)."""
         N, F, NB = mesh.nCells, mesh.nInternal, mesh.nBoundary
         keep = [_f64(diag, (N,)), _f64(lower, (F,)), _f64(upper, (F,)), _f64(source, (N, 6)),
                 None if internalCoeffs is None else _f64(internalCoeffs, (NB, 6)),
@@ -241,8 +244,9 @@ class Context:
         s = LduSymm(*[_p(k) for k in keep])
         x = _f64(psi, (N, 6)).copy()
         perf = (SolverPerf * 6)()
-        rc = lib().rt_symm_solve(self.h, mesh.h, C.byref(s), _p(x), controls.encode(), perf)
-        self.check(rc, "rt_symm_solve")
+        fn = lib().rt_symm_solve_noalloc if noalloc else lib().rt_symm_solve
+        rc = fn(self.h, mesh.h, C.byref(s), _p(x), controls.encode(), perf)
+        self.check(rc, "rt_symm_solve_noalloc" if noalloc else "rt_symm_solve")
         return x, [perf[i] for i in range(6)]
 
 

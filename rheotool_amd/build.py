@@ -26,6 +26,7 @@ GPU_SOURCES = [
     "rt_dict.cpp",
     "rt_mesh.cpp",
     "rt_serve.cpp",
+    "rt_noalloc.cpp",
     "kernels_cell.hip",
     "kernels_krylov.hip",
 ]

@@ -940,6 +940,14 @@ KryCtl kryCtl(const Controls& C, bool mixed) {
 void Krylov::ensure(Mesh& m, bool mixed, bool bicg) {
     if (!ctx) ctx = m.ctx;
     const size_t vec = (size_t)6 * m.N;
+    if (noAlloc) {
+        const bool ok = N == m.N && partials.p && state.p && rD.p && (mixed ? rs.p && r0s.p && ps.p && vs.p &&
+                                                                          ts.p && es.p && ahat32.p
+                                                                    : r.p && r0.p && p.p && v.p && t.p && ahat64.p) &&
+                        (!bicg || (atval.p && (mixed ? s32.p && a32.p && at32.p : s64.p != nullptr)));
+        if (!ok) throw Error{RT_E_BAD_ARG, "Krylov workspace bound to scratch memory does not fit the solve"};
+        return;
+    }
     if (N != m.N) {
         N = m.N;
         grid = grid_for(ctx, N);
@@ -992,10 +1000,11 @@ struct SolveRun {
     bool bicg = false;                // PBiCG instead of PBiCGStab
     int64_t nEll = 0;                 // all ELL entries (the float copies of A, A^T)
     hipEvent_t ev = nullptr;
+    bool ownsHost = false;            // hostState is a pinned block of ctx
     bool pending = false, done = false;
     int rounds = 0;
     ~SolveRun() {
-        if (hostState) ctx->givePinnedBlock(hostState);
+        if (hostState && ownsHost) ctx->givePinnedBlock(hostState);
         if (ev) (void)hipEventDestroy(ev);
     }
 };
@@ -1188,7 +1197,7 @@ constexpr int kRoundsAhead = 3;
 void queue_pull(Ctx* ctx, Krylov& K, SolveRun& R) {
     SolveState* st = K.state.as<SolveState>();
     RT_HIP(hipMemcpyAsync(R.hostState, st, sizeof(SolveState), hipMemcpyDeviceToHost, ctx->stream));
-    RT_HIP(hipEventRecord(R.ev, ctx->stream));
+    if (R.ev) RT_HIP(hipEventRecord(R.ev, ctx->stream));
     R.pending = true;
 }
 
@@ -1201,13 +1210,21 @@ void solve_begin(Ctx* ctx, const SolveArgs& sa) {
     const bool bicg = sa.ctl->solver == SV_PBICG;
     K.ensure(m, mixed, bicg);
     if (bicg && !sa.atval) throw Error{RT_E_BAD_ARG, "PBiCG needs the transposed matrix"};
+    if (K.run && sa.hostStatus && K.run->hostState != sa.hostStatus) K.run.reset();
     if (!K.run) {
         static_assert(sizeof(SolveState) <= Ctx::kPinnedBlock, "status block too small");
         K.run.reset(new SolveRun);
         K.run->ctx = ctx;
-        K.run->hostState = static_cast<SolveState*>(ctx->takePinnedBlock());
+        if (sa.hostStatus) {
+            // the caller's pageable block, no event: a blocking solve waits on
+            // the stream (the no-allocation path)
+            K.run->hostState = sa.hostStatus;
+        } else {
+            K.run->hostState = static_cast<SolveState*>(ctx->takePinnedBlock());
+            K.run->ownsHost = true;
+            RT_HIP(hipEventCreateWithFlags(&K.run->ev, hipEventDisableTiming));
+        }
         K.run->host = K.run->hostState->c;
-        RT_HIP(hipEventCreateWithFlags(&K.run->ev, hipEventDisableTiming));
     }
     SolveRun& R = *K.run;
     R.mixed = mixed;
@@ -1242,7 +1259,10 @@ bool solve_advance(Ctx* ctx, Krylov& K, bool block) {
     if (R.done) return true;
     if (R.pending) {
         if (block) {
-            RT_HIP(hipEventSynchronize(R.ev));
+            if (R.ev) RT_HIP(hipEventSynchronize(R.ev));
+            else RT_HIP(hipStreamSynchronize(ctx->stream));
+        } else if (!R.ev) {
+            throw Error{RT_E_BAD_ARG, "a solve without a status event can only be advanced blocking"};
         } else {
             hipError_t e = hipEventQuery(R.ev);
             if (e == hipErrorNotReady) return false;

@@ -23,14 +23,16 @@ DBuf& DBuf::operator=(DBuf&& o) noexcept {
         ctx = o.ctx;
         p = o.p;
         bytes = o.bytes;
+        view = o.view;
         o.ctx = nullptr;
         o.p = nullptr;
         o.bytes = 0;
+        o.view = false;
     }
     return *this;
 }
 void DBuf::release() {
-    if (p) {
+    if (p && !view) {
         if (ctx) {
             ctx->inUse -= bytes;
             ctx->recycle(p, bytes);
@@ -41,6 +43,7 @@ void DBuf::release() {
     p = nullptr;
     bytes = 0;
     ctx = nullptr;
+    view = false;
 }
 
 namespace {
@@ -683,6 +686,7 @@ int rt_ctx_create(int device, size_t hbm_budget, rt_ctx** out) {
         x.numCUs = prop.multiProcessorCount;
         x.budget = hbm_budget;
         RT_HIP(hipStreamCreateWithFlags(&x.stream, hipStreamNonBlocking));
+        noallocInit(device);
     });
     if (rc != RT_OK) {
         delete c;

@@ -29,12 +29,22 @@ struct Error {
 // ----------------------------------------------------------------- context
 struct Ctx;
 
-// Device buffer with budget accounting; move-only.
+// Device buffer with budget accounting; move-only.  A view (view = true)
+// points into memory owned elsewhere (the no-allocation scratch array) and
+// releases nothing.
 struct DBuf {
     Ctx* ctx = nullptr;
     void* p = nullptr;
     size_t bytes = 0;
+    bool view = false;
     DBuf() = default;
+    static DBuf viewOf(void* p, size_t bytes) {
+        DBuf b;
+        b.p = p;
+        b.bytes = bytes;
+        b.view = true;
+        return b;
+    }
     DBuf(const DBuf&) = delete;
     DBuf& operator=(const DBuf&) = delete;
     DBuf(DBuf&& o) noexcept { *this = std::move(o); }
@@ -235,6 +245,9 @@ struct Krylov {
     DBuf partials;         // double [grid][24]
     DBuf state;            // SolveState
     std::unique_ptr<SolveRun, SolveRunDeleter> run;
+    // set when the buffers above are views into the no-allocation scratch
+    // array: ensure() then refuses to allocate instead of calling hipMalloc
+    bool noAlloc = false;
     void ensure(Mesh& m, bool mixed, bool bicg = false);
 };
 
@@ -424,6 +437,9 @@ struct SolveArgs {
     const Controls* ctl;
     bool xbarKnown;      // CompState.xbar already set
     bool initDone;       // r, r0, rD and the initial state already written (by the assembly)
+    // pageable host block for the status read-back; NULL: a page-locked block
+    // of the context and an event (the blocking path then waits on the stream)
+    SolveState* hostStatus = nullptr;
 };
 KryCtl kryCtl(const Controls& C, bool mixed);
 // mean of theta per component -> CompState.xbar (gAverage(psi) of the normalisation)
@@ -450,6 +466,9 @@ bool solve_advance(Ctx* ctx, Krylov& K, bool block);
 void solve_finish(Krylov& K, rt_solver_perf perf[6]);
 
 int grid_for(const Ctx* ctx, int32_t N);
+// looks up the device address of the no-allocation scratch array on the
+// current device, once per device (rt_noalloc.cpp)
+void noallocInit(int device);
 
 // correct() as a resumable job (the serving loop interleaves several):
 // begin launches the assembly and the initial residual; advance returns true
