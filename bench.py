@@ -70,11 +70,14 @@ def make_case(w, n=None):
                        jitter=w.get("jitter", 0.0))
 
 
+PRECOND = "DILU"  # the fvSolution preconditioner keyword the bench's solves name
+
+
 def new_model(ctx, dm, w, f, precision):
     from rheotool_amd import abi
     eq = ctx.constitutiveEq("", dm, abi.model_dict(w["model"], etaS=0.59, etaP=0.41, lam=w["lam"],
                                                    **w["kw"]))
-    eq.set_solver(abi.solver_dict(1e-10, precision=precision))
+    eq.set_solver(abi.solver_dict(1e-10, precision=precision, precond=PRECOND))
     eq.set_state(f.theta, f.eigVals, f.eigVecs)
     eq.set_flow(f.U, f.Ub, f.phi)
     return eq
@@ -184,52 +187,56 @@ def cpu_baseline(w, n_single, n_all, reps=5):
     return m, f, (th, tau), cpu
 
 
-SERVE_DESC = ("serving stream: {n} stress-step requests, Poisson arrivals at {rate}/s, 2D/3D "
-              "Cartesian channels (coin flip) with cells log-uniform on [1e3, {mx:.0e}], model uniform "
+SERVE_DESC = ("serving stream: {n} stress-step requests, Poisson arrivals at {rate}/s, each on its own "
+              "2D or 3D Cartesian channel (coin flip) with cells log-uniform on [1e3, {mx:.0e}], model uniform "
               "over Oldroyd-B / Giesekus alpha=0.2 / linear PTT eps=0.25, Wi log-uniform [0.1, 20], "
-              "dt = 0.1 dx; FP32 Krylov + FP64 refinement; HBM budget 200 GB")
+              "fields from seed {seed}+i, dt = 0.1 dx; FP32 Krylov + FP64 refinement; HBM budget 200 GB")
 
 
-def serve_bench(args):
-    """Continuous-batching stream: requests arrive as a Poisson process, are
-    admitted by the HBM budget and run overlapped; reports p50/p99 latency and
-    aggregate cell-updates/s.  Meshes are uploaded once per distinct size and
-    shared (a serving system caches them); fields are generated per mesh."""
-    from rheotool_amd import abi, cases
-    rng = np.random.default_rng(4)
-    nreq = args.requests
+def serve_specs(nreq, max_cells, seed=4):
+    """Per request: its own near-square (2D) or near-cubic (3D) Cartesian
+    shape for a cell count log-uniform on [1e3, max_cells], the model and Wi."""
+    rng = np.random.default_rng(seed)
     specs = []
-    for i in range(nreq):
+    for _ in range(nreq):
         twoD = bool(rng.integers(2))
-        ncell = float(np.exp(rng.uniform(np.log(1e3), np.log(args.max_cells))))
-        # quantise the size: ~12 distinct shapes per dimension keep host-side
-        # generation bounded while spanning the range
+        ncell = float(np.exp(rng.uniform(np.log(1e3), np.log(max_cells))))
         if twoD:
-            n = int(round(np.sqrt(ncell)))
-            n = int(2 ** (round(np.log2(n) * 3) / 3))
-            key = ("2d", max(n, 8))
+            nx = max(8, int(round(np.sqrt(ncell))))
+            shape = (nx, max(8, int(round(ncell / nx))), 1)
         else:
-            n = int(round(ncell ** (1 / 3)))
-            n = int(2 ** (round(np.log2(n) * 3) / 3))
-            key = ("3d", max(n, 6))
+            nx = max(6, int(round(ncell ** (1 / 3))))
+            ny = max(6, int(round(np.sqrt(ncell / nx))))
+            shape = (nx, ny, max(6, int(round(ncell / (nx * ny)))))
         model = ["Oldroyd-BLog", "GiesekusLog", "PTTLog"][int(rng.integers(3))]
         lam = float(np.exp(rng.uniform(np.log(0.1), np.log(20.0))))
-        specs.append((key, model, lam))
+        specs.append((shape, model, lam))
+    return specs
+
+
+def serve_stream(ctx, nreq, rate, max_cells, pageable=False, upload_eigen=False, batching=True,
+                 seed_base=1000):
+    """Continuous-batching stream: requests arrive as a Poisson process, each
+    on its own mesh with its own fields (seed seed_base + i); they are
+    admitted by the HBM budget and run overlapped, their Krylov rounds batched.
+    Meshes are uploaded before the stream (a serving system keeps its meshes
+    resident) and fields are generated beforehand; the timed stream covers
+    submit to result of every request.  Returns the measured figures."""
+    import dataclasses
+    from rheotool_amd import abi, cases
+    specs = serve_specs(nreq, max_cells)
     t0 = time.perf_counter()
-    ctx = abi.Context(0)
-    meshes = {}
-    for key in sorted({s[0] for s in specs}):
-        kind, n = key
-        m = (cases.cartesian(n, n, 1, twoD=True, channel=True) if kind == "2d"
-             else cases.cartesian(n, n, n, channel=True))
-        f = cases.make_fields(m, cases.FIELDS_CHANNEL, seed=1000 + n)
-        meshes[key] = (m, f, ctx.mesh(m), 0.1 / n)
-    log(f"[serve] {len(meshes)} distinct meshes prepared in {time.perf_counter() - t0:.1f} s")
+    reqs = []
+    for i, (shape, model, lam) in enumerate(specs):
+        nx, ny, nz = shape
+        m = cases.cartesian(nx, ny, nz, twoD=nz == 1, channel=True)
+        f = cases.make_fields(m, cases.FIELDS_CHANNEL, seed=seed_base + i)
+        reqs.append([m, f, ctx.mesh(m), 0.1 / nx])
+    log(f"[serve] {nreq} meshes and field sets prepared in {time.perf_counter() - t0:.1f} s")
     outs = [None] * nreq
-    if not args.pageable:
+    if not pageable:
         # a server's receive and send buffers: page-locked, filled before the
-        # requests arrive (the inputs once per mesh, one result pair per request)
-        import dataclasses
+        # requests arrive
         t1 = time.perf_counter()
 
         def pin(a):
@@ -237,19 +244,19 @@ def serve_bench(args):
             p[...] = a
             return p
 
-        for key, (m, f, dm, dt) in list(meshes.items()):
-            pf = dataclasses.replace(f, **{k: pin(getattr(f, k)) for k in
-                                           ("theta", "eigVals", "eigVecs", "U", "Ub", "phi")})
-            meshes[key] = (m, pf, dm, dt)
-        for i, (key, _, _) in enumerate(specs):
-            N = meshes[key][0].nCells
+        for r in reqs:
+            m, f = r[0], r[1]
+            r[1] = dataclasses.replace(f, **{k: pin(getattr(f, k)) for k in
+                                             ("theta", "eigVals", "eigVecs", "U", "Ub", "phi")})
+        for i, r in enumerate(reqs):
+            N = r[0].nCells
             outs[i] = (ctx.pinned_empty((N, 6)), ctx.pinned_empty((N, 6)))
         log(f"[serve] page-locked request buffers ready in {time.perf_counter() - t1:.1f} s")
     kwmap = {"GiesekusLog": dict(alpha=0.2), "PTTLog": dict(epsilon=0.25, zeta=0.0)}
-    controls = abi.solver_dict(1e-10, precision="mixed")
-    srv = abi.Server(ctx, hbm_budget=200 << 30, max_inflight=16)
-    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, nreq))
-    tickets = [None] * nreq
+    controls = abi.solver_dict(1e-10, precision="mixed", precond=PRECOND)
+    srv = abi.Server(ctx, hbm_budget=200 << 30, max_inflight=16, batching=batching)
+    rng = np.random.default_rng(5)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
     index_of = {}
     done = [None] * nreq
     cells = 0
@@ -258,34 +265,50 @@ def serve_bench(args):
     while True:
         now = time.perf_counter() - t_start
         while i < nreq and arrivals[i] <= now:
-            key, model, lam = specs[i]
-            m, f, dm, dt = meshes[key]
-            tickets[i] = srv.submit(dm, f, abi.model_dict(model, lam=lam, **kwmap.get(model, {})), dt,
-                                    controls, out=outs[i], eigen=args.upload_eigen)
-            index_of[tickets[i]] = i
+            _, model, lam = specs[i]
+            m, f, dm, dt = reqs[i]
+            t = srv.submit(dm, f, abi.model_dict(model, lam=lam, **kwmap.get(model, {})), dt, controls,
+                           out=outs[i], eigen=upload_eigen)
+            index_of[t] = i
             i += 1
         live = srv.progress()
         for t in srv.finished():  # poll only the requests that finished
             j = index_of[t]
             out = srv.poll(t)
             done[j] = out["latency_ms"]
-            cells += meshes[specs[j][0]][0].nCells
+            cells += reqs[j][0].nCells
         if i == nreq and live == 0 and all(d is not None for d in done):
             break
     elapsed = time.perf_counter() - t_start
     lat = np.array(done)
+    rounds = srv.batched_rounds
     srv.close()
-    line = dict(metric=METRIC, value=round(cells / elapsed, 1), unit="cell-updates/s", n_gpus=1,
-                steps=nreq, warmup=0, ms_per_step=round(1e3 * elapsed / nreq, 3),
+    for r in reqs:
+        r[2].close()
+    return dict(requests=nreq, rate=rate, total_cells=int(cells), value=round(cells / elapsed, 1),
+                p50_ms=round(float(np.percentile(lat, 50)), 3), p99_ms=round(float(np.percentile(lat, 99)), 3),
+                max_ms=round(float(lat.max()), 3), wall_s=round(elapsed, 3), batched_rounds=rounds,
+                batching=batching, host_buffers="pageable" if pageable else "page-locked",
+                eigen_state="uploaded" if upload_eigen else "computed from theta on the device",
+                workload=SERVE_DESC.format(n=nreq, rate=rate, mx=max_cells, seed=seed_base))
+
+
+def serve_bench(args):
+    """The serving stream as the bench line (--workload serve)."""
+    from rheotool_amd import abi
+    ctx = abi.Context(0)
+    r = serve_stream(ctx, args.requests, args.rate, args.max_cells, pageable=args.pageable,
+                     upload_eigen=args.upload_eigen, batching=not args.no_batching)
+    nreq = r["requests"]
+    line = dict(metric=METRIC, value=r["value"], unit="cell-updates/s", n_gpus=1,
+                steps=nreq, warmup=0, ms_per_step=round(1e3 * r["wall_s"] / nreq, 3),
                 higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f32+f64",
                 data="synthetic",
-                config=dict(workload=SERVE_DESC.format(n=nreq, rate=args.rate, mx=args.max_cells),
-                            requests=nreq, total_cells=int(cells), distinct_meshes=len(meshes),
-                            host_buffers="pageable" if args.pageable else "page-locked",
-                            eigen_state="uploaded" if args.upload_eigen else "computed from theta on the device"),
-                p50_ms=round(float(np.percentile(lat, 50)), 3),
-                p99_ms=round(float(np.percentile(lat, 99)), 3),
-                max_ms=round(float(lat.max()), 3), wall_s=round(elapsed, 3))
+                config=dict(workload=r["workload"], requests=nreq, total_cells=r["total_cells"],
+                            host_buffers=r["host_buffers"], eigen_state=r["eigen_state"],
+                            batching=r["batching"], preconditioner=PRECOND),
+                p50_ms=r["p50_ms"], p99_ms=r["p99_ms"], max_ms=r["max_ms"], wall_s=r["wall_s"],
+                batched_rounds=r["batched_rounds"])
     print(json.dumps(line), flush=True)
     ctx.close()
 
@@ -303,11 +326,22 @@ def main():
                     help="serve: requests in pageable host arrays instead of page-locked ones")
     ap.add_argument("--upload-eigen", action="store_true",
                     help="serve: send each request's eigen state instead of computing it from theta")
+    ap.add_argument("--no-batching", action="store_true",
+                    help="serve: each request launches its own Krylov rounds")
+    ap.add_argument("--no-serve-sample", action="store_true",
+                    help="leave the serving-stream sample out of the default line")
+    ap.add_argument("--serve-requests", type=int, default=128, help="serving sample: requests")
+    ap.add_argument("--serve-rate", type=float, default=1000.0, help="serving sample: arrivals per second")
+    ap.add_argument("--serve-max-cells", type=float, default=2e5, help="serving sample: largest request")
+    ap.add_argument("--no-transfers", action="store_true",
+                    help="leave the transfer-inclusive drop-in measurement out")
     ap.add_argument("--n", type=int, default=0, help="override the mesh size")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--renumber", default="auto", choices=["auto", "none", "rcm", "sfc"],
                     help="device cell order (auto: RCM when the input order has poor locality)")
     ap.add_argument("--no-fp64-compare", action="store_true")
+    ap.add_argument("--precond", default="DILU",
+                    help="preconditioner keyword of the solves (DILU, multicolourILU0, diagonal, none)")
     ap.add_argument("--cpu-sample", type=int, default=64,
                     help="cells per side of the one-thread CPU sample")
     ap.add_argument("--cpu-sample-all", type=int, default=128,
@@ -319,6 +353,8 @@ def main():
               "only the one-GPU path is built)", file=sys.stderr)
         sys.exit(2)
 
+    global PRECOND
+    PRECOND = args.precond
     from rheotool_amd import abi
     if args.workload == "copy":
         # HBM calibration: a plain copy of 2 GiB each way
@@ -404,6 +440,29 @@ def main():
                        GBps=round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None)
                for k, v in sorted(stats.items(), key=lambda kv: -kv[1]["ms"])}
 
+    # the drop-in as a rheoFoam caller drives it:
+    # per step, U and phi go up, correct() runs, theta and tau come back
+    # (pageable host arrays, the eigen state stays on the device)
+    transfers = None
+    if not args.no_transfers:
+        th_h, tau_h = np.empty((m.nCells, 6)), np.empty((m.nCells, 6))
+        ts = []
+        for k in range(max(2, min(args.steps, 5)) + 1):
+            t1 = time.perf_counter()
+            eq.set_flow(f.U, f.Ub, f.phi)
+            eq.correct(w["dt"])
+            eq.get_into(abi.RT_THETA, th_h)
+            eq.get_into(abi.RT_TAU, tau_h)
+            if k:
+                ts.append(time.perf_counter() - t1)
+        t_tr = float(np.median(ts))
+        up = m.nCells * 24 + m.nBoundary * 24 + (m.nInternal + m.nBoundary) * 8
+        transfers = dict(value=round(m.nCells / t_tr, 1), unit="cell-updates/s", ms_per_step=round(1e3 * t_tr, 3),
+                         host_to_device_bytes=int(up), device_to_host_bytes=int(m.nCells * 96),
+                         steps=len(ts), note="per step: set_flow(U, Ub, phi) upload, correct(), theta and tau "
+                                             "downloaded into pageable arrays; median step")
+        log(f"[bench] drop-in with transfers {transfers['ms_per_step']} ms per step")
+
     cpu = None
     err_sample = None
     if not args.no_cpu_baseline and w["model"] != "Oldroyd-BLog" or (
@@ -427,7 +486,7 @@ def main():
         higher_is_better=True, scaling="weak", vs_baseline=None,
         dtype="f32+f64" if w["precision"] == "mixed" else "f64", data="synthetic",
         config=dict(workload=w["desc"], cells=m.nCells, internal_faces=m.nInternal,
-                    model=w["model"], precision=w["precision"]),
+                    model=w["model"], precision=w["precision"], preconditioner=PRECOND),
         roofline=roofline, cpu_baseline=cpu,
         p50_ms=round(float(np.percentile(step_ms, 50)), 3),
         p99_ms=round(float(np.percentile(step_ms, 99)), 3),
@@ -440,10 +499,16 @@ def main():
         fp64_dropin_ms_per_step=None if fp64_ms is None else round(fp64_ms, 3),
         speedup_vs_fp64_dropin=None if fp64_ms is None else round(fp64_ms / (1e3 * elapsed / args.steps), 3),
         err_vs_fp64=errors, err_vs_oracle_sample=err_sample, kernels=kernels,
+        dropin_with_transfers=transfers,
     )
-    print(json.dumps(line), flush=True)
     eq.close()
     dm.close()
+    if not args.no_serve_sample and args.workload == "precision":
+        # the serving configuration, at a size the default run affords: its own
+        # p50/p99 and aggregate rate beside the single-step figures
+        line["serve"] = serve_stream(ctx, args.serve_requests, args.serve_rate, args.serve_max_cells)
+        log(f"[bench] serve sample {line['serve']}")
+    print(json.dumps(line), flush=True)
     ctx.close()
 
 

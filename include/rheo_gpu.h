@@ -124,15 +124,20 @@ typedef struct {
                                  need the patch delta coefficients, are unavailable) */
 } rt_mesh_desc;
 
-enum rt_renumber { RT_RENUMBER_AUTO = 0, RT_RENUMBER_NONE = 1, RT_RENUMBER_RCM = 2, RT_RENUMBER_SFC = 3 };
+enum rt_renumber { RT_RENUMBER_AUTO = 0, RT_RENUMBER_NONE = 1, RT_RENUMBER_RCM = 2, RT_RENUMBER_SFC = 3,
+                   /* flag: keep a two-colourable mesh in one block (no red-black ILU0) */
+                   RT_RENUMBER_NO_COLOUR = 16 };
 
 /* Uploads a mesh.  The device copy is renumbered for locality when `renumber`
  * asks for it: RCM = reverse Cuthill-McKee on the face graph, SFC = Morton
  * (Z-order) curve of the cell centres (needs desc->C), which keeps a cell's
  * neighbours in every direction close in memory.  RT_RENUMBER_AUTO picks RCM
  * when the input ordering has poor locality (on a 256^3 box RCM and SFC give
- * the same mixed-precision step time; SFC's FP64 step is 4% faster).  Every
- * field crossing the C-ABI stays in the caller's numbering. */
+ * the same mixed-precision step time; SFC's FP64 step is 4% faster).  A mesh
+ * whose cell graph is two-colourable (any hexahedral or quadrilateral mesh) is
+ * then stored colour by colour, each colour in that order, for the red-black
+ * ILU0 preconditioner (multicolourILU0); RT_RENUMBER_NO_COLOUR keeps one
+ * block.  Every field crossing the C-ABI stays in the caller's numbering. */
 int rt_mesh_create(rt_ctx* ctx, const rt_mesh_desc* desc, int renumber, rt_mesh** out);
 void rt_mesh_destroy(rt_mesh* mesh);
 
@@ -372,6 +377,11 @@ typedef struct {
 } rt_request;
 
 int rt_server_create(rt_ctx* ctx, size_t hbm_budget, int max_inflight, rt_server** out);
+/* Device bytes the server reserves against its HBM budget for a request on
+ * `mesh` (model state, flow, matrix, Krylov workspace, staging, the prescribed
+ * gradient when gradU != 0, the result buffers): an upper bound of what the
+ * request allocates. */
+size_t rt_request_bytes_estimate(const rt_mesh* mesh, int mixed, int gradU);
 void rt_server_destroy(rt_server* server);
 /* Queues the request; it is admitted when a slot and the HBM budget allow.
  * The strings are copied; the host arrays are NOT: they are read when the
@@ -394,6 +404,13 @@ int rt_server_poll(rt_server* server, int64_t ticket, double* theta, double* tau
  * oldest first, and returns how many; a caller then polls only those instead
  * of every pending ticket. */
 int rt_server_finished(rt_server* server, int64_t* tickets, int max);
+/* Batching of the in-flight requests' Krylov solves (on by default): one
+ * launch per kernel runs the rounds of every request of the same precision and
+ * preconditioner form; off, each request launches its own rounds.  Results are
+ * the same bits either way.  Changed only while no request is in flight. */
+int rt_server_set_batching(rt_server* server, int enable);
+/* Batched Krylov rounds launched so far (each covers every request then in its batch). */
+int64_t rt_server_batched_rounds(const rt_server* server);
 
 /* Page-locked host memory (hipHostMalloc on the context's device) for a
  * serving caller's request and result arrays.  Transfers from and to it run

@@ -517,6 +517,35 @@ void orc_rhs(const orc_params* p, int32_t n, const double* gradU, const double* 
     orc_rhs_tau(p, n, gradU, theta, eigVals, eigVecs, nullptr, nullptr, S, fOut);
 }
 
+// PTTLog.C:137-152: gamma(beta), then gamma(alpha k + beta) for k = 0, 1, ...
+// while k < maxIterMittagLeffler and the last value stays below 1e100; the
+// series then runs over as many terms as the table holds.
+static std::vector<double> mlGammaTable(const orc_params* p, int* maxIter) {
+    std::vector<double> g{std::tgamma(p->mlBeta)};
+    int k = 0;
+    while (k < p->mlMaxIter && g.back() < 1e+100) {
+        g.push_back(std::tgamma(p->mlAlpha * k + p->mlBeta));
+        k++;
+    }
+    *maxIter = k;
+    return g;
+}
+
+// PTTLog.C:207-227: E_{alpha,beta}(z) summed term by term until the relative
+// change of the sum falls to rTolMittagLeffler (or the table ends)
+static double mittagLeffler(const std::vector<double>& g, int maxIter, double rtol, double z) {
+    double sum = 0.0, sumOld = 0.0, error = 1.0;
+    int k = 0;
+    while (k < maxIter && error > rtol) {
+        const double Eabk = std::pow(z, k) / g[k + 1];
+        sumOld = sum;
+        sum += Eabk;
+        error = std::fabs((sumOld - sum) / (sumOld + 1e-12));
+        k++;
+    }
+    return sum;
+}
+
 // SaramitoLog.C:181-212: the plasticity factor from the second invariant of
 // the deviatoric part of tau (over the valid dimensions)
 static double saramitoFac(const orc_params* p, const double* tau) {
@@ -541,7 +570,9 @@ void orc_rhs_tau(const orc_params* p, int32_t n, const double* gradU, const doub
                  double* S, double* fOut) {
     bool mask[3] = {false, false, false};
     if (p->ofInvQuirk && n > 0) invShiftMask(eigVals, mask);
-    const bool ptt = p->model == ORC_PTT_LINEAR || p->model == ORC_PTT_EXP;
+    const bool ptt = p->model == ORC_PTT_LINEAR || p->model == ORC_PTT_EXP || p->model == ORC_PTT_GEN;
+    int mlMax = 0;
+    const std::vector<double> mlG = p->model == ORC_PTT_GEN ? mlGammaTable(p, &mlMax) : std::vector<double>();
     for (int c = 0; c < n; ++c) {
         const double* L = gradU + 9 * (size_t)c;
         if (p->model == ORC_FLUIDITY) {
@@ -592,7 +623,15 @@ void orc_rhs_tau(const orc_params* p, int32_t n, const double* gradU, const doub
         expandSymm(theta + 6 * (size_t)c, th);
         dotTT(Om, th, Ot1);
         dotTT(th, Om, Ot2);
-        for (int q = 0; q < 9; ++q) R[q] = Ot1[q] - Ot2[q] + 2.0 * B[q];
+        if (p->model == ORC_ROOTK) {
+            // Oldroyd_BRootk.C:184-187: (omega & theta) - (theta & omega) + (2/k) (B & theta)
+            T9 Bth;
+            dotTT(B, th, Bth);
+            const double c2k = 2.0 / p->rootK;
+            for (int q = 0; q < 9; ++q) R[q] = Ot1[q] - Ot2[q] + c2k * Bth[q];
+        } else {
+            for (int q = 0; q < 9; ++q) R[q] = Ot1[q] - Ot2[q] + 2.0 * B[q];
+        }
         // model term
         T9 invL, X;
         invField(Lam, mask, invL);
@@ -652,7 +691,7 @@ void orc_rhs_tau(const orc_params* p, int32_t n, const double* gradU, const doub
             dotTT(V, D, t1);
             dotTT(t1, VT, Y);
             for (int q = 0; q < 9; ++q) X[q] = rl * Y[q];
-        } else if (p->model == ORC_PTT_LINEAR || p->model == ORC_PTT_EXP) {
+        } else if (p->model == ORC_PTT_LINEAR || p->model == ORC_PTT_EXP || p->model == ORC_PTT_GEN) {
             T9 VT, t1, C, D, Y;
             transp(V, VT);
             for (int q = 0; q < 9; ++q) D[q] = invL[q] - I9[q];
@@ -662,8 +701,25 @@ void orc_rhs_tau(const orc_params* p, int32_t n, const double* gradU, const doub
             dotTT(t1, VT, C);
             const double k = p->epsilon / (1 - p->zeta);
             const double z = k * (C[0] + C[4] + C[8] - 3.);
-            const double fac = p->model == ORC_PTT_LINEAR ? 1. + z : std::exp(z);
+            double fac;
+            if (p->model == ORC_PTT_LINEAR) fac = 1. + z;
+            else if (p->model == ORC_PTT_EXP) fac = std::exp(z);
+            else fac = mlG[0] * mittagLeffler(mlG, mlMax, p->mlRtol, z);  // PTTLog.C:229, gammaBeta*Eab
             for (int q = 0; q < 9; ++q) X[q] = (rl * Y[q]) * fac;
+        } else if (p->model == ORC_ROOTK) {
+            // Oldroyd_BRootk.C:165-173,188: opert = V diag(Lam^(1/k - 1)) V^T - theta,
+            // scaled by 1/(lambda k)
+            T9 VT, t1, Od, Y;
+            transp(V, VT);
+            const double e = 1. / p->rootK - 1.;
+            for (int q = 0; q < 9; ++q) Od[q] = 0.0;
+            Od[0] = std::pow(Lam[0], e);
+            Od[4] = std::pow(Lam[4], e);
+            Od[8] = std::pow(Lam[8], e);
+            dotTT(V, Od, t1);
+            dotTT(t1, VT, Y);
+            const double cf = 1.0 / (p->lambda * p->rootK);
+            for (int q = 0; q < 9; ++q) X[q] = cf * (Y[q] - th[q]);
         } else if (p->model == ORC_FENECR) {
             // FENE_CRLog.C:142-164: f = L2/(L2 - tr(V Lam V^T)), X = (1/lambda) f V (inv(Lam) - I) V^T
             T9 VT, t1, C, D, Y;
@@ -1221,6 +1277,18 @@ void orc_calc_eig(int32_t n, const double* theta, double* eigVals, double* eigVe
     }
 }
 
+// Oldroyd_BRootk.C:194-200: after calcEig (eigVals = exp(eig theta)) the
+// conformation eigenvalues are log(eigVals)^k
+static void rootkEigVals(const orc_params* p, int32_t n, double* eigVals) {
+    if (p->model != ORC_ROOTK) return;
+    for (int c = 0; c < n; ++c) {
+        double* e = eigVals + 9 * (size_t)c;
+        e[0] = std::pow(std::log(e[0]), p->rootK);
+        e[4] = std::pow(std::log(e[4]), p->rootK);
+        e[8] = std::pow(std::log(e[8]), p->rootK);
+    }
+}
+
 void orc_tau(const orc_params* p, int32_t n, const double* eigVals, const double* eigVecs,
              const double* fOld, double* tau) {
     for (int c = 0; c < n; ++c) {
@@ -1239,7 +1307,8 @@ void orc_tau(const orc_params* p, int32_t n, const double* eigVals, const double
             coef = p->etaP / p->lambda;
         } else {
             for (int q = 0; q < 9; ++q) D[q] = C[q] - I9[q];
-            if (p->model == ORC_PTT_LINEAR || p->model == ORC_PTT_EXP || p->model == ORC_SARAMITO)
+            if (p->model == ORC_PTT_LINEAR || p->model == ORC_PTT_EXP || p->model == ORC_PTT_GEN ||
+                p->model == ORC_SARAMITO)
                 coef = p->etaP / (p->lambda * (1 - p->zeta));  // PTTLog.C, SaramitoLog.C:292
             else if (p->model == ORC_BMP)  // BMPLog.C:199
                 coef = p->G0;
@@ -1345,6 +1414,7 @@ void orc_correct_aux(const orc_mesh* m, const orc_params* p, const double* U, co
                          bC.data(), bcType, valid, c, theta, perf);
     if (p->model == ORC_FLUIDITY) return;
     orc_calc_eig(N, theta, eigVals, eigVecs, 0);
+    rootkEigVals(p, N, eigVals);
     orc_tau(p, N, eigVals, eigVecs, f.data(), tau);
 }
 
@@ -1580,6 +1650,7 @@ void domainStep(orc_decomp* d, int s, const orc_params* p, double deltaT, const 
         if (s == 0) perf[q] = pf;
     }
     orc_calc_eig(n, th, ev, vec, 0);
+    rootkEigVals(p, n, ev);
     orc_tau(p, n, ev, vec, f, tau);
 }
 

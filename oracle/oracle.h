@@ -41,7 +41,12 @@ enum { ORC_OLDROYDB = 0, ORC_GIESEKUS = 1, ORC_FENEP = 2, ORC_PTT_LINEAR = 3, OR
        ORC_FENECR = 5, ORC_WMCY = 6, ORC_ROLIEPOLY = 7, ORC_XPP = 8, ORC_SARAMITO = 9,
        /* BMPLog (BMPLog.C:142-202): the theta equation, and its scalar fluidity
         * equation, held in component xx of a [6n] field (the others unused) */
-       ORC_BMP = 10, ORC_FLUIDITY = 11 };
+       ORC_BMP = 10, ORC_FLUIDITY = 11,
+       /* Oldroyd-BRootk (Oldroyd_BRootk.C:129-206): theta = C^(1/k) */
+       ORC_ROOTK = 12,
+       /* PTTLog with destructionFunctionType generalized (PTTLog.C:135-152,202-230):
+        * the Mittag-Leffler function E_{alpha,beta} of the trace */
+       ORC_PTT_GEN = 13 };
 
 typedef struct {
     int32_t model;
@@ -65,6 +70,12 @@ typedef struct {
     /* BMP (BMPLog.C:113-119): modulus, rate constant, fluidity at rest and at
      * infinite shear; lambda above */
     double G0, kBMP, Phi0, PhiInf;
+    /* Oldroyd-BRootk: the root k */
+    double rootK;
+    /* PTT generalized: alpha and beta of E_{alpha,beta}, the series' relative
+     * tolerance rTolMittagLeffler and maxIterMittagLeffler (PTTLog.C:135-136) */
+    double mlAlpha, mlBeta, mlRtol;
+    int32_t mlMaxIter;
 } orc_params;
 
 /* ORC_BC_COUPLED: a processor patch of a decomposed run.  The upwind

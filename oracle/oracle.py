@@ -21,9 +21,11 @@ PBICGSTAB, PBICG = 0, 1
 PC_NONE, PC_DIAGONAL, PC_DILU = 0, 1, 2
 
 FENECR, WMCY, ROLIEPOLY, XPP, SARAMITO, BMP, FLUIDITY = 5, 6, 7, 8, 9, 10, 11
+ROOTK, PTT_GEN = 12, 13
 MODEL_IDS = {"Oldroyd-BLog": OLDROYDB, "GiesekusLog": GIESEKUS, "FENE-PLog": FENEP,
              "PTTLog": PTT_LINEAR, "FENE-CRLog": FENECR, "WhiteMetznerCYLog": WMCY,
-             "Rolie-PolyLog": ROLIEPOLY, "XPomPomLog": XPP, "SaramitoLog": SARAMITO, "BMPLog": BMP}
+             "Rolie-PolyLog": ROLIEPOLY, "XPomPomLog": XPP, "SaramitoLog": SARAMITO, "BMPLog": BMP,
+             "Oldroyd-BRootk": ROOTK}
 SARAMITO_PTT = {"none": 0, "linear": 1, "exponential": 2}
 
 
@@ -49,7 +51,9 @@ class Params(C.Structure):
                 ("tau0", C.c_double), ("sarN", C.c_double), ("sarK", C.c_double),
                 ("dimX", C.c_double), ("dimY", C.c_double), ("dimZ", C.c_double),
                 ("sarPTT", C.c_int32),
-                ("G0", C.c_double), ("kBMP", C.c_double), ("Phi0", C.c_double), ("PhiInf", C.c_double)]
+                ("G0", C.c_double), ("kBMP", C.c_double), ("Phi0", C.c_double), ("PhiInf", C.c_double),
+                ("rootK", C.c_double), ("mlAlpha", C.c_double), ("mlBeta", C.c_double),
+                ("mlRtol", C.c_double), ("mlMaxIter", C.c_int32)]
 
 
 class Controls(C.Structure):
@@ -136,20 +140,27 @@ def params(model="Oldroyd-BLog", etaS=0.59, etaP=0.41, lam=1.0, alpha=0.0, L2=10
            lambdaR=1.0, lambdaD=1.0, beta=0.0, delta=-0.5, chiMax=0.0,
            lambdaS=1.0, lambdaB=1.0, q=1.0, n=0.0, m=1.0, K=0.0, Lc=0.0, a=2.0, b=2.0,
            wm_n=1.0, tau0=0.0, sar_n=1.0, sar_k=None, dims=(1.0, 1.0, 1.0), ptt_function="none",
-           G0=1.0, k_bmp=0.0, Phi0=0.0, PhiInf=0.0):
+           G0=1.0, k_bmp=0.0, Phi0=0.0, PhiInf=0.0, root_k=2.0, ptt_gen=None, ml_rtol=1e-12,
+           ml_max_iter=200):
     """Model constants.  Rolie-Poly: lambdaR lambdaD beta delta chiMax; XPP:
     lambdaS lambdaB alpha q n; White-Metzner CY: lam m wm_n K Lc(=L) a b;
     Saramito: tau0 sar_n sar_k (etaP when sar_n = 1) dims ptt_function
-    (n = 1 only) epsilon zeta; BMP: lam G0 k_bmp Phi0 PhiInf."""
+    (n = 1 only) epsilon zeta; BMP: lam G0 k_bmp Phi0 PhiInf; Oldroyd-BRootk:
+    root_k; PTTLog generalized: ptt_gen = (alpha, beta) of the Mittag-Leffler
+    function, ml_rtol, ml_max_iter."""
     mid = MODEL_IDS[model]
     if model == "PTTLog" and ptt_exp:
         mid = PTT_EXP
+    ml_a, ml_b = 1.0, 1.0
+    if model == "PTTLog" and ptt_gen is not None:
+        mid = PTT_GEN
+        ml_a, ml_b = (float(x) for x in ptt_gen)
     k = etaP if (sar_k is None or sar_n == 1.0) else sar_k
     ptt = SARAMITO_PTT[ptt_function] if sar_n == 1.0 else 0
     return Params(mid, rho, etaS, etaP, lam, alpha, L2, epsilon, zeta, int(of_inv_quirk),
                   lambdaR, lambdaD, beta, delta, chiMax, lambdaS, lambdaB, q, n,
                   m, wm_n, K, Lc, a, b, tau0, sar_n, k, *[float(x) for x in dims], ptt,
-                  G0, k_bmp, Phi0, PhiInf)
+                  G0, k_bmp, Phi0, PhiInf, root_k, ml_a, ml_b, ml_rtol, int(ml_max_iter))
 
 
 def controls(solver=PBICGSTAB, precond=PC_DILU, tol=1e-10, relTol=0.0, maxIter=1000, minIter=0):

@@ -29,6 +29,7 @@ RT_OK, RT_E_UNKNOWN_TYPE, RT_E_BAD_ARG, RT_E_OOM, RT_E_HIP, RT_E_UNSUPPORTED, RT
 RT_W_NOT_CONVERGED = 0x100
 RT_THETA, RT_TAU, RT_EIGVALS, RT_EIGVECS = range(4)
 RT_RENUMBER_AUTO, RT_RENUMBER_NONE, RT_RENUMBER_RCM, RT_RENUMBER_SFC = range(4)
+RT_RENUMBER_NO_COLOUR = 16
 RT_BC_ZEROGRADIENT, RT_BC_FIXEDVALUE, RT_BC_EMPTY = range(3)
 
 
@@ -129,6 +130,9 @@ def lib():
             "rt_server_progress": (i32, [vp]),
             "rt_server_poll": (i32, [vp, C.c_int64, vp, vp, vp, vp]),
             "rt_server_finished": (i32, [vp, vp, i32]),
+            "rt_server_set_batching": (i32, [vp, i32]),
+            "rt_request_bytes_estimate": (C.c_size_t, [vp, i32, i32]),
+            "rt_server_batched_rounds": (C.c_int64, [vp]),
             "rt_host_alloc": (i32, [vp, C.c_size_t, vp]),
             "rt_host_free": (None, [vp]),
         }
@@ -385,6 +389,14 @@ class ConstitutiveEq:
         self.ctx.check(lib().rt_model_get(self.h, which, _p(out)), "get")
         return out
 
+    def get_into(self, which, out):
+        """rt_model_get into a caller's C-contiguous float64 array (theta, tau: [N, 6])."""
+        if not (isinstance(out, np.ndarray) and out.dtype == np.float64 and out.flags.c_contiguous
+                and out.size == (9 if which in (RT_EIGVALS, RT_EIGVECS) else 6) * self.N):
+            raise ValueError("get_into: a C-contiguous float64 array of the field's size is needed")
+        self.ctx.check(lib().rt_model_get(self.h, which, _p(out)), "get")
+        return out
+
     def theta(self):
         return self._get(RT_THETA, (self.N, 6))
 
@@ -497,12 +509,18 @@ def mesh_desc(m):
 class Server:
     """Continuous-batching request loop (rt_server_*)."""
 
-    def __init__(self, ctx: Context, hbm_budget=0, max_inflight=8):
+    def __init__(self, ctx: Context, hbm_budget=0, max_inflight=8, batching=True):
         h = C.c_void_p()
         ctx.check(lib().rt_server_create(ctx.h, hbm_budget, max_inflight, C.byref(h)),
                   "rt_server_create")
         self.ctx, self.h = ctx, h
         self.held = {}  # ticket -> (input arrays, theta out, tau out): the server reads/writes them
+        ctx.check(lib().rt_server_set_batching(h, int(batching)), "rt_server_set_batching")
+
+    @property
+    def batched_rounds(self) -> int:
+        """Batched Krylov rounds launched so far (rt_server_batched_rounds)."""
+        return int(lib().rt_server_batched_rounds(self.h))
 
     def submit(self, mesh: "Mesh", fields, dict_text, deltaT, controls=None, gradU=None, out=None,
                eigen=True) -> int:

@@ -70,7 +70,7 @@ template <int KIND>
 __device__ __forceinline__ void cell_rhs(const ModelParams& p, int invMask, const double* L, const double* th6,
                                          const double* lam, const double* V, const double* tau6, double aux0,
                                          double* S, double& fOut) {
-    constexpr bool ptt = KIND == M_PTT_LINEAR || KIND == M_PTT_EXP;
+    constexpr bool ptt = KIND == M_PTT_LINEAR || KIND == M_PTT_EXP || KIND == M_PTT_GEN;
     if constexpr (KIND == M_FLUIDITY) {
         // BMPLog.C:152-160: Phi0/lambda + k (PhiInf - Phi)(tau && symm(L)); the
         // implicit -Sp(1/lambda) is on the diagonal
@@ -124,8 +124,17 @@ __device__ __forceinline__ void cell_rhs(const ModelParams& p, int invMask, cons
     double R[9], t2[9];
     dotTT(Om, th, tmp);
     dotTT(th, Om, t2);
+    if constexpr (KIND == M_ROOTK) {
+        // Oldroyd_BRootk.C:184-187: (omega & theta) - (theta & omega) + (2/k) (B & theta)
+        double Bth[9];
+        dotTT(B, th, Bth);
+        const double c2k = 2.0 / p.rootK;
 #pragma unroll
-    for (int q = 0; q < 9; ++q) R[q] = tmp[q] - t2[q] + 2.0 * B[q];
+        for (int q = 0; q < 9; ++q) R[q] = tmp[q] - t2[q] + c2k * Bth[q];
+    } else {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) R[q] = tmp[q] - t2[q] + 2.0 * B[q];
+    }
     double il[3];
     invDiag(lam, invMask, il);
     const double rl = 1.0 / p.lambda;
@@ -156,16 +165,41 @@ __device__ __forceinline__ void cell_rhs(const ModelParams& p, int invMask, cons
         vdvt(V, dd, Y);
 #pragma unroll
         for (int q = 0; q < 9; ++q) X[q] = rl * Y[q];
-    } else if constexpr (KIND == M_PTT_LINEAR || KIND == M_PTT_EXP) {
+    } else if constexpr (KIND == M_PTT_LINEAR || KIND == M_PTT_EXP || KIND == M_PTT_GEN) {
         const double dd[3] = {il[0] - 1.0, il[1] - 1.0, il[2] - 1.0};
         double Y[9], C[9];
         vdvt(V, dd, Y);
         vdvt(V, lam, C);
         const double k = p.epsilon / (1 - p.zeta);
         const double z = k * (C[0] + C[4] + C[8] - 3.);
-        const double fac = KIND == M_PTT_LINEAR ? 1. + z : exp(z);
+        double fac;
+        if constexpr (KIND == M_PTT_LINEAR) {
+            fac = 1. + z;
+        } else if constexpr (KIND == M_PTT_EXP) {
+            fac = exp(z);
+        } else {
+            // PTTLog.C:207-229: gamma(beta) E_{alpha,beta}(z), summed term by
+            // term until the relative change falls to rTolMittagLeffler
+            double sum = 0.0, sumOld = 0.0, error = 1.0;
+            for (int j = 0; j < p.mlMaxIter && error > p.mlRtol; ++j) {
+                const double Eabk = pow(z, (double)j) / p.mlGamma[j];
+                sumOld = sum;
+                sum += Eabk;
+                error = fabs((sumOld - sum) / (sumOld + 1e-12));
+            }
+            fac = p.mlGammaBeta * sum;
+        }
 #pragma unroll
         for (int q = 0; q < 9; ++q) X[q] = (rl * Y[q]) * fac;
+    } else if constexpr (KIND == M_ROOTK) {
+        // Oldroyd_BRootk.C:165-173,188: (1/(lambda k)) (V diag(Lam^(1/k - 1)) V^T - theta)
+        const double e = 1. / p.rootK - 1.;
+        const double od[3] = {pow(lam[0], e), pow(lam[1], e), pow(lam[2], e)};
+        double Y[9];
+        vdvt(V, od, Y);
+        const double cf = 1.0 / (p.lambda * p.rootK);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) X[q] = cf * (Y[q] - th[q]);
     } else if constexpr (KIND == M_BMP) {
         // BMPLog.C:166-187: (Phi G0) V (inv(Lam) - I) V^T with the step's fluidity
         const double dd[3] = {il[0] - 1.0, il[1] - 1.0, il[2] - 1.0};
@@ -295,8 +329,11 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3, 8))
     T* rOut = static_cast<T*>(a.r);
     T* r0Out = static_cast<T*>(a.r0);
     for (int64_t it = 0;; ++it) {
-        const int64_t tl = tile_of(it, a.N);
-        if (tl < 0) break;
+        const int64_t tl = tile_of(it, a.N, a.nRed);
+        if (tl < 0) {
+            if (tl == -1) break;
+            continue;
+        }
         const int64_t t0 = tl * BLK;
         const int32_t r = (int32_t)(t0 + threadIdx.x);
         if (r >= a.N) continue;
@@ -650,6 +687,8 @@ __global__ void __launch_bounds__(BLK) k_eig_tau(int32_t N, ModelParams p, const
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         lam[q] = exp(ev[q]);
+        // Oldroyd_BRootk.C:194-200: the conformation eigenvalues log(exp(.))^k
+        if (p.kind == M_ROOTK) lam[q] = pow(log(lam[q]), p.rootK);
         eigVal[q * (int64_t)N + r] = lam[q];
     }
 #pragma unroll
@@ -669,7 +708,7 @@ __global__ void __launch_bounds__(BLK) k_eig_tau(int32_t N, ModelParams p, const
     } else {
 #pragma unroll
         for (int q = 0; q < 9; ++q) D[q] = C[q] - I9[q];
-        if (p.kind == M_PTT_LINEAR || p.kind == M_PTT_EXP || p.kind == M_SARAMITO) {
+        if (p.kind == M_PTT_LINEAR || p.kind == M_PTT_EXP || p.kind == M_PTT_GEN || p.kind == M_SARAMITO) {
             coef = p.etaP / (p.lambda * (1 - p.zeta));  // PTTLog.C, SaramitoLog.C:292
         } else if (p.kind == M_FENECR) {  // FENE_CRLog.C:176, f of the old state
             coef = (p.etaP / p.lambda) * fOld[r];
@@ -822,8 +861,11 @@ static inline unsigned nblk(int64_t n) { return (unsigned)((n + BLK - 1) / BLK);
 __global__ void __launch_bounds__(BLK) k_grad_theta(AssembleArgs a, double* __restrict__ gradTh) {
     const int64_t NB = a.NB > 0 ? a.NB : 1;
     for (int64_t it = 0;; ++it) {
-        const int64_t tl = tile_of(it, a.N);
-        if (tl < 0) break;
+        const int64_t tl = tile_of(it, a.N, a.nRed);
+        if (tl < 0) {
+            if (tl == -1) break;
+            continue;
+        }
         const int32_t r = (int32_t)(tl * BLK + threadIdx.x);
         if (r >= a.N) continue;
         double th[6];
@@ -901,8 +943,11 @@ __global__ void __launch_bounds__(BLK) k_defc(AssembleArgs a, const double* __re
                                               const double* __restrict__ gradTh, double* __restrict__ souT) {
     using L = Limiter<S>;
     for (int64_t it = 0;; ++it) {
-        const int64_t tl = tile_of(it, a.N);
-        if (tl < 0) break;
+        const int64_t tl = tile_of(it, a.N, a.nRed);
+        if (tl < 0) {
+            if (tl == -1) break;
+            continue;
+        }
         const int32_t r = (int32_t)(tl * BLK + threadIdx.x);
         if (r >= a.N) continue;
         double th[6];
@@ -1014,8 +1059,11 @@ __global__ void __launch_bounds__(BLK) k_div_tau(AssembleArgs a, const double* _
                                                  const double* __restrict__ bDc, double* __restrict__ out) {
     const int64_t NB = a.NB > 0 ? a.NB : 1;
     for (int64_t it = 0;; ++it) {
-        const int64_t tl = tile_of(it, a.N);
-        if (tl < 0) break;
+        const int64_t tl = tile_of(it, a.N, a.nRed);
+        if (tl < 0) {
+            if (tl == -1) break;
+            continue;
+        }
         const int32_t r = (int32_t)(tl * BLK + threadIdx.x);
         if (r >= a.N) continue;
         double t[6];
@@ -1191,8 +1239,11 @@ __device__ __forceinline__ void gauss_grad_u(const AssembleArgs& a, int32_t r, d
 __global__ void __launch_bounds__(BLK) k_tau_total(AssembleArgs a, const double* __restrict__ tau, double etaS,
                                                    double* __restrict__ out) {
     for (int64_t it = 0;; ++it) {
-        const int64_t tl = tile_of(it, a.N);
-        if (tl < 0) break;
+        const int64_t tl = tile_of(it, a.N, a.nRed);
+        if (tl < 0) {
+            if (tl == -1) break;
+            continue;
+        }
         const int32_t r = (int32_t)(tl * BLK + threadIdx.x);
         if (r >= a.N) continue;
         double g[9];
@@ -1217,8 +1268,11 @@ __global__ void __launch_bounds__(BLK) k_tau_total(AssembleArgs a, const double*
 // terms of divTau interpolate to the faces)
 __global__ void __launch_bounds__(BLK) k_grad_u(AssembleArgs a, double* __restrict__ G) {
     for (int64_t it = 0;; ++it) {
-        const int64_t tl = tile_of(it, a.N);
-        if (tl < 0) break;
+        const int64_t tl = tile_of(it, a.N, a.nRed);
+        if (tl < 0) {
+            if (tl == -1) break;
+            continue;
+        }
         const int32_t r = (int32_t)(tl * BLK + threadIdx.x);
         if (r >= a.N) continue;
         double g[9];
@@ -1270,6 +1324,12 @@ static void launch_assemble_kind(Ctx* ctx, const AssembleArgs& a, int grid) {
         break;
     case M_PTT_EXP:
         hipLaunchKernelGGL((k_assemble<T, M_PTT_EXP>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
+        break;
+    case M_PTT_GEN:
+        hipLaunchKernelGGL((k_assemble<T, M_PTT_GEN>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
+        break;
+    case M_ROOTK:
+        hipLaunchKernelGGL((k_assemble<T, M_ROOTK>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
         break;
     case M_FENECR:
         hipLaunchKernelGGL((k_assemble<T, M_FENECR>), dim3(grid), dim3(BLK), 0, ctx->stream, a);

@@ -54,6 +54,8 @@ struct KryArgs {
     int32_t valid[6];
     KryCtl ctl;
     int32_t G;           // workgroups of the row kernels (partials rows)
+    int32_t nRed;        // colour-blocked rows: the first colour's count (0: not coloured)
+    int32_t rowBegin;    // k_residual: rows [rowBegin, N) (the red-black sweep's second colour)
 };
 
 // ---- true residual r = b - A x (FP64): MODE 0 initial, MODE 1 refinement ----
@@ -78,7 +80,7 @@ __device__ __forceinline__ bool inner_running(const KryArgs<T>& a) {
 }
 
 template <class T, int MODE>
-__global__ void __launch_bounds__(BLK) k_residual(KryArgs<T> a) {
+__device__ __forceinline__ void k_residual_body(const KryArgs<T>& a, int gG, int gB) {
     constexpr int NQ = 18;
     bool on[6];
     bool any = false;
@@ -94,11 +96,17 @@ __global__ void __launch_bounds__(BLK) k_residual(KryArgs<T> a) {
     double xbar[6];
 #pragma unroll
     for (int c = 0; c < 6; ++c) xbar[c] = MODE == 0 ? a.cs[c].xbar : 0.0;
+    // rows [rowBegin, N): all rows, or the second colour of a red-black sweep
+    // (the first colour's residual is zero once it has been eliminated)
+    const int64_t nRows = (int64_t)a.N - a.rowBegin;
     for (int64_t it = 0;; ++it) {
-        const int64_t tl = tile_of(it, a.N);
-        if (tl < 0) break;
+        const int64_t tl = a.rowBegin > 0 ? tile_of_g(it, nRows, 0, gG, gB) : tile_of_g(it, a.N, a.nRed, gG, gB);
+        if (tl < 0) {
+            if (tl == -1) break;
+            continue;
+        }
         const int64_t t0 = tl * BLK;
-        const int32_t r = (int32_t)(t0 + threadIdx.x);
+        const int32_t r = (int32_t)(a.rowBegin + t0 + threadIdx.x);
         if (r >= a.N) continue;
         const double d = a.diag[r];
         double y[6], xv[6];
@@ -147,11 +155,16 @@ __global__ void __launch_bounds__(BLK) k_residual(KryArgs<T> a) {
         st6(a.r + (int64_t)NCMP * r, rt);
         st6(a.r0 + (int64_t)NCMP * r, rt);
     }
-    write_partials<NQ>(q, a.partials);
+    write_partials<NQ>(q, a.partials, gB);
 }
 
 template <class T, int MODE>
-__global__ void __launch_bounds__(BLK) k_fin_residual(KryArgs<T> a) {
+__global__ void __launch_bounds__(BLK) k_residual(KryArgs<T> a) {
+    k_residual_body<T, MODE>(a, gridDim.x, blockIdx.x);
+}
+
+template <class T, int MODE>
+__device__ __forceinline__ void k_fin_residual_body(const KryArgs<T>& a, int gG, int gB) {
     constexpr int NQ = 18;
     bool on[6];
     bool any = false;
@@ -171,6 +184,7 @@ __global__ void __launch_bounds__(BLK) k_fin_residual(KryArgs<T> a) {
                 finish_initial(cs, a.ctl, sums[c], sums[6 + c], sums[12 + c]);
             } else {
                 cs.needRefine = 0;
+                cs.foldE = 0;
                 cs.nRefine += 1;
                 cs.finalRes = sums[c] / cs.normFactor;
                 if (conv_outer(cs, a.ctl, cs.finalRes) && cs.iter >= a.ctl.minIter) {
@@ -186,9 +200,14 @@ __global__ void __launch_bounds__(BLK) k_fin_residual(KryArgs<T> a) {
     }
 }
 
+template <class T, int MODE>
+__global__ void __launch_bounds__(BLK) k_fin_residual(KryArgs<T> a) {
+    k_fin_residual_body<T, MODE>(a, gridDim.x, blockIdx.x);
+}
+
 // ---- p = r  |  p = r + beta (p - omega v) ---------------------------------
 template <class T>
-__global__ void __launch_bounds__(BLK) k_kry_p(KryArgs<T> a) {
+__device__ __forceinline__ void k_kry_p_body(const KryArgs<T>& a, int gG, int gB) {
     bool on[6];
     bool first[6];
     T beta[6], omega[6];
@@ -207,14 +226,14 @@ __global__ void __launch_bounds__(BLK) k_kry_p(KryArgs<T> a) {
     if (!update) {
         // every active component starts an inner solve: p = r.  The other
         // components' p is dead until their own restart sets it.
-        for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < N; i += (int64_t)gridDim.x * BLK) {
+        for (int64_t i = (int64_t)gB * BLK + threadIdx.x; i < N; i += (int64_t)gG * BLK) {
             T rr[6];
             ld6(a.r + NCMP * i, rr);
             st6(a.p + NCMP * i, rr);
         }
         return;
     }
-    for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < N; i += (int64_t)gridDim.x * BLK) {
+    for (int64_t i = (int64_t)gB * BLK + threadIdx.x; i < N; i += (int64_t)gG * BLK) {
         T rr[6], pp[6], vv[6];
         ld6(a.r + NCMP * i, rr);
         ld6(a.p + NCMP * i, pp);
@@ -226,9 +245,14 @@ __global__ void __launch_bounds__(BLK) k_kry_p(KryArgs<T> a) {
     }
 }
 
+template <class T>
+__global__ void __launch_bounds__(BLK) k_kry_p(KryArgs<T> a) {
+    k_kry_p_body<T>(a, gridDim.x, blockIdx.x);
+}
+
 // ---- v = A M^-1 p = p + Ahat p ;  sigma = (r0, v) -------------------------
 template <class T>
-__global__ void __launch_bounds__(BLK) k_kry_v(KryArgs<T> a) {
+__device__ __forceinline__ void k_kry_v_body(const KryArgs<T>& a, int gG, int gB) {
     bool on[6];
     bool any = false;
 #pragma unroll
@@ -239,8 +263,11 @@ __global__ void __launch_bounds__(BLK) k_kry_v(KryArgs<T> a) {
     if (!any) return;
     T q[6] = {0, 0, 0, 0, 0, 0};
     for (int64_t it = 0;; ++it) {
-        const int64_t tl = tile_of(it, a.N);
-        if (tl < 0) break;
+        const int64_t tl = tile_of_g(it, a.N, a.nRed, gG, gB);
+        if (tl < 0) {
+            if (tl == -1) break;
+            continue;
+        }
         const int64_t t0 = tl * BLK;
         const int32_t r = (int32_t)(t0 + threadIdx.x);
         if (r >= a.N) continue;
@@ -276,11 +303,16 @@ __global__ void __launch_bounds__(BLK) k_kry_v(KryArgs<T> a) {
     double qd[6];
 #pragma unroll
     for (int c = 0; c < 6; ++c) qd[c] = q[c];
-    write_partials<6>(qd, a.partials);
+    write_partials<6>(qd, a.partials, gB);
 }
 
 template <class T>
-__global__ void __launch_bounds__(BLK) k_fin_v(KryArgs<T> a) {
+__global__ void __launch_bounds__(BLK) k_kry_v(KryArgs<T> a) {
+    k_kry_v_body<T>(a, gridDim.x, blockIdx.x);
+}
+
+template <class T>
+__device__ __forceinline__ void k_fin_v_body(const KryArgs<T>& a, int gG, int gB) {
     bool on[6];
     bool any = false;
 #pragma unroll
@@ -296,9 +328,14 @@ __global__ void __launch_bounds__(BLK) k_fin_v(KryArgs<T> a) {
             if (on[c]) a.cs[c].alpha = a.cs[c].rho / sums[c];
 }
 
+template <class T>
+__global__ void __launch_bounds__(BLK) k_fin_v(KryArgs<T> a) {
+    k_fin_v_body<T>(a, gridDim.x, blockIdx.x);
+}
+
 // ---- s = r - alpha v ; t = A M^-1 s ; |s|, (t,s), (t,t) --------------------
 template <class T>
-__global__ void __launch_bounds__(BLK) k_kry_t(KryArgs<T> a) {
+__device__ __forceinline__ void k_kry_t_body(const KryArgs<T>& a, int gG, int gB) {
     bool on[6];
     T alpha[6];
     bool any = false;
@@ -313,8 +350,11 @@ __global__ void __launch_bounds__(BLK) k_kry_t(KryArgs<T> a) {
 #pragma unroll
     for (int i = 0; i < 18; ++i) q[i] = 0;
     for (int64_t it = 0;; ++it) {
-        const int64_t tl = tile_of(it, a.N);
-        if (tl < 0) break;
+        const int64_t tl = tile_of_g(it, a.N, a.nRed, gG, gB);
+        if (tl < 0) {
+            if (tl == -1) break;
+            continue;
+        }
         const int64_t t0 = tl * BLK;
         const int32_t r = (int32_t)(t0 + threadIdx.x);
         if (r >= a.N) continue;
@@ -355,11 +395,16 @@ __global__ void __launch_bounds__(BLK) k_kry_t(KryArgs<T> a) {
     double qd[18];
 #pragma unroll
     for (int i = 0; i < 18; ++i) qd[i] = q[i];
-    write_partials<18>(qd, a.partials);
+    write_partials<18>(qd, a.partials, gB);
 }
 
 template <class T>
-__global__ void __launch_bounds__(BLK) k_fin_t(KryArgs<T> a) {
+__global__ void __launch_bounds__(BLK) k_kry_t(KryArgs<T> a) {
+    k_kry_t_body<T>(a, gridDim.x, blockIdx.x);
+}
+
+template <class T>
+__device__ __forceinline__ void k_fin_t_body(const KryArgs<T>& a, int gG, int gB) {
     bool on[6];
     bool any = false;
 #pragma unroll
@@ -385,9 +430,14 @@ __global__ void __launch_bounds__(BLK) k_fin_t(KryArgs<T> a) {
     }
 }
 
+template <class T>
+__global__ void __launch_bounds__(BLK) k_fin_t(KryArgs<T> a) {
+    k_fin_t_body<T>(a, gridDim.x, blockIdx.x);
+}
+
 // ---- x += alpha M^-1 p + omega M^-1 s ; r = s - omega t ; |r|, (r0, r) ------
 template <class T>
-__global__ void __launch_bounds__(BLK) k_kry_xr(KryArgs<T> a) {
+__device__ __forceinline__ void k_kry_xr_body(const KryArgs<T>& a, int gG, int gB) {
     // FP64: x += alpha M^-1 p + omega M^-1 s in place.  Mixed: the inner
     // solve accumulates the same update into its FP32 correction e (from 0 at
     // the solve's first iteration) and k_fold adds e to x in FP64 before the
@@ -410,8 +460,11 @@ __global__ void __launch_bounds__(BLK) k_kry_xr(KryArgs<T> a) {
 #pragma unroll
     for (int i = 0; i < 12; ++i) q[i] = 0;
     for (int64_t it = 0;; ++it) {
-        const int64_t tl = tile_of(it, a.N);
-        if (tl < 0) break;
+        const int64_t tl = tile_of_g(it, a.N, a.nRed, gG, gB);
+        if (tl < 0) {
+            if (tl == -1) break;
+            continue;
+        }
         const int64_t t0 = tl * BLK;
         const int32_t r = (int32_t)(t0 + threadIdx.x);
         if (r >= a.N) continue;
@@ -460,7 +513,12 @@ __global__ void __launch_bounds__(BLK) k_kry_xr(KryArgs<T> a) {
     double qd[12];
 #pragma unroll
     for (int i = 0; i < 12; ++i) qd[i] = q[i];
-    write_partials<12>(qd, a.partials);
+    write_partials<12>(qd, a.partials, gB);
+}
+
+template <class T>
+__global__ void __launch_bounds__(BLK) k_kry_xr(KryArgs<T> a) {
+    k_kry_xr_body<T>(a, gridDim.x, blockIdx.x);
 }
 
 // ---- mixed: x += e (FP64) for the components about to be refined ----------
@@ -468,16 +526,16 @@ __global__ void __launch_bounds__(BLK) k_kry_xr(KryArgs<T> a) {
 // component waiting for refinement); the sweep's finalize then clears the
 // wait, so a correction is added once.
 template <class T>
-__global__ void __launch_bounds__(BLK) k_fold(KryArgs<T> a) {
+__device__ __forceinline__ void k_fold_body(const KryArgs<T>& a, int gG, int gB) {
     bool on[6];
     bool any = false;
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
-        on[c] = a.valid[c] > 0 && a.cs[c].needRefine;
+        on[c] = a.valid[c] > 0 && a.cs[c].needRefine && a.cs[c].foldE;
         any |= on[c];
     }
     if (!any || inner_running(a)) return;
-    for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < a.N; i += (int64_t)gridDim.x * BLK) {
+    for (int64_t i = (int64_t)gB * BLK + threadIdx.x; i < a.N; i += (int64_t)gG * BLK) {
         double xv[6];
         T ev[6];
         ld6(a.x + NCMP * i, xv);
@@ -490,7 +548,12 @@ __global__ void __launch_bounds__(BLK) k_fold(KryArgs<T> a) {
 }
 
 template <class T>
-__global__ void __launch_bounds__(BLK) k_fin_xr(KryArgs<T> a) {
+__global__ void __launch_bounds__(BLK) k_fold(KryArgs<T> a) {
+    k_fold_body<T>(a, gridDim.x, blockIdx.x);
+}
+
+template <class T>
+__device__ __forceinline__ void k_fin_xr_body(const KryArgs<T>& a, int gG, int gB) {
     bool on[6];
     bool any = false;
 #pragma unroll
@@ -539,14 +602,329 @@ __global__ void __launch_bounds__(BLK) k_fin_xr(KryArgs<T> a) {
                 // DILU preconditioner meets breakdowns in other places than
                 // right Jacobi does, so stopping would leave results that differ
                 // from it by more than the tolerance.
-                if (a.ctl.mixed || breakdown) {
+                if (a.ctl.mixed || breakdown || a.ctl.rb) {
+                    // mixed: fold the inner solve's correction, then refine;
+                    // red-black: the closing sweep eliminates the first colour
                     cs.needRefine = 1;
+                    cs.foldE = a.ctl.mixed && cs.innerIter > 0 ? 1 : 0;
                 } else {
                     cs.converged = conv_outer(cs, a.ctl, cs.finalRes);
                 }
             }
         }
     }
+}
+
+template <class T>
+__global__ void __launch_bounds__(BLK) k_fin_xr(KryArgs<T> a) {
+    k_fin_xr_body<T>(a, gridDim.x, blockIdx.x);
+}
+
+// ================================== red-black ILU0 (multicolour ILU0, two colours)
+// On a two-colourable cell graph stored colour by colour (rows [0, nRed) =: R,
+// [nRed, N) =: B; A_RR = D_R and A_BB = D_B are diagonal), ILU(0) in this
+// ordering factors A exactly except for the pivots of B, which become
+// Ds = diag(S) of the Schur complement S = D_B - A_BR D_R^-1 A_RB (D_B itself
+// when every product A_ij A_ji vanishes, as for the upwind theta matrix).
+// ILU(0)-preconditioned Krylov is the same method as Krylov on S with the
+// Jacobi preconditioner Ds and R eliminated exactly,
+// x_R = D_R^-1 (b_R - A_RB x_B): the residual of the whole system is then zero
+// on R and S's residual on B.  So the lockstep PBiCGStab runs on B's rows with
+// right preconditioning M = Ds, each product S M^-1 y in two passes:
+//   R rows gather  w   = D_R^-1 A_RB Ds^-1 y   (ahat = A_jk / (D_j Ds_k) there)
+//   B rows gather  S M^-1 y = (D_B Ds^-1) y - A_BR w   (ahat = A_ij there).
+// r, r0, p, v, t live on B's rows; w uses R's rows of v, w' R's rows of t.
+// Every solve opens with a sweep (x_R from x_B, then the true residual on B)
+// and closes with one, so x_R is always the exact elimination of x_B.
+
+// R rows: w = D_R^-1 A_RB Ds^-1 y, y = p (SRC 0) or s = r - alpha v (SRC 1)
+template <class T, int SRC>
+__device__ __forceinline__ void k_rb_w_body(const KryArgs<T>& a, int gG, int gB) {
+    T* __restrict__ w = SRC == 0 ? a.v : a.t;
+    T alpha[6];
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        alpha[c] = (T)a.cs[c].alpha;
+        any |= a.cs[c].active != 0;
+    }
+    if (!any) return;
+    const int64_t n = a.nRed;
+    for (int64_t it = 0;; ++it) {
+        const int64_t tl = tile_of_g(it, n, 0, gG, gB);
+        if (tl < 0) {
+            if (tl == -1) break;
+            continue;
+        }
+        const int32_t r = (int32_t)(tl * BLK + threadIdx.x);
+        if (r >= n) continue;
+        T acc[6] = {0, 0, 0, 0, 0, 0};
+        const int s = r >> 6, lane = r & 63;
+        const int64_t base = a.ellOff[s] + lane;
+        const int width = a.ellWidth[s];
+        const uint32_t live = a.slotMask ? a.slotMask[s] : 0xffffffffu;
+        for (int k = 0; k < width; ++k) {
+            if (k < 32 && !((live >> k) & 1u)) continue;  // zero in every row of the slice
+            const int64_t e = base + (int64_t)SLICE * k;
+            const int32_t col = ld_stream<T>(a.ellCol + e);
+            const T av = ld_stream<T>(a.ahat + e);
+            if (av == (T)0) continue;
+            if constexpr (SRC == 0) {
+                T pc[6];
+                ld6(a.p + (int64_t)NCMP * col, pc);
+#pragma unroll
+                for (int c = 0; c < 6; ++c) acc[c] += av * pc[c];
+            } else {
+                T rc[6], vc[6];
+                ld6(a.r + (int64_t)NCMP * col, rc);
+                ld6(a.v + (int64_t)NCMP * col, vc);
+#pragma unroll
+                for (int c = 0; c < 6; ++c) acc[c] += av * (rc[c] - alpha[c] * vc[c]);
+            }
+        }
+        st6(w + (int64_t)NCMP * r, acc);
+    }
+}
+
+template <class T, int SRC>
+__global__ void __launch_bounds__(BLK) k_rb_w(KryArgs<T> a) {
+    k_rb_w_body<T, SRC>(a, gridDim.x, blockIdx.x);
+}
+
+// B rows: v = S M^-1 p = (D_B Ds^-1) p - A_BR w ;  sigma = (r0, v)
+template <class T>
+__device__ __forceinline__ void k_rb_v_body(const KryArgs<T>& a, int gG, int gB) {
+    const T* __restrict__ w = a.v;
+    bool on[6];
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        on[c] = a.cs[c].active;
+        any |= on[c];
+    }
+    if (!any) return;
+    T q[6] = {0, 0, 0, 0, 0, 0};
+    const int64_t n = (int64_t)a.N - a.nRed;
+    for (int64_t it = 0;; ++it) {
+        const int64_t tl = tile_of_g(it, n, 0, gG, gB);
+        if (tl < 0) {
+            if (tl == -1) break;
+            continue;
+        }
+        const int32_t r = (int32_t)(a.nRed + tl * BLK + threadIdx.x);
+        if (r >= a.N) continue;
+        T acc[6];
+        ld6(a.p + (int64_t)NCMP * r, acc);
+        if (a.dg) {
+            const T dgr = (T)a.dg[r];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) acc[c] = dgr * acc[c];
+        }
+        const int s = r >> 6, lane = r & 63;
+        const int64_t base = a.ellOff[s] + lane;
+        const int width = a.ellWidth[s];
+        const uint32_t live = a.slotMask ? a.slotMask[s] : 0xffffffffu;
+        for (int k = 0; k < width; ++k) {
+            if (k < 32 && !((live >> k) & 1u)) continue;
+            const int64_t e = base + (int64_t)SLICE * k;
+            const int32_t col = ld_stream<T>(a.ellCol + e);
+            const T av = ld_stream<T>(a.ahat + e);
+            if (av == (T)0) continue;
+            T wc[6];
+            ld6(w + (int64_t)NCMP * col, wc);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) acc[c] -= av * wc[c];
+        }
+        st6(a.v + (int64_t)NCMP * r, acc);
+        T rr0[6];
+        ld6(a.r0 + (int64_t)NCMP * r, rr0);
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+            if (on[c]) q[c] += rr0[c] * acc[c];
+    }
+    double qd[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) qd[c] = q[c];
+    write_partials<6>(qd, a.partials, gB);
+}
+
+template <class T>
+__global__ void __launch_bounds__(BLK) k_rb_v(KryArgs<T> a) {
+    k_rb_v_body<T>(a, gridDim.x, blockIdx.x);
+}
+
+// B rows: s = r - alpha v ; t = S M^-1 s = (D_B Ds^-1) s - A_BR w' ;
+// |s|, (t, s), (t, t)
+template <class T>
+__device__ __forceinline__ void k_rb_t_body(const KryArgs<T>& a, int gG, int gB) {
+    const T* __restrict__ w = a.t;
+    bool on[6];
+    T alpha[6];
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        on[c] = a.cs[c].active;
+        alpha[c] = (T)a.cs[c].alpha;
+        any |= on[c];
+    }
+    if (!any) return;
+    T q[18];
+#pragma unroll
+    for (int i = 0; i < 18; ++i) q[i] = 0;
+    const int64_t n = (int64_t)a.N - a.nRed;
+    for (int64_t it = 0;; ++it) {
+        const int64_t tl = tile_of_g(it, n, 0, gG, gB);
+        if (tl < 0) {
+            if (tl == -1) break;
+            continue;
+        }
+        const int32_t r = (int32_t)(a.nRed + tl * BLK + threadIdx.x);
+        if (r >= a.N) continue;
+        T sv[6], acc[6], rr[6], vv[6];
+        ld6(a.r + (int64_t)NCMP * r, rr);
+        ld6(a.v + (int64_t)NCMP * r, vv);
+        const T dgr = a.dg ? (T)a.dg[r] : (T)1;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            sv[c] = rr[c] - alpha[c] * vv[c];
+            acc[c] = a.dg ? dgr * sv[c] : sv[c];
+        }
+        const int s = r >> 6, lane = r & 63;
+        const int64_t base = a.ellOff[s] + lane;
+        const int width = a.ellWidth[s];
+        const uint32_t live = a.slotMask ? a.slotMask[s] : 0xffffffffu;
+        for (int k = 0; k < width; ++k) {
+            if (k < 32 && !((live >> k) & 1u)) continue;
+            const int64_t e = base + (int64_t)SLICE * k;
+            const int32_t col = ld_stream<T>(a.ellCol + e);
+            const T av = ld_stream<T>(a.ahat + e);
+            if (av == (T)0) continue;
+            T wc[6];
+            ld6(w + (int64_t)NCMP * col, wc);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) acc[c] -= av * wc[c];
+        }
+        st6(a.t + (int64_t)NCMP * r, acc);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            if (!on[c]) continue;
+            q[c] += fabs(sv[c]);
+            q[6 + c] += acc[c] * sv[c];
+            q[12 + c] += acc[c] * acc[c];
+        }
+    }
+    double qd[18];
+#pragma unroll
+    for (int i = 0; i < 18; ++i) qd[i] = q[i];
+    write_partials<18>(qd, a.partials, gB);
+}
+
+template <class T>
+__global__ void __launch_bounds__(BLK) k_rb_t(KryArgs<T> a) {
+    k_rb_t_body<T>(a, gridDim.x, blockIdx.x);
+}
+
+// The sweep's elimination of R (FP64): x_R = D_R^-1 (b_R - A_RB x_B) for the
+// components about to be refined; gated like the refinement sweep it opens.
+template <class T>
+__device__ __forceinline__ void k_rb_backsub_body(const KryArgs<T>& a, int gG, int gB) {
+    bool on[6];
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        on[c] = a.valid[c] > 0 && a.cs[c].needRefine;
+        any |= on[c];
+    }
+    if (!any || inner_running(a)) return;
+    const int64_t n = a.nRed;
+    for (int64_t it = 0;; ++it) {
+        const int64_t tl = tile_of_g(it, n, 0, gG, gB);
+        if (tl < 0) {
+            if (tl == -1) break;
+            continue;
+        }
+        const int32_t r = (int32_t)(tl * BLK + threadIdx.x);
+        if (r >= n) continue;
+        double acc[6];
+        ld6(a.b + (int64_t)NCMP * r, acc);
+        const int s = r >> 6, lane = r & 63;
+        const int64_t base = a.ellOff[s] + lane;
+        const int width = a.ellWidth[s];
+        const uint32_t live = a.slotMask ? a.slotMask[s] : 0xffffffffu;
+        for (int k = 0; k < width; ++k) {
+            if (k < 32 && !((live >> k) & 1u)) continue;
+            const int64_t e = base + (int64_t)SLICE * k;
+            const int32_t col = a.ellCol[e];
+            const double av = a.aval[e];
+            if (av == 0.0) continue;
+            double xc[6];
+            ld6(a.x + (int64_t)NCMP * col, xc);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) acc[c] -= av * xc[c];
+        }
+        double xv[6];
+        ld6(a.x + (int64_t)NCMP * r, xv);
+        const double rd = a.rD[r];
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+            if (on[c]) xv[c] = rd * acc[c];
+        st6(a.x + (int64_t)NCMP * r, xv);
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(BLK) k_rb_backsub(KryArgs<T> a) {
+    k_rb_backsub_body<T>(a, gridDim.x, blockIdx.x);
+}
+
+// ahat of the red-black solve: A_jk / (D_j Ds_k) on R's rows, A_ij on B's
+template <class T>
+__global__ void k_rb_scale(int32_t N, int32_t nRed, const int64_t* __restrict__ off,
+                           const int32_t* __restrict__ width, const uint32_t* __restrict__ slotMask,
+                           const int32_t* __restrict__ col, const double* __restrict__ aval,
+                           const double* __restrict__ rD, const double* __restrict__ rDs, T* __restrict__ ahat) {
+    const int64_t r = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (r >= N) return;
+    const int s = (int)(r >> 6), lane = (int)(r & 63);
+    const int64_t base = off[s] + lane;
+    const int w = width[s];
+    const uint32_t live = slotMask ? slotMask[s] : 0xffffffffu;
+    const bool red = r < nRed;
+    const double rdr = red ? rD[r] : 1.0;
+    for (int k = 0; k < w; ++k) {
+        if (k < 32 && !((live >> k) & 1u)) continue;
+        const int64_t e = base + (int64_t)SLICE * k;
+        ahat[e] = red ? (T)((aval[e] * rdr) * rDs[col[e]]) : (T)aval[e];
+    }
+}
+
+// Pivots of B for a general LDU matrix (the sparseSolver surface):
+// Ds_i = D_i - sum over i's faces of lower_f upper_f / D_other, in face order;
+// out[0, N) = 1/Ds (1/D on R), out[N, 2N) = D Ds^-1 (1 on R)
+__global__ void k_rb_schur_diag(int32_t N, int32_t nRed, const int64_t* __restrict__ off,
+                                const int32_t* __restrict__ width, const int32_t* __restrict__ fid,
+                                const uint8_t* __restrict__ slot, const int32_t* __restrict__ col,
+                                const double* __restrict__ lower, const double* __restrict__ upper,
+                                const double* __restrict__ diag, double* __restrict__ out) {
+    const int64_t r = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (r >= N) return;
+    const double D = diag[r];
+    double ds = D;
+    if (r >= nRed) {
+        const int s = (int)(r >> 6), lane = (int)(r & 63);
+        const int64_t base = off[s] + lane;
+        const int w = width[s];
+        for (int k = 0; k < w; ++k) {
+            const int64_t e = base + (int64_t)SLICE * k;
+            const int32_t f = fid[e];
+            if (f == RT_FID_PAD) continue;
+            const int32_t face = f >= 0 ? f : -f - 1;
+            const int32_t other = col[base + (int64_t)SLICE * slot[e]];
+            ds = ds - (lower[face] * upper[face]) / diag[other];
+        }
+    }
+    out[r] = 1.0 / ds;
+    out[N + r] = D / ds;
 }
 
 // ================================================= PBiCG ([OF9] PBiCG::solve)
@@ -572,8 +950,11 @@ __global__ void __launch_bounds__(BLK) k_bicg_init(KryArgs<T> a) {
     if (!any) return;
     double q[6] = {0, 0, 0, 0, 0, 0};
     for (int64_t it = 0;; ++it) {
-        const int64_t tl = tile_of(it, a.N);
-        if (tl < 0) break;
+        const int64_t tl = tile_of(it, a.N, a.nRed);
+        if (tl < 0) {
+            if (tl == -1) break;
+            continue;
+        }
         const int32_t r = (int32_t)(tl * BLK + threadIdx.x);
         if (r >= a.N) continue;
         const double d = a.diag[r];
@@ -672,8 +1053,11 @@ __global__ void __launch_bounds__(BLK) k_bicg_w(KryArgs<T> a) {
     if (!any) return;
     T q[6] = {0, 0, 0, 0, 0, 0};
     for (int64_t it = 0;; ++it) {
-        const int64_t tl = tile_of(it, a.N);
-        if (tl < 0) break;
+        const int64_t tl = tile_of(it, a.N, a.nRed);
+        if (tl < 0) {
+            if (tl == -1) break;
+            continue;
+        }
         const int32_t r = (int32_t)(tl * BLK + threadIdx.x);
         if (r >= a.N) continue;
         const T d = (T)a.diag[r];
@@ -741,6 +1125,8 @@ __global__ void __launch_bounds__(BLK) k_fin_bicg_w(KryArgs<T> a) {
             if (fabs(sums[c]) / cs.normFactor < 1e-300) {
                 cs.active = 0;
                 cs.needRefine = 1;
+                // a correction to fold exists once this inner solve has updated e
+                cs.foldE = a.ctl.mixed && cs.innerIter > 0 ? 1 : 0;
             } else {
                 cs.alpha = cs.rho / sums[c];
             }
@@ -766,8 +1152,11 @@ __global__ void __launch_bounds__(BLK) k_bicg_xr(KryArgs<T> a) {
 #pragma unroll
     for (int i = 0; i < 12; ++i) q[i] = 0;
     for (int64_t it = 0;; ++it) {
-        const int64_t tl = tile_of(it, a.N);
-        if (tl < 0) break;
+        const int64_t tl = tile_of(it, a.N, a.nRed);
+        if (tl < 0) {
+            if (tl == -1) break;
+            continue;
+        }
         const int32_t r = (int32_t)(tl * BLK + threadIdx.x);
         if (r >= a.N) continue;
         const int64_t j = (int64_t)NCMP * r;
@@ -847,8 +1236,12 @@ __global__ void __launch_bounds__(BLK) k_fin_bicg_xr(KryArgs<T> a) {
             }
             if (!cont) {
                 cs.active = 0;
-                if (a.ctl.mixed || breakdown) cs.needRefine = 1;
-                else cs.converged = conv_outer(cs, a.ctl, cs.finalRes);
+                if (a.ctl.mixed || breakdown) {
+                    cs.needRefine = 1;
+                    cs.foldE = a.ctl.mixed && cs.innerIter > 0 ? 1 : 0;
+                } else {
+                    cs.converged = conv_outer(cs, a.ctl, cs.finalRes);
+                }
             }
         }
     }
@@ -890,7 +1283,10 @@ __global__ void __launch_bounds__(BLK) k_mean6(int32_t N, const double* __restri
     double q[6] = {0, 0, 0, 0, 0, 0};
     for (int64_t it = 0;; ++it) {
         const int64_t tl = tile_of(it, N);
-        if (tl < 0) break;
+        if (tl < 0) {
+            if (tl == -1) break;
+            continue;
+        }
         const int64_t t0 = tl * BLK;
         const int64_t r = t0 + threadIdx.x;
         if (r >= N) continue;
@@ -912,6 +1308,16 @@ __global__ void __launch_bounds__(BLK) k_fin_mean6(int32_t N, int G, const doubl
 // ----------------------------------------------------------------- driver
 static inline unsigned kblk(int64_t n) { return (unsigned)((n + BLK - 1) / BLK); }
 
+void launch_rb_schur_diag(Ctx* ctx, const Mesh& m, const double* lower, const double* upper, const double* diag,
+                          double* out) {
+    if (m.N <= 0) return;
+    ProfScope ps(ctx, "k_rb_schur_diag", (double)m.N * 32 + (double)m.nEll * 9 + (double)m.F * 16);
+    hipLaunchKernelGGL(k_rb_schur_diag, dim3(kblk(m.N)), dim3(BLK), 0, ctx->stream, m.N, m.nRed,
+                       m.ellOff.as<int64_t>(), m.ellWidth.as<int32_t>(), m.ellFid.as<int32_t>(),
+                       m.ellSlot.as<uint8_t>(), m.ellCol.as<int32_t>(), lower, upper, diag, out);
+    RT_HIP(hipGetLastError());
+}
+
 void launch_mean6(Ctx* ctx, int32_t N, const double* x, double* partials, CompState* cs,
                   int grid) {
     {
@@ -922,7 +1328,7 @@ void launch_mean6(Ctx* ctx, int32_t N, const double* x, double* partials, CompSt
     RT_HIP(hipGetLastError());
 }
 
-KryCtl kryCtl(const Controls& C, bool mixed) {
+KryCtl kryCtl(const Controls& C, bool mixed, const Mesh& m) {
     KryCtl k{};
     k.tol = C.tolerance;
     k.relTol = C.relTol;
@@ -934,6 +1340,18 @@ KryCtl kryCtl(const Controls& C, bool mixed) {
     k.mixed = mixed ? 1 : 0;
     k.bicg = C.solver == SV_PBICG ? 1 : 0;
     k.precond = C.precond;
+    // multicolour ILU0 runs as the red-black solve on a colour-blocked mesh
+    // with PBiCGStab; the DILU keyword takes it where it is available and the
+    // diagonal preconditioner elsewhere, multicolourILU0 demands it
+    k.rb = C.precond == PC_MCILU0 && m.nRed > 0 && C.solver == SV_PBICGSTAB ? 1 : 0;
+    if (C.precond == PC_MCILU0 && !k.rb) {
+        if (C.precondStrict)
+            throw Error{RT_E_UNSUPPORTED,
+                        std::string("preconditioner multicolourILU0 needs solver PBiCGStab and a mesh whose cell "
+                                    "graph is two-colourable (this one is ") +
+                            (m.nRed > 0 ? "two-coloured; the solver is " + C.solverName : std::string("not")) + ")"};
+        k.precond = PC_JACOBI;
+    }
     return k;
 }
 
@@ -998,6 +1416,12 @@ struct SolveRun {
     bool neFromDevice = false;        // matrix entry count comes with the first status pull
     bool needScale = false;           // Ahat (or the float copies of A, A^T) not built yet
     bool bicg = false;                // PBiCG instead of PBiCGStab
+    bool rb = false;                  // red-black ILU0: Krylov on the second colour's Schur complement
+    double nRed = 0;                  // first colour's rows
+    KryArgs<float> bf{};              // red-black: af/ad shifted to the second colour's rows
+    KryArgs<double> bd{};             // (the pointwise passes p, x/r and the fold)
+    KryArgs<float> sf{};              // red-black: the residual restricted to the second colour
+    KryArgs<double> sd{};
     int64_t nEll = 0;                 // all ELL entries (the float copies of A, A^T)
     hipEvent_t ev = nullptr;
     bool ownsHost = false;            // hostState is a pinned block of ctx
@@ -1019,6 +1443,18 @@ template <>
 KryArgs<float>& argsOf<float>(SolveRun& R) { return R.af; }
 template <>
 KryArgs<double>& argsOf<double>(SolveRun& R) { return R.ad; }
+template <class T>
+KryArgs<T>& blackOf(SolveRun& R);
+template <>
+KryArgs<float>& blackOf<float>(SolveRun& R) { return R.bf; }
+template <>
+KryArgs<double>& blackOf<double>(SolveRun& R) { return R.bd; }
+template <class T>
+KryArgs<T>& sweepOf(SolveRun& R);
+template <>
+KryArgs<float>& sweepOf<float>(SolveRun& R) { return R.sf; }
+template <>
+KryArgs<double>& sweepOf<double>(SolveRun& R) { return R.sd; }
 
 template <class T>
 void fill_args(Ctx* ctx, const SolveArgs& sa, SolveRun& R) {
@@ -1058,13 +1494,40 @@ void fill_args(Ctx* ctx, const SolveArgs& sa, SolveRun& R) {
     a.partials = K.partials.as<double>();
     a.cs = st->c;
     for (int c = 0; c < 6; ++c) a.valid[c] = sa.valid[c];
-    a.ctl = kryCtl(*sa.ctl, mixed);
+    a.ctl = kryCtl(*sa.ctl, mixed, m);
     a.G = K.grid;
+    a.nRed = m.nRed;
+    a.rowBegin = 0;
     // the preconditioner: Jacobi M = D (M^-1 = rD, A M^-1 has a unit diagonal)
-    // or none (M^-1 = 1, diag(A M^-1) = D)
-    const bool jac = sa.ctl->precond == PC_JACOBI;
+    // or none (M^-1 = 1, diag(A M^-1) = D); red-black: M = Ds on the second
+    // colour (M^-1 = 1/Ds, diag(S M^-1) = D Ds^-1, 1 when Ds = D)
+    const bool jac = a.ctl.precond == PC_JACOBI;
     a.rM = jac ? rD : nullptr;
     a.dg = jac ? nullptr : sa.diag;
+    R.rb = a.ctl.rb != 0;
+    R.nRed = m.nRed;
+    if (R.rb) {
+        a.rM = sa.rDs ? sa.rDs : rD;
+        a.dg = sa.rDs ? sa.rDs + N : nullptr;
+        // pointwise passes on the second colour's rows: the same kernels on
+        // pointers shifted by nRed rows
+        const int64_t o = (int64_t)NCMP * m.nRed;
+        KryArgs<T>& b = blackOf<T>(R);
+        b = a;
+        b.N = N - m.nRed;
+        b.nRed = 0;
+        b.x = a.x + o;
+        b.b = a.b + o;
+        b.r += o; b.r0 += o; b.p += o; b.v += o; b.t += o;
+        if (b.e) b.e += o;
+        b.rM = a.rM + m.nRed;
+        b.dg = a.dg ? a.dg + m.nRed : nullptr;
+        b.rD = a.rD + m.nRed;
+        b.diag = a.diag + m.nRed;
+        KryArgs<T>& sw = sweepOf<T>(R);
+        sw = a;
+        sw.rowBegin = m.nRed;
+    }
     R.bicg = sa.ctl->solver == SV_PBICG;
     if (R.bicg) {
         a.atval = sa.atval;
@@ -1091,7 +1554,34 @@ struct Bytes {
     double bp(double nc) const { return N * 8.0 + nc * N * 6 * w; }
     double bw(double nc) const { return N * 8.0 + 2 * ne * (4 + w) + nc * N * 4 * w; }
     double bxr(double nc) const { return N * 8.0 + nc * N * (w == 4 ? 9 * w : 16 + 7 * w); }
+    // red-black (N = rows of the colour the pass runs on, ne its ELL entries):
+    // w gathers one or two vectors of the other colour and writes one; the
+    // second colour's v and t passes gather w and stream their own vectors;
+    // the elimination of the first colour and the sweep's residual in FP64
+    double rbw(double nc, int src) const { return ne * (4 + w) + nc * N * (src ? 3 : 2) * w; }
+    double rbv(double nc) const { return ne * (4 + w) + nc * N * 4 * w; }
+    double rbt(double nc) const { return ne * (4 + w) + nc * N * 5 * w; }
+    double rbx(double nc) const { return N * 8.0 + ne * 12 + nc * N * (8 + 8 + 8); }
 };
+
+// the scaled off-diagonals of the solve (Jacobi: A D^-1; red-black: its two
+// colours' forms), built before the first iteration
+template <class T>
+void launch_scale_of(Ctx* ctx, SolveRun& R) {
+    const bool mixed = std::is_same<T, float>::value;
+    KryArgs<T>& a = argsOf<T>(R);
+    if (R.rb) {
+        ProfScope ps(ctx, mixed ? "k_rb_scale<f32>" : "k_rb_scale<f64>", R.ne * (4 + 8 + sizeof(T)) + R.N * 8);
+        hipLaunchKernelGGL(k_rb_scale<T>, dim3(kblk((int64_t)R.N)), dim3(BLK), 0, ctx->stream, a.N, a.nRed, a.ellOff,
+                           a.ellWidth, a.slotMask, a.ellCol, a.aval, a.rD, a.rM, const_cast<T*>(a.ahat));
+    } else {
+        ProfScope ps(ctx, mixed ? "k_scale<f32>" : "k_scale<f64>", R.ne * (4 + 8 + sizeof(T)) + R.N * 8);
+        hipLaunchKernelGGL(k_scale<T>, dim3(kblk((int64_t)R.N)), dim3(BLK), 0, ctx->stream, a.N, a.ellOff,
+                           a.ellWidth, a.slotMask, a.ellCol, a.aval, a.rM, const_cast<T*>(a.ahat));
+    }
+    RT_HIP(hipGetLastError());
+    R.needScale = false;
+}
 
 template <class T>
 void launch_init(Ctx* ctx, SolveRun& R) {
@@ -1118,12 +1608,35 @@ void launch_bicg_init(Ctx* ctx, SolveRun& R) {
 
 // The refinement sweep: mixed-precision fold, then the FP64 true residual and
 // its finalize (which converges, restarts or stops each waiting component).
+// fold: a component about to be refined holds an inner correction (the
+// blocking path knows; queued rounds launch it gated on the device).
+// Red-black: the fold and the residual run on the second colour's rows and
+// the elimination of the first colour runs between them.
 template <class T>
-void launch_refine(Ctx* ctx, SolveRun& R, double nRef) {
+void launch_refine(Ctx* ctx, SolveRun& R, double nRef, bool fold = true) {
     const bool mixed = std::is_same<T, float>::value;
     Bytes B{R.N, R.ne, (double)sizeof(T)};
     KryArgs<T>& a = argsOf<T>(R);
-    if (mixed) {
+    if (R.rb) {
+        const double w = (double)sizeof(T);
+        Bytes Bb{R.N - R.nRed, 0.5 * R.ne, w}, Br{R.nRed, 0.5 * R.ne, w};
+        const int pg = std::max(1, std::min<int>(R.pgrid, (int)((R.N - R.nRed + BLK - 1) / BLK)));
+        if (mixed && fold) {
+            ProfScope ps(ctx, "k_fold<f32>", Bb.fold(nRef));
+            hipLaunchKernelGGL(k_fold<T>, dim3(pg), dim3(BLK), 0, ctx->stream, blackOf<T>(R));
+        }
+        {
+            ProfScope ps(ctx, mixed ? "k_rb_backsub<f32>" : "k_rb_backsub<f64>", Br.rbx(nRef));
+            hipLaunchKernelGGL(k_rb_backsub<T>, dim3(R.G), dim3(BLK), 0, ctx->stream, a);
+        }
+        {
+            ProfScope ps(ctx, mixed ? "k_residual<f32>" : "k_residual<f64>", Bb.res(nRef));
+            hipLaunchKernelGGL((k_residual<T, 1>), dim3(R.G), dim3(BLK), 0, ctx->stream, sweepOf<T>(R));
+        }
+        hipLaunchKernelGGL((k_fin_residual<T, 1>), dim3(1), dim3(BLK), 0, ctx->stream, a);
+        return;
+    }
+    if (mixed && fold) {
         ProfScope ps(ctx, "k_fold<f32>", B.fold(nRef));
         hipLaunchKernelGGL(k_fold<T>, dim3(R.pgrid), dim3(BLK), 0, ctx->stream, a);
     }
@@ -1164,12 +1677,43 @@ void launch_iter(Ctx* ctx, SolveRun& R, double nAct, bool start) {
         hipLaunchKernelGGL(k_fin_bicg_xr<T>, dim3(1), dim3(BLK), 0, ctx->stream, a);
         return;
     }
-    if (R.needScale) {
-        ProfScope ps(ctx, mixed ? "k_scale<f32>" : "k_scale<f64>", R.ne * (4 + 8 + sizeof(T)) + R.N * 8);
-        hipLaunchKernelGGL(k_scale<T>, dim3(kblk((int64_t)R.N)), dim3(BLK), 0, ctx->stream, a.N, a.ellOff,
-                           a.ellWidth, a.slotMask, a.ellCol, a.aval, a.rM, const_cast<T*>(a.ahat));
-        R.needScale = false;
+    if (R.rb) {
+        if (R.needScale) launch_scale_of<T>(ctx, R);
+        const double w = (double)sizeof(T);
+        Bytes Bb{R.N - R.nRed, 0.5 * R.ne, w}, Br{R.nRed, 0.5 * R.ne, w};
+        KryArgs<T>& b = blackOf<T>(R);
+        const int pg = std::max(1, std::min<int>(R.pgrid, (int)((R.N - R.nRed + BLK - 1) / BLK)));
+        // w lives on the first colour's rows of v, w' on those of t
+        {
+            ProfScope ps(ctx, mixed ? "k_kry_p<f32>" : "k_kry_p<f64>", Bb.p(nAct, start));
+            hipLaunchKernelGGL(k_kry_p<T>, dim3(pg), dim3(BLK), 0, ctx->stream, b);
+        }
+        {
+            ProfScope ps(ctx, mixed ? "k_rb_w<f32>" : "k_rb_w<f64>", Br.rbw(nAct, 0));
+            hipLaunchKernelGGL((k_rb_w<T, 0>), dim3(R.G), dim3(BLK), 0, ctx->stream, a);
+        }
+        {
+            ProfScope ps(ctx, mixed ? "k_rb_v<f32>" : "k_rb_v<f64>", Bb.rbv(nAct));
+            hipLaunchKernelGGL(k_rb_v<T>, dim3(R.G), dim3(BLK), 0, ctx->stream, a);
+        }
+        hipLaunchKernelGGL(k_fin_v<T>, dim3(1), dim3(BLK), 0, ctx->stream, a);
+        {
+            ProfScope ps(ctx, mixed ? "k_rb_ws<f32>" : "k_rb_ws<f64>", Br.rbw(nAct, 1));
+            hipLaunchKernelGGL((k_rb_w<T, 1>), dim3(R.G), dim3(BLK), 0, ctx->stream, a);
+        }
+        {
+            ProfScope ps(ctx, mixed ? "k_rb_t<f32>" : "k_rb_t<f64>", Bb.rbt(nAct));
+            hipLaunchKernelGGL(k_rb_t<T>, dim3(R.G), dim3(BLK), 0, ctx->stream, a);
+        }
+        hipLaunchKernelGGL(k_fin_t<T>, dim3(1), dim3(BLK), 0, ctx->stream, a);
+        {
+            ProfScope ps(ctx, mixed ? "k_kry_xr<f32>" : "k_kry_xr<f64>", Bb.xr(nAct));
+            hipLaunchKernelGGL(k_kry_xr<T>, dim3(R.G), dim3(BLK), 0, ctx->stream, b);
+        }
+        hipLaunchKernelGGL(k_fin_xr<T>, dim3(1), dim3(BLK), 0, ctx->stream, a);
+        return;
     }
+    if (R.needScale) launch_scale_of<T>(ctx, R);
     {
         ProfScope ps(ctx, mixed ? "k_kry_p<f32>" : "k_kry_p<f64>", B.p(nAct, start));
         hipLaunchKernelGGL(k_kry_p<T>, dim3(R.pgrid), dim3(BLK), 0, ctx->stream, a);
@@ -1301,8 +1845,10 @@ bool solve_advance(Ctx* ctx, Krylov& K, bool block) {
             if (R.mixed) launch_iter<float>(ctx, R, nAct, start);
             else launch_iter<double>(ctx, R, nAct, start);
         } else {
-            if (R.mixed) launch_refine<float>(ctx, R, nRef);
-            else launch_refine<double>(ctx, R, nRef);
+            bool fold = false;
+            for (int c = 0; c < 6; ++c) fold |= R.valid[c] > 0 && R.host[c].needRefine && R.host[c].foldE;
+            if (R.mixed) launch_refine<float>(ctx, R, nRef, fold);
+            else launch_refine<double>(ctx, R, nRef, fold);
         }
         ++R.rounds;
     } else {
@@ -1327,6 +1873,230 @@ bool solve_advance(Ctx* ctx, Krylov& K, bool block) {
     RT_HIP(hipGetLastError());
     queue_pull(ctx, K, R);
     return false;
+}
+
+// ---------------------------------------------------------- grouped rounds
+namespace {
+
+// the member of workgroup blk: starts[i] <= blk < starts[i + 1]
+__device__ __forceinline__ int group_of(const int32_t* __restrict__ starts, int n, int blk) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (starts[mid] <= blk) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <class T, void (*Body)(const KryArgs<T>&, int, int)>
+__global__ void __launch_bounds__(BLK) k_group(const KryArgs<T>* __restrict__ tab, const int32_t* __restrict__ starts,
+                                               int n) {
+    const int i = group_of(starts, n, blockIdx.x);
+    const KryArgs<T> a = tab[i];
+    Body(a, starts[i + 1] - starts[i], blockIdx.x - starts[i]);
+}
+
+// device tables of a group: the members' arguments (whole rows; the red-black
+// second colour's rows; its sweep residual) and the workgroup offsets of each
+// launch shape (row kernels, pointwise passes, finalizers)
+template <class T>
+struct GroupDev {
+    const KryArgs<T>* A;
+    const KryArgs<T>* B;
+    const KryArgs<T>* S;
+    const int32_t *G, *P, *One;
+    int n, totG, totP;
+};
+
+template <class T, void (*Body)(const KryArgs<T>&, int, int)>
+void glaunch(hipStream_t st, const KryArgs<T>* tab, const int32_t* starts, int total, int n) {
+    if (total > 0) hipLaunchKernelGGL((k_group<T, Body>), dim3(total), dim3(BLK), 0, st, tab, starts, n);
+}
+
+// One round as solve_advance queues it ahead for one solve: the refinement
+// sweep (every kernel gated on the device) and one iteration.
+template <class T>
+void group_round(bool rb, hipStream_t st, const GroupDev<T>& d) {
+    const int n = d.n;
+    if (rb) {
+        glaunch<T, &k_fold_body<T>>(st, d.B, d.P, d.totP, n);
+        glaunch<T, &k_rb_backsub_body<T>>(st, d.A, d.G, d.totG, n);
+        glaunch<T, &k_residual_body<T, 1>>(st, d.S, d.G, d.totG, n);
+        glaunch<T, &k_fin_residual_body<T, 1>>(st, d.A, d.One, n, n);
+        glaunch<T, &k_kry_p_body<T>>(st, d.B, d.P, d.totP, n);
+        glaunch<T, &k_rb_w_body<T, 0>>(st, d.A, d.G, d.totG, n);
+        glaunch<T, &k_rb_v_body<T>>(st, d.A, d.G, d.totG, n);
+        glaunch<T, &k_fin_v_body<T>>(st, d.A, d.One, n, n);
+        glaunch<T, &k_rb_w_body<T, 1>>(st, d.A, d.G, d.totG, n);
+        glaunch<T, &k_rb_t_body<T>>(st, d.A, d.G, d.totG, n);
+        glaunch<T, &k_fin_t_body<T>>(st, d.A, d.One, n, n);
+        glaunch<T, &k_kry_xr_body<T>>(st, d.B, d.G, d.totG, n);
+        glaunch<T, &k_fin_xr_body<T>>(st, d.A, d.One, n, n);
+    } else {
+        glaunch<T, &k_fold_body<T>>(st, d.A, d.P, d.totP, n);
+        glaunch<T, &k_residual_body<T, 1>>(st, d.A, d.G, d.totG, n);
+        glaunch<T, &k_fin_residual_body<T, 1>>(st, d.A, d.One, n, n);
+        glaunch<T, &k_kry_p_body<T>>(st, d.A, d.P, d.totP, n);
+        glaunch<T, &k_kry_v_body<T>>(st, d.A, d.G, d.totG, n);
+        glaunch<T, &k_fin_v_body<T>>(st, d.A, d.One, n, n);
+        glaunch<T, &k_kry_t_body<T>>(st, d.A, d.G, d.totG, n);
+        glaunch<T, &k_fin_t_body<T>>(st, d.A, d.One, n, n);
+        glaunch<T, &k_kry_xr_body<T>>(st, d.A, d.G, d.totG, n);
+        glaunch<T, &k_fin_xr_body<T>>(st, d.A, d.One, n, n);
+    }
+    RT_HIP(hipGetLastError());
+}
+
+// the workgroups of a member's pointwise passes (p, fold)
+int point_grid(const SolveRun& R) {
+    if (!R.rb) return R.pgrid;
+    return std::max(1, std::min<int>(R.pgrid, (int)((R.N - R.nRed + BLK - 1) / BLK)));
+}
+
+template <class T>
+void group_launch(SolveGroup& g) {
+    const int n = (int)g.members.size();
+    const size_t szA = sizeof(KryArgs<T>) * n, szI = sizeof(int32_t) * (n + 1);
+    const size_t offB = szA, offS = 2 * szA, offG = 3 * szA, offP = offG + szI, offO = offP + szI;
+    const size_t bytes = offO + szI;
+    int totG = 0, totP = 0;
+    if (g.dirty) {
+        if (g.hostBytes < bytes) {
+            if (g.hostTables) (void)hipHostFree(g.hostTables);
+            g.hostTables = nullptr;
+            RT_HIP(hipHostMalloc(&g.hostTables, bytes, hipHostMallocDefault));
+            g.hostBytes = bytes;
+        }
+        if (g.tables.bytes < bytes) g.tables = g.ctx->alloc(bytes);
+        unsigned char* h = static_cast<unsigned char*>(g.hostTables);
+        auto* tA = reinterpret_cast<KryArgs<T>*>(h);
+        auto* tB = reinterpret_cast<KryArgs<T>*>(h + offB);
+        auto* tS = reinterpret_cast<KryArgs<T>*>(h + offS);
+        auto* sG = reinterpret_cast<int32_t*>(h + offG);
+        auto* sP = reinterpret_cast<int32_t*>(h + offP);
+        auto* sO = reinterpret_cast<int32_t*>(h + offO);
+        sG[0] = sP[0] = sO[0] = 0;
+        for (int i = 0; i < n; ++i) {
+            SolveRun& R = *g.members[i].K->run;
+            tA[i] = argsOf<T>(R);
+            tB[i] = g.rb ? blackOf<T>(R) : argsOf<T>(R);
+            tS[i] = g.rb ? sweepOf<T>(R) : argsOf<T>(R);
+            sG[i + 1] = sG[i] + R.G;
+            sP[i + 1] = sP[i] + point_grid(R);
+            sO[i + 1] = i + 1;
+        }
+        // the previous launch's kernels have all finished (their pulls are back)
+        RT_HIP(hipMemcpyAsync(g.tables.p, h, bytes, hipMemcpyHostToDevice, g.ctx->stream));
+        g.dirty = false;
+    }
+    for (auto& mb : g.members) {
+        SolveRun& R = *mb.K->run;
+        totG += R.G;
+        totP += point_grid(R);
+    }
+    const unsigned char* d = static_cast<const unsigned char*>(g.tables.p);
+    GroupDev<T> dv{reinterpret_cast<const KryArgs<T>*>(d), reinterpret_cast<const KryArgs<T>*>(d + offB),
+                   reinterpret_cast<const KryArgs<T>*>(d + offS), reinterpret_cast<const int32_t*>(d + offG),
+                   reinterpret_cast<const int32_t*>(d + offP), reinterpret_cast<const int32_t*>(d + offO),
+                   n, totG, totP};
+    for (int k = 0; k < kRoundsAhead; ++k) group_round<T>(g.rb, g.ctx->stream, dv);
+    for (auto& mb : g.members) {
+        SolveRun& R = *mb.K->run;
+        R.rounds += kRoundsAhead;
+        queue_pull(g.ctx, *mb.K, R);
+    }
+    g.launched = true;
+    ++g.launches;
+}
+
+}  // namespace
+
+SolveGroup::~SolveGroup() {
+    if (hostTables) (void)hipHostFree(hostTables);
+}
+
+bool solve_groupable(const Krylov& K, bool* mixed, bool* rb) {
+    if (!K.run || K.run->bicg || !K.run->ev) return false;
+    if (mixed) *mixed = K.run->mixed;
+    if (rb) *rb = K.run->rb;
+    return true;
+}
+
+void group_join(SolveGroup& g, Krylov& K, hipStream_t home) {
+    SolveRun& R = *K.run;
+    if (R.bicg || R.mixed != g.mixed || R.rb != g.rb || !R.ev)
+        throw Error{RT_E_BAD_ARG, "group_join: the solve does not belong to this group"};
+    // the group's stream waits for the solve's own queued work (the assembly,
+    // the initial residual and the first state pull)
+    RT_HIP(hipStreamWaitEvent(g.ctx->stream, R.ev, 0));
+    if (R.needScale) {  // the scaled matrix its first iteration would build
+        if (R.mixed) launch_scale_of<float>(g.ctx, R);
+        else launch_scale_of<double>(g.ctx, R);
+    }
+    g.members.push_back({&K, home});
+    g.dirty = true;
+}
+
+void group_remove(SolveGroup& g, Krylov& K) {
+    for (size_t i = 0; i < g.members.size(); ++i)
+        if (g.members[i].K == &K) {
+            RT_HIP(hipStreamSynchronize(g.ctx->stream));
+            if (K.run) K.run->pending = false;
+            g.members.erase(g.members.begin() + i);
+            g.dirty = true;
+            return;
+        }
+}
+
+void group_advance(SolveGroup& g, std::vector<Krylov*>& finished) {
+    bool allBack = true;
+    for (auto& mb : g.members) {
+        SolveRun& R = *mb.K->run;
+        if (!R.pending) continue;
+        hipError_t e = hipEventQuery(R.ev);
+        if (e == hipErrorNotReady) {
+            allBack = false;
+            continue;
+        }
+        RT_HIP(e);
+        R.pending = false;
+        if (R.neFromDevice) {
+            R.ne = R.hostState->misc[0];
+            R.neFromDevice = false;
+        }
+    }
+    if (!allBack) return;
+    g.launched = false;
+    // members whose solve has finished leave; their streams go on after the
+    // group's queued work (which reads them) has drained
+    hipEvent_t drained = nullptr;
+    for (size_t i = 0; i < g.members.size();) {
+        SolveRun& R = *g.members[i].K->run;
+        double nAct = 0, nRef = 0;
+        for (int c = 0; c < 6; ++c) {
+            if (R.valid[c] <= 0) continue;
+            nAct += R.host[c].active != 0;
+            nRef += R.host[c].needRefine != 0;
+        }
+        if ((nAct == 0 && nRef == 0) || R.rounds >= 100000) {
+            if (!drained) {
+                RT_HIP(hipEventCreateWithFlags(&drained, hipEventDisableTiming));
+                RT_HIP(hipEventRecord(drained, g.ctx->stream));
+            }
+            RT_HIP(hipStreamWaitEvent(g.members[i].home, drained, 0));
+            R.done = true;
+            finished.push_back(g.members[i].K);
+            g.members.erase(g.members.begin() + i);
+            g.dirty = true;
+        } else {
+            ++i;
+        }
+    }
+    if (drained) RT_HIP(hipEventDestroy(drained));  // released once the waits are satisfied
+    if (g.members.empty()) return;
+    if (g.mixed) group_launch<float>(g);
+    else group_launch<double>(g);
 }
 
 void solve_finish(Krylov& K, rt_solver_perf perf[6]) {

@@ -195,22 +195,24 @@ Controls parseControls(const char* text) {
     // DILU's sweep is sequential; the device replaces it by the diagonal
     // (Jacobi) preconditioner, so iterates differ from the reference's and only
     // converged fields are compared.  diagonal/Jacobi is the same
-    // preconditioner, none is none.  DIC is a symmetric-matrix preconditioner
-    // and the theta matrix is asymmetric ([OF9] rejects it there); the
-    // multicolour ILU0 is not built.  Both are refused rather than replaced.
+    // preconditioner, none is none.  multicolourILU0 is ILU(0) in a two-colour
+    // ordering, run as the red-black Schur-complement solve (a mesh whose cell
+    // graph is two-colourable, solver PBiCGStab; otherwise refused at the
+    // solve).  DIC is a symmetric-matrix preconditioner and the theta matrix
+    // is asymmetric ([OF9] rejects it there).
     if (pc == "DILU" || pc == "diagonal" || pc == "Jacobi") {
         c.precond = PC_JACOBI;
     } else if (pc == "none") {
         c.precond = PC_NONE;
+    } else if (pc == "multicolourILU0") {
+        c.precond = PC_MCILU0;
+        c.precondStrict = true;
     } else if (pc == "DIC" || pc == "FDIC" || pc == "GAMG") {
         throw Error{RT_E_UNKNOWN_TYPE, "Unknown asymmetric matrix preconditioner " + pc +
-                                           "\n\nValid asymmetric matrix preconditioners are :\n4\n(\nDILU\nJacobi\ndiagonal\nnone\n)"};
-    } else if (pc == "multicolourILU0") {
-        throw Error{RT_E_UNSUPPORTED, "preconditioner multicolourILU0 is not available on the device path; "
-                                      "valid: DILU (run as diagonal), diagonal, Jacobi, none"};
+                                           "\n\nValid asymmetric matrix preconditioners are :\n5\n(\nDILU\nJacobi\ndiagonal\nmulticolourILU0\nnone\n)"};
     } else {
         throw Error{RT_E_UNKNOWN_TYPE, "Unknown asymmetric matrix preconditioner " + pc +
-                                           "\n\nValid asymmetric matrix preconditioners are :\n4\n(\nDILU\nJacobi\ndiagonal\nnone\n)"};
+                                           "\n\nValid asymmetric matrix preconditioners are :\n5\n(\nDILU\nJacobi\ndiagonal\nmulticolourILU0\nnone\n)"};
     }
     double v;
     if (e->scalar("tolerance", v)) c.tolerance = v;
@@ -230,9 +232,9 @@ Controls parseControls(const char* text) {
 // ------------------------------------------------------------ model setup
 namespace {
 
-const char* kTypes[] = {"FENE-CRLog", "FENE-PLog",          "GiesekusLog", "Oldroyd-BLog", "PTTLog",
-                        "Rolie-PolyLog", "SaramitoLog", "WhiteMetznerCYLog", "XPomPomLog", "BMPLog",
-                        "multiMode"};
+const char* kTypes[] = {"FENE-CRLog", "FENE-PLog",          "GiesekusLog", "Oldroyd-BLog", "Oldroyd-BRootk",
+                        "PTTLog",     "Rolie-PolyLog", "SaramitoLog", "WhiteMetznerCYLog", "XPomPomLog",
+                        "BMPLog",     "multiMode"};
 
 double need(const Dict& d, const char* key, const std::string& dictName) {
     double v;
@@ -246,6 +248,7 @@ void parseModel(Model& M, const Dict& d, const std::string& dn) {
     if (!d.word("type", type)) throw Error{RT_E_BAD_ARG, "keyword type is undefined in dictionary " + dn};
     ModelParams& p = M.prm;
     if (type == "Oldroyd-BLog") p.kind = M_OLDROYDB;
+    else if (type == "Oldroyd-BRootk") p.kind = M_ROOTK;
     else if (type == "GiesekusLog") p.kind = M_GIESEKUS;
     else if (type == "FENE-PLog") p.kind = M_FENEP;
     else if (type == "PTTLog") p.kind = M_PTT_LINEAR;
@@ -321,6 +324,7 @@ void parseModel(Model& M, const Dict& d, const std::string& dn) {
         p.wmB = need(d, "b", dn);
     }
     if (p.kind == M_GIESEKUS) p.alpha = need(d, "alpha", dn);
+    if (p.kind == M_ROOTK) p.rootK = need(d, "k", dn);  // Oldroyd_BRootk.C:121
     if (p.kind == M_FENEP || p.kind == M_FENECR) p.L2 = need(d, "L2", dn);
     if (p.kind == M_PTT_LINEAR) {
         std::string f;
@@ -328,9 +332,29 @@ void parseModel(Model& M, const Dict& d, const std::string& dn) {
             throw Error{RT_E_BAD_ARG, "keyword destructionFunctionType is undefined in dictionary " + dn};
         if (f == "linear") p.kind = M_PTT_LINEAR;
         else if (f == "exponential") p.kind = M_PTT_EXP;
-        else if (f == "generalized")
-            throw Error{RT_E_UNSUPPORTED, "PTTLog destructionFunctionType generalized is not supported on the device path"};
-        else
+        else if (f == "generalized") {
+            // PTTLog.C:133-152: the Mittag-Leffler function's alpha and beta,
+            // its tolerance and term limit, and the gamma table of the series
+            p.kind = M_PTT_GEN;
+            const double a = need(d, "alpha", dn), b = need(d, "beta", dn);
+            p.alpha = a;
+            p.beta = b;
+            double v;
+            p.mlRtol = d.scalar("rTolMittagLeffler", v) ? v : 1e-12;
+            int maxIter = d.scalar("maxIterMittagLeffler", v) ? (int)v : 200;
+            if (a <= 0 || b <= 0)
+                throw Error{RT_E_BAD_ARG, "Both alpha and beta should be positive values for the Mittag-Leffler "
+                                          "function to converge."};
+            std::vector<double> g{std::tgamma(b)};
+            int k = 0;
+            while (k < maxIter && g.back() < 1e+100) {
+                g.push_back(std::tgamma(a * k + b));
+                k++;
+            }
+            p.mlMaxIter = k;
+            p.mlGammaBeta = g[0];
+            M.mlGammaHost.assign(g.begin() + 1, g.end());  // [j] = gamma(alpha j + beta)
+        } else
             throw Error{RT_E_BAD_ARG, f + " is not in enumeration: 3(linear exponential generalized)"};
     }
     // checkForStab (constitutiveEq.C:430-439): the key is mandatory
@@ -547,7 +571,8 @@ void modelCorrectBegin(Model& M, double deltaT, const double* gradU, int iterati
     a.r = mixed ? M.kry.rs.p : M.kry.r.p;
     a.r0 = mixed ? M.kry.r0s.p : M.kry.r0.p;
     a.rD = M.kry.rD.as<double>();
-    a.ctl = kryCtl(M.ctl, mixed);
+    a.ctl = kryCtl(M.ctl, mixed, m);
+    a.nRed = m.nRed;
     a.mixed = mixed ? 1 : 0;
     a.noConv = M.scheme == DS_NONE ? 1 : 0;
     a.bounded = M.bounded ? 1 : 0;
@@ -577,14 +602,18 @@ void modelCorrectBegin(Model& M, double deltaT, const double* gradU, int iterati
     solve_begin(ctx, sa);
 }
 
-bool modelCorrectAdvance(Model& M, bool block) {
+void modelCorrectFinish(Model& M) {
     Ctx* ctx = M.ctx;
-    if (!solve_advance(ctx, M.kry, block)) return false;
     solve_finish(M.kry, M.perf);
-    if (M.scalar()) return true;  // the fluidity has no eigen state or tau
+    if (M.scalar()) return;  // the fluidity has no eigen state or tau
     launch_eig_tau(ctx, M.mesh->N, M.prm, M.theta.as<double>(), M.eigVal.as<double>(),
                    M.eigVec.as<double>(), M.tau.as<double>(),
                    modelKeepsFactor(M.prm.kind) ? M.fOld.as<double>() : nullptr);
+}
+
+bool modelCorrectAdvance(Model& M, bool block) {
+    if (!solve_advance(M.ctx, M.kry, block)) return false;
+    modelCorrectFinish(M);
     return true;
 }
 
@@ -596,17 +625,22 @@ void modelCorrect(Model& M, double deltaT, const double* gradU, int iteration) {
 }
 
 // device bytes of one model on an existing mesh (state, flow, matrix, Krylov)
-size_t modelBytesEstimate(int32_t N, int32_t F, int32_t NB, int64_t nEll, bool mixed) {
+size_t modelBytesEstimate(int32_t N, int32_t F, int32_t NB, int64_t nEll, bool mixed, bool gradU, bool download) {
     const double n = N, f = F, nb = NB, ne = (double)nEll;
     double by = 0;
-    // model state + flow + matrix: theta tau eigVal eigVec U b diag fOld, Ub, phi, bc, aval
+    // model state + flow + matrix: theta tau eigVal eigVec U b diag fOld, Ub, phi, bc, ePhi, aval
     by += 8 * n * (6 + 6 + 3 + 9 + 4 + 6 + 1 + 1) + 8 * (3 * nb + f + nb) + 6 * 8 * nb + 8 * ne + 8 * ne +
           4 * n / 64;
-    // Krylov: 5 vectors x 6 components, rD, ahat, partials, state
-    by += 6 * n * 5 * (mixed ? 4 : 8) + 8 * n + ne * (mixed ? 4 : 8) + 2048 * 24 * 8 + 4096;
+    // Krylov: r r0 p v t (and the mixed solve's correction e) x 6 components,
+    // rD, ahat, partials, state
+    by += 6 * n * (mixed ? 6 * 4 : 5 * 8) + 8 * n + ne * (mixed ? 4 : 8) + 2048 * 24 * 8 + 4096;
     // staging for AoS transfers (9 comps)
     by += 9 * 8 * n;
-    return (size_t)(by * 1.1) + (1u << 20);
+    if (gradU) by += 9 * 8 * n;      // the prescribed gradient
+    if (download) by += 2 * 6 * 8 * n;  // theta and tau in the caller's order on their way home
+    // size classes round each buffer up by at most 1/8, and the cache hands
+    // out blocks up to 1.5 times a request; 1.25 covers the usual mix
+    return (size_t)(by * 1.25) + (1u << 20);
 }
 
 }  // namespace rt
@@ -909,6 +943,14 @@ void createModel(Model& M, Ctx* c, Mesh* mesh, const std::string& name, const Di
     }
     parseModel(M, d, dn);
     allocModelState(M);
+    if (M.prm.kind == M_PTT_GEN) {
+        M.mlGamma = c->alloc(std::max<size_t>(1, M.mlGammaHost.size()) * 8);
+        if (!M.mlGammaHost.empty())
+            RT_HIP(hipMemcpyAsync(M.mlGamma.p, M.mlGammaHost.data(), M.mlGammaHost.size() * 8,
+                                  hipMemcpyHostToDevice, c->stream));
+        RT_HIP(hipStreamSynchronize(c->stream));
+        M.prm.mlGamma = M.mlGamma.as<double>();
+    }
     if (M.prm.kind == M_BMP) {
         // the fluidity equation Phi<name> (BMPLog.C:98-111, 151-163)
         auto* fl = new rt_model;
@@ -1398,6 +1440,12 @@ int rt_symm_solve(rt_ctx* ctx, const rt_mesh* mesh, const rt_ldu_symm* sys, doub
         sa.ctl = &ctl;
         sa.xbarKnown = true;
         sa.initDone = false;
+        DBuf dS;
+        if (kryCtl(ctl, ctl.mixed, m).rb) {  // the Schur pivots of a general LDU matrix
+            dS = c->alloc((size_t)2 * N * 8);
+            launch_rb_schur_diag(c, m, dLo.as<double>(), dUp.as<double>(), dD.as<double>(), dS.as<double>());
+            sa.rDs = dS.as<double>();
+        }
         solve_lockstep(c, sa, perf);
         launch_unpermute_rows(c, N, 6, 6, dX.as<double>(), m.permD.as<int32_t>(), st.as<double>());
         RT_HIP(hipMemcpyAsync(psi, st.p, (size_t)6 * N * 8, hipMemcpyDeviceToHost, c->stream));

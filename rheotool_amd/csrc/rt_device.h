@@ -13,20 +13,51 @@ constexpr int BLK = 256;
 // the slab together: tile i*(G/8) + blockIdx.x/8 of the slab at step i.  The
 // rows a tile gathers (bandwidth-reduced numbering keeps them within a level
 // of the row) were touched moments before by the same XCD, so they are still
-// in its L2.  Returns -1 once the workgroup's share is done.
-__device__ __forceinline__ int64_t tile_of(int64_t i, int64_t N) {
-    const int G = gridDim.x;
-    const int b = blockIdx.x;
+// in its L2.
+// nRed > 0: the rows are colour-blocked (rows [0, nRed) have neighbours only
+// in [nRed, N) and the other way round).  The tiles of each block are cut
+// into 8 slabs and XCD x walks slab x of both blocks, alternating, so the
+// other colour's rows a tile gathers are the ones the same XCD walks at the
+// same time.
+// Returns the tile, -2 for a step with no tile (a slab shorter than the
+// others), -1 once the workgroup's share is done.
+// tile_of_g: the same for workgroup b of G (a request's share of a grouped launch)
+__device__ __forceinline__ int64_t tile_of_g(int64_t i, int64_t N, int64_t nRed, int G, int b) {
     const int64_t T = (N + BLK - 1) / BLK;
     if (G % 8 != 0) {
         const int64_t t = i * G + b;
         return t < T ? t : -1;
     }
-    const int64_t S = (T + 7) / 8;
     const int64_t local = i * (G / 8) + b / 8;
-    if (local >= S) return -1;
-    const int64_t t = (int64_t)(b % 8) * S + local;
-    return t < T ? t : -1;
+    const int64_t x = b % 8;
+    const int64_t Ts = nRed / BLK;  // tiles [0, Ts) hold first-colour rows only
+    if (nRed <= 0 || Ts == 0 || Ts >= T) {
+        const int64_t S = (T + 7) / 8;
+        if (local >= S) return -1;
+        const int64_t t = x * S + local;
+        return t < T ? t : -1;
+    }
+    const int64_t S1 = (Ts + 7) / 8, S2 = (T - Ts + 7) / 8;
+    if (local >= S1 + S2) return -1;
+    const int64_t m = S1 < S2 ? S1 : S2;
+    bool first;
+    int64_t k;
+    if (local < 2 * m) {
+        first = (local & 1) == 0;
+        k = local >> 1;
+    } else {
+        first = S1 > S2;
+        k = local - m;
+    }
+    if (first) {
+        const int64_t t = x * S1 + k;
+        return t < Ts ? t : -2;
+    }
+    const int64_t t = Ts + x * S2 + k;
+    return t < T ? t : -2;
+}
+__device__ __forceinline__ int64_t tile_of(int64_t i, int64_t N, int64_t nRed = 0) {
+    return tile_of_g(i, N, nRed, gridDim.x, blockIdx.x);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -42,7 +73,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 // boundary makes the partials visible across the XCDs' L2s without per-block
 // agent-scope release fences.
 template <int NQ>
-__device__ __forceinline__ void write_partials(const double (&v)[NQ], double* partials) {
+__device__ __forceinline__ void write_partials(const double (&v)[NQ], double* partials, int blk) {
     __shared__ double red[NQ][BLK / 64];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
@@ -54,8 +85,12 @@ __device__ __forceinline__ void write_partials(const double (&v)[NQ], double* pa
     if (threadIdx.x < NQ) {
         double s = 0;
         for (int w = 0; w < BLK / 64; ++w) s += red[threadIdx.x][w];
-        partials[(int64_t)blockIdx.x * NQ + threadIdx.x] = s;
+        partials[(int64_t)blk * NQ + threadIdx.x] = s;
     }
+}
+template <int NQ>
+__device__ __forceinline__ void write_partials(const double (&v)[NQ], double* partials) {
+    write_partials<NQ>(v, partials, blockIdx.x);
 }
 
 // In a one-workgroup finalize kernel: sums over the G blocks' partials
@@ -147,8 +182,16 @@ __device__ __forceinline__ void finish_initial(CompState& cs, const KryCtl& k, d
     cs.solved = 1;
     cs.converged = 0;
     cs.needRefine = 0;
+    cs.foldE = 0;
     if (k.minIter > 0 || !conv_outer(cs, k, cs.initRes)) {
-        start_inner(cs, k, rr);
+        if (k.rb) {
+            // red-black: a first sweep eliminates the first colour (x there
+            // from the second colour's x) and leaves the Schur residual
+            cs.active = 0;
+            cs.needRefine = 1;
+        } else {
+            start_inner(cs, k, rr);
+        }
     } else {
         cs.active = 0;
         cs.converged = 1;

@@ -175,6 +175,10 @@ struct Mesh {
                     // unless cell and boundary face centres are given
     DBuf permD;     // int32 [N] internal row -> external cell
     double meanAbsBandwidth = 0;
+    // two-colourable cell graphs (hexahedral and quadrilateral meshes) are
+    // stored colour by colour: rows [0, nRed) have only neighbours in
+    // [nRed, N) and the other way round (0: not coloured)
+    int32_t nRed = 0;
 };
 
 // --------------------------------------------------------- solver controls
@@ -185,6 +189,7 @@ struct Controls {
     int solver = SV_PBICGSTAB;
     std::string precondName = "DILU";
     int precond = PC_JACOBI;
+    bool precondStrict = false;  // multicolourILU0 named: refuse a mesh or solver it cannot run on
     double tolerance = 1e-6, relTol = 0.0;
     int maxIter = 1000, minIter = 0;
     bool mixed = false;
@@ -203,7 +208,9 @@ struct alignas(64) CompState {
     int32_t iter, active, converged, solved;
     int32_t exitAtS, needRefine, nRefine, innerIter;
     double innerTarget;  // inner (FP32) residual target per sweep
-    double pad[3];
+    int32_t foldE;       // mixed: e holds an inner solve's correction not yet added to x
+    int32_t pad0;
+    double pad[2];
 };
 static_assert(sizeof(CompState) % 8 == 0, "CompState layout");
 
@@ -213,7 +220,11 @@ struct KryCtl {
     int32_t maxIter, minIter, maxInner, maxRefine;
     int32_t mixed;
     int32_t bicg;   // PBiCG (1) instead of PBiCGStab (0)
-    int32_t precond;  // PC_NONE or PC_JACOBI
+    int32_t precond;  // PC_NONE, PC_JACOBI or PC_MCILU0
+    // red-black ILU0 (PC_MCILU0 on a colour-blocked mesh): the Krylov solve runs
+    // on the Schur complement of the second colour; every solve starts and
+    // ends with a sweep that eliminates the first colour exactly
+    int32_t rb;
 };
 
 struct SolveState {
@@ -242,6 +253,8 @@ struct Krylov {
     DBuf ahat32;           // float [nEll]  column-scaled off-diagonals A D^-1
     DBuf ahat64;           // double [nEll]
     DBuf rD;               // double [N]    1/diag
+    DBuf rDs;              // double [N]    red-black: 1/diag of the Schur complement
+                           // (the ILU0 diagonal) where it differs from 1/diag
     DBuf partials;         // double [grid][24]
     DBuf state;            // SolveState
     std::unique_ptr<SolveRun, SolveRunDeleter> run;
@@ -259,7 +272,10 @@ enum ModelKind { M_OLDROYDB = 0, M_GIESEKUS = 1, M_FENEP = 2, M_PTT_LINEAR = 3, 
                  M_FENECR = 5, M_WMCY = 6, M_ROLIEPOLY = 7, M_XPP = 8, M_SARAMITO = 9,
                  // BMPLog: the theta equation, and its scalar fluidity equation (an
                  // owned sub-model holding Phi in component xx of its theta)
-                 M_BMP = 10, M_FLUIDITY = 11 };
+                 M_BMP = 10, M_FLUIDITY = 11,
+                 // Oldroyd-BRootk (theta = C^(1/k)); PTTLog with the generalized
+                 // (Mittag-Leffler) destruction function
+                 M_ROOTK = 12, M_PTT_GEN = 13 };
 // models whose tau needs a per-cell factor of the old state (Model::fOld)
 inline bool modelKeepsFactor(int kind) { return kind == M_FENEP || kind == M_FENECR || kind == M_WMCY; }
 
@@ -276,6 +292,14 @@ struct ModelParams {
     int sarPTT = 0;  // 0 none, 1 linear, 2 exponential
     // BMP: modulus, rate constant, fluidity at rest and at infinite shear
     double G0 = 1, kBMP = 0, Phi0 = 0, PhiInf = 0;
+    // Oldroyd-BRootk: the root k (Oldroyd_BRootk.C:121)
+    double rootK = 1;
+    // PTT generalized (PTTLog.C:135-152): gamma(beta), the number of series
+    // terms, the relative tolerance, and the device table
+    // mlGamma[j] = gamma(alpha j + beta) of the series denominators
+    double mlGammaBeta = 1, mlRtol = 1e-12;
+    int32_t mlMaxIter = 0;
+    const double* mlGamma = nullptr;
 };
 
 struct Model {
@@ -305,6 +329,8 @@ struct Model {
     bool iterating = false;    // thetaOld holds the current step's oldTime
     int scheme = 0;            // DivScheme of div(phi,theta)
     bool bounded = false;      // the `bounded` wrapper: diagonal minus the cell's net outflux
+    DBuf mlGamma;              // PTT generalized: the gamma table of the series (prm.mlGamma)
+    std::vector<double> mlGammaHost;
     DBuf gradTh;               // double [N][18] component gradients (limited schemes)
     DBuf souT;                 // double [N][6] explicit correction source
     DBuf bcType;            // int32 [nPatches]
@@ -379,6 +405,7 @@ struct AssembleArgs {
     double* rD;           // [N] 1/diag
     KryCtl ctl;
     int32_t mixed;        // r/r0 in FP32
+    int32_t nRed;         // colour-blocked rows: the first colour's count (tile order), 0: none
 };
 
 void launch_assemble(Ctx* ctx, const AssembleArgs& a, int grid);
@@ -437,11 +464,19 @@ struct SolveArgs {
     const Controls* ctl;
     bool xbarKnown;      // CompState.xbar already set
     bool initDone;       // r, r0, rD and the initial state already written (by the assembly)
+    // red-black: 1/diag of the Schur complement, NULL when it is 1/diag (every
+    // product A_ij A_ji vanishes, as for the upwind theta matrix)
+    const double* rDs = nullptr;
     // pageable host block for the status read-back; NULL: a page-locked block
     // of the context and an event (the blocking path then waits on the stream)
     SolveState* hostStatus = nullptr;
 };
-KryCtl kryCtl(const Controls& C, bool mixed);
+KryCtl kryCtl(const Controls& C, bool mixed, const Mesh& m);
+// red-black ILU0 of a general LDU matrix: out[0, N) = 1/Ds (the Schur
+// complement's diagonal on the second colour, 1/D on the first), out[N, 2N) =
+// D/Ds; lower/upper per internal face
+void launch_rb_schur_diag(Ctx* ctx, const Mesh& m, const double* lower, const double* upper, const double* diag,
+                          double* out);
 // mean of theta per component -> CompState.xbar (gAverage(psi) of the normalisation)
 struct Dict;
 // a model (or a multiMode and its modes) from its parameters dictionary
@@ -466,6 +501,40 @@ bool solve_advance(Ctx* ctx, Krylov& K, bool block);
 void solve_finish(Krylov& K, rt_solver_perf perf[6]);
 
 int grid_for(const Ctx* ctx, int32_t N);
+
+// Grouped Krylov rounds (the serving loop): one launch per kernel runs the
+// lockstep PBiCGStab of every member solve, each member on the workgroups its
+// own launch would use and with its own fixed-order reductions, so every member
+// computes the bits it computes alone.  Members share the precision and the
+// preconditioner form; each is begun (solve_begin) on its own stream, and the
+// rounds run on the group's stream.
+struct SolveGroup {
+    Ctx* ctx = nullptr;  // the group's stream and table memory
+    bool mixed = false, rb = false;
+    struct Member {
+        Krylov* K;
+        hipStream_t home;  // the stream the solve was begun on (and continues on)
+    };
+    std::vector<Member> members;
+    bool dirty = true;   // tables to upload before the next launch
+    DBuf tables;
+    void* hostTables = nullptr;  // page-locked staging of the tables
+    size_t hostBytes = 0;
+    bool launched = false;       // rounds queued whose member states are not all back
+    int64_t launches = 0;        // grouped rounds launched (for tests and measurements)
+    ~SolveGroup();
+};
+// a begun solve that grouped rounds can run (PBiCGStab), and its group class
+bool solve_groupable(const Krylov& K, bool* mixed, bool* rb);
+// adds a begun solve: the group's stream waits for the work queued on `home`
+void group_join(SolveGroup& g, Krylov& K, hipStream_t home);
+// Polls the members' states without blocking.  Members whose solve has
+// finished leave the group (their `home` stream first waits for the group's
+// queued work, which may still read them) and are appended to `finished`.
+// When every member's state is back, launches the next grouped rounds.
+void group_advance(SolveGroup& g, std::vector<Krylov*>& finished);
+// takes a member out (a failed request): waits for the group's queued work
+void group_remove(SolveGroup& g, Krylov& K);
 // looks up the device address of the no-allocation scratch array on the
 // current device, once per device (rt_noalloc.cpp)
 void noallocInit(int device);
@@ -478,9 +547,12 @@ void noallocInit(int device);
 // current iterate with the saved oldTime
 void modelCorrectBegin(Model& M, double deltaT, const double* gradU, int iteration = -1);
 bool modelCorrectAdvance(Model& M, bool block);
+// after the solve: its performance record, then calcEig and tau (launched)
+void modelCorrectFinish(Model& M);
 void modelCorrect(Model& M, double deltaT, const double* gradU, int iteration = -1);
 // device bytes a model of this size allocates (for admission control)
-size_t modelBytesEstimate(int32_t N, int32_t F, int32_t NB, int64_t nEll, bool mixed);
+size_t modelBytesEstimate(int32_t N, int32_t F, int32_t NB, int64_t nEll, bool mixed, bool gradU = false,
+                          bool download = false);
 
 }  // namespace rt
 

@@ -175,6 +175,8 @@ int solveNoAlloc(Ctx* c, const Mesh& mc, const rt_ldu_symm* sys, double* psi, co
             K.s64 = view(n6 * 8);
         }
     }
+    const bool rb = kryCtl(ctl, mixed, m).rb != 0;
+    double* dS = rb ? (double*)cv.take((size_t)2 * std::max(1, N) * 8) : nullptr;
     K.rD = view((size_t)std::max(1, N) * 8);
     K.partials = view((size_t)grid * 24 * sizeof(double));
     K.state = view(sizeof(SolveState));
@@ -207,6 +209,7 @@ int solveNoAlloc(Ctx* c, const Mesh& mc, const rt_ldu_symm* sys, double* psi, co
                            m.bPtr.as<int32_t>(), m.bFace.as<int32_t>(), diagExt, stage, iC, bC, dD, dB);
     RT_HIP(hipGetLastError());
     launch_ldu_to_ell(c, m, lo, up, dA);
+    if (rb) launch_rb_schur_diag(c, m, lo, up, dD, dS);  // the Schur pivots of a general LDU matrix
     if (bicg) launch_ldu_to_ell(c, m, up, lo, K.atval.as<double>());  // A^T
     RT_HIP(hipMemcpyAsync(stage, psi, n6 * 8, hipMemcpyHostToDevice, st));
     launch_permute_rows(c, N, 6, 6, stage, m.permD.as<int32_t>(), dX);
@@ -236,6 +239,7 @@ int solveNoAlloc(Ctx* c, const Mesh& mc, const rt_ldu_symm* sys, double* psi, co
     sa.xbarKnown = true;
     sa.initDone = false;
     sa.hostStatus = &slot.status;
+    sa.rDs = dS;
     solve_lockstep(c, sa, perf);
     launch_unpermute_rows(c, N, 6, 6, dX, m.permD.as<int32_t>(), stage);
     RT_HIP(hipMemcpyAsync(psi, stage, n6 * 8, hipMemcpyDeviceToHost, st));

@@ -2,15 +2,20 @@
 //
 // Requests (a mesh, a model dictionary, the state and the flow of one
 // stress step) are admitted while a slot and the HBM budget allow, then
-// advanced together: every progress() call moves each in-flight request one
-// round further (assembly, Krylov rounds, eigen/tau, download) without
-// blocking on any of them (results go home with asynchronous copies, which
-// overlap other requests' kernels when the output arrays are page-locked,
-// rt_host_alloc).  Each slot owns a context with its own stream, so
-// the rounds of different requests overlap on the device, and requests that
-// arrive while others run are admitted as soon as a slot and memory free up.
-// A request's kernels depend only on its own data (fixed grids, per-request
-// reductions in a fixed order), so its result does not depend on the mix.
+// advanced together: every progress() call moves each in-flight request
+// further (assembly, Krylov rounds, eigen/tau, download) without blocking on
+// any of them (results go home with asynchronous copies, which overlap other
+// requests' kernels when the output arrays are page-locked, rt_host_alloc).
+// Each slot owns a context with its own stream for a request's own work
+// (uploads, assembly, eigen/tau, downloads).  The Krylov solves of all
+// in-flight requests are batched: one launch per kernel runs the rounds of
+// every request of the same precision and preconditioner form (grouped
+// rounds over the requests' matrices, SolveGroup), on a stream of the
+// server's own, so small requests no longer pay a launch per kernel each.
+// Requests that arrive while others run are admitted as soon as a slot and
+// memory free up and join the batch at its next round.  A request's kernels
+// depend only on its own data (its own workgroups, per-request reductions in
+// a fixed order), so its result does not depend on the mix.
 #include <chrono>
 #include <deque>
 #include <map>
@@ -42,6 +47,7 @@ struct Request {
     rt_solver_perf perf[6];
     int rc = RT_OK;
     std::string err;
+    int group = -1;  // the batch its Krylov solve runs in, -1: its own rounds
 };
 
 }  // namespace rt
@@ -59,11 +65,22 @@ struct rt_server {
     int64_t nextTicket = 1;
     std::string defaultControls = "solver PBiCGStab; preconditioner DILU; tolerance 1e-10; relTol 0; "
                                   "maxIter 1000; precision mixed;";
+    // batched Krylov rounds: one group per (precision, preconditioner form),
+    // on the stream of a context of the server's own
+    bool batching = true;
+    rt_ctx* gctx = nullptr;
+    SolveGroup groups[4];
+    std::map<Krylov*, Request*> byKrylov;
 };
 
 namespace {
 
 void releaseRequest(rt_server* S, Request* r) {
+    if (r->group >= 0 && r->model) {
+        group_remove(S->groups[r->group], r->model->impl.kry);
+        S->byKrylov.erase(&r->model->impl.kry);
+        r->group = -1;
+    }
     if (r->model) rt_model_destroy(r->model);
     r->model = nullptr;
     if (r->done) (void)hipEventDestroy(r->done);
@@ -125,6 +142,16 @@ void admit(rt_server* S, Request* r, int slot) {
         modelCorrectBegin(r->model->impl, r->deltaT, r->gradU);
         RT_HIP(hipEventCreateWithFlags(&r->done, hipEventDisableTiming));
         r->state = Request::RUNNING;
+        bool mixed = false, rb = false;
+        if (S->batching && solve_groupable(M.kry, &mixed, &rb)) {
+            const int g = (mixed ? 2 : 0) + (rb ? 1 : 0);
+            SolveGroup& G = S->groups[g];
+            G.mixed = mixed;
+            G.rb = rb;
+            group_join(G, M.kry, c->impl.stream);
+            r->group = g;
+            S->byKrylov[&M.kry] = r;
+        }
     } catch (const Error& e) {
         fail(S, r, e.code, e.msg);
     }
@@ -158,16 +185,21 @@ void beginDownload(rt_ctx* c, Request* r) {
     RT_HIP(hipMemcpyAsync(ta, r->dlTau.p, bytes, hipMemcpyDeviceToHost, x->stream));
 }
 
+// a request whose solve has finished: eigen/tau and the way home, on its slot's stream
+void finishSolve(rt_server* S, Request* r, bool solveDone) {
+    rt_ctx* c = S->slots[r->slot];
+    if (!solveDone) modelCorrectFinish(r->model->impl);
+    beginDownload(c, r);
+    RT_HIP(hipEventRecord(r->done, c->impl.stream));
+    r->state = Request::FINISHING;
+}
+
 void advance(rt_server* S, Request* r) {
     rt_ctx* c = S->slots[r->slot];
     try {
         RT_HIP(hipSetDevice(c->impl.device));
-        if (r->state == Request::RUNNING) {
-            if (modelCorrectAdvance(r->model->impl, false)) {
-                beginDownload(c, r);
-                RT_HIP(hipEventRecord(r->done, c->impl.stream));
-                r->state = Request::FINISHING;
-            }
+        if (r->state == Request::RUNNING && r->group < 0) {
+            if (modelCorrectAdvance(r->model->impl, false)) finishSolve(S, r, true);
         }
         if (r->state == Request::FINISHING) {
             hipError_t e = hipEventQuery(r->done);
@@ -200,11 +232,18 @@ int rt_server_create(rt_ctx* ctx, size_t hbm_budget, int max_inflight, rt_server
     auto* S = new rt_server;
     S->parent = ctx;
     S->budget = hbm_budget;
+    if (rt_ctx_create(ctx->impl.device, 0, &S->gctx) != RT_OK) {
+        delete S;
+        ctx->impl.lastError = "rt_server_create: context creation failed";
+        return RT_E_HIP;
+    }
+    for (auto& g : S->groups) g.ctx = &S->gctx->impl;
     for (int i = 0; i < max_inflight; ++i) {
         rt_ctx* c = nullptr;
         int rc = rt_ctx_create(ctx->impl.device, 0, &c);
         if (rc != RT_OK) {
             for (auto* x : S->slots) rt_ctx_destroy(x);
+            rt_ctx_destroy(S->gctx);
             delete S;
             ctx->impl.lastError = "rt_server_create: slot context creation failed";
             return rc;
@@ -219,15 +258,45 @@ int rt_server_create(rt_ctx* ctx, size_t hbm_budget, int max_inflight, rt_server
 
 void rt_server_destroy(rt_server* S) {
     if (!S) return;
+    (void)hipStreamSynchronize(S->gctx->impl.stream);
     for (auto& kv : S->all)
         if (kv.second->slot >= 0) {
             (void)hipStreamSynchronize(S->slots[kv.second->slot]->impl.stream);
             releaseRequest(S, kv.second.get());
         }
     for (auto* c : S->slots) rt_ctx_destroy(c);
+    for (auto& g : S->groups) g.tables.release();
+    rt_ctx_destroy(S->gctx);
     Ctx* parent = &S->parent->impl;
     delete S;
     ctxUnref(parent);
+}
+
+size_t rt_request_bytes_estimate(const rt_mesh* mesh, int mixed, int gradU) {
+    if (!mesh) return 0;
+    const Mesh& m = mesh->impl;
+    return modelBytesEstimate(m.N, m.F, m.NB, m.nEll, mixed != 0, gradU != 0, true);
+}
+
+int rt_server_set_batching(rt_server* S, int enable) {
+    if (!S) return RT_E_BAD_ARG;
+    if (!S->all.empty() && S->batching != (enable != 0)) {
+        for (auto& kv : S->all)
+            if (kv.second->state == Request::QUEUED || kv.second->state == Request::RUNNING ||
+                kv.second->state == Request::FINISHING) {
+                S->parent->impl.lastError = "rt_server_set_batching: requests are in flight";
+                return RT_E_BAD_ARG;
+            }
+    }
+    S->batching = enable != 0;
+    return RT_OK;
+}
+
+int64_t rt_server_batched_rounds(const rt_server* S) {
+    if (!S) return 0;
+    int64_t n = 0;
+    for (auto& g : S->groups) n += g.launches;
+    return n;
 }
 
 int rt_server_submit(rt_server* S, const rt_request* q, int64_t* ticket) {
@@ -256,7 +325,7 @@ int rt_server_submit(rt_server* S, const rt_request* q, int64_t* ticket) {
     r->tauOut = q->tauOut;
     r->gradU = q->gradU;
     const bool mixed = r->controls.find("mixed") != std::string::npos;
-    r->estBytes = modelBytesEstimate((int32_t)N, (int32_t)F, (int32_t)NB, mm.nEll, mixed);
+    r->estBytes = modelBytesEstimate((int32_t)N, (int32_t)F, (int32_t)NB, mm.nEll, mixed, q->gradU != nullptr, true);
     if (S->budget && r->estBytes > S->budget) {
         P->impl.lastError = "rt_server_submit: request needs " + std::to_string(r->estBytes) +
                             " bytes, more than the whole HBM budget";
@@ -288,6 +357,33 @@ int rt_server_progress(rt_server* S) {
         }
         it = S->queue.erase(it);
         admit(S, r, slot);
+    }
+    // batched Krylov rounds; requests whose solve finished go on with eigen/tau
+    std::vector<Krylov*> done;
+    for (auto& g : S->groups) {
+        if (g.members.empty()) continue;
+        done.clear();
+        try {
+            RT_HIP(hipSetDevice(S->gctx->impl.device));
+            group_advance(g, done);
+        } catch (const Error& e) {
+            // the batch's launch failed: every member fails with it
+            std::vector<Request*> members;
+            for (auto& mb : g.members) members.push_back(S->byKrylov[mb.K]);
+            for (Request* r : members) fail(S, r, e.code, e.msg);
+            continue;
+        }
+        for (Krylov* K : done) {
+            Request* r = S->byKrylov[K];
+            S->byKrylov.erase(K);
+            r->group = -1;
+            try {
+                RT_HIP(hipSetDevice(S->slots[r->slot]->impl.device));
+                finishSolve(S, r, false);
+            } catch (const Error& e) {
+                fail(S, r, e.code, e.msg);
+            }
+        }
     }
     int live = (int)S->queue.size();
     for (size_t i = 0; i < S->slots.size(); ++i) {

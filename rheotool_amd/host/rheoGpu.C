@@ -225,6 +225,7 @@ static constitutiveEq::adddictionaryConstructorToTable<RoliePolyLog> addRP("Roli
 static constitutiveEq::adddictionaryConstructorToTable<XPomPomLog> addXPP("XPomPomLog");
 static constitutiveEq::adddictionaryConstructorToTable<SaramitoLog> addSaramito("SaramitoLog");
 static constitutiveEq::adddictionaryConstructorToTable<BMPLog> addBMP("BMPLog");
+static constitutiveEq::adddictionaryConstructorToTable<Oldroyd_BRootk> addRootk("Oldroyd-BRootk");
 static constitutiveEq::adddictionaryConstructorToTable<multiMode> addMulti("multiMode");
 
 multiMode::multiMode(const word& name, const volVectorField& U, const surfaceScalarField& phi,

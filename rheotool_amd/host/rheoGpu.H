@@ -276,6 +276,8 @@ RHEOGPU_LOG_MODEL(RoliePolyLog, "Rolie-PolyLog")
 RHEOGPU_LOG_MODEL(XPomPomLog, "XPomPomLog")
 RHEOGPU_LOG_MODEL(SaramitoLog, "SaramitoLog")
 RHEOGPU_LOG_MODEL(BMPLog, "BMPLog")
+// the root-k conformation form (theta = C^(1/k)) runs on the same device path
+RHEOGPU_LOG_MODEL(Oldroyd_BRootk, "Oldroyd-BRootk")
 
 //- IOdictionary wrapper reading constitutiveProperties (constitutiveModel.C:41-59)
 class constitutiveModel {

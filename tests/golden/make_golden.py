@@ -18,6 +18,11 @@ that are independent of the oracle's code:
 * ldu_<m>.npz   -- tiny meshes: dense upwind+Euler matrix and right-hand side
                    assembled cell-by-cell from the face list, and the spsolve
                    solution of every component.
+* rhs_more.npz  -- the further log models, as rhs.npz.
+* rhs_kernel.npz -- Oldroyd-B in the root-k form theta = C^(1/k) (Daleckii-
+                   Krein derivative of x^(1/k)) and PTT with the Mittag-Leffler
+                   destruction function from its closed forms E_{1,1} = exp,
+                   E_{1,2} = (e^z - 1)/z, E_{2,1} = cosh(sqrt z).
 
 Run from the repo root: python tests/golden/make_golden.py
 Meshes and fields come from rheotool_amd.cases (the synthetic case
@@ -395,9 +400,98 @@ def make_rhs_more():
     np.savez_compressed(os.path.join(HERE, "rhs_more.npz"), **out)
 
 
+# ------------------------- root-k conformation and PTT's Mittag-Leffler function
+def dpow(C, E, e):
+    """Daleckii-Krein: directional derivative of C^e at SPD C along E, with
+    the divided differences of x^e."""
+    lam, V = np.linalg.eigh(C)
+    Et = V.T @ E @ V
+    G = np.empty((3, 3))
+    for i in range(3):
+        for j in range(3):
+            G[i, j] = e * lam[i] ** (e - 1) if i == j else (lam[i] ** e - lam[j] ** e) / (lam[i] - lam[j])
+    return V @ (Et * G) @ V.T
+
+
+def mittag_leffler_closed(a, b, z):
+    """E_{a,b}(z) from closed forms: E_{1,1} = exp z, E_{1,2} = (e^z - 1)/z,
+    E_{2,1} = cosh(sqrt z) (cos(sqrt(-z)) for z < 0)."""
+    if (a, b) == (1, 1):
+        return np.exp(z)
+    if (a, b) == (1, 2):
+        return np.expm1(z) / z
+    if (a, b) == (2, 1):
+        return np.cosh(np.sqrt(z)) if z >= 0 else np.cos(np.sqrt(-z))
+    raise KeyError((a, b))
+
+
+ROOTK = {"k2": dict(lam=1.4, k=2.0), "k3p5": dict(lam=0.7, k=3.5)}
+PTT_GEN = {"ml11": dict(lam=1.1, epsilon=0.25, zeta=0.1, ab=(1, 1)),
+           "ml12": dict(lam=0.9, epsilon=0.3, zeta=0.0, ab=(1, 2)),
+           "ml21": dict(lam=1.3, epsilon=0.2, zeta=0.15, ab=(2, 1))}
+
+
+def make_rhs_kernel():
+    """rhs_kernel.npz: the root-k theta = C^(1/k) right-hand side of
+    Oldroyd-B (d(C^(1/k))/dt along the conformation equation, Daleckii-Krein)
+    and the log right-hand side of PTT with the Mittag-Leffler destruction
+    function Gamma(b) E_{a,b}(eps/(1-zeta)(tr C - 3)) from its closed forms."""
+    rng = np.random.default_rng(20261204)
+    out = {}
+    for key, prm in ROOTK.items():
+        k = prm["k"]
+        ths, evs, vecs, Ls, S = [], [], [], [], []
+        while len(ths) < 60:
+            R = rand_rot(rng)
+            ln = np.sort(rng.uniform(-2.0, 2.5, 3))
+            if np.min(np.diff(ln)) < 0.05:
+                continue
+            C = R @ np.diag(np.exp(ln)) @ R.T
+            th = R @ np.diag(np.exp(ln / k)) @ R.T          # theta = C^(1/k)
+            w, V = np.linalg.eigh(th)
+            L = rng.standard_normal((3, 3))
+            kap = L.T
+            E = kap @ C + C @ kap.T - (C - np.eye(3)) / prm["lam"]
+            ths.append(sym6(th))
+            evs.append(np.diag(w ** k))                      # conformation eigenvalues
+            vecs.append(V)
+            Ls.append(L)
+            S.append(sym6(dpow(C, E, 1.0 / k)))
+        for name, arr in (("theta", ths), ("eigVals", evs), ("eigVecs", vecs), ("gradU", Ls), ("S", S)):
+            out[f"rootk_{key}_{name}"] = np.array(arr)
+        out[f"rootk_{key}_params"] = np.array([prm["lam"], k])
+    for key, prm in PTT_GEN.items():
+        ths, evs, vecs, Ls, S = [], [], [], [], []
+        a, b = prm["ab"]
+        while len(ths) < 60:
+            R = rand_rot(rng)
+            ln = np.sort(rng.uniform(-2.0, 2.0, 3))
+            if np.min(np.diff(ln)) < 0.05:
+                continue
+            C = R @ np.diag(np.exp(ln)) @ R.T
+            th = R @ np.diag(ln) @ R.T
+            w, V = np.linalg.eigh(th)
+            L = rng.standard_normal((3, 3))
+            kap = L.T
+            kap = kap - prm["zeta"] * 0.5 * (kap + kap.T)
+            z = prm["epsilon"] / (1 - prm["zeta"]) * (np.trace(C) - 3.0)
+            import math
+            Y = math.gamma(b) * mittag_leffler_closed(a, b, z)
+            E = kap @ C + C @ kap.T - Y * (C - np.eye(3)) / prm["lam"]
+            ths.append(sym6(th))
+            evs.append(np.diag(np.exp(w)))
+            vecs.append(V)
+            Ls.append(L)
+            S.append(sym6(dlog(C, E)))
+        for name, arr in (("theta", ths), ("eigVals", evs), ("eigVecs", vecs), ("gradU", Ls), ("S", S)):
+            out[f"pttgen_{key}_{name}"] = np.array(arr)
+        out[f"pttgen_{key}_params"] = np.array([prm["lam"], prm["epsilon"], prm["zeta"], a, b])
+    np.savez_compressed(os.path.join(HERE, "rhs_kernel.npz"), **out)
+
+
 if __name__ == "__main__":
-    # python tests/golden/make_golden.py [eig rhs steady ldu rhs_more]
-    which = sys.argv[1:] or ["eig", "rhs", "steady", "ldu", "rhs_more"]
+    # python tests/golden/make_golden.py [eig rhs steady ldu rhs_more rhs_kernel]
+    which = sys.argv[1:] or ["eig", "rhs", "steady", "ldu", "rhs_more", "rhs_kernel"]
     if set(which) & {"eig", "rhs", "steady", "ldu"}:
         rng = np.random.default_rng(20261202)  # one stream through these four, in this order
         make_eig(rng)
@@ -406,4 +500,6 @@ if __name__ == "__main__":
         make_ldu(rng)
     if "rhs_more" in which:
         make_rhs_more()
+    if "rhs_kernel" in which:
+        make_rhs_kernel()
     print("golden fixtures written to", HERE)

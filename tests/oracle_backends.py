@@ -44,12 +44,18 @@ def oracle_params(e):
             kw["sar_k"] = e.scalar("k")
     elif "n" in e:
         kw["n" if t == "XPomPomLog" else "wm_n"] = e.scalar("n")
+    if t == "Oldroyd-BRootk":  # Oldroyd_BRootk.C:121
+        kw["root_k"] = e.scalar("k")
     if t == "PTTLog":
         # PTTLog.C:139 reads destructionFunctionType (no default)
         fn = e.word("destructionFunctionType")
-        if fn not in ("linear", "exponential"):
+        if fn not in ("linear", "exponential", "generalized"):
             raise NotImplementedError(f"PTT destructionFunctionType {fn}")
         kw["ptt_exp"] = fn == "exponential"
+        if fn == "generalized":  # PTTLog.C:135-152
+            kw["ptt_gen"] = (kw.pop("alpha"), kw.pop("beta"))
+            kw["ml_rtol"] = e.scalar("rTolMittagLeffler", 1e-12)
+            kw["ml_max_iter"] = int(e.scalar("maxIterMittagLeffler", 200))
     return O.params(t, **kw), kw.get("etaS", 0.0)
 
 

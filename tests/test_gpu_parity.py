@@ -343,13 +343,11 @@ def test_pbicg_symm_solve_matches_spsolve_fixtures(ctx, golden_dir):
 
 
 @pytest.mark.parametrize("pc,code", [("DIC", abi.RT_E_UNKNOWN_TYPE), ("FDIC", abi.RT_E_UNKNOWN_TYPE),
-                                     ("multicolourILU0", abi.RT_E_UNSUPPORTED),
                                      ("ILU7", abi.RT_E_UNKNOWN_TYPE)])
 def test_unavailable_preconditioners_are_refused(ctx, pc, code):
     """No silent replacement: DIC/FDIC are symmetric-matrix preconditioners
     that [OF9] rejects for the asymmetric theta matrix (listing the valid
-    asymmetric ones), multicolour ILU0 is not built, unknown names list the
-    valid ones."""
+    asymmetric ones), unknown names list the valid ones."""
     m = cases.cartesian(3, 3, 3)
     dm = ctx.mesh(m)
     eq = ctx.constitutiveEq("", dm, abi.model_dict("Oldroyd-BLog"))
@@ -557,6 +555,125 @@ def test_div_tau_stabilization_needs_face_centres(ctx):
         eq.set_laplacian_scheme("Gauss linear limited 0.5")
 
 
+# ------------------------------------------- multicolour (red-black) ILU0
+RB_CASES = {
+    "box_giesekus": (lambda: cases.box3d(24, lnLo=-9.0, lnHi=5.0), "GiesekusLog", 10.0, 5e-4,
+                     dict(alpha=0.1)),
+    "perm_jitter_fenep": (lambda: cases.box3d(20, lnLo=-8.0, lnHi=np.log(30.0), perm_seed=3, jitter=0.1),
+                          "FENE-PLog", 5.0, 5e-4, dict(L2=100.0)),
+    "channel2d_ob": (lambda: cases.channel2d(48), "Oldroyd-BLog", 1.0, 1e-3, {}),
+}
+
+
+@pytest.mark.parametrize("case", list(RB_CASES))
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_multicolour_ilu0_matches_oracle(ctx, case, precision):
+    """preconditioner multicolourILU0: ILU(0) in the two-colour order of a
+    hexahedral / quadrilateral mesh, run as PBiCGStab on the second colour's
+    Schur complement with the first colour eliminated exactly.  Converged
+    theta/tau against the reference algorithm (oracle, PBiCGStab + DILU)
+    within 1e-9, and fewer Krylov iterations than the diagonal
+    preconditioner."""
+    mk, model, lam, dt, kw = RB_CASES[case]
+    m, f = mk()
+    g, o = run_pair(ctx, m, f, model, lam, dt, kw, precision=precision, precond="multicolourILU0")
+    check_step(g, o, m.validComponents)
+    gj, _ = run_pair(ctx, m, f, model, lam, dt, kw, precision=precision, precond="diagonal")
+    it_rb = max(q.nIterations for q in g["perf"] if q.solved)
+    it_j = max(q.nIterations for q in gj["perf"] if q.solved)
+    print(f"[case] iterations red-black {it_rb}, diagonal {it_j}", flush=True)
+    assert it_rb < it_j
+    assert rel_l2(g["theta"], gj["theta"]) < THETA_TOL
+
+
+@pytest.mark.parametrize("scheme", ["cubista", "bounded upwind"])
+def test_multicolour_ilu0_limited_schemes_inlet_relax(ctx, scheme):
+    """The red-black solve under a limited scheme's deferred correction, a
+    fixedValue inlet and a relaxation factor (both change the diagonal only)."""
+    m, f = cases.channel2d(40)
+    inlet = 0  # the channel's inlet patch
+    dm = ctx.mesh(m)
+    s0, n0 = int(m.patchStart[inlet]), int(m.patchSize[inlet])
+    bv = f.theta[m.owner[m.nInternal + s0:m.nInternal + s0 + n0]] + 0.05
+    out = {}
+    for pc in ("multicolourILU0", "diagonal"):
+        eq = ctx.constitutiveEq("", dm, abi.model_dict("Oldroyd-BLog", lam=1.0))
+        eq.set_solver(abi.solver_dict(1e-12, precond=pc))
+        eq.set_scheme(scheme if scheme.startswith("bounded") else "GaussDefCmpw " + scheme)
+        eq.set_relax(0.8)
+        eq.set_bc(inlet, abi.RT_BC_FIXEDVALUE, bv)
+        eq.set_state(f.theta, f.eigVals, f.eigVecs)
+        eq.set_flow(f.U, f.Ub, f.phi)
+        assert eq.correct(1e-3) == abi.RT_OK, eq.solver_perf()
+        out[pc] = eq.theta()
+        eq.close()
+    assert rel_l2(out["multicolourILU0"], out["diagonal"]) < 1e-10
+
+
+def test_multicolour_ilu0_symm_solve_general_ldu(ctx, golden_dir):
+    """The sparseSolver surface with a general LDU matrix, whose products
+    A_ij A_ji do not vanish (a diffusion-like coupling on both sides of every
+    face): the Schur pivots D_i - sum A_ij A_ji / D_j enter the second
+    colour's preconditioner.  Against scipy's spsolve, both precisions."""
+    import scipy.sparse
+    import scipy.sparse.linalg
+    m = cases.cartesian(9, 8, 7, jitter=0.1, perm_seed=2)
+    N, F = m.nCells, m.nInternal
+    rng = np.random.default_rng(5)
+    lower = -rng.uniform(0.05, 0.3, F)
+    upper = -rng.uniform(0.05, 0.3, F)
+    diag = np.zeros(N)
+    np.add.at(diag, m.owner[:F], -upper)  # diagonal dominance by a margin
+    np.add.at(diag, m.neighbour[:F], -lower)
+    diag = diag * 1.3 + rng.uniform(0.1, 1.0, N)
+    b = rng.standard_normal((N, 6))
+    A = scipy.sparse.csr_matrix((np.concatenate([diag, upper, lower]),
+                                 (np.concatenate([np.arange(N), m.owner[:F], m.neighbour[:F]]),
+                                  np.concatenate([np.arange(N), m.neighbour[:F], m.owner[:F]]))), shape=(N, N))
+    want = np.stack([scipy.sparse.linalg.spsolve(A.tocsc(), b[:, q]) for q in range(6)], 1)
+    dm = ctx.mesh(m)
+    for prec in ("double", "mixed"):
+        x, perf = ctx.symm_solve(dm, diag, lower, upper, b, np.zeros((N, 6)),
+                                 abi.solver_dict(1e-13, precond="multicolourILU0", precision=prec))
+        for q in range(6):
+            assert rel_l2(x[:, q], want[:, q]) < 1e-11, (prec, q, perf[q])
+
+
+def test_multicolour_ilu0_refused_where_it_cannot_run(ctx):
+    """multicolourILU0 needs a two-colourable cell graph and PBiCGStab: a random
+    face graph (odd cycles) and the PBiCG solver are refused at the solve,
+    never replaced."""
+    g = cases.random_graph(3000, kNear=5, sigma=0.5, seed=3)
+    f = cases.make_fields(g, cases.FIELDS_TG, seed=11, lnLo=-2.0, lnHi=2.0)
+    for mesh, solver in ((g, "PBiCGStab"), (cases.cartesian(6, 6, 6), "PBiCG")):
+        ff = f if mesh is g else cases.make_fields(mesh, cases.FIELDS_TG, seed=1, lnLo=-2.0, lnHi=2.0)
+        dm = ctx.mesh(mesh)
+        eq = ctx.constitutiveEq("", dm, abi.model_dict("Oldroyd-BLog", lam=1.0))
+        eq.set_solver(abi.solver_dict(1e-10, precond="multicolourILU0", solver=solver))
+        eq.set_state(ff.theta, ff.eigVals, ff.eigVecs)
+        eq.set_flow(ff.U, ff.Ub, ff.phi)
+        with pytest.raises(abi.RheoError) as e:
+            eq.correct(1e-3, ff.gradU if mesh is g else None)
+        assert e.value.code == abi.RT_E_UNSUPPORTED and "multicolourILU0" in e.value.msg
+
+
+def test_colour_blocked_order_matches_one_block(ctx):
+    """A two-colourable mesh is stored colour by colour by default; with
+    RT_RENUMBER_NO_COLOUR it stays one block.  The diagonal-preconditioned
+    step gives the same fields either way: the row order changes only the
+    order of the reductions' sums, so the two Krylov runs take slightly
+    different paths and each stops below the normalised residual 1e-10; the
+    fields then differ by the solves' tolerance, not by more (observed
+    1.1e-11 on this mesh)."""
+    m, f = cases.box3d(18, lnLo=-3.0, lnHi=2.0, perm_seed=6, jitter=0.1)
+    out = []
+    for flag in (abi.RT_RENUMBER_AUTO, abi.RT_RENUMBER_AUTO | abi.RT_RENUMBER_NO_COLOUR):
+        g, _ = run_pair(ctx, m, f, "GiesekusLog", 2.0, 1e-3, dict(alpha=0.1), renumber=flag, precond="diagonal")
+        out.append(g)
+    assert rel_l2(out[0]["theta"], out[1]["theta"]) < 1e-10
+    assert rel_l2(out[0]["tau"], out[1]["tau"]) < 1e-10
+
+
 # the further log models (FENE_CRLog.C, WhiteMetznerCYLog.C, RoliePolyLog.C,
 # XPomPomLog.C): device keys, oracle keyword names
 MORE = {
@@ -604,6 +721,83 @@ def test_further_log_models_explicit_step(ctx, model):
     d = np.abs(g["theta"] - o["theta"]).max() / np.abs(o["theta"]).max()
     assert d < 1e-12, (model, d)
     assert rel_l2(g["tau"], o["tau"]) < 1e-12
+
+
+# Oldroyd-BRootk (theta = C^(1/k), Oldroyd_BRootk.C) and PTTLog's generalized
+# destruction function (Mittag-Leffler, PTTLog.C:135-152,202-230): device keys,
+# oracle keyword names
+KERNEL = {
+    "rootk2": ("Oldroyd-BRootk", dict(k=2.0), dict(root_k=2.0)),
+    "rootk3.5": ("Oldroyd-BRootk", dict(k=3.5), dict(root_k=3.5)),
+    "pttgen21": ("PTTLog", dict(epsilon=0.2, zeta=0.1, destruction="generalized", alpha=2.0, beta=1.0),
+                 dict(epsilon=0.2, zeta=0.1, ptt_gen=(2.0, 1.0))),
+    "pttgen.8_1.3": ("PTTLog", dict(epsilon=0.3, zeta=0.0, destruction="generalized", alpha=0.8, beta=1.3,
+                                    rTolMittagLeffler=1e-13),
+                     dict(epsilon=0.3, zeta=0.0, ptt_gen=(0.8, 1.3), ml_rtol=1e-13)),
+}
+
+
+def kernel_fields(f, model, dev):
+    """The log-conformation fields as the model holds them: theta = C^(1/k)
+    for the root-k form (eigVals stay the conformation eigenvalues)."""
+    import dataclasses
+    if model != "Oldroyd-BRootk":
+        return f
+    lam = np.diagonal(f.eigVals, axis1=1, axis2=2)
+    V = f.eigVecs
+    th = np.einsum("nik,nk,njk->nij", V, lam ** (1.0 / dev["k"]), V)
+    th6 = np.stack([th[:, 0, 0], th[:, 0, 1], th[:, 0, 2], th[:, 1, 1], th[:, 1, 2], th[:, 2, 2]], 1)
+    return dataclasses.replace(f, theta=np.ascontiguousarray(th6))
+
+
+def run_kernel(ctx, key, m, f, dt, precision="double"):
+    model, dev, orc = KERNEL[key]
+    f = kernel_fields(f, model, dev)
+    print(f"[case] {key} N={m.nCells} precision={precision}", flush=True)
+    dm = ctx.mesh(m)
+    eq = ctx.constitutiveEq("", dm, abi.model_dict(model, lam=1.2, **dev))
+    eq.set_solver(abi.solver_dict(1e-10, precision=precision))
+    eq.set_state(f.theta, f.eigVals, f.eigVecs)
+    eq.set_flow(f.U, f.Ub, f.phi)
+    assert eq.correct(dt) == abi.RT_OK, eq.solver_perf()
+    g = dict(theta=eq.theta(), tau=eq.tau(), ev=eq.eigVals(), perf=eq.solver_perf())
+    p = O.params(model, etaS=0.59, etaP=0.41, lam=1.2, **orc)
+    th, ev, vec, tau, perf = O.correct(m, p, f, dt)
+    return g, dict(theta=th, tau=tau, ev=ev, perf=perf)
+
+
+@pytest.mark.parametrize("key", list(KERNEL))
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_rootk_and_ptt_generalized_match_oracle(ctx, key, precision):
+    m = cases.cartesian(12, 10, 9, jitter=0.1, perm_seed=4)
+    f = cases.make_fields(m, cases.FIELDS_TG, seed=8, lnLo=-1.5, lnHi=1.5)
+    g, o = run_kernel(ctx, key, m, f, 2e-3, precision)
+    check_step(g, o, m.validComponents)
+    lg = np.sort(np.diagonal(g["ev"], axis1=1, axis2=2), 1)
+    lo = np.sort(np.diagonal(o["ev"], axis1=1, axis2=2), 1)
+    assert rel_l2(lg, lo) < THETA_TOL
+
+
+@pytest.mark.parametrize("key", list(KERNEL))
+def test_rootk_and_ptt_generalized_explicit_step(ctx, key):
+    """phi = 0: theta_new = theta_old + dt S.  pow/exp/log of the device and
+    host libraries may differ in the last ulp, so the bar is 1e-12."""
+    m, f = cases.box3d(10, lnLo=-1.5, lnHi=1.5)
+    f.phi[:] = 0.0
+    g, o = run_kernel(ctx, key, m, f, 1e-2)
+    d = np.abs(g["theta"] - o["theta"]).max() / np.abs(o["theta"]).max()
+    assert d < 1e-12, (key, d)
+    assert rel_l2(g["tau"], o["tau"]) < 1e-12
+
+
+def test_ptt_generalized_dictionary_errors(ctx):
+    """PTTLog.C:141-149: alpha and beta must be positive."""
+    dm = ctx.mesh(cases.cartesian(3, 3, 3))
+    for a, b in ((0.0, 1.0), (1.0, -0.5)):
+        with pytest.raises(abi.RheoError) as e:
+            ctx.constitutiveEq("", dm, abi.model_dict("PTTLog", epsilon=0.2, zeta=0.0, destruction="generalized",
+                                                      alpha=a, beta=b))
+        assert e.value.code == abi.RT_E_BAD_ARG and "Mittag-Leffler" in e.value.msg
 
 
 SARAMITO = {

@@ -41,13 +41,19 @@ def run_alone(ctx, req, controls):
     return eq.theta(), eq.tau()
 
 
-@pytest.mark.parametrize("precision", ["mixed", "double"])
-def test_same_result_alone_and_in_mix(precision):
-    controls = abi.solver_dict(1e-10, precision=precision)
+@pytest.mark.parametrize("batching", [True, False])
+@pytest.mark.parametrize("precision,precond", [("mixed", "DILU"), ("double", "DILU"),
+                                               ("mixed", "multicolourILU0"), ("double", "multicolourILU0")])
+def test_same_result_alone_and_in_mix(precision, precond, batching):
+    """Each request's result is the same bits alone (a blocking step) and in a
+    mix of staggered arrivals, with the requests' Krylov rounds batched into
+    one launch per kernel or each request launching its own."""
+    controls = abi.solver_dict(1e-10, precision=precision, precond=precond)
     reqs = [make_request(i) for i in range(10)]
+    print(f"[case] serve mix {precision} {precond} batching={batching}", flush=True)
     with abi.Context(0) as ctx:
         alone = [run_alone(ctx, r, controls) for r in reqs]
-        srv = abi.Server(ctx, hbm_budget=0, max_inflight=4)
+        srv = abi.Server(ctx, hbm_budget=0, max_inflight=4, batching=batching)
         tickets = {}
         meshes = [ctx.mesh(r[0]) for r in reqs]
         # staggered arrivals: submit some, progress, submit more while running
@@ -67,6 +73,7 @@ def test_same_result_alone_and_in_mix(precision):
             assert np.array_equal(out["theta"], alone[i][0]), i
             assert np.array_equal(out["tau"], alone[i][1]), i
             assert out["latency_ms"] > 0
+        assert (srv.batched_rounds > 0) == batching
         srv.close()
 
 
@@ -273,3 +280,28 @@ def test_random_graph_requests_with_prescribed_gradu():
             assert np.linalg.norm(out["theta"] - th) / np.linalg.norm(th) < 1e-9
             assert np.linalg.norm(out["tau"] - tau) / np.linalg.norm(tau) < 1e-9
         srv.close()
+
+
+@pytest.mark.parametrize("precision,with_grad", [("mixed", False), ("double", False), ("mixed", True),
+                                                  ("double", True)])
+def test_admission_estimate_covers_what_a_request_allocates(precision, with_grad):
+    """rt_request_bytes_estimate, the amount admission reserves against the HBM
+    budget, is at least what a request's step allocates in a fresh context
+    (the model, its solve, the prescribed gradient, the staging and the two
+    result buffers the server downloads through) and not grossly more."""
+    from rheotool_amd.abi import lib
+    m = cases.cartesian(40, 36, 30, channel=True)
+    f = cases.make_fields(m, cases.FIELDS_TG, seed=3, lnLo=-2.0, lnHi=2.0)
+    with abi.Context(0) as ctx:
+        dm = ctx.mesh(m)
+        base = ctx.bytes_in_use
+        eq = ctx.constitutiveEq("", dm, abi.model_dict("GiesekusLog", lam=1.0, alpha=0.2))
+        eq.set_solver(abi.solver_dict(1e-10, precision=precision))
+        eq.set_state(f.theta, f.eigVals, f.eigVecs)
+        eq.set_flow(f.U, f.Ub, f.phi)
+        eq.correct(1e-3, f.gradU if with_grad else None)
+        used = ctx.bytes_in_use - base + 2 * 6 * 8 * m.nCells  # + the server's result buffers
+        est = lib().rt_request_bytes_estimate(dm.h, int(precision == "mixed"), int(with_grad))
+        print(f"[case] estimate {est} used {used}", flush=True)
+        assert used <= est <= 1.6 * used
+        eq.close()
