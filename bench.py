@@ -339,6 +339,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--renumber", default="auto", choices=["auto", "none", "rcm", "sfc"],
                     help="device cell order (auto: RCM when the input order has poor locality)")
+    ap.add_argument("--no-colour", action="store_true",
+                    help="keep a two-colourable mesh in one block (no colour-by-colour row order)")
     ap.add_argument("--no-fp64-compare", action="store_true")
     ap.add_argument("--precond", default="DILU",
                     help="preconditioner keyword of the solves (DILU, multicolourILU0, diagonal, none)")
@@ -375,7 +377,8 @@ def main():
     ctx = abi.Context(0)
     t0 = time.perf_counter()
     dm = ctx.mesh(m, {"auto": abi.RT_RENUMBER_AUTO, "none": abi.RT_RENUMBER_NONE,
-                      "rcm": abi.RT_RENUMBER_RCM, "sfc": abi.RT_RENUMBER_SFC}[args.renumber])
+                      "rcm": abi.RT_RENUMBER_RCM, "sfc": abi.RT_RENUMBER_SFC}[args.renumber]
+                  | (abi.RT_RENUMBER_NO_COLOUR if args.no_colour else 0))
     log(f"[bench] mesh uploaded in {time.perf_counter() - t0:.1f} s")
     eq = new_model(ctx, dm, w, f, w["precision"])
     log(f"[bench] device bytes in use {ctx.bytes_in_use / 1e9:.2f} GB")

@@ -192,18 +192,21 @@ Controls parseControls(const char* text) {
         e->word("preconditioner", pc);
     }
     c.precondName = pc;
-    // DILU's sweep is sequential; the device replaces it by the diagonal
-    // (Jacobi) preconditioner, so iterates differ from the reference's and only
-    // converged fields are compared.  diagonal/Jacobi is the same
-    // preconditioner, none is none.  multicolourILU0 is ILU(0) in a two-colour
-    // ordering, run as the red-black Schur-complement solve (a mesh whose cell
-    // graph is two-colourable, solver PBiCGStab; otherwise refused at the
-    // solve).  DIC is a symmetric-matrix preconditioner and the theta matrix
+    // multicolourILU0 is ILU(0) in a two-colour ordering, run as the
+    // red-black Schur-complement solve (a mesh whose cell graph is
+    // two-colourable, solver PBiCGStab; otherwise refused at the solve).
+    // DILU's sweep is sequential; the device replaces it by that red-black
+    // ILU(0) where it can run and by the diagonal (Jacobi) preconditioner
+    // elsewhere, so iterates differ from the reference's and only converged
+    // fields are compared.  diagonal/Jacobi is the same preconditioner, none
+    // is none.  DIC is a symmetric-matrix preconditioner and the theta matrix
     // is asymmetric ([OF9] rejects it there).
-    if (pc == "DILU" || pc == "diagonal" || pc == "Jacobi") {
+    if (pc == "diagonal" || pc == "Jacobi") {
         c.precond = PC_JACOBI;
     } else if (pc == "none") {
         c.precond = PC_NONE;
+    } else if (pc == "DILU") {
+        c.precond = PC_MCILU0;
     } else if (pc == "multicolourILU0") {
         c.precond = PC_MCILU0;
         c.precondStrict = true;

@@ -4,7 +4,7 @@ runs through the oracle, and the files written at each write time read back
 equal to the device state.
 
 Bar: each step's solve stops at the normalised residual 1e-10 with a different
-preconditioner on each side (right Jacobi against DILU), so the two runs'
+preconditioner on each side (the device's red-black ILU0 against DILU), so the two runs'
 fields differ by up to ~1e-10 per step and the differences add up over a run;
 the single-step bar of 1e-9 relative L2 is applied per step taken (1e-8 after
 ten steps)."""

@@ -293,15 +293,17 @@ def test_symm_solve_matches_spsolve_fixtures(ctx, golden_dir):
 
 @pytest.mark.parametrize("solver,precond,precision", [
     ("PBiCG", "DILU", "double"), ("PBiCG", "DILU", "mixed"), ("PBiCG", "none", "double"),
-    ("PBiCG", "diagonal", "double"), ("PBiCGStab", "none", "double"), ("PBiCGStab", "none", "mixed")])
+    ("PBiCG", "diagonal", "double"), ("PBiCGStab", "none", "double"), ("PBiCGStab", "none", "mixed"),
+    ("PBiCGStab", "DILU", "double"), ("PBiCGStab", "DILU", "mixed")])
 def test_solver_and_preconditioner_keywords_match_oracle(ctx, solver, precond, precision):
     """fvSolution's solver and preconditioner keywords select the device
     algorithm: PBiCG ([OF9] PBiCG::solve, the tutorials' theta solver) and
-    PBiCGStab, with the diagonal preconditioner (DILU's device replacement) or
-    none.  Converged theta/tau against the reference algorithm (oracle,
-    PBiCG/PBiCGStab + DILU) within 1e-9.  With the same preconditioner on both
-    sides (diagonal, none) in FP64 the device runs the oracle's recurrence, so
-    the iteration counts agree too."""
+    PBiCGStab, with DILU's device replacement (the red-black ILU0 with
+    PBiCGStab on this two-colourable mesh, the diagonal preconditioner with
+    PBiCG), the diagonal preconditioner or none.  Converged theta/tau against
+    the reference algorithm (oracle, PBiCG/PBiCGStab + DILU) within 1e-9.
+    With the same preconditioner on both sides (diagonal, none) in FP64 the
+    device runs the oracle's recurrence, so the iteration counts agree too."""
     m, f = cases.box3d(20, lnLo=-3.0, lnHi=2.0, perm_seed=5, jitter=0.1)
     print(f"[case] {solver} {precond} {precision}", flush=True)
     dm = ctx.mesh(m)
@@ -655,6 +657,36 @@ def test_multicolour_ilu0_refused_where_it_cannot_run(ctx):
         with pytest.raises(abi.RheoError) as e:
             eq.correct(1e-3, ff.gradU if mesh is g else None)
         assert e.value.code == abi.RT_E_UNSUPPORTED and "multicolourILU0" in e.value.msg
+
+
+def test_dilu_keyword_runs_red_black_ilu0_where_it_can(ctx):
+    """preconditioner DILU runs the red-black ILU0 on a two-colourable mesh
+    with PBiCGStab (the same bits as multicolourILU0), and the diagonal
+    preconditioner where the red-black solve cannot run: a mesh kept in one
+    block, a cell graph that is not two-colourable, or solver PBiCG (the same
+    bits as diagonal)."""
+    m, f = cases.box3d(14, lnLo=-3.0, lnHi=2.0, perm_seed=2, jitter=0.1)
+    gm = cases.random_graph(3000, kNear=5, sigma=0.5, seed=3)
+    gf = cases.make_fields(gm, cases.FIELDS_TG, seed=11, lnLo=-2.0, lnHi=2.0)
+
+    def run(mesh, fields, precond, solver="PBiCGStab", renumber=0):
+        dm = ctx.mesh(mesh, renumber)
+        eq = ctx.constitutiveEq("", dm, abi.model_dict("GiesekusLog", lam=2.0, alpha=0.1))
+        eq.set_solver(abi.solver_dict(1e-10, precond=precond, precision="mixed", solver=solver))
+        eq.set_state(fields.theta, fields.eigVals, fields.eigVecs)
+        eq.set_flow(fields.U, fields.Ub, fields.phi)
+        assert eq.correct(1e-3, fields.gradU if mesh is gm else None) == abi.RT_OK
+        return eq.theta(), [p.nIterations for p in eq.solver_perf()]
+
+    print("[case] DILU keyword: red-black where it can run", flush=True)
+    dilu, mc, dg = run(m, f, "DILU"), run(m, f, "multicolourILU0"), run(m, f, "diagonal")
+    assert np.array_equal(dilu[0], mc[0]) and dilu[1] == mc[1]
+    assert sum(dilu[1]) < sum(dg[1]), (dilu[1], dg[1])
+    for kw in (dict(renumber=abi.RT_RENUMBER_NO_COLOUR), dict(solver="PBiCG")):
+        a, b = run(m, f, "DILU", **kw), run(m, f, "diagonal", **kw)
+        assert np.array_equal(a[0], b[0]) and a[1] == b[1], kw
+    a, b = run(gm, gf, "DILU"), run(gm, gf, "diagonal")
+    assert np.array_equal(a[0], b[0]) and a[1] == b[1]
 
 
 def test_colour_blocked_order_matches_one_block(ctx):

@@ -42,8 +42,8 @@ def run_alone(ctx, req, controls):
 
 
 @pytest.mark.parametrize("batching", [True, False])
-@pytest.mark.parametrize("precision,precond", [("mixed", "DILU"), ("double", "DILU"),
-                                               ("mixed", "multicolourILU0"), ("double", "multicolourILU0")])
+@pytest.mark.parametrize("precision,precond", [("mixed", "diagonal"), ("double", "diagonal"),
+                                               ("mixed", "DILU"), ("double", "multicolourILU0")])
 def test_same_result_alone_and_in_mix(precision, precond, batching):
     """Each request's result is the same bits alone (a blocking step) and in a
     mix of staggered arrivals, with the requests' Krylov rounds batched into
