@@ -1307,51 +1307,48 @@ void launch_tau_total(Ctx* ctx, const AssembleArgs& a, const double* tau, double
     RT_HIP(hipGetLastError());
 }
 
+// Workgroups of k_assemble<T, KIND> that fit on the chip at once (VGPRs allow
+// three per CU).  The kernel walks its tiles in a loop, so a grid larger than
+// that runs in rounds: the later workgroups start when the first have walked
+// their whole share, and neither walk keeps the neighbours it gathers in L2.
+template <class T, int KIND>
+static int resident_grid(const Ctx* ctx, int cap) {
+    static int perCU = 0;  // the same on every gfx950 device
+    if (perCU == 0) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_assemble<T, KIND>, BLK, 0) != hipSuccess || nb < 1)
+            nb = 1;
+        perCU = nb;
+    }
+    int64_t g = std::min<int64_t>(cap, (int64_t)ctx->numCUs * perCU);
+    if (g >= 64) g = g / 8 * 8;
+    return (int)std::max<int64_t>(1, g);
+}
+
+template <class T, int KIND>
+static int launch_assemble_one(Ctx* ctx, const AssembleArgs& a, int cap) {
+    const int grid = resident_grid<T, KIND>(ctx, cap);
+    hipLaunchKernelGGL((k_assemble<T, KIND>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
+    return grid;
+}
+
 template <class T>
-static void launch_assemble_kind(Ctx* ctx, const AssembleArgs& a, int grid) {
+static int launch_assemble_kind(Ctx* ctx, const AssembleArgs& a, int cap) {
     switch (a.prm.kind) {
-    case M_OLDROYDB:
-        hipLaunchKernelGGL((k_assemble<T, M_OLDROYDB>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
-        break;
-    case M_GIESEKUS:
-        hipLaunchKernelGGL((k_assemble<T, M_GIESEKUS>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
-        break;
-    case M_FENEP:
-        hipLaunchKernelGGL((k_assemble<T, M_FENEP>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
-        break;
-    case M_PTT_LINEAR:
-        hipLaunchKernelGGL((k_assemble<T, M_PTT_LINEAR>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
-        break;
-    case M_PTT_EXP:
-        hipLaunchKernelGGL((k_assemble<T, M_PTT_EXP>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
-        break;
-    case M_PTT_GEN:
-        hipLaunchKernelGGL((k_assemble<T, M_PTT_GEN>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
-        break;
-    case M_ROOTK:
-        hipLaunchKernelGGL((k_assemble<T, M_ROOTK>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
-        break;
-    case M_FENECR:
-        hipLaunchKernelGGL((k_assemble<T, M_FENECR>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
-        break;
-    case M_WMCY:
-        hipLaunchKernelGGL((k_assemble<T, M_WMCY>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
-        break;
-    case M_ROLIEPOLY:
-        hipLaunchKernelGGL((k_assemble<T, M_ROLIEPOLY>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
-        break;
-    case M_SARAMITO:
-        hipLaunchKernelGGL((k_assemble<T, M_SARAMITO>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
-        break;
-    case M_BMP:
-        hipLaunchKernelGGL((k_assemble<T, M_BMP>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
-        break;
-    case M_FLUIDITY:
-        hipLaunchKernelGGL((k_assemble<T, M_FLUIDITY>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
-        break;
-    default:
-        hipLaunchKernelGGL((k_assemble<T, M_XPP>), dim3(grid), dim3(BLK), 0, ctx->stream, a);
-        break;
+    case M_OLDROYDB: return launch_assemble_one<T, M_OLDROYDB>(ctx, a, cap);
+    case M_GIESEKUS: return launch_assemble_one<T, M_GIESEKUS>(ctx, a, cap);
+    case M_FENEP: return launch_assemble_one<T, M_FENEP>(ctx, a, cap);
+    case M_PTT_LINEAR: return launch_assemble_one<T, M_PTT_LINEAR>(ctx, a, cap);
+    case M_PTT_EXP: return launch_assemble_one<T, M_PTT_EXP>(ctx, a, cap);
+    case M_PTT_GEN: return launch_assemble_one<T, M_PTT_GEN>(ctx, a, cap);
+    case M_ROOTK: return launch_assemble_one<T, M_ROOTK>(ctx, a, cap);
+    case M_FENECR: return launch_assemble_one<T, M_FENECR>(ctx, a, cap);
+    case M_WMCY: return launch_assemble_one<T, M_WMCY>(ctx, a, cap);
+    case M_ROLIEPOLY: return launch_assemble_one<T, M_ROLIEPOLY>(ctx, a, cap);
+    case M_SARAMITO: return launch_assemble_one<T, M_SARAMITO>(ctx, a, cap);
+    case M_BMP: return launch_assemble_one<T, M_BMP>(ctx, a, cap);
+    case M_FLUIDITY: return launch_assemble_one<T, M_FLUIDITY>(ctx, a, cap);
+    default: return launch_assemble_one<T, M_XPP>(ctx, a, cap);
     }
 }
 
@@ -1364,12 +1361,12 @@ void launch_assemble(Ctx* ctx, const AssembleArgs& a, int grid) {
     const double by = (double)a.N * (48 + 96 + 32 + 8 + 48 + 8 + 8 + 12 * w + (a.thetaOld ? 48 : 0) +
                                      (a.tauOld ? 48 : 0) + (a.aux ? 8 : 0)) +
                       (double)a.nEll * (4 + 1 + 4 + 8 + 8 + (a.gradU ? 0 : 24 + 8));
+    int g;
     {
         ProfScope ps(ctx, a.mixed ? "k_assemble<f32>" : "k_assemble<f64>", by);
-        if (a.mixed) launch_assemble_kind<float>(ctx, a, grid);
-        else launch_assemble_kind<double>(ctx, a, grid);
+        g = a.mixed ? launch_assemble_kind<float>(ctx, a, grid) : launch_assemble_kind<double>(ctx, a, grid);
     }
-    hipLaunchKernelGGL(k_fin_assemble, dim3(1), dim3(BLK), 0, ctx->stream, a, grid);
+    hipLaunchKernelGGL(k_fin_assemble, dim3(1), dim3(BLK), 0, ctx->stream, a, g);
     RT_HIP(hipGetLastError());
 }
 void launch_eig_tau(Ctx* ctx, int32_t N, const ModelParams& prm, const double* theta,

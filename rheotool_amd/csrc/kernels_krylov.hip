@@ -1408,6 +1408,7 @@ struct SolveRun {
     KryArgs<float> af{};
     KryArgs<double> ad{};
     int G = 1, pgrid = 1;
+    int Gw = 1;                       // red-black: the first colour's gather passes (no partial sums)
     double N = 0, ne = 0, nValid = 0;
     int32_t valid[6];
     Ctx* ctx = nullptr;
@@ -1456,6 +1457,71 @@ KryArgs<float>& sweepOf<float>(SolveRun& R) { return R.sf; }
 template <>
 KryArgs<double>& sweepOf<double>(SolveRun& R) { return R.sd; }
 
+// Workgroups of the solve's row kernels (the passes that launch with the
+// solve grid and write partial sums) that fit on the chip at once: the least
+// over those kernels of the workgroups per CU their VGPRs allow, times the CUs.
+// Each workgroup walks its share of the tiles in a loop, so a grid larger than
+// that runs in rounds, and the workgroups of a later round find the
+// neighbours they gather gone from L2.
+template <class T>
+int solve_grid(const Ctx* ctx, int cap, bool rb, bool bicg) {
+    static int perCU[2][2] = {{0, 0}, {0, 0}};  // [rb][bicg], the same on every gfx950 device
+    int& pc = perCU[rb ? 1 : 0][bicg ? 1 : 0];
+    if (pc == 0) {
+        int least = 8;
+        auto take = [&](auto fn) {
+            int nb = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, BLK, 0) == hipSuccess && nb > 0)
+                least = std::min(least, nb);
+        };
+        take(k_residual<T, 0>);
+        take(k_residual<T, 1>);
+        if (bicg) {
+            take(k_bicg_init<T>);
+            take(k_bicg_w<T>);
+            take(k_bicg_xr<T>);
+        } else if (rb) {
+            take(k_rb_backsub<T>);
+            take(k_rb_w<T, 0>);
+            take(k_rb_w<T, 1>);
+            take(k_rb_v<T>);
+            take(k_rb_t<T>);
+            take(k_kry_xr<T>);
+        } else {
+            take(k_kry_v<T>);
+            take(k_kry_t<T>);
+            take(k_kry_xr<T>);
+        }
+        pc = least;
+    }
+    int64_t g = std::min<int64_t>(cap, (int64_t)ctx->numCUs * pc);
+    if (g >= 64) g = g / 8 * 8;
+    return (int)std::max<int64_t>(1, g);
+}
+
+// The red-black passes over the first colour's rows (w, w', the elimination)
+// reduce nothing, so they take a grid of their own: every workgroup their
+// VGPRs allow at once.
+template <class T>
+int rb_wide_grid(const Ctx* ctx, int cap) {
+    static int perCU = 0;
+    if (perCU == 0) {
+        int least = 8;
+        auto take = [&](auto fn) {
+            int nb = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, BLK, 0) == hipSuccess && nb > 0)
+                least = std::min(least, nb);
+        };
+        take(k_rb_w<T, 0>);
+        take(k_rb_w<T, 1>);
+        take(k_rb_backsub<T>);
+        perCU = least;
+    }
+    int64_t g = std::min<int64_t>(cap, (int64_t)ctx->numCUs * perCU);
+    if (g >= 64) g = g / 8 * 8;
+    return (int)std::max<int64_t>(1, g);
+}
+
 template <class T>
 void fill_args(Ctx* ctx, const SolveArgs& sa, SolveRun& R) {
     Mesh& m = *sa.mesh;
@@ -1495,7 +1561,9 @@ void fill_args(Ctx* ctx, const SolveArgs& sa, SolveRun& R) {
     a.cs = st->c;
     for (int c = 0; c < 6; ++c) a.valid[c] = sa.valid[c];
     a.ctl = kryCtl(*sa.ctl, mixed, m);
-    a.G = K.grid;
+    R.G = solve_grid<T>(ctx, K.grid, a.ctl.rb != 0, sa.ctl->solver == SV_PBICG);
+    R.Gw = a.ctl.rb ? rb_wide_grid<T>(ctx, K.grid) : R.G;
+    a.G = R.G;
     a.nRed = m.nRed;
     a.rowBegin = 0;
     // the preconditioner: Jacobi M = D (M^-1 = rD, A M^-1 has a unit diagonal)
@@ -1627,7 +1695,7 @@ void launch_refine(Ctx* ctx, SolveRun& R, double nRef, bool fold = true) {
         }
         {
             ProfScope ps(ctx, mixed ? "k_rb_backsub<f32>" : "k_rb_backsub<f64>", Br.rbx(nRef));
-            hipLaunchKernelGGL(k_rb_backsub<T>, dim3(R.G), dim3(BLK), 0, ctx->stream, a);
+            hipLaunchKernelGGL(k_rb_backsub<T>, dim3(R.Gw), dim3(BLK), 0, ctx->stream, a);
         }
         {
             ProfScope ps(ctx, mixed ? "k_residual<f32>" : "k_residual<f64>", Bb.res(nRef));
@@ -1690,7 +1758,7 @@ void launch_iter(Ctx* ctx, SolveRun& R, double nAct, bool start) {
         }
         {
             ProfScope ps(ctx, mixed ? "k_rb_w<f32>" : "k_rb_w<f64>", Br.rbw(nAct, 0));
-            hipLaunchKernelGGL((k_rb_w<T, 0>), dim3(R.G), dim3(BLK), 0, ctx->stream, a);
+            hipLaunchKernelGGL((k_rb_w<T, 0>), dim3(R.Gw), dim3(BLK), 0, ctx->stream, a);
         }
         {
             ProfScope ps(ctx, mixed ? "k_rb_v<f32>" : "k_rb_v<f64>", Bb.rbv(nAct));
@@ -1699,7 +1767,7 @@ void launch_iter(Ctx* ctx, SolveRun& R, double nAct, bool start) {
         hipLaunchKernelGGL(k_fin_v<T>, dim3(1), dim3(BLK), 0, ctx->stream, a);
         {
             ProfScope ps(ctx, mixed ? "k_rb_ws<f32>" : "k_rb_ws<f64>", Br.rbw(nAct, 1));
-            hipLaunchKernelGGL((k_rb_w<T, 1>), dim3(R.G), dim3(BLK), 0, ctx->stream, a);
+            hipLaunchKernelGGL((k_rb_w<T, 1>), dim3(R.Gw), dim3(BLK), 0, ctx->stream, a);
         }
         {
             ProfScope ps(ctx, mixed ? "k_rb_t<f32>" : "k_rb_t<f64>", Bb.rbt(nAct));
@@ -1905,8 +1973,8 @@ struct GroupDev {
     const KryArgs<T>* A;
     const KryArgs<T>* B;
     const KryArgs<T>* S;
-    const int32_t *G, *P, *One;
-    int n, totG, totP;
+    const int32_t *G, *P, *One, *W;
+    int n, totG, totP, totW;
 };
 
 template <class T, void (*Body)(const KryArgs<T>&, int, int)>
@@ -1921,14 +1989,14 @@ void group_round(bool rb, hipStream_t st, const GroupDev<T>& d) {
     const int n = d.n;
     if (rb) {
         glaunch<T, &k_fold_body<T>>(st, d.B, d.P, d.totP, n);
-        glaunch<T, &k_rb_backsub_body<T>>(st, d.A, d.G, d.totG, n);
+        glaunch<T, &k_rb_backsub_body<T>>(st, d.A, d.W, d.totW, n);
         glaunch<T, &k_residual_body<T, 1>>(st, d.S, d.G, d.totG, n);
         glaunch<T, &k_fin_residual_body<T, 1>>(st, d.A, d.One, n, n);
         glaunch<T, &k_kry_p_body<T>>(st, d.B, d.P, d.totP, n);
-        glaunch<T, &k_rb_w_body<T, 0>>(st, d.A, d.G, d.totG, n);
+        glaunch<T, &k_rb_w_body<T, 0>>(st, d.A, d.W, d.totW, n);
         glaunch<T, &k_rb_v_body<T>>(st, d.A, d.G, d.totG, n);
         glaunch<T, &k_fin_v_body<T>>(st, d.A, d.One, n, n);
-        glaunch<T, &k_rb_w_body<T, 1>>(st, d.A, d.G, d.totG, n);
+        glaunch<T, &k_rb_w_body<T, 1>>(st, d.A, d.W, d.totW, n);
         glaunch<T, &k_rb_t_body<T>>(st, d.A, d.G, d.totG, n);
         glaunch<T, &k_fin_t_body<T>>(st, d.A, d.One, n, n);
         glaunch<T, &k_kry_xr_body<T>>(st, d.B, d.G, d.totG, n);
@@ -1958,9 +2026,10 @@ template <class T>
 void group_launch(SolveGroup& g) {
     const int n = (int)g.members.size();
     const size_t szA = sizeof(KryArgs<T>) * n, szI = sizeof(int32_t) * (n + 1);
-    const size_t offB = szA, offS = 2 * szA, offG = 3 * szA, offP = offG + szI, offO = offP + szI;
-    const size_t bytes = offO + szI;
-    int totG = 0, totP = 0;
+    const size_t offB = szA, offS = 2 * szA, offG = 3 * szA, offP = offG + szI, offO = offP + szI,
+                 offW = offO + szI;
+    const size_t bytes = offW + szI;
+    int totG = 0, totP = 0, totW = 0;
     if (g.dirty) {
         if (g.hostBytes < bytes) {
             if (g.hostTables) (void)hipHostFree(g.hostTables);
@@ -1976,7 +2045,8 @@ void group_launch(SolveGroup& g) {
         auto* sG = reinterpret_cast<int32_t*>(h + offG);
         auto* sP = reinterpret_cast<int32_t*>(h + offP);
         auto* sO = reinterpret_cast<int32_t*>(h + offO);
-        sG[0] = sP[0] = sO[0] = 0;
+        auto* sW = reinterpret_cast<int32_t*>(h + offW);
+        sG[0] = sP[0] = sO[0] = sW[0] = 0;
         for (int i = 0; i < n; ++i) {
             SolveRun& R = *g.members[i].K->run;
             tA[i] = argsOf<T>(R);
@@ -1985,6 +2055,7 @@ void group_launch(SolveGroup& g) {
             sG[i + 1] = sG[i] + R.G;
             sP[i + 1] = sP[i] + point_grid(R);
             sO[i + 1] = i + 1;
+            sW[i + 1] = sW[i] + R.Gw;
         }
         // the previous launch's kernels have all finished (their pulls are back)
         RT_HIP(hipMemcpyAsync(g.tables.p, h, bytes, hipMemcpyHostToDevice, g.ctx->stream));
@@ -1994,12 +2065,13 @@ void group_launch(SolveGroup& g) {
         SolveRun& R = *mb.K->run;
         totG += R.G;
         totP += point_grid(R);
+        totW += R.Gw;
     }
     const unsigned char* d = static_cast<const unsigned char*>(g.tables.p);
     GroupDev<T> dv{reinterpret_cast<const KryArgs<T>*>(d), reinterpret_cast<const KryArgs<T>*>(d + offB),
                    reinterpret_cast<const KryArgs<T>*>(d + offS), reinterpret_cast<const int32_t*>(d + offG),
                    reinterpret_cast<const int32_t*>(d + offP), reinterpret_cast<const int32_t*>(d + offO),
-                   n, totG, totP};
+                   reinterpret_cast<const int32_t*>(d + offW), n, totG, totP, totW};
     for (int k = 0; k < kRoundsAhead; ++k) group_round<T>(g.rb, g.ctx->stream, dv);
     for (auto& mb : g.members) {
         SolveRun& R = *mb.K->run;
