@@ -71,13 +71,14 @@ def make_case(w, n=None):
 
 
 PRECOND = "DILU"  # the fvSolution preconditioner keyword the bench's solves name
+INNER_REL_TOL = None  # mixed precision: the inner solves' relative target (None: the library default)
 
 
 def new_model(ctx, dm, w, f, precision):
     from rheotool_amd import abi
     eq = ctx.constitutiveEq("", dm, abi.model_dict(w["model"], etaS=0.59, etaP=0.41, lam=w["lam"],
                                                    **w["kw"]))
-    eq.set_solver(abi.solver_dict(1e-10, precision=precision, precond=PRECOND))
+    eq.set_solver(abi.solver_dict(1e-10, precision=precision, precond=PRECOND, innerRelTol=INNER_REL_TOL))
     eq.set_state(f.theta, f.eigVals, f.eigVecs)
     eq.set_flow(f.U, f.Ub, f.phi)
     return eq
@@ -253,7 +254,7 @@ def serve_stream(ctx, nreq, rate, max_cells, pageable=False, upload_eigen=False,
             outs[i] = (ctx.pinned_empty((N, 6)), ctx.pinned_empty((N, 6)))
         log(f"[serve] page-locked request buffers ready in {time.perf_counter() - t1:.1f} s")
     kwmap = {"GiesekusLog": dict(alpha=0.2), "PTTLog": dict(epsilon=0.25, zeta=0.0)}
-    controls = abi.solver_dict(1e-10, precision="mixed", precond=PRECOND)
+    controls = abi.solver_dict(1e-10, precision="mixed", precond=PRECOND, innerRelTol=INNER_REL_TOL)
     srv = abi.Server(ctx, hbm_budget=200 << 30, max_inflight=16, batching=batching)
     rng = np.random.default_rng(5)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
@@ -342,6 +343,8 @@ def main():
     ap.add_argument("--no-colour", action="store_true",
                     help="keep a two-colourable mesh in one block (no colour-by-colour row order)")
     ap.add_argument("--no-fp64-compare", action="store_true")
+    ap.add_argument("--inner-rel-tol", type=float, default=None,
+                    help="mixed precision: relative target of each FP32 inner solve (innerRelTol)")
     ap.add_argument("--precond", default="DILU",
                     help="preconditioner keyword of the solves (DILU, multicolourILU0, diagonal, none)")
     ap.add_argument("--cpu-sample", type=int, default=64,
@@ -355,8 +358,9 @@ def main():
               "only the one-GPU path is built)", file=sys.stderr)
         sys.exit(2)
 
-    global PRECOND
+    global PRECOND, INNER_REL_TOL
     PRECOND = args.precond
+    INNER_REL_TOL = args.inner_rel_tol
     from rheotool_amd import abi
     if args.workload == "copy":
         # HBM calibration: a plain copy of 2 GiB each way

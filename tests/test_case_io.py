@@ -205,6 +205,60 @@ def prepare(tmp_path, name, vel):
     return dst
 
 
+def write_velocity(case, vel):
+    """0/U of a case directory from an analytic velocity: fixedValue on every
+    patch but the outlet (zeroGradient) and empty patches."""
+    pm = polymesh.case_mesh(case)
+    m = pm.to_mesh()
+    ent = {}
+    for pname, ptype, start, n in pm.patches:
+        if ptype == "empty":
+            ent[pname] = ("empty", None)
+        elif pname == "outlet":
+            ent[pname] = ("zeroGradient", None)
+        else:
+            ent[pname] = ("fixedValue", vel(m.Cf[start:start + n]))
+    polymesh.write_field(os.path.join(case, "0", "U"), "U", vel(m.C), pm, ent, dims="[0 1 -1 0 0 0 0]")
+    return m
+
+
+# the reference's own rheoFoam tutorials, read where they lie when the
+# reference tree is present (they are not copied into this repository)
+REF_TUTORIALS = "/root/reference/of90/tutorials/rheoFoam"
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_TUTORIALS), reason="the reference tree is not present")
+@pytest.mark.parametrize("case,vel,ncells", [("Cavity/Oldroyd-BLog", cavity_velocity, 127 * 127),
+                                             ("Channel/Oldroyd-BLog", channel_velocity, None)])
+def test_reference_tutorial_cases_mesh_parse_and_step(tmp_path, case, vel, ncells):
+    """The reference's rheoFoam tutorial cases as they stand (blockMeshDict,
+    fvSchemes, fvSolution, controlDict, constitutiveProperties, 0/theta; the
+    run scripts are not used) mesh, parse and run two stress steps through the
+    case runner, with an analytic velocity written as 0/U in place of the
+    momentum solution: the Oldroyd-B-Log model and its parameters, the
+    GaussDefCmpw scheme and the theta solver entries come from the tutorial's
+    own files, and theta stays finite and moves away from its initial zero
+    where the flow shears it."""
+    dst = str(tmp_path / "case")
+    for sub in ("0", "constant", "system"):
+        shutil.copytree(os.path.join(REF_TUTORIALS, case, sub), os.path.join(dst, sub))
+    m = write_velocity(dst, vel)
+    if ncells is not None:
+        assert m.nCells == ncells
+    props = foamdict.parse(open(os.path.join(dst, "constant", "constitutiveProperties")).read())
+    assert "Oldroyd-BLog" in str(props)
+    sc = oracle_backends.OracleStressCase(dst)
+    assert [md.name for md in sc.modes]
+    th0 = {md.name: np.array(sc.state()[0][md.name]["theta"]) for md in sc.modes}
+    sc.run(steps=2, write=False)
+    d, tau = sc.state()
+    for md in sc.modes:
+        th = d[md.name]["theta"]
+        assert th.shape == (m.nCells, 6) and np.all(np.isfinite(th))
+        assert np.abs(th - th0[md.name]).max() > 0
+    assert np.all(np.isfinite(tau))
+
+
 @pytest.mark.parametrize("name,vel", [("stresscase_cavity", cavity_velocity),
                                       ("stresscase_channel", channel_velocity),
                                       ("stresscase_bmp", channel_velocity)])
