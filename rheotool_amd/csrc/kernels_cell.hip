@@ -550,8 +550,10 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3, 8))
                 qs[12 + c][threadIdx.x] += fabs(y - pA) + fabs(src[c] - pA);
             }
         }
-        st6(rOut + (int64_t)NCMP * r, rt);
-        st6(r0Out + (int64_t)NCMP * r, rt);
+        if (rOut) {  // NULL: the solve's first sweep writes r and r0 itself (red-black)
+            st6(rOut + (int64_t)NCMP * r, rt);
+            st6(r0Out + (int64_t)NCMP * r, rt);
+        }
     }
     double q[19];
 #pragma unroll
@@ -1354,11 +1356,11 @@ static int launch_assemble_kind(Ctx* ctx, const AssembleArgs& a, int cap) {
 
 void launch_assemble(Ctx* ctx, const AssembleArgs& a, int grid) {
     // algorithmic bytes per cell: theta 48, eigen state 96, U 32, V 8, b 48, diag 8, 1/diag 8,
-    // r and r0 (2 x 6 x w), theta.oldTime() 48 more in inner iterations, the old tau 48 for
-    // Saramito; per ELL entry: face id 4, slot 1, column 4, phi 8, value 8
-    // (+ Sf 24 and w 8 for the gradient)
+    // r and r0 (2 x 6 x w, unless the solve writes them), theta.oldTime() 48 more in inner
+    // iterations, the old tau 48 for Saramito; per ELL entry: face id 4, slot 1, column 4,
+    // phi 8, value 8 (+ Sf 24 and w 8 for the gradient)
     const double w = a.mixed ? 4 : 8;
-    const double by = (double)a.N * (48 + 96 + 32 + 8 + 48 + 8 + 8 + 12 * w + (a.thetaOld ? 48 : 0) +
+    const double by = (double)a.N * (48 + 96 + 32 + 8 + 48 + 8 + 8 + (a.r ? 12 * w : 0) + (a.thetaOld ? 48 : 0) +
                                      (a.tauOld ? 48 : 0) + (a.aux ? 8 : 0)) +
                       (double)a.nEll * (4 + 1 + 4 + 8 + 8 + (a.gradU ? 0 : 24 + 8));
     int g;

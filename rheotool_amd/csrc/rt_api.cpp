@@ -571,10 +571,13 @@ void modelCorrectBegin(Model& M, double deltaT, const double* gradU, int iterati
     a.nnzOut = &st->misc[0];
     for (int q = 0; q < 6; ++q) M.valid6[q] = M.scalar() ? (q == 0 ? 1 : -1) : m.valid[q];
     for (int q = 0; q < 6; ++q) a.valid[q] = M.valid6[q];
-    a.r = mixed ? M.kry.rs.p : M.kry.r.p;
-    a.r0 = mixed ? M.kry.r0s.p : M.kry.r0.p;
     a.rD = M.kry.rD.as<double>();
     a.ctl = kryCtl(M.ctl, mixed, m);
+    // the red-black solve opens with a sweep that writes r and r0 on the
+    // second colour from the eliminated first colour; the assembly's own
+    // residual then only feeds the normalisation and the initial residual
+    a.r = a.ctl.rb ? nullptr : (mixed ? M.kry.rs.p : M.kry.r.p);
+    a.r0 = a.ctl.rb ? nullptr : (mixed ? M.kry.r0s.p : M.kry.r0.p);
     a.nRed = m.nRed;
     a.mixed = mixed ? 1 : 0;
     a.noConv = M.scheme == DS_NONE ? 1 : 0;
