@@ -799,10 +799,42 @@ void orc_rhs_tau(const orc_params* p, int32_t n, const double* gradU, const doub
 // fvm::ddt(theta) + fvm::div(phi, theta) == S (Oldroyd_BLog.C:141-164).
 // scheme: the GaussDefCmpw limiter; souT (NULL for upwind/none) the explicit
 // correction, entering the div matrix's source as -souT (:166).
+// [OF9] basicSymmetryFvPatchField::evaluate: (theta_P + T theta_P T^T)/2 with
+// T = I - 2 n n, n the unit normal of boundary face b (F + b in Sf)
+static void symmetryValue(const orc_mesh* m, int b, const double* th, double* out) {
+    const int64_t f = (int64_t)m->nInternal + b;
+    const double sx = m->Sf[3 * f], sy = m->Sf[3 * f + 1], sz = m->Sf[3 * f + 2];
+    const double mag = std::sqrt(sx * sx + sy * sy + sz * sz);
+    const double n[3] = {sx / mag, sy / mag, sz / mag};
+    double T[3][3], A[3][3], M[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) T[i][j] = (i == j ? 1.0 : 0.0) - 2.0 * n[i] * n[j];
+    A[0][0] = th[0]; A[0][1] = A[1][0] = th[1]; A[0][2] = A[2][0] = th[2];
+    A[1][1] = th[3]; A[1][2] = A[2][1] = th[4]; A[2][2] = th[5];
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) M[i][k] = T[i][0] * A[0][k] + T[i][1] * A[1][k] + T[i][2] * A[2][k];
+    static const int I6[6] = {0, 0, 0, 1, 1, 2}, J6[6] = {0, 1, 2, 1, 2, 2};
+    for (int q = 0; q < 6; ++q) {
+        const int i = I6[q], l = J6[q];
+        const double r = M[i][0] * T[l][0] + M[i][1] * T[l][1] + M[i][2] * T[l][2];
+        out[q] = (th[q] + r) / 2.0;
+    }
+}
+// [OF9] basicSymmetryFvPatchField::snGradTransformDiag for a symmTensor:
+// |n_i||n_j| of component (i, j); valueInternalCoeffs = 1 minus it
+static double symmetryInternalCoeff(const orc_mesh* m, int b, int q) {
+    const int64_t f = (int64_t)m->nInternal + b;
+    const double sx = m->Sf[3 * f], sy = m->Sf[3 * f + 1], sz = m->Sf[3 * f + 2];
+    const double mag = std::sqrt(sx * sx + sy * sy + sz * sz);
+    const double d[3] = {std::fabs(sx / mag), std::fabs(sy / mag), std::fabs(sz / mag)};
+    static const int I6[6] = {0, 0, 0, 1, 1, 2}, J6[6] = {0, 1, 2, 1, 2, 2};
+    return 1.0 - d[I6[q]] * d[J6[q]];
+}
+
 static void assembleImpl(const orc_mesh* m, const double* phi, const double* thetaOld, const double* S,
                          double deltaT, const int32_t* bcType, const double* bcValue, int scheme,
                          const double* souT, double* diag, double* lower, double* upper,
-                         double* source, double* iC, double* bC) {
+                         double* source, double* iC, double* bC, const double* thetaCur = nullptr) {
     const int N = m->nCells, F = m->nInternal;
     const double rDeltaT = 1.0 / deltaT;
     const bool bounded = (scheme & ORC_DC_BOUNDED) != 0;
@@ -841,13 +873,23 @@ static void assembleImpl(const orc_mesh* m, const double* phi, const double* the
     }
     for (int c = 0; c < N; ++c) diag[c] = ddtDiag[c] + divDiag[c];
     // patch coefficients with plim = 1 (:102-116)
+    const double* thP = thetaCur ? thetaCur : thetaOld;  // the patch fields' internal values
     for (int p = 0; p < m->nPatches; ++p)
         for (int k = 0; k < m->patchSize[p]; ++k) {
             const int b = m->patchStart[p] + k;
             const double pphi = phi[(int64_t)F + b];
+            double thb[6] = {0, 0, 0, 0, 0, 0};
+            const double* thc = thP + 6 * (size_t)m->owner[(int64_t)F + b];
+            if (conv && bcType[p] == ORC_BC_SYMMETRY) symmetryValue(m, b, thc, thb);
             for (int q = 0; q < 6; ++q) {
                 double ic = 0, bc = 0;
                 if (!conv) {
+                } else if (bcType[p] == ORC_BC_SYMMETRY) {
+                    // [OF9] transformFvPatchField::valueInternalCoeffs /
+                    // valueBoundaryCoeffs
+                    const double vic = symmetryInternalCoeff(m, b, q);
+                    ic = pphi * vic;
+                    bc = -pphi * (thb[q] - vic * thc[q]);
                 } else if (bcType[p] == ORC_BC_ZEROGRADIENT) {
                     ic = pphi * 1.0;
                     bc = -pphi * 0.0;
@@ -1396,6 +1438,8 @@ void orc_correct_aux(const orc_mesh* m, const orc_params* p, const double* U, co
                 for (int q = 0; q < 6; ++q)
                     thB[6 * (size_t)b + q] = bcType[pp] == ORC_BC_FIXEDVALUE ? bcValue[6 * (size_t)b + q]
                                                                               : theta[6 * (size_t)cell + q];
+                if (bcType[pp] == ORC_BC_SYMMETRY)
+                    symmetryValue(m, b, theta + 6 * (size_t)cell, thB.data() + 6 * (size_t)b);
             }
         orc_grad_gauss_symm(m, theta, thB.data(), g.data());
         orc_defc_faces(m, scheme & ~ORC_DC_BOUNDED, phi, theta, g.data(), C, dc.data());
@@ -1404,7 +1448,7 @@ void orc_correct_aux(const orc_mesh* m, const orc_params* p, const double* U, co
     }
     assembleImpl(m, phi, thetaOld ? thetaOld : theta, S.data(), deltaT, bcType, bcValue, scheme,
                  souT.empty() ? nullptr : souT.data(),
-                 diag.data(), lower.data(), upper.data(), source.data(), iC.data(), bC.data());
+                 diag.data(), lower.data(), upper.data(), source.data(), iC.data(), bC.data(), theta);
     if (p->model == ORC_FLUIDITY)  // == -fvm::Sp(1/lambda, Phi) (BMPLog.C:157)
         for (int cc = 0; cc < N; ++cc) diag[cc] = diag[cc] - (-(m->V[cc] * (1.0 / p->lambda)));
     if (relaxFactor > 0)

@@ -82,7 +82,14 @@ typedef struct {
  * convection puts phi_b*pos0(phi_b) on the diagonal (internalCoeffs) and
  * nothing in the source; -phi_b*(1 - pos0(phi_b)) multiplies the
  * neighbouring domain's value inside Amul (interfaceBouCoeffs). */
-enum { ORC_BC_ZEROGRADIENT = 0, ORC_BC_FIXEDVALUE = 1, ORC_BC_EMPTY = 2, ORC_BC_COUPLED = 3 };
+/* ORC_BC_SYMMETRY: symmetryPlane / symmetry ([OF9] basicSymmetryFvPatchField,
+ * a transformFvPatchField): the patch value is (theta_P + R theta_P R^T)/2,
+ * R = I - 2 n n; the upwind convection's internalCoeffs are phi_b times
+ * 1 - |n_i||n_j| per component (i, j) and its boundaryCoeffs -phi_b times
+ * (theta_b - (1 - |n_i||n_j|) theta_P) (gaussDefCmpwConvectionScheme.C:103-117
+ * through the patch field's valueInternalCoeffs/valueBoundaryCoeffs). */
+enum { ORC_BC_ZEROGRADIENT = 0, ORC_BC_FIXEDVALUE = 1, ORC_BC_EMPTY = 2, ORC_BC_COUPLED = 3,
+       ORC_BC_SYMMETRY = 4 };
 
 enum { ORC_PBICGSTAB = 0, ORC_PBICG = 1 };
 enum { ORC_PC_NONE = 0, ORC_PC_DIAGONAL = 1, ORC_PC_DILU = 2 };

@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
 OLDROYDB, GIESEKUS, FENEP, PTT_LINEAR, PTT_EXP = 0, 1, 2, 3, 4
-BC_ZEROGRADIENT, BC_FIXEDVALUE, BC_EMPTY, BC_COUPLED = 0, 1, 2, 3
+BC_ZEROGRADIENT, BC_FIXEDVALUE, BC_EMPTY, BC_COUPLED, BC_SYMMETRY = 0, 1, 2, 3, 4
 PBICGSTAB, PBICG = 0, 1
 PC_NONE, PC_DIAGONAL, PC_DILU = 0, 1, 2
 

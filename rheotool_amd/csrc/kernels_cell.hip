@@ -501,7 +501,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3, 8))
                 const int t = a.bcType[a.bPatch[bf]];
                 if (t == RT_BC_EMPTY) continue;
                 const double ph = a.noConv ? 0.0 : a.phiB[bf];
-                D += t == RT_BC_ZEROGRADIENT ? fabs(ph * 1.0) : fabs(ph * 0.0);
+                D += t != RT_BC_FIXEDVALUE ? fabs(ph * 1.0) : fabs(ph * 0.0);
             }
             D = fmax(fabs(D), sumOff);
             D /= a.relax;
@@ -510,7 +510,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3, 8))
                 const int t = a.bcType[a.bPatch[bf]];
                 if (t == RT_BC_EMPTY) continue;
                 const double ph = a.noConv ? 0.0 : a.phiB[bf];
-                D -= t == RT_BC_ZEROGRADIENT ? ph * 1.0 : ph * 0.0;
+                D -= t != RT_BC_FIXEDVALUE ? ph * 1.0 : ph * 0.0;
             }
 #pragma unroll
             for (int qq = 0; qq < 6; ++qq) src[qq] += (D - D0) * th[qq];
@@ -522,6 +522,19 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3, 8))
             const double ph = a.noConv ? 0.0 : a.phiB[bf];
             if (t == RT_BC_ZEROGRADIENT) {
                 D += ph * 1.0;
+            } else if (t == RT_BC_SYMMETRY) {
+                // the reference's coefficients are phi_b (1 - |n_i||n_j|) on the
+                // diagonal of component (i, j) and -phi_b (theta_b - (1 -
+                // |n_i||n_j|) theta_P) in its source; the lockstep solve shares
+                // one diagonal, so it takes phi_b and -phi_b (theta_b - theta_P),
+                // which differ from them only through phi_b (zero on a symmetry
+                // plane, up to rounding) times (theta_new - theta_P)
+                D += ph * 1.0;
+                const double sf[3] = {a.bSf[bf], a.bSf[NB + bf], a.bSf[2 * NB + bf]};
+                double sym[6];
+                symmetry_value(sf, th, sym);
+#pragma unroll
+                for (int c = 0; c < 6; ++c) src[c] += -ph * (sym[c] - th[c]);
             } else {
                 D += ph * 0.0;
 #pragma unroll
@@ -902,9 +915,12 @@ __global__ void __launch_bounds__(BLK) k_grad_theta(AssembleArgs a, double* __re
             const int t = a.bcType[a.bPatch[bf]];
             if (t == RT_BC_EMPTY) continue;
             const double sf[3] = {a.bSf[bf], a.bSf[NB + bf], a.bSf[2 * NB + bf]};
+            double sym[6];
+            if (t == RT_BC_SYMMETRY) symmetry_value(sf, th, sym);
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
-                const double tb = t == RT_BC_FIXEDVALUE ? a.bcVal[(int64_t)q * NB + bf] : th[q];
+                const double tb = t == RT_BC_FIXEDVALUE ? a.bcVal[(int64_t)q * NB + bf]
+                                                        : (t == RT_BC_SYMMETRY ? sym[q] : th[q]);
 #pragma unroll
                 for (int i = 0; i < 3; ++i) g[3 * q + i] += sf[i] * tb;
             }

@@ -1044,14 +1044,16 @@ int rt_model_set_bc(rt_model* model, int patch, int bcType, const double* values
     return guarded(M.ctx, [&] {
         Mesh& m = *M.mesh;
         if (patch < 0 || patch >= m.nPatches) throw Error{RT_E_BAD_ARG, "rt_model_set_bc: no such patch"};
-        if (bcType < RT_BC_ZEROGRADIENT || bcType > RT_BC_EMPTY)
+        if (bcType < RT_BC_ZEROGRADIENT || bcType > RT_BC_SYMMETRY)
             throw Error{RT_E_UNKNOWN_TYPE, "rt_model_set_bc: unknown patchField type"};
         const bool emptyPatch = m.patches[patch].kind == RT_PATCH_EMPTY;
         if (emptyPatch != (bcType == RT_BC_EMPTY))
             throw Error{RT_E_BAD_ARG, "rt_model_set_bc: empty patches take exactly the empty condition"};
         if (bcType == RT_BC_FIXEDVALUE && !values)
             throw Error{RT_E_BAD_ARG, "rt_model_set_bc: fixedValue needs values"};
-        M.hBcType[patch] = bcType;
+        // a scalar field's symmetry value is its cell value: the fluidity takes
+        // the zeroGradient form (the same coefficients while the patch flux is zero)
+        M.hBcType[patch] = bcType == RT_BC_SYMMETRY && M.scalar() ? RT_BC_ZEROGRADIENT : bcType;
         // copies on the model's stream, ordered with the work already queued
         // there; the host arrays are read before the call returns
         Ctx* c = M.ctx;

@@ -147,6 +147,32 @@ __device__ __forceinline__ void st6(T* p, const T* v) {
     q[0] = a; q[1] = b; q[2] = c;
 }
 
+// symmetryPlane / symmetry patch value of a symmTensor ([OF9]
+// basicSymmetryFvPatchField::evaluate): (theta_P + T theta_P T^T)/2 with
+// T = I - 2 n n, n = Sf/|Sf|; the same operations as the oracle's
+__device__ __forceinline__ void symmetry_value(const double sf[3], const double th[6], double out[6]) {
+    const double mag = sqrt(sf[0] * sf[0] + sf[1] * sf[1] + sf[2] * sf[2]);
+    const double n[3] = {sf[0] / mag, sf[1] / mag, sf[2] / mag};
+    double T[3][3], A[3][3], M[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) T[i][j] = (i == j ? 1.0 : 0.0) - 2.0 * n[i] * n[j];
+    A[0][0] = th[0]; A[0][1] = A[1][0] = th[1]; A[0][2] = A[2][0] = th[2];
+    A[1][1] = th[3]; A[1][2] = A[2][1] = th[4]; A[2][2] = th[5];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) M[i][k] = T[i][0] * A[0][k] + T[i][1] * A[1][k] + T[i][2] * A[2][k];
+    constexpr int I6[6] = {0, 0, 0, 1, 1, 2}, J6[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const int i = I6[q], l = J6[q];
+        const double r = M[i][0] * T[l][0] + M[i][1] * T[l][1] + M[i][2] * T[l][2];
+        out[q] = (th[q] + r) / 2.0;
+    }
+}
+
 // ------------------------------------------------- Krylov control (OpenFOAM)
 // SolverPerformance::checkConvergence: final < tolerance, or relTol > 0 and
 // final < relTol * initial

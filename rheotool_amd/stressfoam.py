@@ -39,7 +39,14 @@ import numpy as np
 
 from . import foamdict, polymesh
 
-BC_ZG, BC_FV, BC_EMPTY = 0, 1, 2
+BC_ZG, BC_FV, BC_EMPTY, BC_SYM = 0, 1, 2, 3
+SYMMETRY_TYPES = ("symmetryPlane", "symmetry")
+
+
+def _bc_entry(t, vals):
+    """The volField patch entry written back for a boundary condition code."""
+    return {BC_EMPTY: ("empty", None), BC_ZG: ("zeroGradient", None), BC_SYM: ("symmetryPlane", None)}.get(
+        t, ("fixedValue", vals))
 
 
 def _time_name(t, prec=6):
@@ -170,7 +177,10 @@ class StressCase:
             if md.bmp:
                 pf = polymesh.read_field(os.path.join(t0, "Phi" + md.name), self.pm)
                 md.Phi = pf.internal[:, 0].copy()
-                md.fbc = [(p, t, None if v is None else v[:, 0].copy()) for p, t, v in self._theta_bc(pf)]
+                # a scalar's symmetry value is its cell value (zeroGradient while the
+                # patch flux is zero)
+                md.fbc = [(p, BC_ZG if t == BC_SYM else t, None if v is None else v[:, 0].copy())
+                          for p, t, v in self._theta_bc(pf)]
         ut = os.path.join(path, u_time if u_time is not None else self.ctl.startName)
         self.U, self.Ub = self._velocity(ut)
         self.phi = self._flux(ut)
@@ -195,9 +205,11 @@ class StressCase:
                 out.append((i, BC_ZG, None))
             elif t == "fixedValue":
                 out.append((i, BC_FV, fld.patch_values(name, n)))
+            elif t in SYMMETRY_TYPES:
+                out.append((i, BC_SYM, None))
             else:
                 raise NotImplementedError(f"{fld.name} patch {name}: boundary type {t} is not supported "
-                                          "(zeroGradient, fixedValue, empty)")
+                                          "(zeroGradient, fixedValue, symmetryPlane, symmetry, empty)")
         return out
 
     def _velocity(self, tdir):
@@ -217,6 +229,13 @@ class StressCase:
                 Ub[s:s + n] = 0.0
             elif t == "zeroGradient":
                 Ub[s:s + n] = U[self.pm.owner[start:start + n]]
+            elif t in SYMMETRY_TYPES:
+                # basicSymmetry: (U_P + (I - 2 n n) U_P)/2, the cell value without
+                # its normal component
+                Uc = U[self.pm.owner[start:start + n]]
+                sf = self.m.Sf[start:start + n]
+                nh = sf / np.linalg.norm(sf, axis=1)[:, None]
+                Ub[s:s + n] = (Uc + (Uc - 2.0 * nh * np.sum(nh * Uc, axis=1)[:, None])) / 2.0
             else:
                 raise NotImplementedError(f"U patch {name}: type {t} without a value entry")
         return U, Ub
@@ -303,8 +322,7 @@ class StressCase:
             th_bc = {}
             for patch, t, vals in md.bc:
                 name = pm.patches[patch][0]
-                th_bc[name] = ("empty", None) if t == BC_EMPTY else (
-                    ("zeroGradient", None) if t == BC_ZG else ("fixedValue", vals))
+                th_bc[name] = _bc_entry(t, vals)
             polymesh.write_field(os.path.join(tdir, md.field), md.field, s["theta"], pm, th_bc, tname)
             calc = {n: ("empty", None) if th_bc[n][0] == "empty" else ("calculated", _face_vals(pm, s["tau"], n))
                     for n in th_bc}
@@ -317,8 +335,7 @@ class StressCase:
             if md.bmp:
                 f_bc = {}
                 for patch, t, vals in md.fbc:
-                    f_bc[pm.patches[patch][0]] = ("empty", None) if t == BC_EMPTY else (
-                        ("zeroGradient", None) if t == BC_ZG else ("fixedValue", vals))
+                    f_bc[pm.patches[patch][0]] = _bc_entry(t, vals)
                 polymesh.write_field(os.path.join(tdir, "Phi" + md.name), "Phi" + md.name, s["Phi"], pm, f_bc,
                                      tname, dims="[-1 1 1 0 0 0 0]")
         if len(self.modes) > 1:

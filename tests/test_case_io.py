@@ -228,14 +228,17 @@ REF_TUTORIALS = "/root/reference/of90/tutorials/rheoFoam"
 
 
 @pytest.mark.skipif(not os.path.isdir(REF_TUTORIALS), reason="the reference tree is not present")
-@pytest.mark.parametrize("case,vel,ncells", [("Cavity/Oldroyd-BLog", cavity_velocity, 127 * 127),
-                                             ("Channel/Oldroyd-BLog", channel_velocity, None)])
-def test_reference_tutorial_cases_mesh_parse_and_step(tmp_path, case, vel, ncells):
-    """The reference's rheoFoam tutorial cases as they stand (blockMeshDict,
+@pytest.mark.parametrize("case,vel,ncells,model", [
+    ("Cavity/Oldroyd-BLog", cavity_velocity, 127 * 127, "Oldroyd-BLog"),
+    ("Channel/Oldroyd-BLog", channel_velocity, None, "Oldroyd-BLog"),
+    # a symmetryPlane theta patch along the channel's centre line
+    ("../rheoHeatFoam/channel/PTTLog", channel_velocity, None, "PTTLog")])
+def test_reference_tutorial_cases_mesh_parse_and_step(tmp_path, case, vel, ncells, model):
+    """The reference's tutorial cases as they stand (blockMeshDict,
     fvSchemes, fvSolution, controlDict, constitutiveProperties, 0/theta; the
     run scripts are not used) mesh, parse and run two stress steps through the
     case runner, with an analytic velocity written as 0/U in place of the
-    momentum solution: the Oldroyd-B-Log model and its parameters, the
+    momentum solution: the model and its parameters, the
     GaussDefCmpw scheme and the theta solver entries come from the tutorial's
     own files, and theta stays finite and moves away from its initial zero
     where the flow shears it."""
@@ -246,7 +249,7 @@ def test_reference_tutorial_cases_mesh_parse_and_step(tmp_path, case, vel, ncell
     if ncells is not None:
         assert m.nCells == ncells
     props = foamdict.parse(open(os.path.join(dst, "constant", "constitutiveProperties")).read())
-    assert "Oldroyd-BLog" in str(props)
+    assert model in str(props)
     sc = oracle_backends.OracleStressCase(dst)
     assert [md.name for md in sc.modes]
     th0 = {md.name: np.array(sc.state()[0][md.name]["theta"]) for md in sc.modes}
@@ -261,7 +264,8 @@ def test_reference_tutorial_cases_mesh_parse_and_step(tmp_path, case, vel, ncell
 
 @pytest.mark.parametrize("name,vel", [("stresscase_cavity", cavity_velocity),
                                       ("stresscase_channel", channel_velocity),
-                                      ("stresscase_bmp", channel_velocity)])
+                                      ("stresscase_bmp", channel_velocity),
+                                      ("stresscase_symm", channel_velocity)])
 def test_runner_restart_matches_one_run(tmp_path, name, vel):
     a = prepare(tmp_path / "a", name, vel)
     b = prepare(tmp_path / "b", name, vel)

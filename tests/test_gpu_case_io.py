@@ -27,7 +27,8 @@ def rel(a, b):
 
 @pytest.mark.parametrize("name,vel", [("stresscase_cavity", cavity_velocity),
                                       ("stresscase_channel", channel_velocity),
-                                      ("stresscase_bmp", channel_velocity)])
+                                      ("stresscase_bmp", channel_velocity),
+                                      ("stresscase_symm", channel_velocity)])
 def test_case_run_device_matches_oracle(tmp_path, name, vel):
     print(f"[case] stress-only case run {name}, 10 steps", flush=True)
     dev_case = prepare(tmp_path / "dev", name, vel)

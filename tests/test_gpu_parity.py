@@ -1139,3 +1139,38 @@ def test_device_step_is_frame_indifferent_and_basis_free(ctx, model, kw):
         scale = np.abs(base[k]).max()
         assert np.abs(rot[k] - R @ base[k] @ Rt).max() / scale < 1e-11, (k, model)
         assert np.abs(basis[k] - base[k]).max() / scale < 1e-11, (k, model)
+
+
+@pytest.mark.parametrize("scheme,precision", [("upwind", "double"), ("cubista", "double"),
+                                              ("bounded smart", "mixed")])
+def test_symmetry_plane_matches_oracle_and_the_mirror(ctx, scheme, precision):
+    """A symmetryPlane theta patch (RT_BC_SYMMETRY): the half box cut at its
+    mirror plane against the oracle's step with the same patch (1e-9), and
+    against the device's own step on the whole box restricted to the half
+    (the mirror; 1e-9, the two solves stop at 1e-10 on different systems)."""
+    from test_oracle_symmetry import fields_of, halves
+    full, half, match = halves()
+    sym = half.patchNames.index("ymax")
+    p = oracle_params("GiesekusLog", 1.5, dict(alpha=0.2))
+    out = {}
+    for name, m, sp in (("half", half, sym), ("full", full, None)):
+        f = fields_of(m, sp)
+        print(f"[case] symmetryPlane {name} {scheme} {precision}", flush=True)
+        dm = ctx.mesh(m)
+        eq = ctx.constitutiveEq("", dm, abi.model_dict("GiesekusLog", lam=1.5, alpha=0.2))
+        eq.set_solver(abi.solver_dict(1e-10, precision=precision))
+        if scheme != "upwind":
+            eq.set_scheme(scheme if scheme.startswith("bounded") else "GaussDefCmpw " + scheme)
+        if sp is not None:
+            eq.set_bc(sp, abi.RT_BC_SYMMETRY)
+        eq.set_state(f.theta, f.eigVals, f.eigVecs)
+        eq.set_flow(f.U, f.Ub, f.phi)
+        assert eq.correct(0.02) == abi.RT_OK, eq.solver_perf()
+        out[name] = (eq.theta(), eq.tau(), f)
+    bc = O.default_bc(half)
+    bc[sym] = O.BC_SYMMETRY
+    th, ev, vec, tau, perf = O.correct(half, p, out["half"][2], 0.02, ctl=O.controls(tol=1e-10), bc=bc,
+                                       scheme=scheme)
+    assert rel_l2(out["half"][0], th) < 1e-9 and rel_l2(out["half"][1], tau) < 1e-9
+    assert rel_l2(out["half"][0], out["full"][0][match]) < 1e-9
+    assert rel_l2(out["half"][1], out["full"][1][match]) < 1e-9
