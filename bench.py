@@ -233,6 +233,9 @@ def serve_stream(ctx, nreq, rate, max_cells, pageable=False, upload_eigen=False,
         m = cases.cartesian(nx, ny, nz, twoD=nz == 1, channel=True)
         f = cases.make_fields(m, cases.FIELDS_CHANNEL, seed=seed_base + i)
         reqs.append([m, f, ctx.mesh(m), 0.1 / nx])
+        if (i + 1) % 32 == 0:
+            log(f"[serve] {i + 1} of {nreq} meshes and field sets prepared "
+                f"({time.perf_counter() - t0:.1f} s)")
     log(f"[serve] {nreq} meshes and field sets prepared in {time.perf_counter() - t0:.1f} s")
     outs = [None] * nreq
     if not pageable:
