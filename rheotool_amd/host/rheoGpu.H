@@ -110,7 +110,10 @@ private:
     mutable rt_mesh* dev_ = nullptr;
 };
 
-enum patchFieldType { zeroGradient = RT_BC_ZEROGRADIENT, fixedValue = RT_BC_FIXEDVALUE, empty = RT_BC_EMPTY };
+// the theta patch field types the device takes (symmetryPlane stands for
+// OpenFOAM's symmetryPlane and symmetry, both basicSymmetry patch fields)
+enum patchFieldType { zeroGradient = RT_BC_ZEROGRADIENT, fixedValue = RT_BC_FIXEDVALUE, empty = RT_BC_EMPTY,
+                      symmetryPlane = RT_BC_SYMMETRY };
 
 //- GeometricField<Type, fvPatchField, volMesh> with nCmpt components
 template <int nCmpt>
