@@ -216,7 +216,7 @@ def serve_specs(nreq, max_cells, seed=4):
 
 
 def serve_stream(ctx, nreq, rate, max_cells, pageable=False, upload_eigen=False, batching=True,
-                 seed_base=1000):
+                 seed_base=1000, max_inflight=16):
     """Continuous-batching stream: requests arrive as a Poisson process, each
     on its own mesh with its own fields (seed seed_base + i); they are
     admitted by the HBM budget and run overlapped, their Krylov rounds batched.
@@ -258,7 +258,7 @@ def serve_stream(ctx, nreq, rate, max_cells, pageable=False, upload_eigen=False,
         log(f"[serve] page-locked request buffers ready in {time.perf_counter() - t1:.1f} s")
     kwmap = {"GiesekusLog": dict(alpha=0.2), "PTTLog": dict(epsilon=0.25, zeta=0.0)}
     controls = abi.solver_dict(1e-10, precision="mixed", precond=PRECOND, innerRelTol=INNER_REL_TOL)
-    srv = abi.Server(ctx, hbm_budget=200 << 30, max_inflight=16, batching=batching)
+    srv = abi.Server(ctx, hbm_budget=200 << 30, max_inflight=max_inflight, batching=batching)
     rng = np.random.default_rng(5)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
     index_of = {}
@@ -302,7 +302,8 @@ def serve_bench(args):
     from rheotool_amd import abi
     ctx = abi.Context(0)
     r = serve_stream(ctx, args.requests, args.rate, args.max_cells, pageable=args.pageable,
-                     upload_eigen=args.upload_eigen, batching=not args.no_batching)
+                     upload_eigen=args.upload_eigen, batching=not args.no_batching,
+                     max_inflight=args.max_inflight)
     nreq = r["requests"]
     line = dict(metric=METRIC, value=r["value"], unit="cell-updates/s", n_gpus=1,
                 steps=nreq, warmup=0, ms_per_step=round(1e3 * r["wall_s"] / nreq, 3),
@@ -326,6 +327,7 @@ def main():
     ap.add_argument("--requests", type=int, default=512)
     ap.add_argument("--rate", type=float, default=100.0, help="serve: Poisson arrival rate, 1/s")
     ap.add_argument("--max-cells", type=float, default=2e6)
+    ap.add_argument("--max-inflight", type=int, default=16, help="serve: requests on the device at once")
     ap.add_argument("--pageable", action="store_true",
                     help="serve: requests in pageable host arrays instead of page-locked ones")
     ap.add_argument("--upload-eigen", action="store_true",
