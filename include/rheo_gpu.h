@@ -202,11 +202,17 @@ int rt_model_set_state(rt_model* model, const double* theta, const double* eigVa
 /* Boundary condition of theta on one patch: 0 zeroGradient, 1 fixedValue
  * (values [6*patchSize]), 2 empty, 3 symmetryPlane / symmetry (the patch value
  * (theta_P + R theta_P R^T)/2 with R = I - 2 n n, as OpenFOAM's
- * basicSymmetryFvPatchField evaluates it; the solve's coefficients there
- * differ from the reference's per-component ones only through the patch
- * flux, which the symmetry condition makes zero).  Default: zeroGradient
+ * basicSymmetryFvPatchField evaluates it), 4 wedge (the patch value
+ * T theta_P T^T, T the rotation that takes the wedge's centre-plane normal
+ * onto the patch's mean face normal, as OpenFOAM's wedgePolyPatch /
+ * wedgeFvPatchField compute it; the patch must be planar and its centre
+ * plane normal to a coordinate axis, else RT_E_BAD_ARG).  On the transform
+ * patches (3 and 4) the solve's coefficients differ from the reference's
+ * per-component ones only through the patch flux, which these conditions
+ * make zero.  values is read only for fixedValue.  Default: zeroGradient
  * (empty on empty patches). */
-enum rt_bc { RT_BC_ZEROGRADIENT = 0, RT_BC_FIXEDVALUE = 1, RT_BC_EMPTY = 2, RT_BC_SYMMETRY = 3 };
+enum rt_bc { RT_BC_ZEROGRADIENT = 0, RT_BC_FIXEDVALUE = 1, RT_BC_EMPTY = 2, RT_BC_SYMMETRY = 3,
+             RT_BC_WEDGE = 4 };
 int rt_model_set_bc(rt_model* model, int patch, int bcType, const double* values);
 
 /* The referenced flow fields (constitutiveEq.H:72-75 hold U and phi by

@@ -17,6 +17,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <omp.h>
@@ -820,6 +822,63 @@ static void symmetryValue(const orc_mesh* m, int b, const double* th, double* ou
         out[q] = (th[q] + r) / 2.0;
     }
 }
+// [OF9] wedgePolyPatch::calcGeometry; faceT = rotationTensor(cn, n) of
+// [OF9] transform.H: s I + (1 - s) a a / |a|^2 + (n cn - cn n), s = cn . n,
+// a = cn ^ n
+int orc_wedge_face_t(const orc_mesh* m, int32_t patch, double faceT[9]) {
+    const int n0 = m->patchStart[patch], nf = m->patchSize[patch];
+    if (nf <= 0) return 3;
+    double n[3] = {0, 0, 0};
+    for (int k = 0; k < nf; ++k) {
+        const int64_t f = (int64_t)m->nInternal + n0 + k;
+        const double* sf = m->Sf + 3 * f;
+        const double mag = std::sqrt(sf[0] * sf[0] + sf[1] * sf[1] + sf[2] * sf[2]);
+        for (int d = 0; d < 3; ++d) n[d] += sf[d] / mag;
+    }
+    for (int d = 0; d < 3; ++d) n[d] /= nf;  // gAverage
+    double cn[3];
+    for (int d = 0; d < 3; ++d) {
+        const double sg = n[d] > 0 ? 1.0 : (n[d] < 0 ? -1.0 : 0.0);
+        cn[d] = sg * (std::max(std::fabs(n[d]), 0.5) - 0.5);
+    }
+    const double mc = std::sqrt(cn[0] * cn[0] + cn[1] * cn[1] + cn[2] * cn[2]);
+    if (mc == 0) return 1;
+    for (int d = 0; d < 3; ++d) cn[d] /= mc;
+    if (std::fabs(cn[0] + cn[1] + cn[2]) < 1 - 1e-15) return 1;
+    const double s = cn[0] * n[0] + cn[1] * n[1] + cn[2] * n[2];
+    const double a[3] = {cn[1] * n[2] - cn[2] * n[1], cn[2] * n[0] - cn[0] * n[2], cn[0] * n[1] - cn[1] * n[0]};
+    const double ma = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+    if (std::sqrt(ma) < 1e-15) return 2;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            faceT[3 * i + j] = (i == j ? s : 0.0) + (1 - s) * a[i] * a[j] / ma + (n[i] * cn[j] - cn[i] * n[j]);
+    return 0;
+}
+// [OF9] wedgeFvPatchField::evaluate: transform(faceT, theta_P) = T theta_P T^T
+static void wedgeValue(const double* T, const double* th, double* out) {
+    double A[3][3], M[3][3];
+    A[0][0] = th[0]; A[0][1] = A[1][0] = th[1]; A[0][2] = A[2][0] = th[2];
+    A[1][1] = th[3]; A[1][2] = A[2][1] = th[4]; A[2][2] = th[5];
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) M[i][k] = T[3 * i] * A[0][k] + T[3 * i + 1] * A[1][k] + T[3 * i + 2] * A[2][k];
+    static const int I6[6] = {0, 0, 0, 1, 1, 2}, J6[6] = {0, 1, 2, 1, 2, 2};
+    for (int q = 0; q < 6; ++q) {
+        const int i = I6[q], l = J6[q];
+        out[q] = M[i][0] * T[3 * l] + M[i][1] * T[3 * l + 1] + M[i][2] * T[3 * l + 2];
+    }
+}
+// [OF9] wedgeFvPatchField::snGradTransformDiag for a symmTensor: d_i d_j,
+// d = diag(I - cellT)/2 with cellT = faceT & faceT; valueInternalCoeffs = 1
+// minus it
+static double wedgeInternalCoeff(const double* T, int q) {
+    double d[3];
+    for (int i = 0; i < 3; ++i) {
+        const double cii = T[3 * i] * T[i] + T[3 * i + 1] * T[3 + i] + T[3 * i + 2] * T[6 + i];
+        d[i] = 0.5 * (1.0 - cii);
+    }
+    static const int I6[6] = {0, 0, 0, 1, 1, 2}, J6[6] = {0, 1, 2, 1, 2, 2};
+    return 1.0 - d[I6[q]] * d[J6[q]];
+}
 // [OF9] basicSymmetryFvPatchField::snGradTransformDiag for a symmTensor:
 // |n_i||n_j| of component (i, j); valueInternalCoeffs = 1 minus it
 static double symmetryInternalCoeff(const orc_mesh* m, int b, int q) {
@@ -874,16 +933,28 @@ static void assembleImpl(const orc_mesh* m, const double* phi, const double* the
     for (int c = 0; c < N; ++c) diag[c] = ddtDiag[c] + divDiag[c];
     // patch coefficients with plim = 1 (:102-116)
     const double* thP = thetaCur ? thetaCur : thetaOld;  // the patch fields' internal values
-    for (int p = 0; p < m->nPatches; ++p)
+    for (int p = 0; p < m->nPatches; ++p) {
+        double wT[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+        if (bcType[p] == ORC_BC_WEDGE && m->patchSize[p] > 0 && orc_wedge_face_t(m, p, wT) != 0) {
+            std::fprintf(stderr, "oracle: patch %d is no valid wedge (orc_wedge_face_t)\n", p);
+            std::abort();
+        }
         for (int k = 0; k < m->patchSize[p]; ++k) {
             const int b = m->patchStart[p] + k;
             const double pphi = phi[(int64_t)F + b];
             double thb[6] = {0, 0, 0, 0, 0, 0};
             const double* thc = thP + 6 * (size_t)m->owner[(int64_t)F + b];
             if (conv && bcType[p] == ORC_BC_SYMMETRY) symmetryValue(m, b, thc, thb);
+            if (conv && bcType[p] == ORC_BC_WEDGE) wedgeValue(wT, thc, thb);
             for (int q = 0; q < 6; ++q) {
                 double ic = 0, bc = 0;
                 if (!conv) {
+                } else if (bcType[p] == ORC_BC_WEDGE) {
+                    // [OF9] transformFvPatchField::valueInternalCoeffs /
+                    // valueBoundaryCoeffs
+                    const double vic = wedgeInternalCoeff(wT, q);
+                    ic = pphi * vic;
+                    bc = -pphi * (thb[q] - vic * thc[q]);
                 } else if (bcType[p] == ORC_BC_SYMMETRY) {
                     // [OF9] transformFvPatchField::valueInternalCoeffs /
                     // valueBoundaryCoeffs
@@ -905,6 +976,7 @@ static void assembleImpl(const orc_mesh* m, const double* phi, const double* the
                 bC[6 * (size_t)b + q] = bc;
             }
         }
+    }
     // sources: ddt (rDeltaT*theta_old*V) [+ div: -souT] + (== S): V*S
     for (int c = 0; c < N; ++c)
         for (int q = 0; q < 6; ++q) {
@@ -1440,6 +1512,14 @@ void orc_correct_aux(const orc_mesh* m, const orc_params* p, const double* U, co
                                                                               : theta[6 * (size_t)cell + q];
                 if (bcType[pp] == ORC_BC_SYMMETRY)
                     symmetryValue(m, b, theta + 6 * (size_t)cell, thB.data() + 6 * (size_t)b);
+                if (bcType[pp] == ORC_BC_WEDGE) {
+                    double wT[9];
+                    if (orc_wedge_face_t(m, pp, wT) != 0) {
+                        std::fprintf(stderr, "oracle: patch %d is no valid wedge (orc_wedge_face_t)\n", pp);
+                        std::abort();
+                    }
+                    wedgeValue(wT, theta + 6 * (size_t)cell, thB.data() + 6 * (size_t)b);
+                }
             }
         orc_grad_gauss_symm(m, theta, thB.data(), g.data());
         orc_defc_faces(m, scheme & ~ORC_DC_BOUNDED, phi, theta, g.data(), C, dc.data());

@@ -88,8 +88,20 @@ typedef struct {
  * 1 - |n_i||n_j| per component (i, j) and its boundaryCoeffs -phi_b times
  * (theta_b - (1 - |n_i||n_j|) theta_P) (gaussDefCmpwConvectionScheme.C:103-117
  * through the patch field's valueInternalCoeffs/valueBoundaryCoeffs). */
+/* ORC_BC_WEDGE: wedge ([OF9] wedgeFvPatchField, a transformFvPatchField on a
+ * wedgePolyPatch): the patch value is faceT theta_P faceT^T
+ * (orc_wedge_face_t); internalCoeffs phi_b (1 - d_i d_j) per component (i, j)
+ * with d = diag(I - cellT)/2, cellT = faceT & faceT, and boundaryCoeffs
+ * -phi_b (theta_b - (1 - d_i d_j) theta_P). */
 enum { ORC_BC_ZEROGRADIENT = 0, ORC_BC_FIXEDVALUE = 1, ORC_BC_EMPTY = 2, ORC_BC_COUPLED = 3,
-       ORC_BC_SYMMETRY = 4 };
+       ORC_BC_SYMMETRY = 4, ORC_BC_WEDGE = 5 };
+
+/* [OF9] wedgePolyPatch::calcGeometry: faceT (row-major) of patch p, the
+ * rotation taking the centre-plane normal (the coordinate axis nearest the
+ * mean unit face normal n) onto n.  Returns 0, or 1 when the centre plane is
+ * not normal to a coordinate axis, 2 when n is a coordinate axis (no angle),
+ * 3 when the patch has no faces. */
+int orc_wedge_face_t(const orc_mesh* m, int32_t patch, double faceT[9]);
 
 enum { ORC_PBICGSTAB = 0, ORC_PBICG = 1 };
 enum { ORC_PC_NONE = 0, ORC_PC_DIAGONAL = 1, ORC_PC_DILU = 2 };

@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
 OLDROYDB, GIESEKUS, FENEP, PTT_LINEAR, PTT_EXP = 0, 1, 2, 3, 4
-BC_ZEROGRADIENT, BC_FIXEDVALUE, BC_EMPTY, BC_COUPLED, BC_SYMMETRY = 0, 1, 2, 3, 4
+BC_ZEROGRADIENT, BC_FIXEDVALUE, BC_EMPTY, BC_COUPLED, BC_SYMMETRY, BC_WEDGE = 0, 1, 2, 3, 4, 5
 PBICGSTAB, PBICG = 0, 1
 PC_NONE, PC_DIAGONAL, PC_DILU = 0, 1, 2
 
@@ -112,6 +112,8 @@ def lib():
         L.orc_decomp_correct.restype = C.c_int32
         L.orc_decomp_correct.argtypes = [vp, vp, C.c_double, vp, vp, vp, C.c_int32, vp]
         L.orc_decomp_get.argtypes = [vp] * 5
+        L.orc_wedge_face_t.argtypes = [vp, C.c_int32, vp]
+        L.orc_wedge_face_t.restype = C.c_int
         _LIB = L
     return _LIB
 
@@ -169,6 +171,16 @@ def controls(solver=PBICGSTAB, precond=PC_DILU, tol=1e-10, relTol=0.0, maxIter=1
 
 def default_bc(m):
     return np.where(m.patchKind == 2, BC_EMPTY, BC_ZEROGRADIENT).astype(np.int32)
+
+
+def wedge_face_t(m, patch):
+    """faceT of a wedge patch ([OF9] wedgePolyPatch), or raise ValueError."""
+    om = OracleMesh(m)
+    T = np.empty(9)
+    rc = lib().orc_wedge_face_t(om.ref, int(patch), _p(T))
+    if rc != 0:
+        raise ValueError(f"patch {patch} is no valid wedge (code {rc})")
+    return T.reshape(3, 3)
 
 
 def grad(m, U, Ub):

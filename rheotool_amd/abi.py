@@ -30,7 +30,7 @@ RT_W_NOT_CONVERGED = 0x100
 RT_THETA, RT_TAU, RT_EIGVALS, RT_EIGVECS = range(4)
 RT_RENUMBER_AUTO, RT_RENUMBER_NONE, RT_RENUMBER_RCM, RT_RENUMBER_SFC = range(4)
 RT_RENUMBER_NO_COLOUR = 16
-RT_BC_ZEROGRADIENT, RT_BC_FIXEDVALUE, RT_BC_EMPTY, RT_BC_SYMMETRY = range(4)
+RT_BC_ZEROGRADIENT, RT_BC_FIXEDVALUE, RT_BC_EMPTY, RT_BC_SYMMETRY, RT_BC_WEDGE = range(5)
 
 
 class RheoError(RuntimeError):

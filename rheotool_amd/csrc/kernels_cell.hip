@@ -535,6 +535,14 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3, 8))
                 symmetry_value(sf, th, sym);
 #pragma unroll
                 for (int c = 0; c < 6; ++c) src[c] += -ph * (sym[c] - th[c]);
+            } else if (t == RT_BC_WEDGE) {
+                // the same shared-diagonal form as symmetry: the reference's
+                // per-component coefficient is 1 - d_i d_j, d = diag(I - cellT)/2
+                D += ph * 1.0;
+                double wv[6];
+                wedge_value(a.patchT + 9 * (int64_t)a.bPatch[bf], th, wv);
+#pragma unroll
+                for (int c = 0; c < 6; ++c) src[c] += -ph * (wv[c] - th[c]);
             } else {
                 D += ph * 0.0;
 #pragma unroll
@@ -917,10 +925,12 @@ __global__ void __launch_bounds__(BLK) k_grad_theta(AssembleArgs a, double* __re
             const double sf[3] = {a.bSf[bf], a.bSf[NB + bf], a.bSf[2 * NB + bf]};
             double sym[6];
             if (t == RT_BC_SYMMETRY) symmetry_value(sf, th, sym);
+            else if (t == RT_BC_WEDGE) wedge_value(a.patchT + 9 * (int64_t)a.bPatch[bf], th, sym);
+            const bool transformed = t == RT_BC_SYMMETRY || t == RT_BC_WEDGE;
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
                 const double tb = t == RT_BC_FIXEDVALUE ? a.bcVal[(int64_t)q * NB + bf]
-                                                        : (t == RT_BC_SYMMETRY ? sym[q] : th[q]);
+                                                        : (transformed ? sym[q] : th[q]);
 #pragma unroll
                 for (int i = 0; i < 3; ++i) g[3 * q + i] += sf[i] * tb;
             }

@@ -496,6 +496,7 @@ AssembleArgs meshArgs(Model& M) {
     a.bPatch = m.bPatch.as<int32_t>();
     a.bcType = M.bcType.as<int32_t>();
     a.bcVal = M.bcVal.as<double>();
+    a.patchT = M.patchT.as<double>();
     a.NB = m.NB;
     a.bSf = m.bSf.as<double>();
     a.phiB = M.phi.as<double>() + m.F;
@@ -655,6 +656,53 @@ using namespace rt;
 
 // ======================================================================= C-ABI
 namespace {
+
+// The wedge patch's faceT ([OF9] wedgePolyPatch::calcGeometry): n the mean of
+// the patch's unit face normals; the centre-plane normal cn takes each
+// component of n to sign(n_i) (max(|n_i|, 1/2) - 1/2) and is normalised, so
+// it is the coordinate axis nearest n; faceT is the rotation taking cn onto n
+// ([OF9] transform.H rotationTensor: s I + (1 - s) a a / |a|^2 + n cn - cn n,
+// s = cn . n, a = cn x n).  A patch whose cn is no coordinate axis, or whose
+// n is one (no angle), is refused as the reference refuses it.
+void wedgeFaceT(Ctx* c, const Mesh& m, int patch, double T[9]) {
+    const rt_patch& p = m.patches[patch];
+    if (p.size <= 0) throw Error{RT_E_BAD_ARG, "rt_model_set_bc: wedge patch " + std::to_string(patch) + " has no faces"};
+    std::vector<double> sf((size_t)3 * p.size);
+    for (int d = 0; d < 3; ++d)
+        RT_HIP(hipMemcpyAsync(sf.data() + (size_t)d * p.size, m.bSf.as<double>() + (int64_t)d * m.NB + p.start,
+                              (size_t)p.size * 8, hipMemcpyDeviceToHost, c->stream));
+    RT_HIP(hipStreamSynchronize(c->stream));
+    double n[3] = {0, 0, 0};
+    for (int k = 0; k < p.size; ++k) {
+        const double x = sf[k], y = sf[(size_t)p.size + k], z = sf[(size_t)2 * p.size + k];
+        const double mag = std::sqrt(x * x + y * y + z * z);
+        n[0] += x / mag;
+        n[1] += y / mag;
+        n[2] += z / mag;
+    }
+    for (double& v : n) v /= p.size;
+    double cn[3];
+    for (int d = 0; d < 3; ++d) {
+        const double sg = n[d] > 0 ? 1.0 : (n[d] < 0 ? -1.0 : 0.0);
+        cn[d] = sg * (std::max(std::fabs(n[d]), 0.5) - 0.5);
+    }
+    const double mc = std::sqrt(cn[0] * cn[0] + cn[1] * cn[1] + cn[2] * cn[2]);
+    if (mc == 0) throw Error{RT_E_BAD_ARG, "rt_model_set_bc: wedge patch " + std::to_string(patch) +
+                                               " is not aligned with a coordinate axis"};
+    for (double& v : cn) v /= mc;
+    if (std::fabs(cn[0] + cn[1] + cn[2]) < 1 - 1e-15)
+        throw Error{RT_E_BAD_ARG, "rt_model_set_bc: wedge patch " + std::to_string(patch) +
+                                      " is not aligned with a coordinate axis"};
+    const double s = cn[0] * n[0] + cn[1] * n[1] + cn[2] * n[2];
+    const double ax[3] = {cn[1] * n[2] - cn[2] * n[1], cn[2] * n[0] - cn[0] * n[2], cn[0] * n[1] - cn[1] * n[0]};
+    const double ms = ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2];
+    if (std::sqrt(ms) < 1e-15)
+        throw Error{RT_E_BAD_ARG, "rt_model_set_bc: wedge patch " + std::to_string(patch) +
+                                      " lies in a coordinate plane (no wedge angle)"};
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            T[3 * i + j] = (i == j ? s : 0.0) + (1 - s) * ax[i] * ax[j] / ms + (n[i] * cn[j] - cn[i] * n[j]);
+}
 
 template <class F>
 int guarded(rt::Ctx* ctx, F&& f) {
@@ -989,7 +1037,9 @@ void allocModelState(Model& M) {
         M.ePhi = c->alloc((size_t)std::max<int64_t>(1, M.mesh->nEll) * 8);
         M.bcType = c->alloc((size_t)std::max(1, M.mesh->nPatches) * 4);
         M.bcVal = c->alloc((size_t)6 * std::max(1, NB) * 8);
+        M.patchT = c->alloc((size_t)9 * std::max(1, M.mesh->nPatches) * 8);
         RT_HIP(hipMemsetAsync(M.bcVal.p, 0, M.bcVal.bytes, c->stream));
+        RT_HIP(hipMemsetAsync(M.patchT.p, 0, M.patchT.bytes, c->stream));
         RT_HIP(hipMemsetAsync(M.tau.p, 0, M.tau.bytes, c->stream));
         M.hBcType.resize(M.mesh->nPatches);
         for (int p = 0; p < M.mesh->nPatches; ++p)
@@ -1044,19 +1094,28 @@ int rt_model_set_bc(rt_model* model, int patch, int bcType, const double* values
     return guarded(M.ctx, [&] {
         Mesh& m = *M.mesh;
         if (patch < 0 || patch >= m.nPatches) throw Error{RT_E_BAD_ARG, "rt_model_set_bc: no such patch"};
-        if (bcType < RT_BC_ZEROGRADIENT || bcType > RT_BC_SYMMETRY)
+        if (bcType < RT_BC_ZEROGRADIENT || bcType > RT_BC_WEDGE)
             throw Error{RT_E_UNKNOWN_TYPE, "rt_model_set_bc: unknown patchField type"};
         const bool emptyPatch = m.patches[patch].kind == RT_PATCH_EMPTY;
         if (emptyPatch != (bcType == RT_BC_EMPTY))
             throw Error{RT_E_BAD_ARG, "rt_model_set_bc: empty patches take exactly the empty condition"};
         if (bcType == RT_BC_FIXEDVALUE && !values)
             throw Error{RT_E_BAD_ARG, "rt_model_set_bc: fixedValue needs values"};
-        // a scalar field's symmetry value is its cell value: the fluidity takes
-        // the zeroGradient form (the same coefficients while the patch flux is zero)
-        M.hBcType[patch] = bcType == RT_BC_SYMMETRY && M.scalar() ? RT_BC_ZEROGRADIENT : bcType;
+        // a scalar field's symmetry or wedge value is its cell value: the
+        // fluidity takes the zeroGradient form (the same coefficients while the
+        // patch flux is zero)
+        const bool transform = bcType == RT_BC_SYMMETRY || bcType == RT_BC_WEDGE;
         // copies on the model's stream, ordered with the work already queued
         // there; the host arrays are read before the call returns
         Ctx* c = M.ctx;
+        if (bcType == RT_BC_WEDGE && !M.scalar()) {
+            double T[9];
+            wedgeFaceT(c, m, patch, T);
+            RT_HIP(hipMemcpyAsync(M.patchT.as<double>() + 9 * (int64_t)patch, T, sizeof T, hipMemcpyHostToDevice,
+                                  c->stream));
+            RT_HIP(hipStreamSynchronize(c->stream));
+        }
+        M.hBcType[patch] = transform && M.scalar() ? RT_BC_ZEROGRADIENT : bcType;
         RT_HIP(hipMemcpyAsync(M.bcType.as<int32_t>() + patch, &M.hBcType[patch], 4, hipMemcpyHostToDevice,
                               c->stream));
         std::vector<double> cols;

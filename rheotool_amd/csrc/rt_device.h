@@ -147,6 +147,25 @@ __device__ __forceinline__ void st6(T* p, const T* v) {
     q[0] = a; q[1] = b; q[2] = c;
 }
 
+// wedge patch value of a symmTensor ([OF9] wedgeFvPatchField::evaluate,
+// transform(faceT, theta_P)): T theta_P T^T, T (row-major) the patch's faceT;
+// the same operations as the oracle's
+__device__ __forceinline__ void wedge_value(const double* __restrict__ T, const double th[6], double out[6]) {
+    double A[3][3], M[3][3];
+    A[0][0] = th[0]; A[0][1] = A[1][0] = th[1]; A[0][2] = A[2][0] = th[2];
+    A[1][1] = th[3]; A[1][2] = A[2][1] = th[4]; A[2][2] = th[5];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) M[i][k] = T[3 * i] * A[0][k] + T[3 * i + 1] * A[1][k] + T[3 * i + 2] * A[2][k];
+    constexpr int I6[6] = {0, 0, 0, 1, 1, 2}, J6[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const int i = I6[q], l = J6[q];
+        out[q] = M[i][0] * T[3 * l] + M[i][1] * T[3 * l + 1] + M[i][2] * T[3 * l + 2];
+    }
+}
+
 // symmetryPlane / symmetry patch value of a symmTensor ([OF9]
 // basicSymmetryFvPatchField::evaluate): (theta_P + T theta_P T^T)/2 with
 // T = I - 2 n n, n = Sf/|Sf|; the same operations as the oracle's

@@ -335,6 +335,7 @@ struct Model {
     DBuf souT;                 // double [N][6] explicit correction source
     DBuf bcType;            // int32 [nPatches]
     DBuf bcVal;             // double [6][NB]
+    DBuf patchT;            // double [nPatches][9] faceT of the wedge patches
     std::vector<int32_t> hBcType;
     bool hasFlow = false, hasState = false;
     Krylov kry;
@@ -369,6 +370,7 @@ struct AssembleArgs {
     const int32_t* bPatch;
     const int32_t* bcType;
     const double* bcVal;  // [6][NB]
+    const double* patchT; // [nPatches][9] wedge faceT (row-major)
     int32_t NB;
     const double* bSf;    // [3][NB]
     const double* phiB;   // [NB] boundary face fluxes

@@ -111,9 +111,10 @@ private:
 };
 
 // the theta patch field types the device takes (symmetryPlane stands for
-// OpenFOAM's symmetryPlane and symmetry, both basicSymmetry patch fields)
+// OpenFOAM's symmetryPlane and symmetry, both basicSymmetry patch fields;
+// wedge takes its rotation from the patch's geometry)
 enum patchFieldType { zeroGradient = RT_BC_ZEROGRADIENT, fixedValue = RT_BC_FIXEDVALUE, empty = RT_BC_EMPTY,
-                      symmetryPlane = RT_BC_SYMMETRY };
+                      symmetryPlane = RT_BC_SYMMETRY, wedge = RT_BC_WEDGE };
 
 //- GeometricField<Type, fvPatchField, volMesh> with nCmpt components
 template <int nCmpt>

@@ -39,14 +39,34 @@ import numpy as np
 
 from . import foamdict, polymesh
 
-BC_ZG, BC_FV, BC_EMPTY, BC_SYM = 0, 1, 2, 3
+BC_ZG, BC_FV, BC_EMPTY, BC_SYM, BC_WEDGE = 0, 1, 2, 3, 4
 SYMMETRY_TYPES = ("symmetryPlane", "symmetry")
 
 
 def _bc_entry(t, vals):
     """The volField patch entry written back for a boundary condition code."""
-    return {BC_EMPTY: ("empty", None), BC_ZG: ("zeroGradient", None), BC_SYM: ("symmetryPlane", None)}.get(
-        t, ("fixedValue", vals))
+    return {BC_EMPTY: ("empty", None), BC_ZG: ("zeroGradient", None), BC_SYM: ("symmetryPlane", None),
+            BC_WEDGE: ("wedge", None)}.get(t, ("fixedValue", vals))
+
+
+def wedge_face_t(sf):
+    """faceT of a wedge patch from its face area vectors sf [n, 3] ([OF9]
+    wedgePolyPatch::calcGeometry): the rotation taking the centre-plane normal
+    (each component of the mean unit normal n taken to sign(n_i)(max(|n_i|,
+    1/2) - 1/2), normalised) onto n ([OF9] transform.H rotationTensor)."""
+    n = np.mean(sf / np.linalg.norm(sf, axis=1)[:, None], axis=0)
+    cn = np.sign(n) * (np.maximum(np.abs(n), 0.5) - 0.5)
+    if not np.any(cn):
+        raise ValueError("wedge patch not aligned with a coordinate axis")
+    cn = cn / np.linalg.norm(cn)
+    if abs(cn.sum()) < 1 - 1e-15:
+        raise ValueError("wedge patch not aligned with a coordinate axis")
+    s = float(cn @ n)
+    a = np.cross(cn, n)
+    ma = float(a @ a)
+    if np.sqrt(ma) < 1e-15:
+        raise ValueError("wedge patch lies in a coordinate plane (no wedge angle)")
+    return s * np.eye(3) + (1 - s) * np.outer(a, a) / ma + (np.outer(n, cn) - np.outer(cn, n))
 
 
 def _time_name(t, prec=6):
@@ -179,7 +199,7 @@ class StressCase:
                 md.Phi = pf.internal[:, 0].copy()
                 # a scalar's symmetry value is its cell value (zeroGradient while the
                 # patch flux is zero)
-                md.fbc = [(p, BC_ZG if t == BC_SYM else t, None if v is None else v[:, 0].copy())
+                md.fbc = [(p, BC_ZG if t in (BC_SYM, BC_WEDGE) else t, None if v is None else v[:, 0].copy())
                           for p, t, v in self._theta_bc(pf)]
         ut = os.path.join(path, u_time if u_time is not None else self.ctl.startName)
         self.U, self.Ub = self._velocity(ut)
@@ -207,9 +227,11 @@ class StressCase:
                 out.append((i, BC_FV, fld.patch_values(name, n)))
             elif t in SYMMETRY_TYPES:
                 out.append((i, BC_SYM, None))
+            elif t == "wedge":
+                out.append((i, BC_WEDGE, None))
             else:
                 raise NotImplementedError(f"{fld.name} patch {name}: boundary type {t} is not supported "
-                                          "(zeroGradient, fixedValue, symmetryPlane, symmetry, empty)")
+                                          "(zeroGradient, fixedValue, symmetryPlane, symmetry, wedge, empty)")
         return out
 
     def _velocity(self, tdir):
@@ -236,6 +258,10 @@ class StressCase:
                 sf = self.m.Sf[start:start + n]
                 nh = sf / np.linalg.norm(sf, axis=1)[:, None]
                 Ub[s:s + n] = (Uc + (Uc - 2.0 * nh * np.sum(nh * Uc, axis=1)[:, None])) / 2.0
+            elif t == "wedge":
+                # wedgeFvPatchField: transform(faceT, U_P)
+                T = wedge_face_t(self.m.Sf[start:start + n])
+                Ub[s:s + n] = U[self.pm.owner[start:start + n]] @ T.T
             else:
                 raise NotImplementedError(f"U patch {name}: type {t} without a value entry")
         return U, Ub

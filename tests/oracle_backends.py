@@ -72,7 +72,7 @@ class OracleStressCase(stressfoam.StressCase):
             bc = np.zeros(self.m.nPatches, np.int32)
             bcval = np.zeros((max(self.m.nBoundary, 1), 6))
             for patch, t, vals in md.bc:
-                bc[patch] = O.BC_SYMMETRY if t == stressfoam.BC_SYM else t
+                bc[patch] = {stressfoam.BC_SYM: O.BC_SYMMETRY, stressfoam.BC_WEDGE: O.BC_WEDGE}.get(t, t)
                 if vals is not None:
                     s = self.m.patchStart[patch]
                     bcval[s:s + len(vals)] = vals

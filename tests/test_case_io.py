@@ -186,6 +186,24 @@ def channel_velocity(C):
     return np.stack([1.5 * (1 - (2 * y - 1) ** 2), 0 * y, 0 * y], 1)
 
 
+def wedge_velocity(C):
+    """Axisymmetric flow about the x axis in the annulus 1/4 < r < 1: axial
+    u(r) vanishing on both walls, and the source flow u_r = 0.05 / r."""
+    x, y, z = C[:, 0], C[:, 1], C[:, 2]
+    r = np.hypot(y, z)
+    u = 7.1 * (r - 0.25) * (1.0 - r)
+    ur = 0.05 / r
+    return np.stack([u, ur * y / r, ur * z / r], 1)
+
+
+def _u_entry(pm, m, vel, pname, ptype, start, n):
+    if ptype in ("empty", "wedge"):
+        return (ptype, None)
+    if pname == "outlet":
+        return ("zeroGradient", None)
+    return ("fixedValue", vel(m.Cf[start:start + n]))
+
+
 def prepare(tmp_path, name, vel):
     """A copy of a fixture case with 0/U written from an analytic velocity."""
     dst = str(tmp_path / name)
@@ -193,31 +211,17 @@ def prepare(tmp_path, name, vel):
     pm = polymesh.case_mesh(dst)
     m = pm.to_mesh()
     U = vel(m.C)
-    ent = {}
-    for pname, ptype, start, n in pm.patches:
-        if ptype == "empty":
-            ent[pname] = ("empty", None)
-        elif pname == "outlet":
-            ent[pname] = ("zeroGradient", None)
-        else:
-            ent[pname] = ("fixedValue", vel(m.Cf[start:start + n]))
+    ent = {pname: _u_entry(pm, m, vel, pname, ptype, start, n) for pname, ptype, start, n in pm.patches}
     polymesh.write_field(os.path.join(dst, "0", "U"), "U", U, pm, ent, dims="[0 1 -1 0 0 0 0]")
     return dst
 
 
 def write_velocity(case, vel):
     """0/U of a case directory from an analytic velocity: fixedValue on every
-    patch but the outlet (zeroGradient) and empty patches."""
+    patch but the outlet (zeroGradient), empty and wedge patches."""
     pm = polymesh.case_mesh(case)
     m = pm.to_mesh()
-    ent = {}
-    for pname, ptype, start, n in pm.patches:
-        if ptype == "empty":
-            ent[pname] = ("empty", None)
-        elif pname == "outlet":
-            ent[pname] = ("zeroGradient", None)
-        else:
-            ent[pname] = ("fixedValue", vel(m.Cf[start:start + n]))
+    ent = {pname: _u_entry(pm, m, vel, pname, ptype, start, n) for pname, ptype, start, n in pm.patches}
     polymesh.write_field(os.path.join(case, "0", "U"), "U", vel(m.C), pm, ent, dims="[0 1 -1 0 0 0 0]")
     return m
 
@@ -265,7 +269,8 @@ def test_reference_tutorial_cases_mesh_parse_and_step(tmp_path, case, vel, ncell
 @pytest.mark.parametrize("name,vel", [("stresscase_cavity", cavity_velocity),
                                       ("stresscase_channel", channel_velocity),
                                       ("stresscase_bmp", channel_velocity),
-                                      ("stresscase_symm", channel_velocity)])
+                                      ("stresscase_symm", channel_velocity),
+                                      ("stresscase_wedge", wedge_velocity)])
 def test_runner_restart_matches_one_run(tmp_path, name, vel):
     a = prepare(tmp_path / "a", name, vel)
     b = prepare(tmp_path / "b", name, vel)

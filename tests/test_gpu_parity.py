@@ -1174,3 +1174,55 @@ def test_symmetry_plane_matches_oracle_and_the_mirror(ctx, scheme, precision):
     assert rel_l2(out["half"][0], th) < 1e-9 and rel_l2(out["half"][1], tau) < 1e-9
     assert rel_l2(out["half"][0], out["full"][0][match]) < 1e-9
     assert rel_l2(out["half"][1], out["full"][1][match]) < 1e-9
+
+
+@pytest.mark.parametrize("scheme,precision", [("upwind", "double"), ("cubista", "double"),
+                                              ("bounded smart", "mixed")])
+def test_wedge_matches_oracle_and_the_ring(ctx, scheme, precision):
+    """wedge theta patches (RT_BC_WEDGE) on a 6-degree sector about the x axis:
+    the device's step against the oracle's with the same patches (1e-9), and
+    the device's wedge against its own step on the full ring of 30 sectors,
+    which a wedge approximates to O(a^2): within twice the oracle's own
+    wedge-to-ring difference."""
+    from test_oracle_wedge import _ring_error, fields, mesh
+    w, ring = mesh(6.0, 1), mesh(6.0, 30)
+    wp = [w.patchNames.index("front"), w.patchNames.index("back")]
+    p = oracle_params("GiesekusLog", 0.8, dict(alpha=0.2))
+    out = {}
+    for name, m, pats in (("wedge", w, wp), ("ring", ring, [])):
+        f = fields(m)
+        print(f"[case] wedge {name} {scheme} {precision}", flush=True)
+        dm = ctx.mesh(m)
+        eq = ctx.constitutiveEq("", dm, abi.model_dict("GiesekusLog", lam=0.8, alpha=0.2))
+        eq.set_solver(abi.solver_dict(1e-10, precision=precision))
+        if scheme != "upwind":
+            eq.set_scheme(scheme if scheme.startswith("bounded") else "GaussDefCmpw " + scheme)
+        for q in pats:
+            eq.set_bc(q, abi.RT_BC_WEDGE)
+        eq.set_state(f.theta, f.eigVals, f.eigVecs)
+        eq.set_flow(f.U, f.Ub, f.phi)
+        assert eq.correct(0.02) == abi.RT_OK, eq.solver_perf()
+        out[name] = (eq.theta(), eq.tau(), f)
+    bc = O.default_bc(w)
+    bc[wp] = O.BC_WEDGE
+    th, ev, vec, tau, perf = O.correct(w, p, out["wedge"][2], 0.02, ctl=O.controls(tol=1e-10), bc=bc,
+                                       scheme=scheme)
+    assert rel_l2(out["wedge"][0], th) < 1e-9 and rel_l2(out["wedge"][1], tau) < 1e-9
+    key = lambda C: np.round(C * 1e8).astype(np.int64)
+    idx = {tuple(k): i for i, k in enumerate(key(ring.C))}
+    match = np.array([idx[tuple(k)] for k in key(w.C)])
+    e_orc = _ring_error(6.0, scheme)
+    assert np.abs(out["wedge"][0] - out["ring"][0][match]).max() < 2 * e_orc
+
+
+def test_wedge_on_a_plane_patch_is_refused(ctx):
+    """A wedge needs a patch turned off its coordinate plane: the x = 0 inlet
+    of the sector is refused with RT_E_BAD_ARG, and the model keeps working."""
+    from test_oracle_wedge import mesh
+    w = mesh(6.0, 1)
+    dm = ctx.mesh(w)
+    eq = ctx.constitutiveEq("", dm, abi.model_dict("GiesekusLog", lam=0.8, alpha=0.2))
+    with pytest.raises(abi.RheoError) as e:
+        eq.set_bc(w.patchNames.index("inlet"), abi.RT_BC_WEDGE)
+    assert e.value.code == abi.RT_E_BAD_ARG and "wedge" in e.value.msg
+    eq.set_bc(w.patchNames.index("front"), abi.RT_BC_WEDGE)
