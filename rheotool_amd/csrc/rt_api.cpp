@@ -1056,6 +1056,17 @@ void allocModelState(Model& M) {
     }
 }
 
+// multiMode models whose modes' solves may run as one group: two or more
+// modes on the model's own context, no fluidity equation (BMP solves it
+// before theta), and no profiling (a profiled run charges each launch to
+// one solve)
+bool multiGroupable(const Model& M) {
+    if (M.modes.size() < 2 || M.ctx->profiling) return false;
+    for (const rt_model* mode : M.modes)
+        if (mode->impl.ctx != M.ctx || mode->impl.fluid) return false;
+    return true;
+}
+
 }  // namespace rt
 
 extern "C" {
@@ -1221,6 +1232,61 @@ int rt_model_correct(rt_model* model, double deltaT, const double* gradU) {
 int rt_model_correct_iter(rt_model* model, double deltaT, const double* gradU, int iteration) {
     if (!model) return RT_E_BAD_ARG;
     Model& M = model->impl;
+    if (M.isMulti() && multiGroupable(M)) {
+        // multiMode::correct (multiMode.C:250-260) with the modes' Krylov
+        // rounds batched: every mode's assembly is queued, then their solves
+        // run as one group (one launch per kernel for all modes; each mode's
+        // reductions stay its own, so the fields are bit-identical to solving
+        // the modes in turn), and each mode's eigen/tau follows its own solve;
+        // then tau = the sum of the modes' tau (multiMode.C:219-227)
+        const int rc = guarded(M.ctx, [&] {
+            RT_HIP(hipSetDevice(M.ctx->device));
+            for (rt_model* mode : M.modes) {
+                try {
+                    modelCorrectBegin(mode->impl, deltaT, gradU, iteration);
+                } catch (const Error& e) {
+                    throw Error{e.code, "mode " + mode->impl.name + ": " + e.msg};
+                }
+            }
+            bool same = true, mixed0 = false, rb0 = false;
+            for (size_t i = 0; i < M.modes.size(); ++i) {
+                bool mx = false, rb = false;
+                if (!solve_groupable(M.modes[i]->impl.kry, &mx, &rb)) same = false;
+                if (i == 0) mixed0 = mx, rb0 = rb;
+                else if (mx != mixed0 || rb != rb0) same = false;
+            }
+            if (!same) {  // PBiCG, or modes with different solve forms: one after another
+                for (rt_model* mode : M.modes)
+                    while (!modelCorrectAdvance(mode->impl, true)) {
+                    }
+            } else {
+                SolveGroup G;
+                G.ctx = M.ctx;
+                G.mixed = mixed0;
+                G.rb = rb0;
+                for (rt_model* mode : M.modes) group_join(G, mode->impl.kry, M.ctx->stream);
+                std::vector<Krylov*> done;
+                while (!G.members.empty()) {
+                    RT_HIP(hipStreamSynchronize(M.ctx->stream));  // the queued rounds' states are back
+                    done.clear();
+                    group_advance(G, done);
+                    for (Krylov* K : done)
+                        for (rt_model* mode : M.modes)
+                            if (&mode->impl.kry == K) modelCorrectFinish(mode->impl);
+                }
+                RT_HIP(hipStreamSynchronize(M.ctx->stream));  // the group's tables are read no more
+            }
+            const int64_t n = (int64_t)6 * M.mesh->N;
+            launch_fill(M.ctx, M.tau.as<double>(), n, 0.0);
+            for (rt_model* mode : M.modes) launch_add(M.ctx, n, mode->impl.tau.as<double>(), M.tau.as<double>());
+            RT_HIP(hipStreamSynchronize(M.ctx->stream));
+        });
+        if (rc != RT_OK) return rc;
+        for (rt_model* mode : M.modes)
+            for (int q = 0; q < 6; ++q)
+                if (mode->impl.perf[q].solved && !mode->impl.perf[q].converged) return RT_W_NOT_CONVERGED;
+        return RT_OK;
+    }
     if (M.isMulti()) {
         // multiMode::correct (multiMode.C:250-260): each mode in turn, then
         // tau = sum of the modes' tau (multiMode.C:219-227)

@@ -1226,3 +1226,50 @@ def test_wedge_on_a_plane_patch_is_refused(ctx):
         eq.set_bc(w.patchNames.index("inlet"), abi.RT_BC_WEDGE)
     assert e.value.code == abi.RT_E_BAD_ARG and "wedge" in e.value.msg
     eq.set_bc(w.patchNames.index("front"), abi.RT_BC_WEDGE)
+
+
+@pytest.mark.parametrize("precision,precond", [("mixed", "DILU"), ("double", "DILU"), ("mixed", "diagonal"),
+                                               ("double", "none")])
+def test_multimode_batched_solves_equal_each_mode_alone(ctx, precision, precond):
+    """A multiMode model runs its modes' Krylov rounds as one batch (one
+    launch per kernel for all modes, each mode's reductions its own): every
+    mode's theta, tau and solver record equal, bit for bit, the same mode
+    stepped as a model of its own, over two steps."""
+    import dataclasses
+    m = cases.cartesian(12, 10, 9, jitter=0.1, perm_seed=5)
+    base = cases.make_fields(m, cases.FIELDS_TG, seed=31, lnLo=-2.0, lnHi=2.0)
+    spec = [("M1", "Oldroyd-BLog", dict(lam=0.8, etaS=0.1, etaP=0.3)),
+            ("M2", "GiesekusLog", dict(lam=2.5, alpha=0.2, etaS=0.05, etaP=0.55)),
+            ("M3", "PTTLog", dict(lam=0.3, epsilon=0.25, zeta=0.0, etaS=0.02, etaP=0.2))]
+    fs = []
+    for k in range(len(spec)):
+        g = cases.make_fields(m, cases.FIELDS_TG, seed=40 + k, lnLo=-2.0, lnHi=2.0)
+        fs.append(dataclasses.replace(base, theta=g.theta, eigVals=g.eigVals, eigVecs=g.eigVecs))
+    controls = abi.solver_dict(1e-10, precond=precond, precision=precision)
+    print(f"[case] multiMode batched {precision} {precond}", flush=True)
+    dm = ctx.mesh(m)
+    text = ("parameters\n{\n    type multiMode;\n    models\n    (\n" +
+            "\n".join(_mode_entry(n, mdl, **kw) for n, mdl, kw in spec) + "\n    );\n}\n")
+    eq = ctx.constitutiveEq("", dm, text)
+    for md, f in zip(eq.modes(), fs):
+        md.set_state(f.theta, f.eigVals, f.eigVecs)
+    eq.set_solver(controls)
+    eq.set_flow(base.U, base.Ub, base.phi)
+    alone = []
+    for (n, mdl, kw), f in zip(spec, fs):
+        a = ctx.constitutiveEq("", dm, abi.model_dict(mdl, **kw))
+        a.set_state(f.theta, f.eigVals, f.eigVecs)
+        a.set_solver(controls)
+        a.set_flow(base.U, base.Ub, base.phi)
+        alone.append(a)
+    for step in range(2):
+        assert eq.correct(2e-3) == abi.RT_OK
+        for a in alone:
+            assert a.correct(2e-3) == abi.RT_OK
+        for md, a in zip(eq.modes(), alone):
+            assert np.array_equal(md.theta(), a.theta()), (step, md)
+            assert np.array_equal(md.tau(), a.tau()), (step, md)
+            pm, pa = md.solver_perf(), a.solver_perf()
+            assert [(q.nIterations, q.finalResidual) for q in pm] == [(q.nIterations, q.finalResidual)
+                                                                      for q in pa]
+    assert np.array_equal(eq.tau(), alone[0].tau() + alone[1].tau() + alone[2].tau())
