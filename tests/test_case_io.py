@@ -236,7 +236,17 @@ REF_TUTORIALS = "/root/reference/of90/tutorials/rheoFoam"
     ("Cavity/Oldroyd-BLog", cavity_velocity, 127 * 127, "Oldroyd-BLog"),
     ("Channel/Oldroyd-BLog", channel_velocity, None, "Oldroyd-BLog"),
     # a symmetryPlane theta patch along the channel's centre line
-    ("../rheoHeatFoam/channel/PTTLog", channel_velocity, None, "PTTLog")])
+    ("../rheoHeatFoam/channel/PTTLog", channel_velocity, None, "PTTLog"),
+    ("Contraction41/Oldroyd-BLog", channel_velocity, None, "Oldroyd-BLog"),
+    ("CrossSlot/Oldroyd-BLog", channel_velocity, None, "Oldroyd-BLog"),
+    ("CrossSlot/PTTLog", channel_velocity, None, "PTTLog"),
+    ("OtherTests/Channel3D_VE/Oldroyd-BLog", channel_velocity, None, "Oldroyd-BLog"),
+    # every model of the 2D channel tests, multiMode included
+    *[("OtherTests/Channel2D_VE/" + d, channel_velocity, None, mdl) for d, mdl in [
+        ("FENE-CRLog", "FENE-CRLog"), ("FENE-PLog", "FENE-PLog"), ("GiesekusLog", "GiesekusLog"),
+        ("PTTLog/exponential", "PTTLog"), ("PTTLog/generalized", "PTTLog"), ("PTTLog/linear", "PTTLog"),
+        ("Rolie-PolyLog", "Rolie-PolyLog"), ("WhiteMetznerCYLog", "WhiteMetznerCYLog"),
+        ("XPomPomLog", "XPomPomLog"), ("multiMode", "multiMode"), ("otherModels/SaramitoLog", "SaramitoLog")]]])
 def test_reference_tutorial_cases_mesh_parse_and_step(tmp_path, case, vel, ncells, model):
     """The reference's tutorial cases as they stand (blockMeshDict,
     fvSchemes, fvSolution, controlDict, constitutiveProperties, 0/theta; the
