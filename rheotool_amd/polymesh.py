@@ -9,12 +9,19 @@ interpolation weights) is computed by the case library's `rtc_mesh_from_poly`
 (rheotool_amd/csrc/case/rtcase.cpp).
 
 The blockMesh subset builds what tutorial cases of the stress step use:
-hex blocks with straight edges, `simpleGrading` with an expansion ratio or
-multi-grading sections per direction, `convertToMeters`/`scale`, block faces
-merged by position, patches listed in `boundary` (or the older `patches`
-form), and the unlisted faces in `defaultPatch` (`defaultFaces`, empty).
-Curved `edges`, `edgeGrading`, `mergePatchPairs` and projection are not
-built: a blockMeshDict that uses them raises NotImplementedError.
+hex blocks, `simpleGrading` with an expansion ratio or multi-grading
+sections per direction, `arc` edges (a circular arc through its end points
+and one point on it), `convertToMeters`/`scale`, block faces merged by
+position, patches listed in `boundary` (or the older `patches` form), and
+the unlisted faces in `defaultPatch` (`defaultFaces`, empty).  A block's
+points follow its edges: the trilinear map of its corners plus, for each
+curved edge, the edge's offset from the straight line at the point's
+position along it, weighted by the point's distance from the opposite edges
+(transfinite interpolation from the twelve edges); with straight edges
+this is the trilinear map.  The points along an arc are placed by angle,
+which is arc length, with the block's grading.  Other edge types (spline,
+polyLine, projected), `edgeGrading`, `mergePatchPairs` and projection are
+not built: a blockMeshDict that uses them raises NotImplementedError.
 """
 from __future__ import annotations
 
@@ -211,6 +218,89 @@ def _grading_points(n, g):
     return x
 
 
+class _Arc:
+    """A circular arc from p0 through pm to p1; position(lam) is the point a
+    fraction lam of the way along it (by angle, which is arc length)."""
+
+    def __init__(self, p0, pm, p1):
+        a, b = p0 - pm, p1 - pm
+        axb = np.cross(a, b)
+        n2 = float(axb @ axb)
+        if n2 < 1e-30 * float(a @ a) * float(b @ b):
+            raise ValueError("arc: its three points are collinear")
+        self.c = pm + np.cross(float(a @ a) * b - float(b @ b) * a, axb) / (2.0 * n2)
+        r0 = p0 - self.c
+        self.r = float(np.linalg.norm(r0))
+        self.e1 = r0 / self.r
+        nrm = np.cross(p0 - self.c, pm - self.c)
+        nrm /= np.linalg.norm(nrm)
+        self.e2 = np.cross(nrm, self.e1)
+        q = p1 - self.c
+        ang = float(np.arctan2(q @ self.e2, q @ self.e1))
+        self.span = ang if ang > 0 else ang + 2.0 * np.pi
+
+    def position(self, lam):
+        t = np.asarray(lam, dtype=np.float64)[..., None] * self.span
+        return self.c + self.r * (np.cos(t) * self.e1 + np.sin(t) * self.e2)
+
+
+def _edges(d, V, scale):
+    """{(a, b): position(lam)} of the curved edges (vertex labels a -> b);
+    V are the scaled vertices, the points on the arcs are scaled here."""
+    out = {}
+    if "edges" not in d or not d["edges"] or not d["edges"][0]:
+        return out
+    toks = d["edges"][0]
+    i = 0
+    while i < len(toks):
+        kind = toks[i]
+        if kind != "arc":
+            raise NotImplementedError(f"edges: {kind!r} edges are not built (arc only)")
+        a, b = int(toks[i + 1]), int(toks[i + 2])
+        if not isinstance(toks[i + 3], list) or len(toks[i + 3]) != 3:
+            raise NotImplementedError("edges: an arc takes one point on it (not an origin form)")
+        pm = np.array(toks[i + 3], dtype=np.float64) * scale
+        out[(a, b)] = _Arc(V[a], pm, V[b]).position
+        i += 4
+    return out
+
+
+# block edges along each direction: (start corner, end corner, the two other
+# directions' sides 0/1)
+_HEX_EDGES = (
+    (0, [(0, 1, 0, 0), (3, 2, 1, 0), (4, 5, 0, 1), (7, 6, 1, 1)]),
+    (1, [(0, 3, 0, 0), (1, 2, 1, 0), (4, 7, 0, 1), (5, 6, 1, 1)]),
+    (2, [(0, 4, 0, 0), (1, 5, 1, 0), (3, 7, 0, 1), (2, 6, 1, 1)]),
+)
+
+
+def _curve_block(P, verts, X, lams, curved):
+    """Adds each curved edge's offset from its straight line to the block's
+    points P [nx+1, ny+1, nz+1, 3], blended by the other two parameters."""
+    for ax, edges in _HEX_EDGES:
+        o1, o2 = [q for q in range(3) if q != ax]
+        for c0, c1, s1, s2 in edges:
+            a, b = verts[c0], verts[c1]
+            if (a, b) in curved:
+                pos = curved[(a, b)](lams[ax])
+            elif (b, a) in curved:
+                pos = curved[(b, a)](1.0 - lams[ax])
+            else:
+                continue
+            lam = lams[ax][:, None]
+            off = pos - ((1.0 - lam) * X[c0] + lam * X[c1])  # [n+1, 3], zero at both ends
+            w1 = lams[o1] if s1 else 1.0 - lams[o1]
+            w2 = lams[o2] if s2 else 1.0 - lams[o2]
+            shape = [1, 1, 1, 3]
+            shape[ax] = len(lams[ax])
+            wshape1 = [1, 1, 1, 1]
+            wshape1[o1] = len(w1)
+            wshape2 = [1, 1, 1, 1]
+            wshape2[o2] = len(w2)
+            P += off.reshape(shape) * w1.reshape(wshape1) * w2.reshape(wshape2)
+    return P
+
+
 def _blocks(d):
     toks = d["blocks"][0]
     blocks, i = [], 0
@@ -252,14 +342,14 @@ def _patch_specs(d):
 
 def block_mesh(d: foamdict.FoamDict) -> PolyMesh:
     """polyMesh of a blockMeshDict (the subset described in the module doc)."""
-    for key in ("edges", "mergePatchPairs"):
-        if key in d and d[key] and d[key][0]:
-            raise NotImplementedError(f"{key} are not built")
+    if "mergePatchPairs" in d and d["mergePatchPairs"] and d["mergePatchPairs"][0]:
+        raise NotImplementedError("mergePatchPairs are not built")
     if "geometry" in d:
         raise NotImplementedError("projected geometry is not built")
     scale = d.scalar("convertToMeters", d.scalar("scale", 1.0))
     V = np.array(d["vertices"][0], dtype=np.float64) * scale
     blocks = _blocks(d)
+    curved = _edges(d, V, scale)
     # point grids per block, then merged by position
     grids, cell_pts, cell_off = [], [], [0]
     min_edge = np.inf
@@ -276,6 +366,8 @@ def block_mesh(d: foamdict.FoamDict) -> PolyMesh:
              + u * v * (1 - w) * X[2] + (1 - u) * v * (1 - w) * X[3]
              + (1 - u) * (1 - v) * w * X[4] + u * (1 - v) * w * X[5]
              + u * v * w * X[6] + (1 - u) * v * w * X[7])
+        if curved:
+            P = _curve_block(P, verts, X, (lx, ly, lz), curved)
         grids.append(P)
         for ax in range(3):
             dd = np.linalg.norm(np.diff(P, axis=ax), axis=-1)
@@ -412,11 +504,90 @@ def read_block_mesh(case):
     return block_mesh(foamdict.read(os.path.join(case, "system", "blockMeshDict")))
 
 
+def mirror_mesh(pm: PolyMesh, base, normal, tol=1e-7) -> PolyMesh:
+    """OpenFOAM's mirrorMesh: the mesh and its mirror image across the plane
+    through `base` with normal `normal`.  Points within tol of the plane stay;
+    the mirrored cells follow the originals (cell c's image is nCells + c);
+    boundary faces that lie in the plane become internal faces between a cell
+    and its image; every other face is copied mirrored with its points
+    reversed, so that its normal leaves its owner.  Internal faces are put in
+    upper-triangular order and boundary faces grouped by patch, as the
+    originals; patches left with no faces (the plane's) are dropped."""
+    n = np.asarray(normal, dtype=np.float64)
+    n = n / np.linalg.norm(n)
+    base = np.asarray(base, dtype=np.float64)
+    P = pm.points
+    dist = (P - base) @ n
+    onp = np.abs(dist) <= tol
+    nP, nC, nI = len(P), pm.nCells, pm.nInternal
+    # image of every point: itself on the plane, else a new point
+    img = np.arange(nP)
+    off = np.nonzero(~onp)[0]
+    img[off] = nP + np.arange(len(off))
+    points = np.concatenate([P, (P - 2.0 * dist[:, None] * n)[off]])
+    faces, own, nbr = [], [], []
+    for f in range(nI):
+        faces.append(np.asarray(pm.faces[f]))
+        own.append(pm.owner[f])
+        nbr.append(pm.neighbour[f])
+    for f in range(nI):
+        faces.append(img[np.asarray(pm.faces[f])][::-1])
+        own.append(pm.owner[f] + nC)
+        nbr.append(pm.neighbour[f] + nC)
+    bfaces = []  # (patch, face points, owner)
+    for pi, (name, ptype, start, size) in enumerate(pm.patches):
+        for f in range(start, start + size):
+            pts = np.asarray(pm.faces[f])
+            c = int(pm.owner[f])
+            if np.all(onp[pts]):
+                faces.append(pts)  # its normal leaves c, the original cell
+                own.append(c)
+                nbr.append(c + nC)
+            else:
+                bfaces.append((pi, pts, c))
+                bfaces.append((pi, img[pts][::-1], c + nC))
+    own, nbr = np.array(own), np.array(nbr)
+    order = np.lexsort((nbr, own))
+    faces = [faces[k] for k in order]
+    own, nbr = own[order], nbr[order]
+    patches = []
+    owner = list(own)
+    for pi, (name, ptype, start, size) in enumerate(pm.patches):
+        mine = [(pts, c) for q, pts, c in bfaces if q == pi]
+        if not mine:
+            continue
+        patches.append((name, ptype, len(faces), len(mine)))
+        for pts, c in mine:
+            faces.append(pts)
+            owner.append(c)
+    return PolyMesh(points, faces, np.array(owner), nbr, patches)
+
+
+def read_mirror_dict(case):
+    """(basePoint, normalVector, planeTolerance) of system/mirrorMeshDict
+    (planeType pointAndNormal), or None when the case has none."""
+    path = os.path.join(case, "system", "mirrorMeshDict")
+    if not os.path.isfile(path):
+        return None
+    d = foamdict.read(path)
+    if d.word("planeType") != "pointAndNormal":
+        raise NotImplementedError("mirrorMeshDict: only planeType pointAndNormal is built")
+    sub = d.sub("pointAndNormalDict")
+    return (np.array(sub["basePoint"][0], dtype=np.float64), np.array(sub["normalVector"][0], dtype=np.float64),
+            d.scalar("planeTolerance", 1e-7))
+
+
 def case_mesh(case) -> PolyMesh:
-    """constant/polyMesh when present, else the case's system/blockMeshDict."""
+    """constant/polyMesh when present, else the case's system/blockMeshDict;
+    a case meshed here from its blockMeshDict that carries a mirrorMeshDict
+    (its run script follows blockMesh with mirrorMesh) is mirrored too."""
     if os.path.isfile(os.path.join(case, "constant", "polyMesh", "faces")):
         return read_polymesh(case)
-    return read_block_mesh(case)
+    pm = read_block_mesh(case)
+    mirror = read_mirror_dict(case)
+    if mirror is not None:
+        pm = mirror_mesh(pm, *mirror)
+    return pm
 
 
 # -------------------------------------------------------------------- fields

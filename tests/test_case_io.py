@@ -136,10 +136,14 @@ def test_polymesh_rejects_inconsistent_descriptions():
 
 
 def test_unsupported_blockmesh_features_raise():
-    txt = BOX.format(nx=2, ny=2, nz=2, s=1, gx=1, gy=1, gz=1).replace(
-        "edges ();", "edges ( arc 0 1 (0.5 -0.1 0) );")
-    with pytest.raises(NotImplementedError):
-        polymesh.block_mesh(foamdict.parse(txt))
+    box = BOX.format(nx=2, ny=2, nz=2, s=1, gx=1, gy=1, gz=1)
+    for edges in ("edges ( spline 0 1 ((0.5 -0.1 0)) );", "edges ( polyLine 0 1 ((0.5 -0.1 0)) );",
+                  "edges ( arc 0 1 origin (0.5 -1 0) );"):
+        with pytest.raises(NotImplementedError):
+            polymesh.block_mesh(foamdict.parse(box.replace("edges ();", edges)))
+    # an arc through a point on it is built, and bows the edge out
+    pm = polymesh.block_mesh(foamdict.parse(box.replace("edges ();", "edges ( arc 0 1 (0.5 -0.1 0) );")))
+    assert pm.points[:, 1].min() < -0.09
 
 
 def test_field_files_round_trip(tmp_path):
@@ -238,6 +242,8 @@ REF_TUTORIALS = "/root/reference/of90/tutorials/rheoFoam"
     # a symmetryPlane theta patch along the channel's centre line
     ("../rheoHeatFoam/channel/PTTLog", channel_velocity, None, "PTTLog"),
     ("Contraction41/Oldroyd-BLog", channel_velocity, None, "Oldroyd-BLog"),
+    # arc edges around the cylinder, and the half mesh mirrored (mirrorMeshDict)
+    ("Cylinder/Oldroyd-BLog", channel_velocity, 24894, "Oldroyd-BLog"),
     ("CrossSlot/Oldroyd-BLog", channel_velocity, None, "Oldroyd-BLog"),
     ("CrossSlot/PTTLog", channel_velocity, None, "PTTLog"),
     ("OtherTests/Channel3D_VE/Oldroyd-BLog", channel_velocity, None, "Oldroyd-BLog"),
@@ -323,3 +329,96 @@ def test_oracle_parameters_of_the_case_models():
                        "dims (1 1 0); stabilization coupling; }").sub("parameters")
     p, _ = oracle_backends.oracle_params(s)
     assert (p.tau0, p.sarN, p.sarK, p.dimX, p.dimY, p.dimZ, p.sarPTT) == (2.5, 0.75, 1.5, 1.0, 1.0, 0.0, 0)
+
+
+def _block_dict(verts, nxyz, edges="", boundary="", grading="(1 1 1)"):
+    vt = " ".join("(%r %r %r)" % tuple(map(float, v)) for v in verts)
+    return foamdict.parse(f"convertToMeters 1;\nvertices ( {vt} );\nblocks ( hex (0 1 2 3 4 5 6 7) "
+                          f"({nxyz[0]} {nxyz[1]} {nxyz[2]}) simpleGrading {grading} );\n"
+                          f"edges ( {edges} );\nboundary ( {boundary} );\n")
+
+
+def test_block_mesh_arc_edges_lie_on_their_circles():
+    """A quarter annulus 1 < r < 2 whose inner and outer edges are arcs
+    through their 45-degree points: every point of those edges lies on its
+    circle, the cells fill the annulus' area less the chords' segments, and
+    halving the cell size quarters that shortfall (second order)."""
+    def area(n):
+        s = float(np.sqrt(0.5))
+        verts = [(1, 0, 0), (2, 0, 0), (0, 2, 0), (0, 1, 0), (1, 0, 1), (2, 0, 1), (0, 2, 1), (0, 1, 1)]
+        edges = (f"arc 0 3 ({s!r} {s!r} 0) arc 1 2 ({2 * s!r} {2 * s!r} 0) "
+                 f"arc 4 7 ({s!r} {s!r} 1) arc 5 6 ({2 * s!r} {2 * s!r} 1)")
+        pm = polymesh.block_mesh(_block_dict(verts, (4, n, 1), edges, grading="(2 1 1)"))
+        r = np.hypot(pm.points[:, 0], pm.points[:, 1])
+        inner, outer = np.abs(r - 1) < 1e-9, np.abs(r - 2) < 1e-9
+        assert inner.sum() == 2 * (n + 1) and outer.sum() == 2 * (n + 1)
+        m = pm.to_mesh()
+        assert m.V.min() > 0
+        return 0.75 * np.pi - m.V.sum()
+    d1, d2 = area(8), area(16)
+    assert d1 > 0 and 3.8 < d1 / d2 < 4.2
+
+
+def test_mirror_mesh_equals_the_whole_box():
+    """mirrorMesh of the lower half of a box across its mid-plane is the whole
+    box: the same cells (centres, volumes) and faces (centres, area vectors
+    per owner), the plane's faces turned internal, the patches' faces
+    doubled, and the oracle's step on it equal to the step on the box."""
+    verts = [(0, 0, 0), (1, 0, 0), (1, 0.5, 0), (0, 0.5, 0), (0, 0, 1), (1, 0, 1), (1, 0.5, 1), (0, 0.5, 1)]
+    bnd = ("walls { type wall; faces ( (0 1 5 4) ); } "
+           "ends { type patch; faces ( (0 4 7 3) (1 2 6 5) (0 3 2 1) (4 5 6 7) ); }")
+    half = polymesh.block_mesh(_block_dict(verts, (3, 2, 4), boundary=bnd))
+    full = polymesh.mirror_mesh(half, (0.3, 0.5, 0.7), (0, -2, 0))
+    assert [p[0] for p in full.patches] == ["walls", "ends"]
+    assert [p[3] for p in full.patches] == [2 * p[3] for p in half.patches[:2]]
+    verts2 = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+    bnd2 = ("walls { type wall; faces ( (0 1 5 4) (3 7 6 2) ); } "
+            "ends { type patch; faces ( (0 4 7 3) (1 2 6 5) (0 3 2 1) (4 5 6 7) ); }")
+    box = polymesh.block_mesh(_block_dict(verts2, (3, 4, 4), boundary=bnd2))
+    mf, mb = full.to_mesh(), box.to_mesh()
+    assert mf.nCells == mb.nCells and mf.nInternal == mb.nInternal
+    key = lambda C: [tuple(k) for k in np.round(C * 1e9).astype(np.int64)]
+    kb = {k: i for i, k in enumerate(key(mb.C))}
+    match = np.array([kb[k] for k in key(mf.C)])
+    assert np.allclose(mf.V, mb.V[match], rtol=0, atol=1e-15)
+    # each face's centre and its area vector as seen from the lower-numbered
+    # of its cells (in the box's numbering) agree
+    def face_set(m, cmap):
+        out = {}
+        for f in range(m.nInternal):
+            a, b = cmap[m.owner[f]], cmap[m.neighbour[f]]
+            sf = m.Sf[f] if a < b else -m.Sf[f]
+            out[tuple(np.round(m.Cf[f] * 1e9).astype(np.int64))] = np.round(sf * 1e9).astype(np.int64)
+        return out
+    ff, fb = face_set(mf, match), face_set(mb, np.arange(mb.nCells))
+    assert ff.keys() == fb.keys() and all(np.array_equal(ff[k], fb[k]) for k in ff)
+    th_f, th_b = _analytic_step(mf), _analytic_step(mb)
+    assert np.abs(th_f - th_b[match]).max() < 1e-12
+
+
+def _analytic_step(m):
+    """The oracle's Giesekus step for fields given by their position."""
+    from oracle import oracle as O
+    def theta_at(C):
+        x, y, z = C[:, 0], C[:, 1], C[:, 2]
+        return np.stack([0.3 + 0.2 * np.sin(2 * x + y), 0.1 * np.cos(y + z), 0.05 * x * z,
+                         -0.2 + 0.3 * y * y, 0.1 * np.sin(3 * z), 0.15 + 0.1 * np.cos(x * y)], 1)
+    def u_at(P):
+        x, y, z = P[:, 0], P[:, 1], P[:, 2]
+        return np.stack([1.0 + 0.5 * np.sin(np.pi * y) * z, 0.3 * np.cos(np.pi * x) * y, 0.2 * x * y], 1)
+    th = theta_at(m.C)
+    A = np.zeros((m.nCells, 3, 3))
+    A[:, [0, 0, 0, 1, 1, 2], [0, 1, 2, 1, 2, 2]] = th
+    A[:, [1, 2, 2], [0, 0, 1]] = th[:, [1, 2, 4]]
+    w, V = np.linalg.eigh(A)
+    ev = np.zeros_like(A)
+    ev[:, [0, 1, 2], [0, 1, 2]] = np.exp(w)
+    U = u_at(m.C)
+    F = m.nInternal
+    Ub = u_at(m.Cf[F:])
+    phi = np.empty(F + m.nBoundary)
+    phi[:F] = np.sum((m.w[:, None] * (U[m.owner[:F]] - U[m.neighbour]) + U[m.neighbour]) * m.Sf[:F], axis=1)
+    phi[F:] = np.sum(Ub * m.Sf[F:], axis=1)
+    f = cases.Fields(U=U, Ub=Ub, phi=phi, gradU=None, theta=th, eigVals=ev, eigVecs=V)
+    p = O.params("GiesekusLog", lam=0.7, alpha=0.2)
+    return O.correct(m, p, f, 0.01, ctl=O.controls(tol=1e-15, maxIter=500))[0]
