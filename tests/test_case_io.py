@@ -200,6 +200,11 @@ def wedge_velocity(C):
     return np.stack([u, ur * y / r, ur * z / r], 1)
 
 
+def rotation_velocity(C):
+    """Solid-body rotation about the z axis, u = (-y, x, 0)."""
+    return np.stack([-C[:, 1], C[:, 0], np.zeros(len(C))], 1)
+
+
 def _u_entry(pm, m, vel, pname, ptype, start, n):
     if ptype in ("empty", "wedge"):
         return (ptype, None)
@@ -286,7 +291,8 @@ def test_reference_tutorial_cases_mesh_parse_and_step(tmp_path, case, vel, ncell
                                       ("stresscase_channel", channel_velocity),
                                       ("stresscase_bmp", channel_velocity),
                                       ("stresscase_symm", channel_velocity),
-                                      ("stresscase_wedge", wedge_velocity)])
+                                      ("stresscase_wedge", wedge_velocity),
+                                      ("stresscase_arc", rotation_velocity)])
 def test_runner_restart_matches_one_run(tmp_path, name, vel):
     a = prepare(tmp_path / "a", name, vel)
     b = prepare(tmp_path / "b", name, vel)

@@ -16,7 +16,7 @@ import pytest
 import oracle_backends
 from rheotool_amd import polymesh, stressfoam
 
-from test_case_io import cavity_velocity, channel_velocity, prepare, wedge_velocity
+from test_case_io import cavity_velocity, channel_velocity, prepare, rotation_velocity, wedge_velocity
 
 pytestmark = pytest.mark.gpu
 
@@ -29,7 +29,8 @@ def rel(a, b):
                                       ("stresscase_channel", channel_velocity),
                                       ("stresscase_bmp", channel_velocity),
                                       ("stresscase_symm", channel_velocity),
-                                      ("stresscase_wedge", wedge_velocity)])
+                                      ("stresscase_wedge", wedge_velocity),
+                                      ("stresscase_arc", rotation_velocity)])
 def test_case_run_device_matches_oracle(tmp_path, name, vel):
     print(f"[case] stress-only case run {name}, 10 steps", flush=True)
     dev_case = prepare(tmp_path / "dev", name, vel)
