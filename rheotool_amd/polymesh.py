@@ -1,6 +1,8 @@
 """OpenFOAM case geometry and fields for the stress step: constant/polyMesh
-(ASCII) read and write, a blockMesh subset, and volField files.
+read and write, a blockMesh subset, and volField files.
 
+Files may be ASCII or OpenFOAM's binary format (little-endian, the label
+and scalar sizes of the header's `arch`; faces as a faceCompactList).
 polyMesh follows OpenFOAM's file layout (points, faces, owner, neighbour,
 boundary; internal faces first in upper-triangular order, boundary faces
 grouped by patch, face points ordered so that the right-hand normal leaves
@@ -90,7 +92,7 @@ def _strip(text):
 def _check_ascii(text, path):
     m = re.search(r"format\s+(\w+)\s*;", text)
     if m and m.group(1) != "ascii":
-        raise NotImplementedError(f"{path}: only ascii polyMesh files are read")
+        raise NotImplementedError(f"{path}: only ascii and binary files are read")
 
 
 def _list_body(text, path):
@@ -103,6 +105,68 @@ def _list_body(text, path):
     return n, text[m.end():end]
 
 
+# ------------------------------------------------------------ binary format
+# OpenFOAM's `format binary`: the header (ASCII) gives the sizes in `arch`
+# ("LSB;label=32;scalar=64"); a list is written as `N (` followed by the N
+# elements' raw bytes and `)`.  A faceCompactList (the faces file) is two
+# such label lists: the N + 1 offsets of the faces, then their points.
+_NCOMP_OF = {"label": 1, "scalar": 1, "vector": 3, "symmTensor": 6, "tensor": 9}
+
+
+def _header(raw: bytes):
+    m = re.search(rb"FoamFile\s*\{([^}]*)\}", raw)
+    head = m.group(1).decode("latin-1") if m else ""
+    fmt = re.search(r"format\s+(\w+)\s*;", head)
+    arch = re.search(r'arch\s+"([^"]*)"', head)
+    label, scalar = 4, 8
+    if arch:
+        a = arch.group(1)
+        if "MSB" in a:
+            raise NotImplementedError("big-endian binary files are not read")
+        lm, sm = re.search(r"label=(\d+)", a), re.search(r"scalar=(\d+)", a)
+        label = int(lm.group(1)) // 8 if lm else 4
+        scalar = int(sm.group(1)) // 8 if sm else 8
+    return (fmt.group(1) if fmt else "ascii"), label, scalar, (m.end() if m else 0)
+
+
+def _bin_list(raw: bytes, pos, dtype, ncomp, path):
+    """The binary list `N (bytes)` at or after pos: (array [N, ncomp], end)."""
+    m = re.compile(rb"(\d+)\s*\(").search(raw, pos)
+    if not m:
+        raise ValueError(f"{path}: no binary list found")
+    n = int(m.group(1))
+    nb = n * ncomp * np.dtype(dtype).itemsize
+    a = np.frombuffer(raw, dtype=dtype, count=n * ncomp, offset=m.end()).reshape(n, ncomp)
+    if raw[m.end() + nb:m.end() + nb + 1] != b")":
+        raise ValueError(f"{path}: binary list of {n} does not end where its size says")
+    return a.copy(), m.end() + nb + 1
+
+
+def _binary_to_ascii(raw: bytes, label, scalar, path):
+    """A binary field file's text with every `List<T> N (bytes)` written out
+    in ASCII, so that the ASCII reader takes it."""
+    out, pos = [], 0
+    pat = re.compile(rb"List<(\w+)>\s*(\d+)\s*\(")
+    while True:
+        m = pat.search(raw, pos)
+        if not m:
+            out.append(raw[pos:].decode("latin-1"))
+            return "".join(out)
+        t = m.group(1).decode()
+        nc = _NCOMP_OF.get(t)
+        if nc is None:
+            raise NotImplementedError(f"{path}: binary List<{t}> is not read")
+        dt = f"<i{label}" if t == "label" else f"<f{scalar}"
+        a, end = _bin_list(raw, m.start(2), dt, nc, path)
+        out.append(raw[pos:m.start()].decode("latin-1"))
+        if nc == 1:
+            body = " ".join(map(repr, a[:, 0].tolist()))
+        else:
+            body = " ".join("(" + " ".join(map(repr, r)) + ")" for r in a.tolist())
+        out.append(f"List<{t}> {len(a)} ( {body} )")
+        pos = end
+
+
 def _read_list_file(path):
     with open(path) as fh:
         raw = fh.read()
@@ -110,7 +174,22 @@ def _read_list_file(path):
     return _list_body(_strip(raw), path)
 
 
+def _read_binary(path):
+    """(raw bytes, label size, scalar size, header end) of a binary file, or
+    None for an ASCII one."""
+    with open(path, "rb") as fh:
+        raw = fh.read()
+    fmt, label, scalar, pos = _header(raw)
+    if fmt != "binary":
+        return None
+    return raw, label, scalar, pos
+
+
 def read_points(path):
+    b = _read_binary(path)
+    if b is not None:
+        raw, label, scalar, pos = b
+        return _bin_list(raw, pos, f"<f{scalar}", 3, path)[0].astype(np.float64)
     n, body = _read_list_file(path)
     pts = np.array(body.replace("(", " ").replace(")", " ").split(), dtype=np.float64).reshape(-1, 3)
     if len(pts) != n:
@@ -119,6 +198,10 @@ def read_points(path):
 
 
 def read_labels(path):
+    b = _read_binary(path)
+    if b is not None:
+        raw, label, scalar, pos = b
+        return _bin_list(raw, pos, f"<i{label}", 1, path)[0][:, 0].astype(np.int64)
     n, body = _read_list_file(path)
     lab = np.array(body.split(), dtype=np.int64)
     if len(lab) != n:
@@ -127,6 +210,15 @@ def read_labels(path):
 
 
 def read_faces(path):
+    b = _read_binary(path)
+    if b is not None:  # faceCompactList: offsets, then the points
+        raw, label, scalar, pos = b
+        off, pos = _bin_list(raw, pos, f"<i{label}", 1, path)
+        pts, _ = _bin_list(raw, pos, f"<i{label}", 1, path)
+        off, pts = off[:, 0].astype(np.int64), pts[:, 0].astype(np.int64)
+        if len(off) == 0 or off[-1] != len(pts):
+            raise ValueError(f"{path}: face offsets do not match the point list")
+        return [pts[off[i]:off[i + 1]] for i in range(len(off) - 1)]
     n, body = _read_list_file(path)
     tok = np.array(body.replace("(", " ").replace(")", " ").split(), dtype=np.int64)
     if n and len(tok) == 5 * n and np.all(tok[::5] == 4):  # all quads
@@ -162,7 +254,26 @@ def read_polymesh(case):
                     read_boundary(os.path.join(d, "boundary")))
 
 
-def write_polymesh(case, pm: PolyMesh):
+_BIN_HEADER = """FoamFile
+{{
+    format      binary;
+    arch        "LSB;label=32;scalar=64";
+    class       {cls};
+    location    "{loc}";
+    object      {obj};
+}}
+"""
+
+
+def _bin_block(a, dtype):
+    a = np.ascontiguousarray(a, dtype=dtype)
+    n = a.shape[0] if a.ndim else 0
+    return f"\n{n}\n(".encode() + a.tobytes() + b")\n"
+
+
+def write_polymesh(case, pm: PolyMesh, binary=False):
+    """constant/polyMesh of pm, ASCII or (binary=True) OpenFOAM's binary
+    format with 32-bit labels and 64-bit scalars (faces as faceCompactList)."""
     d = os.path.join(case, "constant", "polyMesh")
     os.makedirs(d, exist_ok=True)
     loc = "constant/polyMesh"
@@ -173,6 +284,27 @@ def write_polymesh(case, pm: PolyMesh):
             fh.write(f"\n{n}\n(\n")
             fh.writelines(rows)
             fh.write(")\n")
+
+    if binary:
+        def putb(name, cls, blocks):
+            with open(os.path.join(d, name), "wb") as fh:
+                fh.write(_BIN_HEADER.format(cls=cls, loc=loc, obj=name).encode())
+                for blk in blocks:
+                    fh.write(blk)
+
+        putb("points", "vectorField", [_bin_block(pm.points, "<f8")])
+        off = np.zeros(len(pm.faces) + 1, np.int64)
+        off[1:] = np.cumsum([len(f) for f in pm.faces])
+        flat = np.concatenate([np.asarray(f) for f in pm.faces]) if pm.faces else np.zeros(0, np.int64)
+        putb("faces", "faceCompactList", [_bin_block(off, "<i4"), _bin_block(flat, "<i4")])
+        putb("owner", "labelList", [_bin_block(pm.owner, "<i4")])
+        putb("neighbour", "labelList", [_bin_block(pm.neighbour, "<i4")])
+        rows = []
+        for n, t, s_, z in pm.patches:
+            rows.append(f"    {n}\n    {{\n        type            {t};\n"
+                        f"        nFaces          {z};\n        startFace       {s_};\n    }}\n")
+        put("boundary", "polyBoundaryMesh", len(pm.patches), rows)
+        return
 
     put("points", "vectorField", len(pm.points),
         (f"({p[0]!r} {p[1]!r} {p[2]!r})\n" for p in pm.points.tolist()))
@@ -645,9 +777,14 @@ class VolField:
 
 
 def read_field(path, pm: PolyMesh) -> VolField:
-    with open(path) as fh:
-        raw = fh.read()
-    _check_ascii(raw, path)
+    b = _read_binary(path)
+    if b is not None:
+        raw = _binary_to_ascii(b[0], b[1], b[2], path)
+    else:
+        with open(path) as fh:
+            raw = fh.read()
+    if b is None:
+        _check_ascii(raw, path)
     cls = re.search(r"class\s+(\w+)\s*;", raw).group(1)
     nc = {"volScalarField": 1, "volVectorField": 3, "volSymmTensorField": 6, "volTensorField": 9}[cls]
     fast = _internal_fast(raw, nc)
@@ -671,9 +808,13 @@ def read_field(path, pm: PolyMesh) -> VolField:
 
 def read_surface_field(path, pm: PolyMesh):
     """A surfaceScalarField (e.g. phi): (internal [nInternal], boundary [nBoundary])."""
-    with open(path) as fh:
-        raw = fh.read()
-    _check_ascii(raw, path)
+    b = _read_binary(path)
+    if b is not None:
+        raw = _binary_to_ascii(b[0], b[1], b[2], path)
+    else:
+        with open(path) as fh:
+            raw = fh.read()
+        _check_ascii(raw, path)
     fast = _internal_fast(raw, 1)
     internal = None
     if fast is not None:
@@ -700,11 +841,30 @@ def _fmt(row):
     return row[0] if len(row) == 1 else "(" + " ".join(row) + ")"
 
 
-def write_field(path, name, values, pm: PolyMesh, patch_entries, location="0", dims="[0 0 0 0 0 0 0]"):
-    """Writes a volField; patch_entries: patch -> (type, values or None)."""
+def write_field(path, name, values, pm: PolyMesh, patch_entries, location="0", dims="[0 0 0 0 0 0 0]",
+                binary=False):
+    """Writes a volField; patch_entries: patch -> (type, values or None).
+    binary=True writes OpenFOAM's binary format (64-bit scalars)."""
     values = np.asarray(values, dtype=np.float64).reshape(pm.nCells, -1)
     nc = values.shape[1]
     os.makedirs(os.path.dirname(path), exist_ok=True)
+    if binary:
+        with open(path, "wb") as fh:
+            fh.write(_BIN_HEADER.format(cls=CLASS[nc], loc=location, obj=name).encode())
+            fh.write(f"\ndimensions      {dims};\n\ninternalField   nonuniform List<{TYPE[nc]}> ".encode())
+            fh.write(_bin_block(values, "<f8").lstrip(b"\n"))
+            fh.write(b";\n\nboundaryField\n{\n")
+            for pname, _, _, nf in pm.patches:
+                t, v = patch_entries[pname]
+                fh.write(f"    {pname}\n    {{\n        type            {t};\n".encode())
+                if v is not None:
+                    v = np.asarray(v, dtype=np.float64).reshape(nf, nc)
+                    fh.write(f"        value           nonuniform List<{TYPE[nc]}> ".encode())
+                    fh.write(_bin_block(v, "<f8").lstrip(b"\n"))
+                    fh.write(b";\n")
+                fh.write(b"    }\n")
+            fh.write(b"}\n")
+        return
     with open(path, "w") as fh:
         fh.write(_HEADER.format(cls=CLASS[nc], loc=location, obj=name))
         fh.write(f"\ndimensions      {dims};\n\n")
