@@ -10,7 +10,7 @@ path, on a case directory:
 - system/fvSchemes gives `div(phi,theta<mode>)` (regex keys allowed, as in
   OpenFOAM), system/fvSolution the theta solver entry and the relaxation
   factor, system/controlDict the time controls (startFrom, deltaT, endTime,
-  writeControl/writeInterval, timePrecision);
+  writeControl/writeInterval, timePrecision, writeFormat ascii or binary);
 - <start>/theta<mode> (MUST_READ) with zeroGradient, fixedValue and empty
   patches; eigVals<mode>/eigVecs<mode> when present (READ_IF_PRESENT);
   tau<mode> for SaramitoLog and BMPLog (MUST_READ: their sources read it);
@@ -92,6 +92,7 @@ class Controls:
         self.writeControl = cd.word("writeControl", "timeStep")
         self.writeInterval = cd.scalar("writeInterval", 1.0)
         self.timePrecision = int(cd.scalar("timePrecision", 6.0))
+        self.binary = cd.word("writeFormat", "ascii") == "binary"
         start = cd.word("startFrom", "startTime")
         times = _time_dirs(case)
         if start == "latestTime" and times:
@@ -338,6 +339,9 @@ class StressCase:
         tau = sum(s["tau"] for s in out.values())
         return out, tau
 
+    def _write_field(self, *args, **kw):
+        polymesh.write_field(*args, binary=self.ctl.binary, **kw)
+
     def write(self):
         tname = _time_name(self.time, self.ctl.timePrecision)
         tdir = os.path.join(self.path, tname)
@@ -349,25 +353,25 @@ class StressCase:
             for patch, t, vals in md.bc:
                 name = pm.patches[patch][0]
                 th_bc[name] = _bc_entry(t, vals)
-            polymesh.write_field(os.path.join(tdir, md.field), md.field, s["theta"], pm, th_bc, tname)
+            self._write_field(os.path.join(tdir, md.field), md.field, s["theta"], pm, th_bc, tname)
             calc = {n: ("empty", None) if th_bc[n][0] == "empty" else ("calculated", _face_vals(pm, s["tau"], n))
                     for n in th_bc}
-            polymesh.write_field(os.path.join(tdir, "tau" + md.name), "tau" + md.name, s["tau"], pm, calc,
+            self._write_field(os.path.join(tdir, "tau" + md.name), "tau" + md.name, s["tau"], pm, calc,
                                  tname, dims="[1 -1 -2 0 0 0 0]")
             for fname in ("eigVals", "eigVecs"):
                 v = s[fname].reshape(-1, 9)
                 zg = {n: ("empty", None) if th_bc[n][0] == "empty" else ("zeroGradient", None) for n in th_bc}
-                polymesh.write_field(os.path.join(tdir, fname + md.name), fname + md.name, v, pm, zg, tname)
+                self._write_field(os.path.join(tdir, fname + md.name), fname + md.name, v, pm, zg, tname)
             if md.bmp:
                 f_bc = {}
                 for patch, t, vals in md.fbc:
                     f_bc[pm.patches[patch][0]] = _bc_entry(t, vals)
-                polymesh.write_field(os.path.join(tdir, "Phi" + md.name), "Phi" + md.name, s["Phi"], pm, f_bc,
+                self._write_field(os.path.join(tdir, "Phi" + md.name), "Phi" + md.name, s["Phi"], pm, f_bc,
                                      tname, dims="[-1 1 1 0 0 0 0]")
         if len(self.modes) > 1:
             calc = {p[0]: ("empty", None) if p[1] == "empty" else ("calculated", _face_vals(pm, tau, p[0]))
                     for p in pm.patches}
-            polymesh.write_field(os.path.join(tdir, "tau"), "tau", tau, pm, calc, tname,
+            self._write_field(os.path.join(tdir, "tau"), "tau", tau, pm, calc, tname,
                                  dims="[1 -1 -2 0 0 0 0]")
         return tdir
 

@@ -491,3 +491,25 @@ def test_binary_surface_field_reads_back(tmp_path):
     p.write_bytes(b"".join(parts))
     got_i, got_b = polymesh.read_surface_field(str(p), pm)
     assert np.array_equal(got_i, inner) and np.array_equal(got_b, want)
+
+
+def test_write_format_binary(tmp_path):
+    """controlDict writeFormat binary: the runner writes its fields in binary,
+    and they read back to the values of the same run written in ASCII."""
+    a = prepare(tmp_path / "a", "stresscase_cavity", cavity_velocity)
+    b = prepare(tmp_path / "b", "stresscase_cavity", cavity_velocity)
+    cd = os.path.join(b, "system", "controlDict")
+    with open(cd) as fh:
+        txt = fh.read()
+    with open(cd, "w") as fh:
+        fh.write(txt + "writeFormat     binary;\n")
+    wa = oracle_backends.OracleStressCase(a).run(steps=3)
+    wb = oracle_backends.OracleStressCase(b).run(steps=3)
+    pm = polymesh.case_mesh(a)
+    assert [os.path.basename(x) for x in wa] == [os.path.basename(x) for x in wb]
+    for fname in sorted(os.listdir(wa[-1])):
+        with open(os.path.join(wb[-1], fname), "rb") as fh:
+            assert b"format      binary" in fh.read(300), fname
+        fa = polymesh.read_field(os.path.join(wa[-1], fname), pm)
+        fb = polymesh.read_field(os.path.join(wb[-1], fname), pm)
+        assert np.array_equal(fa.internal, fb.internal), fname
