@@ -695,12 +695,22 @@ def mirror_mesh(pm: PolyMesh, base, normal, tol=1e-7) -> PolyMesh:
     return PolyMesh(points, faces, np.array(owner), nbr, patches)
 
 
-def read_mirror_dict(case):
-    """(basePoint, normalVector, planeTolerance) of system/mirrorMeshDict
-    (planeType pointAndNormal), or None when the case has none."""
-    path = os.path.join(case, "system", "mirrorMeshDict")
-    if not os.path.isfile(path):
-        return None
+def read_mirror_dicts(case):
+    """[(basePoint, normalVector, planeTolerance)] of the case's mirror
+    planes (planeType pointAndNormal): system/mirrorMeshDict, else the
+    numbered system/mirrorMeshDict0, mirrorMeshDict1, ... that a run script
+    copies in turn to mirrorMeshDict before each mirrorMesh; [] for none."""
+    sysdir = os.path.join(case, "system")
+    if os.path.isfile(os.path.join(sysdir, "mirrorMeshDict")):
+        return [_mirror_plane(os.path.join(sysdir, "mirrorMeshDict"))]
+    out, k = [], 0
+    while os.path.isfile(os.path.join(sysdir, f"mirrorMeshDict{k}")):
+        out.append(_mirror_plane(os.path.join(sysdir, f"mirrorMeshDict{k}")))
+        k += 1
+    return out
+
+
+def _mirror_plane(path):
     d = foamdict.read(path)
     if d.word("planeType") != "pointAndNormal":
         raise NotImplementedError("mirrorMeshDict: only planeType pointAndNormal is built")
@@ -711,14 +721,14 @@ def read_mirror_dict(case):
 
 def case_mesh(case) -> PolyMesh:
     """constant/polyMesh when present, else the case's system/blockMeshDict;
-    a case meshed here from its blockMeshDict that carries a mirrorMeshDict
-    (its run script follows blockMesh with mirrorMesh) is mirrored too."""
+    a case meshed here from its blockMeshDict that carries mirror planes
+    (its run script follows blockMesh with mirrorMesh) is mirrored too, in
+    the planes' order (read_mirror_dicts)."""
     if os.path.isfile(os.path.join(case, "constant", "polyMesh", "faces")):
         return read_polymesh(case)
     pm = read_block_mesh(case)
-    mirror = read_mirror_dict(case)
-    if mirror is not None:
-        pm = mirror_mesh(pm, *mirror)
+    for plane in read_mirror_dicts(case):
+        pm = mirror_mesh(pm, *plane)
     return pm
 
 

@@ -249,6 +249,9 @@ REF_TUTORIALS = "/root/reference/of90/tutorials/rheoFoam"
     ("Contraction41/Oldroyd-BLog", channel_velocity, None, "Oldroyd-BLog"),
     # arc edges around the cylinder, and the half mesh mirrored (mirrorMeshDict)
     ("Cylinder/Oldroyd-BLog", channel_velocity, 24894, "Oldroyd-BLog"),
+    # arcs and two mirror planes in turn (mirrorMeshDict0, mirrorMeshDict1)
+    ("../rheoEFoam/EEI/PoissonBoltzmann/CrossSlot", channel_velocity, 26500, "Log"),
+    ("../rheoEFoam/EEI/PoissonBoltzmann/FlowFocusing", channel_velocity, 26500, "Log"),
     ("CrossSlot/Oldroyd-BLog", channel_velocity, None, "Oldroyd-BLog"),
     ("CrossSlot/PTTLog", channel_velocity, None, "PTTLog"),
     ("OtherTests/Channel3D_VE/Oldroyd-BLog", channel_velocity, None, "Oldroyd-BLog"),
