@@ -297,6 +297,76 @@ def serve_stream(ctx, nreq, rate, max_cells, pageable=False, upload_eigen=False,
                 workload=SERVE_DESC.format(n=nreq, rate=rate, mx=max_cells, seed=seed_base))
 
 
+MM_SPEC = [("M1", "Oldroyd-BLog", dict(lam=0.8, etaS=0.1, etaP=0.3)),
+           ("M2", "GiesekusLog", dict(lam=2.5, alpha=0.2, etaS=0.05, etaP=0.55)),
+           ("M3", "PTTLog", dict(lam=0.3, epsilon=0.25, zeta=0.0, etaS=0.02, etaP=0.2)),
+           ("M4", "GiesekusLog", dict(lam=0.05, alpha=0.3, etaS=0.02, etaP=0.1))]
+
+
+def multimode_bench(args):
+    """A four-mode multiMode step (--workload multimode): its modes' solves
+    batched, against the same four modes stepped as models of their own one
+    after another (what a step without batching launches)."""
+    from rheotool_amd import abi, cases
+    n = args.n or 48
+    ctx = abi.Context(0)
+    m = cases.cartesian(n, n, n, jitter=0.1)
+    base = cases.make_fields(m, cases.FIELDS_TG, seed=31, lnLo=-2.0, lnHi=2.0)
+    states = [cases.make_fields(m, cases.FIELDS_TG, seed=40 + k, lnLo=-2.0, lnHi=2.0) for k in range(len(MM_SPEC))]
+    dm = ctx.mesh(m)
+    controls = abi.solver_dict(1e-10, precond=PRECOND, precision="mixed")
+
+    def entry(name, model, **kw):
+        body = abi.model_dict(model, **kw).split("{", 1)[1].rsplit("}", 1)[0]
+        return f"{name} {{{body}}}"
+    text = ("parameters\n{\n    type multiMode;\n    models\n    (\n" +
+            "\n".join(entry(a, b, **c) for a, b, c in MM_SPEC) + "\n    );\n}\n")
+    eq = ctx.constitutiveEq("", dm, text)
+    for md, g in zip(eq.modes(), states):
+        md.set_state(g.theta, g.eigVals, g.eigVecs)
+    eq.set_solver(controls)
+    eq.set_flow(base.U, base.Ub, base.phi)
+    alone = []
+    for (a, b, c), g in zip(MM_SPEC, states):
+        x = ctx.constitutiveEq("", dm, abi.model_dict(b, **c))
+        x.set_state(g.theta, g.eigVals, g.eigVecs)
+        x.set_solver(controls)
+        x.set_flow(base.U, base.Ub, base.phi)
+        alone.append(x)
+    dt = 1e-3
+
+    def timed(step):
+        for _ in range(args.warmup):
+            step()
+        ts = []
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            step()
+            ts.append(1e3 * (time.perf_counter() - t0))
+        return float(np.median(ts)), float(np.percentile(ts, 99))
+
+    def in_turn():
+        for x in alone:
+            x.correct(dt)
+    batched, b99 = timed(lambda: eq.correct(dt))
+    turn, t99 = timed(in_turn)
+    for md, x in zip(eq.modes(), alone):  # the same bits either way
+        if not np.array_equal(md.theta(), x.theta()):
+            raise SystemExit("multimode bench: batched and one-by-one modes differ")
+    nm = len(MM_SPEC)
+    line = dict(metric=METRIC, value=round(m.nCells * nm / (batched * 1e-3), 1), unit="cell-updates/s",
+                n_gpus=1, steps=args.steps, warmup=args.warmup, ms_per_step=round(batched, 3),
+                higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f32+f64", data="synthetic",
+                config=dict(workload=f"multiMode of {nm} modes (Oldroyd-B, Giesekus, PTT, Giesekus) on a {n}^3 "
+                                     "box, nodes jittered +-10%; FP32 Krylov + FP64 refinement to 1e-10; "
+                                     "value counts each mode's cells; median step without kernel events",
+                            cells=m.nCells, modes=nm, preconditioner=PRECOND),
+                p99_ms=round(b99, 3), modes_in_turn_ms_per_step=round(turn, 3), modes_in_turn_p99_ms=round(t99, 3),
+                speedup_vs_modes_in_turn=round(turn / batched, 3))
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
 def serve_bench(args):
     """The serving stream as the bench line (--workload serve)."""
     from rheotool_amd import abi
@@ -323,7 +393,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="precision", choices=sorted(WORKLOADS) + ["copy", "serve"])
+    ap.add_argument("--workload", default="precision",
+                    choices=sorted(WORKLOADS) + ["copy", "serve", "multimode"])
     ap.add_argument("--requests", type=int, default=512)
     ap.add_argument("--rate", type=float, default=100.0, help="serve: Poisson arrival rate, 1/s")
     ap.add_argument("--max-cells", type=float, default=2e6)
@@ -377,6 +448,8 @@ def main():
         return
     if args.workload == "serve":
         return serve_bench(args)
+    if args.workload == "multimode":
+        return multimode_bench(args)
     w = WORKLOADS[args.workload]
     n = args.n or w["n"]
     t0 = time.perf_counter()
