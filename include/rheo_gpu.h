@@ -282,8 +282,10 @@ rt_model* rt_model_fluidity(rt_model* model);
  * current tau on the device, out [3N] (to be added to the momentum source as
  * the caller's fvc terms are); rt_model_div_tau_coeff gives nu, the
  * coefficient of the implicit fvm::laplacian the caller's momentum matrix
- * holds.  tauB [6*nBoundaryFaces]: tau's patch values, NULL for
- * zeroGradient.  U's patch values (rt_model_set_flow's Ub) are read as a
+ * holds.  tauB [6*nBoundaryFaces]: tau's patch values; NULL takes the
+ * cell value (zeroGradient), and on the symmetryPlane/symmetry and wedge
+ * theta patches the transform of the cell value that theta takes there (an
+ * OpenFOAM constraint patch gives every field its type).  U's patch values (rt_model_set_flow's Ub) are read as a
  * fixedValue patch's (snGrad = (Ub - U_P) |n (n & (Cf - C_P))|^-1); a
  * zeroGradient velocity patch gives Ub = U_P, hence a zero snGrad, as the
  * reference's.  BSD and coupling need the mesh's cell and boundary face

@@ -1175,10 +1175,17 @@ __global__ void __launch_bounds__(BLK) k_div_tau(AssembleArgs a, const double* _
         const int32_t b0 = a.bPtr[r], b1 = a.bPtr[r + 1];
         for (int32_t qb = b0; qb < b1; ++qb) {
             const int32_t bf = a.bFace[qb];
+            const double sf[3] = {a.bSf[bf], a.bSf[NB + bf], a.bSf[2 * NB + bf]};
             double tb[6];
 #pragma unroll
             for (int q = 0; q < 6; ++q) tb[q] = tauB ? tauB[(int64_t)q * NB + bf] / rho : t[q];
-            const double sf[3] = {a.bSf[bf], a.bSf[NB + bf], a.bSf[2 * NB + bf]};
+            if (!tauB) {
+                // a constraint patch gives every field its type: tau takes the
+                // transform theta takes there (symmetry: mirrored, wedge: rotated)
+                const int bt = a.bcType[a.bPatch[bf]];
+                if (bt == RT_BC_SYMMETRY) symmetry_value(sf, t, tb);
+                else if (bt == RT_BC_WEDGE) wedge_value(a.patchT + 9 * (int64_t)a.bPatch[bf], t, tb);
+            }
             acc[0] += sf[0] * tb[0] + sf[1] * tb[1] + sf[2] * tb[2];
             acc[1] += sf[0] * tb[1] + sf[1] * tb[3] + sf[2] * tb[4];
             acc[2] += sf[0] * tb[2] + sf[1] * tb[4] + sf[2] * tb[5];

@@ -1273,3 +1273,50 @@ def test_multimode_batched_solves_equal_each_mode_alone(ctx, precision, precond)
             assert [(q.nIterations, q.finalResidual) for q in pm] == [(q.nIterations, q.finalResidual)
                                                                       for q in pa]
     assert np.array_equal(eq.tau(), alone[0].tau() + alone[1].tau() + alone[2].tau())
+
+
+@pytest.mark.parametrize("case", ["wedge", "symmetry"])
+def test_div_tau_default_patch_values_on_constraint_patches(ctx, case):
+    """rt_model_div_tau without tau patch values takes, on wedge and symmetry
+    theta patches, the transform of the cell value that theta takes there (an
+    OpenFOAM constraint patch gives every field its type): equal to the
+    oracle's divTau (coupling) given those values; the plain cell value there
+    gives a different field."""
+    if case == "wedge":
+        from test_oracle_wedge import fields, mesh
+        m = mesh(6.0, 1)
+        f = fields(m)
+        pats, bct = [m.patchNames.index("front"), m.patchNames.index("back")], abi.RT_BC_WEDGE
+    else:
+        from test_oracle_symmetry import fields_of, halves
+        _, m, _ = halves()
+        pats, bct = [m.patchNames.index("ymax")], abi.RT_BC_SYMMETRY
+        f = fields_of(m, pats[0])
+    print(f"[case] divTau default patch values, {case}", flush=True)
+    dm = ctx.mesh(m)
+    eq = ctx.constitutiveEq("", dm, abi.model_dict("GiesekusLog", lam=0.8, alpha=0.2, rho=1.2))
+    for q in pats:
+        eq.set_bc(q, bct)
+    eq.set_state(f.theta, f.eigVals, f.eigVecs)
+    eq.set_flow(f.U, f.Ub, f.phi)
+    assert eq.correct(0.01) == abi.RT_OK
+    tau = eq.tau()
+    F = m.nInternal
+    tauB = tau[m.owner[F:]].copy()
+    ij = ([0, 0, 0, 1, 1, 2], [0, 1, 2, 1, 2, 2])
+    for q in pats:
+        T = O.wedge_face_t(m, q) if case == "wedge" else None
+        for b in range(m.patchStart[q], m.patchStart[q] + m.patchSize[q]):
+            t = tau[m.owner[F + b]]
+            A = np.array([[t[0], t[1], t[2]], [t[1], t[3], t[4]], [t[2], t[4], t[5]]])
+            if T is not None:
+                B = T @ A @ T.T
+            else:
+                n = m.Sf[F + b] / np.linalg.norm(m.Sf[F + b])
+                R = np.eye(3) - 2.0 * np.outer(n, n)
+                B = 0.5 * (A + R @ A @ R.T)
+            tauB[b] = B[ij]
+    want = O.div_tau_stab(m, f.U, f.Ub, tau, tauB, 1.2, 0.41, "coupling")
+    assert rel_l2(eq.div_tau(), want) < 1e-12
+    plain = O.div_tau_stab(m, f.U, f.Ub, tau, None, 1.2, 0.41, "coupling")
+    assert rel_l2(plain, want) > 1e-6
