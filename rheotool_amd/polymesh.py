@@ -13,7 +13,8 @@ interpolation weights) is computed by the case library's `rtc_mesh_from_poly`
 The blockMesh subset builds what tutorial cases of the stress step use:
 hex blocks, `simpleGrading` with an expansion ratio or multi-grading
 sections per direction, `arc` edges (a circular arc through its end points
-and one point on it), `convertToMeters`/`scale`, block faces merged by
+and one point on it) and `polyLine` edges (placed by length along the
+line), `convertToMeters`/`scale`, block faces merged by
 position, patches listed in `boundary` (or the older `patches` form), and
 the unlisted faces in `defaultPatch` (`defaultFaces`, empty).  A block's
 points follow its edges: the trilinear map of its corners plus, for each
@@ -22,7 +23,7 @@ position along it, weighted by the point's distance from the opposite edges
 (transfinite interpolation from the twelve edges); with straight edges
 this is the trilinear map.  The points along an arc are placed by angle,
 which is arc length, with the block's grading.  Other edge types (spline,
-polyLine, projected), `edgeGrading`, `mergePatchPairs` and projection are
+projected), `edgeGrading`, `mergePatchPairs` and projection are
 not built: a blockMeshDict that uses them raises NotImplementedError.
 """
 from __future__ import annotations
@@ -376,6 +377,22 @@ class _Arc:
         return self.c + self.r * (np.cos(t) * self.e1 + np.sin(t) * self.e2)
 
 
+class _PolyLine:
+    """A polyline from p0 through the given points to p1; position(lam) is the
+    point a fraction lam of its length along it."""
+
+    def __init__(self, p0, pts, p1):
+        self.P = np.vstack([p0[None], np.asarray(pts, dtype=np.float64).reshape(-1, 3), p1[None]])
+        seg = np.linalg.norm(np.diff(self.P, axis=0), axis=1)
+        self.s = np.concatenate([[0.0], np.cumsum(seg)]) / seg.sum()
+
+    def position(self, lam):
+        lam = np.asarray(lam, dtype=np.float64)
+        k = np.clip(np.searchsorted(self.s, lam, side="right") - 1, 0, len(self.s) - 2)
+        t = (lam - self.s[k]) / (self.s[k + 1] - self.s[k])
+        return self.P[k] + t[..., None] * (self.P[k + 1] - self.P[k])
+
+
 def _edges(d, V, scale):
     """{(a, b): position(lam)} of the curved edges (vertex labels a -> b);
     V are the scaled vertices, the points on the arcs are scaled here."""
@@ -386,8 +403,14 @@ def _edges(d, V, scale):
     i = 0
     while i < len(toks):
         kind = toks[i]
+        if kind == "polyLine":
+            a, b = int(toks[i + 1]), int(toks[i + 2])
+            pts = np.array(toks[i + 3], dtype=np.float64).reshape(-1, 3) * scale
+            out[(a, b)] = _PolyLine(V[a], pts, V[b]).position
+            i += 4
+            continue
         if kind != "arc":
-            raise NotImplementedError(f"edges: {kind!r} edges are not built (arc only)")
+            raise NotImplementedError(f"edges: {kind!r} edges are not built (arc and polyLine only)")
         a, b = int(toks[i + 1]), int(toks[i + 2])
         if not isinstance(toks[i + 3], list) or len(toks[i + 3]) != 3:
             raise NotImplementedError("edges: an arc takes one point on it (not an origin form)")

@@ -137,8 +137,7 @@ def test_polymesh_rejects_inconsistent_descriptions():
 
 def test_unsupported_blockmesh_features_raise():
     box = BOX.format(nx=2, ny=2, nz=2, s=1, gx=1, gy=1, gz=1)
-    for edges in ("edges ( spline 0 1 ((0.5 -0.1 0)) );", "edges ( polyLine 0 1 ((0.5 -0.1 0)) );",
-                  "edges ( arc 0 1 origin (0.5 -1 0) );"):
+    for edges in ("edges ( spline 0 1 ((0.5 -0.1 0)) );", "edges ( arc 0 1 origin (0.5 -1 0) );"):
         with pytest.raises(NotImplementedError):
             polymesh.block_mesh(foamdict.parse(box.replace("edges ();", edges)))
     # an arc through a point on it is built, and bows the edge out
@@ -516,3 +515,22 @@ def test_write_format_binary(tmp_path):
         fa = polymesh.read_field(os.path.join(wa[-1], fname), pm)
         fb = polymesh.read_field(os.path.join(wb[-1], fname), pm)
         assert np.array_equal(fa.internal, fb.internal), fname
+
+
+def test_block_mesh_polyline_edge_points():
+    """A polyLine edge through (0.5, -0.2, 0) on the z = 0 side of a unit box:
+    the block's points along it lie on its two segments at equal length
+    steps, and the cells add up to the box plus the bulge, a triangle of
+    area 0.1 tapering linearly to nothing at z = 1 (volume 1.05)."""
+    box = BOX.format(nx=4, ny=2, nz=1, s=1, gx=1, gy=1, gz=1)
+    pm = polymesh.block_mesh(foamdict.parse(box.replace("edges ();", "edges ( polyLine 0 1 ((0.5 -0.2 0)) );")))
+    P = pm.points
+    edge = P[(np.abs(P[:, 2]) < 1e-12) & (P[:, 1] < 1e-12)]
+    edge = edge[np.argsort(edge[:, 0])]
+    seg = np.hypot(0.5, 0.2)
+    want_x = np.array([0.0, 0.5 * 0.5, 0.5, 0.75, 1.0])
+    assert np.allclose(edge[:, 0], want_x, atol=1e-12)
+    assert np.allclose(edge[:, 1], -0.2 * (1 - np.abs(want_x - 0.5) / 0.5), atol=1e-12)
+    assert np.all(np.diff(np.hypot(np.diff(edge[:, 0]), np.diff(edge[:, 1]))) < 1e-12) and seg > 0
+    m = pm.to_mesh()
+    assert m.V.min() > 0 and abs(m.V.sum() - 1.05) < 1e-12
